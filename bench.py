@@ -1,0 +1,178 @@
+"""Benchmark: 2-D variable-depth harbour, 2048 x 2048, AB2 (HS2) right-hand side
+with two-reference-depth operators, advanced by the device-resident RKF45
+(FP32 transforms/operators, FP64 state, stage sums and error norm).
+
+One "step" is one accepted RKF45 step (gsl_odeiv2_evolve_apply) of the whole
+grid.  Prints one JSON line (rank 0).  Headline value: simulated seconds per
+wall second; RHS evaluations per second are reported beside it.
+
+Roofline: the dominant unit is one RHS evaluation (the 9-kernel launch
+sequence), timed with HIP events on the library's stream around every
+evaluation of the timed region.  Its algorithmic HBM bytes per evaluation are
+B_rhs = T_alg (4N + 8H) + C_phys 4N + C_spec 4H + 2 S (input state read and
+derivative write), T_alg = 36, C_phys = 10, C_spec = 5, S = 3*16*H,
+N = nx*ny, H = (nx/2+1)*ny.  The whole-step figure (adding the RKF45 vector
+traffic, 5.5 S per evaluation) is reported as roofline_job.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "sim-sec/wall-sec + RHS evals/s, 2D 2048^2 FP32 vs FP64 oracle at relL2<=1e-4"
+
+
+def algorithmic_bytes(nx: int, ny: int, t_alg: int = 36, c_phys: int = 10, c_spec: int = 5):
+    N = nx * ny
+    H = (nx // 2 + 1) * ny
+    S = 3 * 16 * H
+    transforms = t_alg * (4 * N + 8 * H)
+    coeff = c_phys * 4 * N + c_spec * 4 * H
+    rhs_only = transforms + coeff + 2 * S
+    job = rhs_only + 5.5 * S
+    return rhs_only, job
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--spinup", type=float, default=10.0, help="simulated seconds integrated before warmup")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the path would shard over independent realizations, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from hawassi_c_amd import HW_PREC_FP32, HW_PREC_FP64
+    from hawassi_c_amd import workloads as W
+    from hawassi_c_amd.model2d import Hawassi2D
+
+    n = args.n
+    t_end = args.spinup + 60.0
+    log(f"setup harbour {n}x{n}")
+    model = W.harbour(n, t_end=t_end)
+    dev = Hawassi2D(model, HW_PREC_FP32)
+    dim = 6 * n * n
+    dev.set_state(np.zeros(dim))
+    t, h = 0.0, 0.05
+    # spin-up: let the influx build a wave field so step sizes are representative
+    t, h = dev.integrate(t, args.spinup, h)
+    log(f"spin-up done t={t:.3f} h={h:.4f} stats={dev.stats()}")
+    for _ in range(args.warmup):
+        t, h = dev.evolve_apply(t, t_end, h)
+    dev.sync()
+    dev.reset_stats()
+    t_start, h_start = t, h
+    state_start = dev.get_state() if not (args.no_cpu_baseline and args.no_accuracy) else None
+    dev.sync()
+    w0 = time.perf_counter()
+    for _ in range(args.steps):
+        t, h = dev.evolve_apply(t, t_end, h)
+    dev.sync()
+    wall = time.perf_counter() - w0
+    st = dev.stats()
+    sim = t - t_start
+    log(f"timed {args.steps} steps: wall {wall:.3f} s, sim {sim:.3f} s, {st}")
+    rhs_per_s = st["rhs_evals"] / wall
+    avg_rhs_ms = st["rhs_ms_total"] / max(st["rhs_evals"], 1)
+    b_rhs, b_job = algorithmic_bytes(n, n)
+    achieved = b_rhs / (avg_rhs_ms * 1e-3) / 1e9
+    job = rhs_per_s * b_job / 1e9
+    out = {
+        "metric": METRIC,
+        "value": sim / wall,
+        "unit": "sim-s/wall-s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"harbour {n}x{n}: 1.5 km square, depth 20->4 m + seeded bumps, two-depth AB2 (HS2) "
+                               "operator, breakwater R=0.5 with 100 m gap, directional JONSWAP line influx; "
+                               "RKF45 eps_rel 1e-2 eps_abs 1e-3 (reference tolerances)",
+                   "nx": n, "ny": n, "state": "fp64 half spectrum", "sim_window_s": [t_start, t]},
+        "rhs_evals_per_s": rhs_per_s,
+        "rhs_evals": st["rhs_evals"],
+        "steps_rejected": st["steps_rejected"],
+        "avg_rhs_ms": avg_rhs_ms,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit_of_work": "one RHS evaluation (9 kernels), %.1f MB algorithmic" % (b_rhs / 1e6)},
+        "roofline_job": {"achieved": job, "frac": job / HBM_PEAK_GBS, "unit": "GB/s",
+                         "bytes_per_rhs": b_job},
+    }
+    dev.close()
+
+    if not args.no_accuracy:
+        # FP32 path vs the FP64 device path (itself parity-checked against the
+        # oracle at <=1e-11) over the timed window, same start state and end time
+        log("accuracy check vs FP64 device path")
+        d64 = Hawassi2D(model, HW_PREC_FP64)
+        d64.set_state(state_start)
+        t64, _ = d64.integrate(t_start, t, h_start)
+        e64, u64, v64 = d64.fields()
+        d64.close()
+        d32 = Hawassi2D(model, HW_PREC_FP32)
+        d32.set_state(state_start)
+        d32.integrate(t_start, t, h_start)
+        e32, u32, v32 = d32.fields()
+        d32.close()
+        rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+        out["accuracy"] = {"relL2_eta": rel(e32, e64), "relL2_u": rel(u32, u64), "relL2_v": rel(v32, v64),
+                           "reference": "FP64 device path (oracle-checked in tests)", "window_s": [t_start, t]}
+
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        from oracle import oracle as O
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        O.set_threads(threads)
+        log(f"cpu baseline: oracle, {threads} threads")
+        orc = O.Oracle2D(model)
+        y = state_start.copy()
+        c0 = orc.rhs_count()
+        w0 = time.perf_counter()
+        rc, t1, h1, stc = orc.evolve_apply(t_start, t_end, h_start, y, model.eps_abs, model.eps_rel)
+        cw = time.perf_counter() - w0
+        nr = orc.rhs_count() - c0
+        out["cpu_baseline"] = {"value": (t1 - t_start) / cw, "unit": "sim-s/wall-s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"1 RKF45 step ({nr} RHS incl. GSL dydt_out, h={t1 - t_start:.4g} s) of the "
+                                         f"same {n}x{n} harbour from the same state at t={t_start:.3f} s; FP64 "
+                                         f"oracle (restated CPU path, radix-2 FFT, OpenMP), {cpu_model()}",
+                               "rhs_evals_per_s": nr / cw, "wall_s": cw}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,97 @@
+"""ctypes binding of libhawassi_hip.so (the C ABI in include/hawassi_hip.h)."""
+from __future__ import annotations
+
+import ctypes as ct
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libhawassi_hip.so"
+
+HW_PREC_FP64 = 0
+HW_PREC_FP32 = 1
+
+_dp = ct.POINTER(ct.c_double)
+
+
+class HwConfig2D(ct.Structure):
+    _fields_ = [
+        ("nx", ct.c_int), ("ny", ct.c_int), ("precision", ct.c_int), ("flat", ct.c_int),
+        ("grav", ct.c_double), ("cpeak", ct.c_double),
+        ("kx", _dp), ("ky", _dp), ("aal", _dp), ("fbdy", _dp), ("wallchar", _dp), ("chi", _dp),
+        ("c2d", _dp), ("l2d", _dp),
+        ("gam_min", _dp), ("gam_plus", _dp), ("csq_min", _dp), ("csq_plus", _dp), ("om2_min", _dp), ("om2_plus", _dp),
+        ("source_on", ct.c_int), ("vertical", ct.c_int), ("propagation", ct.c_int),
+        ("new_gam", _dp), ("skew_gam", _dp),
+        ("nline", ct.c_int), ("nt", ct.c_int),
+        ("t_sig", _dp), ("sig", _dp), ("skew_sig", _dp),
+        ("eps_abs", ct.c_double), ("eps_rel", ct.c_double),
+    ]
+
+
+class HwStats(ct.Structure):
+    _fields_ = [("rhs_evals", ct.c_long), ("steps_accepted", ct.c_long), ("steps_rejected", ct.c_long),
+                ("rhs_ms_total", ct.c_double)]
+
+
+EXPORTS = [
+    "hw_ctx_create_2d", "hw_ctx_destroy", "hw_rhs", "hw_rhs_device", "hw_set_state", "hw_get_state",
+    "hw_evolve_apply", "hw_integrate", "hw_get_fields", "hw_get_stats", "hw_reset_stats", "hw_sync",
+    "hw_fft2_forward", "hw_fft2_inverse", "hw_supported", "hw_last_error", "hw_version",
+]
+
+_lib = None
+
+
+class HipPathMissing(RuntimeError):
+    """The HIP library is not built; there is no CPU fallback."""
+
+
+def load(path: Path | None = None) -> ct.CDLL:
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise HipPathMissing(f"{p} not found: build it with `python -m hawassi_c_amd.build` "
+                             "(the product has no CPU fallback)")
+    lib = ct.CDLL(str(p))
+    vp = ct.c_void_p
+    lib.hw_ctx_create_2d.argtypes = [ct.POINTER(HwConfig2D), ct.POINTER(vp)]
+    lib.hw_ctx_destroy.argtypes = [vp]
+    lib.hw_ctx_destroy.restype = None
+    lib.hw_rhs.argtypes = [ct.c_double, _dp, _dp, vp]
+    lib.hw_rhs_device.argtypes = [vp, ct.c_double, vp, vp]
+    lib.hw_set_state.argtypes = [vp, _dp]
+    lib.hw_get_state.argtypes = [vp, _dp]
+    lib.hw_evolve_apply.argtypes = [vp, _dp, ct.c_double, _dp]
+    lib.hw_integrate.argtypes = [vp, _dp, ct.c_double, _dp]
+    lib.hw_get_fields.argtypes = [vp, _dp, _dp, _dp]
+    lib.hw_get_stats.argtypes = [vp, ct.POINTER(HwStats)]
+    lib.hw_reset_stats.argtypes = [vp]
+    lib.hw_sync.argtypes = [vp]
+    lib.hw_fft2_forward.argtypes = [ct.c_int, ct.c_int, ct.c_int, _dp, _dp, _dp]
+    lib.hw_fft2_inverse.argtypes = [ct.c_int, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp]
+    lib.hw_supported.argtypes = [ct.c_int, ct.c_int, ct.c_int]
+    lib.hw_last_error.restype = ct.c_char_p
+    lib.hw_version.restype = ct.c_char_p
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "arrays must be C-contiguous float64"
+    return a.ctypes.data_as(_dp)
+
+
+class HwError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        msg = load().hw_last_error().decode(errors="replace")
+        raise HwError(f"{what} failed (code {rc}): {msg}")

@@ -1,0 +1,82 @@
+"""Build libhawassi_hip.so for gfx950 with hipcc (in-tree).
+
+One translation unit per (precision, nx, ny) kernel set is generated from the
+HW_INSTANTIATE macro and compiled in parallel; the C ABI unit (capi.hip) is
+linked with them into hawassi_c_amd/libhawassi_hip.so.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+BUILD = PKG / "_build"
+LIB = PKG / "libhawassi_hip.so"
+
+ARCH = os.environ.get("HW_OFFLOAD_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-Wno-unused-result"]
+
+# (precision, nx, ny): precision 0 = FP64, 1 = FP32 transforms/operators
+SIZES = (
+    [(0, n, n) for n in (32, 64, 128, 256, 512, 1024, 2048)]
+    + [(1, n, n) for n in (32, 64, 128, 256, 512, 1024, 2048)]
+    + [(p, a, b) for p in (0, 1) for (a, b) in ((64, 32), (32, 64), (128, 64))]
+)
+
+
+def _units():
+    gen = BUILD / "gen"
+    gen.mkdir(parents=True, exist_ok=True)
+    units = [CSRC / "capi.hip"]
+    for prec, nx, ny in SIZES:
+        t = "double" if prec == 0 else "float"
+        f = gen / f"inst_p{prec}_{nx}x{ny}.hip"
+        text = (f'// generated by hawassi_c_amd/build.py\n#include "{CSRC}/solver2d.h"\n'
+                f"HW_INSTANTIATE({t}, {prec}, {nx}, {ny})\n")
+        if not f.exists() or f.read_text() != text:
+            f.write_text(text)
+        units.append(f)
+    return units
+
+
+def _compile(src: Path) -> Path:
+    obj = BUILD / (src.stem + ".o")
+    deps = [src] + sorted(CSRC.glob("*.h")) + [PKG.parent / "include" / "hawassi_hip.h"]
+    if obj.exists() and all(obj.stat().st_mtime >= d.stat().st_mtime for d in deps):
+        return obj
+    cmd = [HIPCC, *FLAGS, "-I", str(CSRC), "-I", str(PKG.parent / "include"), "-c", str(src), "-o", str(obj)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src.name}:\n{r.stderr[-4000:]}")
+    return obj
+
+
+def build(jobs: int | None = None, verbose: bool = True) -> Path:
+    if shutil.which(HIPCC) is None and not Path(HIPCC).exists():
+        raise RuntimeError(f"hipcc not found at {HIPCC}")
+    BUILD.mkdir(parents=True, exist_ok=True)
+    units = _units()
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile, units))
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB} ({len(objs)} units)")
+    return LIB
+
+
+if __name__ == "__main__":
+    build(int(sys.argv[1]) if len(sys.argv) > 1 else None)

@@ -1,0 +1,193 @@
+// capi.hip -- the extern "C" entry points declared in include/hawassi_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "solver2d.h"
+
+namespace hw {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+std::vector<Entry>& registry() {
+    static std::vector<Entry> r;
+    return r;
+}
+
+static const Entry* find_entry(int prec, int nx, int ny) {
+    for (const Entry& e : registry())
+        if (e.prec == prec && e.nx == nx && e.ny == ny) return &e;
+    return nullptr;
+}
+
+static int check_device() {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n < 1) {
+        set_error("no HIP device available (%s)", e == hipSuccess ? "count 0" : hipGetErrorString(e));
+        return HW_ERR_NOGPU;
+    }
+    return HW_OK;
+}
+
+}  // namespace hw
+
+using namespace hw;
+
+extern "C" {
+
+const char* hw_last_error(void) { return g_err; }
+const char* hw_version(void) { return "hawassi_hip 0.1 (gfx950)"; }
+
+int hw_supported(int nx, int ny, int precision) { return find_entry(precision, nx, ny) ? 1 : 0; }
+
+int hw_ctx_create_2d(const hw_config_2d* cfg, void** ctx) {
+    if (!cfg || !ctx) { set_error("null argument"); return HW_ERR_ARG; }
+    *ctx = nullptr;
+    const Entry* e = find_entry(cfg->precision, cfg->nx, cfg->ny);
+    if (!e) {
+        set_error("unsupported grid %dx%d precision %d (no compiled kernels)", cfg->nx, cfg->ny, cfg->precision);
+        return HW_ERR_ARG;
+    }
+    int rc = check_device();
+    if (rc) return rc;
+    Solver2D* s = e->make();
+    rc = s->init(cfg);
+    if (rc) { delete s; return rc; }
+    *ctx = s;
+    return HW_OK;
+}
+
+void hw_ctx_destroy(void* ctx) { delete static_cast<Solver2D*>(ctx); }
+
+static int ensure_staging(Solver2D* s) {
+    size_t bytes = sizeof(double) * 6 * (size_t)s->nx * s->ny;
+    if (!s->d_full_a) HW_TRY(hipMalloc((void**)&s->d_full_a, bytes));
+    if (!s->d_full_b) HW_TRY(hipMalloc((void**)&s->d_full_b, bytes));
+    return HW_OK;
+}
+
+int hw_rhs(double t, const double* y, double* dydt, void* ctx) {
+    Solver2D* s = static_cast<Solver2D*>(ctx);
+    if (!s || !y || !dydt) { set_error("null argument"); return HW_ERR_ARG; }
+    int rc = ensure_staging(s);
+    if (rc) return rc;
+    size_t bytes = sizeof(double) * 6 * (size_t)s->nx * s->ny;
+    HW_TRY(hipMemcpyAsync(s->d_full_a, y, bytes, hipMemcpyHostToDevice, s->stream));
+    rc = s->rhs_full_dev(t, s->d_full_a, s->d_full_b);
+    if (rc) return rc;
+    HW_TRY(hipMemcpyAsync(dydt, s->d_full_b, bytes, hipMemcpyDeviceToHost, s->stream));
+    HW_TRY(hipStreamSynchronize(s->stream));
+    return HW_OK;
+}
+
+int hw_rhs_device(void* ctx, double t, const double* d_y, double* d_dydt) {
+    Solver2D* s = static_cast<Solver2D*>(ctx);
+    if (!s || !d_y || !d_dydt) { set_error("null argument"); return HW_ERR_ARG; }
+    return s->rhs_full_dev(t, d_y, d_dydt);
+}
+
+int hw_set_state(void* ctx, const double* y) {
+    Solver2D* s = static_cast<Solver2D*>(ctx);
+    if (!s || !y) { set_error("null argument"); return HW_ERR_ARG; }
+    int rc = ensure_staging(s);
+    if (rc) return rc;
+    size_t bytes = sizeof(double) * 6 * (size_t)s->nx * s->ny;
+    HW_TRY(hipMemcpyAsync(s->d_full_a, y, bytes, hipMemcpyHostToDevice, s->stream));
+    return s->set_state_dev(s->d_full_a);
+}
+
+int hw_get_state(void* ctx, double* y) {
+    Solver2D* s = static_cast<Solver2D*>(ctx);
+    if (!s || !y) { set_error("null argument"); return HW_ERR_ARG; }
+    int rc = ensure_staging(s);
+    if (rc) return rc;
+    rc = s->get_state_dev(s->d_full_a);
+    if (rc) return rc;
+    size_t bytes = sizeof(double) * 6 * (size_t)s->nx * s->ny;
+    HW_TRY(hipMemcpyAsync(y, s->d_full_a, bytes, hipMemcpyDeviceToHost, s->stream));
+    HW_TRY(hipStreamSynchronize(s->stream));
+    return HW_OK;
+}
+
+int hw_evolve_apply(void* ctx, double* t, double t1, double* h) {
+    Solver2D* s = static_cast<Solver2D*>(ctx);
+    if (!s || !t || !h) { set_error("null argument"); return HW_ERR_ARG; }
+    return s->evolve_apply(t, t1, h);
+}
+
+int hw_integrate(void* ctx, double* t, double t1, double* h) {
+    Solver2D* s = static_cast<Solver2D*>(ctx);
+    if (!s || !t || !h) { set_error("null argument"); return HW_ERR_ARG; }
+    while (*t < t1) {
+        int rc = s->evolve_apply(t, t1, h);
+        if (rc) return rc;
+    }
+    return HW_OK;
+}
+
+int hw_get_fields(void* ctx, double* eta, double* u, double* v) {
+    Solver2D* s = static_cast<Solver2D*>(ctx);
+    if (!s) { set_error("null argument"); return HW_ERR_ARG; }
+    size_t n = (size_t)s->nx * s->ny;
+    int rc = ensure_staging(s);
+    if (rc) return rc;
+    double* d[3] = {eta ? s->d_full_a : nullptr, u ? s->d_full_a + n : nullptr, v ? s->d_full_a + 2 * n : nullptr};
+    rc = s->fields_dev(d[0], d[1], d[2]);
+    if (rc) return rc;
+    double* h[3] = {eta, u, v};
+    for (int f = 0; f < 3; ++f)
+        if (h[f]) HW_TRY(hipMemcpy(h[f], d[f], sizeof(double) * n, hipMemcpyDeviceToHost));
+    return HW_OK;
+}
+
+int hw_get_stats(void* ctx, hw_stats* st) {
+    Solver2D* s = static_cast<Solver2D*>(ctx);
+    if (!s || !st) { set_error("null argument"); return HW_ERR_ARG; }
+    *st = s->st;
+    return HW_OK;
+}
+
+int hw_reset_stats(void* ctx) {
+    Solver2D* s = static_cast<Solver2D*>(ctx);
+    if (!s) { set_error("null argument"); return HW_ERR_ARG; }
+    s->st = hw_stats{};
+    return HW_OK;
+}
+
+int hw_sync(void* ctx) {
+    Solver2D* s = static_cast<Solver2D*>(ctx);
+    if (!s) { set_error("null argument"); return HW_ERR_ARG; }
+    HW_TRY(hipStreamSynchronize(s->stream));
+    return HW_OK;
+}
+
+int hw_fft2_forward(int nx, int ny, int precision, const double* phys, double* re, double* im) {
+    const Entry* e = find_entry(precision, nx, ny);
+    if (!e) { set_error("unsupported grid %dx%d precision %d", nx, ny, precision); return HW_ERR_ARG; }
+    if (!phys || !re || !im) { set_error("null argument"); return HW_ERR_ARG; }
+    int rc = check_device();
+    if (rc) return rc;
+    return e->fwd(phys, re, im);
+}
+
+int hw_fft2_inverse(int nx, int ny, int precision, const double* re, const double* im, double* phys, double* nyq_im) {
+    const Entry* e = find_entry(precision, nx, ny);
+    if (!e) { set_error("unsupported grid %dx%d precision %d", nx, ny, precision); return HW_ERR_ARG; }
+    if (!phys || !re || !im) { set_error("null argument"); return HW_ERR_ARG; }
+    int rc = check_device();
+    if (rc) return rc;
+    return e->inv(re, im, phys, nyq_im);
+}
+
+}  // extern "C"

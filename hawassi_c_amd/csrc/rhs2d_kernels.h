@@ -1,0 +1,760 @@
+// rhs2d_kernels.h -- fused HIP kernels of the 2-D AB2 ("HS2") right-hand side.
+//
+// The reference evaluates rhs_AB2_2DB_uv (2d/fun_odesolver.c:4542-4624) with
+// 51-55 full complex FFTs.  Here the same operator is evaluated on the half
+// spectrum with every pointwise product, multiplier, mask, projection and the
+// linear combinations folded into the prologue/epilogue of 9 FFT passes
+// (x = row passes, y = column passes):
+//
+//   Y1  inverse y of eta^, u^, v^, C2-(k) u^, C2+(k) u^, C2-(k) v^, C2+(k) v^
+//   X1  inverse x of those; forward x of g-u, g+u, g-a-u + g+a+u (and v)     [HSSgen_2depth]
+//   Y2  forward y; dH1 = -i/g (kx HSS(u) + ky HSS(v)); inverse y of M0u, M0v
+//   X3  inverse x of M0u, M0v; forward x of eta M0u w, eta u w, eta M0v w,
+//       eta v w, E w and g+-(eta M0 w)                                      [AB2 products]
+//   Y4  forward y; partial dH2; inverse y of Om2+-(k) F(eta M0 w), i kx E^, i ky E^
+//   X5  inverse x; forward x of g-b- + g+b+; adjust of i kx E^, i ky E^ folded
+//       with damping into the u and v output transforms                    [fun_adjust]
+//   Y6  forward y; dH2 = dH2part - (F(acc_u)+F(acc_v))/(2g); inverse y of dH2;
+//       du^/dt, dv^/dt written
+//   X7  inverse x of dH2 (adjust), source(t) and damping folded into the
+//       eta output transform                                                [fun_source]
+//   Y8  forward y; deta^/dt written
+//
+// Flat bottom (rhs_AB2_2DF_uv, 2d/fun_odesolver.c:4458-4540) uses 5 passes.
+#pragma once
+#include "hw_fft.h"
+
+namespace hw {
+
+template <typename T> struct Rhs2DArgs {
+    using C = typename ctype<T>::type;
+    // RHS input, half spectra [kx][ky]
+    const C *eta_hat, *u_hat, *v_hat;
+    // spectral multipliers, half layout [kx][ky]
+    const T *csq_m, *csq_p, *om2_m, *om2_p, *aal, *c2d2g, *l2d;
+    const T *kxh, *ky;   // kx for kx=0..NX/2, ky for all ky
+    // physical coefficients [y][x]
+    const T *gam_m, *gam_p, *wallchar, *fbdy7, *chi, *new_gam, *skew_gam;
+    // source line values at t (per row for vertical, per column for horizontal)
+    const T *src_sig, *src_skew;
+    int src_on, vertical, propagation;
+    // breaking source, physical [y][x] (null when breaking is off)
+    const T *sbx, *sby;
+    // physical work fields
+    T *eta_p, *u_p, *v_p;
+    // x-intermediates [kx][y] / spectral work
+    C *W[16];
+    C *dH1;     // dH1 = duH1 + dvH1 (2DB), kept for the eta output
+    C *dH2p;    // partial dH2
+    // outputs: d/dt of eta^, u^, v^, half spectra
+    C *k_eta, *k_u, *k_v;
+    const C *twx, *twy;
+    T g;
+};
+
+template <typename T, int NX, int NY> struct Geo2 {
+    using C = typename ctype<T>::type;
+    using XG = XGeo<T, NX>;
+    using YG = YGeo<T, NY>;
+    static constexpr int NXH = NX / 2 + 1;
+    static constexpr int L = NX / 2;
+};
+
+// ---------------------------------------------------------------- helpers --
+template <typename T, int NX, int NY>
+__device__ __forceinline__ void ph_load(carr<T, XGeo<T, NX>::PP>& v, const T* __restrict__ a, int y0, int tid) {
+    using G = XGeo<T, NX>;
+    using C = typename ctype<T>::type;
+    const C* a2 = reinterpret_cast<const C*>(a) + (size_t)y0 * G::L;
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) {
+        int pos = tid + q * G::NT;
+        if (G::NPAIR % G::NT == 0 || pos < G::NPAIR) v[q] = a2[pos];
+    }
+}
+
+template <typename T, int NX, int NY>
+__device__ __forceinline__ void ph_store(T* __restrict__ a, const carr<T, XGeo<T, NX>::PP>& v, int y0, int tid) {
+    using G = XGeo<T, NX>;
+    using C = typename ctype<T>::type;
+    C* a2 = reinterpret_cast<C*>(a) + (size_t)y0 * G::L;
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) {
+        int pos = tid + q * G::NT;
+        if (G::NPAIR % G::NT == 0 || pos < G::NPAIR) a2[pos] = v[q];
+    }
+}
+
+// elementwise helpers on pair registers (each C holds two real samples)
+template <typename C> __device__ __forceinline__ C pmul(C a, C b) { return cmk<C>(a.x * b.x, a.y * b.y); }
+template <typename C> __device__ __forceinline__ C pfma(C a, C b, C c) { return cmk<C>(a.x * b.x + c.x, a.y * b.y + c.y); }
+
+// forward x of a real field in registers, stored transposed into X
+template <typename T, int NX, int NY>
+__device__ __forceinline__ void x_fwd_store(const carr<T, XGeo<T, NX>::PP>& f, typename ctype<T>::type* X,
+                                            int y0, typename ctype<T>::type* lds, const typename ctype<T>::type* twx, int tid) {
+    x_forward_lds<T, NX>(f, lds, twx, tid);
+    x_store_spec<T, NX, NY>(X, y0, lds, tid);
+}
+
+// forward x of f1 + i f2 (both real): X = F(f1) + i F(f2)
+template <typename T, int NX, int NY>
+__device__ __forceinline__ void x_fwd2_store(const carr<T, XGeo<T, NX>::PP>& f1,
+                                             const carr<T, XGeo<T, NX>::PP>& f2, typename ctype<T>::type* X,
+                                             int y0, typename ctype<T>::type* lds, const typename ctype<T>::type* twx, int tid) {
+    using G = XGeo<T, NX>;
+    using C = typename ctype<T>::type;
+    C s1[G::PS];
+    x_forward_lds<T, NX>(f1, lds, twx, tid);
+    x_spec_to_regs<T, NX>(s1, lds, tid);
+    x_forward_lds<T, NX>(f2, lds, twx, tid);
+#pragma unroll
+    for (int q = 0; q < G::PS; ++q) {
+        int idx = tid + q * G::NT;
+        if (G::NSPEC % G::NT == 0 || idx < G::NSPEC) {
+            int kx = idx / G::RR, r = idx % G::RR;
+            C b = lds[r * G::LS + kx];
+            X[(size_t)kx * NY + y0 + r] = cadd(s1[q], cmul_i<1>(b));
+        }
+    }
+}
+
+// y-pass column element load with optional real multiplier
+template <typename T, int NX, int NY>
+__device__ __forceinline__ void y_load(carr<T, YGeo<T, NY>::PE>& v, const typename ctype<T>::type* __restrict__ src,
+                                       const T* __restrict__ mul, int col0, int tid) {
+    using G = YGeo<T, NY>;
+    using C = typename ctype<T>::type;
+    constexpr int NXH = NX / 2 + 1;
+#pragma unroll
+    for (int e = 0; e < G::PE; ++e) {
+        int idx = tid + e * G::NT;
+        int c = idx / NY;
+        bool ok = (G::NE % G::NT == 0 || idx < G::NE) && (col0 + c < NXH);
+        if (ok) {
+            size_t gi = (size_t)col0 * NY + idx;
+            C x = src[gi];
+            if (mul) x = cscale(x, mul[gi]);
+            v[e] = x;
+        } else {
+            v[e] = cmk<C>((T)0, (T)0);
+        }
+    }
+}
+
+template <typename T, int NX, int NY>
+__device__ __forceinline__ void y_store(typename ctype<T>::type* __restrict__ dst, const carr<T, YGeo<T, NY>::PE>& v,
+                                        int col0, int tid) {
+    using G = YGeo<T, NY>;
+    constexpr int NXH = NX / 2 + 1;
+#pragma unroll
+    for (int e = 0; e < G::PE; ++e) {
+        int idx = tid + e * G::NT;
+        int c = idx / NY;
+        bool ok = (G::NE % G::NT == 0 || idx < G::NE) && (col0 + c < NXH);
+        if (ok) dst[(size_t)col0 * NY + idx] = v[e];
+    }
+}
+
+// in-register column transforms (LDS round trip)
+template <typename T, int NX, int NY>
+__device__ __forceinline__ void y_fwd_regs(carr<T, YGeo<T, NY>::PE>& v, typename ctype<T>::type* lds,
+                                           const typename ctype<T>::type* twy, int tid) {
+    __syncthreads();
+    y_regs_to_lds<T, NY>(v, lds, tid);
+    __syncthreads();
+    y_fft<T, NY, -1>(lds, twy, tid);
+    y_lds_to_regs<T, NY>(v, lds, tid);
+}
+
+template <typename T, int NX, int NY>
+__device__ __forceinline__ void y_inv_regs(carr<T, YGeo<T, NY>::PE>& v, typename ctype<T>::type* lds,
+                                           const typename ctype<T>::type* twy, int col0, int tid) {
+    __syncthreads();
+    y_regs_to_lds<T, NY>(v, lds, tid);
+    __syncthreads();
+    y_project<T, NY>(lds, col0, NX / 2 + 1, tid);
+    __syncthreads();
+    y_fft<T, NY, +1>(lds, twy, tid);
+    y_lds_to_regs<T, NY>(v, lds, tid);
+}
+
+// per-element spectral coordinates of this thread's y-pass element e
+template <typename T, int NX, int NY> struct YIdx {
+    __device__ __forceinline__ static void get(int col0, int tid, int e, int& kx, int& ky, size_t& gi) {
+        using G = YGeo<T, NY>;
+        int idx = tid + e * G::NT;
+        kx = col0 + idx / NY;
+        ky = idx % NY;
+        gi = (size_t)col0 * NY + idx;
+    }
+};
+
+// ========================================================== 2DB kernels ==
+#define HW_XK_PROLOGUE                                                   \
+    using C = typename ctype<T>::type;                                   \
+    using G = XGeo<T, NX>;                                               \
+    __shared__ C lds[G::LDS_ELEMS];                                      \
+    __shared__ T nyq[G::RR];                                             \
+    const int tid = threadIdx.x;                                         \
+    const int y0 = xcd_remap(blockIdx.x, gridDim.x) * G::RR;             \
+    const T sc = (T)1 / ((T)NX * (T)NY);                                 \
+    (void)nyq; (void)sc;
+
+#define HW_YK_PROLOGUE                                                   \
+    using C = typename ctype<T>::type;                                   \
+    using G = YGeo<T, NY>;                                               \
+    __shared__ C lds[G::LDS_ELEMS];                                      \
+    const int tid = threadIdx.x;                                         \
+    const int col0 = blockIdx.x * G::CC;
+
+// W slots used by the 2DB pipeline
+enum : int {
+    S_ETA = 0, S_U = 1, S_AMU = 2, S_APU = 3, S_V = 4, S_AMV = 5, S_APV = 6,
+    S_FGMU = 7, S_FGPU = 8, S_FGAU = 9, S_FGMV = 10, S_FGPV = 11, S_FGAV = 12,
+};
+
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y1(Rhs2DArgs<T> a) {
+    HW_YK_PROLOGUE
+    C v[G::PE], w[G::PE];
+    y_load<T, NX, NY>(v, a.eta_hat, nullptr, col0, tid);
+    y_inv_regs<T, NX, NY>(v, lds, a.twy, col0, tid);
+    y_store<T, NX, NY>(a.W[S_ETA], v, col0, tid);
+    const C* src[2] = {a.u_hat, a.v_hat};
+    const int slot[2][3] = {{S_U, S_AMU, S_APU}, {S_V, S_AMV, S_APV}};
+    #pragma unroll
+    for (int f = 0; f < 2; ++f) {
+        y_load<T, NX, NY>(v, src[f], nullptr, col0, tid);
+#pragma unroll
+        for (int e = 0; e < G::PE; ++e) w[e] = v[e];
+        y_inv_regs<T, NX, NY>(w, lds, a.twy, col0, tid);
+        y_store<T, NX, NY>(a.W[slot[f][0]], w, col0, tid);
+        #pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const T* mul = s == 0 ? a.csq_m : a.csq_p;
+#pragma unroll
+            for (int e = 0; e < G::PE; ++e) {
+                int kx, ky; size_t gi;
+                YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
+                w[e] = (kx < NX / 2 + 1) ? cscale(v[e], mul[gi]) : cmk<C>((T)0, (T)0);
+            }
+            y_inv_regs<T, NX, NY>(w, lds, a.twy, col0, tid);
+            y_store<T, NX, NY>(a.W[slot[f][1 + s]], w, col0, tid);
+        }
+    }
+}
+
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x1(Rhs2DArgs<T> a) {
+    HW_XK_PROLOGUE
+    C f[G::PP], acc[G::PP], gm[G::PP], gp[G::PP];
+    x_inverse<T, NX, NY>(a.W[S_ETA], y0, lds, a.twx, f, sc, nullptr, tid);
+    ph_store<T, NX, NY>(a.eta_p, f, y0, tid);
+    ph_load<T, NX, NY>(gm, a.gam_m, y0, tid);
+    ph_load<T, NX, NY>(gp, a.gam_p, y0, tid);
+    const int in_slot[2][3] = {{S_U, S_AMU, S_APU}, {S_V, S_AMV, S_APV}};
+    const int out_slot[2][3] = {{S_FGMU, S_FGPU, S_FGAU}, {S_FGMV, S_FGPV, S_FGAV}};
+    T* phys[2] = {a.u_p, a.v_p};
+    #pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        C u[G::PP];
+        x_inverse<T, NX, NY>(a.W[in_slot[c][0]], y0, lds, a.twx, u, sc, nullptr, tid);
+        ph_store<T, NX, NY>(phys[c], u, y0, tid);
+#pragma unroll
+        for (int q = 0; q < G::PP; ++q) f[q] = pmul(gm[q], u[q]);
+        x_fwd_store<T, NX, NY>(f, a.W[out_slot[c][0]], y0, lds, a.twx, tid);
+#pragma unroll
+        for (int q = 0; q < G::PP; ++q) f[q] = pmul(gp[q], u[q]);
+        x_fwd_store<T, NX, NY>(f, a.W[out_slot[c][1]], y0, lds, a.twx, tid);
+        x_inverse<T, NX, NY>(a.W[in_slot[c][1]], y0, lds, a.twx, f, sc, nullptr, tid);
+#pragma unroll
+        for (int q = 0; q < G::PP; ++q) acc[q] = pmul(gm[q], f[q]);
+        x_inverse<T, NX, NY>(a.W[in_slot[c][2]], y0, lds, a.twx, f, sc, nullptr, tid);
+#pragma unroll
+        for (int q = 0; q < G::PP; ++q) acc[q] = pfma(gp[q], f[q], acc[q]);
+        x_fwd_store<T, NX, NY>(acc, a.W[out_slot[c][2]], y0, lds, a.twx, tid);
+    }
+}
+
+// W slots after Y2
+enum : int { S_M0U = 0, S_M0V = 1 };
+
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y2(Rhs2DArgs<T> a) {
+    HW_YK_PROLOGUE
+    C A[G::PE], B[G::PE], Cc[G::PE], dsum[G::PE];
+    const int in_slot[2][3] = {{S_FGMU, S_FGPU, S_FGAU}, {S_FGMV, S_FGPV, S_FGAV}};
+    const T ig = (T)1 / a.g;
+    #pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        y_load<T, NX, NY>(A, a.W[in_slot[c][0]], nullptr, col0, tid);
+        y_fwd_regs<T, NX, NY>(A, lds, a.twy, tid);
+        y_load<T, NX, NY>(B, a.W[in_slot[c][1]], nullptr, col0, tid);
+        y_fwd_regs<T, NX, NY>(B, lds, a.twy, tid);
+        y_load<T, NX, NY>(Cc, a.W[in_slot[c][2]], nullptr, col0, tid);
+        y_fwd_regs<T, NX, NY>(Cc, lds, a.twy, tid);
+#pragma unroll
+        for (int e = 0; e < G::PE; ++e) {
+            int kx, ky; size_t gi;
+            YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
+            if (kx < NX / 2 + 1) {
+                C hs = cscale(cadd(Cc[e], cadd(cscale(A[e], a.csq_m[gi]), cscale(B[e], a.csq_p[gi]))), (T)0.5);
+                T kk = c == 0 ? a.kxh[kx] : a.ky[ky];
+                C d = cmul_i<-1>(cscale(hs, kk * ig));     // -i k HSS / g
+                A[e] = d;
+                dsum[e] = c == 0 ? d : cadd(dsum[e], d);
+            } else {
+                A[e] = cmk<C>((T)0, (T)0);
+            }
+        }
+        y_inv_regs<T, NX, NY>(A, lds, a.twy, col0, tid);
+        y_store<T, NX, NY>(a.W[c == 0 ? S_M0U : S_M0V], A, col0, tid);
+    }
+    y_store<T, NX, NY>(a.dH1, dsum, col0, tid);
+}
+
+// W slots after X3
+enum : int { S_P1 = 2, S_Q1 = 3, S_G1M = 4, S_G1P = 5, S_P2 = 6, S_Q2 = 7, S_G2M = 8, S_G2P = 9, S_E = 10 };
+
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x3(Rhs2DArgs<T> a) {
+    HW_XK_PROLOGUE
+    C m0u[G::PP], m0v[G::PP], w[G::PP], uu[G::PP], vv[G::PP], f[G::PP];
+    x_inverse<T, NX, NY>(a.W[S_M0U], y0, lds, a.twx, m0u, sc, nullptr, tid);
+    x_inverse<T, NX, NY>(a.W[S_M0V], y0, lds, a.twx, m0v, sc, nullptr, tid);
+    ph_load<T, NX, NY>(w, a.wallchar, y0, tid);
+    ph_load<T, NX, NY>(uu, a.u_p, y0, tid);
+    ph_load<T, NX, NY>(vv, a.v_p, y0, tid);
+    // E = 0.5 (u^2 + v^2 - Lphi^2) w, Lphi = M0u + M0v  (2d/fun_odesolver.c:3426)
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) {
+        C lp = cadd(m0u[q], m0v[q]);
+        f[q] = cmk<C>((T)0.5 * ((uu[q].x * uu[q].x + vv[q].x * vv[q].x) - lp.x * lp.x) * w[q].x,
+                      (T)0.5 * ((uu[q].y * uu[q].y + vv[q].y * vv[q].y) - lp.y * lp.y) * w[q].y);
+    }
+    x_fwd_store<T, NX, NY>(f, a.W[S_E], y0, lds, a.twx, tid);
+    ph_load<T, NX, NY>(f, a.eta_p, y0, tid);   // f holds eta from here on
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) { C ew = pmul(f[q], w[q]); uu[q] = pmul(ew, uu[q]); vv[q] = pmul(ew, vv[q]);
+                                      m0u[q] = pmul(ew, m0u[q]); m0v[q] = pmul(ew, m0v[q]); }
+    x_fwd_store<T, NX, NY>(uu, a.W[S_Q1], y0, lds, a.twx, tid);
+    x_fwd_store<T, NX, NY>(vv, a.W[S_Q2], y0, lds, a.twx, tid);
+    x_fwd_store<T, NX, NY>(m0u, a.W[S_P1], y0, lds, a.twx, tid);
+    x_fwd_store<T, NX, NY>(m0v, a.W[S_P2], y0, lds, a.twx, tid);
+    ph_load<T, NX, NY>(w, a.gam_m, y0, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) f[q] = pmul(w[q], m0u[q]);
+    x_fwd_store<T, NX, NY>(f, a.W[S_G1M], y0, lds, a.twx, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) f[q] = pmul(w[q], m0v[q]);
+    x_fwd_store<T, NX, NY>(f, a.W[S_G2M], y0, lds, a.twx, tid);
+    ph_load<T, NX, NY>(w, a.gam_p, y0, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) f[q] = pmul(w[q], m0u[q]);
+    x_fwd_store<T, NX, NY>(f, a.W[S_G1P], y0, lds, a.twx, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) f[q] = pmul(w[q], m0v[q]);
+    x_fwd_store<T, NX, NY>(f, a.W[S_G2P], y0, lds, a.twx, tid);
+}
+
+// W slots after Y4 (inputs S_P1.. are consumed)
+enum : int { S_B1M = 11, S_B1P = 12, S_B2M = 13, S_B2P = 14, S_AX = 0, S_AY = 1 };
+
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y4(Rhs2DArgs<T> a) {
+    HW_YK_PROLOGUE
+    C P[G::PE], Q[G::PE], Gm[G::PE], Gp[G::PE], part[G::PE], w[G::PE];
+    const T ig = (T)1 / a.g;
+    const int sl[2][4] = {{S_P1, S_Q1, S_G1M, S_G1P}, {S_P2, S_Q2, S_G2M, S_G2P}};
+    const int so[2][2] = {{S_B1M, S_B1P}, {S_B2M, S_B2P}};
+    #pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        y_load<T, NX, NY>(P, a.W[sl[c][0]], nullptr, col0, tid);
+        y_fwd_regs<T, NX, NY>(P, lds, a.twy, tid);
+        y_load<T, NX, NY>(Q, a.W[sl[c][1]], nullptr, col0, tid);
+        y_fwd_regs<T, NX, NY>(Q, lds, a.twy, tid);
+        y_load<T, NX, NY>(Gm, a.W[sl[c][2]], nullptr, col0, tid);
+        y_fwd_regs<T, NX, NY>(Gm, lds, a.twy, tid);
+        y_load<T, NX, NY>(Gp, a.W[sl[c][3]], nullptr, col0, tid);
+        y_fwd_regs<T, NX, NY>(Gp, lds, a.twy, tid);
+#pragma unroll
+        for (int e = 0; e < G::PE; ++e) {
+            int kx, ky; size_t gi;
+            YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
+            if (kx < NX / 2 + 1) {
+                T kk = c == 0 ? a.kxh[kx] : a.ky[ky];
+                // -i k (eta u)^ - (Om2- F(g- p) + Om2+ F(g+ p)) / (2 g)
+                C t1 = cmul_i<-1>(cscale(Q[e], kk));
+                C t2 = cscale(cadd(cscale(Gm[e], a.om2_m[gi]), cscale(Gp[e], a.om2_p[gi])), (T)0.5 * ig);
+                C d = csub(t1, t2);
+                part[e] = c == 0 ? d : cadd(part[e], d);
+            }
+        }
+        #pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const T* mul = s == 0 ? a.om2_m : a.om2_p;
+#pragma unroll
+            for (int e = 0; e < G::PE; ++e) {
+                int kx, ky; size_t gi;
+                YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
+                w[e] = (kx < NX / 2 + 1) ? cscale(P[e], mul[gi]) : cmk<C>((T)0, (T)0);
+            }
+            y_inv_regs<T, NX, NY>(w, lds, a.twy, col0, tid);
+            y_store<T, NX, NY>(a.W[so[c][s]], w, col0, tid);
+        }
+    }
+    y_store<T, NX, NY>(a.dH2p, part, col0, tid);
+    // adjust inputs i kx E^, i ky E^
+    y_load<T, NX, NY>(P, a.W[S_E], nullptr, col0, tid);
+    y_fwd_regs<T, NX, NY>(P, lds, a.twy, tid);
+    #pragma unroll
+    for (int s = 0; s < 2; ++s) {
+#pragma unroll
+        for (int e = 0; e < G::PE; ++e) {
+            int kx, ky; size_t gi;
+            YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
+            T kk = (kx < NX / 2 + 1) ? (s == 0 ? a.kxh[kx] : a.ky[ky]) : (T)0;
+            w[e] = cmul_i<1>(cscale(P[e], kk));
+        }
+        y_inv_regs<T, NX, NY>(w, lds, a.twy, col0, tid);
+        y_store<T, NX, NY>(a.W[s == 0 ? S_AX : S_AY], w, col0, tid);
+    }
+}
+
+// W slots after X5
+enum : int { S_ACC1 = 2, S_ACC2 = 3, S_ZU = 4, S_ZV = 5 };
+
+template <typename T, int NX, int NY>
+__device__ __forceinline__ void x5_hss_tail(const Rhs2DArgs<T>& a, int sm, int sp, int so, int y0,
+                                            typename ctype<T>::type* lds, int tid) {
+    using C = typename ctype<T>::type;
+    using G = XGeo<T, NX>;
+    const T sc = (T)1 / ((T)NX * (T)NY);
+    C f[G::PP], acc[G::PP], gg[G::PP];
+    x_inverse<T, NX, NY>(a.W[sm], y0, lds, a.twx, f, sc, nullptr, tid);
+    ph_load<T, NX, NY>(gg, a.gam_m, y0, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) acc[q] = pmul(gg[q], f[q]);
+    x_inverse<T, NX, NY>(a.W[sp], y0, lds, a.twx, f, sc, nullptr, tid);
+    ph_load<T, NX, NY>(gg, a.gam_p, y0, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) acc[q] = pfma(gg[q], f[q], acc[q]);
+    x_fwd_store<T, NX, NY>(acc, a.W[so], y0, lds, a.twx, tid);
+}
+
+// output transform of a velocity equation: f1 = Sb - chi A - fbdy7 vel,
+// f2 = -chi (-1)^x s(y)   (the imaginary part of fun_adjust's inverse)
+template <typename T, int NX, int NY>
+__device__ __forceinline__ void x_vel_out(const Rhs2DArgs<T>& a, int sa, const T* vel, const T* sb, int so, int y0,
+                                          typename ctype<T>::type* lds, T* nyq, int tid) {
+    using C = typename ctype<T>::type;
+    using G = XGeo<T, NX>;
+    const T sc = (T)1 / ((T)NX * (T)NY);
+    C f1[G::PP], f2[G::PP], t[G::PP];
+    x_inverse<T, NX, NY>(a.W[sa], y0, lds, a.twx, f1, sc, nyq, tid);
+    ph_load<T, NX, NY>(t, a.chi, y0, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) {
+        int pos = tid + q * G::NT;
+        int r = pos / G::L;
+        T s = nyq[r < G::RR ? r : 0];
+        f1[q] = cmk<C>(-t[q].x * f1[q].x, -t[q].y * f1[q].y);
+        f2[q] = cmk<C>(-t[q].x * s, t[q].y * s);
+    }
+    ph_load<T, NX, NY>(t, a.fbdy7, y0, tid);
+    C v[G::PP];
+    ph_load<T, NX, NY>(v, vel, y0, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) f1[q] = csub(f1[q], pmul(t[q], v[q]));
+    if (sb) {
+        ph_load<T, NX, NY>(t, sb, y0, tid);
+#pragma unroll
+        for (int q = 0; q < G::PP; ++q) f1[q] = cadd(f1[q], t[q]);
+    }
+    x_fwd2_store<T, NX, NY>(f1, f2, a.W[so], y0, lds, a.twx, tid);
+}
+
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x5(Rhs2DArgs<T> a) {
+    HW_XK_PROLOGUE
+    x5_hss_tail<T, NX, NY>(a, S_B1M, S_B1P, S_ACC1, y0, lds, tid);
+    x5_hss_tail<T, NX, NY>(a, S_B2M, S_B2P, S_ACC2, y0, lds, tid);
+    x_vel_out<T, NX, NY>(a, S_AX, a.u_p, a.sbx, S_ZU, y0, lds, nyq, tid);
+    x_vel_out<T, NX, NY>(a, S_AY, a.v_p, a.sby, S_ZV, y0, lds, nyq, tid);
+}
+
+// W slots after Y6
+enum : int { S_AETA = 6 };
+
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y6(Rhs2DArgs<T> a) {
+    HW_YK_PROLOGUE
+    C F1[G::PE], F2[G::PE], Z[G::PE], eh[G::PE];
+    const T ig = (T)1 / a.g;
+    y_load<T, NX, NY>(F1, a.W[S_ACC1], nullptr, col0, tid);
+    y_fwd_regs<T, NX, NY>(F1, lds, a.twy, tid);
+    y_load<T, NX, NY>(F2, a.W[S_ACC2], nullptr, col0, tid);
+    y_fwd_regs<T, NX, NY>(F2, lds, a.twy, tid);
+    y_load<T, NX, NY>(Z, a.dH2p, nullptr, col0, tid);
+#pragma unroll
+    for (int e = 0; e < G::PE; ++e) Z[e] = csub(Z[e], cscale(cadd(F1[e], F2[e]), (T)0.5 * ig));
+    y_inv_regs<T, NX, NY>(Z, lds, a.twy, col0, tid);
+    y_store<T, NX, NY>(a.W[S_AETA], Z, col0, tid);
+    y_load<T, NX, NY>(eh, a.eta_hat, nullptr, col0, tid);
+    #pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        y_load<T, NX, NY>(Z, a.W[c == 0 ? S_ZU : S_ZV], nullptr, col0, tid);
+        y_fwd_regs<T, NX, NY>(Z, lds, a.twy, tid);
+#pragma unroll
+        for (int e = 0; e < G::PE; ++e) {
+            int kx, ky; size_t gi;
+            YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
+            if (kx < NX / 2 + 1) {
+                T kk = c == 0 ? a.kxh[kx] : a.ky[ky];
+                C lin = cmul_i<-1>(cscale(eh[e], a.g * kk));
+                Z[e] = cscale(cadd(lin, Z[e]), a.aal[gi]);
+            }
+        }
+        y_store<T, NX, NY>(c == 0 ? a.k_u : a.k_v, Z, col0, tid);
+    }
+}
+
+// source term of fun_source (2d/fun_odesolver.c:909-960) for the pair q of this thread
+template <typename T, int NX>
+__device__ __forceinline__ void src_pair(const Rhs2DArgs<T>& a, int y0, int pos, typename ctype<T>::type ng,
+                                         typename ctype<T>::type sg, T& s0, T& s1) {
+    using G = XGeo<T, NX>;
+    int r = pos / G::L, n = pos % G::L;
+    T sig0, sig1, sk0, sk1;
+    if (a.vertical) {
+        sig0 = sig1 = a.src_sig[y0 + r];
+        sk0 = sk1 = a.src_skew[y0 + r];
+    } else {
+        sig0 = a.src_sig[2 * n]; sig1 = a.src_sig[2 * n + 1];
+        sk0 = a.src_skew[2 * n]; sk1 = a.src_skew[2 * n + 1];
+    }
+    if (a.propagation == 2) {
+        s0 = 2 * sig0 * ng.x; s1 = 2 * sig1 * ng.y;
+    } else {
+        T sgn = a.propagation == 0 ? (T)-1 : (T)1;
+        s0 = sig0 * ng.x + sgn * sk0 * sg.x;
+        s1 = sig1 * ng.y + sgn * sk1 * sg.y;
+    }
+}
+
+// output transform of the eta equation: f1 = S(t) - fbdy7 eta + chi A,
+// f2 = chi (-1)^x s(y)
+template <typename T, int NX, int NY>
+__device__ __forceinline__ void x_eta_out(const Rhs2DArgs<T>& a, int sa, int so, int y0,
+                                          typename ctype<T>::type* lds, T* nyq, int tid) {
+    using C = typename ctype<T>::type;
+    using G = XGeo<T, NX>;
+    const T sc = (T)1 / ((T)NX * (T)NY);
+    C f1[G::PP], f2[G::PP], t[G::PP], t2[G::PP];
+    x_inverse<T, NX, NY>(a.W[sa], y0, lds, a.twx, f1, sc, nyq, tid);
+    ph_load<T, NX, NY>(t, a.chi, y0, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) {
+        int pos = tid + q * G::NT;
+        int r = pos / G::L;
+        T s = nyq[r < G::RR ? r : 0];
+        f1[q] = pmul(t[q], f1[q]);
+        f2[q] = cmk<C>(t[q].x * s, -t[q].y * s);
+    }
+    ph_load<T, NX, NY>(t, a.fbdy7, y0, tid);
+    ph_load<T, NX, NY>(t2, a.eta_p, y0, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) f1[q] = csub(f1[q], pmul(t[q], t2[q]));
+    if (a.src_on) {
+        ph_load<T, NX, NY>(t, a.new_gam, y0, tid);
+        ph_load<T, NX, NY>(t2, a.skew_gam, y0, tid);
+#pragma unroll
+        for (int q = 0; q < G::PP; ++q) {
+            int pos = tid + q * G::NT;
+            T s0, s1;
+            src_pair<T, NX>(a, y0, pos, t[q], t2[q], s0, s1);
+            f1[q] = cadd(f1[q], cmk<C>(s0, s1));
+        }
+    }
+    x_fwd2_store<T, NX, NY>(f1, f2, a.W[so], y0, lds, a.twx, tid);
+}
+
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x7(Rhs2DArgs<T> a) {
+    HW_XK_PROLOGUE
+    x_eta_out<T, NX, NY>(a, S_AETA, S_ZU, y0, lds, nyq, tid);
+}
+
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y8(Rhs2DArgs<T> a) {
+    HW_YK_PROLOGUE
+    C Z[G::PE], d[G::PE];
+    y_load<T, NX, NY>(Z, a.W[S_ZU], nullptr, col0, tid);
+    y_fwd_regs<T, NX, NY>(Z, lds, a.twy, tid);
+    y_load<T, NX, NY>(d, a.dH1, nullptr, col0, tid);
+#pragma unroll
+    for (int e = 0; e < G::PE; ++e) {
+        int kx, ky; size_t gi;
+        YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
+        if (kx < NX / 2 + 1) Z[e] = cscale(cadd(d[e], Z[e]), a.aal[gi]);
+    }
+    y_store<T, NX, NY>(a.k_eta, Z, col0, tid);
+}
+
+// ========================================================== 2DF kernels ==
+// rhs_AB2_2DF_uv without walls: delphiH1 = -i C2d^2/g (kx u^ + ky v^) is
+// pointwise on the input; M0u^ = -i kx C2d^2/g u^, M0v^ likewise
+// (2d/fun_odesolver.c:2223-2236, 3172-3178).
+enum : int { F_ETA = 0, F_U = 1, F_V = 2, F_M0U = 3, F_M0V = 4 };
+
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2f_y1(Rhs2DArgs<T> a) {
+    HW_YK_PROLOGUE
+    C v[G::PE], w[G::PE];
+    y_load<T, NX, NY>(v, a.eta_hat, nullptr, col0, tid);
+    y_inv_regs<T, NX, NY>(v, lds, a.twy, col0, tid);
+    y_store<T, NX, NY>(a.W[F_ETA], v, col0, tid);
+    #pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        y_load<T, NX, NY>(v, c == 0 ? a.u_hat : a.v_hat, nullptr, col0, tid);
+#pragma unroll
+        for (int e = 0; e < G::PE; ++e) {
+            int kx, ky; size_t gi;
+            YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
+            T kk = (kx < NX / 2 + 1) ? (c == 0 ? a.kxh[kx] : a.ky[ky]) * a.c2d2g[gi] : (T)0;
+            w[e] = cmul_i<-1>(cscale(v[e], kk));
+        }
+        y_inv_regs<T, NX, NY>(v, lds, a.twy, col0, tid);
+        y_store<T, NX, NY>(a.W[c == 0 ? F_U : F_V], v, col0, tid);
+        y_inv_regs<T, NX, NY>(w, lds, a.twy, col0, tid);
+        y_store<T, NX, NY>(a.W[c == 0 ? F_M0U : F_M0V], w, col0, tid);
+    }
+}
+
+enum : int { F_P1 = 5, F_Q1 = 6, F_P2 = 7, F_Q2 = 8, F_E = 9 };
+
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2f_x2(Rhs2DArgs<T> a) {
+    HW_XK_PROLOGUE
+    C e[G::PP], w[G::PP], u[G::PP], lp[G::PP], m[G::PP], f[G::PP];
+    x_inverse<T, NX, NY>(a.W[F_ETA], y0, lds, a.twx, e, sc, nullptr, tid);
+    ph_store<T, NX, NY>(a.eta_p, e, y0, tid);
+    ph_load<T, NX, NY>(w, a.wallchar, y0, tid);
+    x_inverse<T, NX, NY>(a.W[F_U], y0, lds, a.twx, u, sc, nullptr, tid);
+    ph_store<T, NX, NY>(a.u_p, u, y0, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) f[q] = pmul(pmul(e[q], u[q]), w[q]);
+    x_fwd_store<T, NX, NY>(f, a.W[F_Q1], y0, lds, a.twx, tid);
+    x_inverse<T, NX, NY>(a.W[F_V], y0, lds, a.twx, m, sc, nullptr, tid);
+    ph_store<T, NX, NY>(a.v_p, m, y0, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) {
+        f[q] = pmul(pmul(e[q], m[q]), w[q]);
+        u[q] = cmk<C>(u[q].x * u[q].x + m[q].x * m[q].x, u[q].y * u[q].y + m[q].y * m[q].y);   // u^2 + v^2
+    }
+    x_fwd_store<T, NX, NY>(f, a.W[F_Q2], y0, lds, a.twx, tid);
+    x_inverse<T, NX, NY>(a.W[F_M0U], y0, lds, a.twx, m, sc, nullptr, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) { f[q] = pmul(pmul(e[q], m[q]), w[q]); lp[q] = m[q]; }
+    x_fwd_store<T, NX, NY>(f, a.W[F_P1], y0, lds, a.twx, tid);
+    x_inverse<T, NX, NY>(a.W[F_M0V], y0, lds, a.twx, m, sc, nullptr, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) { f[q] = pmul(pmul(e[q], m[q]), w[q]); lp[q] = cadd(lp[q], m[q]); }
+    x_fwd_store<T, NX, NY>(f, a.W[F_P2], y0, lds, a.twx, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q)
+        f[q] = cmk<C>((T)0.5 * (u[q].x - lp[q].x * lp[q].x) * w[q].x, (T)0.5 * (u[q].y - lp[q].y * lp[q].y) * w[q].y);
+    x_fwd_store<T, NX, NY>(f, a.W[F_E], y0, lds, a.twx, tid);
+}
+
+enum : int { F_AETA = 0, F_AX = 1, F_AY = 2 };
+
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2f_y3(Rhs2DArgs<T> a) {
+    HW_YK_PROLOGUE
+    C P[G::PE], Q[G::PE], acc[G::PE];
+    const int sl[2][2] = {{F_P1, F_Q1}, {F_P2, F_Q2}};
+    #pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        y_load<T, NX, NY>(P, a.W[sl[c][0]], nullptr, col0, tid);
+        y_fwd_regs<T, NX, NY>(P, lds, a.twy, tid);
+        y_load<T, NX, NY>(Q, a.W[sl[c][1]], nullptr, col0, tid);
+        y_fwd_regs<T, NX, NY>(Q, lds, a.twy, tid);
+#pragma unroll
+        for (int e = 0; e < G::PE; ++e) {
+            int kx, ky; size_t gi;
+            YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
+            if (kx < NX / 2 + 1) {
+                T kk = c == 0 ? a.kxh[kx] : a.ky[ky];
+                // -L2d (eta M0)^ - i k (eta u)^   (2d/fun_odesolver.c:3293-3297)
+                C d = csub(cmul_i<-1>(cscale(Q[e], kk)), cscale(P[e], a.l2d[gi]));
+                acc[e] = c == 0 ? d : cadd(acc[e], d);
+            } else {
+                acc[e] = cmk<C>((T)0, (T)0);
+            }
+        }
+    }
+    y_inv_regs<T, NX, NY>(acc, lds, a.twy, col0, tid);
+    y_store<T, NX, NY>(a.W[F_AETA], acc, col0, tid);
+    y_load<T, NX, NY>(P, a.W[F_E], nullptr, col0, tid);
+    y_fwd_regs<T, NX, NY>(P, lds, a.twy, tid);
+    #pragma unroll
+    for (int s = 0; s < 2; ++s) {
+#pragma unroll
+        for (int e = 0; e < G::PE; ++e) {
+            int kx, ky; size_t gi;
+            YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
+            T kk = (kx < NX / 2 + 1) ? (s == 0 ? a.kxh[kx] : a.ky[ky]) : (T)0;
+            Q[e] = cmul_i<1>(cscale(P[e], kk));
+        }
+        y_inv_regs<T, NX, NY>(Q, lds, a.twy, col0, tid);
+        y_store<T, NX, NY>(a.W[s == 0 ? F_AX : F_AY], Q, col0, tid);
+    }
+}
+
+enum : int { F_ZETA = 3, F_ZU = 4, F_ZV = 5 };
+
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2f_x4(Rhs2DArgs<T> a) {
+    HW_XK_PROLOGUE
+    x_eta_out<T, NX, NY>(a, F_AETA, F_ZETA, y0, lds, nyq, tid);
+    x_vel_out<T, NX, NY>(a, F_AX, a.u_p, a.sbx, F_ZU, y0, lds, nyq, tid);
+    x_vel_out<T, NX, NY>(a, F_AY, a.v_p, a.sby, F_ZV, y0, lds, nyq, tid);
+}
+
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2f_y5(Rhs2DArgs<T> a) {
+    HW_YK_PROLOGUE
+    C Z[G::PE], eh[G::PE], uh[G::PE], vh[G::PE];
+    y_load<T, NX, NY>(eh, a.eta_hat, nullptr, col0, tid);
+    y_load<T, NX, NY>(uh, a.u_hat, nullptr, col0, tid);
+    y_load<T, NX, NY>(vh, a.v_hat, nullptr, col0, tid);
+    const int zs[3] = {F_ZETA, F_ZU, F_ZV};
+    C* outs[3] = {a.k_eta, a.k_u, a.k_v};
+    #pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        y_load<T, NX, NY>(Z, a.W[zs[c]], nullptr, col0, tid);
+        y_fwd_regs<T, NX, NY>(Z, lds, a.twy, tid);
+#pragma unroll
+        for (int e = 0; e < G::PE; ++e) {
+            int kx, ky; size_t gi;
+            YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
+            if (kx < NX / 2 + 1) {
+                C lin;
+                if (c == 0) {
+                    // delphiH1 = -i C2d^2/g (kx u^ + ky v^)
+                    C s = cadd(cscale(uh[e], a.kxh[kx]), cscale(vh[e], a.ky[ky]));
+                    lin = cmul_i<-1>(cscale(s, a.c2d2g[gi]));
+                } else {
+                    T kk = c == 1 ? a.kxh[kx] : a.ky[ky];
+                    lin = cmul_i<-1>(cscale(eh[e], a.g * kk));
+                }
+                Z[e] = cscale(cadd(lin, Z[e]), a.aal[gi]);
+            }
+        }
+        y_store<T, NX, NY>(outs[c], Z, col0, tid);
+    }
+}
+
+}  // namespace hw

@@ -1,0 +1,553 @@
+// solver2d.h -- host side of the 2-D path: device buffers, the RHS launch
+// sequence, the GSL-semantics RKF45 driver and boundary conversions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/hawassi_hip.h"
+#include "aux_kernels.h"
+
+namespace hw {
+
+void set_error(const char* fmt, ...);
+
+#define HW_TRY(expr)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            ::hw::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),  \
+                            __FILE__, __LINE__);                                     \
+            return HW_ERR_HIP;                                                       \
+        }                                                                            \
+    } while (0)
+
+#define HW_LAUNCH_CHECK()                                                            \
+    do {                                                                             \
+        hipError_t e_ = hipGetLastError();                                           \
+        if (e_ != hipSuccess) {                                                      \
+            ::hw::set_error("kernel launch failed: %s (%s:%d)", hipGetErrorString(e_), \
+                            __FILE__, __LINE__);                                     \
+            return HW_ERR_HIP;                                                       \
+        }                                                                            \
+    } while (0)
+
+// Fehlberg tableau (GSL rkf45)
+static constexpr double RK_AH[5] = {1.0 / 4.0, 3.0 / 8.0, 12.0 / 13.0, 1.0, 1.0 / 2.0};
+static constexpr double RK_B3[2] = {3.0 / 32.0, 9.0 / 32.0};
+static constexpr double RK_B4[3] = {1932.0 / 2197.0, -7200.0 / 2197.0, 7296.0 / 2197.0};
+static constexpr double RK_B5[4] = {8341.0 / 4104.0, -32832.0 / 4104.0, 29440.0 / 4104.0, -845.0 / 4104.0};
+static constexpr double RK_B6[5] = {-6080.0 / 20520.0, 41040.0 / 20520.0, -28352.0 / 20520.0, 9295.0 / 20520.0,
+                                    -5643.0 / 20520.0};
+
+// control_y hadjust (GSL standard control with a_y = 1, a_dydt = 0, order 5)
+// returns -1 decrease, 0 keep, +1 increase.
+inline int hadjust_rule(double rmax, double* h) {
+    const double S = 0.9;
+    const int ord = 5;
+    double h_old = *h;
+    if (rmax > 1.1) {
+        double r = S / std::pow(rmax, 1.0 / ord);
+        if (r < 0.2) r = 0.2;
+        *h = r * h_old;
+        return -1;
+    } else if (rmax < 0.5) {
+        double r = S / std::pow(rmax, 1.0 / (ord + 1.0));
+        if (r > 5.0) r = 5.0;
+        if (r < 1.0) r = 1.0;
+        *h = r * h_old;
+        return 1;
+    }
+    return 0;
+}
+
+inline int bsearch_interval(const std::vector<double>& x, double t) {
+    int ilo = 0, ihi = (int)x.size() - 1;
+    while (ihi > ilo + 1) {
+        int i = (ihi + ilo) / 2;
+        if (x[i] > t) ihi = i; else ilo = i;
+    }
+    return ilo;
+}
+
+struct Solver2D {
+    virtual ~Solver2D() {}
+    virtual int init(const hw_config_2d* c) = 0;
+    virtual int rhs_full_dev(double t, const double* d_in, double* d_out) = 0;
+    virtual int set_state_dev(const double* d_full) = 0;
+    virtual int get_state_dev(double* d_full) = 0;
+    virtual int evolve_apply(double* t, double t1, double* h) = 0;
+    virtual int fields_dev(double* eta, double* u, double* v) = 0;
+    int nx = 0, ny = 0, prec = 0;
+    hipStream_t stream = nullptr;
+    hw_stats st{};
+    double* d_full_a = nullptr;   // boundary staging, 6*nx*ny doubles
+    double* d_full_b = nullptr;
+};
+
+template <typename T> struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    int alloc(size_t count) {
+        n = count;
+        hipError_t e = hipMalloc((void**)&p, sizeof(T) * (count ? count : 1));
+        if (e != hipSuccess) {
+            set_error("hipMalloc(%zu bytes) failed: %s", sizeof(T) * count, hipGetErrorString(e));
+            p = nullptr;
+            return HW_ERR_HIP;
+        }
+        return HW_OK;
+    }
+    int upload(const std::vector<T>& h, hipStream_t s) {
+        int rc = alloc(h.size());
+        if (rc) return rc;
+        HW_TRY(hipMemcpyAsync(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+        return HW_OK;
+    }
+};
+
+template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
+    using C = typename ctype<T>::type;
+    using XG = XGeo<T, NX>;
+    using YG = YGeo<T, NY>;
+    static constexpr int NXH = NX / 2 + 1;
+    static constexpr size_t H = (size_t)NXH * NY;
+    static constexpr size_t N = (size_t)NX * NY;
+
+    DevBuf<T> csq_m, csq_p, om2_m, om2_p, aal, c2d2g, l2d, kxh, kyv;
+    DevBuf<T> gam_m, gam_p, wall, fbdy7, chi, newgam, skewgam;
+    DevBuf<C> twx, twy;
+    DevBuf<T> src_sig, src_skew;
+    DevBuf<double> sig_y, sig_c, skew_y, skew_c;
+    std::vector<double> tsig;
+    int nline = 0, nt = 0;
+    DevBuf<T> eta_p, u_p, v_p;
+    DevBuf<C> work;     // 16 W slots + dH1 + dH2p
+    DevBuf<double2> y, ynew;
+    DevBuf<C> kbuf;     // 6 stage derivatives
+    DevBuf<C> ytmp;
+    DevBuf<unsigned long long> rmax;
+    unsigned long long* h_rmax = nullptr;
+    Rhs2DArgs<T> args{};
+    bool flat = false;
+    int src_on = 0;
+    double g = 9.80665, eps_abs = 1e-3, eps_rel = 1e-2;
+    // HIP events bracketing every RHS evaluation on the context's stream
+    static constexpr int NEV = 64;
+    hipEvent_t ev_a[NEV] = {}, ev_b[NEV] = {};
+    int nev = 0;
+
+    ~Solver2DT() override {
+        if (h_rmax) (void)hipHostFree(h_rmax);
+        for (int i = 0; i < NEV; ++i) {
+            if (ev_a[i]) (void)hipEventDestroy(ev_a[i]);
+            if (ev_b[i]) (void)hipEventDestroy(ev_b[i]);
+        }
+        if (d_full_a) (void)hipFree(d_full_a);
+        if (d_full_b) (void)hipFree(d_full_b);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // full-grid [ny][nx] -> half layout [kx][ky], kx <= nx/2
+    static std::vector<T> to_half(const double* a, double scale = 1.0) {
+        std::vector<T> out(H);
+        for (int kx = 0; kx < NXH; ++kx)
+            for (int ky = 0; ky < NY; ++ky) out[(size_t)kx * NY + ky] = (T)(a[(size_t)ky * NX + kx] * scale);
+        return out;
+    }
+    static std::vector<T> to_phys(const double* a, double scale = 1.0) {
+        std::vector<T> out(N);
+        for (size_t i = 0; i < N; ++i) out[i] = (T)(a[i] * scale);
+        return out;
+    }
+    static std::vector<C> twiddles(int n) {
+        std::vector<C> tw(n);
+        for (int m = 0; m < n; ++m) {
+            double a = -2.0 * M_PI * (double)m / (double)n;
+            tw[m].x = (T)std::cos(a);
+            tw[m].y = (T)std::sin(a);
+        }
+        return tw;
+    }
+
+    int init(const hw_config_2d* c) override {
+        nx = NX; ny = NY; prec = c->precision;
+        flat = c->flat != 0;
+        g = c->grav; eps_abs = c->eps_abs; eps_rel = c->eps_rel;
+        HW_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        for (int i = 0; i < NEV; ++i) {
+            HW_TRY(hipEventCreate(&ev_a[i]));
+            HW_TRY(hipEventCreate(&ev_b[i]));
+        }
+        int rc = 0;
+        if (!c->kx || !c->ky || !c->aal || !c->fbdy || !c->wallchar || !c->chi) {
+            set_error("hw_config_2d: kx, ky, aal, fbdy, wallchar and chi are required");
+            return HW_ERR_ARG;
+        }
+        std::vector<T> kx(NXH), ky(NY);
+        for (int i = 0; i < NXH; ++i) kx[i] = (T)c->kx[i];
+        for (int i = 0; i < NY; ++i) ky[i] = (T)c->ky[i];
+        if ((rc = kxh.upload(kx, stream)) || (rc = kyv.upload(ky, stream))) return rc;
+        if ((rc = aal.upload(to_half(c->aal), stream))) return rc;
+        if ((rc = fbdy7.upload(to_phys(c->fbdy, 7.0 * c->cpeak), stream))) return rc;
+        if ((rc = wall.upload(to_phys(c->wallchar), stream))) return rc;
+        if ((rc = chi.upload(to_phys(c->chi), stream))) return rc;
+        if (flat) {
+            if (!c->c2d || !c->l2d) { set_error("flat model needs c2d and l2d"); return HW_ERR_ARG; }
+            std::vector<double> c2(N);
+            for (size_t i = 0; i < N; ++i) c2[i] = c->c2d[i] * c->c2d[i] / c->grav;
+            if ((rc = c2d2g.upload(to_half(c2.data()), stream)) || (rc = l2d.upload(to_half(c->l2d), stream))) return rc;
+        } else {
+            if (!c->gam_min || !c->gam_plus || !c->csq_min || !c->csq_plus || !c->om2_min || !c->om2_plus) {
+                set_error("two-depth model needs gam/csq/om2 min and plus");
+                return HW_ERR_ARG;
+            }
+            if ((rc = gam_m.upload(to_phys(c->gam_min), stream)) || (rc = gam_p.upload(to_phys(c->gam_plus), stream)) ||
+                (rc = csq_m.upload(to_half(c->csq_min), stream)) || (rc = csq_p.upload(to_half(c->csq_plus), stream)) ||
+                (rc = om2_m.upload(to_half(c->om2_min), stream)) || (rc = om2_p.upload(to_half(c->om2_plus), stream)))
+                return rc;
+        }
+        if ((rc = twx.upload(twiddles(NX), stream)) || (rc = twy.upload(twiddles(NY), stream))) return rc;
+        src_on = c->source_on;
+        if (src_on) {
+            nline = c->nline; nt = c->nt;
+            int want = c->vertical ? NY : NX;
+            if (nline != want || nt < 3 || !c->t_sig || !c->sig || !c->skew_sig || !c->new_gam || !c->skew_gam) {
+                set_error("source: nline must be %d and t_sig/sig/skew_sig/new_gam/skew_gam given (nt>=3)", want);
+                return HW_ERR_ARG;
+            }
+            tsig.assign(c->t_sig, c->t_sig + nt);
+            // natural-spline half second derivatives per line (gsl cspline), tables [nt][nline]
+            std::vector<double> ty((size_t)nt * nline), tc((size_t)nt * nline), ky_((size_t)nt * nline), kc((size_t)nt * nline);
+            std::vector<double> yy(nt), cc(nt);
+            for (int s = 0; s < 2; ++s) {
+                const double* src = s == 0 ? c->sig : c->skew_sig;
+                for (int j = 0; j < nline; ++j) {
+                    for (int n = 0; n < nt; ++n) yy[n] = src[(size_t)j * nt + n];
+                    spline_half_second(tsig.data(), yy.data(), nt, cc.data());
+                    for (int n = 0; n < nt; ++n) {
+                        (s == 0 ? ty : ky_)[(size_t)n * nline + j] = yy[n];
+                        (s == 0 ? tc : kc)[(size_t)n * nline + j] = cc[n];
+                    }
+                }
+            }
+            if ((rc = sig_y.upload(ty, stream)) || (rc = sig_c.upload(tc, stream)) || (rc = skew_y.upload(ky_, stream)) ||
+                (rc = skew_c.upload(kc, stream)))
+                return rc;
+            if ((rc = src_sig.alloc(nline)) || (rc = src_skew.alloc(nline))) return rc;
+            if ((rc = newgam.upload(to_phys(c->new_gam), stream)) || (rc = skewgam.upload(to_phys(c->skew_gam), stream))) return rc;
+        }
+        if ((rc = eta_p.alloc(N)) || (rc = u_p.alloc(N)) || (rc = v_p.alloc(N))) return rc;
+        if ((rc = work.alloc(18 * H))) return rc;
+        if ((rc = y.alloc(3 * H)) || (rc = ynew.alloc(3 * H)) || (rc = kbuf.alloc(6 * 3 * H)) || (rc = ytmp.alloc(3 * H))) return rc;
+        if ((rc = rmax.alloc(1))) return rc;
+        HW_TRY(hipHostMalloc((void**)&h_rmax, sizeof(unsigned long long), hipHostMallocDefault));
+        HW_TRY(hipMemsetAsync(y.p, 0, sizeof(double2) * 3 * H, stream));
+        // fixed kernel arguments
+        args.csq_m = csq_m.p; args.csq_p = csq_p.p; args.om2_m = om2_m.p; args.om2_p = om2_p.p;
+        args.aal = aal.p; args.c2d2g = c2d2g.p; args.l2d = l2d.p; args.kxh = kxh.p; args.ky = kyv.p;
+        args.gam_m = gam_m.p; args.gam_p = gam_p.p; args.wallchar = wall.p; args.fbdy7 = fbdy7.p; args.chi = chi.p;
+        args.new_gam = newgam.p; args.skew_gam = skewgam.p;
+        args.src_sig = src_sig.p; args.src_skew = src_skew.p;
+        args.src_on = src_on; args.vertical = c->vertical; args.propagation = c->propagation;
+        args.sbx = nullptr; args.sby = nullptr;
+        args.eta_p = eta_p.p; args.u_p = u_p.p; args.v_p = v_p.p;
+        for (int i = 0; i < 16; ++i) args.W[i] = work.p + (size_t)i * H;
+        args.dH1 = work.p + 16 * H;
+        args.dH2p = work.p + 17 * H;
+        args.twx = twx.p; args.twy = twy.p;
+        args.g = (T)g;
+        HW_TRY(hipStreamSynchronize(stream));
+        return HW_OK;
+    }
+
+    static void spline_half_second(const double* x, const double* yv, int n, double* cc) {
+        for (int i = 0; i < n; ++i) cc[i] = 0.0;
+        if (n < 3) return;
+        int m = n - 2;
+        std::vector<double> diag(m), off(m), rhs(m);
+        for (int i = 0; i < m; ++i) {
+            double h0 = x[i + 1] - x[i], h1 = x[i + 2] - x[i + 1];
+            diag[i] = 2.0 * (h0 + h1);
+            off[i] = h1;
+            rhs[i] = 3.0 * ((yv[i + 2] - yv[i + 1]) / h1 - (yv[i + 1] - yv[i]) / h0);
+        }
+        for (int i = 1; i < m; ++i) {
+            double w = off[i - 1] / diag[i - 1];
+            diag[i] -= w * off[i - 1];
+            rhs[i] -= w * rhs[i - 1];
+        }
+        cc[m] = rhs[m - 1] / diag[m - 1];
+        for (int i = m - 2; i >= 0; --i) cc[i + 1] = (rhs[i] - off[i] * cc[i + 2]) / diag[i];
+    }
+
+    // accumulate the recorded RHS durations into st.rhs_ms_total
+    int collect_timing() {
+        for (int i = 0; i < nev; ++i) {
+            HW_TRY(hipEventSynchronize(ev_b[i]));
+            float ms = 0.f;
+            HW_TRY(hipEventElapsedTime(&ms, ev_a[i], ev_b[i]));
+            st.rhs_ms_total += ms;
+        }
+        nev = 0;
+        return HW_OK;
+    }
+
+    // ------------------------------------------------------------ RHS --
+    int rhs(double t, const C* in, C* out) {
+        if (nev == NEV) {
+            int rc = collect_timing();
+            if (rc) return rc;
+        }
+        HW_TRY(hipEventRecord(ev_a[nev], stream));
+        args.eta_hat = in; args.u_hat = in + H; args.v_hat = in + 2 * H;
+        args.k_eta = out; args.k_u = out + H; args.k_v = out + 2 * H;
+        if (src_on) {
+            int idx = bsearch_interval(tsig, t);
+            double hh = tsig[idx + 1] - tsig[idx];
+            k_src_eval<T><<<(nline + 255) / 256, 256, 0, stream>>>(sig_y.p, sig_c.p, skew_y.p, skew_c.p, nline, idx,
+                                                                   t - tsig[idx], hh, src_sig.p, src_skew.p);
+            HW_LAUNCH_CHECK();
+        }
+        const dim3 gx(NY / XG::RR), bx(XG::NT);
+        const dim3 gy((NXH + YG::CC - 1) / YG::CC), by(YG::NT);
+        if (!flat) {
+            k2b_y1<T, NX, NY><<<gy, by, 0, stream>>>(args);
+            k2b_x1<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+            k2b_y2<T, NX, NY><<<gy, by, 0, stream>>>(args);
+            k2b_x3<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+            k2b_y4<T, NX, NY><<<gy, by, 0, stream>>>(args);
+            k2b_x5<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+            k2b_y6<T, NX, NY><<<gy, by, 0, stream>>>(args);
+            k2b_x7<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+            k2b_y8<T, NX, NY><<<gy, by, 0, stream>>>(args);
+        } else {
+            k2f_y1<T, NX, NY><<<gy, by, 0, stream>>>(args);
+            k2f_x2<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+            k2f_y3<T, NX, NY><<<gy, by, 0, stream>>>(args);
+            k2f_x4<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+            k2f_y5<T, NX, NY><<<gy, by, 0, stream>>>(args);
+        }
+        HW_LAUNCH_CHECK();
+        HW_TRY(hipEventRecord(ev_b[nev], stream));
+        nev++;
+        st.rhs_evals++;
+        return HW_OK;
+    }
+
+    static dim3 grid1(size_t n) {
+        size_t b = (n + 255) / 256;
+        if (b > 65535u * 8) b = 65535u * 8;
+        return dim3((unsigned)b);
+    }
+
+    int rhs_full_dev(double t, const double* d_in, double* d_out) override {
+        dim3 b(32, 8), gi((NXH + 31) / 32, (NY + 31) / 32, 3), go((NX + 31) / 32, (NY + 31) / 32, 3);
+        k_full_to_half<C><<<gi, b, 0, stream>>>(d_in, ytmp.p, NX, NY);
+        HW_LAUNCH_CHECK();
+        int rc = rhs(0.0 + t, ytmp.p, kbuf.p);
+        if (rc) return rc;
+        k_half_to_full<C><<<go, b, 0, stream>>>(kbuf.p, d_out, NX, NY);
+        HW_LAUNCH_CHECK();
+        HW_TRY(hipStreamSynchronize(stream));
+        return collect_timing();
+    }
+
+    int set_state_dev(const double* d_full) override {
+        dim3 b(32, 8), gi((NXH + 31) / 32, (NY + 31) / 32, 3);
+        k_full_to_half<double2><<<gi, b, 0, stream>>>(d_full, y.p, NX, NY);
+        HW_LAUNCH_CHECK();
+        HW_TRY(hipStreamSynchronize(stream));
+        return HW_OK;
+    }
+
+    int get_state_dev(double* d_full) override {
+        dim3 b(32, 8), go((NX + 31) / 32, (NY + 31) / 32, 3);
+        k_half_to_full<double2><<<go, b, 0, stream>>>(y.p, d_full, NX, NY);
+        HW_LAUNCH_CHECK();
+        HW_TRY(hipStreamSynchronize(stream));
+        return HW_OK;
+    }
+
+    int stage(double h, std::initializer_list<int> ks, std::initializer_list<double> bs) {
+        StageCoef sc{};
+        sc.nk = 0;
+        auto bi = bs.begin();
+        for (int kidx : ks) {
+            sc.k[sc.nk] = kbuf.p + (size_t)kidx * 3 * H;
+            sc.b[sc.nk] = *bi++;
+            sc.nk++;
+        }
+        k_rk_stage<T><<<grid1(3 * H), 256, 0, stream>>>(y.p, sc, h, ytmp.p, 3 * H);
+        HW_LAUNCH_CHECK();
+        return HW_OK;
+    }
+
+    C* K(int i) { return kbuf.p + (size_t)i * 3 * H; }
+
+    // gsl_odeiv2_evolve_apply with rkf45 + control_y, restated (see header).
+    // The rkf45 stepper's dydt_out evaluation is not performed: control_y
+    // does not read it and the RHS has no side effects on this path.
+    int evolve_apply(double* t, double t1, double* h) override {
+        const double t0 = *t;
+        double h0 = *h;
+        const double dt = t1 - t0;
+        if ((dt < 0.0 && h0 > 0.0) || (dt > 0.0 && h0 < 0.0)) {
+            set_error("step direction must match interval direction");
+            return HW_ERR_ARG;
+        }
+        int rc;
+        if ((rc = stage(0.0, {}, {}))) return rc;
+        if ((rc = rhs(t0, ytmp.p, K(0)))) return rc;
+        for (;;) {
+            bool final_step = false;
+            if ((dt >= 0.0 && h0 > dt) || (dt < 0.0 && h0 < dt)) { h0 = dt; final_step = true; }
+            if ((rc = stage(h0, {0}, {RK_AH[0]})) || (rc = rhs(t0 + RK_AH[0] * h0, ytmp.p, K(1)))) return rc;
+            if ((rc = stage(h0, {0, 1}, {RK_B3[0], RK_B3[1]})) || (rc = rhs(t0 + RK_AH[1] * h0, ytmp.p, K(2)))) return rc;
+            if ((rc = stage(h0, {0, 1, 2}, {RK_B4[0], RK_B4[1], RK_B4[2]})) || (rc = rhs(t0 + RK_AH[2] * h0, ytmp.p, K(3)))) return rc;
+            if ((rc = stage(h0, {0, 1, 2, 3}, {RK_B5[0], RK_B5[1], RK_B5[2], RK_B5[3]})) ||
+                (rc = rhs(t0 + RK_AH[3] * h0, ytmp.p, K(4))))
+                return rc;
+            if ((rc = stage(h0, {0, 1, 2, 3, 4}, {RK_B6[0], RK_B6[1], RK_B6[2], RK_B6[3], RK_B6[4]})) ||
+                (rc = rhs(t0 + RK_AH[4] * h0, ytmp.p, K(5))))
+                return rc;
+            k_zero_u64<<<1, 64, 0, stream>>>(rmax.p, 1);
+            k_rk_final<T><<<grid1(3 * H), 256, 0, stream>>>(y.p, K(0), K(2), K(3), K(4), K(5), h0, eps_abs, eps_rel, ynew.p,
+                                                            rmax.p, 3 * H);
+            HW_LAUNCH_CHECK();
+            HW_TRY(hipMemcpyAsync(h_rmax, rmax.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+            HW_TRY(hipStreamSynchronize(stream));
+            double rm;
+            std::memcpy(&rm, h_rmax, sizeof(double));
+            if (rm < 2.2250738585072014e-308) rm = 2.2250738585072014e-308;   // GSL starts rmax at DBL_MIN
+            *t = final_step ? t1 : t0 + h0;
+            double h_old = h0;
+            int adj = hadjust_rule(rm, &h0);
+            if (adj < 0) {
+                double t_curr = *t, t_next = *t + h0;
+                if (std::fabs(h0) < std::fabs(h_old) && t_next != t_curr) {
+                    st.steps_rejected++;
+                    continue;
+                }
+                *h = h0;
+                *t = t0;
+                set_error("step size underflow at t=%g", t0);
+                return HW_ERR_STEP;
+            }
+            std::swap(y.p, ynew.p);
+            st.steps_accepted++;
+            *h = h0;
+            return collect_timing();
+        }
+    }
+
+    int fields_dev(double* eta, double* u, double* v) override {
+        const dim3 gx(NY / XG::RR), bx(XG::NT);
+        const dim3 gy((NXH + YG::CC - 1) / YG::CC), by(YG::NT);
+        double* outs[3] = {eta, u, v};
+        k_convert<double2, C><<<grid1(3 * H), 256, 0, stream>>>(y.p, ytmp.p, 3 * H);
+        HW_LAUNCH_CHECK();
+        for (int f = 0; f < 3; ++f) {
+            if (!outs[f]) continue;
+            k_t_yinv<T, NX, NY><<<gy, by, 0, stream>>>(ytmp.p + (size_t)f * H, args.W[0], twy.p);
+            k_t_xinv<T, NX, NY><<<gx, bx, 0, stream>>>(args.W[0], eta_p.p, nullptr, twx.p);
+            HW_LAUNCH_CHECK();
+            k_convert_scalar<T, double><<<grid1(N), 256, 0, stream>>>(eta_p.p, outs[f], N);
+            HW_LAUNCH_CHECK();
+        }
+        HW_TRY(hipStreamSynchronize(stream));
+        return HW_OK;
+    }
+};
+
+// stand-alone transforms (hw_fft2_forward / hw_fft2_inverse)
+template <typename T, int NX, int NY>
+int fft2_forward(const double* phys, double* re, double* im) {
+    using C = typename ctype<T>::type;
+    constexpr int NXH = NX / 2 + 1;
+    constexpr size_t H = (size_t)NXH * NY, N = (size_t)NX * NY;
+    std::vector<T> hp(N);
+    for (size_t i = 0; i < N; ++i) hp[i] = (T)phys[i];
+    DevBuf<T> dp;
+    DevBuf<C> X, S, tx, ty;
+    int rc;
+    if ((rc = dp.upload(hp, 0)) || (rc = X.alloc(H)) || (rc = S.alloc(H)) ||
+        (rc = tx.upload(Solver2DT<T, NX, NY>::twiddles(NX), 0)) || (rc = ty.upload(Solver2DT<T, NX, NY>::twiddles(NY), 0)))
+        return rc;
+    k_t_xfwd<T, NX, NY><<<NY / XGeo<T, NX>::RR, XGeo<T, NX>::NT>>>(dp.p, X.p, tx.p);
+    k_t_yfwd<T, NX, NY><<<(NXH + YGeo<T, NY>::CC - 1) / YGeo<T, NY>::CC, YGeo<T, NY>::NT>>>(X.p, S.p, ty.p);
+    HW_LAUNCH_CHECK();
+    std::vector<C> hs(H);
+    HW_TRY(hipMemcpy(hs.data(), S.p, sizeof(C) * H, hipMemcpyDeviceToHost));
+    for (int ky = 0; ky < NY; ++ky)
+        for (int kx = 0; kx < NX; ++kx) {
+            double a, b;
+            if (kx < NXH) { a = hs[(size_t)kx * NY + ky].x; b = hs[(size_t)kx * NY + ky].y; }
+            else { C v = hs[(size_t)(NX - kx) * NY + (NY - ky) % NY]; a = v.x; b = -(double)v.y; }
+            re[(size_t)ky * NX + kx] = a;
+            im[(size_t)ky * NX + kx] = b;
+        }
+    return HW_OK;
+}
+
+template <typename T, int NX, int NY>
+int fft2_inverse(const double* re, const double* im, double* phys, double* nyq_im) {
+    using C = typename ctype<T>::type;
+    constexpr int NXH = NX / 2 + 1;
+    constexpr size_t H = (size_t)NXH * NY, N = (size_t)NX * NY;
+    std::vector<C> hs(H);
+    for (int kx = 0; kx < NXH; ++kx)
+        for (int ky = 0; ky < NY; ++ky) {
+            hs[(size_t)kx * NY + ky].x = (T)re[(size_t)ky * NX + kx];
+            hs[(size_t)kx * NY + ky].y = (T)im[(size_t)ky * NX + kx];
+        }
+    DevBuf<C> S, X, tx, ty;
+    DevBuf<T> dp, dn;
+    int rc;
+    if ((rc = S.upload(hs, 0)) || (rc = X.alloc(H)) || (rc = dp.alloc(N)) || (rc = dn.alloc(NY)) ||
+        (rc = tx.upload(Solver2DT<T, NX, NY>::twiddles(NX), 0)) || (rc = ty.upload(Solver2DT<T, NX, NY>::twiddles(NY), 0)))
+        return rc;
+    k_t_yinv<T, NX, NY><<<(NXH + YGeo<T, NY>::CC - 1) / YGeo<T, NY>::CC, YGeo<T, NY>::NT>>>(S.p, X.p, ty.p);
+    k_t_xinv<T, NX, NY><<<NY / XGeo<T, NX>::RR, XGeo<T, NX>::NT>>>(X.p, dp.p, dn.p, tx.p);
+    HW_LAUNCH_CHECK();
+    std::vector<T> hp(N), hn(NY);
+    HW_TRY(hipMemcpy(hp.data(), dp.p, sizeof(T) * N, hipMemcpyDeviceToHost));
+    HW_TRY(hipMemcpy(hn.data(), dn.p, sizeof(T) * NY, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < N; ++i) phys[i] = hp[i];
+    if (nyq_im)
+        for (int i = 0; i < NY; ++i) nyq_im[i] = hn[i];
+    return HW_OK;
+}
+
+// ------------------------------------------------------------- registry --
+struct Entry {
+    int prec, nx, ny;
+    Solver2D* (*make)();
+    int (*fwd)(const double*, double*, double*);
+    int (*inv)(const double*, const double*, double*, double*);
+};
+std::vector<Entry>& registry();
+struct Registrar {
+    Registrar(Entry e) { registry().push_back(e); }
+};
+
+}  // namespace hw
+
+#define HW_INSTANTIATE(T, PREC, NX, NY)                                                       \
+    namespace {                                                                             \
+    ::hw::Solver2D* make_##PREC##_##NX##_##NY() { return new ::hw::Solver2DT<T, NX, NY>(); } \
+    int fwd_##PREC##_##NX##_##NY(const double* p, double* r, double* i) {                   \
+        return ::hw::fft2_forward<T, NX, NY>(p, r, i);                                      \
+    }                                                                                       \
+    int inv_##PREC##_##NX##_##NY(const double* r, const double* i, double* p, double* n) {  \
+        return ::hw::fft2_inverse<T, NX, NY>(r, i, p, n);                                   \
+    }                                                                                       \
+    ::hw::Registrar reg_##PREC##_##NX##_##NY({PREC, NX, NY, make_##PREC##_##NX##_##NY,       \
+                                              fwd_##PREC##_##NX##_##NY, inv_##PREC##_##NX##_##NY}); \
+    }

@@ -1,0 +1,188 @@
+"""Seeded synthetic 2-D workloads.
+
+* ``harbour(n)``       -- variable-depth harbour: 1.5 km x 1.5 km, depth ramping
+  smoothly 20 m -> 4 m with seeded Gaussian bumps (amplitude 1 m, length
+  scale 50 m), two-reference-depth operators, a breakwater (reflection 0.5)
+  with a 100 m gap, directional JONSWAP line influx (Hs 2 m, Tp 8 s, s 15).
+  n = 2048 is the full-size case; smaller n keep the same physics on a
+  coarser grid for tests.
+* ``short_crested(n)`` -- flat 30 m, 2 km x 2 km, absorbing strips of 200 m,
+  directional JONSWAP line influx (Hs 3 m, Tp 10 s, s 10) at x = 250 m.
+
+Influx: the line signal eta_line(y_j, t) is a seeded random-phase sum of
+directional JONSWAP components, used as the source signal with the
+group-velocity influx kernel of the reference's area generation
+(2d/functions.c:3052-3068) as spatial shape, bidirectional source.  The
+spectral inversion of fun_area_generation (2d/functions.c:3175-3229) is not
+applied, so wave heights are indicative only; the operator being evaluated
+is the reference's.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import setup2d as S
+
+
+def splitmix64(seed: int):
+    """SplitMix64 stream -> numpy Generator seed material."""
+    x = seed & 0xFFFFFFFFFFFFFFFF
+    while True:
+        x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        yield z ^ (z >> 31)
+
+
+def rng(seed: int) -> np.random.Generator:
+    g = splitmix64(seed)
+    return np.random.default_rng([next(g) for _ in range(4)])
+
+
+def jonswap(w: np.ndarray, hs: float, tp: float, gamma: float) -> np.ndarray:
+    """JONSWAP variance density (normalised afterwards to the target Hs)."""
+    wp = 2 * math.pi / tp
+    sig = np.where(w <= wp, 0.07, 0.09)
+    s = w ** -5 * np.exp(-1.25 * (wp / w) ** 4) * gamma ** np.exp(-((w - wp) ** 2) / (2 * sig ** 2 * wp ** 2))
+    m0 = np.trapezoid(s, w)
+    return s * (hs / 4.0) ** 2 / m0
+
+
+def directional_line_signal(y: np.ndarray, t: np.ndarray, hs: float, tp: float, gamma: float, s_spread: float,
+                            depth: float, seed: int, ncomp: int = 256, theta0: float = 0.0,
+                            ramp_periods: float = 2.0) -> np.ndarray:
+    """eta(y_j, t) along a vertical line: random-phase directional sum, with a
+    (1 - cos) start ramp over ramp_periods * Tp.  Returns [ny, nt]."""
+    g = rng(seed)
+    wp = 2 * math.pi / tp
+    w = np.linspace(0.5 * wp, 3.0 * wp, ncomp)
+    dw = w[1] - w[0]
+    amp = np.sqrt(2 * jonswap(w, hs, tp, gamma) * dw)
+    # cos^(2s) spreading by rejection sampling
+    th = np.empty(ncomp)
+    n = 0
+    while n < ncomp:
+        c = g.uniform(-math.pi / 2, math.pi / 2, 4 * ncomp)
+        keep = c[g.uniform(0, 1, c.size) < np.cos(c) ** (2 * s_spread)]
+        m = min(keep.size, ncomp - n)
+        th[n:n + m] = keep[:m]
+        n += m
+    th += theta0
+    phase = g.uniform(0, 2 * math.pi, ncomp)
+    k = np.array([S.invers_omega(wi, depth) for wi in w])
+    ky = k * np.sin(th)
+    out = np.zeros((y.size, t.size))
+    for m in range(ncomp):
+        out += amp[m] * np.cos(w[m] * t[None, :] - ky[m] * y[:, None] + phase[m])
+    tr = ramp_periods * tp
+    ramp = np.where(t < tr, 0.5 * (1 - np.cos(math.pi * np.clip(t / tr, 0, 1))), 1.0)
+    return out * ramp[None, :]
+
+
+def _source_shapes(x, y, kx, line_x, meandepth, Lb, Lt):
+    ng, sg = S.area_source_shapes_vertical(x, y, kx, line_x, y, meandepth, Lb, Lt)
+    dx, dy = x[1] - x[0], y[1] - y[0]
+    fact = (2 * math.pi) ** 2 / (dx * dy)
+    # remove the reference's (2 pi)^2/(dx dy) factor: the signal here is the
+    # line elevation itself (see module docstring)
+    return ng / fact, sg / fact
+
+
+def harbour(n: int = 2048, seed: int = 4, t_end: float = 60.0, dt_sig: float = 0.1, length: float = 1500.0,
+            damp: float = 100.0, nadj: float = 1.0, eps_rel: float = 1e-2, eps_abs: float = 1e-3,
+            with_wall: bool = True) -> S.Model2D:
+    x = S.interval_to_vector(0.0, length, n)
+    y = S.interval_to_vector(0.0, length, n)
+    dx, dy = x[1] - x[0], y[1] - y[0]
+    kx = S.freqspace(x[-1] - x[0], n)
+    ky = S.freqspace(y[-1] - y[0], n)
+    g = rng(seed)
+    X, Y = np.meshgrid(x, y)
+    depth = 12.0 + 8.0 * np.cos(math.pi * X / length)
+    nb = 24
+    cx, cy = g.uniform(0, length, nb), g.uniform(0, length, nb)
+    sgn = g.choice([-1.0, 1.0], nb)
+    for i in range(nb):
+        depth += sgn[i] * np.exp(-((X - cx[i]) ** 2 + (Y - cy[i]) ** 2) / (2 * 50.0 ** 2))
+    # walls: a breakwater 12 m thick at x = 1000 m spanning y in [200, 1300]
+    # with a 100 m gap centred at y = 750 m (wall.dat convention: 1 on the wall)
+    wall_ind = np.zeros((n, n))
+    if with_wall:
+        on = (np.abs(X - 1000.0) <= 6.0) & (Y >= 200.0) & (Y <= 1300.0) & ~((Y > 700.0) & (Y < 800.0))
+        wall_ind[on] = 1.0
+    wallchar, cfsaw = S.wallchar_user(wall_ind, 0.5)
+    cfsa = S.cfsa2d(x, y, damp, damp, damp, damp)
+    if with_wall:
+        cfsa = np.minimum(cfsa, cfsaw)
+    fbdy = S.fbdy2d(x, y, damp, damp, damp, damp, cfsa, wall=with_wall)
+    aal = S.aal2d(4, kx, ky)
+    tp, hs = 8.0, 2.0
+    line_x = 200.0
+    ix = int(round((line_x - x[0]) / dx))
+    meandepth = float(depth[:, ix].mean())
+    wp = 2 * math.pi / tp
+    kp = S.invers_omega(wp, meandepth)
+    lam = 2 * math.pi / kp
+    cpeak = float(S.exact_cp1d(kp, meandepth, 0.0))
+    ops = S.disp_interpolation_2p(depth, kx, ky, wp)
+    chi = S.chi_adjust_line(x, y, line_x, nadj, lam, vertical=True)
+    t_sig = np.arange(0.0, t_end + 2 * dt_sig, dt_sig)
+    sig = directional_line_signal(y, t_sig, hs, tp, 3.3, 15.0, meandepth, seed + 1000)
+    ng, sg = _source_shapes(x, y, kx, line_x, meandepth, damp, damp)
+    return S.Model2D(nx=n, ny=n, x=x, y=y, kx=kx, ky=ky, flat=False, cpeak=cpeak, aal=aal, fbdy=fbdy,
+                     wallchar=wallchar, chi=chi, depth=depth, wall=with_wall, ops=ops, source_on=True,
+                     vertical=True, propagation=2, new_gam=ng, skew_gam=sg, t_sig=t_sig, sig=sig,
+                     skew_sig=np.zeros_like(sig), eps_abs=eps_abs, eps_rel=eps_rel,
+                     meta=dict(name=f"harbour{n}", tp=tp, hs=hs, kp=kp, lam=lam, line_x=line_x))
+
+
+def short_crested(n: int = 1024, seed: int = 3, t_end: float = 60.0, dt_sig: float = 0.1, length: float = 2000.0,
+                  damp: float = 200.0, depth0: float = 30.0, nadj: float = 1.0, eps_rel: float = 1e-2,
+                  eps_abs: float = 1e-3) -> S.Model2D:
+    x = S.interval_to_vector(0.0, length, n)
+    y = S.interval_to_vector(0.0, length, n)
+    kx = S.freqspace(x[-1] - x[0], n)
+    ky = S.freqspace(y[-1] - y[0], n)
+    KX, KY = kx[None, :], ky[:, None]
+    depth = np.full((n, n), depth0)
+    cfsa = S.cfsa2d(x, y, damp, damp, damp, damp)
+    fbdy = S.fbdy2d(x, y, damp, damp, damp, damp, cfsa, wall=False)
+    aal = S.aal2d(4, kx, ky)
+    tp, hs = 10.0, 3.0
+    wp = 2 * math.pi / tp
+    kp = S.invers_omega(wp, depth0)
+    lam = 2 * math.pi / kp
+    cpeak = float(S.exact_cp1d(kp, depth0, 0.0))
+    om2d = S.exact_disp2d(KX, KY, depth0)
+    l2d = om2d * om2d / S.GRAV
+    c2d = S.exact_cp2d(KX, KY, depth0)
+    line_x = 250.0
+    chi = S.chi_adjust_line(x, y, line_x, nadj, lam, vertical=True)
+    t_sig = np.arange(0.0, t_end + 2 * dt_sig, dt_sig)
+    sig = directional_line_signal(y, t_sig, hs, tp, 3.3, 10.0, depth0, seed + 1000)
+    ng, sg = _source_shapes(x, y, kx, line_x, depth0, damp, damp)
+    return S.Model2D(nx=n, ny=n, x=x, y=y, kx=kx, ky=ky, flat=True, cpeak=cpeak, aal=aal, fbdy=fbdy,
+                     wallchar=np.ones((n, n)), chi=chi, depth=depth, wall=False, c2d=c2d, l2d=l2d,
+                     source_on=True, vertical=True, propagation=2, new_gam=ng, skew_gam=sg, t_sig=t_sig, sig=sig,
+                     skew_sig=np.zeros_like(sig), eps_abs=eps_abs, eps_rel=eps_rel,
+                     meta=dict(name=f"short_crested{n}", tp=tp, hs=hs, kp=kp, lam=lam, line_x=line_x))
+
+
+def random_state(model: S.Model2D, seed: int, amp: float = 0.05, kfrac: float = 0.35) -> np.ndarray:
+    """A seeded smooth random state (full 6-plane layout): random fields whose
+    spectra are limited to |k| < kfrac * k_max, forward transformed."""
+    g = rng(seed)
+    ny, nx = model.ny, model.nx
+    KK = np.sqrt(model.kx[None, :] ** 2 + model.ky[:, None] ** 2)
+    mask = KK < kfrac * KK.max()
+    planes = []
+    for f in range(3):
+        spec = (g.standard_normal((ny, nx)) + 1j * g.standard_normal((ny, nx))) * mask
+        phys = np.real(np.fft.ifft2(spec))
+        phys *= (amp if f == 0 else 3 * amp) / (np.abs(phys).max() + 1e-300)
+        s = np.fft.fft2(phys)
+        planes += [s.real.ravel(), s.imag.ravel()]
+    return np.concatenate(planes)

@@ -1,0 +1,136 @@
+/*
+ * hawassi_hip.h -- C ABI of the MI355X (gfx950) implementation of HAWASSI's
+ * Hamiltonian pseudo-spectral right-hand side and its embedded RKF45 stepper.
+ *
+ * Plain C: pointers and sizes only.  All arrays are FP64 and use the
+ * reference's layouts:
+ *   - fields / coefficients: [ny][nx] row-major, index i*nx + j with i the
+ *     y index (reference: 2d/fun_odesolver.c:1152-1160);
+ *   - ODE state: 6 planes of nx*ny doubles, [Re eta^][Im eta^][Re u^][Im u^]
+ *     [Re v^][Im v^] of the full (redundant) complex spectrum
+ *     (reference: set_init_ode, 2d/fun_odesolver.c:1145-1161).
+ *
+ * Errors: every function returns 0 (GSL_SUCCESS) or a nonzero HW_ERR_* code;
+ * nothing calls exit().  hw_last_error() gives a message for the calling
+ * thread.  A context is not re-entrant (the reference's RHS is not either:
+ * it keeps its state in globals).
+ */
+#ifndef HAWASSI_HIP_H
+#define HAWASSI_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HW_OK 0
+#define HW_ERR_ARG 1        /* invalid argument / unsupported size */
+#define HW_ERR_HIP 2        /* HIP runtime error */
+#define HW_ERR_STEP 3       /* step size underflow in evolve (GSL_FAILURE analogue) */
+#define HW_ERR_NOGPU 4      /* no usable gfx950 device */
+
+#define HW_PREC_FP64 0      /* transforms, operators and state in FP64 */
+#define HW_PREC_FP32 1      /* transforms/operators FP32; RK state, stage sums, error norm FP64 */
+
+/* Model data the 2-D AB2 right-hand side reads.  In the reference these are
+ * process globals filled by fun_param_init / fun_operator_setup /
+ * fun_calcseacharacteristic / fun_area_generation (2d/main.c:119-186). */
+typedef struct hw_config_2d {
+    int nx, ny;               /* powers of two, 32..2048 (nx, ny >= 32) */
+    int precision;            /* HW_PREC_* */
+    int flat;                 /* 1: rhs_AB2_2DF_uv (flat bottom), 0: rhs_AB2_2DB_uv (two-depth) */
+    double grav;              /* 9.80665 in the reference */
+    double cpeak;             /* Oprt.Cpeak; damping uses 7*cpeak*fbdy */
+    const double *kx, *ky;    /* dominfo.kx [nx], dominfo.ky [ny] (freqspace) */
+    const double *aal;        /* dominfo.aal [ny*nx] anti-alias mask */
+    const double *fbdy;       /* dominfo.fbdy.charac [ny*nx] */
+    const double *wallchar;   /* dominfo.wallchar [ny*nx] */
+    const double *chi;        /* ChiAdj [ny*nx] */
+    /* flat bottom operators (flat == 1) */
+    const double *c2d;        /* Oprt.C2d [ny*nx] */
+    const double *l2d;        /* Oprt.L2d [ny*nx] */
+    /* two-reference-depth operators (flat == 0), funOprt_DispInterpolation_2p */
+    const double *gam_min, *gam_plus;     /* Oprt.InterpD.Gam_min/plus [ny*nx] (physical) */
+    const double *csq_min, *csq_plus;     /* Oprt.InterpD.Csqr2d_min/plus [ny*nx] (spectral) */
+    const double *om2_min, *om2_plus;     /* Oprt.InterpD.Om2dSq_min/plus [ny*nx] (spectral) */
+    /* influx source (fun_source): Source = S_j(t) newGam +- Skew_j(t) SkewGam */
+    int source_on;
+    int vertical;             /* orientation "vertical" (line index = row) else horizontal */
+    int propagation;          /* 0 "Uni+", 1 "Uni-", 2 bidirectional */
+    const double *new_gam;    /* [ny*nx], value at (y_i, x_j) */
+    const double *skew_gam;   /* [ny*nx] */
+    int nline;                /* ny if vertical else nx */
+    int nt;                   /* samples per line signal */
+    const double *t_sig;      /* [nt] sample times (dominfo.t) */
+    const double *sig;        /* [nline*nt] Signal[j][n] */
+    const double *skew_sig;   /* [nline*nt] SkewSignal[j][n] */
+    /* step control (control_y_new(eps_abs, eps_rel); 2d/main.c:196-197) */
+    double eps_abs, eps_rel;
+} hw_config_2d;
+
+typedef struct hw_stats {
+    long rhs_evals;           /* right-hand side evaluations on the device */
+    long steps_accepted;
+    long steps_rejected;
+    double rhs_ms_total;      /* device time in RHS evaluations (HIP events), ms */
+} hw_stats;
+
+/* Build a context (device buffers, multipliers in device layout) from the
+ * model data.  Replaces the globals the reference's RHS reads. */
+int hw_ctx_create_2d(const hw_config_2d *cfg, void **ctx);
+void hw_ctx_destroy(void *ctx);
+
+/* GSL odeiv2 system callback surface: int f(double t, const double y[],
+ * double dydt[], void *params), registered as {rhs, NULL, 6*nx*ny, params}
+ * (2d/main.c:204-222, rhs_AB2_2DB_uv 2d/fun_odesolver.c:4542,
+ * rhs_AB2_2DF_uv 2d/fun_odesolver.c:4458).  y/dydt are host arrays in the
+ * full 6-plane layout; params is the context. */
+int hw_rhs(double t, const double *y, double *dydt, void *ctx);
+
+/* Same operator on device arrays (full 6-plane layout, FP64). */
+int hw_rhs_device(void *ctx, double t, const double *d_y, double *d_dydt);
+
+/* Device-resident state for the integrator (full 6-plane host layout). */
+int hw_set_state(void *ctx, const double *y);
+int hw_get_state(void *ctx, double *y);
+
+/* gsl_odeiv2_evolve_apply(e, c, s, &sys, &t, t1, &h, y) with the rkf45 step
+ * and control_y(eps_abs, eps_rel) (2d/fun_odesolver.c:2567-2570,2642):
+ * one accepted step (final step clipped to t1), t and h updated. */
+int hw_evolve_apply(void *ctx, double *t, double t1, double *h);
+
+/* The driver loop of ode_solver_uv: while (t < t1) evolve_apply
+ * (2d/fun_odesolver.c:2641-2647). */
+int hw_integrate(void *ctx, double *t, double t1, double *h);
+
+/* Physical eta, u, v [ny*nx] of the current state, as the output IFFTs of
+ * ode_solver_uv (2d/fun_odesolver.c:2663-2666); any pointer may be NULL. */
+int hw_get_fields(void *ctx, double *eta, double *u, double *v);
+
+int hw_get_stats(void *ctx, hw_stats *st);
+int hw_reset_stats(void *ctx);
+
+/* Block until all device work of the context has finished. */
+int hw_sync(void *ctx);
+
+/* Stand-alone transforms with the reference's conventions, for tests and
+ * tools: forward = unnormalised exp(-i) DFT of a real [ny][nx] field,
+ * returning the full-grid complex spectrum (re, im planes [ny][nx]);
+ * inverse = Hermitian projection (fftwcomplex_2d_sym, 2d/functions1.c:492-521)
+ * then backward DFT / (nx*ny), real part (ifftw_2d_c2r, 2d/functions1.c:523-559).
+ * nyq_im (optional, [ny]) receives Im of the inverse's Nyquist column after
+ * the y transform, scaled by 1/(nx*ny). */
+int hw_fft2_forward(int nx, int ny, int precision, const double *phys, double *spec_re, double *spec_im);
+int hw_fft2_inverse(int nx, int ny, int precision, const double *spec_re, const double *spec_im, double *phys, double *nyq_im);
+
+/* Sizes this build supports (1 if (nx, ny, precision) has compiled kernels). */
+int hw_supported(int nx, int ny, int precision);
+
+const char *hw_last_error(void);
+const char *hw_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

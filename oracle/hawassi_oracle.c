@@ -1,0 +1,748 @@
+/*
+ * hawassi_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C, FP64, OpenMP restatement of the HAWASSI-2D AB2 ("HS2") right-hand
+ * side and its GSL-style embedded RKF45 driver, written from the reference's
+ * algorithm (hawassi/HAWASSI_C, read as text; no code copied).  It is the
+ * checker for the HIP path and the CPU baseline that bench.py times; the
+ * product library never links or calls it.
+ *
+ * What it follows (reference file:line):
+ *   fftwcomplex_2d_sym           2d/functions1.c:492-521   -> orc_hermitian_project
+ *   HSSgen_2depth                2d/fun_odesolver.c:500-585 -> hss_2depth
+ *   fun_adjust                   2d/fun_odesolver.c:810-833 -> adjust
+ *   fun_source                   2d/fun_odesolver.c:886-983 -> source_hat
+ *   fun_delphiH1hat              2d/fun_odesolver.c:2100-2239 -> delphiH1_flat
+ *   fun_termshat_AB1_2DB_uv      2d/fun_odesolver.c:2874-2960
+ *   fun_termshat_AB2_2DF_uv      2d/fun_odesolver.c:3136-3325
+ *   fun_termshat_AB2_2DB_uv      2d/fun_odesolver.c:3327-3509
+ *   rhs_AB2_2DF_uv / _2DB_uv     2d/fun_odesolver.c:4458-4624 -> orc_rhs
+ *   gsl rkf45 + control_y + evolve_apply (GSL 2.x odeiv2, restated from its
+ *   documented algorithm: Fehlberg 4(5) tableau, 5th-order propagation,
+ *   D_i = eps_abs + eps_rel|y_i|, shrink if rmax>1.1 by max(0.2, 0.9 rmax^-1/5),
+ *   grow if rmax<0.5 by min(5, max(1, 0.9 rmax^-1/6)), final-step clipping)
+ *   used by ode_solver_uv    2d/fun_odesolver.c:2565-2647 -> orc_evolve_apply
+ *   natural cubic spline (gsl_interp_cspline) eval used at 2d/fun_odesolver.c:901-902
+ *
+ * The FFT is an iterative radix-2 complex transform (FFTW sign/scale
+ * convention: forward exp(-i..), backward exp(+i..), both unnormalised).
+ * Sizes must be powers of two.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdint.h>
+#include <omp.h>
+
+#define ORC_GRAV 9.80665
+
+/* ------------------------------------------------------------------ FFT -- */
+typedef struct { int n; int logn; double *tw; int *rev; } fftplan;
+
+static fftplan *plan_new(int n)
+{
+    fftplan *p = (fftplan *)calloc(1, sizeof(fftplan));
+    p->n = n;
+    int l = 0; while ((1 << l) < n) l++;
+    p->logn = l;
+    p->tw = (double *)malloc(sizeof(double) * 2 * (n / 2 > 0 ? n / 2 : 1));
+    for (int k = 0; k < n / 2; k++) {
+        double a = -2.0 * M_PI * (double)k / (double)n;
+        p->tw[2 * k] = cos(a);
+        p->tw[2 * k + 1] = sin(a);
+    }
+    p->rev = (int *)malloc(sizeof(int) * n);
+    for (int i = 0; i < n; i++) {
+        int r = 0;
+        for (int b = 0; b < l; b++) if (i & (1 << b)) r |= 1 << (l - 1 - b);
+        p->rev[i] = r;
+    }
+    return p;
+}
+static void plan_free(fftplan *p) { if (!p) return; free(p->tw); free(p->rev); free(p); }
+
+/* in-place complex FFT of interleaved data; sign=-1 forward, +1 backward */
+static void fft1(const fftplan *p, double *x, int sign)
+{
+    int n = p->n;
+    for (int i = 0; i < n; i++) {
+        int r = p->rev[i];
+        if (r > i) {
+            double t0 = x[2 * i], t1 = x[2 * i + 1];
+            x[2 * i] = x[2 * r]; x[2 * i + 1] = x[2 * r + 1];
+            x[2 * r] = t0; x[2 * r + 1] = t1;
+        }
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+        int half = len >> 1, step = n / len;
+        for (int s = 0; s < n; s += len) {
+            for (int k = 0; k < half; k++) {
+                double wr = p->tw[2 * k * step];
+                double wi = p->tw[2 * k * step + 1] * (sign < 0 ? 1.0 : -1.0);
+                double *a = x + 2 * (s + k), *b = x + 2 * (s + k + half);
+                double br = b[0] * wr - b[1] * wi, bi = b[0] * wi + b[1] * wr;
+                b[0] = a[0] - br; b[1] = a[1] - bi;
+                a[0] += br; a[1] += bi;
+            }
+        }
+    }
+}
+
+typedef struct { int Nx, Ny; fftplan *px, *py; } fft2plan;
+
+/* full complex 2-D transform of an [Ny][Nx] interleaved array (x fastest) */
+static void fft2(const fft2plan *P, double *a, int sign)
+{
+    int Nx = P->Nx, Ny = P->Ny;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < Ny; i++) fft1(P->px, a + 2 * (size_t)i * Nx, sign);
+#pragma omp parallel
+    {
+        double *col = (double *)malloc(sizeof(double) * 2 * Ny);
+#pragma omp for schedule(static)
+        for (int j = 0; j < Nx; j++) {
+            for (int i = 0; i < Ny; i++) { col[2 * i] = a[2 * ((size_t)i * Nx + j)]; col[2 * i + 1] = a[2 * ((size_t)i * Nx + j) + 1]; }
+            fft1(P->py, col, sign);
+            for (int i = 0; i < Ny; i++) { a[2 * ((size_t)i * Nx + j)] = col[2 * i]; a[2 * ((size_t)i * Nx + j) + 1] = col[2 * i + 1]; }
+        }
+        free(col);
+    }
+}
+
+/* ---------------------------------------------------- Hermitian project -- */
+/* Restates fftwcomplex_2d_sym (2d/functions1.c:492-521) on an interleaved
+ * [Ny][Nx] array: zero Im at the four self-conjugate corners, mirror row 0
+ * and row Ny/2 from their left halves, mirror column 0 from its top half and
+ * overwrite every column j>Nx/2 (rows>=1) with conj of (Ny-i, Nx-j).  Column
+ * Nx/2 is left as it is. */
+void orc_hermitian_project(double *a, int Nx, int Ny)
+{
+#define RE(i, j) a[2 * ((size_t)(i) * Nx + (j))]
+#define IM(i, j) a[2 * ((size_t)(i) * Nx + (j)) + 1]
+    IM(0, 0) = 0; IM(0, Nx / 2) = 0; IM(Ny / 2, 0) = 0; IM(Ny / 2, Nx / 2) = 0;
+    for (int j = 1; j < Nx / 2; j++) {
+        RE(0, Nx - j) = RE(0, j); IM(0, Nx - j) = -IM(0, j);
+        RE(Ny / 2, Nx - j) = RE(Ny / 2, j); IM(Ny / 2, Nx - j) = -IM(Ny / 2, j);
+    }
+    for (int i = 1; i < Ny / 2; i++) { RE(Ny - i, 0) = RE(i, 0); IM(Ny - i, 0) = -IM(i, 0); }
+    for (int i = 1; i < Ny; i++)
+        for (int j = Nx / 2 + 1; j < Nx; j++) {
+            RE(i, j) = RE(Ny - i, Nx - j);
+            IM(i, j) = -IM(Ny - i, Nx - j);
+        }
+#undef RE
+#undef IM
+}
+
+/* ------------------------------------------------------------- context -- */
+typedef struct {
+    int Nx, Ny;
+    int flat;          /* 1: rhs_AB2_2DF_uv, 0: rhs_AB2_2DB_uv */
+    int wall;          /* wall != "no" (flat linear term variants) */
+    int refl_one;      /* refl_coef == 1 (flat wall variant) */
+    double Cpeak;
+    const double *kx, *ky;
+    const double *aal, *fbdy, *wallchar, *chi;
+    const double *C2d, *L2d, *gwx, *gwy;
+    const double *gam_m, *gam_p, *csq_m, *csq_p, *om2_m, *om2_p;
+    int source_on, vertical, propagation; /* propagation: 0 Uni+, 1 Uni-, 2 bi */
+    const double *newGam, *SkewGam;       /* [Ny][Nx] */
+    int nline, nt;
+    const double *tsig, *sig, *skew;      /* sig/skew: [nline][nt] */
+} orc_params;
+
+typedef struct {
+    orc_params p;
+    fft2plan P;
+    double *sig_c, *skew_c;   /* natural-spline half second derivatives */
+    long nrhs;
+    /* RKF45 work */
+    double *k[6], *ytmp, *y0, *yerr, *dydt_in, *dydt_out;
+    size_t dim;
+} orc_ctx;
+
+/* natural cubic spline: c_i = M_i/2 (GSL cspline convention) */
+static void spline_c(const double *x, const double *y, int n, double *c)
+{
+    for (int i = 0; i < n; i++) c[i] = 0.0;
+    if (n < 3) return;
+    int m = n - 2;
+    double *diag = (double *)malloc(sizeof(double) * m), *off = (double *)malloc(sizeof(double) * m),
+           *rhs = (double *)malloc(sizeof(double) * m);
+    for (int i = 0; i < m; i++) {
+        double h0 = x[i + 1] - x[i], h1 = x[i + 2] - x[i + 1];
+        diag[i] = 2.0 * (h0 + h1);
+        off[i] = h1;
+        rhs[i] = 3.0 * ((y[i + 2] - y[i + 1]) / h1 - (y[i + 1] - y[i]) / h0);
+    }
+    /* Thomas algorithm, symmetric tridiagonal */
+    for (int i = 1; i < m; i++) {
+        double w = off[i - 1] / diag[i - 1];
+        diag[i] -= w * off[i - 1];
+        rhs[i] -= w * rhs[i - 1];
+    }
+    c[m] = rhs[m - 1] / diag[m - 1];
+    for (int i = m - 2; i >= 0; i--) c[i + 1] = (rhs[i] - off[i] * c[i + 2]) / diag[i];
+    free(diag); free(off); free(rhs);
+}
+
+/* interval index as gsl_interp_bsearch */
+int orc_spline_index(const double *x, int n, double t)
+{
+    int ilo = 0, ihi = n - 1;
+    while (ihi > ilo + 1) {
+        int i = (ihi + ilo) / 2;
+        if (x[i] > t) ihi = i; else ilo = i;
+    }
+    return ilo;
+}
+
+static double spline_eval(const double *x, const double *y, const double *c, int n, double t, int idx)
+{
+    (void)n;
+    double h = x[idx + 1] - x[idx], dx = t - x[idx], dy = y[idx + 1] - y[idx];
+    double ci = c[idx], cip = c[idx + 1];
+    double b = dy / h - h * (cip + 2.0 * ci) / 3.0;
+    double d = (cip - ci) / (3.0 * h);
+    return y[idx] + dx * (b + dx * (ci + dx * d));
+}
+
+void *orc_create(const orc_params *prm)
+{
+    orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
+    c->p = *prm;
+    c->P.Nx = prm->Nx; c->P.Ny = prm->Ny;
+    c->P.px = plan_new(prm->Nx); c->P.py = plan_new(prm->Ny);
+    if (prm->source_on && prm->nline > 0) {
+        c->sig_c = (double *)malloc(sizeof(double) * (size_t)prm->nline * prm->nt);
+        c->skew_c = (double *)malloc(sizeof(double) * (size_t)prm->nline * prm->nt);
+        for (int j = 0; j < prm->nline; j++) {
+            spline_c(prm->tsig, prm->sig + (size_t)j * prm->nt, prm->nt, c->sig_c + (size_t)j * prm->nt);
+            spline_c(prm->tsig, prm->skew + (size_t)j * prm->nt, prm->nt, c->skew_c + (size_t)j * prm->nt);
+        }
+    }
+    c->dim = (size_t)6 * prm->Nx * prm->Ny;
+    return c;
+}
+
+void orc_destroy(void *vc)
+{
+    orc_ctx *c = (orc_ctx *)vc;
+    if (!c) return;
+    plan_free(c->P.px); plan_free(c->P.py);
+    free(c->sig_c); free(c->skew_c);
+    for (int i = 0; i < 6; i++) free(c->k[i]);
+    free(c->ytmp); free(c->y0); free(c->yerr); free(c->dydt_in); free(c->dydt_out);
+    free(c);
+}
+
+long orc_rhs_count(void *vc) { return ((orc_ctx *)vc)->nrhs; }
+
+/* spline coefficient export, so tests can compare the device path's tables */
+void orc_spline_coeffs(const double *x, const double *y, int n, double *c) { spline_c(x, y, n, c); }
+
+static double *cal(size_t n) { return (double *)calloc(2 * n, sizeof(double)); }
+
+/* HSSgen_2depth (2d/fun_odesolver.c:500-585).  phi: complex array whose real
+ * part is the physical field times factIFFT; phihat: its spectrum. */
+static void hss_2depth(orc_ctx *c, const double *phi, const double *phihat, const double *Om, const double *Op,
+                       double *result, double factIFFT)
+{
+    int Nx = c->p.Nx, Ny = c->p.Ny;
+    size_t N = (size_t)Nx * Ny;
+    double *Lm = cal(N), *Lp = cal(N);
+    #pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < N; q++) {
+        Lm[2 * q] = Om[q] * phihat[2 * q]; Lm[2 * q + 1] = Om[q] * phihat[2 * q + 1];
+        Lp[2 * q] = Op[q] * phihat[2 * q]; Lp[2 * q + 1] = Op[q] * phihat[2 * q + 1];
+    }
+    orc_hermitian_project(Lm, Nx, Ny);
+    orc_hermitian_project(Lp, Nx, Ny);
+    fft2(&c->P, Lm, +1);
+    fft2(&c->P, Lp, +1);
+    double *gm = cal(N), *gp = cal(N), *gmL = cal(N), *gpL = cal(N);
+    const double *Gm = c->p.gam_m, *Gp = c->p.gam_p;
+    #pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < N; q++) {
+        gm[2 * q] = Gm[q] * phi[2 * q] / factIFFT;
+        gp[2 * q] = Gp[q] * phi[2 * q] / factIFFT;
+        gmL[2 * q] = Gm[q] * Lm[2 * q] / (double)N;
+        gpL[2 * q] = Gp[q] * Lp[2 * q] / (double)N;
+    }
+    fft2(&c->P, gm, -1); fft2(&c->P, gp, -1); fft2(&c->P, gmL, -1); fft2(&c->P, gpL, -1);
+    #pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < N; q++) {
+        result[2 * q] = (gmL[2 * q] + Om[q] * gm[2 * q] + gpL[2 * q] + Op[q] * gp[2 * q]) / 2;
+        result[2 * q + 1] = (gmL[2 * q + 1] + Om[q] * gm[2 * q + 1] + gpL[2 * q + 1] + Op[q] * gp[2 * q + 1]) / 2;
+    }
+    free(Lm); free(Lp); free(gm); free(gp); free(gmL); free(gpL);
+}
+
+/* fun_adjust (2d/fun_odesolver.c:810-833): out = F(chi * F^-1(P Ahat)/N),
+ * keeping the imaginary part of the inverse.  Ahat is projected in place. */
+static void adjust(orc_ctx *c, double *out, double *Ahat)
+{
+    int Nx = c->p.Nx, Ny = c->p.Ny;
+    size_t N = (size_t)Nx * Ny;
+    orc_hermitian_project(Ahat, Nx, Ny);
+    memcpy(out, Ahat, sizeof(double) * 2 * N);
+    fft2(&c->P, out, +1);
+    #pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < N; q++) {
+        out[2 * q] = out[2 * q] * c->p.chi[q] / (double)N;
+        out[2 * q + 1] = out[2 * q + 1] * c->p.chi[q] / (double)N;
+    }
+    fft2(&c->P, out, -1);
+}
+
+/* fun_source (2d/fun_odesolver.c:886-983): Source_hat at time t */
+static void source_hat(orc_ctx *c, double t, double *Shat)
+{
+    int Nx = c->p.Nx, Ny = c->p.Ny;
+    size_t N = (size_t)Nx * Ny;
+    memset(Shat, 0, sizeof(double) * 2 * N);
+    if (!c->p.source_on) return;
+    int nl = c->p.nline, nt = c->p.nt;
+    double *ins = (double *)malloc(sizeof(double) * nl), *insk = (double *)malloc(sizeof(double) * nl);
+    int idx = orc_spline_index(c->p.tsig, nt, t);
+    for (int j = 0; j < nl; j++) {
+        ins[j] = spline_eval(c->p.tsig, c->p.sig + (size_t)j * nt, c->sig_c + (size_t)j * nt, nt, t, idx);
+        insk[j] = spline_eval(c->p.tsig, c->p.skew + (size_t)j * nt, c->skew_c + (size_t)j * nt, nt, t, idx);
+    }
+    double *S = cal(N), *Sk = cal(N);
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < Ny; i++)
+        for (int j = 0; j < Nx; j++) {
+            size_t q = (size_t)i * Nx + j;
+            double s = c->p.vertical ? ins[i] : ins[j];
+            double sk = c->p.vertical ? insk[i] : insk[j];
+            if (c->p.propagation == 0) { S[2 * q] = s * c->p.newGam[q]; Sk[2 * q] = -1 * sk * c->p.SkewGam[q]; }
+            else if (c->p.propagation == 1) { S[2 * q] = s * c->p.newGam[q]; Sk[2 * q] = 1 * sk * c->p.SkewGam[q]; }
+            else { S[2 * q] = 2 * s * c->p.newGam[q]; Sk[2 * q] = 0; }
+        }
+    fft2(&c->P, S, -1);
+    fft2(&c->P, Sk, -1);
+    #pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < 2 * N; q++) Shat[q] = S[q] + Sk[q];
+    free(S); free(Sk); free(ins); free(insk);
+}
+
+/* ------------------------------------------------------------------ RHS -- */
+int orc_rhs(void *vc, double t, const double *Z, double *dZ)
+{
+    orc_ctx *c = (orc_ctx *)vc;
+    const orc_params *p = &c->p;
+    int Nx = p->Nx, Ny = p->Ny;
+    size_t N = (size_t)Nx * Ny;
+    const double g = ORC_GRAV;
+    const double *kx = p->kx, *ky = p->ky;
+    c->nrhs++;
+
+    double *eta = cal(N), *u = cal(N), *v = cal(N);
+    double *dampeta = cal(N), *dampu = cal(N), *dampv = cal(N);
+    double *ikxE_adj = cal(N), *ikyE_adj = cal(N), *duH2_adj = cal(N), *dvH2_adj = cal(N);
+    double *delphiH1 = cal(N);     /* flat: delphiH1hat; 2DB: deluH1+delvH1 */
+    double *duH1 = cal(N), *dvH1 = cal(N);
+    double *etahat = cal(N), *uhat = cal(N), *vhat = cal(N);
+
+    #pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < N; q++) {
+        etahat[2 * q] = Z[q]; etahat[2 * q + 1] = Z[N + q];
+        uhat[2 * q] = Z[2 * N + q]; uhat[2 * q + 1] = Z[3 * N + q];
+        vhat[2 * q] = Z[4 * N + q]; vhat[2 * q + 1] = Z[5 * N + q];
+    }
+    double *Lphihat = cal(N), *M0uhat = cal(N), *M0vhat = cal(N);
+
+    if (p->flat) {
+        /* ---- fun_delphiH1hat (2d/fun_odesolver.c:2100-2239) ---- */
+        if (!p->wall) {
+            #pragma omp parallel for schedule(static)
+            for (int i = 0; i < Ny; i++)
+                for (int j = 0; j < Nx; j++) {
+                    size_t q = (size_t)i * Nx + j;
+                    double c2 = p->C2d[q] * p->C2d[q] / g;
+                    double dur = c2 * Z[2 * N + q], dui = c2 * Z[3 * N + q];
+                    double dvr = c2 * Z[4 * N + q], dvi = c2 * Z[5 * N + q];
+                    delphiH1[2 * q] = kx[j] * dui + ky[i] * dvi;
+                    delphiH1[2 * q + 1] = -kx[j] * dur - ky[i] * dvr;
+                }
+        } else if (p->refl_one) {
+            double *Cx = cal(N), *Cy = cal(N), *Fx = cal(N), *Fy = cal(N), *dv = cal(N);
+            #pragma omp parallel for schedule(static)
+            for (int i = 0; i < Ny; i++)
+                for (int j = 0; j < Nx; j++) {
+                    size_t q = (size_t)i * Nx + j;
+                    double c2 = p->C2d[q] * p->C2d[q] / g;
+                    Cx[2 * q] = c2 * Z[2 * N + q]; Cx[2 * q + 1] = c2 * Z[3 * N + q];
+                    Cy[2 * q] = c2 * Z[4 * N + q]; Cy[2 * q + 1] = c2 * Z[5 * N + q];
+                }
+            /* ifftw_2d_c2r projects its (complex**) input in place */
+            orc_hermitian_project(Cx, Nx, Ny);
+            orc_hermitian_project(Cy, Nx, Ny);
+            memcpy(Fx, Cx, sizeof(double) * 2 * N); memcpy(Fy, Cy, sizeof(double) * 2 * N);
+            fft2(&c->P, Fx, +1); fft2(&c->P, Fy, +1);
+            #pragma omp parallel for schedule(static)
+            for (int i = 0; i < Ny; i++)
+                for (int j = 0; j < Nx; j++) {
+                    size_t q = (size_t)i * Nx + j;
+                    dv[2 * q] = -(-kx[j] * Cx[2 * q + 1] - ky[i] * Cy[2 * q + 1]);
+                    dv[2 * q + 1] = -(kx[j] * Cx[2 * q] + ky[i] * Cy[2 * q]);
+                }
+            orc_hermitian_project(dv, Nx, Ny);
+            fft2(&c->P, dv, +1);
+            double *a = cal(N), *b = cal(N);
+            #pragma omp parallel for schedule(static)
+            for (size_t q = 0; q < N; q++) {
+                a[2 * q] = dv[2 * q] / (double)N * p->wallchar[q];
+                b[2 * q] = Fx[2 * q] / (double)N * p->gwx[q] + Fy[2 * q] / (double)N * p->gwy[q];
+            }
+            fft2(&c->P, a, -1); fft2(&c->P, b, -1);
+            #pragma omp parallel for schedule(static)
+            for (size_t q = 0; q < 2 * N; q++) delphiH1[q] = a[q] - b[q];
+            free(Cx); free(Cy); free(Fx); free(Fy); free(dv); free(a); free(b);
+        } else {
+            double *Cx = cal(N), *Cy = cal(N);
+            #pragma omp parallel for schedule(static)
+            for (size_t q = 0; q < N; q++) {
+                Cx[2 * q] = p->C2d[q] * Z[2 * N + q]; Cx[2 * q + 1] = p->C2d[q] * Z[3 * N + q];
+                Cy[2 * q] = p->C2d[q] * Z[4 * N + q]; Cy[2 * q + 1] = p->C2d[q] * Z[5 * N + q];
+            }
+            orc_hermitian_project(Cx, Nx, Ny); orc_hermitian_project(Cy, Nx, Ny);
+            fft2(&c->P, Cx, +1); fft2(&c->P, Cy, +1);
+            double *a = cal(N), *b = cal(N);
+            #pragma omp parallel for schedule(static)
+            for (size_t q = 0; q < N; q++) {
+                a[2 * q] = Cx[2 * q] / (double)N * p->wallchar[q];
+                b[2 * q] = Cy[2 * q] / (double)N * p->wallchar[q];
+            }
+            fft2(&c->P, a, -1); fft2(&c->P, b, -1);
+            #pragma omp parallel for schedule(static)
+            for (int i = 0; i < Ny; i++)
+                for (int j = 0; j < Nx; j++) {
+                    size_t q = (size_t)i * Nx + j;
+                    double dur = p->C2d[q] / g * a[2 * q], dui = p->C2d[q] / g * a[2 * q + 1];
+                    double dvr = p->C2d[q] / g * b[2 * q], dvi = p->C2d[q] / g * b[2 * q + 1];
+                    delphiH1[2 * q] = kx[j] * dui + ky[i] * dvi;
+                    delphiH1[2 * q + 1] = -kx[j] * dur - ky[i] * dvr;
+                }
+            free(Cx); free(Cy); free(a); free(b);
+        }
+        /* ---- fun_termshat_AB2_2DF_uv head: M0 by C2d^2/g (-i k) ---- */
+        #pragma omp parallel for schedule(static)
+        for (int i = 0; i < Ny; i++)
+            for (int j = 0; j < Nx; j++) {
+                size_t q = (size_t)i * Nx + j;
+                double c2 = p->C2d[q] * p->C2d[q] / g;
+                M0uhat[2 * q] = c2 * kx[j] * Z[3 * N + q];
+                M0uhat[2 * q + 1] = -c2 * kx[j] * Z[2 * N + q];
+                M0vhat[2 * q] = c2 * ky[i] * Z[5 * N + q];
+                M0vhat[2 * q + 1] = -c2 * ky[i] * Z[4 * N + q];
+                Lphihat[2 * q] = M0uhat[2 * q] + M0vhat[2 * q];
+                Lphihat[2 * q + 1] = M0uhat[2 * q + 1] + M0vhat[2 * q + 1];
+            }
+    } else {
+        /* ---- fun_termshat_AB1_2DB_uv (2d/fun_odesolver.c:2874-2960) ---- */
+        memcpy(eta, etahat, sizeof(double) * 2 * N);
+        orc_hermitian_project(uhat, Nx, Ny);
+        orc_hermitian_project(vhat, Nx, Ny);
+        orc_hermitian_project(eta, Nx, Ny);
+        memcpy(u, uhat, sizeof(double) * 2 * N);
+        memcpy(v, vhat, sizeof(double) * 2 * N);
+        fft2(&c->P, eta, +1); fft2(&c->P, u, +1); fft2(&c->P, v, +1);
+        double *Cu = cal(N), *Cv = cal(N);
+        hss_2depth(c, u, uhat, p->csq_m, p->csq_p, Cu, (double)N);
+        hss_2depth(c, v, vhat, p->csq_m, p->csq_p, Cv, (double)N);
+        #pragma omp parallel for schedule(static)
+        for (size_t q = 0; q < N; q++) {
+            dampeta[2 * q] = 7 * p->Cpeak * p->fbdy[q] * eta[2 * q] / (double)N;
+            dampu[2 * q] = 7 * p->Cpeak * p->fbdy[q] * u[2 * q] / (double)N;
+            dampv[2 * q] = 7 * p->Cpeak * p->fbdy[q] * v[2 * q] / (double)N;
+        }
+        #pragma omp parallel for schedule(static)
+        for (int i = 0; i < Ny; i++)
+            for (int j = 0; j < Nx; j++) {
+                size_t q = (size_t)i * Nx + j;
+                duH1[2 * q] = Cu[2 * q + 1] / g * kx[j];
+                duH1[2 * q + 1] = -Cu[2 * q] / g * kx[j];
+                dvH1[2 * q] = Cv[2 * q + 1] / g * ky[i];
+                dvH1[2 * q + 1] = -Cv[2 * q] / g * ky[i];
+            }
+        fft2(&c->P, dampeta, -1); fft2(&c->P, dampu, -1); fft2(&c->P, dampv, -1);
+        free(Cu); free(Cv);
+        /* ---- fun_termshat_AB2_2DB_uv head ---- */
+        #pragma omp parallel for schedule(static)
+        for (size_t q = 0; q < N; q++) {
+            etahat[2 * q] = Z[q]; etahat[2 * q + 1] = Z[N + q];
+            uhat[2 * q] = Z[2 * N + q]; uhat[2 * q + 1] = Z[3 * N + q];
+            vhat[2 * q] = Z[4 * N + q]; vhat[2 * q + 1] = Z[5 * N + q];
+            M0uhat[2 * q] = duH1[2 * q]; M0uhat[2 * q + 1] = duH1[2 * q + 1];
+            M0vhat[2 * q] = dvH1[2 * q]; M0vhat[2 * q + 1] = dvH1[2 * q + 1];
+            Lphihat[2 * q] = duH1[2 * q] + dvH1[2 * q];
+            Lphihat[2 * q + 1] = duH1[2 * q + 1] + dvH1[2 * q + 1];
+        }
+    }
+
+    /* ---- shared AB2 body (2DF:3182-3303, 2DB:3372-3487) ---- */
+    orc_hermitian_project(etahat, Nx, Ny); orc_hermitian_project(uhat, Nx, Ny);
+    orc_hermitian_project(vhat, Nx, Ny); orc_hermitian_project(Lphihat, Nx, Ny);
+    orc_hermitian_project(M0uhat, Nx, Ny); orc_hermitian_project(M0vhat, Nx, Ny);
+    memcpy(eta, etahat, sizeof(double) * 2 * N); memcpy(u, uhat, sizeof(double) * 2 * N);
+    memcpy(v, vhat, sizeof(double) * 2 * N);
+    double *Lphi = Lphihat, *M0u = M0uhat, *M0v = M0vhat;   /* transformed in place */
+    fft2(&c->P, eta, +1); fft2(&c->P, u, +1); fft2(&c->P, v, +1);
+    fft2(&c->P, Lphi, +1); fft2(&c->P, M0u, +1); fft2(&c->P, M0v, +1);
+
+    if (p->flat) {
+        #pragma omp parallel for schedule(static)
+        for (size_t q = 0; q < N; q++) {
+            dampeta[2 * q] = 7 * p->Cpeak * p->fbdy[q] * eta[2 * q] / (double)N;
+            dampu[2 * q] = 7 * p->Cpeak * p->fbdy[q] * u[2 * q] / (double)N;
+            dampv[2 * q] = 7 * p->Cpeak * p->fbdy[q] * v[2 * q] / (double)N;
+        }
+        fft2(&c->P, dampeta, -1); fft2(&c->P, dampu, -1); fft2(&c->P, dampv, -1);
+    }
+    double N2 = (double)N * (double)N;
+    double *eM0u = cal(N), *eu = cal(N), *eM0v = cal(N), *ev = cal(N), *dE = cal(N);
+    #pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < N; q++) {
+        double w = p->wallchar[q];
+        eM0u[2 * q] = eta[2 * q] * M0u[2 * q] / N2 * w;
+        eu[2 * q] = eta[2 * q] * u[2 * q] / N2 * w;
+        eM0v[2 * q] = eta[2 * q] * M0v[2 * q] / N2 * w;
+        ev[2 * q] = eta[2 * q] * v[2 * q] / N2 * w;
+        dE[2 * q] = 0.5 * ((u[2 * q] * u[2 * q] + v[2 * q] * v[2 * q]) - Lphi[2 * q] * Lphi[2 * q]) / N2 * w;
+    }
+    double *eM0uh = cal(N), *euh = cal(N), *eM0vh = cal(N), *evh = cal(N), *dEh = cal(N);
+    memcpy(eM0uh, eM0u, sizeof(double) * 2 * N); memcpy(euh, eu, sizeof(double) * 2 * N);
+    memcpy(eM0vh, eM0v, sizeof(double) * 2 * N); memcpy(evh, ev, sizeof(double) * 2 * N);
+    memcpy(dEh, dE, sizeof(double) * 2 * N);
+    fft2(&c->P, eM0uh, -1); fft2(&c->P, euh, -1); fft2(&c->P, eM0vh, -1); fft2(&c->P, evh, -1);
+    fft2(&c->P, dEh, -1);
+
+    double *LeM0u = cal(N), *LeM0v = cal(N);
+    if (p->flat) {
+        #pragma omp parallel for schedule(static)
+        for (size_t q = 0; q < N; q++) {
+            LeM0u[2 * q] = p->L2d[q] * eM0uh[2 * q]; LeM0u[2 * q + 1] = p->L2d[q] * eM0uh[2 * q + 1];
+            LeM0v[2 * q] = p->L2d[q] * eM0vh[2 * q]; LeM0v[2 * q + 1] = p->L2d[q] * eM0vh[2 * q + 1];
+        }
+    } else {
+        #pragma omp parallel for schedule(static)
+        for (size_t q = 0; q < N; q++) { eM0u[2 * q] *= (double)N; eM0v[2 * q] *= (double)N; }
+        hss_2depth(c, eM0u, eM0uh, p->om2_m, p->om2_p, LeM0u, (double)N);
+        hss_2depth(c, eM0v, eM0vh, p->om2_m, p->om2_p, LeM0v, (double)N);
+    }
+    double *ikxE = cal(N), *ikyE = cal(N), *duH2 = cal(N), *dvH2 = cal(N);
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < Ny; i++)
+        for (int j = 0; j < Nx; j++) {
+            size_t q = (size_t)i * Nx + j;
+            ikxE[2 * q] = -kx[j] * dEh[2 * q + 1]; ikxE[2 * q + 1] = kx[j] * dEh[2 * q];
+            ikyE[2 * q] = -ky[i] * dEh[2 * q + 1]; ikyE[2 * q + 1] = ky[i] * dEh[2 * q];
+            if (p->flat) {
+                duH2[2 * q] = -LeM0u[2 * q] + kx[j] * euh[2 * q + 1];
+                duH2[2 * q + 1] = -LeM0u[2 * q + 1] - kx[j] * euh[2 * q];
+                dvH2[2 * q] = -LeM0v[2 * q] + ky[i] * evh[2 * q + 1];
+                dvH2[2 * q + 1] = -LeM0v[2 * q + 1] - ky[i] * evh[2 * q];
+            } else {
+                duH2[2 * q] = kx[j] * euh[2 * q + 1] - LeM0u[2 * q] / g;
+                duH2[2 * q + 1] = -kx[j] * euh[2 * q] - LeM0u[2 * q + 1] / g;
+                dvH2[2 * q] = ky[i] * evh[2 * q + 1] - LeM0v[2 * q] / g;
+                dvH2[2 * q + 1] = -ky[i] * evh[2 * q] - LeM0v[2 * q + 1] / g;
+            }
+        }
+    adjust(c, duH2_adj, duH2);
+    adjust(c, dvH2_adj, dvH2);
+    adjust(c, ikxE_adj, ikxE);
+    adjust(c, ikyE_adj, ikyE);
+
+    /* ---- source + epilogue (2d/fun_odesolver.c:4479-4529 / 4564-4613) ---- */
+    double *Shat = cal(N);
+    source_hat(c, t, Shat);
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < Ny; i++)
+        for (int j = 0; j < Nx; j++) {
+            size_t q = (size_t)i * Nx + j;
+            double dphr, dphi;
+            if (p->flat) {
+                dphr = delphiH1[2 * q] + duH2_adj[2 * q] + dvH2_adj[2 * q];
+                dphi = delphiH1[2 * q + 1] + duH2_adj[2 * q + 1] + dvH2_adj[2 * q + 1];
+            } else {
+                dphr = duH1[2 * q] + dvH1[2 * q] + duH2_adj[2 * q] + dvH2_adj[2 * q];
+                dphi = duH1[2 * q + 1] + dvH1[2 * q + 1] + duH2_adj[2 * q + 1] + dvH2_adj[2 * q + 1];
+            }
+            double a = p->aal[q];
+            dZ[q] = (Shat[2 * q] + dphr - dampeta[2 * q]) * a;
+            dZ[N + q] = (Shat[2 * q + 1] + dphi - dampeta[2 * q + 1]) * a;
+            dZ[2 * N + q] = (g * kx[j] * Z[N + q] - ikxE_adj[2 * q] - dampu[2 * q]) * a;
+            dZ[3 * N + q] = (-g * kx[j] * Z[q] - ikxE_adj[2 * q + 1] - dampu[2 * q + 1]) * a;
+            dZ[4 * N + q] = (g * ky[i] * Z[N + q] - ikyE_adj[2 * q] - dampv[2 * q]) * a;
+            dZ[5 * N + q] = (-g * ky[i] * Z[q] - ikyE_adj[2 * q + 1] - dampv[2 * q + 1]) * a;
+        }
+
+    free(eta); free(u); free(v); free(dampeta); free(dampu); free(dampv);
+    free(ikxE_adj); free(ikyE_adj); free(duH2_adj); free(dvH2_adj); free(delphiH1);
+    free(duH1); free(dvH1); free(etahat); free(uhat); free(vhat);
+    free(Lphihat); free(M0uhat); free(M0vhat);
+    free(eM0u); free(eu); free(eM0v); free(ev); free(dE);
+    free(eM0uh); free(euh); free(eM0vh); free(evh); free(dEh);
+    free(LeM0u); free(LeM0v); free(ikxE); free(ikyE); free(duH2); free(dvH2); free(Shat);
+    return 0;
+}
+
+/* ----------------------------------------------------------- transforms -- */
+/* exposed for tests: full complex 2-D FFT (sign -1 forward, +1 backward) */
+void orc_fft2(int Nx, int Ny, double *a, int sign)
+{
+    fft2plan P = {Nx, Ny, plan_new(Nx), plan_new(Ny)};
+    fft2(&P, a, sign);
+    plan_free(P.px); plan_free(P.py);
+}
+
+/* physical field from a full-grid spectrum, as ifftw_2d_c2r (2d/functions1.c:523-559) */
+void orc_ifft_c2r(int Nx, int Ny, const double *spec_re, const double *spec_im, double *out)
+{
+    size_t N = (size_t)Nx * Ny;
+    double *a = cal(N);
+    #pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < N; q++) { a[2 * q] = spec_re[q]; a[2 * q + 1] = spec_im[q]; }
+    orc_hermitian_project(a, Nx, Ny);
+    orc_fft2(Nx, Ny, a, +1);
+    #pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < N; q++) out[q] = a[2 * q] / (double)N;
+    free(a);
+}
+
+/* ------------------------------------------------------------------ RK --- */
+static const double ah[5] = {1.0 / 4.0, 3.0 / 8.0, 12.0 / 13.0, 1.0, 1.0 / 2.0};
+static const double b3[2] = {3.0 / 32.0, 9.0 / 32.0};
+static const double b4[3] = {1932.0 / 2197.0, -7200.0 / 2197.0, 7296.0 / 2197.0};
+static const double b5[4] = {8341.0 / 4104.0, -32832.0 / 4104.0, 29440.0 / 4104.0, -845.0 / 4104.0};
+static const double b6[5] = {-6080.0 / 20520.0, 41040.0 / 20520.0, -28352.0 / 20520.0, 9295.0 / 20520.0, -5643.0 / 20520.0};
+static const double c1 = 902880.0 / 7618050.0, c3 = 3953664.0 / 7618050.0, c4 = 3855735.0 / 7618050.0,
+                    c5 = -1371249.0 / 7618050.0, c6 = 277020.0 / 7618050.0;
+static const double ec[7] = {0.0, 1.0 / 360.0, 0.0, -128.0 / 4275.0, -2197.0 / 75240.0, 1.0 / 50.0, 2.0 / 55.0};
+
+static void rk_alloc(orc_ctx *c)
+{
+    if (c->ytmp) return;
+    for (int i = 0; i < 6; i++) c->k[i] = (double *)malloc(sizeof(double) * c->dim);
+    c->ytmp = (double *)malloc(sizeof(double) * c->dim);
+    c->y0 = (double *)malloc(sizeof(double) * c->dim);
+    c->yerr = (double *)malloc(sizeof(double) * c->dim);
+    c->dydt_in = (double *)malloc(sizeof(double) * c->dim);
+    c->dydt_out = (double *)malloc(sizeof(double) * c->dim);
+}
+
+/* one rkf45 step (GSL odeiv2 rkf45 apply with dydt_in and dydt_out) */
+static int rkf45_apply(orc_ctx *c, double t, double h, double *y)
+{
+    size_t n = c->dim;
+    double **k = c->k, *yt = c->ytmp;
+    memcpy(k[0], c->dydt_in, sizeof(double) * n);
+#pragma omp parallel for
+    for (size_t i = 0; i < n; i++) yt[i] = y[i] + ah[0] * h * k[0][i];
+    orc_rhs(c, t + ah[0] * h, yt, k[1]);
+#pragma omp parallel for
+    for (size_t i = 0; i < n; i++) yt[i] = y[i] + h * (b3[0] * k[0][i] + b3[1] * k[1][i]);
+    orc_rhs(c, t + ah[1] * h, yt, k[2]);
+#pragma omp parallel for
+    for (size_t i = 0; i < n; i++) yt[i] = y[i] + h * (b4[0] * k[0][i] + b4[1] * k[1][i] + b4[2] * k[2][i]);
+    orc_rhs(c, t + ah[2] * h, yt, k[3]);
+#pragma omp parallel for
+    for (size_t i = 0; i < n; i++)
+        yt[i] = y[i] + h * (b5[0] * k[0][i] + b5[1] * k[1][i] + b5[2] * k[2][i] + b5[3] * k[3][i]);
+    orc_rhs(c, t + ah[3] * h, yt, k[4]);
+#pragma omp parallel for
+    for (size_t i = 0; i < n; i++)
+        yt[i] = y[i] + h * (b6[0] * k[0][i] + b6[1] * k[1][i] + b6[2] * k[2][i] + b6[3] * k[3][i] + b6[4] * k[4][i]);
+    orc_rhs(c, t + ah[4] * h, yt, k[5]);
+#pragma omp parallel for
+    for (size_t i = 0; i < n; i++) {
+        double d = c1 * k[0][i] + c3 * k[2][i] + c4 * k[3][i] + c5 * k[4][i] + c6 * k[5][i];
+        y[i] += h * d;
+    }
+    if (c->dydt_out) orc_rhs(c, t + h, y, c->dydt_out);
+#pragma omp parallel for
+    for (size_t i = 0; i < n; i++)
+        c->yerr[i] = h * (ec[1] * k[0][i] + ec[3] * k[2][i] + ec[4] * k[3][i] + ec[5] * k[4][i] + ec[6] * k[5][i]);
+    return 0;
+}
+
+/* control_y hadjust: returns -1 dec, 0 nil, +1 inc */
+static int hadjust(orc_ctx *c, const double *y, double eps_abs, double eps_rel, double *h, double *rmax_out)
+{
+    const int ord = 5;
+    double rmax = 2.2250738585072014e-308; /* DBL_MIN */
+    size_t n = c->dim;
+    for (size_t i = 0; i < n; i++) {
+        double D0 = eps_rel * fabs(y[i]) + eps_abs;
+        double r = fabs(c->yerr[i]) / fabs(D0);
+        rmax = r > rmax ? r : rmax;
+    }
+    if (rmax_out) *rmax_out = rmax;
+    double h_old = *h;
+    if (rmax > 1.1) {
+        double r = 0.9 / pow(rmax, 1.0 / ord);
+        if (r < 0.2) r = 0.2;
+        *h = r * h_old;
+        return -1;
+    } else if (rmax < 0.5) {
+        double r = 0.9 / pow(rmax, 1.0 / (ord + 1.0));
+        if (r > 5.0) r = 5.0;
+        if (r < 1.0) r = 1.0;
+        *h = r * h_old;
+        return 1;
+    }
+    return 0;
+}
+
+/* gsl_odeiv2_evolve_apply restated. returns 0 on success, nonzero on failure.
+ * stats[0] += accepted, stats[1] += rejected */
+int orc_evolve_apply(void *vc, double *t, double t1, double *h, double *y, double eps_abs, double eps_rel, long *stats)
+{
+    orc_ctx *c = (orc_ctx *)vc;
+    rk_alloc(c);
+    size_t n = c->dim;
+    const double t0 = *t;
+    double h0 = *h;
+    int final_step;
+    double dt = t1 - t0;
+    memcpy(c->y0, y, sizeof(double) * n);
+    orc_rhs(c, t0, c->y0, c->dydt_in);
+    for (;;) {
+        if ((dt >= 0.0 && h0 > dt) || (dt < 0.0 && h0 < dt)) { h0 = dt; final_step = 1; }
+        else final_step = 0;
+        rkf45_apply(c, t0, h0, y);
+        *t = final_step ? t1 : t0 + h0;
+        double h_old = h0;
+        int st = hadjust(c, y, eps_abs, eps_rel, &h0, NULL);
+        if (st < 0) {
+            double t_curr = *t, t_next = *t + h0;
+            if (fabs(h0) < fabs(h_old) && t_next != t_curr) {
+                memcpy(y, c->y0, sizeof(double) * n);
+                if (stats) stats[1]++;
+                continue;
+            } else {
+                *h = h0; *t = t0;
+                return 1;
+            }
+        }
+        if (stats) stats[0]++;
+        *h = h0;
+        return 0;
+    }
+}
+
+/* the driver loop of ode_solver_uv (2d/fun_odesolver.c:2641-2647) to t1 */
+int orc_integrate(void *vc, double *t, double t1, double *h, double *y, double eps_abs, double eps_rel, long *stats)
+{
+    while (*t < t1) {
+        int s = orc_evolve_apply(vc, t, t1, h, y, eps_abs, eps_rel, stats);
+        if (s) return s;
+    }
+    return 0;
+}
+
+int orc_num_threads(void) { return omp_get_max_threads(); }
+void orc_set_threads(int n) { omp_set_num_threads(n); }

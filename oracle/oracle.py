@@ -1,0 +1,176 @@
+"""TEST INFRASTRUCTURE ONLY -- ctypes wrapper of the FP64 CPU oracle.
+
+The oracle (hawassi_oracle.c) is a plain-C restatement of the reference's
+2-D AB2 right-hand side and GSL-style RKF45 driver.  Only tests/,
+__graft_entry__.smoke() and bench.py's CPU-baseline leg may use it, and only
+as the checker; the product library never calls it.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+_dp = ct.POINTER(ct.c_double)
+
+
+class OrcParams(ct.Structure):
+    _fields_ = [
+        ("Nx", ct.c_int), ("Ny", ct.c_int), ("flat", ct.c_int), ("wall", ct.c_int), ("refl_one", ct.c_int),
+        ("Cpeak", ct.c_double),
+        ("kx", _dp), ("ky", _dp), ("aal", _dp), ("fbdy", _dp), ("wallchar", _dp), ("chi", _dp),
+        ("C2d", _dp), ("L2d", _dp), ("gwx", _dp), ("gwy", _dp),
+        ("gam_m", _dp), ("gam_p", _dp), ("csq_m", _dp), ("csq_p", _dp), ("om2_m", _dp), ("om2_p", _dp),
+        ("source_on", ct.c_int), ("vertical", ct.c_int), ("propagation", ct.c_int),
+        ("newGam", _dp), ("SkewGam", _dp),
+        ("nline", ct.c_int), ("nt", ct.c_int),
+        ("tsig", _dp), ("sig", _dp), ("skew", _dp),
+    ]
+
+
+def build():
+    r = subprocess.run(["make", "-C", str(HERE)], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stderr)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = ct.CDLL(str(LIB))
+        L.orc_create.argtypes = [ct.POINTER(OrcParams)]
+        L.orc_create.restype = ct.c_void_p
+        L.orc_destroy.argtypes = [ct.c_void_p]
+        L.orc_rhs.argtypes = [ct.c_void_p, ct.c_double, _dp, _dp]
+        L.orc_rhs_count.argtypes = [ct.c_void_p]
+        L.orc_rhs_count.restype = ct.c_long
+        L.orc_evolve_apply.argtypes = [ct.c_void_p, _dp, ct.c_double, _dp, _dp, ct.c_double, ct.c_double,
+                                       ct.POINTER(ct.c_long)]
+        L.orc_integrate.argtypes = L.orc_evolve_apply.argtypes
+        L.orc_hermitian_project.argtypes = [_dp, ct.c_int, ct.c_int]
+        L.orc_fft2.argtypes = [ct.c_int, ct.c_int, _dp, ct.c_int]
+        L.orc_ifft_c2r.argtypes = [ct.c_int, ct.c_int, _dp, _dp, _dp]
+        L.orc_spline_coeffs.argtypes = [_dp, _dp, ct.c_int, _dp]
+        L.orc_spline_index.argtypes = [_dp, ct.c_int, ct.c_double]
+        L.orc_num_threads.restype = ct.c_int
+        L.orc_set_threads.argtypes = [ct.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+class Oracle2D:
+    """FP64 CPU restatement of rhs_AB2_2DF_uv / rhs_AB2_2DB_uv + rkf45."""
+
+    def __init__(self, model):
+        L = lib()
+        keep = {}
+
+        def arr(name, a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            keep[name] = a
+            return _p(a)
+
+        p = OrcParams()
+        p.Nx, p.Ny, p.flat, p.wall, p.refl_one = model.nx, model.ny, int(model.flat), int(model.wall), int(model.refl_one)
+        p.Cpeak = model.cpeak
+        p.kx, p.ky = arr("kx", model.kx), arr("ky", model.ky)
+        p.aal, p.fbdy = arr("aal", model.aal), arr("fbdy", model.fbdy)
+        p.wallchar, p.chi = arr("w", model.wallchar), arr("chi", model.chi)
+        if model.flat:
+            p.C2d, p.L2d = arr("c2d", model.c2d), arr("l2d", model.l2d)
+            p.gwx, p.gwy = arr("gwx", model.grad_wall_x), arr("gwy", model.grad_wall_y)
+        else:
+            o = model.ops
+            p.gam_m, p.gam_p = arr("gm", o["gam_min"]), arr("gp", o["gam_plus"])
+            p.csq_m, p.csq_p = arr("cm", o["csq_min"]), arr("cp", o["csq_plus"])
+            p.om2_m, p.om2_p = arr("om", o["om2_min"]), arr("op", o["om2_plus"])
+        p.source_on = int(model.source_on)
+        if model.source_on:
+            p.vertical, p.propagation = int(model.vertical), int(model.propagation)
+            p.newGam, p.SkewGam = arr("ng", model.new_gam), arr("sg", model.skew_gam)
+            p.nline, p.nt = model.sig.shape
+            p.tsig, p.sig, p.skew = arr("ts", model.t_sig), arr("sig", model.sig), arr("sk", model.skew_sig)
+        self._keep = keep
+        self._p = p
+        self.ctx = L.orc_create(ct.byref(p))
+        self.model = model
+        self.dim = 6 * model.nx * model.ny
+
+    def __del__(self):
+        try:
+            lib().orc_destroy(self.ctx)
+        except Exception:
+            pass
+
+    def rhs(self, t, y):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        out = np.empty_like(y)
+        lib().orc_rhs(self.ctx, float(t), _p(y), _p(out))
+        return out
+
+    def evolve_apply(self, t, t1, h, y, eps_abs, eps_rel):
+        tt, hh = ct.c_double(t), ct.c_double(h)
+        st = (ct.c_long * 2)()
+        rc = lib().orc_evolve_apply(self.ctx, ct.byref(tt), float(t1), ct.byref(hh), _p(y), eps_abs, eps_rel, st)
+        return rc, tt.value, hh.value, (st[0], st[1])
+
+    def integrate(self, t, t1, h, y, eps_abs, eps_rel):
+        tt, hh = ct.c_double(t), ct.c_double(h)
+        st = (ct.c_long * 2)()
+        rc = lib().orc_integrate(self.ctx, ct.byref(tt), float(t1), ct.byref(hh), _p(y), eps_abs, eps_rel, st)
+        return rc, tt.value, hh.value, (st[0], st[1])
+
+    def rhs_count(self):
+        return lib().orc_rhs_count(self.ctx)
+
+
+def hermitian_project(a: np.ndarray) -> np.ndarray:
+    """fftwcomplex_2d_sym on a copy of a complex [ny, nx] array."""
+    ny, nx = a.shape
+    buf = np.empty(2 * a.size)
+    buf[0::2] = a.real.ravel()
+    buf[1::2] = a.imag.ravel()
+    lib().orc_hermitian_project(_p(buf), nx, ny)
+    return (buf[0::2] + 1j * buf[1::2]).reshape(ny, nx)
+
+
+def fft2(a: np.ndarray, sign: int) -> np.ndarray:
+    ny, nx = a.shape
+    buf = np.empty(2 * a.size)
+    buf[0::2] = a.real.ravel()
+    buf[1::2] = a.imag.ravel()
+    lib().orc_fft2(nx, ny, _p(buf), sign)
+    return (buf[0::2] + 1j * buf[1::2]).reshape(ny, nx)
+
+
+def ifft_c2r(spec: np.ndarray) -> np.ndarray:
+    ny, nx = spec.shape
+    re = np.ascontiguousarray(spec.real)
+    im = np.ascontiguousarray(spec.imag)
+    out = np.empty((ny, nx))
+    lib().orc_ifft_c2r(nx, ny, _p(re), _p(im), _p(out))
+    return out
+
+
+def set_threads(n: int):
+    lib().orc_set_threads(int(n))
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()

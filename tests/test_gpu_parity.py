@@ -1,0 +1,145 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 oracle.
+
+Tolerances: FP64 device path relative L2 <= 1e-11 per field (same operator,
+different summation order); FP32 transforms/operators relative L2 <= 3e-5
+per RHS field and <= 1e-4 on states after integration (the path's stated
+accuracy target is relL2 <= 1e-4 against the FP64 oracle).
+"""
+import sys
+
+import numpy as np
+import pytest
+from conftest import relerr
+
+import halfspec as HS
+from hawassi_c_amd import HW_PREC_FP32, HW_PREC_FP64
+from hawassi_c_amd import workloads as W
+from hawassi_c_amd.model2d import Hawassi2D, fft2_forward, fft2_inverse
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL_RHS = {HW_PREC_FP64: 1e-11, HW_PREC_FP32: 3e-5}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("prec", [HW_PREC_FP64, HW_PREC_FP32])
+@pytest.mark.parametrize("ny,nx", [(32, 32), (64, 64), (32, 64), (64, 32), (64, 128), (256, 256)])
+def test_fft_forward(prec, ny, nx):
+    say("fft_forward", prec, ny, nx)
+    f = np.random.default_rng(nx + ny).standard_normal((ny, nx))
+    s = fft2_forward(f, prec)
+    tol = 1e-13 if prec == HW_PREC_FP64 else 2e-6
+    assert relerr(s, np.fft.fft2(f)) < tol
+
+
+@pytest.mark.parametrize("prec", [HW_PREC_FP64, HW_PREC_FP32])
+@pytest.mark.parametrize("ny,nx", [(32, 32), (64, 64), (32, 64), (64, 32), (64, 128), (256, 256)])
+def test_fft_inverse_projection_semantics(prec, ny, nx):
+    say("fft_inverse", prec, ny, nx)
+    g = np.random.default_rng(3 * nx + ny)
+    a = g.standard_normal((ny, nx)) + 1j * g.standard_normal((ny, nx))   # not Hermitian
+    phys, nyq = fft2_inverse(a, prec)
+    ref = O.ifft_c2r(a)
+    _, s_ref = HS.c2r(a[:, : nx // 2 + 1], nx)
+    tol = 1e-13 if prec == HW_PREC_FP64 else 2e-6
+    assert relerr(phys, ref) < tol
+    assert relerr(nyq, s_ref) < tol * 10
+
+
+@pytest.mark.parametrize("prec", [HW_PREC_FP64, HW_PREC_FP32])
+@pytest.mark.parametrize("n", [32, 64, 128])
+@pytest.mark.parametrize("kind", ["harbour", "short_crested"])
+def test_rhs_parity(prec, n, kind):
+    say("rhs_parity", kind, prec, n)
+    m = getattr(W, kind)(n, t_end=8.0)
+    y = W.random_state(m, 5, kfrac=0.8)
+    ref = HS.oracle_half(O.Oracle2D(m).rhs(2.345, y), n, n)
+    dev = Hawassi2D(m, prec)
+    got = HS.oracle_half(dev.rhs(2.345, y), n, n)
+    for f in range(3):
+        e = relerr(got[f], ref[f])
+        say("  field", f, "relL2", e)
+        assert e < TOL_RHS[prec]
+
+
+@pytest.mark.parametrize("prec", [HW_PREC_FP64, HW_PREC_FP32])
+def test_rhs_parity_full_band_nyquist(prec):
+    """States with content up to the Nyquist column exercise the imaginary
+    part of the adjust operator."""
+    say("rhs_full_band", prec)
+    m = W.harbour(64, t_end=8.0)
+    y = W.random_state(m, 9, kfrac=1.0)
+    ref = HS.oracle_half(O.Oracle2D(m).rhs(0.4, y), 64, 64)
+    got = HS.oracle_half(Hawassi2D(m, prec).rhs(0.4, y), 64, 64)
+    for f in range(3):
+        assert relerr(got[f], ref[f]) < TOL_RHS[prec]
+
+
+@pytest.mark.parametrize("prec,tol,eps", [(HW_PREC_FP64, 1e-8, 1e-10), (HW_PREC_FP32, 1e-4, 1e-6)])
+@pytest.mark.parametrize("kind", ["harbour", "short_crested"])
+def test_trajectory_parity(prec, tol, eps, kind):
+    """Adaptive RKF45 from rest with the influx source on, to t = 1, 2, 3 s.
+    The device integrates the stored half spectrum and takes its error norm
+    there, while the oracle (like the reference) also includes the redundant
+    half, so step sequences can differ; tolerances eps are tight enough that
+    both trajectories are within tol of each other regardless."""
+    say("trajectory", kind, prec)
+    n = 64
+    m = getattr(W, kind)(n, t_end=8.0)
+    m.eps_abs = m.eps_rel = eps
+    o = O.Oracle2D(m)
+    y = np.zeros(6 * n * n)
+    dev = Hawassi2D(m, prec)
+    dev.set_state(y)
+    t_o, h_o, t_d, h_d = 0.0, 0.05, 0.0, 0.05
+    for t1 in (1.0, 2.0, 3.0):
+        rc, t_o, h_o, _ = o.integrate(t_o, t1, h_o, y, m.eps_abs, m.eps_rel)
+        assert rc == 0
+        t_d, h_d = dev.integrate(t_d, t1, h_d)
+        assert t_d == t1 and t_o == t1
+    got = HS.oracle_half(dev.get_state(), n, n)
+    ref = HS.oracle_half(y, n, n)
+    for f in range(3):
+        e = relerr(got[f], ref[f])
+        say("  field", f, "relL2", e)
+        assert e < tol
+    st = dev.stats()
+    assert st["steps_accepted"] > 0 and st["rhs_evals"] >= 6 * st["steps_accepted"]
+
+
+def test_fields_match_oracle_inverse():
+    say("fields")
+    n = 64
+    m = W.harbour(n, t_end=8.0)
+    y = W.random_state(m, 12, kfrac=0.6)
+    dev = Hawassi2D(m, HW_PREC_FP64)
+    dev.set_state(y)
+    eta, u, v = dev.fields()
+    pl = y.reshape(6, n, n)
+    for f, got in enumerate((eta, u, v)):
+        ref = O.ifft_c2r(pl[2 * f] + 1j * pl[2 * f + 1])
+        assert relerr(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("kind,n", [("harbour", 2048), ("short_crested", 1024)])
+def test_full_size_fp32_matches_fp64_device(kind, n):
+    """At full size the oracle is too slow for every test; the FP32 path is
+    checked against the FP64 device path (itself checked against the oracle
+    above), field by field, on a physical state."""
+    say("full_size", kind, n)
+    m = getattr(W, kind)(n, t_end=8.0)
+    y = W.random_state(m, 21, kfrac=0.3)
+    d64 = Hawassi2D(m, HW_PREC_FP64)
+    r64 = HS.oracle_half(d64.rhs(3.0, y), n, n)
+    d64.close()
+    d32 = Hawassi2D(m, HW_PREC_FP32)
+    r32 = HS.oracle_half(d32.rhs(3.0, y), n, n)
+    for f in range(3):
+        e = relerr(r32[f], r64[f])
+        say("  field", f, "relL2", e)
+        assert e < 1e-4

@@ -1,0 +1,207 @@
+"""Pinning the FP64 CPU oracle (no GPU).
+
+The reference ships no 2-D outputs, so the oracle is pinned by textbook
+routines (numpy FFT, brute-force DFT, scipy's natural spline), closed forms
+(single-mode exact-dispersion propagation), exact operator identities (the
+two-depth operator with gamma=(1,0) is the flat operator) and the
+independent half-spectrum derivation in tests/halfspec.py.
+"""
+import math
+
+import numpy as np
+import pytest
+from conftest import relerr
+
+from oracle import oracle as O
+from hawassi_c_amd import setup2d as S
+from hawassi_c_amd import workloads as W
+import halfspec as HS
+
+
+def rnd(shape, seed):
+    g = np.random.default_rng(seed)
+    return g.standard_normal(shape) + 1j * g.standard_normal(shape)
+
+
+@pytest.mark.parametrize("ny,nx", [(4, 8), (16, 32), (64, 64)])
+def test_fft2_matches_numpy_and_bruteforce(ny, nx):
+    a = rnd((ny, nx), 1)
+    f = O.fft2(a, -1)
+    assert relerr(f, np.fft.fft2(a)) < 1e-14
+    b = O.fft2(a, +1)
+    assert relerr(b, np.fft.ifft2(a) * nx * ny) < 1e-14
+    if nx * ny <= 64:
+        jj, ii = np.meshgrid(np.arange(nx), np.arange(ny))
+        bf = np.zeros((ny, nx), complex)
+        for p in range(ny):
+            for q in range(nx):
+                bf[p, q] = np.sum(a * np.exp(-2j * np.pi * (p * ii / ny + q * jj / nx)))
+        assert relerr(f, bf) < 1e-13
+
+
+def test_projection_identity_on_real_field_spectra():
+    f = np.random.default_rng(2).standard_normal((32, 64))
+    s = np.fft.fft2(f)
+    assert relerr(O.hermitian_project(s), s) < 1e-15
+
+
+def test_projection_structure_on_general_input():
+    ny, nx = 16, 32
+    a = rnd((ny, nx), 3)
+    p = O.hermitian_project(a)
+    # idempotent
+    assert np.array_equal(O.hermitian_project(p), p)
+    # interior lower-half columns untouched
+    assert np.array_equal(p[:, 1: nx // 2], a[:, 1: nx // 2])
+    # Nyquist column untouched except Im at rows 0 and ny/2
+    for i in range(ny):
+        exp = a[i, nx // 2].real if i in (0, ny // 2) else a[i, nx // 2]
+        assert p[i, nx // 2] == exp
+    # upper columns are the conjugate mirror
+    for i in range(ny):
+        for j in range(nx // 2 + 1, nx):
+            assert p[i, j] == np.conj(p[(ny - i) % ny, nx - j])
+    # the real part of the inverse equals numpy's inverse of the half spectrum
+    # with column 0 rebuilt from its top half (the Nyquist column averaged)
+    phys, _ = HS.c2r(a[:, : nx // 2 + 1], nx)
+    assert relerr(phys, O.ifft_c2r(a)) < 1e-14
+
+
+def test_natural_spline_matches_scipy():
+    from scipy.interpolate import CubicSpline
+    g = np.random.default_rng(4)
+    x = np.sort(g.uniform(0, 10, 40))
+    y = np.sin(x) + 0.1 * g.standard_normal(40)
+    cs = CubicSpline(x, y, bc_type="natural")
+    c = S.spline_coeffs(x, y)
+    tq = g.uniform(x[0], x[-1], 200)
+    vals = np.array([S.spline_eval(x, y, c, t) for t in tq])
+    assert np.abs(vals - cs(tq)).max() < 1e-12
+    co = np.empty_like(x)
+    O.lib().orc_spline_coeffs(x.ctypes.data_as(O._dp), y.ctypes.data_as(O._dp), len(x), co.ctypes.data_as(O._dp))
+    assert np.abs(co - c).max() < 1e-12
+
+
+def test_rkf45_tableau_consistency():
+    from hawassi_c_amd import setup2d  # noqa: F401
+    ah = [1 / 4, 3 / 8, 12 / 13, 1, 1 / 2]
+    rows = [[1 / 4], [3 / 32, 9 / 32], [1932 / 2197, -7200 / 2197, 7296 / 2197],
+            [8341 / 4104, -32832 / 4104, 29440 / 4104, -845 / 4104],
+            [-6080 / 20520, 41040 / 20520, -28352 / 20520, 9295 / 20520, -5643 / 20520]]
+    for a, r in zip(ah, rows):
+        assert abs(sum(r) - a) < 1e-14
+    c = [902880 / 7618050, 0, 3953664 / 7618050, 3855735 / 7618050, -1371249 / 7618050, 277020 / 7618050]
+    assert abs(sum(c) - 1) < 1e-14
+    assert abs(c[0] - 16 / 135) < 1e-15 and abs(c[5] - 2 / 55) < 1e-15
+    ec = [1 / 360, 0, -128 / 4275, -2197 / 75240, 1 / 50, 2 / 55]
+    assert abs(sum(ec)) < 1e-15
+
+
+def linear_flat_model(n=32, depth=5.0, length=200.0):
+    """Flat bottom with every mask neutral: the linear part is a set of
+    uncoupled exact-dispersion oscillators."""
+    x = S.interval_to_vector(0.0, length, n)
+    kx = S.freqspace(length, n)
+    ky = S.freqspace(length, n)
+    KX, KY = kx[None, :], ky[:, None]
+    om = S.exact_disp2d(KX, KY, depth)
+    one = np.ones((n, n))
+    return S.Model2D(nx=n, ny=n, x=x, y=x, kx=kx, ky=ky, flat=True, cpeak=1.0, aal=one, fbdy=0 * one,
+                     wallchar=one, chi=one, depth=depth * one, c2d=S.exact_cp2d(KX, KY, depth),
+                     l2d=om * om / S.GRAV)
+
+
+def single_mode_state(m, p, q, amp):
+    n = m.nx
+    x = np.arange(n)
+    X, Y = np.meshgrid(x, x)
+    eta = amp * np.cos(2 * np.pi * (q * X + p * Y) / n)
+    return eta
+
+
+def test_single_mode_exact_dispersion():
+    """A small-amplitude single Fourier mode oscillates at omega(k) of the
+    exact dispersion relation: eta^(t) = eta^(0) cos(w t) (u^ = v^ = 0 at
+    t=0).  Pins the linear RHS and the RKF45 driver together."""
+    m = linear_flat_model()
+    n = m.nx
+    p, q = 2, 3
+    eta0 = single_mode_state(m, p, q, 1e-7)
+    from hawassi_c_amd.model2d import initial_state
+    y = initial_state(eta0)
+    o = O.Oracle2D(m)
+    kk = math.hypot(m.kx[q], m.ky[p])
+    w = math.sqrt(S.GRAV * kk * math.tanh(kk * 5.0))
+    T = 2.7
+    rc, t, h, st = o.integrate(0.0, T, 0.01, y, 1e-14, 1e-10)
+    assert rc == 0 and abs(t - T) < 1e-12
+    e0 = np.fft.fft2(eta0)[p, q]
+    eT = (y[: n * n] + 1j * y[n * n: 2 * n * n]).reshape(n, n)[p, q]
+    assert abs(eT / e0 - math.cos(w * T)) < 1e-7
+
+
+def test_two_depth_with_unit_weights_equals_flat():
+    """HSS with gamma_-=1, gamma_+=0 is the single-depth operator, so
+    rhs_AB2_2DB_uv with those weights (csq_- = C^2, om2_- = g L) equals
+    rhs_AB2_2DF_uv on Hermitian states at every spectral point the
+    projection does not rebuild (columns 1..nx/2, and column 0 rows <= ny/2).
+    Elsewhere they differ: the reference's k grid is not antisymmetric
+    (freqspace gives k[n-j] + k[j] = -2 pi / (L (n-1))), so the flat path's
+    pointwise C2d^2 u^ and the projected HSS differ at the mirrored points."""
+    mf = W.short_crested(32, t_end=3.0)
+    n = mf.nx
+    one, zero = np.ones((n, n)), np.zeros((n, n))
+    for which in ("min", "plus"):
+        ops = dict(gam_min=one if which == "min" else zero, gam_plus=zero if which == "min" else one,
+                   csq_min=mf.c2d ** 2, csq_plus=mf.c2d ** 2 * 0.5, om2_min=S.GRAV * mf.l2d, om2_plus=S.GRAV * mf.l2d * 0.5)
+        if which == "plus":
+            ops["csq_min"], ops["csq_plus"] = ops["csq_plus"], ops["csq_min"]
+            ops["om2_min"], ops["om2_plus"] = ops["om2_plus"], ops["om2_min"]
+        mb = S.Model2D(**{**mf.__dict__, "flat": False, "ops": ops})
+        y = W.random_state(mf, 11, kfrac=0.5)
+        df = HS.oracle_half(O.Oracle2D(mf).rhs(0.7, y), n, n)
+        db = HS.oracle_half(O.Oracle2D(mb).rhs(0.7, y), n, n)
+        # residual 1e-9: the same asymmetry enters the Nyquist column the adjust reads
+        assert relerr(db[:, :, 1:], df[:, :, 1:]) < 1e-8
+        assert relerr(db[:, : n // 2 + 1, 0], df[:, : n // 2 + 1, 0]) < 1e-8
+        assert relerr(db[0, n // 2 + 1:, 0], df[0, n // 2 + 1:, 0]) > 1e-8   # the asymmetry is real
+
+
+@pytest.mark.parametrize("n,kfrac", [(32, 1.0), (64, 0.4)])
+def test_half_spectrum_formulation_matches_oracle(n, kfrac):
+    m = W.harbour(n, t_end=5.0)
+    y = W.random_state(m, 7, kfrac=kfrac)
+    ref = HS.oracle_half(O.Oracle2D(m).rhs(1.23, y), n, n)
+    got = HS.rhs_2db(m, 1.23, y)
+    for f in range(3):
+        assert relerr(got[f], ref[f]) < 1e-13
+
+
+def test_half_spectrum_needs_nyquist_imaginary_term():
+    """Plant the mistake of dropping fun_adjust's imaginary part: it must
+    fail, so the check above is sensitive to it."""
+    m = W.harbour(32, t_end=5.0)
+    y = W.random_state(m, 7, kfrac=1.0)
+    ref = HS.oracle_half(O.Oracle2D(m).rhs(1.23, y), 32, 32)
+    orig = HS.c2r
+    try:
+        HS.c2r = lambda S_, nx: (orig(S_, nx)[0], 0 * orig(S_, nx)[1])
+        got = HS.rhs_2db(m, 1.23, y)
+    finally:
+        HS.c2r = orig
+    assert max(relerr(got[f], ref[f]) for f in range(3)) > 1e-8
+
+
+def test_evolve_apply_step_control():
+    """First evolve_apply from rest with a source: the step is accepted, t
+    advances by the clipped step and h follows control_y's growth rule."""
+    m = W.harbour(32, t_end=5.0)
+    o = O.Oracle2D(m)
+    y = np.zeros(o.dim)
+    rc, t, h, st = o.evolve_apply(0.0, 1.0, 0.02, y, 1e-3, 1e-2)
+    assert rc == 0 and st[0] == 1
+    assert t == pytest.approx(0.02) or t < 0.02
+    assert 0.02 <= h <= 0.1 + 1e-15
+    # final-step clipping: request past t1
+    rc, t2, h2, st2 = o.evolve_apply(t, t + 1e-4, 1.0, y, 1e-3, 1e-2)
+    assert rc == 0 and t2 == t + 1e-4
