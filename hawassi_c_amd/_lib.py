@@ -26,6 +26,11 @@ class HwConfig2D(ct.Structure):
         ("nline", ct.c_int), ("nt", ct.c_int),
         ("t_sig", _dp), ("sig", _dp), ("skew_sig", _dp),
         ("eps_abs", ct.c_double), ("eps_rel", ct.c_double),
+        ("breaking_on", ct.c_int),
+        ("KBC", ct.c_double), ("TC", ct.c_double), ("Tchar", ct.c_double), ("delb", ct.c_double),
+        ("max_eta_init", ct.c_double), ("tstar", ct.c_double), ("rsearch", ct.c_double), ("current", ct.c_double),
+        ("pbchar", _dp), ("depth", _dp), ("x", _dp), ("y", _dp),
+        ("t0", ct.c_double),
     ]
 
 
@@ -38,6 +43,7 @@ EXPORTS = [
     "hw_ctx_create_2d", "hw_ctx_destroy", "hw_rhs", "hw_rhs_device", "hw_set_state", "hw_get_state",
     "hw_evolve_apply", "hw_integrate", "hw_get_fields", "hw_get_stats", "hw_reset_stats", "hw_sync",
     "hw_fft2_forward", "hw_fft2_inverse", "hw_supported", "hw_last_error", "hw_version",
+    "hw_set_tprev", "hw_get_breaking",
 ]
 
 _lib = None
@@ -70,6 +76,8 @@ def load(path: Path | None = None) -> ct.CDLL:
     lib.hw_get_stats.argtypes = [vp, ct.POINTER(HwStats)]
     lib.hw_reset_stats.argtypes = [vp]
     lib.hw_sync.argtypes = [vp]
+    lib.hw_set_tprev.argtypes = [vp, ct.c_double]
+    lib.hw_get_breaking.argtypes = [vp, _dp, ct.POINTER(ct.c_int)]
     lib.hw_fft2_forward.argtypes = [ct.c_int, ct.c_int, ct.c_int, _dp, _dp, _dp]
     lib.hw_fft2_inverse.argtypes = [ct.c_int, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp]
     lib.hw_supported.argtypes = [ct.c_int, ct.c_int, ct.c_int]

@@ -75,6 +75,15 @@ class Hawassi2D:
             c.nline, c.nt = model.sig.shape
             c.t_sig, c.sig, c.skew_sig = arr("ts", model.t_sig), arr("sig", model.sig), arr("sk", model.skew_sig)
         c.eps_abs, c.eps_rel = model.eps_abs, model.eps_rel
+        br = model.breaking
+        if br is not None:
+            c.breaking_on = 1
+            c.KBC, c.TC, c.Tchar, c.delb = br["KBC"], br["TC"], br["Tchar"], br["delb"]
+            c.max_eta_init, c.tstar, c.rsearch, c.current = br["MaxEtaInit"], br["Tstar"], br["Rsearch"], 0.0
+            c.pbchar = arr("pbchar", br["char"])
+            c.depth = arr("depth", model.depth)
+            c.x, c.y = arr("x", model.x), arr("y", model.y)
+        c.t0 = float(model.meta.get("t0", 0.0))
         self._cfg, self._keep = c, keep
         h = ct.c_void_p()
         _lib.check(self.lib.hw_ctx_create_2d(ct.byref(c), ct.byref(h)), "hw_ctx_create_2d")
@@ -131,6 +140,15 @@ class Hawassi2D:
         _lib.check(self.lib.hw_get_stats(self.ctx, ct.byref(s)), "hw_get_stats")
         return dict(rhs_evals=s.rhs_evals, steps_accepted=s.steps_accepted, steps_rejected=s.steps_rejected,
                     rhs_ms_total=s.rhs_ms_total)
+
+    def set_tprev(self, t: float):
+        _lib.check(self.lib.hw_set_tprev(self.ctx, float(t)), "hw_set_tprev")
+
+    def breaking_state(self):
+        B = np.empty(self.nx * self.ny)
+        f = ct.c_int()
+        _lib.check(self.lib.hw_get_breaking(self.ctx, _lib.ptr(B), ct.byref(f)), "hw_get_breaking")
+        return B.reshape(self.ny, self.nx), f.value
 
     def reset_stats(self):
         _lib.check(self.lib.hw_reset_stats(self.ctx), "hw_reset_stats")

@@ -451,6 +451,18 @@ def _make_1dcom_sym(a):
     return a
 
 
+# ------------------------------------------------------------ breaking --
+def breaking_params(hs: float, tp: float, kp: float, cfsa: np.ndarray, chi: np.ndarray,
+                    KBC: float = 0.85, TC: float = 0.2, Tchar: float = 0.5, delb: float = 1.5) -> dict:
+    """parambreak (2d/breaking.c:1-21) with pb.delb = 1.5 (2d/functions.c:204):
+    onset threshold MaxEtaInit = 0.9 Hs/2, T* = Tchar Tp, search radius
+    lambda_p/4, and the checked region pb.Char = 1 where cfSA = 1 and
+    ChiAdj = 1 (outside absorbing strips, walls and the adjustment zone)."""
+    char = np.where((cfsa < 1) | (chi < 1), 0.0, cfsa)
+    return dict(KBC=KBC, TC=TC, Tchar=Tchar, delb=delb, MaxEtaInit=hs / 2.0 * 0.9, Tstar=Tchar * tp,
+                Rsearch=2 * math.pi / kp / 4.0, char=char)
+
+
 # ------------------------------------------------------- model bundle --
 @dataclass
 class Model2D:
@@ -485,6 +497,7 @@ class Model2D:
     skew_sig: np.ndarray | None = None  # [nline, nt]
     eps_abs: float = 1e-3
     eps_rel: float = 1e-2
+    breaking: dict | None = None        # see breaking_params()
     meta: dict = field(default_factory=dict)
 
     @property

@@ -67,6 +67,18 @@ typedef struct hw_config_2d {
     const double *skew_sig;   /* [nline*nt] SkewSignal[j][n] */
     /* step control (control_y_new(eps_abs, eps_rel); 2d/main.c:196-197) */
     double eps_abs, eps_rel;
+    /* kinematic breaking (parambreak, 2d/breaking.c:1-21; gated by tprev < t,
+       2d/fun_odesolver.c:4580-4587) */
+    int breaking_on;
+    double KBC, TC, Tchar, delb;      /* initiation, termination (TC, Tchar), delta_b (1.5) */
+    double max_eta_init;              /* pb.MaxEtaInit = 0.9 Hs/2 */
+    double tstar;                     /* pb.Tstar = Tchar Tp */
+    double rsearch;                   /* pb.Rsearch = lambda_p / 4 */
+    double current;                   /* dominfo.U, |current| */
+    const double *pbchar;             /* pb.Char [ny*nx] (0/1) */
+    const double *depth;              /* dominfo.bathy.pos [ny*nx] (positive depth) */
+    const double *x, *y;              /* dominfo.x [nx], dominfo.y [ny] */
+    double t0;                        /* initial tprev (dominfo.t[0], 2d/main.c:120) */
 } hw_config_2d;
 
 typedef struct hw_stats {
@@ -97,7 +109,8 @@ int hw_get_state(void *ctx, double *y);
 
 /* gsl_odeiv2_evolve_apply(e, c, s, &sys, &t, t1, &h, y) with the rkf45 step
  * and control_y(eps_abs, eps_rel) (2d/fun_odesolver.c:2567-2570,2642):
- * one accepted step (final step clipped to t1), t and h updated. */
+ * one accepted step (final step clipped to t1), t and h updated; on success
+ * it also does the driver's tprev = t (2d/fun_odesolver.c:2646). */
 int hw_evolve_apply(void *ctx, double *t, double t1, double *h);
 
 /* The driver loop of ode_solver_uv: while (t < t1) evolve_apply
@@ -107,6 +120,14 @@ int hw_integrate(void *ctx, double *t, double t1, double *h);
 /* Physical eta, u, v [ny*nx] of the current state, as the output IFFTs of
  * ode_solver_uv (2d/fun_odesolver.c:2663-2666); any pointer may be NULL. */
 int hw_get_fields(void *ctx, double *eta, double *u, double *v);
+
+/* The reference's global tprev (breaking runs only when tprev < t). */
+int hw_set_tprev(void *ctx, double tprev);
+
+/* Breaking diagnostics: the eddy-viscosity field B [ny*nx] of the last call
+ * that ran breaking_process, and flagbr (2d/breaking.c:526-709); either
+ * pointer may be NULL. */
+int hw_get_breaking(void *ctx, double *B, int *flagbr);
 
 int hw_get_stats(void *ctx, hw_stats *st);
 int hw_reset_stats(void *ctx);

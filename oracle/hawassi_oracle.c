@@ -149,7 +149,22 @@ typedef struct {
     const double *newGam, *SkewGam;       /* [Ny][Nx] */
     int nline, nt;
     const double *tsig, *sig, *skew;      /* sig/skew: [nline][nt] */
+    /* breaking (2d/breaking.c, parambreak :1-43) */
+    int breaking_on;
+    double KBC, TC, Tchar, delb, MaxEtaInit, Tstar, Rsearch, U;
+    const double *pbchar;                 /* pb.Char [Ny][Nx] (0/1) */
+    const double *depth;                  /* dominfo.bathy.pos [Ny][Nx] (positive depth) */
+    const double *x, *y;                  /* dominfo.x [Nx], dominfo.y [Ny] */
 } orc_params;
+
+typedef struct { int x, y; } orc_v2i;
+typedef struct {
+    int xindex, yindex, xNow, yNow;
+    double Ucrest, theta, time;
+    int kwadran, NB;
+    orc_v2i *nodes, *nodes_prev;
+    int nnode, nnode_prev, cap, cap_prev;
+} orc_crest;
 
 typedef struct {
     orc_params p;
@@ -159,6 +174,15 @@ typedef struct {
     /* RKF45 work */
     double *k[6], *ytmp, *y0, *yerr, *dydt_in, *dydt_out;
     size_t dim;
+    /* breaking state (globals of the reference) */
+    double tprev;
+    double *B, *Sbx, *Sby;                /* B [N]; Sb_xhat, Sb_yhat full complex [2N] */
+    int flagbr;
+    orc_crest *CB, *CBNew, *CBPrev;
+    int ncap;
+    orc_v2i *bn, *bnp;                    /* Break_nodes, Break_nodes_prev */
+    int nbreak_nodes, nbreak_prev;
+    long nbreak_calls;
 } orc_ctx;
 
 /* natural cubic spline: c_i = M_i/2 (GSL cspline convention) */
@@ -222,7 +246,29 @@ void *orc_create(const orc_params *prm)
         }
     }
     c->dim = (size_t)6 * prm->Nx * prm->Ny;
+    size_t N = (size_t)prm->Nx * prm->Ny;
+    c->B = (double *)calloc(N, sizeof(double));
+    c->Sbx = (double *)calloc(2 * N, sizeof(double));
+    c->Sby = (double *)calloc(2 * N, sizeof(double));
+    c->tprev = 0.0;
+    if (prm->breaking_on) {
+        c->ncap = 4096;
+        c->CB = (orc_crest *)calloc(c->ncap, sizeof(orc_crest));
+        c->CBNew = (orc_crest *)calloc(c->ncap, sizeof(orc_crest));
+        c->CBPrev = (orc_crest *)calloc(c->ncap, sizeof(orc_crest));
+        c->bn = (orc_v2i *)malloc(sizeof(orc_v2i) * N);
+        c->bnp = (orc_v2i *)malloc(sizeof(orc_v2i) * N);
+    }
     return c;
+}
+
+void orc_set_tprev(void *vc, double t) { ((orc_ctx *)vc)->tprev = t; }
+double orc_get_tprev(void *vc) { return ((orc_ctx *)vc)->tprev; }
+long orc_break_calls(void *vc) { return ((orc_ctx *)vc)->nbreak_calls; }
+int orc_flagbr(void *vc) { return ((orc_ctx *)vc)->flagbr; }
+void orc_get_B(void *vc, double *out) {
+    orc_ctx *c = (orc_ctx *)vc;
+    memcpy(out, c->B, sizeof(double) * (size_t)c->p.Nx * c->p.Ny);
 }
 
 void orc_destroy(void *vc)
@@ -233,6 +279,10 @@ void orc_destroy(void *vc)
     free(c->sig_c); free(c->skew_c);
     for (int i = 0; i < 6; i++) free(c->k[i]);
     free(c->ytmp); free(c->y0); free(c->yerr); free(c->dydt_in); free(c->dydt_out);
+    free(c->B); free(c->Sbx); free(c->Sby);
+    if (c->CBPrev)
+        for (int i = 0; i < c->ncap; i++) { free(c->CBPrev[i].nodes); free(c->CBPrev[i].nodes_prev); }
+    free(c->CB); free(c->CBNew); free(c->CBPrev); free(c->bn); free(c->bnp);
     free(c);
 }
 
@@ -325,6 +375,308 @@ static void source_hat(orc_ctx *c, double t, double *Shat)
     #pragma omp parallel for schedule(static)
     for (size_t q = 0; q < 2 * N; q++) Shat[q] = S[q] + Sk[q];
     free(S); free(Sk); free(ins); free(insk);
+}
+
+/* ------------------------------------------------------------ breaking -- */
+/* Kinematic breaking criterion and crest tracking (2d/breaking.c) and the
+ * eddy-viscosity source fun_sbreaking (2d/fun_odesolver.c:985-1019).
+ * Deviation: the reference gives every tracked crest fixed 100-entry node
+ * arrays (2d/breaking.c:29-30) and overruns them when a crest has more break
+ * nodes; here the node lists grow as needed. */
+static int kwadran(double ux, double uy)
+{
+    if (ux >= 0 && uy >= 0) return 1;
+    if (ux < 0 && uy >= 0) return 2;
+    if (ux < 0 && uy < 0) return 3;
+    return 4;
+}
+
+static double cp1d(double k, double D, double U)
+{
+    if (k == 0) return sqrt(ORC_GRAV * D) + U;
+    double sg = k < 0 ? -1.0 : 1.0;
+    return (sg * sqrt(ORC_GRAV * k * tanh(k * D)) + k * U) / k;
+}
+
+/* fun_peaks (2d/breaking.c:55-133) on EC [Ny][Nx] */
+static int find_peaks(const double *EC, int Nx, int Ny, double thr, orc_v2i *pk)
+{
+#define E(yy, xx) EC[(size_t)(yy) * Nx + (xx)]
+    int n = 0;
+    for (int row = 1; row < Nx - 1; ++row)
+        for (int col = 1; col < Ny - 1; ++col) {
+            int cnt = 0;
+            for (int i = -1; i < 2; ++i)
+                for (int j = -1; j < 2; ++j) cnt += (E(col + i, row + j) - E(col, row) <= 0.001) ? 1 : 0;
+            if (cnt == 9 && E(col, row) > thr) { pk[n].x = row; pk[n].y = col; n++; }
+        }
+    for (int col = 1; col < Ny - 1; ++col) {           /* x = 0 */
+        int row = 0, cnt = 0;
+        for (int i = -1; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) cnt += (E(col + i, row + j) - E(col, row) <= 0.001) ? 1 : 0;
+        if (cnt == 6 && E(col, row) > thr) { pk[n].x = row; pk[n].y = col; n++; }
+    }
+    for (int col = 1; col < Ny - 1; ++col) {           /* x = Nx-1 */
+        int row = Nx - 1, cnt = 0;
+        for (int i = -1; i < 2; ++i)
+            for (int j = -1; j < 1; ++j) cnt += (E(col + i, row + j) - E(col, row) <= 0.001) ? 1 : 0;
+        if (cnt == 6 && E(col, row) > thr) { pk[n].x = row; pk[n].y = col; n++; }
+    }
+    for (int row = 1; row < Nx - 1; ++row) {           /* y = 0 */
+        int col = 0, cnt = 0;
+        for (int i = 0; i < 2; ++i)
+            for (int j = -1; j < 2; ++j) cnt += (E(col + i, row + j) - E(col, row) <= 0.001) ? 1 : 0;
+        if (cnt == 6 && E(col, row) > thr) { pk[n].x = row; pk[n].y = col; n++; }
+    }
+    for (int row = 1; row < Nx - 1; ++row) {           /* y = Ny-1 */
+        int col = Ny - 1, cnt = 0;
+        for (int i = -1; i < 1; ++i)
+            for (int j = -1; j < 2; ++j) cnt += (E(col + i, row + j) - E(col, row) <= 0.001) ? 1 : 0;
+        if (cnt == 6 && E(col, row) > thr) { pk[n].x = row; pk[n].y = col; n++; }
+    }
+#undef E
+    return n;
+}
+
+/* funBr_Char (2d/breaking.c:359-440): disk of radius Rsearch around the
+ * crest, cut to an 80-degree sector by the crest's quadrant */
+static void crest_char(const orc_ctx *c, const orc_crest *cr, double *Ch)
+{
+    const orc_params *p = &c->p;
+    int Nx = p->Nx, Ny = p->Ny;
+    double xp = p->x[cr->xNow], yp = p->y[cr->yNow];
+    double th = cr->theta, sp = 80.0 * M_PI / 180.0, al, be;
+    int kw = cr->kwadran;
+    if (kw == 1 || kw == 3) { al = th - sp; be = (sp + th) - M_PI / 2.0; }
+    else { al = sp - th; be = (M_PI / 2.0 - th) - sp; }
+    for (int i = 0; i < Ny; i++)
+        for (int j = 0; j < Nx; j++) {
+            double xc = p->x[j] - xp, yc = p->y[i] - yp;
+            double v = (xc * xc + yc * yc) <= p->Rsearch * p->Rsearch ? 1.0 : 0.0;
+            double XX = xc * cos(al) + yc * sin(al);
+            double YY = yc * cos(be) - xc * sin(be);
+            if (kw == 1 && (XX < 0 || YY < 0)) v = 0;
+            if (kw == 2 && (XX > 0 || YY < 0)) v = 0;
+            if (kw == 3 && (XX > 0 || YY > 0)) v = 0;
+            if (kw == 4 && (XX < 0 || YY > 0)) v = 0;
+            Ch[(size_t)i * Nx + j] = v;
+        }
+}
+
+static void push_node(orc_v2i **arr, int *n, int *cap, int x, int y)
+{
+    if (*n >= *cap) { *cap = *cap ? 2 * *cap : 128; *arr = (orc_v2i *)realloc(*arr, sizeof(orc_v2i) * *cap); }
+    (*arr)[*n].x = x; (*arr)[*n].y = y; (*n)++;
+}
+
+static void copy_crest_fields(orc_crest *d, const orc_crest *s)
+{
+    d->xindex = s->xindex; d->yindex = s->yindex; d->xNow = s->xNow; d->yNow = s->yNow;
+    d->Ucrest = s->Ucrest; d->theta = s->theta; d->time = s->time; d->NB = s->NB; d->kwadran = s->kwadran;
+}
+
+/* breaking_process (2d/breaking.c:505-733) + breaking_criterion (:216-334).
+ * eta, u, v: physical fields (already divided by N). */
+static void breaking_process(orc_ctx *c, double t, const double *Z, const double *eta, const double *u, const double *v)
+{
+    const orc_params *p = &c->p;
+    int Nx = p->Nx, Ny = p->Ny;
+    size_t N = (size_t)Nx * Ny;
+    c->nbreak_calls++;
+    double *U = (double *)malloc(sizeof(double) * N);
+    for (size_t q = 0; q < N; q++) U[q] = sqrt(u[q] * u[q] + v[q] * v[q]);
+    memset(c->B, 0, sizeof(double) * N);
+    /* ---- criterion ---- */
+    int ncb = 0;
+    double *EC = (double *)malloc(sizeof(double) * N);
+    double mx = -1e300;
+    for (size_t q = 0; q < N; q++) { EC[q] = eta[q] * p->pbchar[q]; if (EC[q] > mx) mx = EC[q]; }
+    if (mx > p->MaxEtaInit) {
+        orc_v2i *pk = (orc_v2i *)malloc(sizeof(orc_v2i) * N);
+        int npk = find_peaks(EC, Nx, Ny, p->MaxEtaInit, pk);
+        if (npk > 0) {
+            double *H = cal(N);
+            for (int i = 0; i < Ny; i++)
+                for (int j = 0; j < Nx; j++) {
+                    size_t q = (size_t)i * Nx + j;
+                    double sg = (p->kx[j] + p->ky[i]) < 0 ? -1.0 : 1.0;
+                    H[2 * q] = Z[N + q] * sg;
+                    H[2 * q + 1] = -Z[q] * sg;
+                }
+            orc_hermitian_project(H, Nx, Ny);
+            fft2(&c->P, H, +1);
+            double *Hr = (double *)malloc(sizeof(double) * N);
+            for (size_t q = 0; q < N; q++) Hr[q] = H[2 * q] / (double)N;
+            double dx = p->x[1] - p->x[0], dy = p->y[1] - p->y[0];
+            for (int jp = 0; jp < npk; jp++) {
+                int ix = pk[jp].x, iy = pk[jp].y;
+                /* gradient2D (2d/functions1.c:868-885) at (iy, ix) */
+#define GX(A) (ix == 0 ? (A[(size_t)iy * Nx + 1] - A[(size_t)iy * Nx]) / dx : ix == Nx - 1 ? (A[(size_t)iy * Nx + Nx - 1] - A[(size_t)iy * Nx + Nx - 2]) / dx : (A[(size_t)iy * Nx + ix + 1] - A[(size_t)iy * Nx + ix - 1]) / (2 * dx))
+#define GY(A) (iy == 0 ? (A[(size_t)Nx + ix] - A[ix]) / dy : iy == Ny - 1 ? (A[(size_t)(Ny - 1) * Nx + ix] - A[(size_t)(Ny - 2) * Nx + ix]) / dy : (A[(size_t)(iy + 1) * Nx + ix] - A[(size_t)(iy - 1) * Nx + ix]) / (2 * dy))
+                size_t q = (size_t)iy * Nx + ix;
+                double e = eta[q], h = Hr[q];
+                double gxh = GX(Hr), gyh = GY(Hr), gxe = GX(eta), gye = GY(eta);
+#undef GX
+#undef GY
+                double Kx = (e * gxh - h * gxe) / (e * e + h * h);
+                double Ky = (e * gyh - h * gye) / (e * e + h * h);
+                double Kk = sqrt(Kx * Kx + Ky * Ky);
+                double Cc = cp1d(Kk, p->depth[q] + e, p->U);
+                if (fabs(U[q]) >= p->KBC * fabs(Cc)) {
+                    orc_crest *cr = &c->CB[ncb++];
+                    cr->xindex = cr->xNow = ix;
+                    cr->yindex = cr->yNow = iy;
+                    cr->Ucrest = fabs(U[q]);
+                    cr->time = t;
+                    cr->theta = fabs(atan(v[q] / u[q]));
+                    cr->kwadran = kwadran(u[q], v[q]);
+                }
+            }
+            free(H); free(Hr);
+        }
+        free(pk);
+    }
+    free(EC);
+    /* ---- state machine ---- */
+    int nnew = 0;
+    int nprev = c->CBPrev[0].NB;
+    if (ncb > 0 && c->flagbr == 0) {
+        c->flagbr = 1;
+        for (int j = 0; j < ncb; j++) copy_crest_fields(&c->CBNew[j], &c->CB[j]);
+        nnew = ncb;
+    } else if (ncb == 0 && c->flagbr == 2) {
+        for (int j = 0; j < nprev; j++) copy_crest_fields(&c->CBNew[j], &c->CBPrev[j]);
+        nnew = nprev;
+    } else if (ncb > 0 && c->flagbr == 2) {
+        for (int j = 0; j < nprev; j++) copy_crest_fields(&c->CBNew[j], &c->CBPrev[j]);
+        int add = 0;
+        for (int k = 0; k < ncb; k++) {
+            int idx = c->CB[k].xindex, idy = c->CB[k].yindex, isnew = 1;
+            for (int ll = 0; ll < nprev && isnew; ll++)
+                if (c->CBPrev[ll].xindex == idx && c->CBPrev[ll].yindex == idy) isnew = 0;
+            for (int ll = 0; ll < nprev && isnew; ll++)
+                for (int nn = 0; nn < c->CBPrev[ll].nnode; nn++)
+                    if (c->CBPrev[ll].nodes[nn].x == idx && c->CBPrev[ll].nodes[nn].y == idy) { isnew = 0; break; }
+            if (isnew) { copy_crest_fields(&c->CBNew[nprev + add], &c->CB[k]); add++; }
+        }
+        nnew = nprev + add;
+    }
+    if (c->flagbr == 1 || c->flagbr == 2) {
+        c->CBPrev[0].NB = 0;
+        int *term = (int *)calloc(nnew > 0 ? nnew : 1, sizeof(int));
+        int itp = 0, itpp = 0;
+        double *Ch = (double *)malloc(sizeof(double) * N);
+        for (int lx = 0; lx < nnew; lx++) {
+            orc_crest *cr = &c->CBNew[lx];
+            term[lx] = 0;
+            if (c->flagbr == 1) {
+                cr->xNow = cr->xindex; cr->yNow = cr->yindex;
+            } else {
+                crest_char(c, cr, Ch);
+                /* funBr_find_peak_loc + idx_max_matrix (2d/breaking.c:442-454, functions1.c:248-263) */
+                double temp = eta[0] * Ch[0] * p->pbchar[0];
+                int bx = -1, by = -1;
+                for (int i = 0; i < Ny; i++)
+                    for (int j = 0; j < Nx; j++) {
+                        size_t q = (size_t)i * Nx + j;
+                        double vv = eta[q] * Ch[q] * p->pbchar[q];
+                        if (vv > temp) { temp = vv; bx = j; by = i; }
+                    }
+                if (bx < 0) { term[lx] = 1; continue; }
+                cr->xNow = bx; cr->yNow = by;
+                size_t q = (size_t)by * Nx + bx;
+                cr->theta = fabs(atan(v[q] / u[q]));
+                cr->kwadran = kwadran(u[q], v[q]);
+            }
+            double tb = cr->time, UI = cr->Ucrest, UF = p->TC * UI, Us;
+            if ((t - tb) >= p->Tstar) Us = UF;
+            else if ((t - tb) >= 0 && (t - tb) < p->Tstar) Us = UI + ((t - tb) / p->Tstar) * (UF - UI);
+            else Us = UI;
+            /* funBr_BreakNodes_char (2d/breaking.c:456-503) */
+            crest_char(c, cr, Ch);
+            int nb = 0, nbp = 0;
+            for (int i = 0; i < Ny; i++)
+                for (int j = 0; j < Nx; j++) {
+                    size_t q = (size_t)i * Nx + j;
+                    double ch = Ch[q];
+                    if (U[q] < UF || eta[q] < 0) ch = 0;
+                    ch *= p->pbchar[q];
+                    if (ch == 1) {
+                        c->bnp[nbp].x = j; c->bnp[nbp].y = i; nbp++;
+                        if (c->B[q] == 0) {
+                            if (U[q] >= 2 * Us) c->B[q] = 1;
+                            else if (U[q] <= Us) c->B[q] = 0;
+                            else c->B[q] = U[q] / Us - 1;
+                            if (c->B[q] > 0) { c->bn[nb].x = j; c->bn[nb].y = i; nb++; }
+                        }
+                    }
+                }
+            if (nbp > 0) {
+                orc_crest *d = &c->CBPrev[itpp];
+                d->nnode_prev = 0;
+                for (int jj = 0; jj < nbp; jj++) push_node(&d->nodes_prev, &d->nnode_prev, &d->cap_prev, c->bnp[jj].x, c->bnp[jj].y);
+                itpp++;
+            }
+            if (nb == 0) term[lx] = 1;
+            else {
+                term[lx] = 0;
+                orc_crest *d = &c->CBPrev[itp];
+                d->nnode = 0;
+                for (int jj = 0; jj < nb; jj++) push_node(&d->nodes, &d->nnode, &d->cap, c->bn[jj].x, c->bn[jj].y);
+                d->xindex = cr->xindex; d->yindex = cr->yindex; d->xNow = cr->xNow; d->yNow = cr->yNow;
+                d->Ucrest = cr->Ucrest; d->time = cr->time; d->theta = cr->theta; d->kwadran = cr->kwadran;
+                itp++;
+            }
+            c->CBPrev[0].NB = itp;
+        }
+        for (int lx = 0; lx < nnew; lx++) {
+            if (term[lx] == 0) { c->flagbr = 2; break; }
+            else c->flagbr = 0;
+        }
+        free(term); free(Ch);
+    }
+    free(U);
+}
+
+/* fun_sbreaking (2d/fun_odesolver.c:985-1019): projects delphiH in place */
+static void sbreaking(orc_ctx *c, const double *eta, double *delphiH)
+{
+    const orc_params *p = &c->p;
+    int Nx = p->Nx, Ny = p->Ny;
+    size_t N = (size_t)Nx * Ny;
+    orc_hermitian_project(delphiH, Nx, Ny);
+    double *d = cal(N);
+    memcpy(d, delphiH, sizeof(double) * 2 * N);
+    fft2(&c->P, d, +1);
+    double *F = (double *)malloc(sizeof(double) * N);
+    double db2 = p->delb * p->delb;
+    for (size_t q = 0; q < N; q++) {
+        double dp = d[2 * q] / (double)N;
+        F[q] = -c->B[q] * db2 * (eta[q] + p->depth[q]) * dp * dp;
+    }
+    double dx = p->x[1] - p->x[0], dy = p->y[1] - p->y[0];
+    double *sx = cal(N), *sy = cal(N);
+    for (int i = 0; i < Ny; i++)
+        for (int j = 0; j < Nx; j++) {
+            size_t q = (size_t)i * Nx + j;
+            /* fun_gradient (2d/functions.c:392-461): clipped periodic one-sided */
+            double fx = (F[(size_t)i * Nx + (j + 1) % Nx] - F[q]) / dx;
+            double fy = (F[(size_t)((i + 1) % Ny) * Nx + j] - F[q]) / dy;
+            double bx = (F[q] - F[(size_t)i * Nx + (j + Nx - 1) % Nx]) / dx;
+            double by = (F[q] - F[(size_t)((i + Ny - 1) % Ny) * Nx + j]) / dy;
+            if (fx > 0) fx = 0;
+            if (fy > 0) fy = 0;
+            if (bx < 0) bx = 0;
+            if (by < 0) by = 0;
+            double H = eta[q] + p->depth[q];
+            sx[2 * q] = (fx + bx) / H;
+            sy[2 * q] = (fy + by) / H;
+        }
+    fft2(&c->P, sx, -1);
+    fft2(&c->P, sy, -1);
+    memcpy(c->Sbx, sx, sizeof(double) * 2 * N);
+    memcpy(c->Sby, sy, sizeof(double) * 2 * N);
+    free(d); free(F); free(sx); free(sy);
 }
 
 /* ------------------------------------------------------------------ RHS -- */
@@ -559,26 +911,42 @@ int orc_rhs(void *vc, double t, const double *Z, double *dZ)
     /* ---- source + epilogue (2d/fun_odesolver.c:4479-4529 / 4564-4613) ---- */
     double *Shat = cal(N);
     source_hat(c, t, Shat);
-    #pragma omp parallel for schedule(static)
+    double *dph = cal(N);
+#pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < N; q++) {
+        if (p->flat) {
+            dph[2 * q] = delphiH1[2 * q] + duH2_adj[2 * q] + dvH2_adj[2 * q];
+            dph[2 * q + 1] = delphiH1[2 * q + 1] + duH2_adj[2 * q + 1] + dvH2_adj[2 * q + 1];
+        } else {
+            dph[2 * q] = duH1[2 * q] + dvH1[2 * q] + duH2_adj[2 * q] + dvH2_adj[2 * q];
+            dph[2 * q + 1] = duH1[2 * q + 1] + dvH1[2 * q + 1] + duH2_adj[2 * q + 1] + dvH2_adj[2 * q + 1];
+        }
+    }
+    if (p->breaking_on) {
+        memset(c->B, 0, sizeof(double) * N);
+        if (c->tprev < t) {
+            double *ep = (double *)malloc(sizeof(double) * N), *up = (double *)malloc(sizeof(double) * N),
+                   *vp = (double *)malloc(sizeof(double) * N);
+            for (size_t q = 0; q < N; q++) { ep[q] = eta[2 * q] / (double)N; up[q] = u[2 * q] / (double)N; vp[q] = v[2 * q] / (double)N; }
+            breaking_process(c, t, Z, ep, up, vp);
+            sbreaking(c, ep, dph);
+            free(ep); free(up); free(vp);
+        }
+    }
+    const double *Sbx = c->Sbx, *Sby = c->Sby;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < Ny; i++)
         for (int j = 0; j < Nx; j++) {
             size_t q = (size_t)i * Nx + j;
-            double dphr, dphi;
-            if (p->flat) {
-                dphr = delphiH1[2 * q] + duH2_adj[2 * q] + dvH2_adj[2 * q];
-                dphi = delphiH1[2 * q + 1] + duH2_adj[2 * q + 1] + dvH2_adj[2 * q + 1];
-            } else {
-                dphr = duH1[2 * q] + dvH1[2 * q] + duH2_adj[2 * q] + dvH2_adj[2 * q];
-                dphi = duH1[2 * q + 1] + dvH1[2 * q + 1] + duH2_adj[2 * q + 1] + dvH2_adj[2 * q + 1];
-            }
             double a = p->aal[q];
-            dZ[q] = (Shat[2 * q] + dphr - dampeta[2 * q]) * a;
-            dZ[N + q] = (Shat[2 * q + 1] + dphi - dampeta[2 * q + 1]) * a;
-            dZ[2 * N + q] = (g * kx[j] * Z[N + q] - ikxE_adj[2 * q] - dampu[2 * q]) * a;
-            dZ[3 * N + q] = (-g * kx[j] * Z[q] - ikxE_adj[2 * q + 1] - dampu[2 * q + 1]) * a;
-            dZ[4 * N + q] = (g * ky[i] * Z[N + q] - ikyE_adj[2 * q] - dampv[2 * q]) * a;
-            dZ[5 * N + q] = (-g * ky[i] * Z[q] - ikyE_adj[2 * q + 1] - dampv[2 * q + 1]) * a;
+            dZ[q] = (Shat[2 * q] + dph[2 * q] - dampeta[2 * q]) * a;
+            dZ[N + q] = (Shat[2 * q + 1] + dph[2 * q + 1] - dampeta[2 * q + 1]) * a;
+            dZ[2 * N + q] = (g * kx[j] * Z[N + q] + Sbx[2 * q] - ikxE_adj[2 * q] - dampu[2 * q]) * a;
+            dZ[3 * N + q] = (-g * kx[j] * Z[q] + Sbx[2 * q + 1] - ikxE_adj[2 * q + 1] - dampu[2 * q + 1]) * a;
+            dZ[4 * N + q] = (g * ky[i] * Z[N + q] + Sby[2 * q] - ikyE_adj[2 * q] - dampv[2 * q]) * a;
+            dZ[5 * N + q] = (-g * ky[i] * Z[q] + Sby[2 * q + 1] - ikyE_adj[2 * q + 1] - dampv[2 * q + 1]) * a;
         }
+    free(dph);
 
     free(eta); free(u); free(v); free(dampeta); free(dampu); free(dampv);
     free(ikxE_adj); free(ikyE_adj); free(duH2_adj); free(dvH2_adj); free(delphiH1);
@@ -737,9 +1105,11 @@ int orc_evolve_apply(void *vc, double *t, double t1, double *h, double *y, doubl
 /* the driver loop of ode_solver_uv (2d/fun_odesolver.c:2641-2647) to t1 */
 int orc_integrate(void *vc, double *t, double t1, double *h, double *y, double eps_abs, double eps_rel, long *stats)
 {
+    orc_ctx *c = (orc_ctx *)vc;
     while (*t < t1) {
         int s = orc_evolve_apply(vc, t, t1, h, y, eps_abs, eps_rel, stats);
         if (s) return s;
+        c->tprev = *t;
     }
     return 0;
 }

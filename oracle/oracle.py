@@ -30,6 +30,10 @@ class OrcParams(ct.Structure):
         ("newGam", _dp), ("SkewGam", _dp),
         ("nline", ct.c_int), ("nt", ct.c_int),
         ("tsig", _dp), ("sig", _dp), ("skew", _dp),
+        ("breaking_on", ct.c_int),
+        ("KBC", ct.c_double), ("TC", ct.c_double), ("Tchar", ct.c_double), ("delb", ct.c_double),
+        ("MaxEtaInit", ct.c_double), ("Tstar", ct.c_double), ("Rsearch", ct.c_double), ("U", ct.c_double),
+        ("pbchar", _dp), ("depth", _dp), ("x", _dp), ("y", _dp),
     ]
 
 
@@ -62,6 +66,13 @@ def lib():
         L.orc_ifft_c2r.argtypes = [ct.c_int, ct.c_int, _dp, _dp, _dp]
         L.orc_spline_coeffs.argtypes = [_dp, _dp, ct.c_int, _dp]
         L.orc_spline_index.argtypes = [_dp, ct.c_int, ct.c_double]
+        L.orc_set_tprev.argtypes = [ct.c_void_p, ct.c_double]
+        L.orc_get_tprev.argtypes = [ct.c_void_p]
+        L.orc_get_tprev.restype = ct.c_double
+        L.orc_break_calls.argtypes = [ct.c_void_p]
+        L.orc_break_calls.restype = ct.c_long
+        L.orc_flagbr.argtypes = [ct.c_void_p]
+        L.orc_get_B.argtypes = [ct.c_void_p, _dp]
         L.orc_num_threads.restype = ct.c_int
         L.orc_set_threads.argtypes = [ct.c_int]
         _lib = L
@@ -106,6 +117,14 @@ class Oracle2D:
             p.newGam, p.SkewGam = arr("ng", model.new_gam), arr("sg", model.skew_gam)
             p.nline, p.nt = model.sig.shape
             p.tsig, p.sig, p.skew = arr("ts", model.t_sig), arr("sig", model.sig), arr("sk", model.skew_sig)
+        br = getattr(model, "breaking", None)
+        if br is not None:
+            p.breaking_on = 1
+            p.KBC, p.TC, p.Tchar, p.delb = br["KBC"], br["TC"], br["Tchar"], br["delb"]
+            p.MaxEtaInit, p.Tstar, p.Rsearch, p.U = br["MaxEtaInit"], br["Tstar"], br["Rsearch"], 0.0
+            p.pbchar = arr("pbchar", br["char"])
+            p.depth = arr("depth", model.depth)
+            p.x, p.y = arr("x", model.x), arr("y", model.y)
         self._keep = keep
         self._p = p
         self.ctx = L.orc_create(ct.byref(p))
@@ -138,6 +157,23 @@ class Oracle2D:
 
     def rhs_count(self):
         return lib().orc_rhs_count(self.ctx)
+
+    def set_tprev(self, t):
+        lib().orc_set_tprev(self.ctx, float(t))
+
+    def tprev(self):
+        return lib().orc_get_tprev(self.ctx)
+
+    def break_calls(self):
+        return lib().orc_break_calls(self.ctx)
+
+    def flagbr(self):
+        return lib().orc_flagbr(self.ctx)
+
+    def B(self):
+        out = np.empty(self.model.nx * self.model.ny)
+        lib().orc_get_B(self.ctx, _p(out))
+        return out.reshape(self.model.ny, self.model.nx)
 
 
 def hermitian_project(a: np.ndarray) -> np.ndarray:
