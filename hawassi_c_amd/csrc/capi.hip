@@ -151,6 +151,19 @@ int hw_get_fields(void* ctx, double* eta, double* u, double* v) {
     return HW_OK;
 }
 
+int hw_set_tprev(void* ctx, double tprev) {
+    Solver2D* s = static_cast<Solver2D*>(ctx);
+    if (!s) { set_error("null argument"); return HW_ERR_ARG; }
+    s->tprev = tprev;
+    return HW_OK;
+}
+
+int hw_get_breaking(void* ctx, double* B, int* flagbr) {
+    Solver2D* s = static_cast<Solver2D*>(ctx);
+    if (!s) { set_error("null argument"); return HW_ERR_ARG; }
+    return s->breaking_state(B, flagbr);
+}
+
 int hw_get_stats(void* ctx, hw_stats* st) {
     Solver2D* s = static_cast<Solver2D*>(ctx);
     if (!s || !st) { set_error("null argument"); return HW_ERR_ARG; }

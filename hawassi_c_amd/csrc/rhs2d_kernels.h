@@ -50,6 +50,8 @@ template <typename T> struct Rhs2DArgs {
     C *k_eta, *k_u, *k_v;
     const C *twx, *twy;
     T g;
+    T* s_eta_out;   // per-row imaginary Nyquist part of the eta adjust (read by breaking)
+    C* eta_col0;    // when set: the pre-mask eta derivative of column kx = 0, [ky]
 };
 
 template <typename T, int NX, int NY> struct Geo2 {
@@ -513,7 +515,9 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y6(Rhs2DArgs<T> a) {
             if (kx < NX / 2 + 1) {
                 T kk = c == 0 ? a.kxh[kx] : a.ky[ky];
                 C lin = cmul_i<-1>(cscale(eh[e], a.g * kk));
-                Z[e] = cscale(cadd(lin, Z[e]), a.aal[gi]);
+                C pre = cadd(lin, Z[e]);
+                if (c == 0 && a.eta_col0 && kx == 0) a.eta_col0[ky] = pre;
+                Z[e] = cscale(pre, a.aal[gi]);
             }
         }
         y_store<T, NX, NY>(c == 0 ? a.k_u : a.k_v, Z, col0, tid);
@@ -553,6 +557,7 @@ __device__ __forceinline__ void x_eta_out(const Rhs2DArgs<T>& a, int sa, int so,
     const T sc = (T)1 / ((T)NX * (T)NY);
     C f1[G::PP], f2[G::PP], t[G::PP], t2[G::PP];
     x_inverse<T, NX, NY>(a.W[sa], y0, lds, a.twx, f1, sc, nyq, tid);
+    if (a.s_eta_out && tid < G::RR) a.s_eta_out[y0 + tid] = nyq[tid];
     ph_load<T, NX, NY>(t, a.chi, y0, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) {
@@ -597,7 +602,11 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y8(Rhs2DArgs<T> a) {
     for (int e = 0; e < G::PE; ++e) {
         int kx, ky; size_t gi;
         YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
-        if (kx < NX / 2 + 1) Z[e] = cscale(cadd(d[e], Z[e]), a.aal[gi]);
+        if (kx < NX / 2 + 1) {
+            C pre = cadd(d[e], Z[e]);
+            if (a.eta_col0 && kx == 0) a.eta_col0[ky] = pre;
+            Z[e] = cscale(pre, a.aal[gi]);
+        }
     }
     y_store<T, NX, NY>(a.k_eta, Z, col0, tid);
 }
@@ -750,7 +759,9 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2f_y5(Rhs2DArgs<T> a) {
                     T kk = c == 1 ? a.kxh[kx] : a.ky[ky];
                     lin = cmul_i<-1>(cscale(eh[e], a.g * kk));
                 }
-                Z[e] = cscale(cadd(lin, Z[e]), a.aal[gi]);
+                C pre = cadd(lin, Z[e]);
+                if (c == 0 && a.eta_col0 && kx == 0) a.eta_col0[ky] = pre;
+                Z[e] = cscale(pre, a.aal[gi]);
             }
         }
         y_store<T, NX, NY>(outs[c], Z, col0, tid);

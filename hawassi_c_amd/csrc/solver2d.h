@@ -12,6 +12,9 @@
 
 #include "../../include/hawassi_hip.h"
 #include "aux_kernels.h"
+#include "breaking2d.h"
+#include <algorithm>
+#include <climits>
 
 namespace hw {
 
@@ -83,6 +86,8 @@ struct Solver2D {
     virtual int get_state_dev(double* d_full) = 0;
     virtual int evolve_apply(double* t, double t1, double* h) = 0;
     virtual int fields_dev(double* eta, double* u, double* v) = 0;
+    virtual int breaking_state(double* B, int* flagbr) = 0;
+    double tprev = 0.0;    // the reference's global tprev
     int nx = 0, ny = 0, prec = 0;
     hipStream_t stream = nullptr;
     hw_stats st{};
@@ -135,6 +140,26 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
     DevBuf<unsigned long long> rmax;
     unsigned long long* h_rmax = nullptr;
     Rhs2DArgs<T> args{};
+    // ---- breaking state
+    struct CrestRec {                      // CBvar without the node arrays
+        int xindex, yindex, xNow, yNow, kwadran;
+        double Ucrest, theta, time;
+    };
+    int brk_on = 0;
+    double KBC = 0, TC = 0, Tchar = 0, delb = 1.5, MaxEtaInit = 0, Tstar = 0, Rsearch = 0, Ucur = 0;
+    double dxg = 1, dyg = 1;
+    DevBuf<T> pbc, depthv, Bf, flux, sbx, sby, s_eta;
+    DevBuf<double> xg, yg;
+    DevBuf<int> owner, pkcount;
+    DevBuf<BrkPeak> peaks;
+    DevBuf<BrkCrit> crit;
+    DevBuf<BrkCrest> crests;
+    DevBuf<C> sbh;                         // Sb_x^, Sb_y^ half spectra (2H)
+    DevBuf<C> col0;                        // pre-mask eta derivative, column kx = 0 (NY)
+    int maxpk = 0, maxcr = 0;
+    int flagbr = 0;
+    std::vector<CrestRec> cbprev;          // CBPrev entries (count = size)
+    long brk_calls = 0;
     bool flat = false;
     int src_on = 0;
     double g = 9.80665, eps_abs = 1e-3, eps_rel = 1e-2;
@@ -243,6 +268,31 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
             if ((rc = src_sig.alloc(nline)) || (rc = src_skew.alloc(nline))) return rc;
             if ((rc = newgam.upload(to_phys(c->new_gam), stream)) || (rc = skewgam.upload(to_phys(c->skew_gam), stream))) return rc;
         }
+        tprev = c->t0;
+        brk_on = c->breaking_on;
+        if ((rc = s_eta.alloc(NY))) return rc;
+        if (brk_on) {
+            if (!c->pbchar || !c->depth || !c->x || !c->y || c->rsearch <= 0) {
+                set_error("breaking needs pbchar, depth, x, y and rsearch > 0");
+                return HW_ERR_ARG;
+            }
+            KBC = c->KBC; TC = c->TC; Tchar = c->Tchar; delb = c->delb; MaxEtaInit = c->max_eta_init;
+            Tstar = c->tstar; Rsearch = c->rsearch; Ucur = c->current;
+            dxg = c->x[1] - c->x[0]; dyg = c->y[1] - c->y[0];
+            std::vector<double> xv(c->x, c->x + NX), yv(c->y, c->y + NY);
+            if ((rc = pbc.upload(to_phys(c->pbchar), stream)) || (rc = depthv.upload(to_phys(c->depth), stream)) ||
+                (rc = xg.upload(xv, stream)) || (rc = yg.upload(yv, stream)))
+                return rc;
+            maxpk = (int)std::max<size_t>(4096, N / 8);
+            maxcr = 65536;
+            if ((rc = Bf.alloc(N)) || (rc = flux.alloc(N)) || (rc = sbx.alloc(N)) || (rc = sby.alloc(N)) ||
+                (rc = owner.alloc(N)) || (rc = pkcount.alloc(1)) || (rc = peaks.alloc(maxpk)) || (rc = crit.alloc(maxpk)) ||
+                (rc = crests.alloc(maxcr)) || (rc = sbh.alloc(2 * H)) || (rc = col0.alloc(NY)))
+                return rc;
+            k_fill<T><<<grid1(N), 256, 0, stream>>>(Bf.p, (T)0, N);
+            k_fill<C><<<grid1(2 * H), 256, 0, stream>>>(sbh.p, cmk_h((T)0, (T)0), 2 * H);
+            HW_LAUNCH_CHECK();
+        }
         if ((rc = eta_p.alloc(N)) || (rc = u_p.alloc(N)) || (rc = v_p.alloc(N))) return rc;
         if ((rc = work.alloc(18 * H))) return rc;
         if ((rc = y.alloc(3 * H)) || (rc = ynew.alloc(3 * H)) || (rc = kbuf.alloc(6 * 3 * H)) || (rc = ytmp.alloc(3 * H))) return rc;
@@ -263,9 +313,12 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         args.dH2p = work.p + 17 * H;
         args.twx = twx.p; args.twy = twy.p;
         args.g = (T)g;
+        args.s_eta_out = s_eta.p;
         HW_TRY(hipStreamSynchronize(stream));
         return HW_OK;
     }
+
+    static C cmk_h(T a, T b) { C r; r.x = a; r.y = b; return r; }
 
     static void spline_half_second(const double* x, const double* yv, int n, double* cc) {
         for (int i = 0; i < n; ++i) cc[i] = 0.0;
@@ -317,6 +370,9 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         }
         const dim3 gx(NY / XG::RR), bx(XG::NT);
         const dim3 gy((NXH + YG::CC - 1) / YG::CC), by(YG::NT);
+        // 2d/fun_odesolver.c:4580-4587: breaking runs only when tprev < t
+        const bool brk_now = brk_on && tprev < t;
+        args.eta_col0 = brk_now ? col0.p : nullptr;
         if (!flat) {
             k2b_y1<T, NX, NY><<<gy, by, 0, stream>>>(args);
             k2b_x1<T, NX, NY><<<gx, bx, 0, stream>>>(args);
@@ -335,9 +391,186 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
             k2f_y5<T, NX, NY><<<gy, by, 0, stream>>>(args);
         }
         HW_LAUNCH_CHECK();
+        if (brk_on) {
+            // the (possibly stale) Sb^ is added in every call
+            if (brk_now) {
+                int rc = breaking(t);
+                if (rc) return rc;
+                // fun_sbreaking projects delphiH in place (2d/fun_odesolver.c:
+                // 990, fftwcomplex_2d_sym), which the eta derivative keeps
+                k_eta_project<T><<<(NY + 255) / 256, 256, 0, stream>>>(args.k_eta, col0.p, aal.p, NX, NY);
+                HW_LAUNCH_CHECK();
+            }
+            k_add_sb<T><<<grid1(H), 256, 0, stream>>>(args.k_u, args.k_v, sbh.p, sbh.p + H, aal.p, H);
+            HW_LAUNCH_CHECK();
+        }
         HW_TRY(hipEventRecord(ev_b[nev], stream));
         nev++;
         st.rhs_evals++;
+        return HW_OK;
+    }
+
+    // ------------------------------------------------------- breaking --
+    // breaking_process (2d/breaking.c:505-733) + fun_sbreaking
+    // (2d/fun_odesolver.c:985-1019) for one RHS call at time t.
+    int breaking(double t) {
+        brk_calls++;
+        const dim3 gx(NY / XG::RR), bx(XG::NT);
+        const dim3 gy((NXH + YG::CC - 1) / YG::CC), by(YG::NT);
+        int npk = 0;
+        k_fill<int><<<1, 64, 0, stream>>>(pkcount.p, 0, 1);
+        k_brk_peaks<T><<<dim3((NX + 255) / 256, NY), 256, 0, stream>>>(eta_p.p, pbc.p, MaxEtaInit, NX, NY, peaks.p, pkcount.p, maxpk);
+        HW_LAUNCH_CHECK();
+        HW_TRY(hipMemcpyAsync(&npk, pkcount.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HW_TRY(hipStreamSynchronize(stream));
+        if (npk > maxpk) { set_error("breaking: %d peaks exceed the buffer of %d", npk, maxpk); return HW_ERR_ARG; }
+        // ---- criterion: candidate crests in the reference's peak order
+        std::vector<CrestRec> cb;
+        std::vector<int> cb_prevnode;
+        if (npk > 0) {
+            C* Xh = args.W[10];
+            T* Hp = reinterpret_cast<T*>(args.W[11]);
+            k_brk_hilb_y<T, NX, NY><<<gy, by, 0, stream>>>(args, Xh);
+            k_t_xinv<T, NX, NY><<<gx, bx, 0, stream>>>(Xh, Hp, nullptr, twx.p);
+            k_brk_crit<T><<<(npk + 127) / 128, 128, 0, stream>>>(peaks.p, npk, eta_p.p, Hp, u_p.p, v_p.p, depthv.p, Bf.p, dxg,
+                                                                  dyg, Ucur, KBC, NX, NY, crit.p);
+            HW_LAUNCH_CHECK();
+            std::vector<BrkPeak> hp(npk);
+            std::vector<BrkCrit> hc(npk);
+            HW_TRY(hipMemcpyAsync(hp.data(), peaks.p, sizeof(BrkPeak) * npk, hipMemcpyDeviceToHost, stream));
+            HW_TRY(hipMemcpyAsync(hc.data(), crit.p, sizeof(BrkCrit) * npk, hipMemcpyDeviceToHost, stream));
+            HW_TRY(hipStreamSynchronize(stream));
+            // fun_peaks order: interior x-major, then x=0, x=nx-1, y=0, y=ny-1 edges
+            auto key = [](const BrkPeak& p) -> long long {
+                int cls;
+                long long a, b;
+                if (p.x >= 1 && p.x <= NX - 2 && p.y >= 1 && p.y <= NY - 2) { cls = 0; a = p.x; b = p.y; }
+                else if (p.x == 0) { cls = 1; a = p.y; b = 0; }
+                else if (p.x == NX - 1) { cls = 2; a = p.y; b = 0; }
+                else if (p.y == 0) { cls = 3; a = p.x; b = 0; }
+                else { cls = 4; a = p.x; b = 0; }
+                return ((long long)cls << 40) | (a << 20) | b;
+            };
+            std::vector<int> ord(npk);
+            for (int i = 0; i < npk; ++i) ord[i] = i;
+            std::sort(ord.begin(), ord.end(), [&](int a, int b) { return key(hp[a]) < key(hp[b]); });
+            for (int i : ord) {
+                if (!hc[i].pass) continue;
+                CrestRec r;
+                r.xindex = r.xNow = hp[i].x;
+                r.yindex = r.yNow = hp[i].y;
+                r.Ucrest = hc[i].U;
+                r.time = t;
+                r.theta = hc[i].theta;
+                r.kwadran = hc[i].kwadran;
+                cb.push_back(r);
+                cb_prevnode.push_back(hc[i].prevnode);
+            }
+        }
+        // ---- crest list for this call (2d/breaking.c:526-593)
+        const int ncb = (int)cb.size();
+        std::vector<CrestRec> cbnew;
+        if (ncb > 0 && flagbr == 0) {
+            flagbr = 1;
+            cbnew = cb;
+        } else if (ncb == 0 && flagbr == 2) {
+            cbnew = cbprev;
+        } else if (ncb > 0 && flagbr == 2) {
+            cbnew = cbprev;
+            for (int k = 0; k < ncb; ++k) {
+                bool isnew = !cb_prevnode[k];   // fun_check_crest: on a previous break node?
+                for (const CrestRec& p : cbprev)
+                    if (p.xindex == cb[k].xindex && p.yindex == cb[k].yindex) isnew = false;
+                if (isnew) cbnew.push_back(cb[k]);
+            }
+        }
+        bool have_b = false;
+        if (flagbr == 1 || flagbr == 2) {
+            const int nnew = (int)cbnew.size();
+            std::vector<CrestRec> next;
+            std::vector<int> term(nnew, 0);
+            if (nnew > 0) {
+                if (nnew > maxcr) { set_error("breaking: %d crests exceed %d", nnew, maxcr); return HW_ERR_ARG; }
+                std::vector<BrkCrest> hcr(nnew);
+                for (int lx = 0; lx < nnew; ++lx) {
+                    const CrestRec& r = cbnew[lx];
+                    BrkCrest& b = hcr[lx];
+                    b.mode = flagbr == 1 ? 1 : 2;
+                    b.xNow = flagbr == 1 ? r.xindex : r.xNow;
+                    b.yNow = flagbr == 1 ? r.yindex : r.yNow;
+                    b.theta = r.theta;
+                    b.kwadran = r.kwadran;
+                    double UI = r.Ucrest, UF = TC * UI, dt = t - r.time, Us;
+                    if (dt >= Tstar) Us = UF;
+                    else if (dt >= 0 && dt < Tstar) Us = UI + (dt / Tstar) * (UF - UI);
+                    else Us = UI;
+                    b.Ustar = Us;
+                    b.UF = UF;
+                    b.found = 0;
+                    b.nb = 0;
+                }
+                k_fill<T><<<grid1(N), 256, 0, stream>>>(Bf.p, (T)0, N);
+                k_fill<int><<<grid1(N), 256, 0, stream>>>(owner.p, INT_MAX, N);
+                HW_TRY(hipMemcpyAsync(crests.p, hcr.data(), sizeof(BrkCrest) * nnew, hipMemcpyHostToDevice, stream));
+                BrkGeom geo{xg.p, yg.p, Rsearch, NX, NY};
+                k_brk_track<T><<<nnew, 256, 0, stream>>>(crests.p, nnew, geo, eta_p.p, u_p.p, v_p.p, pbc.p, owner.p);
+                k_brk_resolve<T><<<nnew, 256, 0, stream>>>(crests.p, nnew, geo, eta_p.p, u_p.p, v_p.p, pbc.p, owner.p, Bf.p);
+                HW_LAUNCH_CHECK();
+                HW_TRY(hipMemcpyAsync(hcr.data(), crests.p, sizeof(BrkCrest) * nnew, hipMemcpyDeviceToHost, stream));
+                HW_TRY(hipStreamSynchronize(stream));
+                for (int lx = 0; lx < nnew; ++lx) {
+                    const BrkCrest& b = hcr[lx];
+                    if (!b.found) { term[lx] = 1; continue; }
+                    CrestRec r = cbnew[lx];
+                    r.xNow = b.xNow; r.yNow = b.yNow; r.theta = b.theta; r.kwadran = b.kwadran;
+                    if (b.nb == 0) {
+                        term[lx] = 1;
+                    } else {
+                        term[lx] = 0;
+                        next.push_back(r);
+                        have_b = true;
+                    }
+                }
+                for (int lx = 0; lx < nnew; ++lx) {
+                    if (term[lx] == 0) { flagbr = 2; break; }
+                    flagbr = 0;
+                }
+            }
+            cbprev = next;
+        }
+        if (!have_b) {
+            // B == 0 everywhere: zero flux, zero source
+            k_fill<T><<<grid1(N), 256, 0, stream>>>(Bf.p, (T)0, N);
+            k_fill<C><<<grid1(2 * H), 256, 0, stream>>>(sbh.p, cmk_h((T)0, (T)0), 2 * H);
+            HW_LAUNCH_CHECK();
+            return HW_OK;
+        }
+        // ---- fun_sbreaking: delphi, flux, clipped gradient, transforms
+        C* Xg = args.W[12];
+        C* Xd = args.W[13];
+        k_brk_g_x<T, NX, NY><<<gx, bx, 0, stream>>>(args, s_eta.p, Xg);
+        k_brk_del_y<T, NX, NY><<<gy, by, 0, stream>>>(args, flat ? 1 : 0, Xg, Xd);
+        k_brk_flux_x<T, NX, NY><<<gx, bx, 0, stream>>>(args, flat ? (int)F_AETA : (int)S_AETA, Xd, Bf.p, depthv.p,
+                                                      (T)(delb * delb), flux.p);
+        k_brk_sb<T><<<dim3((NX + 255) / 256, NY), 256, 0, stream>>>(flux.p, eta_p.p, depthv.p, dxg, dyg, NX, NY, sbx.p, sby.p);
+        k_brk_sb_x<T, NX, NY><<<gx, bx, 0, stream>>>(args, sbx.p, sby.p, args.W[10], args.W[11]);
+        k_brk_sb_y<T, NX, NY><<<gy, by, 0, stream>>>(args, args.W[10], args.W[11], sbh.p, sbh.p + H);
+        HW_LAUNCH_CHECK();
+        return HW_OK;
+    }
+
+    int breaking_state(double* Bout, int* fl) override {
+        if (fl) *fl = flagbr;
+        if (Bout) {
+            if (!brk_on) {
+                for (size_t i = 0; i < N; ++i) Bout[i] = 0.0;
+            } else {
+                std::vector<T> hb(N);
+                HW_TRY(hipMemcpyAsync(hb.data(), Bf.p, sizeof(T) * N, hipMemcpyDeviceToHost, stream));
+                HW_TRY(hipStreamSynchronize(stream));
+                for (size_t i = 0; i < N; ++i) Bout[i] = hb[i];
+            }
+        }
         return HW_OK;
     }
 
@@ -421,6 +654,13 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
             k_rk_final<T><<<grid1(3 * H), 256, 0, stream>>>(y.p, K(0), K(2), K(3), K(4), K(5), h0, eps_abs, eps_rel, ynew.p,
                                                             rmax.p, 3 * H);
             HW_LAUNCH_CHECK();
+            if (brk_on) {
+                // rkf45's dydt_out = f(t + h, y_new): its breaking side effects
+                // are part of the reference's sequence (K(1) is free here)
+                k_convert<double2, C><<<grid1(3 * H), 256, 0, stream>>>(ynew.p, ytmp.p, 3 * H);
+                HW_LAUNCH_CHECK();
+                if ((rc = rhs(t0 + h0, ytmp.p, K(1)))) return rc;
+            }
             HW_TRY(hipMemcpyAsync(h_rmax, rmax.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
             HW_TRY(hipStreamSynchronize(stream));
             double rm;
@@ -443,6 +683,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
             std::swap(y.p, ynew.p);
             st.steps_accepted++;
             *h = h0;
+            tprev = *t;   // ode_solver_uv: tprev = t after each evolve_apply
             return collect_timing();
         }
     }

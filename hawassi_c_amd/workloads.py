@@ -93,7 +93,7 @@ def _source_shapes(x, y, kx, line_x, meandepth, Lb, Lt):
 
 def harbour(n: int = 2048, seed: int = 4, t_end: float = 60.0, dt_sig: float = 0.1, length: float = 1500.0,
             damp: float = 100.0, nadj: float = 1.0, eps_rel: float = 1e-2, eps_abs: float = 1e-3,
-            with_wall: bool = True) -> S.Model2D:
+            with_wall: bool = True, breaking: bool = True, kbc: float = 0.85) -> S.Model2D:
     x = S.interval_to_vector(0.0, length, n)
     y = S.interval_to_vector(0.0, length, n)
     dx, dy = x[1] - x[0], y[1] - y[0]
@@ -132,10 +132,11 @@ def harbour(n: int = 2048, seed: int = 4, t_end: float = 60.0, dt_sig: float = 0
     t_sig = np.arange(0.0, t_end + 2 * dt_sig, dt_sig)
     sig = directional_line_signal(y, t_sig, hs, tp, 3.3, 15.0, meandepth, seed + 1000)
     ng, sg = _source_shapes(x, y, kx, line_x, meandepth, damp, damp)
+    brk = S.breaking_params(hs, tp, kp, cfsa, chi, KBC=kbc, TC=0.2, Tchar=0.5) if breaking else None
     return S.Model2D(nx=n, ny=n, x=x, y=y, kx=kx, ky=ky, flat=False, cpeak=cpeak, aal=aal, fbdy=fbdy,
                      wallchar=wallchar, chi=chi, depth=depth, wall=with_wall, ops=ops, source_on=True,
                      vertical=True, propagation=2, new_gam=ng, skew_gam=sg, t_sig=t_sig, sig=sig,
-                     skew_sig=np.zeros_like(sig), eps_abs=eps_abs, eps_rel=eps_rel,
+                     skew_sig=np.zeros_like(sig), eps_abs=eps_abs, eps_rel=eps_rel, breaking=brk,
                      meta=dict(name=f"harbour{n}", tp=tp, hs=hs, kp=kp, lam=lam, line_x=line_x))
 
 

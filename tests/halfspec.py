@@ -44,9 +44,25 @@ def r2c(f: np.ndarray) -> np.ndarray:
     return np.fft.rfft2(f)
 
 
-def rhs_2db(m, t: float, Z: np.ndarray, src_vals=None) -> np.ndarray:
+def project_eta_half(pre: np.ndarray) -> np.ndarray:
+    """The stored-half effect of fun_sbreaking's in-place projection of
+    delphiH (2d/fun_odesolver.c:985-1019 via fftwcomplex_2d_sym), applied to
+    the pre-mask eta derivative.  The source and damping terms are transforms
+    of real fields, so they are already Hermitian along column 0 and real at
+    the four self-conjugate bins: projecting the sum equals projecting delphiH."""
+    ny, nh = pre.shape
+    out = pre.copy()
+    out[ny // 2 + 1:, 0] = np.conj(pre[1: ny // 2, 0][::-1])
+    for r in (0, ny // 2):
+        for c in (0, nh - 1):
+            out[r, c] = out[r, c].real
+    return out
+
+
+def rhs_2db(m, t: float, Z: np.ndarray, src_vals=None, project_eta: bool = False) -> np.ndarray:
     """Half-spectrum 2DB RHS; Z is the full 6-plane state; returns the three
-    derivative half spectra stacked [3, ny, nx/2+1]."""
+    derivative half spectra stacked [3, ny, nx/2+1].  project_eta: the call
+    ran breaking (tprev < t), which projects the eta derivative."""
     nx, ny = m.nx, m.ny
     N = nx * ny
     pl = Z.reshape(6, ny, nx)
@@ -101,7 +117,8 @@ def rhs_2db(m, t: float, Z: np.ndarray, src_vals=None) -> np.ndarray:
     Zeta = r2c(src - fb7 * eta + aeta) + 1j * r2c(seta)
     Zu = r2c(-ax - fb7 * u) - 1j * r2c(sx)
     Zv = r2c(-ay - fb7 * v) - 1j * r2c(sy)
-    de = aal * (duH1 + dvH1 + Zeta)
+    pre = duH1 + dvH1 + Zeta
+    de = aal * (project_eta_half(pre) if project_eta else pre)
     du = aal * (-1j * G * kx * eh + Zu)
     dv = aal * (-1j * G * ky * eh + Zv)
     return np.stack([de, du, dv])

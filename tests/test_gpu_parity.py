@@ -87,10 +87,15 @@ def test_trajectory_parity(prec, tol, eps, kind):
     The device integrates the stored half spectrum and takes its error norm
     there, while the oracle (like the reference) also includes the redundant
     half, so step sequences can differ; tolerances eps are tight enough that
-    both trajectories are within tol of each other regardless."""
+    both trajectories are within tol of each other regardless.
+
+    Breaking is off here: with it on, the first stage of every step is gated
+    (tprev == t) while the later stages project the eta derivative in
+    fun_sbreaking, a stage-to-stage inconsistency of relative size ~1e-4 that
+    drives eps = 1e-10 into vanishing steps (in the reference as well)."""
     say("trajectory", kind, prec)
     n = 64
-    m = getattr(W, kind)(n, t_end=8.0)
+    m = getattr(W, kind)(n, t_end=8.0, **({"breaking": False} if kind == "harbour" else {}))
     m.eps_abs = m.eps_rel = eps
     o = O.Oracle2D(m)
     y = np.zeros(6 * n * n)

@@ -167,12 +167,18 @@ def test_two_depth_with_unit_weights_equals_flat():
         assert relerr(db[0, n // 2 + 1:, 0], df[0, n // 2 + 1:, 0]) > 1e-8   # the asymmetry is real
 
 
-@pytest.mark.parametrize("n,kfrac", [(32, 1.0), (64, 0.4)])
-def test_half_spectrum_formulation_matches_oracle(n, kfrac):
-    m = W.harbour(n, t_end=5.0)
+@pytest.mark.parametrize("n,kfrac,brk", [(32, 1.0, False), (64, 0.4, False), (32, 1.0, True), (64, 0.4, True)])
+def test_half_spectrum_formulation_matches_oracle(n, kfrac, brk):
+    """With breaking on and tprev < t the oracle runs breaking_process (no
+    crest here: the state is far below MaxEtaInit, so B = 0 and Sb = 0) and
+    fun_sbreaking still projects the eta derivative in place."""
+    m = W.harbour(n, t_end=5.0, breaking=brk)
     y = W.random_state(m, 7, kfrac=kfrac)
-    ref = HS.oracle_half(O.Oracle2D(m).rhs(1.23, y), n, n)
-    got = HS.rhs_2db(m, 1.23, y)
+    o = O.Oracle2D(m)
+    o.set_tprev(0.0)
+    ref = HS.oracle_half(o.rhs(1.23, y), n, n)
+    assert (o.break_calls() > 0) == brk and o.flagbr() == 0
+    got = HS.rhs_2db(m, 1.23, y, project_eta=brk)
     for f in range(3):
         assert relerr(got[f], ref[f]) < 1e-13
 
