@@ -26,6 +26,11 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes actually moved per RHS evaluation of the 2048^2 harbour, FP32 path:
+# FETCH_SIZE and WRITE_SIZE counter passes over this script (FETCH_SIZE scaled
+# by 2.0, WRITE_SIZE by 1.0, the factors that make a known 512 MiB copy read
+# right), summed over the evaluation's kernels; profiles/rhs2d_harbour2048_fp32.md.
+RHS_TRAFFIC_BYTES_2048 = 2135.9e6
 METRIC = "sim-sec/wall-sec + RHS evals/s, 2D 2048^2 FP32 vs FP64 oracle at relL2<=1e-4"
 
 
@@ -64,15 +69,27 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="launch one elementwise copy kernel (torch.neg) over this many MiB of fp32 first, so a "
+                         "counter pass over this command holds a dispatch of known HBM traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the path would shard over independent realizations, see DESIGN.md)", file=sys.stderr)
+              "(the path would shard over independent sea-state realizations, one context per GPU)", file=sys.stderr)
         sys.exit(2)
 
     from hawassi_c_amd import HW_PREC_FP32, HW_PREC_FP64
     from hawassi_c_amd import workloads as W
     from hawassi_c_amd.model2d import Hawassi2D
+
+    if args.copy_probe_mib:
+        import torch
+        x = torch.ones(args.copy_probe_mib * (1 << 18), device="cuda")
+        y = torch.empty_like(x)
+        torch.neg(x, out=y)
+        torch.cuda.synchronize()
+        log(f"copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written")
+        del x, y
 
     n = args.n
     t_end = args.spinup + 60.0
@@ -127,7 +144,8 @@ def main():
         "steps_rejected": st["steps_rejected"],
         "avg_rhs_ms": avg_rhs_ms,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": RHS_TRAFFIC_BYTES_2048 if n == 2048 else None,
                      "unit_of_work": "one RHS evaluation (9 kernels), %.1f MB algorithmic" % (b_rhs / 1e6)},
         "roofline_job": {"achieved": job, "frac": job / HBM_PEAK_GBS, "unit": "GB/s",
                          "bytes_per_rhs": b_job},
@@ -140,11 +158,13 @@ def main():
         log("accuracy check vs FP64 device path")
         d64 = Hawassi2D(model, HW_PREC_FP64)
         d64.set_state(state_start)
+        d64.set_tprev(t_start)
         t64, _ = d64.integrate(t_start, t, h_start)
         e64, u64, v64 = d64.fields()
         d64.close()
         d32 = Hawassi2D(model, HW_PREC_FP32)
         d32.set_state(state_start)
+        d32.set_tprev(t_start)
         d32.integrate(t_start, t, h_start)
         e32, u32, v32 = d32.fields()
         d32.close()
@@ -159,6 +179,7 @@ def main():
         O.set_threads(threads)
         log(f"cpu baseline: oracle, {threads} threads")
         orc = O.Oracle2D(model)
+        orc.set_tprev(t_start)   # the driver's tprev after the previous accepted step
         y = state_start.copy()
         c0 = orc.rhs_count()
         w0 = time.perf_counter()
