@@ -34,6 +34,22 @@ class HwConfig2D(ct.Structure):
     ]
 
 
+class HwConfig1D(ct.Structure):
+    _fields_ = [
+        ("n", ct.c_int), ("flat", ct.c_int), ("source_on", ct.c_int), ("propagation", ct.c_int),
+        ("breaking_on", ct.c_int), ("dx", ct.c_double),
+        ("k", _dp), ("aal", _dp), ("chi", _dp), ("damp", _dp), ("csqr", _dp), ("lop", _dp),
+        ("csq_min", _dp), ("csq_plus", _dp), ("gam_min", _dp), ("gam_plus", _dp),
+        ("gamx_re", _dp), ("gamx_im", _dp), ("gamx_skew_re", _dp), ("gamx_skew_im", _dp),
+        ("nt", ct.c_int), ("t_sig", _dp), ("sig", _dp), ("sig_skew", _dp),
+        ("KBC", ct.c_double), ("TC", ct.c_double), ("Tstar", ct.c_double), ("max_eta_init", ct.c_double),
+        ("delb", ct.c_double),
+        ("xbstart", ct.c_int), ("xbend", ct.c_int), ("xbendL", ct.c_int), ("xbstartR", ct.c_int),
+        ("lenIF", ct.c_int), ("ILL", ct.c_int), ("IFF", ct.c_int),
+        ("depth", _dp), ("eps_abs", ct.c_double), ("eps_rel", ct.c_double), ("t0", ct.c_double), ("h0", ct.c_double),
+    ]
+
+
 class HwStats(ct.Structure):
     _fields_ = [("rhs_evals", ct.c_long), ("steps_accepted", ct.c_long), ("steps_rejected", ct.c_long),
                 ("rhs_ms_total", ct.c_double)]
@@ -44,6 +60,8 @@ EXPORTS = [
     "hw_evolve_apply", "hw_integrate", "hw_get_fields", "hw_get_stats", "hw_reset_stats", "hw_sync",
     "hw_fft2_forward", "hw_fft2_inverse", "hw_supported", "hw_last_error", "hw_version",
     "hw_set_tprev", "hw_get_breaking",
+    "hw1d_create", "hw1d_destroy", "hw1d_rhs", "hw1d_rhs_at", "hw1d_set_state", "hw1d_get_state",
+    "hw1d_set_time", "hw1d_get_time", "hw1d_integrate", "hw1d_get_breaking", "hw1d_get_stats",
 ]
 
 _lib = None
@@ -81,6 +99,18 @@ def load(path: Path | None = None) -> ct.CDLL:
     lib.hw_fft2_forward.argtypes = [ct.c_int, ct.c_int, ct.c_int, _dp, _dp, _dp]
     lib.hw_fft2_inverse.argtypes = [ct.c_int, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp]
     lib.hw_supported.argtypes = [ct.c_int, ct.c_int, ct.c_int]
+    lib.hw1d_create.argtypes = [ct.POINTER(HwConfig1D), ct.c_int, ct.c_int, ct.POINTER(vp)]
+    lib.hw1d_destroy.argtypes = [vp]
+    lib.hw1d_destroy.restype = None
+    lib.hw1d_rhs.argtypes = [ct.c_double, _dp, _dp, vp]
+    lib.hw1d_rhs_at.argtypes = [vp, ct.c_int, ct.c_double, _dp, _dp]
+    lib.hw1d_set_state.argtypes = [vp, ct.c_int, _dp]
+    lib.hw1d_get_state.argtypes = [vp, ct.c_int, _dp]
+    lib.hw1d_set_time.argtypes = [vp, ct.c_int, ct.c_double, ct.c_double]
+    lib.hw1d_get_time.argtypes = [vp, ct.c_int, _dp, _dp]
+    lib.hw1d_integrate.argtypes = [vp, ct.c_double]
+    lib.hw1d_get_breaking.argtypes = [vp, ct.c_int, _dp, ct.POINTER(ct.c_int)]
+    lib.hw1d_get_stats.argtypes = [vp, ct.c_int, ct.POINTER(HwStats)]
     lib.hw_last_error.restype = ct.c_char_p
     lib.hw_version.restype = ct.c_char_p
     if path is None:

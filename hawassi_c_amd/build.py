@@ -32,10 +32,33 @@ SIZES = (
 )
 
 
+# 1-D batched right-hand side: one unit per (precision, workgroup size)
+NT1D = (64, 128, 256, 512, 1024)
+
+
 def _units():
     gen = BUILD / "gen"
     gen.mkdir(parents=True, exist_ok=True)
-    units = [CSRC / "capi.hip"]
+    units = [CSRC / "capi.hip", CSRC / "capi1d.hip"]
+    for prec in (0, 1):
+        t = "double" if prec == 0 else "float"
+        for nt in NT1D:
+            f = gen / f"inst1d_p{prec}_nt{nt}.hip"
+            text = (f'// generated by hawassi_c_amd/build.py\n#include "{CSRC}/solver1d.h"\n'
+                    f"namespace hw {{\n"
+                    f"hipError_t k1d_launch_{prec}_{nt}(dim3 g, size_t lds, hipStream_t s, const Batch1DArgs<{t}>& a, int st) {{\n"
+                    f"    hipLaunchKernelGGL((k1d_rhs<{t}, {nt}>), g, dim3({nt}), lds, s, a, st);\n"
+                    f"    return hipGetLastError();\n}}\n"
+                    f"hipError_t k1d_control_{prec}_{nt}(dim3 g, hipStream_t s, const Batch1DArgs<{t}>& a) {{\n"
+                    f"    hipLaunchKernelGGL((k1d_control<{t}, {nt}>), g, dim3({nt}), 0, s, a);\n"
+                    f"    return hipGetLastError();\n}}\n"
+                    f"hipError_t k1d_setlds_{prec}_{nt}(size_t b) {{\n"
+                    f"    return hipFuncSetAttribute((const void*)k1d_rhs<{t}, {nt}>, "
+                    f"hipFuncAttributeMaxDynamicSharedMemorySize, (int)b);\n}}\n"
+                    f"}}  // namespace hw\n")
+            if not f.exists() or f.read_text() != text:
+                f.write_text(text)
+            units.append(f)
     for prec, nx, ny in SIZES:
         t = "double" if prec == 0 else "float"
         f = gen / f"inst_p{prec}_{nx}x{ny}.hip"

@@ -1,0 +1,950 @@
+// rhs1d.h -- the 1-D HS2 right-hand side (rhs_HsF2, 1d/rhs.c:96-241) and
+// its RKF45 step control for a batch of independent realizations.
+//
+// One workgroup owns one realization (n <= 16384 complex points in FP32,
+// <= 8192 in FP64) and evaluates its whole right-hand side: every transform
+// is an LDS-resident Stockham FFT of length n, fields between transforms stay
+// in registers or in a per-realization scratch area.  The reference takes the
+// real part of full complex inverse transforms (no Hermitian projection in
+// 1-D, 1d/functions.c:1642-1661); two such real-part inverses are done with
+// one complex FFT of Herm(X) + i Herm(Y), and two forward transforms of real
+// fields with one complex FFT of a + i b.  Thread t owns the spectral (and
+// physical) indices k = t + e*NT (k < n/2) and their partners n - k (0 is
+// paired with n/2), so the Hermitian splits are thread-local.
+//
+// Batches mix sizes: the kernel switches on the realization's n; all blocks
+// of a launch use the same thread count NT with 2 <= n/NT <= 16.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hw_fft.h"
+
+namespace hw {
+
+// ----------------------------------------------------------- descriptors --
+struct Real1D {                 // read-only model description of one realization
+    int n, flat, src, prop;     // prop: 0 Uni+, 1 Uni-, 2 Bi, 3 none
+    int brk, nt, cap, log2n;
+    long long off;              // element offset of this realization (sum of earlier n)
+    long long sig_off;          // offset into the signal arrays
+    long long cb_off;           // offset into the crest arrays (sum of earlier caps)
+    double dx, KBC, TC, Tstar, MaxEtaInit, delb, eps_abs, eps_rel;
+    int xbstart, xbend, xbendL, xbstartR;
+    int lenIF, ILL, IFF, pad;
+};
+
+struct Rk1D {                   // driver state of one realization (FP64)
+    double t, h, h0, t1;        // time, suggested step, step of the current attempt, target
+    int need_in, final_step, active, err;
+    long long accepted, rejected, rhs;
+};
+
+struct Cb1 {                    // CB (1d/header.h:44-52); Bposition is not used by the logic
+    int Bindex, BindexNow;
+    double Ucrest;
+    int DirProp, Bcount;
+    double Btime;
+};
+
+struct Brk1D {
+    int flag;                   // the reference's global `flag` (0 none, 1 first crests, 2 tracking)
+    int overflow;
+    int pad[2];
+};
+
+enum : int { R1_ETA = 0, R1_U, R1_M0U, R1_H, R1_P2, R1_FLUX, R1_NR };
+enum : int { C1_EH = 0, C1_M0H, C1_EUH, C1_U2H, C1_HP, C1_NC };
+
+template <typename T> struct Batch1DArgs {
+    using C = typename ctype<T>::type;
+    const Real1D* real;
+    Rk1D* rk;
+    const int* list;            // realization ids of this launch (blockIdx.x -> id)
+    // per-element model arrays, offset `off`
+    const T *k, *aal, *chi, *damp, *opa, *opb, *gm, *gp, *depth;
+    const C *gamx, *gamxs;
+    // influx signals (natural spline: values and half second derivatives)
+    const double *tsig, *sig, *sigc, *skw, *skwc;
+    // state (FP64) and stage derivatives, offset 2*off: [eta^ (n)][u^ (n)]
+    double2* y;
+    C* K[7];                    // K[0..5]: rkf45 stages, K[6]: dydt_out / stand-alone output
+    const double2* ext;         // stand-alone input (stage 0)
+    double t_ext;
+    // scratch: real fields at off*R1_NR + f*n, complex at off*C1_NC + f*n
+    T* sr;
+    C* sc;
+    // breaking
+    T* B;
+    int* peaks;
+    Cb1 *cb, *cbn, *cbp;
+    int* ide;
+    Brk1D* bs;
+    const C* tw[16];            // tw[log2 n][m] = exp(-2 pi i m / n)
+    T g;
+};
+
+// stage input coefficients: input = y + h0 * sum_j a[s][j] K_j
+__constant__ double c_rk1d[8][6] = {
+    {0, 0, 0, 0, 0, 0},
+    {0, 0, 0, 0, 0, 0},
+    {1.0 / 4.0, 0, 0, 0, 0, 0},
+    {3.0 / 32.0, 9.0 / 32.0, 0, 0, 0, 0},
+    {1932.0 / 2197.0, -7200.0 / 2197.0, 7296.0 / 2197.0, 0, 0, 0},
+    {8341.0 / 4104.0, -32832.0 / 4104.0, 29440.0 / 4104.0, -845.0 / 4104.0, 0, 0},
+    {-6080.0 / 20520.0, 41040.0 / 20520.0, -28352.0 / 20520.0, 9295.0 / 20520.0, -5643.0 / 20520.0, 0},
+    {902880.0 / 7618050.0, 0.0, 3953664.0 / 7618050.0, 3855735.0 / 7618050.0, -1371249.0 / 7618050.0,
+     277020.0 / 7618050.0},
+};
+__constant__ double c_rk1d_ah[5] = {1.0 / 4.0, 3.0 / 8.0, 12.0 / 13.0, 1.0, 1.0 / 2.0};
+__constant__ double c_rk1d_ec[6] = {1.0 / 360.0, 0.0, -128.0 / 4275.0, -2197.0 / 75240.0, 1.0 / 50.0, 2.0 / 55.0};
+
+// ------------------------------------------------------- block helpers --
+// red: LDS scratch of at least 2*NT/64 + 8 words
+template <int NT> __device__ __forceinline__ int block_or(int v, int* red) {
+    v = __any(v) ? 1 : 0;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int r = 0;
+#pragma unroll
+    for (int w = 0; w < (NT + 63) / 64; ++w) r |= red[w];
+    __syncthreads();
+    return r;
+}
+
+template <int NT> __device__ __forceinline__ int block_min(int v, int* red) {
+    for (int o = 32; o > 0; o >>= 1) {
+        int u = __shfl_xor(v, o, 64);
+        v = u < v ? u : v;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int r = red[0];
+#pragma unroll
+    for (int w = 1; w < (NT + 63) / 64; ++w) r = red[w] < r ? red[w] : r;
+    __syncthreads();
+    return r;
+}
+
+template <int NT> __device__ __forceinline__ double block_max_d(double v, double* red) {
+    for (int o = 32; o > 0; o >>= 1) {
+        double u = __shfl_xor(v, o, 64);
+        v = u > v ? u : v;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = red[0];
+#pragma unroll
+    for (int w = 1; w < (NT + 63) / 64; ++w) r = red[w] > r ? red[w] : r;
+    __syncthreads();
+    return r;
+}
+
+// first index of the maximum (ties -> smallest index); idx = INT_MAX if empty
+template <int NT> __device__ __forceinline__ void block_argmax(double& v, int& idx, double* redv, int* redi) {
+    for (int o = 32; o > 0; o >>= 1) {
+        double u = __shfl_xor(v, o, 64);
+        int j = __shfl_xor(idx, o, 64);
+        if (u > v || (u == v && j < idx)) { v = u; idx = j; }
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) { redv[threadIdx.x >> 6] = v; redi[threadIdx.x >> 6] = idx; }
+    __syncthreads();
+    double bv = redv[0];
+    int bi = redi[0];
+#pragma unroll
+    for (int w = 1; w < (NT + 63) / 64; ++w)
+        if (redv[w] > bv || (redv[w] == bv && redi[w] < bi)) { bv = redv[w]; bi = redi[w]; }
+    __syncthreads();
+    v = bv;
+    idx = bi;
+}
+
+// ordered compaction: threads with f set get consecutive slots in thread
+// order; returns this thread's slot (or -1) and the block total in *total
+template <int NT> __device__ __forceinline__ int block_compact(int f, int* red, int* total) {
+    unsigned long long m = __ballot(f);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int pre = __popcll(m & ((1ull << lane) - 1ull));
+    __syncthreads();
+    if (lane == 0) red[w] = __popcll(m);
+    __syncthreads();
+    int base = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < (NT + 63) / 64; ++q) {
+        if (q < w) base += red[q];
+        tot += red[q];
+    }
+    __syncthreads();
+    *total = tot;
+    return f ? base + pre : -1;
+}
+
+// --------------------------------------------------------- pair mapping --
+template <typename T, int N, int NT> struct P1 {
+    using C = typename ctype<T>::type;
+    static constexpr int E = N / NT;
+    static constexpr int EH = E / 2;
+    static_assert(E >= 2 && E <= 16, "n / NT must be in [2, 16]");
+    __device__ static __forceinline__ int kl(int tid, int e) { return tid + e * NT; }
+    __device__ static __forceinline__ int kh(int tid, int e) {
+        int k = tid + e * NT;
+        return k == 0 ? N / 2 : N - k;
+    }
+
+    __device__ static __forceinline__ void ldc(const C* __restrict__ p, C (&l)[EH], C (&h)[EH], int tid) {
+#pragma unroll
+        for (int e = 0; e < EH; ++e) { l[e] = p[kl(tid, e)]; h[e] = p[kh(tid, e)]; }
+    }
+    __device__ static __forceinline__ void stc(C* __restrict__ p, const C (&l)[EH], const C (&h)[EH], int tid) {
+#pragma unroll
+        for (int e = 0; e < EH; ++e) { p[kl(tid, e)] = l[e]; p[kh(tid, e)] = h[e]; }
+    }
+    __device__ static __forceinline__ void ldr(const T* __restrict__ p, T (&l)[EH], T (&h)[EH], int tid) {
+#pragma unroll
+        for (int e = 0; e < EH; ++e) { l[e] = p[kl(tid, e)]; h[e] = p[kh(tid, e)]; }
+    }
+    __device__ static __forceinline__ void str(T* __restrict__ p, const T (&l)[EH], const T (&h)[EH], int tid) {
+#pragma unroll
+        for (int e = 0; e < EH; ++e) { p[kl(tid, e)] = l[e]; p[kh(tid, e)] = h[e]; }
+    }
+
+    // (a, b) = (Re F^-1 X, Re F^-1 Y), F^-1 = backward DFT / N
+    __device__ static void inv_pair(const C (&Xl)[EH], const C (&Xh)[EH], const C (&Yl)[EH], const C (&Yh)[EH],
+                                    T (&al)[EH], T (&ah)[EH], T (&bl)[EH], T (&bh)[EH], C* lds,
+                                    const C* __restrict__ tw, int tid) {
+        const T hf = (T)0.5;
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            const bool z = kl(tid, e) == 0;
+            C xml = z ? Xl[e] : Xh[e], xmh = z ? Xh[e] : Xl[e];
+            C yml = z ? Yl[e] : Yh[e], ymh = z ? Yh[e] : Yl[e];
+            // Herm(X)[k] + i Herm(Y)[k], Herm(X)[k] = (X[k] + conj X[-k]) / 2
+            T hxr = hf * (Xl[e].x + xml.x), hxi = hf * (Xl[e].y - xml.y);
+            T hyr = hf * (Yl[e].x + yml.x), hyi = hf * (Yl[e].y - yml.y);
+            lds[kl(tid, e)] = cmk<C>(hxr - hyi, hxi + hyr);
+            hxr = hf * (Xh[e].x + xmh.x); hxi = hf * (Xh[e].y - xmh.y);
+            hyr = hf * (Yh[e].x + ymh.x); hyi = hf * (Yh[e].y - ymh.y);
+            lds[kh(tid, e)] = cmk<C>(hxr - hyi, hxi + hyr);
+        }
+        __syncthreads();
+        fft_lds<C, N, 1, NT, 1>(lds, N, tw, 1, tid);
+        const T s = (T)1 / (T)N;
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            C zl = lds[kl(tid, e)], zh = lds[kh(tid, e)];
+            al[e] = zl.x * s; bl[e] = zl.y * s;
+            ah[e] = zh.x * s; bh[e] = zh.y * s;
+        }
+    }
+
+    // (A, B) = (F a, F b) for real a, b; F = forward DFT (unnormalised)
+    __device__ static void fwd_pair(const T (&al)[EH], const T (&ah)[EH], const T (&bl)[EH], const T (&bh)[EH],
+                                    C (&Al)[EH], C (&Ah)[EH], C (&Bl)[EH], C (&Bh)[EH], C* lds,
+                                    const C* __restrict__ tw, int tid) {
+        const T hf = (T)0.5;
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            lds[kl(tid, e)] = cmk<C>(al[e], bl[e]);
+            lds[kh(tid, e)] = cmk<C>(ah[e], bh[e]);
+        }
+        __syncthreads();
+        fft_lds<C, N, 1, NT, -1>(lds, N, tw, 1, tid);
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            const bool z = kl(tid, e) == 0;
+            C zl = lds[kl(tid, e)], zh = lds[kh(tid, e)];
+            C ml = z ? zl : zh, mh = z ? zh : zl;
+            Al[e] = cmk<C>(hf * (zl.x + ml.x), hf * (zl.y - ml.y));
+            Bl[e] = cmk<C>(hf * (zl.y + ml.y), -hf * (zl.x - ml.x));
+            Ah[e] = cmk<C>(hf * (zh.x + mh.x), hf * (zh.y - mh.y));
+            Bh[e] = cmk<C>(hf * (zh.y + mh.y), -hf * (zh.x - mh.x));
+        }
+    }
+};
+
+// ------------------------------------------------------------ breaking ----
+// breaking_process + breaking_criterion (1d/breaking.c:63-450) for one
+// realization, by the whole workgroup.  eta, u, H (Hilbert transform of eta)
+// are physical fields in global memory; B is written.  The crest lists and
+// `flag` are per-realization device state.  Sequential parts run on thread 0
+// between barriers; scans over nodes are block-parallel reductions that keep
+// the reference's first-index semantics.
+template <typename T, int N, int NT>
+__device__ void brk1d_process(const Batch1DArgs<T>& a, const Real1D& R, double t, const T* __restrict__ eta,
+                              const T* __restrict__ u, const T* __restrict__ H, T* __restrict__ B, int* sm,
+                              double* smd) {
+    const int tid = threadIdx.x;
+    const int cap = R.cap;
+    Cb1* CB = a.cb + R.cb_off;
+    Cb1* New = a.cbn + R.cb_off;
+    Cb1* Prev = a.cbp + R.cb_off;
+    int* IDE = a.ide + R.cb_off;
+    int* peaks = a.peaks + R.off;
+    Brk1D* bs = a.bs + (&R - a.real);
+    int* red = sm;              // >= 2*NT/64 words
+    int* shv = sm + 40;         // shared scalars
+    double* redv = smd;
+    double* shd = smd + 20;
+
+    for (int i = tid; i < N; i += NT) B[i] = (T)0;
+    // ---- breaking_criterion: threshold test and peaks (ordered)
+    int any = 0;
+    for (int i = tid; i < R.lenIF; i += NT) any |= (double)eta[R.xbstart + i] > R.MaxEtaInit;
+    any = block_or<NT>(any, red);
+    int npk = 0;
+    if (any) {
+        const bool single = R.xbendL == 0 && R.xbstartR == 0;
+        for (int w = 0; w < (single ? 1 : 2); ++w) {
+            const int w0 = single ? R.xbstart : (w == 0 ? R.xbstart : R.xbstartR);
+            const int len = single ? R.lenIF : (w == 0 ? R.ILL : R.IFF);
+            for (int base = 1; base < len - 1; base += NT) {
+                int j = base + tid, f = 0;
+                if (j < len - 1) {
+                    int q = w0 + j;
+                    double v = (double)eta[q];
+                    f = !(v < R.MaxEtaInit) && v > (double)eta[q + 1] && v > (double)eta[q - 1];
+                }
+                int tot;
+                int slot = block_compact<NT>(f, red, &tot);
+                if (slot >= 0) peaks[npk + slot] = w0 + j;
+                npk += tot;
+            }
+        }
+    }
+    __syncthreads();
+    // ---- crest test at each peak; passing peaks fill CB in peak order
+    if (npk > 0) {
+        const double dx = R.dx;
+        int itcb = 0;
+        for (int base = 0; base < npk; base += NT) {
+            int p = base + tid, f = 0;
+            double Uc = 0.0;
+            int J = 0;
+            if (p < npk) {
+                J = peaks[p];
+                auto grad = [&](const T* f_, int i) -> double {
+                    if (i == 0) return ((double)f_[1] - (double)f_[0]) / dx;
+                    if (i == N - 1) return ((double)f_[N - 1] - (double)f_[N - 2]) / dx;
+                    return ((double)f_[i + 1] - (double)f_[i - 1]) / (2 * dx);
+                };
+                double e = eta[J], h = H[J];
+                double Kx = (e * grad(H, J) - h * grad(eta, J)) / (e * e + h * h);
+                double d = (double)a.depth[R.off + J] + e;
+                double Cc = Kx == 0 ? sqrt(9.80665 * d) : (Kx < 0 ? -1.0 : 1.0) * sqrt(9.80665 * Kx * tanh(d * Kx)) / Kx;
+                Uc = u[J];
+                f = fabs(Uc) > R.KBC * fabs(Cc);
+            }
+            int tot;
+            int slot = block_compact<NT>(f, red, &tot);
+            if (slot >= 0 && itcb + slot < cap) {
+                Cb1& c = CB[itcb + slot];
+                c.Bindex = J; c.BindexNow = J; c.Ucrest = fabs(Uc);
+                c.DirProp = (int)(Uc / fabs(Uc)); c.Btime = t;
+            }
+            itcb += tot;
+        }
+        if (itcb > cap) { if (tid == 0) bs->overflow = 1; itcb = cap; }
+        __syncthreads();
+        for (int j = tid; j < itcb; j += NT) CB[j].Bcount = itcb;
+    }
+    __syncthreads();
+    // ---- crest list for this call (1d/breaking.c:212-300), thread 0
+    if (tid == 0) {
+        int flag = bs->flag;
+        int NCB = CB[0].Bcount;
+        if (NCB > 0 && flag == 0) {
+            flag = 1;
+            for (int j = 0; j < NCB; ++j) {
+                New[j].Bindex = CB[j].Bindex; New[j].BindexNow = CB[j].BindexNow; New[j].Ucrest = CB[j].Ucrest;
+                New[j].DirProp = CB[j].DirProp; New[j].Btime = CB[j].Btime; New[j].Bcount = CB[j].Bcount;
+            }
+        } else if (NCB == 0 && flag == 2) {
+            int np = Prev[0].Bcount;
+            for (int j = 0; j < np; ++j) New[j] = Prev[j];
+        } else if (NCB > 0 && flag == 2) {
+            int np = Prev[0].Bcount;
+            for (int j = 0; j < np; ++j) New[j] = Prev[j];
+            int add = 0;
+            for (int kk = 0; kk < NCB; ++kk) {
+                int isnew = 1;
+                for (int l = 0; l < np; ++l) {
+                    int dleft = abs(IDE[l] - Prev[l].BindexNow);
+                    if (Prev[l].DirProp == 1) {
+                        if (CB[kk].Bindex >= Prev[l].BindexNow - 3 * dleft && CB[kk].Bindex <= IDE[l]) { isnew = 0; break; }
+                    } else {
+                        if (CB[kk].Bindex >= Prev[l].BindexNow && CB[kk].Bindex <= IDE[l] + 3 * dleft) { isnew = 0; break; }
+                    }
+                }
+                if (isnew) {
+                    if (np + add >= cap) { bs->overflow = 1; break; }
+                    Cb1& d = New[np + add];
+                    d.Bindex = CB[kk].Bindex; d.BindexNow = CB[kk].Bindex; IDE[np + add] = CB[kk].Bindex;
+                    d.Ucrest = CB[kk].Ucrest; d.Btime = CB[kk].Btime; d.DirProp = CB[kk].DirProp;
+                    d.Bcount = CB[kk].Bcount;
+                    ++add;
+                }
+            }
+            int j = 0;
+            while (j < cap && New[j].Bindex > 0) ++j;
+            for (int i = 0; i < j; ++i) New[i].Bcount = j;
+        }
+        bs->flag = flag;
+        shv[0] = flag;
+        shv[1] = (flag == 1 || flag == 2) ? New[0].Bcount : 0;
+    }
+    __syncthreads();
+    const int flag0 = shv[0];
+    const int NCBnow = shv[1];
+    if (flag0 == 1 || flag0 == 2) {
+        if (tid == 0) shv[2] = 0;   // iterNprev
+        for (int j = tid; j < cap; j += NT) {
+            Prev[j].Bindex = 0; Prev[j].Ucrest = 0; Prev[j].DirProp = 0; Prev[j].Btime = 0; Prev[j].Bcount = 0;
+        }
+        __syncthreads();
+        int anyterm = 0;   // IDTerminate == 1 seen
+        for (int lx = 0; lx < NCBnow; ++lx) {
+            const int Dir = New[lx].DirProp;
+            int now;
+            if (flag0 == 1) {
+                now = New[lx].Bindex;
+            } else {
+                // idx_max over eta[start + j*Dir], j < lenloc (first maximum)
+                const int start = New[lx].BindexNow;
+                const int lenloc = abs(IDE[lx] - start);
+                double bv = -HUGE_VAL;
+                int bi = 0x7fffffff;
+                for (int j = tid; j < lenloc; j += NT) {
+                    double v = eta[start + j * Dir];
+                    if (v > bv || (v == bv && j < bi)) { bv = v; bi = j; }
+                }
+                block_argmax<NT>(bv, bi, redv, red);
+                now = start + (lenloc > 0 ? bi : 0) * Dir;
+            }
+            __syncthreads();
+            if (tid == 0) New[lx].BindexNow = now;
+            if (lx != 0 && now == New[lx - 1].BindexNow) { __syncthreads(); continue; }
+            const double tb = New[lx].Btime, UI = New[lx].Ucrest, UF = R.TC * UI;
+            double Us;
+            if (t - tb >= R.Tstar) Us = UF;
+            else if (t - tb >= 0 && t - tb < R.Tstar) Us = UI + ((t - tb) / R.Tstar) * (UF - UI);
+            else Us = UI;
+            const int indEND = Dir == 1 ? R.xbend : R.xbstart;
+            // nodes ly = now + p*Dir while ly < indEND (and inside the grid)
+            int plen;
+            if (Dir == 1) plen = indEND - now;
+            else if (Dir == -1) plen = now < indEND ? now + 1 : 0;
+            else plen = now < indEND ? 1 : 0;
+            if (plen < 0) plen = 0;
+            int pstop = 0x7fffffff;
+            for (int p = tid; p < plen; p += NT) {
+                int ly = now + p * Dir;
+                if (fabs((double)u[ly]) < UF) { pstop = p; break; }
+            }
+            pstop = block_min<NT>(pstop, red);
+            const int pend = pstop < plen ? pstop : plen;
+            int fl = 0;
+            for (int p = tid; p < pend; p += NT) {
+                int ly = now + p * Dir;
+                double au = fabs((double)u[ly]);
+                T bval;
+                if (au > 2 * Us) bval = (T)1;
+                else if (au <= Us) bval = (T)0;
+                else bval = (T)(au / Us - 1);
+                B[ly] = bval;
+                fl |= au >= Us;
+            }
+            fl = block_or<NT>(fl, red);
+            if (tid == 0 && fl) {
+                int it = shv[2];
+                IDE[it] = pstop < plen ? now + pstop * Dir : now;
+                Prev[it].Bindex = New[lx].Bindex; Prev[it].BindexNow = New[lx].BindexNow;
+                Prev[it].Ucrest = New[lx].Ucrest; Prev[it].Btime = New[lx].Btime;
+                Prev[it].DirProp = New[lx].DirProp; Prev[it].Bcount = New[lx].Bcount;
+                shv[2] = it + 1;
+            }
+            anyterm |= fl;
+            __syncthreads();
+        }
+        if (tid == 0) {
+            int j = 0;
+            while (j < cap && Prev[j].Bindex > 0) ++j;
+            for (int i = 0; i < j; ++i) Prev[i].Bcount = j;
+            if (NCBnow > 0) bs->flag = anyterm ? 2 : 0;
+        }
+        __syncthreads();
+    }
+    for (int j = tid; j < cap; j += NT) {
+        New[j].Bindex = 0; New[j].Ucrest = 0; New[j].DirProp = 0; New[j].Btime = 0; New[j].Bcount = 0;
+        CB[j].Bindex = 0; CB[j].Ucrest = 0; CB[j].DirProp = 0; CB[j].Btime = 0; CB[j].Bcount = 0;
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------ the RHS ----
+template <typename T, int N, int NT>
+__device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename ctype<T>::type* lds, int* sm,
+                           double* smd) {
+    using C = typename ctype<T>::type;
+    using P = P1<T, N, NT>;
+    constexpr int EH = P::EH;
+    const int tid = threadIdx.x;
+    const Real1D& R = a.real[r];
+    Rk1D* S = a.rk + r;
+    const long long off = R.off;
+    const C* tw = a.tw[R.log2n];
+    const T g = a.g;
+    const T* kk = a.k + off;
+    T* sr = a.sr + off * R1_NR;
+    C* sc = a.sc + off * C1_NC;
+
+    // ---- time and stage input (the RK combination is fused into the load)
+    double t, h0 = S->h0;
+    if (stage == 0) t = a.t_ext;
+    else if (stage == 1) t = S->t;
+    else if (stage == 7) t = S->t + h0;
+    else t = S->t + c_rk1d_ah[stage - 2] * h0;
+    if (tid == 0) {
+        double pv = 0.0, pvs = 0.0;
+        if (R.src && R.nt > 1) {
+            const double* ts = a.tsig + R.sig_off;
+            int lo = 0, hi = R.nt - 1;
+            while (hi > lo + 1) { int m = (hi + lo) >> 1; if (ts[m] > t) hi = m; else lo = m; }
+            double hh = ts[lo + 1] - ts[lo], dxs = t - ts[lo];
+            const double *ys[2] = {a.sig + R.sig_off, a.skw + R.sig_off}, *cs[2] = {a.sigc + R.sig_off, a.skwc + R.sig_off};
+            double out[2];
+            for (int q = 0; q < 2; ++q) {
+                double ci = cs[q][lo], cip = cs[q][lo + 1], dy = ys[q][lo + 1] - ys[q][lo];
+                double b = dy / hh - hh * (cip + 2.0 * ci) / 3.0, d = (cip - ci) / (3.0 * hh);
+                out[q] = ys[q][lo] + dxs * (b + dxs * (ci + dxs * d));
+            }
+            pv = out[0];
+            pvs = out[1];
+        }
+        smd[30] = pv;
+        smd[31] = pvs;
+        S->rhs += 1;
+    }
+    C ehl[EH], ehh[EH], uhl[EH], uhh[EH];
+    {
+        const double2* base = (stage == 0 ? a.ext : a.y) + 2 * off;
+        const int nk = stage >= 2 ? 6 : 0;
+        double co[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) co[j] = c_rk1d[stage][j];
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            int idx[4] = {P::kl(tid, e), P::kh(tid, e), N + P::kl(tid, e), N + P::kh(tid, e)};
+            C v[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                double2 b = base[idx[q]];
+                double sx = 0.0, sy = 0.0;
+                for (int j = 0; j < nk; ++j) {
+                    if (co[j] != 0.0) {
+                        C kv = a.K[j][2 * off + idx[q]];
+                        sx += co[j] * (double)kv.x;
+                        sy += co[j] * (double)kv.y;
+                    }
+                }
+                v[q] = cmk<C>((T)(b.x + h0 * sx), (T)(b.y + h0 * sy));
+            }
+            ehl[e] = v[0]; ehh[e] = v[1]; uhl[e] = v[2]; uhh[e] = v[3];
+        }
+    }
+    P::stc(sc + C1_EH * N, ehl, ehh, tid);
+    // ---- eta, u
+    T etl[EH], eth[EH], ul[EH], uh_[EH];
+    P::inv_pair(ehl, ehh, uhl, uhh, etl, eth, ul, uh_, lds, tw, tid);
+    P::str(sr + R1_ETA * N, etl, eth, tid);
+    P::str(sr + R1_U * N, ul, uh_, tid);
+    // ---- M0u^ = deluH1 = -i k HSS(u^)/g (two depths) or -i k C^2 u^/g (flat)
+    C ml[EH], mh[EH];
+    if (!R.flat) {
+        C xl[EH], xh[EH], yl[EH], yh[EH];
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            T cm0 = a.opa[off + i0], cm1 = a.opa[off + i1], cp0 = a.opb[off + i0], cp1 = a.opb[off + i1];
+            xl[e] = cscale(uhl[e], cm0); xh[e] = cscale(uhh[e], cm1);
+            yl[e] = cscale(uhl[e], cp0); yh[e] = cscale(uhh[e], cp1);
+        }
+        T aml[EH], amh[EH], apl[EH], aph[EH];
+        P::inv_pair(xl, xh, yl, yh, aml, amh, apl, aph, lds, tw, tid);
+        T f1l[EH], f1h[EH], f2l[EH], f2h[EH];
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            T g0 = a.gm[off + i0], g1 = a.gm[off + i1], q0 = a.gp[off + i0], q1 = a.gp[off + i1];
+            f1l[e] = g0 * aml[e] + q0 * apl[e]; f1h[e] = g1 * amh[e] + q1 * aph[e];
+            f2l[e] = g0 * ul[e]; f2h[e] = g1 * uh_[e];
+        }
+        P::fwd_pair(f1l, f1h, f2l, f2h, xl, xh, yl, yh, lds, tw, tid);
+        // HP = (F(g- a- + g+ a+) + C- F(g- u)) / 2
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            T cm0 = a.opa[off + i0], cm1 = a.opa[off + i1];
+            ml[e] = cscale(cadd(xl[e], cscale(yl[e], cm0)), (T)0.5);
+            mh[e] = cscale(cadd(xh[e], cscale(yh[e], cm1)), (T)0.5);
+        }
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            f1l[e] = a.gp[off + i0] * ul[e]; f1h[e] = a.gp[off + i1] * uh_[e];
+            f2l[e] = etl[e] * ul[e]; f2h[e] = eth[e] * uh_[e];
+        }
+        P::fwd_pair(f1l, f1h, f2l, f2h, xl, xh, yl, yh, lds, tw, tid);
+        P::stc(sc + C1_EUH * N, yl, yh, tid);   // F(eta u)
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            T cp0 = a.opb[off + i0], cp1 = a.opb[off + i1];
+            C H0 = cadd(ml[e], cscale(xl[e], (T)0.5 * cp0)), H1 = cadd(mh[e], cscale(xh[e], (T)0.5 * cp1));
+            ml[e] = cmul_i<-1>(cscale(H0, kk[i0] / g));
+            mh[e] = cmul_i<-1>(cscale(H1, kk[i1] / g));
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            ml[e] = cmul_i<-1>(cscale(uhl[e], kk[i0] * a.opa[off + i0] / g));
+            mh[e] = cmul_i<-1>(cscale(uhh[e], kk[i1] * a.opa[off + i1] / g));
+        }
+    }
+    P::stc(sc + C1_M0H * N, ml, mh, tid);
+    // ---- M0u and the Hilbert transform of eta (breaking)
+    T m0l[EH], m0h[EH], hl[EH], hh[EH];
+    {
+        C xl[EH], xh[EH];
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            if (R.brk) {
+                int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+                T s0 = kk[i0] == (T)0 ? (T)0 : (kk[i0] > (T)0 ? (T)1 : (T)-1);
+                T s1 = kk[i1] == (T)0 ? (T)0 : (kk[i1] > (T)0 ? (T)1 : (T)-1);
+                xl[e] = cmul_i<-1>(cscale(ehl[e], s0));
+                xh[e] = cmul_i<-1>(cscale(ehh[e], s1));
+            } else {
+                xl[e] = cmk<C>((T)0, (T)0);
+                xh[e] = xl[e];
+            }
+        }
+        P::inv_pair(ml, mh, xl, xh, m0l, m0h, hl, hh, lds, tw, tid);
+    }
+    P::str(sr + R1_M0U * N, m0l, m0h, tid);
+    if (R.brk) P::str(sr + R1_H * N, hl, hh, tid);
+    // ---- products: F(eta M0u), F(u^2 - M0u^2), F(eta u) for flat
+    C eml[EH], emh[EH], u2l[EH], u2h[EH];
+    {
+        T pl[EH], ph[EH], ql[EH], qh[EH];
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            pl[e] = etl[e] * m0l[e]; ph[e] = eth[e] * m0h[e];
+            ql[e] = ul[e] * ul[e] - m0l[e] * m0l[e]; qh[e] = uh_[e] * uh_[e] - m0h[e] * m0h[e];
+        }
+        P::str(sr + R1_P2 * N, pl, ph, tid);
+        if (!R.flat) {
+            P::fwd_pair(pl, ph, ql, qh, eml, emh, u2l, u2h, lds, tw, tid);
+        } else {
+            T el[EH], eh2[EH];
+#pragma unroll
+            for (int e = 0; e < EH; ++e) { el[e] = etl[e] * ul[e]; eh2[e] = eth[e] * uh_[e]; }
+            C al_[EH], ah_[EH];
+            P::fwd_pair(el, eh2, pl, ph, al_, ah_, eml, emh, lds, tw, tid);
+            P::stc(sc + C1_EUH * N, al_, ah_, tid);
+#pragma unroll
+            for (int e = 0; e < EH; ++e) { el[e] = (T)0; eh2[e] = (T)0; }
+            P::fwd_pair(ql, qh, el, eh2, u2l, u2h, al_, ah_, lds, tw, tid);
+        }
+    }
+    // ---- L(eta M0u): k^2 HSS(F(eta M0u))/g (two depths) or Lop F(eta M0u) (flat)
+    C lel[EH], leh[EH];
+    if (!R.flat) {
+        C xl[EH], xh[EH], yl[EH], yh[EH];
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            xl[e] = cscale(eml[e], a.opa[off + i0]); xh[e] = cscale(emh[e], a.opa[off + i1]);
+            yl[e] = cscale(eml[e], a.opb[off + i0]); yh[e] = cscale(emh[e], a.opb[off + i1]);
+        }
+        T bml[EH], bmh[EH], bpl[EH], bph[EH];
+        P::inv_pair(xl, xh, yl, yh, bml, bmh, bpl, bph, lds, tw, tid);
+        T pl[EH], ph[EH];
+        P::ldr(sr + R1_P2 * N, pl, ph, tid);
+        T f1l[EH], f1h[EH], f2l[EH], f2h[EH];
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            T g0 = a.gm[off + i0], g1 = a.gm[off + i1], q0 = a.gp[off + i0], q1 = a.gp[off + i1];
+            f1l[e] = g0 * bml[e] + q0 * bpl[e]; f1h[e] = g1 * bmh[e] + q1 * bph[e];
+            f2l[e] = g0 * pl[e]; f2h[e] = g1 * ph[e];
+        }
+        P::fwd_pair(f1l, f1h, f2l, f2h, xl, xh, yl, yh, lds, tw, tid);
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            lel[e] = cscale(cadd(xl[e], cscale(yl[e], a.opa[off + i0])), (T)0.5);
+            leh[e] = cscale(cadd(xh[e], cscale(yh[e], a.opa[off + i1])), (T)0.5);
+            f1l[e] = a.gp[off + i0] * pl[e]; f1h[e] = a.gp[off + i1] * ph[e];
+            f2l[e] = (T)0; f2h[e] = (T)0;
+        }
+        P::fwd_pair(f1l, f1h, f2l, f2h, xl, xh, yl, yh, lds, tw, tid);
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            T k0 = kk[i0], k1 = kk[i1];
+            lel[e] = cscale(cadd(lel[e], cscale(xl[e], (T)0.5 * a.opb[off + i0])), k0 * k0 / g);
+            leh[e] = cscale(cadd(leh[e], cscale(xh[e], (T)0.5 * a.opb[off + i1])), k1 * k1 / g);
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            lel[e] = cscale(eml[e], a.opb[off + P::kl(tid, e)]);
+            leh[e] = cscale(emh[e], a.opb[off + P::kh(tid, e)]);
+        }
+    }
+    // ---- delH2 terms (1d/rhs.c:141-147) and their adjustment (adjust_delH)
+    T aul[EH], auh[EH], ael[EH], aeh[EH];
+    {
+        C eul[EH], euh[EH];
+        P::ldc(sc + C1_EUH * N, eul, euh, tid);
+        C dul[EH], duh[EH], del_[EH], deh[EH];
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            T k0 = kk[P::kl(tid, e)], k1 = kk[P::kh(tid, e)];
+            dul[e] = csub(cmk<C>(-lel[e].x, -lel[e].y), cmul_i<1>(cscale(eul[e], k0)));
+            duh[e] = csub(cmk<C>(-leh[e].x, -leh[e].y), cmul_i<1>(cscale(euh[e], k1)));
+            del_[e] = cscale(cmul_i<1>(cscale(u2l[e], k0)), (T)-0.5);
+            deh[e] = cscale(cmul_i<1>(cscale(u2h[e], k1)), (T)-0.5);
+        }
+        P::inv_pair(dul, duh, del_, deh, aul, auh, ael, aeh, lds, tw, tid);
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            T c0 = a.chi[off + P::kl(tid, e)], c1 = a.chi[off + P::kh(tid, e)];
+            aul[e] *= c0; auh[e] *= c1; ael[e] *= c0; aeh[e] *= c1;
+        }
+    }
+    // ---- breaking source Rb (rhs_breaking, 1d/rhs.c:617-657)
+    T rbl[EH], rbh[EH];
+#pragma unroll
+    for (int e = 0; e < EH; ++e) { rbl[e] = (T)0; rbh[e] = (T)0; }
+    if (R.brk) {
+        T* Bp = a.B + off;
+        const T* etap = sr + R1_ETA * N;
+        brk1d_process<T, N, NT>(a, R, t, etap, sr + R1_U * N, sr + R1_H * N, Bp, sm, smd);
+        T* flux = sr + R1_FLUX * N;
+        const T db2 = (T)(R.delb * R.delb);
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            T d0 = m0l[e] + aul[e], d1 = m0h[e] + auh[e];   // Re F^-1(deluH12)
+            flux[i0] = -Bp[i0] * db2 * (a.depth[off + i0] + etl[e]) * d0 * d0;
+            flux[i1] = -Bp[i1] * db2 * (a.depth[off + i1] + eth[e]) * d1 * d1;
+        }
+        __syncthreads();
+        const T idx1 = (T)(1.0 / R.dx), idx2 = (T)(0.5 / R.dx);
+        auto gradf = [&](int i) -> T {
+            if (i == 0) return (flux[1] - flux[0]) * idx1;
+            if (i == N - 1) return (flux[N - 1] - flux[N - 2]) * idx1;
+            return (flux[i + 1] - flux[i - 1]) * idx2;
+        };
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            rbl[e] = gradf(i0) / (a.depth[off + i0] + etl[e]);
+            rbh[e] = gradf(i1) / (a.depth[off + i1] + eth[e]);
+        }
+    }
+    // ---- output transforms with damping folded in, forcing, mask
+    {
+        C oel[EH], oeh[EH], oul[EH], ouh[EH];
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+            T d0 = a.damp[off + P::kl(tid, e)], d1 = a.damp[off + P::kh(tid, e)];
+            aul[e] = aul[e] - d0 * etl[e];
+            auh[e] = auh[e] - d1 * eth[e];
+            ael[e] = ael[e] - d0 * ul[e] + rbl[e];
+            aeh[e] = aeh[e] - d1 * uh_[e] + rbh[e];
+        }
+        P::fwd_pair(aul, auh, ael, aeh, oel, oeh, oul, ouh, lds, tw, tid);
+        C m0hl[EH], m0hh[EH];
+        P::ldc(sc + C1_M0H * N, m0hl, m0hh, tid);
+        const T pv = (T)smd[30], pvs = (T)smd[31];
+        C* Ko = (stage == 0 || stage == 7) ? a.K[6] : a.K[stage - 1];
+        Ko += 2 * off;
+#pragma unroll
+        for (int e = 0; e < EH; ++e) {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int i = q == 0 ? P::kl(tid, e) : P::kh(tid, e);
+                C fo = cmk<C>((T)0, (T)0);
+                if (R.src) {
+                    C gx = a.gamx[off + i], gs = a.gamxs[off + i];
+                    if (R.prop == 0) fo = cmk<C>(pv * gx.x - pvs * gs.x, pv * gx.y - pvs * gs.y);
+                    else if (R.prop == 1) fo = cmk<C>(pv * gx.x + pvs * gs.x, pv * gx.y + pvs * gs.y);
+                    else if (R.prop == 2) fo = cmk<C>((T)2 * pv * gx.x, (T)2 * pv * gx.y);
+                }
+                const T al = a.aal[off + i];
+                C m0 = q == 0 ? m0hl[e] : m0hh[e];
+                C oe = q == 0 ? oel[e] : oeh[e];
+                C ou = q == 0 ? oul[e] : ouh[e];
+                C ehv = q == 0 ? ehl[e] : ehh[e];
+                C ke = cscale(cadd(cadd(m0, fo), oe), al);
+                C ku = cscale(cadd(cmul_i<-1>(cscale(ehv, g * kk[i])), ou), al);
+                Ko[i] = ke;
+                Ko[N + i] = ku;
+            }
+        }
+    }
+}
+
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) k1d_rhs(Batch1DArgs<T> a, int stage) {
+    using C = typename ctype<T>::type;
+    extern __shared__ unsigned char hw_lds1d[];
+    __shared__ int sm[64];
+    __shared__ double smd[40];
+    const int r = a.list[blockIdx.x];
+    Rk1D* S = a.rk + r;
+    if (stage >= 1) {
+        if (!S->active) return;
+        if (stage == 1 && !S->need_in) return;
+    }
+    C* lds = reinterpret_cast<C*>(hw_lds1d);
+    switch (a.real[r].n) {
+#define HW_1D_CASE(NN)                                                         \
+    case NN:                                                                   \
+        if constexpr (NN >= 2 * NT && NN <= 16 * NT && NN * sizeof(C) <= 131072) \
+            rhs1d_body<T, NN, NT>(a, r, stage, lds, sm, smd);                  \
+        break;
+        HW_1D_CASE(128) HW_1D_CASE(256) HW_1D_CASE(512) HW_1D_CASE(1024) HW_1D_CASE(2048)
+        HW_1D_CASE(4096) HW_1D_CASE(8192) HW_1D_CASE(16384)
+#undef HW_1D_CASE
+        default: break;
+    }
+}
+
+// ---------------------------------------------------- step control ----
+// Per realization: y_new = y + h0 (c1 K1 + c3 K3 + c4 K4 + c5 K5 + c6 K6),
+// err = h0 (ec1 K1 + ...), rmax = max |err| / (eps_abs + eps_rel |y_new|),
+// then control_y's hadjust and gsl_odeiv_evolve_apply's accept / retry rule;
+// on acceptance y <- y_new, t advances and the next evolve_apply call (which
+// evaluates dydt_in) is armed; the next attempt's step is clipped to t1.
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) k1d_control(Batch1DArgs<T> a) {
+    using C = typename ctype<T>::type;
+    __shared__ double red[20];
+    __shared__ int dec[2];
+    const int r = a.list[blockIdx.x];
+    const Real1D& R = a.real[r];
+    Rk1D* S = a.rk + r;
+    if (!S->active) return;
+    const int n2 = 2 * R.n;
+    const long long o = 2 * R.off;
+    const double h0 = S->h0;
+    const C *K1 = a.K[0] + o, *K3 = a.K[2] + o, *K4 = a.K[3] + o, *K5 = a.K[4] + o, *K6 = a.K[5] + o;
+    double2* y = a.y + o;
+    double rmax = 2.2250738585072014e-308;
+    for (int i = threadIdx.x; i < n2; i += NT) {
+        C k1 = K1[i], k3 = K3[i], k4 = K4[i], k5 = K5[i], k6 = K6[i];
+        double2 yv = y[i];
+        double dxv = c_rk1d[7][0] * (double)k1.x + c_rk1d[7][2] * (double)k3.x + c_rk1d[7][3] * (double)k4.x +
+                     c_rk1d[7][4] * (double)k5.x + c_rk1d[7][5] * (double)k6.x;
+        double dyv = c_rk1d[7][0] * (double)k1.y + c_rk1d[7][2] * (double)k3.y + c_rk1d[7][3] * (double)k4.y +
+                     c_rk1d[7][4] * (double)k5.y + c_rk1d[7][5] * (double)k6.y;
+        double nx = yv.x + h0 * dxv, ny = yv.y + h0 * dyv;
+        double ex = h0 * (c_rk1d_ec[0] * (double)k1.x + c_rk1d_ec[2] * (double)k3.x + c_rk1d_ec[3] * (double)k4.x +
+                          c_rk1d_ec[4] * (double)k5.x + c_rk1d_ec[5] * (double)k6.x);
+        double ey = h0 * (c_rk1d_ec[0] * (double)k1.y + c_rk1d_ec[2] * (double)k3.y + c_rk1d_ec[3] * (double)k4.y +
+                          c_rk1d_ec[4] * (double)k5.y + c_rk1d_ec[5] * (double)k6.y);
+        double rx = fabs(ex) / fabs(R.eps_rel * fabs(nx) + R.eps_abs);
+        double ry = fabs(ey) / fabs(R.eps_rel * fabs(ny) + R.eps_abs);
+        if (rx > rmax) rmax = rx;
+        if (ry > rmax) rmax = ry;
+    }
+    rmax = block_max_d<NT>(rmax, red);
+    if (threadIdx.x == 0) {
+        const double t0 = S->t;
+        double hn = h0;
+        int st = 0;
+        if (rmax > 1.1) {
+            double q = 0.9 / pow(rmax, 1.0 / 5.0);
+            if (q < 0.2) q = 0.2;
+            hn = q * h0;
+            st = -1;
+        } else if (rmax < 0.5) {
+            double q = 0.9 / pow(rmax, 1.0 / 6.0);
+            if (q > 5.0) q = 5.0;
+            if (q < 1.0) q = 1.0;
+            hn = q * h0;
+            st = 1;
+        }
+        const double tn = S->final_step ? S->t1 : t0 + h0;
+        int accept = 1;
+        if (st < 0) {
+            if (fabs(hn) < fabs(h0) && tn + hn != tn) accept = 0;
+            else hn = h0;
+        }
+        if (accept) {
+            S->t = tn;
+            S->h = hn;
+            S->need_in = 1;
+            S->accepted += 1;
+            S->active = tn < S->t1;
+            hn = S->h;
+        } else {
+            S->rejected += 1;
+            S->need_in = 0;
+        }
+        // arm the next attempt: h0 clipped to the remaining interval
+        double dt = S->t1 - S->t;
+        if ((dt >= 0.0 && hn > dt) || (dt < 0.0 && hn < dt)) { S->h0 = dt; S->final_step = 1; }
+        else { S->h0 = hn; S->final_step = 0; }
+        dec[0] = accept;
+    }
+    __syncthreads();
+    if (dec[0]) {
+        for (int i = threadIdx.x; i < n2; i += NT) {
+            C k1 = K1[i], k3 = K3[i], k4 = K4[i], k5 = K5[i], k6 = K6[i];
+            double2 yv = y[i];
+            double dxv = c_rk1d[7][0] * (double)k1.x + c_rk1d[7][2] * (double)k3.x + c_rk1d[7][3] * (double)k4.x +
+                         c_rk1d[7][4] * (double)k5.x + c_rk1d[7][5] * (double)k6.x;
+            double dyv = c_rk1d[7][0] * (double)k1.y + c_rk1d[7][2] * (double)k3.y + c_rk1d[7][3] * (double)k4.y +
+                         c_rk1d[7][4] * (double)k5.y + c_rk1d[7][5] * (double)k6.y;
+            y[i] = make_double2(yv.x + h0 * dxv, yv.y + h0 * dyv);
+        }
+    }
+}
+
+// Start of a driver interval: every realization with t < t1 becomes active,
+// and its first attempt uses the suggested step clipped to t1.
+static __global__ void k1d_begin(Rk1D* rk, int nreal, double t1) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nreal) return;
+    Rk1D* S = rk + r;
+    S->t1 = t1;
+    S->active = S->t < t1;
+    S->need_in = 1;
+    double dt = t1 - S->t, h = S->h;
+    if ((dt >= 0.0 && h > dt) || (dt < 0.0 && h < dt)) { S->h0 = dt; S->final_step = 1; }
+    else { S->h0 = h; S->final_step = 0; }
+}
+
+// active count (for the host's loop condition)
+static __global__ void k1d_count(const Rk1D* rk, int nreal, int* out) {
+    __shared__ int c;
+    if (threadIdx.x == 0) c = 0;
+    __syncthreads();
+    for (int r = threadIdx.x; r < nreal; r += blockDim.x)
+        if (rk[r].active) atomicAdd(&c, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) *out = c;
+}
+
+}  // namespace hw

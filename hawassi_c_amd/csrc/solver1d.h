@@ -1,0 +1,396 @@
+// solver1d.h -- host side of the 1-D path: a batch of independent
+// realizations (uneven n), their device buffers, the fused RHS launches and
+// the device-resident RKF45 driver with per-realization step control.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/hawassi_hip.h"
+#include "rhs1d.h"
+#include "solver2d.h"
+
+namespace hw {
+
+inline void natural_spline_c(const double* x, const double* yv, int n, double* cc) {
+    for (int i = 0; i < n; ++i) cc[i] = 0.0;
+    if (n < 3) return;
+    int m = n - 2;
+    std::vector<double> diag(m), off(m), rhs(m);
+    for (int i = 0; i < m; ++i) {
+        double h0 = x[i + 1] - x[i], h1 = x[i + 2] - x[i + 1];
+        diag[i] = 2.0 * (h0 + h1);
+        off[i] = h1;
+        rhs[i] = 3.0 * ((yv[i + 2] - yv[i + 1]) / h1 - (yv[i + 1] - yv[i]) / h0);
+    }
+    for (int i = 1; i < m; ++i) {
+        double w = off[i - 1] / diag[i - 1];
+        diag[i] -= w * off[i - 1];
+        rhs[i] -= w * rhs[i - 1];
+    }
+    cc[m] = rhs[m - 1] / diag[m - 1];
+    for (int i = m - 2; i >= 0; --i) cc[i + 1] = (rhs[i] - off[i] * cc[i + 2]) / diag[i];
+}
+
+struct Solver1D {
+    int prec = 0, nreal = 0, NT = 0, maxn = 0, any_brk = 0;
+    hipStream_t stream = nullptr;
+    std::vector<Real1D> hreal;
+    long long ntot = 0;
+    virtual ~Solver1D() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    virtual int init(const hw_config_1d* cfg, int n) = 0;
+    virtual int rhs_at(int r, double t, const double* y, double* dydt) = 0;
+    virtual int set_state(int r, const double* y) = 0;
+    virtual int get_state(int r, double* y) = 0;
+    virtual int set_time(int r, double t, double h) = 0;
+    virtual int get_time(int r, double* t, double* h) = 0;
+    virtual int integrate(double t1, long max_rounds) = 0;
+    virtual int breaking_state(int r, double* B, int* flag) = 0;
+    virtual int stats(int r, hw_stats* st) = 0;
+};
+
+// the RHS kernel instances available to a launch (one per NT)
+template <typename T> struct K1D {
+    static hipError_t launch(int NT, dim3 grid, size_t lds, hipStream_t s, const Batch1DArgs<T>& a, int stage);
+    static hipError_t control(int NT, dim3 grid, hipStream_t s, const Batch1DArgs<T>& a);
+    static hipError_t set_lds(int NT, size_t bytes);
+};
+
+template <typename T> struct Solver1DT : Solver1D {
+    using C = typename ctype<T>::type;
+    DevBuf<Real1D> d_real;
+    DevBuf<Rk1D> d_rk;
+    DevBuf<int> d_list, d_count;
+    DevBuf<T> k, aal, chi, damp, opa, opb, gm, gp, depth, sr, B;
+    DevBuf<C> gamx, gamxs, Kb[7], sc;
+    DevBuf<double> tsig, sig, sigc, skw, skwc;
+    DevBuf<double2> y, ext;
+    DevBuf<int> peaks, ide;
+    DevBuf<Cb1> cb, cbn, cbp;
+    DevBuf<Brk1D> bs;
+    DevBuf<C> tw[16];
+    Batch1DArgs<T> args{};
+    int* h_count = nullptr;
+
+    ~Solver1DT() override {
+        if (h_count) (void)hipHostFree(h_count);
+    }
+
+    static int ilog2(int n) { int l = 0; while ((1 << l) < n) ++l; return l; }
+
+    int init(const hw_config_1d* cfg, int n) override {
+        nreal = n;
+        prec = sizeof(T) == 4 ? HW_PREC_FP32 : HW_PREC_FP64;
+        const int nmax_prec = sizeof(T) == 4 ? 16384 : 8192;
+        int nmin = 1 << 30;
+        long long cbtot = 0, sigtot = 0;
+        hreal.resize(n);
+        for (int r = 0; r < n; ++r) {
+            const hw_config_1d& c = cfg[r];
+            int nn = c.n;
+            if (nn < 128 || nn > nmax_prec || (nn & (nn - 1))) {
+                set_error("realization %d: n = %d unsupported (powers of two 128..%d)", r, nn, nmax_prec);
+                return HW_ERR_ARG;
+            }
+            if (!c.k || !c.aal || !c.chi || !c.damp || !c.depth) { set_error("realization %d: null array", r); return HW_ERR_ARG; }
+            if (c.flat ? (!c.csqr || !c.lop) : (!c.csq_min || !c.csq_plus || !c.gam_min || !c.gam_plus)) {
+                set_error("realization %d: operator arrays missing", r);
+                return HW_ERR_ARG;
+            }
+            if (c.source_on && (c.nt < 2 || !c.t_sig || !c.sig || !c.sig_skew || !c.gamx_re || !c.gamx_im ||
+                                !c.gamx_skew_re || !c.gamx_skew_im)) {
+                set_error("realization %d: influx arrays missing", r);
+                return HW_ERR_ARG;
+            }
+            Real1D& R = hreal[r];
+            std::memset(&R, 0, sizeof(R));
+            R.n = nn; R.flat = c.flat; R.src = c.source_on; R.prop = c.propagation; R.brk = c.breaking_on;
+            R.nt = c.source_on ? c.nt : 0;
+            R.cap = nn / 4 < 1024 ? nn / 4 : 1024;
+            R.log2n = ilog2(nn);
+            R.off = ntot;
+            R.sig_off = sigtot;
+            R.cb_off = cbtot;
+            R.dx = c.dx; R.KBC = c.KBC; R.TC = c.TC; R.Tstar = c.Tstar; R.MaxEtaInit = c.max_eta_init;
+            R.delb = c.delb; R.eps_abs = c.eps_abs; R.eps_rel = c.eps_rel;
+            R.xbstart = c.xbstart; R.xbend = c.xbend; R.xbendL = c.xbendL; R.xbstartR = c.xbstartR;
+            R.lenIF = c.lenIF; R.ILL = c.ILL; R.IFF = c.IFF;
+            if (c.breaking_on) {
+                int hi = R.xbstart + (R.xbendL == 0 && R.xbstartR == 0 ? R.lenIF : R.ILL);
+                int hi2 = R.xbstartR + R.IFF;
+                if (R.xbstart < 0 || R.xbend >= nn || hi > nn || hi2 > nn || R.xbstartR < 0 || c.dx <= 0) {
+                    set_error("realization %d: breaking window outside the grid", r);
+                    return HW_ERR_ARG;
+                }
+            }
+            ntot += nn;
+            sigtot += R.nt;
+            cbtot += R.cap;
+            if (nn < nmin) nmin = nn;
+            if (nn > maxn) maxn = nn;
+            any_brk |= c.breaking_on;
+        }
+        NT = nmin / 2 < 1024 ? nmin / 2 : 1024;
+        if (maxn > 16 * NT) {
+            set_error("sizes %d..%d cannot share one launch (largest / smallest must be <= 32)", nmin, maxn);
+            return HW_ERR_ARG;
+        }
+        HW_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        // ---- per-element arrays
+        std::vector<T> hk(ntot), haal(ntot), hchi(ntot), hdamp(ntot), hopa(ntot), hopb(ntot), hgm(ntot), hgp(ntot),
+            hdep(ntot);
+        std::vector<C> hgx(ntot), hgs(ntot);
+        std::vector<double> hts(sigtot), hsig(sigtot), hsigc(sigtot), hskw(sigtot), hskwc(sigtot);
+        for (int r = 0; r < n; ++r) {
+            const hw_config_1d& c = cfg[r];
+            const Real1D& R = hreal[r];
+            for (int i = 0; i < R.n; ++i) {
+                size_t q = R.off + i;
+                hk[q] = (T)c.k[i]; haal[q] = (T)c.aal[i]; hchi[q] = (T)c.chi[i]; hdamp[q] = (T)c.damp[i];
+                hdep[q] = (T)c.depth[i];
+                if (c.flat) { hopa[q] = (T)c.csqr[i]; hopb[q] = (T)c.lop[i]; hgm[q] = 0; hgp[q] = 0; }
+                else {
+                    hopa[q] = (T)c.csq_min[i]; hopb[q] = (T)c.csq_plus[i];
+                    hgm[q] = (T)c.gam_min[i]; hgp[q] = (T)c.gam_plus[i];
+                }
+                if (c.source_on) {
+                    hgx[q].x = (T)c.gamx_re[i]; hgx[q].y = (T)c.gamx_im[i];
+                    hgs[q].x = (T)c.gamx_skew_re[i]; hgs[q].y = (T)c.gamx_skew_im[i];
+                } else {
+                    hgx[q].x = hgx[q].y = hgs[q].x = hgs[q].y = (T)0;
+                }
+            }
+            if (R.nt) {
+                std::memcpy(&hts[R.sig_off], c.t_sig, sizeof(double) * R.nt);
+                std::memcpy(&hsig[R.sig_off], c.sig, sizeof(double) * R.nt);
+                std::memcpy(&hskw[R.sig_off], c.sig_skew, sizeof(double) * R.nt);
+                natural_spline_c(c.t_sig, c.sig, R.nt, &hsigc[R.sig_off]);
+                natural_spline_c(c.t_sig, c.sig_skew, R.nt, &hskwc[R.sig_off]);
+            }
+        }
+        int rc;
+        if ((rc = k.upload(hk, stream)) || (rc = aal.upload(haal, stream)) || (rc = chi.upload(hchi, stream)) ||
+            (rc = damp.upload(hdamp, stream)) || (rc = opa.upload(hopa, stream)) || (rc = opb.upload(hopb, stream)) ||
+            (rc = gm.upload(hgm, stream)) || (rc = gp.upload(hgp, stream)) || (rc = depth.upload(hdep, stream)) ||
+            (rc = gamx.upload(hgx, stream)) || (rc = gamxs.upload(hgs, stream)))
+            return rc;
+        if (sigtot) {
+            if ((rc = tsig.upload(hts, stream)) || (rc = sig.upload(hsig, stream)) || (rc = sigc.upload(hsigc, stream)) ||
+                (rc = skw.upload(hskw, stream)) || (rc = skwc.upload(hskwc, stream)))
+                return rc;
+        }
+        // ---- twiddles exp(-2 pi i m / n) for every size present
+        for (int r = 0; r < n; ++r) {
+            int l = hreal[r].log2n;
+            if (tw[l].p) continue;
+            int nn = 1 << l;
+            std::vector<C> h(nn);
+            for (int m = 0; m < nn; ++m) {
+                double ang = -2.0 * M_PI * (double)m / (double)nn;
+                h[m].x = (T)std::cos(ang);
+                h[m].y = (T)std::sin(ang);
+            }
+            if ((rc = tw[l].upload(h, stream))) return rc;
+        }
+        // ---- state, stages, scratch, breaking
+        std::vector<Rk1D> hrk(n);
+        for (int r = 0; r < n; ++r) {
+            std::memset(&hrk[r], 0, sizeof(Rk1D));
+            hrk[r].t = cfg[r].t0;
+            hrk[r].h = cfg[r].h0 > 0 ? cfg[r].h0 : 1e-4;
+        }
+        std::vector<int> hl(n);
+        for (int r = 0; r < n; ++r) hl[r] = r;
+        std::vector<Brk1D> hb(n);
+        std::memset(hb.data(), 0, sizeof(Brk1D) * n);
+        if ((rc = d_real.upload(hreal, stream)) || (rc = d_rk.upload(hrk, stream)) || (rc = d_list.upload(hl, stream)) ||
+            (rc = d_count.alloc(1)) || (rc = y.alloc(2 * ntot)) || (rc = ext.alloc(2 * (size_t)maxn)) ||
+            (rc = sr.alloc(R1_NR * ntot)) || (rc = sc.alloc(C1_NC * ntot)) || (rc = B.alloc(ntot)) ||
+            (rc = peaks.alloc(ntot)) || (rc = ide.alloc(cbtot)) || (rc = cb.alloc(cbtot)) || (rc = cbn.alloc(cbtot)) ||
+            (rc = cbp.alloc(cbtot)) || (rc = bs.upload(hb, stream)))
+            return rc;
+        for (int j = 0; j < 7; ++j)
+            if ((rc = Kb[j].alloc(2 * ntot))) return rc;
+        HW_TRY(hipMemsetAsync(y.p, 0, sizeof(double2) * 2 * ntot, stream));
+        HW_TRY(hipMemsetAsync(B.p, 0, sizeof(T) * ntot, stream));
+        HW_TRY(hipMemsetAsync(cb.p, 0, sizeof(Cb1) * cbtot, stream));
+        HW_TRY(hipMemsetAsync(cbn.p, 0, sizeof(Cb1) * cbtot, stream));
+        HW_TRY(hipMemsetAsync(cbp.p, 0, sizeof(Cb1) * cbtot, stream));
+        HW_TRY(hipMemsetAsync(ide.p, 0, sizeof(int) * cbtot, stream));
+        HW_TRY(hipHostMalloc((void**)&h_count, sizeof(int), hipHostMallocDefault));
+        HW_TRY(K1D<T>::set_lds(NT, sizeof(C) * maxn));
+        // ---- kernel arguments
+        Batch1DArgs<T>& a = args;
+        a.real = d_real.p; a.rk = d_rk.p; a.list = d_list.p;
+        a.k = k.p; a.aal = aal.p; a.chi = chi.p; a.damp = damp.p; a.opa = opa.p; a.opb = opb.p;
+        a.gm = gm.p; a.gp = gp.p; a.depth = depth.p; a.gamx = gamx.p; a.gamxs = gamxs.p;
+        a.tsig = tsig.p; a.sig = sig.p; a.sigc = sigc.p; a.skw = skw.p; a.skwc = skwc.p;
+        a.y = y.p;
+        for (int j = 0; j < 7; ++j) a.K[j] = Kb[j].p;
+        a.ext = ext.p;
+        a.sr = sr.p; a.sc = sc.p; a.B = B.p; a.peaks = peaks.p; a.cb = cb.p; a.cbn = cbn.p; a.cbp = cbp.p;
+        a.ide = ide.p; a.bs = bs.p;
+        for (int l = 0; l < 16; ++l) a.tw[l] = tw[l].p;
+        a.g = (T)9.80665;
+        HW_TRY(hipStreamSynchronize(stream));
+        return HW_OK;
+    }
+
+    int check_r(int r) {
+        if (r < 0 || r >= nreal) { set_error("realization %d out of range (batch of %d)", r, nreal); return HW_ERR_ARG; }
+        return HW_OK;
+    }
+
+    // 4-plane host layout <-> [eta^][u^] double2
+    int rhs_at(int r, double t, const double* yin, double* dydt) override {
+        int rc = check_r(r);
+        if (rc) return rc;
+        const int n = hreal[r].n;
+        std::vector<double2> h(2 * n);
+        for (int i = 0; i < n; ++i) {
+            h[i] = make_double2(yin[i], yin[n + i]);
+            h[n + i] = make_double2(yin[2 * n + i], yin[3 * n + i]);
+        }
+        HW_TRY(hipMemcpyAsync(ext.p + 0, h.data(), sizeof(double2) * 2 * n, hipMemcpyHostToDevice, stream));
+        // the stand-alone input sits at the realization's own offset in ext
+        Batch1DArgs<T> a = args;
+        a.ext = ext.p - 2 * hreal[r].off;
+        a.t_ext = t;
+        int* dl = d_list.p + r;
+        a.list = dl;
+        HW_TRY(K1D<T>::launch(NT, dim3(1), sizeof(C) * maxn, stream, a, 0));
+        std::vector<C> o(2 * n);
+        HW_TRY(hipMemcpyAsync(o.data(), Kb[6].p + 2 * hreal[r].off, sizeof(C) * 2 * n, hipMemcpyDeviceToHost, stream));
+        HW_TRY(hipStreamSynchronize(stream));
+        for (int i = 0; i < n; ++i) {
+            dydt[i] = (double)o[i].x; dydt[n + i] = (double)o[i].y;
+            dydt[2 * n + i] = (double)o[n + i].x; dydt[3 * n + i] = (double)o[n + i].y;
+        }
+        return HW_OK;
+    }
+
+    int set_state(int r, const double* yin) override {
+        int rc = check_r(r);
+        if (rc) return rc;
+        const int n = hreal[r].n;
+        std::vector<double2> h(2 * n);
+        for (int i = 0; i < n; ++i) {
+            h[i] = make_double2(yin[i], yin[n + i]);
+            h[n + i] = make_double2(yin[2 * n + i], yin[3 * n + i]);
+        }
+        HW_TRY(hipMemcpyAsync(y.p + 2 * hreal[r].off, h.data(), sizeof(double2) * 2 * n, hipMemcpyHostToDevice, stream));
+        HW_TRY(hipStreamSynchronize(stream));
+        return HW_OK;
+    }
+
+    int get_state(int r, double* yout) override {
+        int rc = check_r(r);
+        if (rc) return rc;
+        const int n = hreal[r].n;
+        std::vector<double2> h(2 * n);
+        HW_TRY(hipMemcpyAsync(h.data(), y.p + 2 * hreal[r].off, sizeof(double2) * 2 * n, hipMemcpyDeviceToHost, stream));
+        HW_TRY(hipStreamSynchronize(stream));
+        for (int i = 0; i < n; ++i) {
+            yout[i] = h[i].x; yout[n + i] = h[i].y;
+            yout[2 * n + i] = h[n + i].x; yout[3 * n + i] = h[n + i].y;
+        }
+        return HW_OK;
+    }
+
+    int set_time(int r, double t, double h) override {
+        int rc = check_r(r);
+        if (rc) return rc;
+        Rk1D s;
+        HW_TRY(hipMemcpyAsync(&s, d_rk.p + r, sizeof(Rk1D), hipMemcpyDeviceToHost, stream));
+        HW_TRY(hipStreamSynchronize(stream));
+        s.t = t;
+        s.h = h;
+        HW_TRY(hipMemcpyAsync(d_rk.p + r, &s, sizeof(Rk1D), hipMemcpyHostToDevice, stream));
+        HW_TRY(hipStreamSynchronize(stream));
+        return HW_OK;
+    }
+
+    int get_time(int r, double* t, double* h) override {
+        int rc = check_r(r);
+        if (rc) return rc;
+        Rk1D s;
+        HW_TRY(hipMemcpyAsync(&s, d_rk.p + r, sizeof(Rk1D), hipMemcpyDeviceToHost, stream));
+        HW_TRY(hipStreamSynchronize(stream));
+        if (t) *t = s.t;
+        if (h) *h = s.h;
+        return HW_OK;
+    }
+
+    // every realization: while (t < t1) gsl_odeiv_evolve_apply (ode_solver_Ham's
+    // inner loop, 1d/functions.c:546-552).  Each round is one attempt of
+    // every active realization: dydt_in where needed, five stages, the
+    // stepper's dydt_out evaluation (which matters only through breaking),
+    // then the error norm and the accept / retry decision.
+    int integrate(double t1, long max_rounds) override {
+        k1d_begin<<<(nreal + 255) / 256, 256, 0, stream>>>(d_rk.p, nreal, t1);
+        HW_LAUNCH_CHECK();
+        const size_t lds = sizeof(C) * maxn;
+        for (long round = 0; max_rounds <= 0 || round < max_rounds; ++round) {
+            for (int s = 1; s <= 6; ++s) HW_TRY(K1D<T>::launch(NT, dim3(nreal), lds, stream, args, s));
+            if (any_brk) HW_TRY(K1D<T>::launch(NT, dim3(nreal), lds, stream, args, 7));
+            HW_TRY(K1D<T>::control(NT, dim3(nreal), stream, args));
+            k1d_count<<<1, 256, 0, stream>>>(d_rk.p, nreal, d_count.p);
+            HW_LAUNCH_CHECK();
+            HW_TRY(hipMemcpyAsync(h_count, d_count.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+            HW_TRY(hipStreamSynchronize(stream));
+            if (*h_count == 0) return check_overflow();
+        }
+        set_error("integrate: %ld rounds without reaching t1 = %g", max_rounds, t1);
+        return HW_ERR_STEP;
+    }
+
+    int check_overflow() {
+        std::vector<Brk1D> hb(nreal);
+        HW_TRY(hipMemcpyAsync(hb.data(), bs.p, sizeof(Brk1D) * nreal, hipMemcpyDeviceToHost, stream));
+        HW_TRY(hipStreamSynchronize(stream));
+        for (int r = 0; r < nreal; ++r)
+            if (hb[r].overflow) {
+                set_error("realization %d: more breaking crests than the crest list holds (%d)", r, hreal[r].cap);
+                return HW_ERR_ARG;
+            }
+        return HW_OK;
+    }
+
+    int breaking_state(int r, double* Bout, int* flag) override {
+        int rc = check_r(r);
+        if (rc) return rc;
+        const int n = hreal[r].n;
+        if (Bout) {
+            std::vector<T> h(n);
+            HW_TRY(hipMemcpyAsync(h.data(), B.p + hreal[r].off, sizeof(T) * n, hipMemcpyDeviceToHost, stream));
+            HW_TRY(hipStreamSynchronize(stream));
+            for (int i = 0; i < n; ++i) Bout[i] = (double)h[i];
+        }
+        if (flag) {
+            Brk1D b;
+            HW_TRY(hipMemcpyAsync(&b, bs.p + r, sizeof(Brk1D), hipMemcpyDeviceToHost, stream));
+            HW_TRY(hipStreamSynchronize(stream));
+            *flag = b.flag;
+        }
+        return check_overflow();
+    }
+
+    int stats(int r, hw_stats* st) override {
+        std::vector<Rk1D> h(nreal);
+        HW_TRY(hipMemcpyAsync(h.data(), d_rk.p, sizeof(Rk1D) * nreal, hipMemcpyDeviceToHost, stream));
+        HW_TRY(hipStreamSynchronize(stream));
+        *st = hw_stats{};
+        for (int q = 0; q < nreal; ++q) {
+            if (r >= 0 && q != r) continue;
+            st->rhs_evals += h[q].rhs;
+            st->steps_accepted += h[q].accepted;
+            st->steps_rejected += h[q].rejected;
+        }
+        return HW_OK;
+    }
+};
+
+}  // namespace hw

@@ -145,6 +145,63 @@ int hw_sync(void *ctx);
 int hw_fft2_forward(int nx, int ny, int precision, const double *phys, double *spec_re, double *spec_im);
 int hw_fft2_inverse(int nx, int ny, int precision, const double *spec_re, const double *spec_im, double *phys, double *nyq_im);
 
+/* ------------------------------------------------------------------ 1-D --
+ * The 1-D HS2 path (rhs_HsF2, 1d/rhs.c:96-241) for a batch of independent
+ * realizations of possibly different n.  State layout per realization: 4
+ * planes of n doubles [Re eta^][Im eta^][Re u^][Im u^] of the full complex
+ * spectrum (1d/functions.c:537-538, set_initial_ode). */
+typedef struct hw_config_1d {
+    int n;                    /* points: power of two, 128..16384 (FP32) or 128..8192 (FP64) */
+    int flat;                 /* 1: flat bottom (Csqr, L), 0: two reference depths (Up2IP + HSS) */
+    int source_on;            /* influx forcing (define_forcing, 1d/rhs.c:659-693) */
+    int propagation;          /* 0 "Uni+", 1 "Uni-", 2 "Bi", 3 none */
+    int breaking_on;          /* rhs_breaking + breaking_process (1d/rhs.c:617-657, 1d/breaking.c) */
+    double dx;
+    const double *k;          /* freqspace wavenumbers [n] (1d/functions.c:1173-1197) */
+    const double *aal;        /* alias mask [n] (1d/functions.c:1773-1784) */
+    const double *chi;        /* ChiAdj [n] */
+    const double *damp;       /* dampcoef * dampchar [n] (1d/x_space.c:20-33) */
+    const double *csqr, *lop; /* flat: Cp(k, depthinf)^2, k^2 Cp^2 / g [n] */
+    const double *csq_min, *csq_plus, *gam_min, *gam_plus;   /* two depths [n] (Up2IP, 1d/functions.c:2195-2245) */
+    const double *gamx_re, *gamx_im, *gamx_skew_re, *gamx_skew_im;   /* gamX, gamX_skew [n] */
+    int nt;                   /* influx samples */
+    const double *t_sig, *sig, *sig_skew;   /* insig, insig_skew at t_sig (natural spline in t) */
+    double KBC, TC, Tstar, max_eta_init, delb;              /* parambreak (1d/breaking.c:1-34) */
+    int xbstart, xbend, xbendL, xbstartR, lenIF, ILL, IFF;  /* breaking search windows */
+    const double *depth;      /* positive depth [n] */
+    double eps_abs, eps_rel;  /* control_y_new (1d/functions.c:529-534: 1e-5, 1e-4) */
+    double t0, h0;            /* initial time and step (1d/functions.c:540-541: h = 1e-4) */
+} hw_config_1d;
+
+/* Build a batch of nreal realizations; precision applies to all. */
+int hw1d_create(const hw_config_1d *cfgs, int nreal, int precision, void **ctx);
+void hw1d_destroy(void *ctx);
+
+/* GSL odeiv system callback for realization 0: int f(double t, const double
+ * y[], double dydt[], void *params) registered as {rhs_HsF2, NULL, 4*Nx,
+ * NULL} (1d/functions.c:75-76).  Breaking state advances as in the
+ * reference (every call, 1d/rhs.c:199-201). */
+int hw1d_rhs(double t, const double *y, double *dydt, void *ctx);
+/* Same for realization r. */
+int hw1d_rhs_at(void *ctx, int r, double t, const double *y, double *dydt);
+
+int hw1d_set_state(void *ctx, int r, const double *y);
+int hw1d_get_state(void *ctx, int r, double *y);
+int hw1d_set_time(void *ctx, int r, double t, double h);
+int hw1d_get_time(void *ctx, int r, double *t, double *h);
+
+/* Every realization: while (t < t1) gsl_odeiv_evolve_apply with rkf45 and
+ * control_y (ode_solver_Ham's inner loop, 1d/functions.c:546-552), all on
+ * the device; each realization keeps its own t and step. */
+int hw1d_integrate(void *ctx, double t1);
+
+/* B [n] of the last right-hand side evaluated for realization r and the
+ * breaking flag (1d/breaking.c:212-450); either pointer may be NULL. */
+int hw1d_get_breaking(void *ctx, int r, double *B, int *flag);
+
+/* Counters of realization r, or summed over the batch for r = -1. */
+int hw1d_get_stats(void *ctx, int r, hw_stats *st);
+
 /* Sizes this build supports (1 if (nx, ny, precision) has compiled kernels). */
 int hw_supported(int nx, int ny, int precision);
 

@@ -1,0 +1,628 @@
+/*
+ * hawassi_oracle1d.c -- FP64 CPU restatement of the reference's 1-D HS2 path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, the smoke check and bench.py's CPU
+ * baseline load it, as the checker; the product never links or calls it.
+ *
+ * Restated from the reference (read as text, written in this project's own
+ * words):
+ *   rhs_HsF2                      1d/rhs.c:96-241        -> orc1d_rhs
+ *   define_delH1                  1d/functions.c:3414-3438
+ *   ifft_fft_Hsf2 / _v            1d/functions.c:1458-1526, 3334-3412
+ *   HSS (two reference depths)    1d/functions.c:2861-2927 (the mid[0] != 3 part)
+ *   Op_M0 / Op_M1                 1d/functions.c:2979-3016
+ *   adjust_delH                   1d/rhs.c:695-715
+ *   define_forcing                1d/rhs.c:659-693
+ *   rhs_breaking                  1d/rhs.c:617-657
+ *   breaking_criterion / _process 1d/breaking.c:63-450
+ *   gradient                      1d/functions.c:1850-1856
+ *   ifft_complex_real_nx / fft_real_complex_nx  1d/functions.c:1642-1661, 1745-1769
+ *   gsl_odeiv_evolve_apply + rkf45 + control_y (GSL's documented algorithm,
+ *   with the dydt_out evaluation of rkf45) driven as ode_solver_Ham
+ *   (1d/functions.c:528-552)
+ *
+ * The state is the reference's: 4 planes of N doubles, [Re eta^][Im eta^]
+ * [Re u^][Im u^] of the full complex spectrum (1d/functions.c:537-538).
+ * Every inverse transform is the real part of the full complex backward
+ * transform divided by N (no Hermitian projection in 1-D).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <omp.h>
+
+#include "orc_common.h"
+
+#define ORC_G 9.80665
+
+typedef struct {
+    int N;
+    int flat;            /* 1: flat bottom (Csqr, L), 0: two reference depths */
+    int source_on;
+    int propagation;     /* 0 Uni+, 1 Uni-, 2 Bi, 3 None */
+    int breaking_on;
+    const double *k;     /* freqspace wavenumbers [N] */
+    const double *aal;   /* anti-alias mask [N] */
+    const double *chi;   /* ChiAdj [N] */
+    const double *damp;  /* dampcoef * dampchar [N] */
+    const double *csqr;  /* flat: Cp(k, depthinf)^2 [N] */
+    const double *Lop;   /* flat: k^2 Cp^2 / g [N] (operator_L) */
+    const double *csq_min, *csq_plus, *gam_min, *gam_plus; /* two-depth operator */
+    const double *gamx, *gamx_skew;   /* influx kernels, interleaved complex [N] */
+    int nt;
+    const double *t_sig, *sig, *sig_skew;
+    /* breaking (parambreak, 1d/breaking.c:1-34) */
+    double KBC, TC, Tstar, MaxEtaInit, delb, dx;
+    int xbstart, xbend, xbendL, xbstartR, lenIF, ILL, IFF;
+    const double *depth, *x;
+} orc1d_params;
+
+typedef struct {
+    int Bindex, BindexNow;
+    double Ucrest;
+    int DirProp;
+    double Bposition, Btime;
+    int Bcount;
+} orc1d_cb;
+
+typedef struct {
+    orc1d_params p;
+    fftplan *plan;
+    double *sig_c, *skew_c;
+    /* breaking state (1d/header.h globals) */
+    int flag, ncap;
+    orc1d_cb *CB, *CBNew, *CBPrev;
+    int *ID_Node_end;
+    double *B;
+    double *Rb_hat;    /* interleaved complex [N], kept between calls */
+    long nrhs;
+    /* rk work */
+    size_t dim;
+    double *k[6], *ytmp, *y0, *yerr, *dydt_in, *dydt_out;
+} orc1d_ctx;
+
+static double *cz(size_t n) { return (double *)calloc(2 * n, sizeof(double)); }
+
+/* ifft_complex_real_nx: Re(backward DFT)/N */
+static void ifft_re(const orc1d_ctx *c, const double *Xh, double *out)
+{
+    int N = c->p.N;
+    double *a = (double *)malloc(sizeof(double) * 2 * N);
+    memcpy(a, Xh, sizeof(double) * 2 * N);
+    fft1(c->plan, a, +1);
+    for (int i = 0; i < N; i++) out[i] = a[2 * i] / (double)N;
+    free(a);
+}
+
+/* fft_real_complex_nx: forward DFT of a real field */
+static void fft_re(const orc1d_ctx *c, const double *in, double *Xh)
+{
+    int N = c->p.N;
+    for (int i = 0; i < N; i++) { Xh[2 * i] = in[i]; Xh[2 * i + 1] = 0.0; }
+    fft1(c->plan, Xh, -1);
+}
+
+/* HSS with two reference depths (1d/functions.c:2861-2927 without the mid part) */
+static void hss(const orc1d_ctx *c, const double *uh, double *res)
+{
+    const orc1d_params *p = &c->p;
+    int N = p->N;
+    double *cm = cz(N), *cp = cz(N), *u = malloc(sizeof(double) * N), *am = malloc(sizeof(double) * N),
+           *ap = malloc(sizeof(double) * N), *f = malloc(sizeof(double) * N);
+    double *h1m = cz(N), *h1p = cz(N), *fgm = cz(N), *fgp = cz(N);
+    for (int i = 0; i < N; i++) {
+        cm[2 * i] = p->csq_min[i] * uh[2 * i]; cm[2 * i + 1] = p->csq_min[i] * uh[2 * i + 1];
+        cp[2 * i] = p->csq_plus[i] * uh[2 * i]; cp[2 * i + 1] = p->csq_plus[i] * uh[2 * i + 1];
+    }
+    ifft_re(c, uh, u);
+    ifft_re(c, cm, am);
+    ifft_re(c, cp, ap);
+    for (int i = 0; i < N; i++) f[i] = p->gam_min[i] * am[i];
+    fft_re(c, f, h1m);
+    for (int i = 0; i < N; i++) f[i] = p->gam_plus[i] * ap[i];
+    fft_re(c, f, h1p);
+    for (int i = 0; i < N; i++) f[i] = p->gam_min[i] * u[i];
+    fft_re(c, f, fgm);
+    for (int i = 0; i < N; i++) f[i] = p->gam_plus[i] * u[i];
+    fft_re(c, f, fgp);
+    for (int i = 0; i < N; i++)
+        for (int r = 0; r < 2; r++)
+            res[2 * i + r] = (h1m[2 * i + r] + h1p[2 * i + r] + p->csq_min[i] * fgm[2 * i + r] + p->csq_plus[i] * fgp[2 * i + r]) / 2;
+    free(cm); free(cp); free(u); free(am); free(ap); free(f); free(h1m); free(h1p); free(fgm); free(fgp);
+}
+
+/* gradient (1d/functions.c:1850-1856): one-sided at the ends, central inside */
+static void gradient1(const double *a, double d, int n, double *g)
+{
+    g[0] = (a[1] - a[0]) / d;
+    g[n - 1] = (a[n - 1] - a[n - 2]) / d;
+    for (int i = 1; i < n - 1; i++) g[i] = (a[i + 1] - a[i - 1]) / (2 * d);
+}
+
+static double phase_velocity(double k, double d)
+{
+    if (k == 0) return sqrt(ORC_G * d);
+    return (k < 0 ? -1.0 : 1.0) * sqrt(ORC_G * k * tanh(d * k)) / k;
+}
+
+static void reset_cb(orc1d_cb *a, int n)
+{
+    /* set_init_CB (1d/functions.c:2057-2066) leaves BindexNow alone */
+    for (int j = 0; j < n; j++) {
+        a[j].Bindex = 0; a[j].Ucrest = 0; a[j].DirProp = 0; a[j].Bposition = 0; a[j].Btime = 0; a[j].Bcount = 0;
+    }
+}
+
+/* findpeaks (1d/breaking.c:36-61): out[0] = count, out[1..] = indices */
+static int findpeaks(const orc1d_ctx *c, const double *a, int len, int *locs)
+{
+    int n = 0;
+    for (int j = 1; j < len - 1; j++) {
+        if (a[j] < c->p.MaxEtaInit) continue;
+        if (a[j] > a[j + 1] && a[j] > a[j - 1]) locs[n++] = j;
+    }
+    return n;
+}
+
+/* breaking_criterion (1d/breaking.c:63-198) */
+static void criterion(orc1d_ctx *c, double t, const double *eta, const double *eh, const double *u)
+{
+    const orc1d_params *p = &c->p;
+    int N = p->N;
+    int *indj = (int *)malloc(sizeof(int) * (N + 1));
+    int npk = 0, flaghere = 0;
+    int any = 0;
+    for (int i = 0; i < p->lenIF; i++)
+        if (eta[p->xbstart + i] > p->MaxEtaInit) { any = 1; break; }
+    if (any) {
+        if (p->xbendL == 0 && p->xbstartR == 0) {
+            npk = findpeaks(c, eta + p->xbstart, p->lenIF, indj);
+            for (int i = 0; i < npk; i++) indj[i] += p->xbstart;
+        } else {
+            int *l1 = (int *)malloc(sizeof(int) * (N + 1)), *l2 = (int *)malloc(sizeof(int) * (N + 1));
+            int n1 = findpeaks(c, eta + p->xbstart, p->ILL, l1);
+            int n2 = findpeaks(c, eta + p->xbstartR, p->IFF, l2);
+            for (int i = 0; i < n1; i++) indj[npk++] = l1[i] + p->xbstart;
+            for (int i = 0; i < n2; i++) indj[npk++] = l2[i] + p->xbstartR;
+            free(l1); free(l2);
+        }
+        flaghere = npk > 0;
+    }
+    if (flaghere) {
+        double *hx = cz(N), *H = malloc(sizeof(double) * N), *gH = malloc(sizeof(double) * N),
+               *ge = malloc(sizeof(double) * N);
+        for (int i = 0; i < N; i++) {
+            if (p->k[i] == 0) { hx[2 * i] = 0; hx[2 * i + 1] = 0; }
+            else {
+                double s = p->k[i] / fabs(p->k[i]);
+                hx[2 * i] = s * eh[2 * i + 1];
+                hx[2 * i + 1] = -s * eh[2 * i];
+            }
+        }
+        ifft_re(c, hx, H);
+        gradient1(H, p->dx, N, gH);
+        gradient1(eta, p->dx, N, ge);
+        int it = 0;
+        orc1d_cb *tmp = (orc1d_cb *)calloc(npk > 0 ? npk : 1, sizeof(orc1d_cb));
+        for (int i = 0; i < npk; i++) {
+            int J = indj[i];
+            double Kx = (eta[J] * gH[J] - H[J] * ge[J]) / (eta[J] * eta[J] + H[J] * H[J]);
+            double Cc = phase_velocity(Kx, p->depth[J] + eta[J]);
+            double U = u[J];
+            if (fabs(U) > p->KBC * fabs(Cc)) {
+                tmp[it].Bindex = J; tmp[it].BindexNow = J; tmp[it].Ucrest = fabs(U);
+                tmp[it].DirProp = (int)(U / fabs(U)); tmp[it].Bposition = p->x[J]; tmp[it].Btime = t;
+                it++;
+            }
+        }
+        for (int j = 0; j < it && j < c->ncap; j++) { c->CB[j] = tmp[j]; c->CB[j].Bcount = it; }
+        free(tmp); free(hx); free(H); free(gH); free(ge);
+    } else {
+        reset_cb(c->CB, c->ncap);
+    }
+    free(indj);
+}
+
+static int idx_max(const double *a, int n)
+{
+    /* idx_max (1d/functions.c:1282-1294); n == 0 returns 0 */
+    int idx = 0;
+    if (n <= 0) return 0;
+    double tmp = a[0];
+    for (int i = 0; i < n; i++) if (a[i] > tmp) { tmp = a[i]; idx = i; }
+    return idx;
+}
+
+/* breaking_process (1d/breaking.c:200-450) */
+static void breaking_process(orc1d_ctx *c, double t, const double *eta, const double *eh, const double *u)
+{
+    const orc1d_params *p = &c->p;
+    int N = p->N, cap = c->ncap;
+    orc1d_cb *CB = c->CB, *New = c->CBNew, *Prev = c->CBPrev;
+    int *IDE = c->ID_Node_end;
+    double *B = c->B;
+    for (int j = 0; j < N; j++) B[j] = 0;
+    criterion(c, t, eta, eh, u);
+    int NCB = CB[0].Bcount;
+    if (NCB > 0 && c->flag == 0) {
+        c->flag = 1;
+        for (int j = 0; j < NCB; j++) { int bn = New[j].BindexNow; New[j] = CB[j]; New[j].BindexNow = CB[j].BindexNow; (void)bn; }
+    } else if (NCB == 0 && c->flag == 2) {
+        int np = Prev[0].Bcount;
+        for (int j = 0; j < np; j++) New[j] = Prev[j];
+    } else if (NCB > 0 && c->flag == 2) {
+        int np = Prev[0].Bcount;
+        for (int j = 0; j < np; j++) New[j] = Prev[j];
+        int add = 0;
+        for (int kk = 0; kk < NCB; kk++) {
+            int isnew = 1;
+            for (int l = 0; l < np; l++) {
+                int dleft = abs(IDE[l] - Prev[l].BindexNow);
+                if (Prev[l].DirProp == 1) {
+                    if (CB[kk].Bindex >= Prev[l].BindexNow - 3 * dleft && CB[kk].Bindex <= IDE[l]) { isnew = 0; break; }
+                } else {
+                    if (CB[kk].Bindex >= Prev[l].BindexNow && CB[kk].Bindex <= IDE[l] + 3 * dleft) { isnew = 0; break; }
+                }
+            }
+            if (isnew && np + add < cap) {
+                orc1d_cb *d = &New[np + add];
+                d->Bindex = CB[kk].Bindex; d->BindexNow = CB[kk].Bindex; IDE[np + add] = CB[kk].Bindex;
+                d->Ucrest = CB[kk].Ucrest; d->Bposition = CB[kk].Bposition; d->Btime = CB[kk].Btime;
+                d->DirProp = CB[kk].DirProp; d->Bcount = CB[kk].Bcount;
+                add++;
+            }
+        }
+        int j = 0;
+        while (j < cap && New[j].Bindex > 0) j++;
+        for (int i = 0; i < j; i++) New[i].Bcount = j;
+    }
+    if (c->flag == 1 || c->flag == 2) {
+        int NCBnow = New[0].Bcount;
+        int *IDT = (int *)calloc(NCBnow > 0 ? NCBnow : 1, sizeof(int));
+        int iterNprev = 0;
+        reset_cb(Prev, cap);
+        for (int lx = 0; lx < NCBnow; lx++) {
+            int Dir = New[lx].DirProp, now;
+            if (c->flag == 1) {
+                now = New[lx].Bindex;
+            } else {
+                int start = New[lx].BindexNow;
+                int lenloc = abs(IDE[lx] - start);
+                double *loc = (double *)malloc(sizeof(double) * (lenloc > 0 ? lenloc : 1));
+                for (int j = 0; j < lenloc; j++) loc[j] = eta[start + j * Dir];
+                now = start + idx_max(loc, lenloc) * Dir;
+                free(loc);
+            }
+            New[lx].BindexNow = now;
+            if (lx != 0 && now == New[lx - 1].BindexNow) { IDT[lx] = 0; continue; }
+            double tb = New[lx].Btime, UI = New[lx].Ucrest, UF = p->TC * UI, Us;
+            if (t - tb >= p->Tstar) Us = UF;
+            else if (t - tb >= 0 && t - tb < p->Tstar) Us = UI + ((t - tb) / p->Tstar) * (UF - UI);
+            else Us = UI;
+            int indEND = Dir == 1 ? p->xbend : p->xbstart;
+            int node_end = now, fl = 0;
+            /* the reference's loop runs while ly < indEND, so crests moving
+               to -x never mark nodes (1d/breaking.c:355) */
+            for (int ly = now; ly < indEND && ly >= 0 && ly < N; ly += Dir) {
+                double au = fabs(u[ly]);
+                if (au < UF) { node_end = ly; break; }
+                if (au > 2 * Us) B[ly] = 1;
+                else if (au <= Us) B[ly] = 0;
+                else B[ly] = au / Us - 1;
+                if (au >= Us) fl = 1;
+                if (Dir == 0) break;
+            }
+            if (!fl) {
+                IDT[lx] = 0;
+            } else {
+                IDT[lx] = 1;
+                IDE[iterNprev] = node_end;
+                int bn = Prev[iterNprev].BindexNow;
+                (void)bn;
+                Prev[iterNprev] = New[lx];
+                iterNprev++;
+            }
+        }
+        int j = 0;
+        while (j < cap && Prev[j].Bindex > 0) j++;
+        for (int i = 0; i < j; i++) Prev[i].Bcount = j;
+        for (int i = 0; i < NCBnow; i++) {
+            if (IDT[i] == 1) { c->flag = 2; break; }
+            c->flag = 0;
+        }
+        free(IDT);
+    }
+    reset_cb(New, cap);
+    reset_cb(CB, cap);
+}
+
+void *orc1d_create(const orc1d_params *prm)
+{
+    orc1d_ctx *c = (orc1d_ctx *)calloc(1, sizeof(orc1d_ctx));
+    c->p = *prm;
+    int N = prm->N;
+    c->plan = plan_new(N);
+    if (prm->source_on && prm->nt > 0) {
+        c->sig_c = (double *)malloc(sizeof(double) * prm->nt);
+        c->skew_c = (double *)malloc(sizeof(double) * prm->nt);
+        spline_c(prm->t_sig, prm->sig, prm->nt, c->sig_c);
+        spline_c(prm->t_sig, prm->sig_skew, prm->nt, c->skew_c);
+    }
+    c->ncap = N / 4;
+    c->CB = (orc1d_cb *)calloc(c->ncap, sizeof(orc1d_cb));
+    c->CBNew = (orc1d_cb *)calloc(c->ncap, sizeof(orc1d_cb));
+    c->CBPrev = (orc1d_cb *)calloc(c->ncap, sizeof(orc1d_cb));
+    c->ID_Node_end = (int *)calloc(c->ncap, sizeof(int));
+    c->B = (double *)calloc(N, sizeof(double));
+    c->Rb_hat = cz(N);
+    c->dim = 4 * (size_t)N;
+    return c;
+}
+
+void orc1d_destroy(void *vc)
+{
+    orc1d_ctx *c = (orc1d_ctx *)vc;
+    if (!c) return;
+    plan_free(c->plan);
+    free(c->sig_c); free(c->skew_c);
+    free(c->CB); free(c->CBNew); free(c->CBPrev); free(c->ID_Node_end); free(c->B); free(c->Rb_hat);
+    for (int i = 0; i < 6; i++) free(c->k[i]);
+    free(c->ytmp); free(c->y0); free(c->yerr); free(c->dydt_in); free(c->dydt_out);
+    free(c);
+}
+
+/* rhs_HsF2 (1d/rhs.c:96-241) */
+int orc1d_rhs(void *vc, double t, const double *z, double *dz)
+{
+    orc1d_ctx *c = (orc1d_ctx *)vc;
+    const orc1d_params *p = &c->p;
+    const int N = p->N;
+    const double g = ORC_G, *k = p->k;
+    c->nrhs++;
+    double *eh = cz(N), *uh = cz(N);
+    for (int i = 0; i < N; i++) {
+        eh[2 * i] = z[i]; eh[2 * i + 1] = z[N + i];
+        uh[2 * i] = z[2 * N + i]; uh[2 * i + 1] = z[3 * N + i];
+    }
+    /* define_delH1 */
+    double *duH1 = cz(N), *deH1 = cz(N);
+    double *Cv = NULL;
+    if (p->flat) {
+        for (int i = 0; i < N; i++) {
+            duH1[2 * i] = (p->csqr[i] / g) * k[i] * uh[2 * i + 1];
+            duH1[2 * i + 1] = -(p->csqr[i] / g) * k[i] * uh[2 * i];
+        }
+    } else {
+        Cv = cz(N);
+        hss(c, uh, Cv);
+        for (int i = 0; i < N; i++) {
+            duH1[2 * i] = (Cv[2 * i + 1] / g) * k[i];
+            duH1[2 * i + 1] = -(Cv[2 * i] / g) * k[i];
+        }
+    }
+    for (int i = 0; i < N; i++) {
+        deH1[2 * i] = g * k[i] * eh[2 * i + 1];
+        deH1[2 * i + 1] = -g * k[i] * eh[2 * i];
+    }
+    /* ifft_fft_Hsf2 / ifft_fft_Hsf2_v */
+    double *M0uh = cz(N);
+    if (p->flat) {
+        for (int i = 0; i < N; i++) {
+            double m = -k[i] * p->csqr[i] / g;             /* operator_M0 */
+            M0uh[2 * i] = -m * uh[2 * i + 1];
+            M0uh[2 * i + 1] = m * uh[2 * i];
+        }
+    } else {
+        double *H = cz(N);
+        hss(c, uh, H);                                     /* Op_M0 recomputes HSS */
+        for (int i = 0; i < N; i++) {
+            M0uh[2 * i] = -1.0 / g * (-k[i] * H[2 * i + 1]);
+            M0uh[2 * i + 1] = -1.0 / g * (k[i] * H[2 * i]);
+        }
+        free(H);
+    }
+    double *eta = malloc(sizeof(double) * N), *u = malloc(sizeof(double) * N), *M0u = malloc(sizeof(double) * N),
+           *f = malloc(sizeof(double) * N);
+    ifft_re(c, eh, eta);
+    ifft_re(c, uh, u);
+    ifft_re(c, M0uh, M0u);
+    double *euh = cz(N), *eMh = cz(N), *u2h = cz(N), *Deh = cz(N), *Duh = cz(N);
+    for (int i = 0; i < N; i++) f[i] = eta[i] * u[i];
+    fft_re(c, f, euh);
+    for (int i = 0; i < N; i++) f[i] = eta[i] * M0u[i];
+    fft_re(c, f, eMh);
+    for (int i = 0; i < N; i++) f[i] = u[i] * u[i] - M0u[i] * M0u[i];
+    fft_re(c, f, u2h);
+    for (int i = 0; i < N; i++) f[i] = p->damp[i] * eta[i];
+    fft_re(c, f, Deh);
+    for (int i = 0; i < N; i++) f[i] = p->damp[i] * u[i];
+    fft_re(c, f, Duh);
+    double *LeM = cz(N);
+    if (p->flat) {
+        for (int i = 0; i < N; i++) { LeM[2 * i] = p->Lop[i] * eMh[2 * i]; LeM[2 * i + 1] = p->Lop[i] * eMh[2 * i + 1]; }
+    } else {
+        double *H = cz(N);
+        hss(c, eMh, H);
+        for (int i = 0; i < N; i++) {
+            double m1r = -1.0 / g * (-k[i] * H[2 * i + 1]), m1i = -1.0 / g * (k[i] * H[2 * i]);  /* Op_M1 */
+            LeM[2 * i] = -k[i] * m1i;
+            LeM[2 * i + 1] = k[i] * m1r;
+        }
+        free(H);
+    }
+    /* nonlinear terms (1d/rhs.c:141-147) */
+    double *duH2 = cz(N), *deH2 = cz(N);
+    for (int i = 0; i < N; i++) {
+        duH2[2 * i] = -LeM[2 * i] - (-k[i] * euh[2 * i + 1]);
+        duH2[2 * i + 1] = -LeM[2 * i + 1] - (k[i] * euh[2 * i]);
+        deH2[2 * i] = -(-k[i] * u2h[2 * i + 1]) / 2;
+        deH2[2 * i + 1] = -(k[i] * u2h[2 * i]) / 2;
+    }
+    /* adjust_delH */
+    double *a1 = malloc(sizeof(double) * N), *duA = cz(N), *deA = cz(N);
+    ifft_re(c, duH2, a1);
+    for (int i = 0; i < N; i++) a1[i] *= p->chi[i];
+    fft_re(c, a1, duA);
+    ifft_re(c, deH2, a1);
+    for (int i = 0; i < N; i++) a1[i] *= p->chi[i];
+    fft_re(c, a1, deA);
+    double *du12 = cz(N), *de12 = cz(N);
+    for (size_t q = 0; q < 2 * (size_t)N; q++) { du12[q] = duH1[q] + duA[q]; de12[q] = deH1[q] + deA[q]; }
+    /* rhs_breaking (1d/rhs.c:617-657) */
+    if (p->breaking_on) {
+        double *e2 = malloc(sizeof(double) * N), *u2 = malloc(sizeof(double) * N);
+        ifft_re(c, eh, e2);
+        ifft_re(c, uh, u2);
+        breaking_process(c, t, e2, eh, u2);
+        double *dte = malloc(sizeof(double) * N), *Fl = malloc(sizeof(double) * N), *dF = malloc(sizeof(double) * N);
+        ifft_re(c, du12, dte);
+        for (int j = 0; j < N; j++) Fl[j] = -c->B[j] * p->delb * p->delb * (p->depth[j] + e2[j]) * dte[j] * dte[j];
+        gradient1(Fl, p->dx, N, dF);
+        for (int j = 0; j < N; j++) f[j] = 1.0 / (p->depth[j] + e2[j]) * dF[j];
+        fft_re(c, f, c->Rb_hat);
+        free(e2); free(u2); free(dte); free(Fl); free(dF);
+    }
+    /* define_forcing (1d/rhs.c:659-693) */
+    double pv = 0, pvs = 0;
+    if (p->source_on && p->nt > 0) {
+        int idx = orc_spline_index(p->t_sig, p->nt, t);
+        pv = spline_eval(p->t_sig, p->sig, c->sig_c, p->nt, t, idx);
+        pvs = spline_eval(p->t_sig, p->sig_skew, c->skew_c, p->nt, t, idx);
+    }
+    for (int i = 0; i < N; i++) {
+        double fr = 0, fi = 0;
+        if (p->source_on) {
+            const double *gx = p->gamx + 2 * i, *gs = p->gamx_skew + 2 * i;
+            if (p->propagation == 0) { fr = 0.5 * (pv * gx[0] - pvs * gs[0]); fi = 0.5 * (pv * gx[1] - pvs * gs[1]); }
+            else if (p->propagation == 1) { fr = 0.5 * (pv * gx[0] + pvs * gs[0]); fi = 0.5 * (pv * gx[1] + pvs * gs[1]); }
+            else if (p->propagation == 2) { fr = pv * gx[0]; fi = pv * gx[1]; }
+        }
+        double a = p->aal[i];
+        const double *rb = c->Rb_hat + 2 * i;
+        dz[i] = (du12[2 * i] + 2 * fr - Deh[2 * i]) * a;
+        dz[i + N] = (du12[2 * i + 1] + 2 * fi - Deh[2 * i + 1]) * a;
+        dz[i + 2 * N] = (de12[2 * i] + (p->breaking_on ? rb[0] : 0.0) - Duh[2 * i]) * a;
+        dz[i + 3 * N] = (de12[2 * i + 1] + (p->breaking_on ? rb[1] : 0.0) - Duh[2 * i + 1]) * a;
+    }
+    free(eh); free(uh); free(duH1); free(deH1); free(Cv); free(M0uh); free(eta); free(u); free(M0u); free(f);
+    free(euh); free(eMh); free(u2h); free(Deh); free(Duh); free(LeM); free(duH2); free(deH2); free(a1); free(duA);
+    free(deA); free(du12); free(de12);
+    return 0;
+}
+
+/* ------------------------------------------------------------- RKF45 --- */
+static void rk_alloc1(orc1d_ctx *c)
+{
+    if (c->ytmp) return;
+    for (int i = 0; i < 6; i++) c->k[i] = (double *)malloc(sizeof(double) * c->dim);
+    c->ytmp = (double *)malloc(sizeof(double) * c->dim);
+    c->y0 = (double *)malloc(sizeof(double) * c->dim);
+    c->yerr = (double *)malloc(sizeof(double) * c->dim);
+    c->dydt_in = (double *)malloc(sizeof(double) * c->dim);
+    c->dydt_out = (double *)malloc(sizeof(double) * c->dim);
+}
+
+/* rkf45 apply with dydt_in and dydt_out (GSL odeiv rkf45) */
+static void rkf45_apply1(orc1d_ctx *c, double t, double h, double *y)
+{
+    size_t n = c->dim;
+    double **k = c->k, *yt = c->ytmp;
+    memcpy(k[0], c->dydt_in, sizeof(double) * n);
+    for (size_t i = 0; i < n; i++) yt[i] = y[i] + ah[0] * h * k[0][i];
+    orc1d_rhs(c, t + ah[0] * h, yt, k[1]);
+    for (size_t i = 0; i < n; i++) yt[i] = y[i] + h * (b3[0] * k[0][i] + b3[1] * k[1][i]);
+    orc1d_rhs(c, t + ah[1] * h, yt, k[2]);
+    for (size_t i = 0; i < n; i++) yt[i] = y[i] + h * (b4[0] * k[0][i] + b4[1] * k[1][i] + b4[2] * k[2][i]);
+    orc1d_rhs(c, t + ah[2] * h, yt, k[3]);
+    for (size_t i = 0; i < n; i++)
+        yt[i] = y[i] + h * (b5[0] * k[0][i] + b5[1] * k[1][i] + b5[2] * k[2][i] + b5[3] * k[3][i]);
+    orc1d_rhs(c, t + ah[3] * h, yt, k[4]);
+    for (size_t i = 0; i < n; i++)
+        yt[i] = y[i] + h * (b6[0] * k[0][i] + b6[1] * k[1][i] + b6[2] * k[2][i] + b6[3] * k[3][i] + b6[4] * k[4][i]);
+    orc1d_rhs(c, t + ah[4] * h, yt, k[5]);
+    for (size_t i = 0; i < n; i++) y[i] += h * (c1 * k[0][i] + c3 * k[2][i] + c4 * k[3][i] + c5 * k[4][i] + c6 * k[5][i]);
+    orc1d_rhs(c, t + h, y, c->dydt_out);
+    for (size_t i = 0; i < n; i++)
+        c->yerr[i] = h * (ec[1] * k[0][i] + ec[3] * k[2][i] + ec[4] * k[3][i] + ec[5] * k[4][i] + ec[6] * k[5][i]);
+}
+
+static int hadjust1(const orc1d_ctx *c, const double *y, double eps_abs, double eps_rel, double *h)
+{
+    const int ord = 5;
+    double rmax = 2.2250738585072014e-308;
+    for (size_t i = 0; i < c->dim; i++) {
+        double D0 = eps_rel * fabs(y[i]) + eps_abs;
+        double r = fabs(c->yerr[i]) / fabs(D0);
+        rmax = r > rmax ? r : rmax;
+    }
+    double h_old = *h;
+    if (rmax > 1.1) {
+        double r = 0.9 / pow(rmax, 1.0 / ord);
+        if (r < 0.2) r = 0.2;
+        *h = r * h_old;
+        return -1;
+    } else if (rmax < 0.5) {
+        double r = 0.9 / pow(rmax, 1.0 / (ord + 1.0));
+        if (r > 5.0) r = 5.0;
+        if (r < 1.0) r = 1.0;
+        *h = r * h_old;
+        return 1;
+    }
+    return 0;
+}
+
+/* gsl_odeiv_evolve_apply: dydt_in once per call, final-step clipping,
+ * retry on a decrease; when the decrease cannot change t the step is kept
+ * with the previous h */
+int orc1d_evolve_apply(void *vc, double *t, double t1, double *h, double *y, double eps_abs, double eps_rel, long *stats)
+{
+    orc1d_ctx *c = (orc1d_ctx *)vc;
+    rk_alloc1(c);
+    size_t n = c->dim;
+    const double t0 = *t;
+    double h0 = *h, dt = t1 - t0;
+    memcpy(c->y0, y, sizeof(double) * n);
+    orc1d_rhs(c, t0, y, c->dydt_in);
+    for (;;) {
+        int final_step = 0;
+        if ((dt >= 0.0 && h0 > dt) || (dt < 0.0 && h0 < dt)) { h0 = dt; final_step = 1; }
+        rkf45_apply1(c, t0, h0, y);
+        *t = final_step ? t1 : t0 + h0;
+        double h_old = h0;
+        int st = hadjust1(c, y, eps_abs, eps_rel, &h0);
+        if (st < 0) {
+            double t_curr = *t, t_next = *t + h0;
+            if (fabs(h0) < fabs(h_old) && t_next != t_curr) {
+                memcpy(y, c->y0, sizeof(double) * n);
+                if (stats) stats[1]++;
+                continue;
+            }
+            h0 = h_old;
+        }
+        if (stats) stats[0]++;
+        *h = h0;
+        return 0;
+    }
+}
+
+/* the inner loop of ode_solver_Ham (1d/functions.c:546-552) */
+int orc1d_integrate(void *vc, double *t, double t1, double *h, double *y, double eps_abs, double eps_rel, long *stats)
+{
+    while (*t < t1) {
+        int s = orc1d_evolve_apply(vc, t, t1, h, y, eps_abs, eps_rel, stats);
+        if (s) return s;
+    }
+    return 0;
+}
+
+long orc1d_rhs_count(void *vc) { return ((orc1d_ctx *)vc)->nrhs; }
+int orc1d_flag(void *vc) { return ((orc1d_ctx *)vc)->flag; }
+void orc1d_get_B(void *vc, double *B) { orc1d_ctx *c = (orc1d_ctx *)vc; memcpy(B, c->B, sizeof(double) * c->p.N); }
+
+/* stand-alone pieces for tests */
+void orc1d_fft(int N, double *x, int sign)
+{
+    fftplan *p = plan_new(N);
+    fft1(p, x, sign);
+    plan_free(p);
+}

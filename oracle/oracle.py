@@ -1,7 +1,8 @@
 """TEST INFRASTRUCTURE ONLY -- ctypes wrapper of the FP64 CPU oracle.
 
-The oracle (hawassi_oracle.c) is a plain-C restatement of the reference's
-2-D AB2 right-hand side and GSL-style RKF45 driver.  Only tests/,
+The oracle (hawassi_oracle.c, hawassi_oracle1d.c) is a plain-C restatement
+of the reference's 2-D AB2 and 1-D HS2 right-hand sides and GSL-style RKF45
+drivers.  Only tests/,
 __graft_entry__.smoke() and bench.py's CPU-baseline leg may use it, and only
 as the checker; the product library never calls it.
 """
@@ -34,6 +35,22 @@ class OrcParams(ct.Structure):
         ("KBC", ct.c_double), ("TC", ct.c_double), ("Tchar", ct.c_double), ("delb", ct.c_double),
         ("MaxEtaInit", ct.c_double), ("Tstar", ct.c_double), ("Rsearch", ct.c_double), ("U", ct.c_double),
         ("pbchar", _dp), ("depth", _dp), ("x", _dp), ("y", _dp),
+    ]
+
+
+class Orc1DParams(ct.Structure):
+    _fields_ = [
+        ("N", ct.c_int), ("flat", ct.c_int), ("source_on", ct.c_int), ("propagation", ct.c_int),
+        ("breaking_on", ct.c_int),
+        ("k", _dp), ("aal", _dp), ("chi", _dp), ("damp", _dp), ("csqr", _dp), ("Lop", _dp),
+        ("csq_min", _dp), ("csq_plus", _dp), ("gam_min", _dp), ("gam_plus", _dp),
+        ("gamx", _dp), ("gamx_skew", _dp),
+        ("nt", ct.c_int), ("t_sig", _dp), ("sig", _dp), ("sig_skew", _dp),
+        ("KBC", ct.c_double), ("TC", ct.c_double), ("Tstar", ct.c_double), ("MaxEtaInit", ct.c_double),
+        ("delb", ct.c_double), ("dx", ct.c_double),
+        ("xbstart", ct.c_int), ("xbend", ct.c_int), ("xbendL", ct.c_int), ("xbstartR", ct.c_int),
+        ("lenIF", ct.c_int), ("ILL", ct.c_int), ("IFF", ct.c_int),
+        ("depth", _dp), ("x", _dp),
     ]
 
 
@@ -74,6 +91,17 @@ def lib():
         L.orc_flagbr.argtypes = [ct.c_void_p]
         L.orc_get_B.argtypes = [ct.c_void_p, _dp]
         L.orc_num_threads.restype = ct.c_int
+        L.orc1d_create.argtypes = [ct.POINTER(Orc1DParams)]
+        L.orc1d_create.restype = ct.c_void_p
+        L.orc1d_destroy.argtypes = [ct.c_void_p]
+        L.orc1d_rhs.argtypes = [ct.c_void_p, ct.c_double, _dp, _dp]
+        L.orc1d_evolve_apply.argtypes = L.orc_evolve_apply.argtypes
+        L.orc1d_integrate.argtypes = L.orc_evolve_apply.argtypes
+        L.orc1d_rhs_count.argtypes = [ct.c_void_p]
+        L.orc1d_rhs_count.restype = ct.c_long
+        L.orc1d_flag.argtypes = [ct.c_void_p]
+        L.orc1d_get_B.argtypes = [ct.c_void_p, _dp]
+        L.orc1d_fft.argtypes = [ct.c_int, _dp, ct.c_int]
         L.orc_set_threads.argtypes = [ct.c_int]
         _lib = L
     return _lib
@@ -174,6 +202,103 @@ class Oracle2D:
         out = np.empty(self.model.nx * self.model.ny)
         lib().orc_get_B(self.ctx, _p(out))
         return out.reshape(self.model.ny, self.model.nx)
+
+
+def _cplx(a):
+    a = np.asarray(a, dtype=complex)
+    buf = np.empty(2 * a.size)
+    buf[0::2] = a.real
+    buf[1::2] = a.imag
+    return buf
+
+
+class Oracle1D:
+    """FP64 CPU restatement of rhs_HsF2 (flat or two-depth) + breaking + rkf45."""
+
+    def __init__(self, m):
+        L = lib()
+        keep = {}
+
+        def arr(name, a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            keep[name] = a
+            return _p(a)
+
+        p = Orc1DParams()
+        p.N, p.flat, p.source_on = m.n, int(m.flat), int(m.source_on)
+        p.propagation = {"Uni+": 0, "Uni-": 1, "Bi": 2, "None": 3}[m.propagation]
+        p.k, p.aal, p.chi, p.damp = arr("k", m.k), arr("aal", m.aal), arr("chi", m.chi), arr("damp", m.damp)
+        if m.flat:
+            p.csqr, p.Lop = arr("csqr", m.csqr), arr("lop", m.lop)
+        else:
+            o = m.ops
+            p.csq_min, p.csq_plus = arr("cm", o["csq_min"]), arr("cp", o["csq_plus"])
+            p.gam_min, p.gam_plus = arr("gm", o["gam_min"]), arr("gp", o["gam_plus"])
+        if m.source_on:
+            p.gamx, p.gamx_skew = arr("gx", _cplx(m.gamx)), arr("gs", _cplx(m.gamx_skew))
+            p.nt = len(m.t_sig)
+            p.t_sig, p.sig, p.sig_skew = arr("ts", m.t_sig), arr("sig", m.sig), arr("sk", m.sig_skew)
+        br = m.breaking
+        p.dx = m.dx
+        p.depth, p.x = arr("depth", m.depth), arr("x", m.x)
+        if br is not None:
+            p.breaking_on = 1
+            p.KBC, p.TC, p.Tstar, p.MaxEtaInit, p.delb = br["KBC"], br["TC"], br["Tstar"], br["MaxEtaInit"], br["delb"]
+            for f in ("xbstart", "xbend", "xbendL", "xbstartR", "lenIF", "ILL", "IFF"):
+                setattr(p, f, int(br[f]))
+        self._keep, self._p = keep, p
+        self.ctx = L.orc1d_create(ct.byref(p))
+        self.model = m
+        self.dim = 4 * m.n
+
+    def __del__(self):
+        try:
+            lib().orc1d_destroy(self.ctx)
+        except Exception:
+            pass
+
+    def rhs(self, t, y):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        out = np.empty_like(y)
+        lib().orc1d_rhs(self.ctx, float(t), _p(y), _p(out))
+        return out
+
+    def evolve_apply(self, t, t1, h, y, eps_abs=None, eps_rel=None):
+        m = self.model
+        tt, hh = ct.c_double(t), ct.c_double(h)
+        st = (ct.c_long * 2)()
+        rc = lib().orc1d_evolve_apply(self.ctx, ct.byref(tt), float(t1), ct.byref(hh), _p(y),
+                                      m.eps_abs if eps_abs is None else eps_abs,
+                                      m.eps_rel if eps_rel is None else eps_rel, st)
+        return rc, tt.value, hh.value, (st[0], st[1])
+
+    def integrate(self, t, t1, h, y, eps_abs=None, eps_rel=None):
+        m = self.model
+        tt, hh = ct.c_double(t), ct.c_double(h)
+        st = (ct.c_long * 2)()
+        rc = lib().orc1d_integrate(self.ctx, ct.byref(tt), float(t1), ct.byref(hh), _p(y),
+                                   m.eps_abs if eps_abs is None else eps_abs,
+                                   m.eps_rel if eps_rel is None else eps_rel, st)
+        return rc, tt.value, hh.value, (st[0], st[1])
+
+    def rhs_count(self):
+        return lib().orc1d_rhs_count(self.ctx)
+
+    def flag(self):
+        return lib().orc1d_flag(self.ctx)
+
+    def B(self):
+        out = np.empty(self.model.n)
+        lib().orc1d_get_B(self.ctx, _p(out))
+        return out
+
+
+def fft1d(a: np.ndarray, sign: int) -> np.ndarray:
+    buf = _cplx(a)
+    lib().orc1d_fft(len(a), _p(buf), sign)
+    return buf[0::2] + 1j * buf[1::2]
 
 
 def hermitian_project(a: np.ndarray) -> np.ndarray:

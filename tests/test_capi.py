@@ -14,12 +14,13 @@ ROOT = Path(__file__).resolve().parent.parent
 def declared_symbols():
     text = (ROOT / "include" / "hawassi_hip.h").read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(hw_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(hw(?:1d)?_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_header_declares_the_boundary():
     syms = declared_symbols()
-    for s in ("hw_ctx_create_2d", "hw_rhs", "hw_evolve_apply", "hw_integrate", "hw_get_fields"):
+    for s in ("hw_ctx_create_2d", "hw_rhs", "hw_evolve_apply", "hw_integrate", "hw_get_fields",
+              "hw1d_create", "hw1d_rhs", "hw1d_integrate"):
         assert s in syms
 
 
