@@ -69,6 +69,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--workload", default="harbour2048", choices=["harbour2048", "slope1d"],
+                    help="harbour2048: the 2-D headline; slope1d: the 1-D batch of uneven realizations")
+    ap.add_argument("--nreal", type=int, default=256, help="slope1d: realizations in the batch")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="launch one elementwise copy kernel (torch.neg) over this many MiB of fp32 first, so a "
                          "counter pass over this command holds a dispatch of known HBM traffic")
@@ -78,10 +81,6 @@ def main():
               "(the path would shard over independent sea-state realizations, one context per GPU)", file=sys.stderr)
         sys.exit(2)
 
-    from hawassi_c_amd import HW_PREC_FP32, HW_PREC_FP64
-    from hawassi_c_amd import workloads as W
-    from hawassi_c_amd.model2d import Hawassi2D
-
     if args.copy_probe_mib:
         import torch
         x = torch.ones(args.copy_probe_mib * (1 << 18), device="cuda")
@@ -90,6 +89,12 @@ def main():
         torch.cuda.synchronize()
         log(f"copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written")
         del x, y
+    if args.workload == "slope1d":
+        return bench_slope1d(args)
+
+    from hawassi_c_amd import HW_PREC_FP32, HW_PREC_FP64
+    from hawassi_c_amd import workloads as W
+    from hawassi_c_amd.model2d import Hawassi2D
 
     n = args.n
     t_end = args.spinup + 60.0
@@ -192,6 +197,90 @@ def main():
                                          f"same {n}x{n} harbour from the same state at t={t_start:.3f} s; FP64 "
                                          f"oracle (restated CPU path, radix-2 FFT, OpenMP), {cpu_model()}",
                                "rhs_evals_per_s": nr / cw, "wall_s": cw}
+    print(json.dumps(out))
+
+
+def bench_slope1d(args):
+    """1-D batch: nreal realizations of the sloping-beach HS2 case (two-depth
+    operator, breaking) with n drawn from {4096, 8192, 16384}, FP32
+    transforms / FP64 state, advanced together by the device driver through
+    the reference's output times (every 0.05 s).  A step is one output
+    interval of the whole batch; value = realizations x simulated seconds per
+    wall second."""
+    from hawassi_c_amd import HW_PREC_FP32, HW_PREC_FP64
+    from hawassi_c_amd import workloads1d as W1
+    from hawassi_c_amd.model1d import Hawassi1D
+
+    dt_out = 0.05
+    spin = args.spinup if args.spinup != 10.0 else 2.0
+    t_end = spin + (args.warmup + args.steps + 4) * dt_out
+    log(f"setup {args.nreal} realizations")
+    models = W1.slope_batch(args.nreal, seed=7, t_end=t_end)
+    ns = [m.n for m in models]
+    dev = Hawassi1D(models, HW_PREC_FP32, t0=0.0, h0=1e-4)
+    t = 0.0
+    w0 = time.perf_counter()
+    while t < spin - 1e-12:
+        t = min(spin, t + dt_out)
+        dev.integrate(t)
+        log(f"  spin-up t={t:.2f} wall {time.perf_counter() - w0:.1f} s")
+    log(f"spin-up done t={t:.2f} stats={dev.stats()}")
+    for _ in range(args.warmup):
+        t += dt_out
+        dev.integrate(t)
+    st0 = dev.stats()
+    t_start = t
+    w0 = time.perf_counter()
+    for i in range(args.steps):
+        t += dt_out
+        dev.integrate(t)
+        log(f"  timed {i + 1}/{args.steps} wall {time.perf_counter() - w0:.1f} s")
+    wall = time.perf_counter() - w0
+    st1 = dev.stats()
+    sim = t - t_start
+    nrhs = st1["rhs_evals"] - st0["rhs_evals"]
+    log(f"timed {args.steps} output intervals: wall {wall:.3f} s, sim {sim:.3f} s, rhs {nrhs}")
+    # algorithmic bytes per realization-RHS (transform field traffic model,
+    # T_alg = 20, plus coefficients and the FP64 state in/out)
+    def b_rhs(n):
+        h = n // 2 + 1
+        return 20 * (4 * n + 8 * h) + 8 * 4 * n + 2 * (2 * 16 * h)
+    mean_b = float(np.mean([b_rhs(n) for n in ns]))
+    out = {
+        "metric": "sim-sec/wall-sec (1-D batch, realizations x simulated seconds)",
+        "value": args.nreal * sim / wall,
+        "unit": "sim-s/wall-s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"slope1d batch of {args.nreal}: n in {{4096, 8192, 16384}} "
+                               f"(counts {ns.count(4096)}/{ns.count(8192)}/{ns.count(16384)}), 400 m, depth 1.0 -> 0.2 m "
+                               "two-depth operator, JONSWAP Hs 0.10 Tp 2.0, breaking KBC 0.85",
+                   "sim_window_s": [t_start, t]},
+        "rhs_evals_per_s": nrhs / wall,
+        "realization_rhs_bytes_mean": mean_b,
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        from oracle import oracle as O
+        r = ns.index(min(ns))
+        m = models[r]
+        y = dev.get_state(r)
+        tr, hr = dev.time(r)
+        orc = O.Oracle1D(m)
+        c0 = orc.rhs_count()
+        w0 = time.perf_counter()
+        tt, hh = tr, hr
+        for _ in range(4):
+            rc, tt, hh, _ = orc.integrate(tt, tt + dt_out, hh, y)
+        cw = time.perf_counter() - w0
+        out["cpu_baseline"] = {"value": (tt - tr) / cw, "unit": "sim-s/wall-s per realization", "cores": 1,
+                               "kind": "port",
+                               "sample": f"4 output intervals ({tt - tr:.2f} s) of realization {r} (n={m.n}) from "
+                                         f"the device state at t={tr:.2f}; FP64 oracle, 1 thread, {cpu_model()}",
+                               "rhs_evals_per_s": (orc.rhs_count() - c0) / cw}
+    dev.close()
     print(json.dumps(out))
 
 

@@ -13,7 +13,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .setup2d import GRAV, spline_coeffs, spline_eval  # noqa: F401  (same natural spline)
+from .setup2d import GRAV, _spline_eval_vec, spline_coeffs, spline_eval  # noqa: F401  (same natural spline)
 
 PROPAGATION = {"Uni+": 0, "Uni-": 1, "Bi": 2, "None": 3}
 
@@ -185,8 +185,12 @@ def up2ip(depth: np.ndarray, k: np.ndarray, nu: float, n: int = 101):
         gm[i] = (om_p * cpd - cpp * om_d) / det
         gp[i] = (-om_m * cpd + cpm * om_d) / det
     cgm, cgp = spline_coeffs(D, gm), spline_coeffs(D, gp)
-    gam_min = np.array([spline_eval(D, gm, cgm, float(d)) for d in depth])
-    gam_plus = np.array([spline_eval(D, gp, cgp, float(d)) for d in depth])
+    if depth.size <= 4096:
+        gam_min = np.array([spline_eval(D, gm, cgm, float(d)) for d in depth])
+        gam_plus = np.array([spline_eval(D, gp, cgp, float(d)) for d in depth])
+    else:
+        gam_min = _spline_eval_vec(D, gm, cgm, depth)
+        gam_plus = _spline_eval_vec(D, gp, cgp, depth)
     return dict(gam_min=gam_min, gam_plus=gam_plus, csq_min=phase_velocity_v(k, dmin) ** 2,
                 csq_plus=phase_velocity_v(k, dplus) ** 2, dmin=dmin, dplus=dplus)
 

@@ -617,6 +617,7 @@ int orc1d_integrate(void *vc, double *t, double t1, double *h, double *y, double
 
 long orc1d_rhs_count(void *vc) { return ((orc1d_ctx *)vc)->nrhs; }
 int orc1d_flag(void *vc) { return ((orc1d_ctx *)vc)->flag; }
+int orc1d_ncrest(void *vc) { return ((orc1d_ctx *)vc)->CBPrev[0].Bcount; }
 void orc1d_get_B(void *vc, double *B) { orc1d_ctx *c = (orc1d_ctx *)vc; memcpy(B, c->B, sizeof(double) * c->p.N); }
 
 /* stand-alone pieces for tests */

@@ -100,6 +100,7 @@ def lib():
         L.orc1d_rhs_count.argtypes = [ct.c_void_p]
         L.orc1d_rhs_count.restype = ct.c_long
         L.orc1d_flag.argtypes = [ct.c_void_p]
+        L.orc1d_ncrest.argtypes = [ct.c_void_p]
         L.orc1d_get_B.argtypes = [ct.c_void_p, _dp]
         L.orc1d_fft.argtypes = [ct.c_int, _dp, ct.c_int]
         L.orc_set_threads.argtypes = [ct.c_int]
@@ -288,6 +289,9 @@ class Oracle1D:
 
     def flag(self):
         return lib().orc1d_flag(self.ctx)
+
+    def ncrest(self):
+        return lib().orc1d_ncrest(self.ctx)
 
     def B(self):
         out = np.empty(self.model.n)
