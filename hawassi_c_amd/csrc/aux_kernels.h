@@ -202,19 +202,19 @@ __global__ void k_src_eval(const double* __restrict__ sy, const double* __restri
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k_t_xfwd(const T* __restrict__ phys, typename ctype<T>::type* X,
                                                             const typename ctype<T>::type* twx) {
-    HW_XK_PROLOGUE
+    HW_XK_PROLOGUE_TW(twx)
     C f[G::PP];
     ph_load<T, NX, NY>(f, phys, y0, tid);
-    x_fwd_store<T, NX, NY>(f, X, y0, lds, twx, tid);
+    x_fwd_store<T, NX, NY>(f, X, y0, lds, twxs, tid);
 }
 
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k_t_yfwd(const typename ctype<T>::type* X, typename ctype<T>::type* S,
                                                             const typename ctype<T>::type* twy) {
-    HW_YK_PROLOGUE
+    HW_YK_PROLOGUE_TW(twy)
     C v[G::PE];
     y_load<T, NX, NY>(v, X, nullptr, col0, tid);
-    y_fwd_regs<T, NX, NY>(v, lds, twy, tid);
+    y_fwd_regs<T, NX, NY>(v, lds, twys, tid);
     y_store<T, NX, NY>(S, v, col0, tid);
 }
 
@@ -222,19 +222,19 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k_t_yfwd(const typename cty
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k_t_yinv(const typename ctype<T>::type* S, typename ctype<T>::type* X,
                                                             const typename ctype<T>::type* twy) {
-    HW_YK_PROLOGUE
+    HW_YK_PROLOGUE_TW(twy)
     C v[G::PE];
     y_load<T, NX, NY>(v, S, nullptr, col0, tid);
-    y_inv_regs<T, NX, NY>(v, lds, twy, col0, tid);
+    y_inv_regs<T, NX, NY>(v, lds, twys, col0, tid);
     y_store<T, NX, NY>(X, v, col0, tid);
 }
 
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k_t_xinv(const typename ctype<T>::type* X, T* phys, T* nyq_out,
                                                             const typename ctype<T>::type* twx) {
-    HW_XK_PROLOGUE
+    HW_XK_PROLOGUE_TW(twx)
     C f[G::PP];
-    x_inverse<T, NX, NY>(X, y0, lds, twx, f, sc, nyq, tid);
+    x_inverse<T, NX, NY>(X, y0, lds, twxs, f, sc, nyq, tid);
     ph_store<T, NX, NY>(phys, f, y0, tid);
     if (nyq_out && tid < G::RR) nyq_out[y0 + tid] = nyq[tid];
 }

@@ -114,7 +114,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k_brk_hilb_y(Rhs2DArgs<T> a
             v[e] = cmul_i<-1>(cscale(v[e], s));
         }
     }
-    y_inv_regs<T, NX, NY>(v, lds, a.twy, col0, tid);
+    y_inv_regs<T, NX, NY>(v, lds, twys, col0, tid);
     y_store<T, NX, NY>(X, v, col0, tid);
 }
 
@@ -300,7 +300,7 @@ __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k_brk_g_x(Rhs2DArgs<T> a, c
         T s = s_eta[y0 + (r < G::RR ? r : 0)];
         f[q] = cmk<C>(t[q].x * s, -t[q].y * s);
     }
-    x_fwd_store<T, NX, NY>(f, X, y0, lds, a.twx, tid);
+    x_fwd_store<T, NX, NY>(f, X, y0, lds, twxs, tid);
 }
 
 template <typename T, int NX, int NY>
@@ -309,7 +309,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k_brk_del_y(Rhs2DArgs<T> a,
     HW_YK_PROLOGUE
     C Fg[G::PE], d[G::PE];
     y_load<T, NX, NY>(Fg, Xg, nullptr, col0, tid);
-    y_fwd_regs<T, NX, NY>(Fg, lds, a.twy, tid);
+    y_fwd_regs<T, NX, NY>(Fg, lds, twys, tid);
     if (!flat) {
         y_load<T, NX, NY>(d, a.dH1, nullptr, col0, tid);
     } else {
@@ -330,7 +330,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k_brk_del_y(Rhs2DArgs<T> a,
     }
 #pragma unroll
     for (int e = 0; e < G::PE; ++e) d[e] = cadd(d[e], cmul_i<1>(Fg[e]));
-    y_inv_regs<T, NX, NY>(d, lds, a.twy, col0, tid);
+    y_inv_regs<T, NX, NY>(d, lds, twys, col0, tid);
     y_store<T, NX, NY>(Xo, d, col0, tid);
 }
 
@@ -341,8 +341,8 @@ __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k_brk_flux_x(Rhs2DArgs<T> a
                                                                  T* __restrict__ flux) {
     HW_XK_PROLOGUE
     C dp[G::PP], A[G::PP], t[G::PP];
-    x_inverse<T, NX, NY>(Xd, y0, lds, a.twx, dp, sc, nullptr, tid);
-    x_inverse<T, NX, NY>(a.W[slot_aeta], y0, lds, a.twx, A, sc, nullptr, tid);
+    x_inverse<T, NX, NY>(Xd, y0, lds, twxs, dp, sc, nullptr, tid);
+    x_inverse<T, NX, NY>(a.W[slot_aeta], y0, lds, twxs, A, sc, nullptr, tid);
     ph_load<T, NX, NY>(t, a.chi, y0, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) dp[q] = pfma(t[q], A[q], dp[q]);
@@ -389,9 +389,9 @@ __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k_brk_sb_x(Rhs2DArgs<T> a, 
     HW_XK_PROLOGUE
     C f[G::PP];
     ph_load<T, NX, NY>(f, sbx, y0, tid);
-    x_fwd_store<T, NX, NY>(f, X1, y0, lds, a.twx, tid);
+    x_fwd_store<T, NX, NY>(f, X1, y0, lds, twxs, tid);
     ph_load<T, NX, NY>(f, sby, y0, tid);
-    x_fwd_store<T, NX, NY>(f, X2, y0, lds, a.twx, tid);
+    x_fwd_store<T, NX, NY>(f, X2, y0, lds, twxs, tid);
 }
 
 template <typename T, int NX, int NY>
@@ -401,10 +401,10 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k_brk_sb_y(Rhs2DArgs<T> a, 
     HW_YK_PROLOGUE
     C v[G::PE];
     y_load<T, NX, NY>(v, X1, nullptr, col0, tid);
-    y_fwd_regs<T, NX, NY>(v, lds, a.twy, tid);
+    y_fwd_regs<T, NX, NY>(v, lds, twys, tid);
     y_store<T, NX, NY>(S1, v, col0, tid);
     y_load<T, NX, NY>(v, X2, nullptr, col0, tid);
-    y_fwd_regs<T, NX, NY>(v, lds, a.twy, tid);
+    y_fwd_regs<T, NX, NY>(v, lds, twys, tid);
     y_store<T, NX, NY>(S2, v, col0, tid);
 }
 

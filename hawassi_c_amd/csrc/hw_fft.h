@@ -132,6 +132,14 @@ __device__ __forceinline__ void fft_lds(C* lds, int LS, const C* __restrict__ tw
 }
 
 // ------------------------------------------------------- pass geometry ----
+// Elements each thread holds between transforms: fewer elements per thread
+// means fewer live registers and more waves in flight per CU.
+#ifndef HW_X_PER_THREAD
+#define HW_X_PER_THREAD 4
+#endif
+#ifndef HW_Y_PER_THREAD
+#define HW_Y_PER_THREAD 4
+#endif
 // x-pass: RR rows per block so that an RR-element run of one spectral column
 // (the transposed write/read) is 32 bytes long; blocks are remapped so that
 // the row groups sharing a 128-byte line run on the same XCD (same L2).
@@ -141,8 +149,8 @@ template <typename T, int NX> struct XGeo {
     static constexpr int L = NX / 2;                    // half-length complex FFT
     static constexpr int LS = L + 2;                    // LDS row stride (complex)
     static constexpr int NPAIR = RR * L;                // physical (x,x+1) pairs per block
-    static constexpr int NT0 = NPAIR / 8;
-    static constexpr int NT = NT0 > 512 ? 512 : (NT0 < 64 ? 64 : NT0);
+    static constexpr int NT0 = NPAIR / HW_X_PER_THREAD;
+    static constexpr int NT = NT0 > 1024 ? 1024 : (NT0 < 64 ? 64 : NT0);
     static constexpr int PP = (NPAIR + NT - 1) / NT;    // pairs per thread
     static constexpr int NSPEC = RR * (L + 1);          // spectral values per block
     static constexpr int PS = (NSPEC + NT - 1) / NT;
@@ -155,8 +163,8 @@ template <typename T, int NY> struct YGeo {
     static constexpr int CC = NY >= 1024 ? 1 : 1024 / NY;
     static constexpr int LS = NY;
     static constexpr int NE = CC * NY;
-    static constexpr int NT0 = NE / 8;
-    static constexpr int NT = NT0 > 512 ? 512 : (NT0 < 64 ? 64 : NT0);
+    static constexpr int NT0 = NE / HW_Y_PER_THREAD;
+    static constexpr int NT = NT0 > 1024 ? 1024 : (NT0 < 64 ? 64 : NT0);
     static constexpr int PE = (NE + NT - 1) / NT;       // elements per thread
     static constexpr int LDS_ELEMS = CC * LS;
 };

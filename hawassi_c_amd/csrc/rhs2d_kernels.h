@@ -193,22 +193,30 @@ template <typename T, int NX, int NY> struct YIdx {
 };
 
 // ========================================================== 2DB kernels ==
-#define HW_XK_PROLOGUE                                                   \
+#define HW_XK_PROLOGUE HW_XK_PROLOGUE_TW(a.twx)
+#define HW_XK_PROLOGUE_TW(TWSRC)                                         \
     using C = typename ctype<T>::type;                                   \
     using G = XGeo<T, NX>;                                               \
     __shared__ C lds[G::LDS_ELEMS];                                      \
     __shared__ T nyq[G::RR];                                             \
+    __shared__ C twxs[NX];                                               \
     const int tid = threadIdx.x;                                         \
     const int y0 = xcd_remap(blockIdx.x, gridDim.x) * G::RR;             \
     const T sc = (T)1 / ((T)NX * (T)NY);                                 \
+    for (int i_ = tid; i_ < NX; i_ += G::NT) twxs[i_] = (TWSRC)[i_];     \
+    __syncthreads();                                                     \
     (void)nyq; (void)sc;
 
-#define HW_YK_PROLOGUE                                                   \
+#define HW_YK_PROLOGUE HW_YK_PROLOGUE_TW(a.twy)
+#define HW_YK_PROLOGUE_TW(TWSRC)                                         \
     using C = typename ctype<T>::type;                                   \
     using G = YGeo<T, NY>;                                               \
     __shared__ C lds[G::LDS_ELEMS];                                      \
+    __shared__ C twys[NY];                                               \
     const int tid = threadIdx.x;                                         \
-    const int col0 = blockIdx.x * G::CC;
+    const int col0 = blockIdx.x * G::CC;                                 \
+    for (int i_ = tid; i_ < NY; i_ += G::NT) twys[i_] = (TWSRC)[i_];     \
+    __syncthreads();
 
 // W slots used by the 2DB pipeline
 enum : int {
@@ -221,7 +229,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y1(Rhs2DArgs<T> a) {
     HW_YK_PROLOGUE
     C v[G::PE], w[G::PE];
     y_load<T, NX, NY>(v, a.eta_hat, nullptr, col0, tid);
-    y_inv_regs<T, NX, NY>(v, lds, a.twy, col0, tid);
+    y_inv_regs<T, NX, NY>(v, lds, twys, col0, tid);
     y_store<T, NX, NY>(a.W[S_ETA], v, col0, tid);
     const C* src[2] = {a.u_hat, a.v_hat};
     const int slot[2][3] = {{S_U, S_AMU, S_APU}, {S_V, S_AMV, S_APV}};
@@ -230,7 +238,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y1(Rhs2DArgs<T> a) {
         y_load<T, NX, NY>(v, src[f], nullptr, col0, tid);
 #pragma unroll
         for (int e = 0; e < G::PE; ++e) w[e] = v[e];
-        y_inv_regs<T, NX, NY>(w, lds, a.twy, col0, tid);
+        y_inv_regs<T, NX, NY>(w, lds, twys, col0, tid);
         y_store<T, NX, NY>(a.W[slot[f][0]], w, col0, tid);
         #pragma unroll
         for (int s = 0; s < 2; ++s) {
@@ -241,7 +249,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y1(Rhs2DArgs<T> a) {
                 YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
                 w[e] = (kx < NX / 2 + 1) ? cscale(v[e], mul[gi]) : cmk<C>((T)0, (T)0);
             }
-            y_inv_regs<T, NX, NY>(w, lds, a.twy, col0, tid);
+            y_inv_regs<T, NX, NY>(w, lds, twys, col0, tid);
             y_store<T, NX, NY>(a.W[slot[f][1 + s]], w, col0, tid);
         }
     }
@@ -251,7 +259,7 @@ template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x1(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
     C f[G::PP], acc[G::PP], gm[G::PP], gp[G::PP];
-    x_inverse<T, NX, NY>(a.W[S_ETA], y0, lds, a.twx, f, sc, nullptr, tid);
+    x_inverse<T, NX, NY>(a.W[S_ETA], y0, lds, twxs, f, sc, nullptr, tid);
     ph_store<T, NX, NY>(a.eta_p, f, y0, tid);
     ph_load<T, NX, NY>(gm, a.gam_m, y0, tid);
     ph_load<T, NX, NY>(gp, a.gam_p, y0, tid);
@@ -261,21 +269,21 @@ __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x1(Rhs2DArgs<T> a) {
     #pragma unroll
     for (int c = 0; c < 2; ++c) {
         C u[G::PP];
-        x_inverse<T, NX, NY>(a.W[in_slot[c][0]], y0, lds, a.twx, u, sc, nullptr, tid);
+        x_inverse<T, NX, NY>(a.W[in_slot[c][0]], y0, lds, twxs, u, sc, nullptr, tid);
         ph_store<T, NX, NY>(phys[c], u, y0, tid);
 #pragma unroll
         for (int q = 0; q < G::PP; ++q) f[q] = pmul(gm[q], u[q]);
-        x_fwd_store<T, NX, NY>(f, a.W[out_slot[c][0]], y0, lds, a.twx, tid);
+        x_fwd_store<T, NX, NY>(f, a.W[out_slot[c][0]], y0, lds, twxs, tid);
 #pragma unroll
         for (int q = 0; q < G::PP; ++q) f[q] = pmul(gp[q], u[q]);
-        x_fwd_store<T, NX, NY>(f, a.W[out_slot[c][1]], y0, lds, a.twx, tid);
-        x_inverse<T, NX, NY>(a.W[in_slot[c][1]], y0, lds, a.twx, f, sc, nullptr, tid);
+        x_fwd_store<T, NX, NY>(f, a.W[out_slot[c][1]], y0, lds, twxs, tid);
+        x_inverse<T, NX, NY>(a.W[in_slot[c][1]], y0, lds, twxs, f, sc, nullptr, tid);
 #pragma unroll
         for (int q = 0; q < G::PP; ++q) acc[q] = pmul(gm[q], f[q]);
-        x_inverse<T, NX, NY>(a.W[in_slot[c][2]], y0, lds, a.twx, f, sc, nullptr, tid);
+        x_inverse<T, NX, NY>(a.W[in_slot[c][2]], y0, lds, twxs, f, sc, nullptr, tid);
 #pragma unroll
         for (int q = 0; q < G::PP; ++q) acc[q] = pfma(gp[q], f[q], acc[q]);
-        x_fwd_store<T, NX, NY>(acc, a.W[out_slot[c][2]], y0, lds, a.twx, tid);
+        x_fwd_store<T, NX, NY>(acc, a.W[out_slot[c][2]], y0, lds, twxs, tid);
     }
 }
 
@@ -291,11 +299,11 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y2(Rhs2DArgs<T> a) {
     #pragma unroll
     for (int c = 0; c < 2; ++c) {
         y_load<T, NX, NY>(A, a.W[in_slot[c][0]], nullptr, col0, tid);
-        y_fwd_regs<T, NX, NY>(A, lds, a.twy, tid);
+        y_fwd_regs<T, NX, NY>(A, lds, twys, tid);
         y_load<T, NX, NY>(B, a.W[in_slot[c][1]], nullptr, col0, tid);
-        y_fwd_regs<T, NX, NY>(B, lds, a.twy, tid);
+        y_fwd_regs<T, NX, NY>(B, lds, twys, tid);
         y_load<T, NX, NY>(Cc, a.W[in_slot[c][2]], nullptr, col0, tid);
-        y_fwd_regs<T, NX, NY>(Cc, lds, a.twy, tid);
+        y_fwd_regs<T, NX, NY>(Cc, lds, twys, tid);
 #pragma unroll
         for (int e = 0; e < G::PE; ++e) {
             int kx, ky; size_t gi;
@@ -310,7 +318,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y2(Rhs2DArgs<T> a) {
                 A[e] = cmk<C>((T)0, (T)0);
             }
         }
-        y_inv_regs<T, NX, NY>(A, lds, a.twy, col0, tid);
+        y_inv_regs<T, NX, NY>(A, lds, twys, col0, tid);
         y_store<T, NX, NY>(a.W[c == 0 ? S_M0U : S_M0V], A, col0, tid);
     }
     y_store<T, NX, NY>(a.dH1, dsum, col0, tid);
@@ -323,8 +331,8 @@ template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x3(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
     C m0u[G::PP], m0v[G::PP], w[G::PP], uu[G::PP], vv[G::PP], f[G::PP];
-    x_inverse<T, NX, NY>(a.W[S_M0U], y0, lds, a.twx, m0u, sc, nullptr, tid);
-    x_inverse<T, NX, NY>(a.W[S_M0V], y0, lds, a.twx, m0v, sc, nullptr, tid);
+    x_inverse<T, NX, NY>(a.W[S_M0U], y0, lds, twxs, m0u, sc, nullptr, tid);
+    x_inverse<T, NX, NY>(a.W[S_M0V], y0, lds, twxs, m0v, sc, nullptr, tid);
     ph_load<T, NX, NY>(w, a.wallchar, y0, tid);
     ph_load<T, NX, NY>(uu, a.u_p, y0, tid);
     ph_load<T, NX, NY>(vv, a.v_p, y0, tid);
@@ -335,29 +343,29 @@ __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x3(Rhs2DArgs<T> a) {
         f[q] = cmk<C>((T)0.5 * ((uu[q].x * uu[q].x + vv[q].x * vv[q].x) - lp.x * lp.x) * w[q].x,
                       (T)0.5 * ((uu[q].y * uu[q].y + vv[q].y * vv[q].y) - lp.y * lp.y) * w[q].y);
     }
-    x_fwd_store<T, NX, NY>(f, a.W[S_E], y0, lds, a.twx, tid);
+    x_fwd_store<T, NX, NY>(f, a.W[S_E], y0, lds, twxs, tid);
     ph_load<T, NX, NY>(f, a.eta_p, y0, tid);   // f holds eta from here on
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) { C ew = pmul(f[q], w[q]); uu[q] = pmul(ew, uu[q]); vv[q] = pmul(ew, vv[q]);
                                       m0u[q] = pmul(ew, m0u[q]); m0v[q] = pmul(ew, m0v[q]); }
-    x_fwd_store<T, NX, NY>(uu, a.W[S_Q1], y0, lds, a.twx, tid);
-    x_fwd_store<T, NX, NY>(vv, a.W[S_Q2], y0, lds, a.twx, tid);
-    x_fwd_store<T, NX, NY>(m0u, a.W[S_P1], y0, lds, a.twx, tid);
-    x_fwd_store<T, NX, NY>(m0v, a.W[S_P2], y0, lds, a.twx, tid);
+    x_fwd_store<T, NX, NY>(uu, a.W[S_Q1], y0, lds, twxs, tid);
+    x_fwd_store<T, NX, NY>(vv, a.W[S_Q2], y0, lds, twxs, tid);
+    x_fwd_store<T, NX, NY>(m0u, a.W[S_P1], y0, lds, twxs, tid);
+    x_fwd_store<T, NX, NY>(m0v, a.W[S_P2], y0, lds, twxs, tid);
     ph_load<T, NX, NY>(w, a.gam_m, y0, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) f[q] = pmul(w[q], m0u[q]);
-    x_fwd_store<T, NX, NY>(f, a.W[S_G1M], y0, lds, a.twx, tid);
+    x_fwd_store<T, NX, NY>(f, a.W[S_G1M], y0, lds, twxs, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) f[q] = pmul(w[q], m0v[q]);
-    x_fwd_store<T, NX, NY>(f, a.W[S_G2M], y0, lds, a.twx, tid);
+    x_fwd_store<T, NX, NY>(f, a.W[S_G2M], y0, lds, twxs, tid);
     ph_load<T, NX, NY>(w, a.gam_p, y0, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) f[q] = pmul(w[q], m0u[q]);
-    x_fwd_store<T, NX, NY>(f, a.W[S_G1P], y0, lds, a.twx, tid);
+    x_fwd_store<T, NX, NY>(f, a.W[S_G1P], y0, lds, twxs, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) f[q] = pmul(w[q], m0v[q]);
-    x_fwd_store<T, NX, NY>(f, a.W[S_G2P], y0, lds, a.twx, tid);
+    x_fwd_store<T, NX, NY>(f, a.W[S_G2P], y0, lds, twxs, tid);
 }
 
 // W slots after Y4 (inputs S_P1.. are consumed)
@@ -373,13 +381,13 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y4(Rhs2DArgs<T> a) {
     #pragma unroll
     for (int c = 0; c < 2; ++c) {
         y_load<T, NX, NY>(P, a.W[sl[c][0]], nullptr, col0, tid);
-        y_fwd_regs<T, NX, NY>(P, lds, a.twy, tid);
+        y_fwd_regs<T, NX, NY>(P, lds, twys, tid);
         y_load<T, NX, NY>(Q, a.W[sl[c][1]], nullptr, col0, tid);
-        y_fwd_regs<T, NX, NY>(Q, lds, a.twy, tid);
+        y_fwd_regs<T, NX, NY>(Q, lds, twys, tid);
         y_load<T, NX, NY>(Gm, a.W[sl[c][2]], nullptr, col0, tid);
-        y_fwd_regs<T, NX, NY>(Gm, lds, a.twy, tid);
+        y_fwd_regs<T, NX, NY>(Gm, lds, twys, tid);
         y_load<T, NX, NY>(Gp, a.W[sl[c][3]], nullptr, col0, tid);
-        y_fwd_regs<T, NX, NY>(Gp, lds, a.twy, tid);
+        y_fwd_regs<T, NX, NY>(Gp, lds, twys, tid);
 #pragma unroll
         for (int e = 0; e < G::PE; ++e) {
             int kx, ky; size_t gi;
@@ -402,14 +410,14 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y4(Rhs2DArgs<T> a) {
                 YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
                 w[e] = (kx < NX / 2 + 1) ? cscale(P[e], mul[gi]) : cmk<C>((T)0, (T)0);
             }
-            y_inv_regs<T, NX, NY>(w, lds, a.twy, col0, tid);
+            y_inv_regs<T, NX, NY>(w, lds, twys, col0, tid);
             y_store<T, NX, NY>(a.W[so[c][s]], w, col0, tid);
         }
     }
     y_store<T, NX, NY>(a.dH2p, part, col0, tid);
     // adjust inputs i kx E^, i ky E^
     y_load<T, NX, NY>(P, a.W[S_E], nullptr, col0, tid);
-    y_fwd_regs<T, NX, NY>(P, lds, a.twy, tid);
+    y_fwd_regs<T, NX, NY>(P, lds, twys, tid);
     #pragma unroll
     for (int s = 0; s < 2; ++s) {
 #pragma unroll
@@ -419,7 +427,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y4(Rhs2DArgs<T> a) {
             T kk = (kx < NX / 2 + 1) ? (s == 0 ? a.kxh[kx] : a.ky[ky]) : (T)0;
             w[e] = cmul_i<1>(cscale(P[e], kk));
         }
-        y_inv_regs<T, NX, NY>(w, lds, a.twy, col0, tid);
+        y_inv_regs<T, NX, NY>(w, lds, twys, col0, tid);
         y_store<T, NX, NY>(a.W[s == 0 ? S_AX : S_AY], w, col0, tid);
     }
 }
@@ -429,32 +437,32 @@ enum : int { S_ACC1 = 2, S_ACC2 = 3, S_ZU = 4, S_ZV = 5 };
 
 template <typename T, int NX, int NY>
 __device__ __forceinline__ void x5_hss_tail(const Rhs2DArgs<T>& a, int sm, int sp, int so, int y0,
-                                            typename ctype<T>::type* lds, int tid) {
+                                            typename ctype<T>::type* lds, const typename ctype<T>::type* twl, int tid) {
     using C = typename ctype<T>::type;
     using G = XGeo<T, NX>;
     const T sc = (T)1 / ((T)NX * (T)NY);
     C f[G::PP], acc[G::PP], gg[G::PP];
-    x_inverse<T, NX, NY>(a.W[sm], y0, lds, a.twx, f, sc, nullptr, tid);
+    x_inverse<T, NX, NY>(a.W[sm], y0, lds, twl, f, sc, nullptr, tid);
     ph_load<T, NX, NY>(gg, a.gam_m, y0, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) acc[q] = pmul(gg[q], f[q]);
-    x_inverse<T, NX, NY>(a.W[sp], y0, lds, a.twx, f, sc, nullptr, tid);
+    x_inverse<T, NX, NY>(a.W[sp], y0, lds, twl, f, sc, nullptr, tid);
     ph_load<T, NX, NY>(gg, a.gam_p, y0, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) acc[q] = pfma(gg[q], f[q], acc[q]);
-    x_fwd_store<T, NX, NY>(acc, a.W[so], y0, lds, a.twx, tid);
+    x_fwd_store<T, NX, NY>(acc, a.W[so], y0, lds, twl, tid);
 }
 
 // output transform of a velocity equation: f1 = Sb - chi A - fbdy7 vel,
 // f2 = -chi (-1)^x s(y)   (the imaginary part of fun_adjust's inverse)
 template <typename T, int NX, int NY>
 __device__ __forceinline__ void x_vel_out(const Rhs2DArgs<T>& a, int sa, const T* vel, const T* sb, int so, int y0,
-                                          typename ctype<T>::type* lds, T* nyq, int tid) {
+                                          typename ctype<T>::type* lds, const typename ctype<T>::type* twl, T* nyq, int tid) {
     using C = typename ctype<T>::type;
     using G = XGeo<T, NX>;
     const T sc = (T)1 / ((T)NX * (T)NY);
     C f1[G::PP], f2[G::PP], t[G::PP];
-    x_inverse<T, NX, NY>(a.W[sa], y0, lds, a.twx, f1, sc, nyq, tid);
+    x_inverse<T, NX, NY>(a.W[sa], y0, lds, twl, f1, sc, nyq, tid);
     ph_load<T, NX, NY>(t, a.chi, y0, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) {
@@ -474,16 +482,16 @@ __device__ __forceinline__ void x_vel_out(const Rhs2DArgs<T>& a, int sa, const T
 #pragma unroll
         for (int q = 0; q < G::PP; ++q) f1[q] = cadd(f1[q], t[q]);
     }
-    x_fwd2_store<T, NX, NY>(f1, f2, a.W[so], y0, lds, a.twx, tid);
+    x_fwd2_store<T, NX, NY>(f1, f2, a.W[so], y0, lds, twl, tid);
 }
 
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x5(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
-    x5_hss_tail<T, NX, NY>(a, S_B1M, S_B1P, S_ACC1, y0, lds, tid);
-    x5_hss_tail<T, NX, NY>(a, S_B2M, S_B2P, S_ACC2, y0, lds, tid);
-    x_vel_out<T, NX, NY>(a, S_AX, a.u_p, a.sbx, S_ZU, y0, lds, nyq, tid);
-    x_vel_out<T, NX, NY>(a, S_AY, a.v_p, a.sby, S_ZV, y0, lds, nyq, tid);
+    x5_hss_tail<T, NX, NY>(a, S_B1M, S_B1P, S_ACC1, y0, lds, twxs, tid);
+    x5_hss_tail<T, NX, NY>(a, S_B2M, S_B2P, S_ACC2, y0, lds, twxs, tid);
+    x_vel_out<T, NX, NY>(a, S_AX, a.u_p, a.sbx, S_ZU, y0, lds, twxs, nyq, tid);
+    x_vel_out<T, NX, NY>(a, S_AY, a.v_p, a.sby, S_ZV, y0, lds, twxs, nyq, tid);
 }
 
 // W slots after Y6
@@ -495,19 +503,19 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y6(Rhs2DArgs<T> a) {
     C F1[G::PE], F2[G::PE], Z[G::PE], eh[G::PE];
     const T ig = (T)1 / a.g;
     y_load<T, NX, NY>(F1, a.W[S_ACC1], nullptr, col0, tid);
-    y_fwd_regs<T, NX, NY>(F1, lds, a.twy, tid);
+    y_fwd_regs<T, NX, NY>(F1, lds, twys, tid);
     y_load<T, NX, NY>(F2, a.W[S_ACC2], nullptr, col0, tid);
-    y_fwd_regs<T, NX, NY>(F2, lds, a.twy, tid);
+    y_fwd_regs<T, NX, NY>(F2, lds, twys, tid);
     y_load<T, NX, NY>(Z, a.dH2p, nullptr, col0, tid);
 #pragma unroll
     for (int e = 0; e < G::PE; ++e) Z[e] = csub(Z[e], cscale(cadd(F1[e], F2[e]), (T)0.5 * ig));
-    y_inv_regs<T, NX, NY>(Z, lds, a.twy, col0, tid);
+    y_inv_regs<T, NX, NY>(Z, lds, twys, col0, tid);
     y_store<T, NX, NY>(a.W[S_AETA], Z, col0, tid);
     y_load<T, NX, NY>(eh, a.eta_hat, nullptr, col0, tid);
     #pragma unroll
     for (int c = 0; c < 2; ++c) {
         y_load<T, NX, NY>(Z, a.W[c == 0 ? S_ZU : S_ZV], nullptr, col0, tid);
-        y_fwd_regs<T, NX, NY>(Z, lds, a.twy, tid);
+        y_fwd_regs<T, NX, NY>(Z, lds, twys, tid);
 #pragma unroll
         for (int e = 0; e < G::PE; ++e) {
             int kx, ky; size_t gi;
@@ -551,12 +559,12 @@ __device__ __forceinline__ void src_pair(const Rhs2DArgs<T>& a, int y0, int pos,
 // f2 = chi (-1)^x s(y)
 template <typename T, int NX, int NY>
 __device__ __forceinline__ void x_eta_out(const Rhs2DArgs<T>& a, int sa, int so, int y0,
-                                          typename ctype<T>::type* lds, T* nyq, int tid) {
+                                          typename ctype<T>::type* lds, const typename ctype<T>::type* twl, T* nyq, int tid) {
     using C = typename ctype<T>::type;
     using G = XGeo<T, NX>;
     const T sc = (T)1 / ((T)NX * (T)NY);
     C f1[G::PP], f2[G::PP], t[G::PP], t2[G::PP];
-    x_inverse<T, NX, NY>(a.W[sa], y0, lds, a.twx, f1, sc, nyq, tid);
+    x_inverse<T, NX, NY>(a.W[sa], y0, lds, twl, f1, sc, nyq, tid);
     if (a.s_eta_out && tid < G::RR) a.s_eta_out[y0 + tid] = nyq[tid];
     ph_load<T, NX, NY>(t, a.chi, y0, tid);
 #pragma unroll
@@ -582,13 +590,13 @@ __device__ __forceinline__ void x_eta_out(const Rhs2DArgs<T>& a, int sa, int so,
             f1[q] = cadd(f1[q], cmk<C>(s0, s1));
         }
     }
-    x_fwd2_store<T, NX, NY>(f1, f2, a.W[so], y0, lds, a.twx, tid);
+    x_fwd2_store<T, NX, NY>(f1, f2, a.W[so], y0, lds, twl, tid);
 }
 
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x7(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
-    x_eta_out<T, NX, NY>(a, S_AETA, S_ZU, y0, lds, nyq, tid);
+    x_eta_out<T, NX, NY>(a, S_AETA, S_ZU, y0, lds, twxs, nyq, tid);
 }
 
 template <typename T, int NX, int NY>
@@ -596,7 +604,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y8(Rhs2DArgs<T> a) {
     HW_YK_PROLOGUE
     C Z[G::PE], d[G::PE];
     y_load<T, NX, NY>(Z, a.W[S_ZU], nullptr, col0, tid);
-    y_fwd_regs<T, NX, NY>(Z, lds, a.twy, tid);
+    y_fwd_regs<T, NX, NY>(Z, lds, twys, tid);
     y_load<T, NX, NY>(d, a.dH1, nullptr, col0, tid);
 #pragma unroll
     for (int e = 0; e < G::PE; ++e) {
@@ -622,7 +630,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2f_y1(Rhs2DArgs<T> a) {
     HW_YK_PROLOGUE
     C v[G::PE], w[G::PE];
     y_load<T, NX, NY>(v, a.eta_hat, nullptr, col0, tid);
-    y_inv_regs<T, NX, NY>(v, lds, a.twy, col0, tid);
+    y_inv_regs<T, NX, NY>(v, lds, twys, col0, tid);
     y_store<T, NX, NY>(a.W[F_ETA], v, col0, tid);
     #pragma unroll
     for (int c = 0; c < 2; ++c) {
@@ -634,9 +642,9 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2f_y1(Rhs2DArgs<T> a) {
             T kk = (kx < NX / 2 + 1) ? (c == 0 ? a.kxh[kx] : a.ky[ky]) * a.c2d2g[gi] : (T)0;
             w[e] = cmul_i<-1>(cscale(v[e], kk));
         }
-        y_inv_regs<T, NX, NY>(v, lds, a.twy, col0, tid);
+        y_inv_regs<T, NX, NY>(v, lds, twys, col0, tid);
         y_store<T, NX, NY>(a.W[c == 0 ? F_U : F_V], v, col0, tid);
-        y_inv_regs<T, NX, NY>(w, lds, a.twy, col0, tid);
+        y_inv_regs<T, NX, NY>(w, lds, twys, col0, tid);
         y_store<T, NX, NY>(a.W[c == 0 ? F_M0U : F_M0V], w, col0, tid);
     }
 }
@@ -647,34 +655,34 @@ template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2f_x2(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
     C e[G::PP], w[G::PP], u[G::PP], lp[G::PP], m[G::PP], f[G::PP];
-    x_inverse<T, NX, NY>(a.W[F_ETA], y0, lds, a.twx, e, sc, nullptr, tid);
+    x_inverse<T, NX, NY>(a.W[F_ETA], y0, lds, twxs, e, sc, nullptr, tid);
     ph_store<T, NX, NY>(a.eta_p, e, y0, tid);
     ph_load<T, NX, NY>(w, a.wallchar, y0, tid);
-    x_inverse<T, NX, NY>(a.W[F_U], y0, lds, a.twx, u, sc, nullptr, tid);
+    x_inverse<T, NX, NY>(a.W[F_U], y0, lds, twxs, u, sc, nullptr, tid);
     ph_store<T, NX, NY>(a.u_p, u, y0, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) f[q] = pmul(pmul(e[q], u[q]), w[q]);
-    x_fwd_store<T, NX, NY>(f, a.W[F_Q1], y0, lds, a.twx, tid);
-    x_inverse<T, NX, NY>(a.W[F_V], y0, lds, a.twx, m, sc, nullptr, tid);
+    x_fwd_store<T, NX, NY>(f, a.W[F_Q1], y0, lds, twxs, tid);
+    x_inverse<T, NX, NY>(a.W[F_V], y0, lds, twxs, m, sc, nullptr, tid);
     ph_store<T, NX, NY>(a.v_p, m, y0, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) {
         f[q] = pmul(pmul(e[q], m[q]), w[q]);
         u[q] = cmk<C>(u[q].x * u[q].x + m[q].x * m[q].x, u[q].y * u[q].y + m[q].y * m[q].y);   // u^2 + v^2
     }
-    x_fwd_store<T, NX, NY>(f, a.W[F_Q2], y0, lds, a.twx, tid);
-    x_inverse<T, NX, NY>(a.W[F_M0U], y0, lds, a.twx, m, sc, nullptr, tid);
+    x_fwd_store<T, NX, NY>(f, a.W[F_Q2], y0, lds, twxs, tid);
+    x_inverse<T, NX, NY>(a.W[F_M0U], y0, lds, twxs, m, sc, nullptr, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) { f[q] = pmul(pmul(e[q], m[q]), w[q]); lp[q] = m[q]; }
-    x_fwd_store<T, NX, NY>(f, a.W[F_P1], y0, lds, a.twx, tid);
-    x_inverse<T, NX, NY>(a.W[F_M0V], y0, lds, a.twx, m, sc, nullptr, tid);
+    x_fwd_store<T, NX, NY>(f, a.W[F_P1], y0, lds, twxs, tid);
+    x_inverse<T, NX, NY>(a.W[F_M0V], y0, lds, twxs, m, sc, nullptr, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) { f[q] = pmul(pmul(e[q], m[q]), w[q]); lp[q] = cadd(lp[q], m[q]); }
-    x_fwd_store<T, NX, NY>(f, a.W[F_P2], y0, lds, a.twx, tid);
+    x_fwd_store<T, NX, NY>(f, a.W[F_P2], y0, lds, twxs, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q)
         f[q] = cmk<C>((T)0.5 * (u[q].x - lp[q].x * lp[q].x) * w[q].x, (T)0.5 * (u[q].y - lp[q].y * lp[q].y) * w[q].y);
-    x_fwd_store<T, NX, NY>(f, a.W[F_E], y0, lds, a.twx, tid);
+    x_fwd_store<T, NX, NY>(f, a.W[F_E], y0, lds, twxs, tid);
 }
 
 enum : int { F_AETA = 0, F_AX = 1, F_AY = 2 };
@@ -687,9 +695,9 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2f_y3(Rhs2DArgs<T> a) {
     #pragma unroll
     for (int c = 0; c < 2; ++c) {
         y_load<T, NX, NY>(P, a.W[sl[c][0]], nullptr, col0, tid);
-        y_fwd_regs<T, NX, NY>(P, lds, a.twy, tid);
+        y_fwd_regs<T, NX, NY>(P, lds, twys, tid);
         y_load<T, NX, NY>(Q, a.W[sl[c][1]], nullptr, col0, tid);
-        y_fwd_regs<T, NX, NY>(Q, lds, a.twy, tid);
+        y_fwd_regs<T, NX, NY>(Q, lds, twys, tid);
 #pragma unroll
         for (int e = 0; e < G::PE; ++e) {
             int kx, ky; size_t gi;
@@ -704,10 +712,10 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2f_y3(Rhs2DArgs<T> a) {
             }
         }
     }
-    y_inv_regs<T, NX, NY>(acc, lds, a.twy, col0, tid);
+    y_inv_regs<T, NX, NY>(acc, lds, twys, col0, tid);
     y_store<T, NX, NY>(a.W[F_AETA], acc, col0, tid);
     y_load<T, NX, NY>(P, a.W[F_E], nullptr, col0, tid);
-    y_fwd_regs<T, NX, NY>(P, lds, a.twy, tid);
+    y_fwd_regs<T, NX, NY>(P, lds, twys, tid);
     #pragma unroll
     for (int s = 0; s < 2; ++s) {
 #pragma unroll
@@ -717,7 +725,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2f_y3(Rhs2DArgs<T> a) {
             T kk = (kx < NX / 2 + 1) ? (s == 0 ? a.kxh[kx] : a.ky[ky]) : (T)0;
             Q[e] = cmul_i<1>(cscale(P[e], kk));
         }
-        y_inv_regs<T, NX, NY>(Q, lds, a.twy, col0, tid);
+        y_inv_regs<T, NX, NY>(Q, lds, twys, col0, tid);
         y_store<T, NX, NY>(a.W[s == 0 ? F_AX : F_AY], Q, col0, tid);
     }
 }
@@ -727,9 +735,9 @@ enum : int { F_ZETA = 3, F_ZU = 4, F_ZV = 5 };
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2f_x4(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
-    x_eta_out<T, NX, NY>(a, F_AETA, F_ZETA, y0, lds, nyq, tid);
-    x_vel_out<T, NX, NY>(a, F_AX, a.u_p, a.sbx, F_ZU, y0, lds, nyq, tid);
-    x_vel_out<T, NX, NY>(a, F_AY, a.v_p, a.sby, F_ZV, y0, lds, nyq, tid);
+    x_eta_out<T, NX, NY>(a, F_AETA, F_ZETA, y0, lds, twxs, nyq, tid);
+    x_vel_out<T, NX, NY>(a, F_AX, a.u_p, a.sbx, F_ZU, y0, lds, twxs, nyq, tid);
+    x_vel_out<T, NX, NY>(a, F_AY, a.v_p, a.sby, F_ZV, y0, lds, twxs, nyq, tid);
 }
 
 template <typename T, int NX, int NY>
@@ -744,7 +752,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2f_y5(Rhs2DArgs<T> a) {
     #pragma unroll
     for (int c = 0; c < 3; ++c) {
         y_load<T, NX, NY>(Z, a.W[zs[c]], nullptr, col0, tid);
-        y_fwd_regs<T, NX, NY>(Z, lds, a.twy, tid);
+        y_fwd_regs<T, NX, NY>(Z, lds, twys, tid);
 #pragma unroll
         for (int e = 0; e < G::PE; ++e) {
             int kx, ky; size_t gi;
