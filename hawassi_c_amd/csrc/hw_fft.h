@@ -72,13 +72,50 @@ template <int DIR, typename C> __device__ __forceinline__ void dft8(C* v) {
     v[3] = cadd(e[3], w3);   v[7] = csub(e[3], w3);
 }
 
+// 16-point DFT as 4 x 4: n = 4 n1 + n2, k = k1 + 4 k2; inner twiddles
+// W16^(n2 k1), W16 = exp(DIR 2 pi i / 16)
+template <int DIR, typename C> __device__ __forceinline__ void dft16(C* v) {
+    using R = decltype(C::x);
+    constexpr double cs[10] = {1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173, 0.0,
+                               -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613, -1.0,
+                               -0.92387953251128675613};
+    constexpr double sn[10] = {0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613, 1.0,
+                               0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173, 0.0,
+                               -0.38268343236508977173};
+    C a[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        C t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+        dft4<DIR>(t);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            const int m = n2 * k1;
+            if (m == 0) a[n2][k1] = t[k1];
+            else a[n2][k1] = cmul(t[k1], cmk<C>((R)cs[m], (R)((double)DIR * sn[m])));
+        }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        C t[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+        dft4<DIR>(t);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+    }
+}
+
 template <int R, int DIR, typename C> __device__ __forceinline__ void dftR(C* v) {
-    if constexpr (R == 8) dft8<DIR>(v);
+    if constexpr (R == 16) dft16<DIR>(v);
+    else if constexpr (R == 8) dft8<DIR>(v);
     else if constexpr (R == 4) dft4<DIR>(v);
     else dft2<DIR>(v);
 }
 
-constexpr int pass_radix(int rem) { return rem % 8 == 0 ? 8 : (rem % 4 == 0 ? 4 : 2); }
+#ifndef HW_MAX_RADIX
+#define HW_MAX_RADIX 8
+#endif
+constexpr int pass_radix(int rem) {
+    return (HW_MAX_RADIX >= 16 && rem % 16 == 0) ? 16 : (rem % 8 == 0 ? 8 : (rem % 4 == 0 ? 4 : 2));
+}
 
 // ------------------------------------------------------- Stockham in LDS --
 // B sequences of length L live at lds[b*LS + n].  tw[m*TS] = exp(-2 pi i m/L).
@@ -108,11 +145,36 @@ __device__ __forceinline__ void fft_lds_pass(C* lds, int LS, const C* __restrict
                 int b = bf / M, j = bf % M;
                 int k = j % NS;
                 if constexpr (NS > 1) {
+                    const int ex = k * (L / (NS * R));
+                    if constexpr (R <= 4) {
 #pragma unroll
-                    for (int r = 1; r < R; ++r) {
-                        C w = tw[(r * k * (L / (NS * R))) * TS];
-                        if (DIR > 0) w = cconj(w);
-                        v[p][r] = cmul(v[p][r], w);
+                        for (int r = 1; r < R; ++r) {
+                            C w = tw[(r * ex) * TS];
+                            if (DIR > 0) w = cconj(w);
+                            v[p][r] = cmul(v[p][r], w);
+                        }
+                    } else {
+                        // w^r from w, w^4 (and w^8) read from the table:
+                        // at most two roundings more than a table entry
+                        C wp[R];
+                        C w1 = tw[ex * TS], w4 = tw[4 * ex * TS];
+                        if (DIR > 0) { w1 = cconj(w1); w4 = cconj(w4); }
+                        wp[1] = w1;
+                        wp[2] = cmul(w1, w1);
+                        wp[3] = cmul(wp[2], w1);
+                        wp[4] = w4;
+                        wp[5] = cmul(w4, w1);
+                        wp[6] = cmul(w4, wp[2]);
+                        wp[7] = cmul(w4, wp[3]);
+                        if constexpr (R == 16) {
+                            C w8 = tw[8 * ex * TS];
+                            if (DIR > 0) w8 = cconj(w8);
+                            wp[8] = w8;
+#pragma unroll
+                            for (int r = 1; r < 8; ++r) wp[8 + r] = cmul(w8, wp[r]);
+                        }
+#pragma unroll
+                        for (int r = 1; r < R; ++r) v[p][r] = cmul(v[p][r], wp[r]);
                     }
                 }
                 dftR<R, DIR>(v[p]);
@@ -134,8 +196,11 @@ __device__ __forceinline__ void fft_lds(C* lds, int LS, const C* __restrict__ tw
 // ------------------------------------------------------- pass geometry ----
 // Elements each thread holds between transforms: fewer elements per thread
 // means fewer live registers and more waves in flight per CU.
+#ifndef HW_X_RUN_BYTES
+#define HW_X_RUN_BYTES 16
+#endif
 #ifndef HW_X_PER_THREAD
-#define HW_X_PER_THREAD 4
+#define HW_X_PER_THREAD 8
 #endif
 #ifndef HW_Y_PER_THREAD
 #define HW_Y_PER_THREAD 4
@@ -145,7 +210,8 @@ __device__ __forceinline__ void fft_lds(C* lds, int LS, const C* __restrict__ tw
 // the row groups sharing a 128-byte line run on the same XCD (same L2).
 template <typename T, int NX> struct XGeo {
     using C = typename ctype<T>::type;
-    static constexpr int RR = 32 / (int)sizeof(C);     // 4 (fp32) or 2 (fp64)
+    static constexpr int RR = HW_X_RUN_BYTES / (int)sizeof(C);   // rows per block: 32-byte runs
+    static_assert(RR >= 1, "run shorter than one element");
     static constexpr int L = NX / 2;                    // half-length complex FFT
     static constexpr int LS = L + 2;                    // LDS row stride (complex)
     static constexpr int NPAIR = RR * L;                // physical (x,x+1) pairs per block
@@ -190,20 +256,41 @@ template <typename C> struct Tw { const C* x; const C* y; };
 // Also returns, per row of the block, Im of the Nyquist bin times `scale`
 // in nyq_im[row] (LDS scalar array of RR entries), which the adjust operator
 // needs for the reference's complex inverse.
+// The inverse x-pass in two halves so a kernel can issue the (transposed,
+// 32-byte-run) global loads of the next field before running the FFT of the
+// current one: x_spec_load only issues loads into registers; x_inverse_b
+// consumes them.
+template <typename T, int NX> using XSpec = carr<T, XGeo<T, NX>::PS>;
+
 template <typename T, int NX, int NY>
-__device__ __forceinline__ void x_inverse(const typename ctype<T>::type* __restrict__ X, int y0,
-                                          typename ctype<T>::type* lds, const typename ctype<T>::type* __restrict__ twx,
-                                          carr<T, XGeo<T, NX>::PP>& out, T scale, T* nyq_im, int tid) {
+__device__ __forceinline__ void x_spec_load(const typename ctype<T>::type* __restrict__ X, int y0, XSpec<T, NX>& b,
+                                            int tid) {
+    using G = XGeo<T, NX>;
+#pragma unroll
+    for (int q = 0; q < G::PS; ++q) {
+        int idx = tid + q * G::NT;
+        if (G::NSPEC % G::NT == 0 || idx < G::NSPEC) {
+            int kx = idx / G::RR, r = idx % G::RR;
+            b[q] = X[(size_t)kx * NY + y0 + r];
+        }
+    }
+}
+
+template <typename T, int NX, int NY>
+__device__ __forceinline__ void x_inverse_b(const XSpec<T, NX>& b, int y0,
+                                            typename ctype<T>::type* lds, const typename ctype<T>::type* __restrict__ twx,
+                                            carr<T, XGeo<T, NX>::PP>& out, T scale, T* nyq_im, int tid) {
     using G = XGeo<T, NX>;
     using C = typename ctype<T>::type;
     constexpr int L = G::L, RR = G::RR, LS = G::LS, NT = G::NT;
+    (void)y0;
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < G::PS; ++q) {
         int idx = tid + q * NT;
         if (G::NSPEC % NT == 0 || idx < G::NSPEC) {
             int kx = idx / RR, r = idx % RR;
-            lds[r * LS + kx] = X[(size_t)kx * NY + y0 + r];
+            lds[r * LS + kx] = b[q];
         }
     }
     __syncthreads();
@@ -242,6 +329,15 @@ __device__ __forceinline__ void x_inverse(const typename ctype<T>::type* __restr
             out[q] = cscale(lds[r * LS + n], scale);
         }
     }
+}
+
+template <typename T, int NX, int NY>
+__device__ __forceinline__ void x_inverse(const typename ctype<T>::type* __restrict__ X, int y0,
+                                          typename ctype<T>::type* lds, const typename ctype<T>::type* __restrict__ twx,
+                                          carr<T, XGeo<T, NX>::PP>& out, T scale, T* nyq_im, int tid) {
+    XSpec<T, NX> b;
+    x_spec_load<T, NX, NY>(X, y0, b, tid);
+    x_inverse_b<T, NX, NY>(b, y0, lds, twx, out, scale, nyq_im, tid);
 }
 
 // Forward x-pass of a real field given as (x[2n], x[2n+1]) pairs in registers.

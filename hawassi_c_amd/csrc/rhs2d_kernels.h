@@ -227,28 +227,34 @@ enum : int {
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y1(Rhs2DArgs<T> a) {
     HW_YK_PROLOGUE
-    C v[G::PE], w[G::PE];
-    y_load<T, NX, NY>(v, a.eta_hat, nullptr, col0, tid);
-    y_inv_regs<T, NX, NY>(v, lds, twys, col0, tid);
-    y_store<T, NX, NY>(a.W[S_ETA], v, col0, tid);
-    const C* src[2] = {a.u_hat, a.v_hat};
+    // all inputs and multipliers are loaded up front; the transforms follow
+    C e0[G::PE], uu[G::PE], vv[G::PE], w[G::PE];
+    T cm[G::PE], cp[G::PE];
+    y_load<T, NX, NY>(e0, a.eta_hat, nullptr, col0, tid);
+    y_load<T, NX, NY>(uu, a.u_hat, nullptr, col0, tid);
+    y_load<T, NX, NY>(vv, a.v_hat, nullptr, col0, tid);
+#pragma unroll
+    for (int e = 0; e < G::PE; ++e) {
+        int kx, ky; size_t gi;
+        YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
+        const bool in = kx < NX / 2 + 1;
+        cm[e] = in ? a.csq_m[gi] : (T)0;
+        cp[e] = in ? a.csq_p[gi] : (T)0;
+    }
+    y_inv_regs<T, NX, NY>(e0, lds, twys, col0, tid);
+    y_store<T, NX, NY>(a.W[S_ETA], e0, col0, tid);
     const int slot[2][3] = {{S_U, S_AMU, S_APU}, {S_V, S_AMV, S_APV}};
     #pragma unroll
     for (int f = 0; f < 2; ++f) {
-        y_load<T, NX, NY>(v, src[f], nullptr, col0, tid);
+        auto& v = f == 0 ? uu : vv;
 #pragma unroll
         for (int e = 0; e < G::PE; ++e) w[e] = v[e];
         y_inv_regs<T, NX, NY>(w, lds, twys, col0, tid);
         y_store<T, NX, NY>(a.W[slot[f][0]], w, col0, tid);
         #pragma unroll
         for (int s = 0; s < 2; ++s) {
-            const T* mul = s == 0 ? a.csq_m : a.csq_p;
 #pragma unroll
-            for (int e = 0; e < G::PE; ++e) {
-                int kx, ky; size_t gi;
-                YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
-                w[e] = (kx < NX / 2 + 1) ? cscale(v[e], mul[gi]) : cmk<C>((T)0, (T)0);
-            }
+            for (int e = 0; e < G::PE; ++e) w[e] = cscale(v[e], s == 0 ? cm[e] : cp[e]);
             y_inv_regs<T, NX, NY>(w, lds, twys, col0, tid);
             y_store<T, NX, NY>(a.W[slot[f][1 + s]], w, col0, tid);
         }
@@ -258,33 +264,53 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y1(Rhs2DArgs<T> a) {
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x1(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
-    C f[G::PP], acc[G::PP], gm[G::PP], gp[G::PP];
-    x_inverse<T, NX, NY>(a.W[S_ETA], y0, lds, twxs, f, sc, nullptr, tid);
-    ph_store<T, NX, NY>(a.eta_p, f, y0, tid);
+    // eta, u, v and the HSS inverses of u^ and v^; a rolling pair of load
+    // buffers keeps the next field's global loads in flight during each FFT
+    C f[G::PP], acc[G::PP], gm[G::PP], gp[G::PP], u[G::PP];
+    XSpec<T, NX> b0, b1;
+    x_spec_load<T, NX, NY>(a.W[S_ETA], y0, b0, tid);
+    x_spec_load<T, NX, NY>(a.W[S_U], y0, b1, tid);
     ph_load<T, NX, NY>(gm, a.gam_m, y0, tid);
     ph_load<T, NX, NY>(gp, a.gam_p, y0, tid);
-    const int in_slot[2][3] = {{S_U, S_AMU, S_APU}, {S_V, S_AMV, S_APV}};
-    const int out_slot[2][3] = {{S_FGMU, S_FGPU, S_FGAU}, {S_FGMV, S_FGPV, S_FGAV}};
-    T* phys[2] = {a.u_p, a.v_p};
-    #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-        C u[G::PP];
-        x_inverse<T, NX, NY>(a.W[in_slot[c][0]], y0, lds, twxs, u, sc, nullptr, tid);
-        ph_store<T, NX, NY>(phys[c], u, y0, tid);
+    x_inverse_b<T, NX, NY>(b0, y0, lds, twxs, f, sc, nullptr, tid);
+    x_spec_load<T, NX, NY>(a.W[S_AMU], y0, b0, tid);
+    ph_store<T, NX, NY>(a.eta_p, f, y0, tid);
+    // --- u: b1 = U, b0 = AMU
+    x_inverse_b<T, NX, NY>(b1, y0, lds, twxs, u, sc, nullptr, tid);
+    x_spec_load<T, NX, NY>(a.W[S_APU], y0, b1, tid);
+    ph_store<T, NX, NY>(a.u_p, u, y0, tid);
 #pragma unroll
-        for (int q = 0; q < G::PP; ++q) f[q] = pmul(gm[q], u[q]);
-        x_fwd_store<T, NX, NY>(f, a.W[out_slot[c][0]], y0, lds, twxs, tid);
+    for (int q = 0; q < G::PP; ++q) f[q] = pmul(gm[q], u[q]);
+    x_fwd_store<T, NX, NY>(f, a.W[S_FGMU], y0, lds, twxs, tid);
 #pragma unroll
-        for (int q = 0; q < G::PP; ++q) f[q] = pmul(gp[q], u[q]);
-        x_fwd_store<T, NX, NY>(f, a.W[out_slot[c][1]], y0, lds, twxs, tid);
-        x_inverse<T, NX, NY>(a.W[in_slot[c][1]], y0, lds, twxs, f, sc, nullptr, tid);
+    for (int q = 0; q < G::PP; ++q) f[q] = pmul(gp[q], u[q]);
+    x_fwd_store<T, NX, NY>(f, a.W[S_FGPU], y0, lds, twxs, tid);
+    x_inverse_b<T, NX, NY>(b0, y0, lds, twxs, f, sc, nullptr, tid);
+    x_spec_load<T, NX, NY>(a.W[S_V], y0, b0, tid);
 #pragma unroll
-        for (int q = 0; q < G::PP; ++q) acc[q] = pmul(gm[q], f[q]);
-        x_inverse<T, NX, NY>(a.W[in_slot[c][2]], y0, lds, twxs, f, sc, nullptr, tid);
+    for (int q = 0; q < G::PP; ++q) acc[q] = pmul(gm[q], f[q]);
+    x_inverse_b<T, NX, NY>(b1, y0, lds, twxs, f, sc, nullptr, tid);
+    x_spec_load<T, NX, NY>(a.W[S_AMV], y0, b1, tid);
 #pragma unroll
-        for (int q = 0; q < G::PP; ++q) acc[q] = pfma(gp[q], f[q], acc[q]);
-        x_fwd_store<T, NX, NY>(acc, a.W[out_slot[c][2]], y0, lds, twxs, tid);
-    }
+    for (int q = 0; q < G::PP; ++q) acc[q] = pfma(gp[q], f[q], acc[q]);
+    x_fwd_store<T, NX, NY>(acc, a.W[S_FGAU], y0, lds, twxs, tid);
+    // --- v: b0 = V, b1 = AMV
+    x_inverse_b<T, NX, NY>(b0, y0, lds, twxs, u, sc, nullptr, tid);
+    x_spec_load<T, NX, NY>(a.W[S_APV], y0, b0, tid);
+    ph_store<T, NX, NY>(a.v_p, u, y0, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) f[q] = pmul(gm[q], u[q]);
+    x_fwd_store<T, NX, NY>(f, a.W[S_FGMV], y0, lds, twxs, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) f[q] = pmul(gp[q], u[q]);
+    x_fwd_store<T, NX, NY>(f, a.W[S_FGPV], y0, lds, twxs, tid);
+    x_inverse_b<T, NX, NY>(b1, y0, lds, twxs, f, sc, nullptr, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) acc[q] = pmul(gm[q], f[q]);
+    x_inverse_b<T, NX, NY>(b0, y0, lds, twxs, f, sc, nullptr, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) acc[q] = pfma(gp[q], f[q], acc[q]);
+    x_fwd_store<T, NX, NY>(acc, a.W[S_FGAV], y0, lds, twxs, tid);
 }
 
 // W slots after Y2
@@ -293,30 +319,39 @@ enum : int { S_M0U = 0, S_M0V = 1 };
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y2(Rhs2DArgs<T> a) {
     HW_YK_PROLOGUE
-    C A[G::PE], B[G::PE], Cc[G::PE], dsum[G::PE];
-    const int in_slot[2][3] = {{S_FGMU, S_FGPU, S_FGAU}, {S_FGMV, S_FGPV, S_FGAV}};
+    C A0[G::PE], B0[G::PE], C0[G::PE], A1[G::PE], B1[G::PE], C1[G::PE], dsum[G::PE];
+    T cm[G::PE], cp[G::PE], kk0[G::PE], kk1[G::PE];
     const T ig = (T)1 / a.g;
+    y_load<T, NX, NY>(A0, a.W[S_FGMU], nullptr, col0, tid);
+    y_load<T, NX, NY>(B0, a.W[S_FGPU], nullptr, col0, tid);
+    y_load<T, NX, NY>(C0, a.W[S_FGAU], nullptr, col0, tid);
+    y_load<T, NX, NY>(A1, a.W[S_FGMV], nullptr, col0, tid);
+    y_load<T, NX, NY>(B1, a.W[S_FGPV], nullptr, col0, tid);
+    y_load<T, NX, NY>(C1, a.W[S_FGAV], nullptr, col0, tid);
+#pragma unroll
+    for (int e = 0; e < G::PE; ++e) {
+        int kx, ky; size_t gi;
+        YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
+        const bool in = kx < NX / 2 + 1;
+        cm[e] = in ? a.csq_m[gi] : (T)0;
+        cp[e] = in ? a.csq_p[gi] : (T)0;
+        kk0[e] = in ? a.kxh[kx] : (T)0;
+        kk1[e] = in ? a.ky[ky] : (T)0;
+    }
     #pragma unroll
     for (int c = 0; c < 2; ++c) {
-        y_load<T, NX, NY>(A, a.W[in_slot[c][0]], nullptr, col0, tid);
+        auto& A = c == 0 ? A0 : A1;
+        auto& B = c == 0 ? B0 : B1;
+        auto& Cc = c == 0 ? C0 : C1;
         y_fwd_regs<T, NX, NY>(A, lds, twys, tid);
-        y_load<T, NX, NY>(B, a.W[in_slot[c][1]], nullptr, col0, tid);
         y_fwd_regs<T, NX, NY>(B, lds, twys, tid);
-        y_load<T, NX, NY>(Cc, a.W[in_slot[c][2]], nullptr, col0, tid);
         y_fwd_regs<T, NX, NY>(Cc, lds, twys, tid);
 #pragma unroll
         for (int e = 0; e < G::PE; ++e) {
-            int kx, ky; size_t gi;
-            YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
-            if (kx < NX / 2 + 1) {
-                C hs = cscale(cadd(Cc[e], cadd(cscale(A[e], a.csq_m[gi]), cscale(B[e], a.csq_p[gi]))), (T)0.5);
-                T kk = c == 0 ? a.kxh[kx] : a.ky[ky];
-                C d = cmul_i<-1>(cscale(hs, kk * ig));     // -i k HSS / g
-                A[e] = d;
-                dsum[e] = c == 0 ? d : cadd(dsum[e], d);
-            } else {
-                A[e] = cmk<C>((T)0, (T)0);
-            }
+            C hs = cscale(cadd(Cc[e], cadd(cscale(A[e], cm[e]), cscale(B[e], cp[e]))), (T)0.5);
+            C d = cmul_i<-1>(cscale(hs, (c == 0 ? kk0[e] : kk1[e]) * ig));     // -i k HSS / g
+            A[e] = d;
+            dsum[e] = c == 0 ? d : cadd(dsum[e], d);
         }
         y_inv_regs<T, NX, NY>(A, lds, twys, col0, tid);
         y_store<T, NX, NY>(a.W[c == 0 ? S_M0U : S_M0V], A, col0, tid);
@@ -331,11 +366,14 @@ template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x3(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
     C m0u[G::PP], m0v[G::PP], w[G::PP], uu[G::PP], vv[G::PP], f[G::PP];
-    x_inverse<T, NX, NY>(a.W[S_M0U], y0, lds, twxs, m0u, sc, nullptr, tid);
-    x_inverse<T, NX, NY>(a.W[S_M0V], y0, lds, twxs, m0v, sc, nullptr, tid);
+    XSpec<T, NX> b0, b1;
+    x_spec_load<T, NX, NY>(a.W[S_M0U], y0, b0, tid);
+    x_spec_load<T, NX, NY>(a.W[S_M0V], y0, b1, tid);
     ph_load<T, NX, NY>(w, a.wallchar, y0, tid);
     ph_load<T, NX, NY>(uu, a.u_p, y0, tid);
     ph_load<T, NX, NY>(vv, a.v_p, y0, tid);
+    x_inverse_b<T, NX, NY>(b0, y0, lds, twxs, m0u, sc, nullptr, tid);
+    x_inverse_b<T, NX, NY>(b1, y0, lds, twxs, m0v, sc, nullptr, tid);
     // E = 0.5 (u^2 + v^2 - Lphi^2) w, Lphi = M0u + M0v  (2d/fun_odesolver.c:3426)
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) {
@@ -435,52 +473,30 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y4(Rhs2DArgs<T> a) {
 // W slots after X5
 enum : int { S_ACC1 = 2, S_ACC2 = 3, S_ZU = 4, S_ZV = 5 };
 
-template <typename T, int NX, int NY>
-__device__ __forceinline__ void x5_hss_tail(const Rhs2DArgs<T>& a, int sm, int sp, int so, int y0,
-                                            typename ctype<T>::type* lds, const typename ctype<T>::type* twl, int tid) {
-    using C = typename ctype<T>::type;
-    using G = XGeo<T, NX>;
-    const T sc = (T)1 / ((T)NX * (T)NY);
-    C f[G::PP], acc[G::PP], gg[G::PP];
-    x_inverse<T, NX, NY>(a.W[sm], y0, lds, twl, f, sc, nullptr, tid);
-    ph_load<T, NX, NY>(gg, a.gam_m, y0, tid);
-#pragma unroll
-    for (int q = 0; q < G::PP; ++q) acc[q] = pmul(gg[q], f[q]);
-    x_inverse<T, NX, NY>(a.W[sp], y0, lds, twl, f, sc, nullptr, tid);
-    ph_load<T, NX, NY>(gg, a.gam_p, y0, tid);
-#pragma unroll
-    for (int q = 0; q < G::PP; ++q) acc[q] = pfma(gg[q], f[q], acc[q]);
-    x_fwd_store<T, NX, NY>(acc, a.W[so], y0, lds, twl, tid);
-}
 
 // output transform of a velocity equation: f1 = Sb - chi A - fbdy7 vel,
 // f2 = -chi (-1)^x s(y)   (the imaginary part of fun_adjust's inverse)
 template <typename T, int NX, int NY>
-__device__ __forceinline__ void x_vel_out(const Rhs2DArgs<T>& a, int sa, const T* vel, const T* sb, int so, int y0,
-                                          typename ctype<T>::type* lds, const typename ctype<T>::type* twl, T* nyq, int tid) {
+__device__ __forceinline__ void x_vel_out(const Rhs2DArgs<T>& a, const XSpec<T, NX>& b, const T* vel, const T* sb, int so,
+                                          int y0, typename ctype<T>::type* lds, const typename ctype<T>::type* twl, T* nyq,
+                                          int tid) {
     using C = typename ctype<T>::type;
     using G = XGeo<T, NX>;
     const T sc = (T)1 / ((T)NX * (T)NY);
-    C f1[G::PP], f2[G::PP], t[G::PP];
-    x_inverse<T, NX, NY>(a.W[sa], y0, lds, twl, f1, sc, nyq, tid);
-    ph_load<T, NX, NY>(t, a.chi, y0, tid);
+    C f1[G::PP], f2[G::PP], tc[G::PP], tf[G::PP], v[G::PP], tb[G::PP];
+    ph_load<T, NX, NY>(tc, a.chi, y0, tid);
+    ph_load<T, NX, NY>(tf, a.fbdy7, y0, tid);
+    ph_load<T, NX, NY>(v, vel, y0, tid);
+    if (sb) ph_load<T, NX, NY>(tb, sb, y0, tid);
+    x_inverse_b<T, NX, NY>(b, y0, lds, twl, f1, sc, nyq, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) {
         int pos = tid + q * G::NT;
         int r = pos / G::L;
         T s = nyq[r < G::RR ? r : 0];
-        f1[q] = cmk<C>(-t[q].x * f1[q].x, -t[q].y * f1[q].y);
-        f2[q] = cmk<C>(-t[q].x * s, t[q].y * s);
-    }
-    ph_load<T, NX, NY>(t, a.fbdy7, y0, tid);
-    C v[G::PP];
-    ph_load<T, NX, NY>(v, vel, y0, tid);
-#pragma unroll
-    for (int q = 0; q < G::PP; ++q) f1[q] = csub(f1[q], pmul(t[q], v[q]));
-    if (sb) {
-        ph_load<T, NX, NY>(t, sb, y0, tid);
-#pragma unroll
-        for (int q = 0; q < G::PP; ++q) f1[q] = cadd(f1[q], t[q]);
+        f1[q] = csub(cmk<C>(-tc[q].x * f1[q].x, -tc[q].y * f1[q].y), pmul(tf[q], v[q]));
+        if (sb) f1[q] = cadd(f1[q], tb[q]);
+        f2[q] = cmk<C>(-tc[q].x * s, tc[q].y * s);
     }
     x_fwd2_store<T, NX, NY>(f1, f2, a.W[so], y0, lds, twl, tid);
 }
@@ -488,10 +504,34 @@ __device__ __forceinline__ void x_vel_out(const Rhs2DArgs<T>& a, int sa, const T
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x5(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
-    x5_hss_tail<T, NX, NY>(a, S_B1M, S_B1P, S_ACC1, y0, lds, twxs, tid);
-    x5_hss_tail<T, NX, NY>(a, S_B2M, S_B2P, S_ACC2, y0, lds, twxs, tid);
-    x_vel_out<T, NX, NY>(a, S_AX, a.u_p, a.sbx, S_ZU, y0, lds, twxs, nyq, tid);
-    x_vel_out<T, NX, NY>(a, S_AY, a.v_p, a.sby, S_ZV, y0, lds, twxs, nyq, tid);
+    // F(g- b- + g+ b+) of both HSS tails, then the two velocity output
+    // transforms; the next field's loads are issued before each transform
+    C f[G::PP], acc[G::PP], gm[G::PP], gp[G::PP];
+    XSpec<T, NX> b0, b1;
+    x_spec_load<T, NX, NY>(a.W[S_B1M], y0, b0, tid);
+    x_spec_load<T, NX, NY>(a.W[S_B1P], y0, b1, tid);
+    ph_load<T, NX, NY>(gm, a.gam_m, y0, tid);
+    ph_load<T, NX, NY>(gp, a.gam_p, y0, tid);
+    x_inverse_b<T, NX, NY>(b0, y0, lds, twxs, f, sc, nullptr, tid);
+    x_spec_load<T, NX, NY>(a.W[S_B2M], y0, b0, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) acc[q] = pmul(gm[q], f[q]);
+    x_inverse_b<T, NX, NY>(b1, y0, lds, twxs, f, sc, nullptr, tid);
+    x_spec_load<T, NX, NY>(a.W[S_B2P], y0, b1, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) acc[q] = pfma(gp[q], f[q], acc[q]);
+    x_fwd_store<T, NX, NY>(acc, a.W[S_ACC1], y0, lds, twxs, tid);
+    x_inverse_b<T, NX, NY>(b0, y0, lds, twxs, f, sc, nullptr, tid);
+    x_spec_load<T, NX, NY>(a.W[S_AX], y0, b0, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) acc[q] = pmul(gm[q], f[q]);
+    x_inverse_b<T, NX, NY>(b1, y0, lds, twxs, f, sc, nullptr, tid);
+    x_spec_load<T, NX, NY>(a.W[S_AY], y0, b1, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) acc[q] = pfma(gp[q], f[q], acc[q]);
+    x_fwd_store<T, NX, NY>(acc, a.W[S_ACC2], y0, lds, twxs, tid);
+    x_vel_out<T, NX, NY>(a, b0, a.u_p, a.sbx, S_ZU, y0, lds, twxs, nyq, tid);
+    x_vel_out<T, NX, NY>(a, b1, a.v_p, a.sby, S_ZV, y0, lds, twxs, nyq, tid);
 }
 
 // W slots after Y6
@@ -500,35 +540,40 @@ enum : int { S_AETA = 6 };
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y6(Rhs2DArgs<T> a) {
     HW_YK_PROLOGUE
-    C F1[G::PE], F2[G::PE], Z[G::PE], eh[G::PE];
+    C F1[G::PE], F2[G::PE], Z[G::PE], eh[G::PE], Zu[G::PE], Zv[G::PE];
+    T al[G::PE], kk0[G::PE], kk1[G::PE];
     const T ig = (T)1 / a.g;
     y_load<T, NX, NY>(F1, a.W[S_ACC1], nullptr, col0, tid);
-    y_fwd_regs<T, NX, NY>(F1, lds, twys, tid);
     y_load<T, NX, NY>(F2, a.W[S_ACC2], nullptr, col0, tid);
-    y_fwd_regs<T, NX, NY>(F2, lds, twys, tid);
     y_load<T, NX, NY>(Z, a.dH2p, nullptr, col0, tid);
+    y_load<T, NX, NY>(eh, a.eta_hat, nullptr, col0, tid);
+    y_load<T, NX, NY>(Zu, a.W[S_ZU], nullptr, col0, tid);
+    y_load<T, NX, NY>(Zv, a.W[S_ZV], nullptr, col0, tid);
+#pragma unroll
+    for (int e = 0; e < G::PE; ++e) {
+        int kx, ky; size_t gi;
+        YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
+        const bool in = kx < NX / 2 + 1;
+        al[e] = in ? a.aal[gi] : (T)0;
+        kk0[e] = in ? a.kxh[kx] : (T)0;
+        kk1[e] = in ? a.ky[ky] : (T)0;
+    }
+    y_fwd_regs<T, NX, NY>(F1, lds, twys, tid);
+    y_fwd_regs<T, NX, NY>(F2, lds, twys, tid);
 #pragma unroll
     for (int e = 0; e < G::PE; ++e) Z[e] = csub(Z[e], cscale(cadd(F1[e], F2[e]), (T)0.5 * ig));
     y_inv_regs<T, NX, NY>(Z, lds, twys, col0, tid);
     y_store<T, NX, NY>(a.W[S_AETA], Z, col0, tid);
-    y_load<T, NX, NY>(eh, a.eta_hat, nullptr, col0, tid);
     #pragma unroll
     for (int c = 0; c < 2; ++c) {
-        y_load<T, NX, NY>(Z, a.W[c == 0 ? S_ZU : S_ZV], nullptr, col0, tid);
-        y_fwd_regs<T, NX, NY>(Z, lds, twys, tid);
+        auto& Zc = c == 0 ? Zu : Zv;
+        y_fwd_regs<T, NX, NY>(Zc, lds, twys, tid);
 #pragma unroll
         for (int e = 0; e < G::PE; ++e) {
-            int kx, ky; size_t gi;
-            YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
-            if (kx < NX / 2 + 1) {
-                T kk = c == 0 ? a.kxh[kx] : a.ky[ky];
-                C lin = cmul_i<-1>(cscale(eh[e], a.g * kk));
-                C pre = cadd(lin, Z[e]);
-                if (c == 0 && a.eta_col0 && kx == 0) a.eta_col0[ky] = pre;
-                Z[e] = cscale(pre, a.aal[gi]);
-            }
+            C lin = cmul_i<-1>(cscale(eh[e], a.g * (c == 0 ? kk0[e] : kk1[e])));
+            Zc[e] = cscale(cadd(lin, Zc[e]), al[e]);
         }
-        y_store<T, NX, NY>(c == 0 ? a.k_u : a.k_v, Z, col0, tid);
+        y_store<T, NX, NY>(c == 0 ? a.k_u : a.k_v, Zc, col0, tid);
     }
 }
 
@@ -558,35 +603,35 @@ __device__ __forceinline__ void src_pair(const Rhs2DArgs<T>& a, int y0, int pos,
 // output transform of the eta equation: f1 = S(t) - fbdy7 eta + chi A,
 // f2 = chi (-1)^x s(y)
 template <typename T, int NX, int NY>
-__device__ __forceinline__ void x_eta_out(const Rhs2DArgs<T>& a, int sa, int so, int y0,
+__device__ __forceinline__ void x_eta_out(const Rhs2DArgs<T>& a, const XSpec<T, NX>& b, int so, int y0,
                                           typename ctype<T>::type* lds, const typename ctype<T>::type* twl, T* nyq, int tid) {
     using C = typename ctype<T>::type;
     using G = XGeo<T, NX>;
     const T sc = (T)1 / ((T)NX * (T)NY);
-    C f1[G::PP], f2[G::PP], t[G::PP], t2[G::PP];
-    x_inverse<T, NX, NY>(a.W[sa], y0, lds, twl, f1, sc, nyq, tid);
+    C f1[G::PP], f2[G::PP], tc[G::PP], tf[G::PP], te[G::PP], tn[G::PP], ts[G::PP];
+    ph_load<T, NX, NY>(tc, a.chi, y0, tid);
+    x_inverse_b<T, NX, NY>(b, y0, lds, twl, f1, sc, nyq, tid);
+    ph_load<T, NX, NY>(tf, a.fbdy7, y0, tid);
+    ph_load<T, NX, NY>(te, a.eta_p, y0, tid);
+    if (a.src_on) {
+        ph_load<T, NX, NY>(tn, a.new_gam, y0, tid);
+        ph_load<T, NX, NY>(ts, a.skew_gam, y0, tid);
+    }
     if (a.s_eta_out && tid < G::RR) a.s_eta_out[y0 + tid] = nyq[tid];
-    ph_load<T, NX, NY>(t, a.chi, y0, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) {
         int pos = tid + q * G::NT;
         int r = pos / G::L;
         T s = nyq[r < G::RR ? r : 0];
-        f1[q] = pmul(t[q], f1[q]);
-        f2[q] = cmk<C>(t[q].x * s, -t[q].y * s);
+        f1[q] = csub(pmul(tc[q], f1[q]), pmul(tf[q], te[q]));
+        f2[q] = cmk<C>(tc[q].x * s, -tc[q].y * s);
     }
-    ph_load<T, NX, NY>(t, a.fbdy7, y0, tid);
-    ph_load<T, NX, NY>(t2, a.eta_p, y0, tid);
-#pragma unroll
-    for (int q = 0; q < G::PP; ++q) f1[q] = csub(f1[q], pmul(t[q], t2[q]));
     if (a.src_on) {
-        ph_load<T, NX, NY>(t, a.new_gam, y0, tid);
-        ph_load<T, NX, NY>(t2, a.skew_gam, y0, tid);
 #pragma unroll
         for (int q = 0; q < G::PP; ++q) {
             int pos = tid + q * G::NT;
             T s0, s1;
-            src_pair<T, NX>(a, y0, pos, t[q], t2[q], s0, s1);
+            src_pair<T, NX>(a, y0, pos, tn[q], ts[q], s0, s1);
             f1[q] = cadd(f1[q], cmk<C>(s0, s1));
         }
     }
@@ -596,7 +641,9 @@ __device__ __forceinline__ void x_eta_out(const Rhs2DArgs<T>& a, int sa, int so,
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x7(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
-    x_eta_out<T, NX, NY>(a, S_AETA, S_ZU, y0, lds, twxs, nyq, tid);
+    XSpec<T, NX> b;
+    x_spec_load<T, NX, NY>(a.W[S_AETA], y0, b, tid);
+    x_eta_out<T, NX, NY>(a, b, S_ZU, y0, lds, twxs, nyq, tid);
 }
 
 template <typename T, int NX, int NY>
@@ -735,9 +782,13 @@ enum : int { F_ZETA = 3, F_ZU = 4, F_ZV = 5 };
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2f_x4(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
-    x_eta_out<T, NX, NY>(a, F_AETA, F_ZETA, y0, lds, twxs, nyq, tid);
-    x_vel_out<T, NX, NY>(a, F_AX, a.u_p, a.sbx, F_ZU, y0, lds, twxs, nyq, tid);
-    x_vel_out<T, NX, NY>(a, F_AY, a.v_p, a.sby, F_ZV, y0, lds, twxs, nyq, tid);
+    XSpec<T, NX> b0, b1, b2;
+    x_spec_load<T, NX, NY>(a.W[F_AETA], y0, b0, tid);
+    x_spec_load<T, NX, NY>(a.W[F_AX], y0, b1, tid);
+    x_spec_load<T, NX, NY>(a.W[F_AY], y0, b2, tid);
+    x_eta_out<T, NX, NY>(a, b0, F_ZETA, y0, lds, twxs, nyq, tid);
+    x_vel_out<T, NX, NY>(a, b1, a.u_p, a.sbx, F_ZU, y0, lds, twxs, nyq, tid);
+    x_vel_out<T, NX, NY>(a, b2, a.v_p, a.sby, F_ZV, y0, lds, twxs, nyq, tid);
 }
 
 template <typename T, int NX, int NY>

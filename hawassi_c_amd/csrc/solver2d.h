@@ -651,7 +651,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
                 (rc = rhs(t0 + RK_AH[4] * h0, ytmp.p, K(5))))
                 return rc;
             k_zero_u64<<<1, 64, 0, stream>>>(rmax.p, 1);
-            k_rk_final<T><<<grid1(3 * H), 256, 0, stream>>>(y.p, K(0), K(2), K(3), K(4), K(5), h0, eps_abs, eps_rel, ynew.p,
+            k_rk_final<T><<<dim3(2048), 256, 0, stream>>>(y.p, K(0), K(2), K(3), K(4), K(5), h0, eps_abs, eps_rel, ynew.p,
                                                             rmax.p, 3 * H);
             HW_LAUNCH_CHECK();
             if (brk_on) {
