@@ -231,7 +231,8 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k_t_yinv(const typename cty
 
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k_t_xinv(const typename ctype<T>::type* X, T* phys, T* nyq_out,
-                                                            const typename ctype<T>::type* twx) {
+                                                            const typename ctype<T>::type* twx, const int* gate = nullptr) {
+    if (gate && *gate == 0) return;
     HW_XK_PROLOGUE_TW(twx)
     C f[G::PP];
     x_inverse<T, NX, NY>(X, y0, lds, twxs, f, sc, nyq, tid);

@@ -22,12 +22,12 @@ struct BrkPeak {
     int x, y;
 };
 
-// per-peak criterion output (breaking_criterion, 2d/breaking.c:288-305)
-struct BrkCrit {
-    int pass;          // |U| >= KBC |C(K, D + eta)|
-    int prevnode;      // peak lies on a break node of the previous call
-    double U, theta;   // |grad phi|, |atan(v/u)|
-    int kwadran;
+
+// a peak that passed the crest test, appended by k_brk_crit in any order
+// (the host restores the reference's peak order)
+struct BrkCand {
+    int x, y, prevnode, kwadran;
+    double U, theta;
 };
 
 // per-crest input/output of the tracking kernels
@@ -101,7 +101,9 @@ __global__ void k_brk_peaks(const T* __restrict__ eta, const T* __restrict__ pbc
 
 // Hilbert transform input: -i sign(kx + ky) eta^, inverse y with projection
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__((YGeo<T, NY>::NT)) k_brk_hilb_y(Rhs2DArgs<T> a, typename ctype<T>::type* X) {
+__global__ void __launch_bounds__((YGeo<T, NY>::NT)) k_brk_hilb_y(Rhs2DArgs<T> a, typename ctype<T>::type* X,
+                                                                  const int* __restrict__ gate) {
+    if (*gate == 0) return;   // no peak: the Hilbert transform is not needed
     HW_YK_PROLOGUE
     C v[G::PE];
     y_load<T, NX, NY>(v, a.eta_hat, nullptr, col0, tid);
@@ -119,13 +121,14 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k_brk_hilb_y(Rhs2DArgs<T> a
 }
 
 // breaking_criterion (2d/breaking.c:253-305) at the peaks
+// cnt[0] = number of peaks (k_brk_peaks), cnt[1] = candidates appended here
 template <typename T>
-__global__ void k_brk_crit(const BrkPeak* __restrict__ pk, int npk, const T* __restrict__ eta, const T* __restrict__ H,
-                           const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ depth,
-                           const T* __restrict__ Bprev, double dx, double dy, double U0, double KBC, int nx, int ny,
-                           BrkCrit* __restrict__ out) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= npk) return;
+__global__ void k_brk_crit(const BrkPeak* __restrict__ pk, int* __restrict__ cnt, int maxpk, const T* __restrict__ eta,
+                           const T* __restrict__ H, const T* __restrict__ u, const T* __restrict__ v,
+                           const T* __restrict__ depth, const T* __restrict__ Bprev, double dx, double dy, double U0,
+                           double KBC, int nx, int ny, BrkCand* __restrict__ out) {
+    const int npk = min(cnt[0], maxpk);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < npk; k += gridDim.x * blockDim.x) {
     int ix = pk[k].x, iy = pk[k].y;
     auto at = [&](const T* A, int i, int j) -> double { return (double)A[(size_t)i * nx + j]; };
     // gradient2D (2d/functions1.c:868-885)
@@ -148,13 +151,18 @@ __global__ void k_brk_crit(const BrkPeak* __restrict__ pk, int npk, const T* __r
     double Cc = Kk == 0 ? sqrt(g * D) + U0 : (sqrt(g * Kk * tanh(Kk * D)) + Kk * U0) / Kk;
     double uu = at(u, iy, ix), vv = at(v, iy, ix);
     double U = sqrt(uu * uu + vv * vv);
-    BrkCrit r;
-    r.pass = fabs(U) >= KBC * fabs(Cc) ? 1 : 0;
-    r.prevnode = at(Bprev, iy, ix) > 0 ? 1 : 0;
-    r.U = fabs(U);
-    r.theta = fabs(atan(vv / uu));
-    r.kwadran = kwadran_of(uu, vv);
-    out[k] = r;
+    if (fabs(U) >= KBC * fabs(Cc)) {
+        BrkCand r;
+        r.x = ix;
+        r.y = iy;
+        r.prevnode = at(Bprev, iy, ix) > 0 ? 1 : 0;
+        r.U = fabs(U);
+        r.theta = fabs(atan(vv / uu));
+        r.kwadran = kwadran_of(uu, vv);
+        int slot = atomicAdd(cnt + 1, 1);
+        if (slot < maxpk) out[slot] = r;
+    }
+    }
 }
 
 // one block per crest: re-locate (mode 2) by the first row-major maximum of

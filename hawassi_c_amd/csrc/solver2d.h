@@ -150,9 +150,17 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
     double dxg = 1, dyg = 1;
     DevBuf<T> pbc, depthv, Bf, flux, sbx, sby, s_eta;
     DevBuf<double> xg, yg;
-    DevBuf<int> owner, pkcount;
+    DevBuf<int> owner, pkcount;            // pkcount[0]: peaks, pkcount[1]: crest candidates
     DevBuf<BrkPeak> peaks;
-    DevBuf<BrkCrit> crit;
+    DevBuf<BrkCand> cand;
+    DevBuf<C> hilx;                        // Hilbert transform of eta: x-intermediate
+    DevBuf<T> hilp;                        // ... and physical
+    static constexpr int CAPH = 256;       // candidates copied to the host with the counts
+    int* h_cnt = nullptr;
+    BrkCand* h_cand = nullptr;
+    hipEvent_t ev_brk = nullptr;
+    bool b_zero = true, sb_zero = true;    // Bf / Sb^ known to be all zero
+    bool last_gated = false;               // last RHS call skipped breaking (tprev >= t)
     DevBuf<BrkCrest> crests;
     DevBuf<C> sbh;                         // Sb_x^, Sb_y^ half spectra (2H)
     DevBuf<C> col0;                        // pre-mask eta derivative, column kx = 0 (NY)
@@ -169,6 +177,9 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
     int nev = 0;
 
     ~Solver2DT() override {
+        if (h_cnt) (void)hipHostFree(h_cnt);
+        if (h_cand) (void)hipHostFree(h_cand);
+        if (ev_brk) (void)hipEventDestroy(ev_brk);
         if (h_rmax) (void)hipHostFree(h_rmax);
         for (int i = 0; i < NEV; ++i) {
             if (ev_a[i]) (void)hipEventDestroy(ev_a[i]);
@@ -286,12 +297,16 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
             maxpk = (int)std::max<size_t>(4096, N / 8);
             maxcr = 65536;
             if ((rc = Bf.alloc(N)) || (rc = flux.alloc(N)) || (rc = sbx.alloc(N)) || (rc = sby.alloc(N)) ||
-                (rc = owner.alloc(N)) || (rc = pkcount.alloc(1)) || (rc = peaks.alloc(maxpk)) || (rc = crit.alloc(maxpk)) ||
+                (rc = owner.alloc(N)) || (rc = pkcount.alloc(2)) || (rc = peaks.alloc(maxpk)) || (rc = cand.alloc(maxpk)) ||
+                (rc = hilx.alloc(H)) || (rc = hilp.alloc(N)) ||
                 (rc = crests.alloc(maxcr)) || (rc = sbh.alloc(2 * H)) || (rc = col0.alloc(NY)))
                 return rc;
             k_fill<T><<<grid1(N), 256, 0, stream>>>(Bf.p, (T)0, N);
             k_fill<C><<<grid1(2 * H), 256, 0, stream>>>(sbh.p, cmk_h((T)0, (T)0), 2 * H);
             HW_LAUNCH_CHECK();
+            HW_TRY(hipHostMalloc((void**)&h_cnt, 2 * sizeof(int), hipHostMallocDefault));
+            HW_TRY(hipHostMalloc((void**)&h_cand, CAPH * sizeof(BrkCand), hipHostMallocDefault));
+            HW_TRY(hipEventCreateWithFlags(&ev_brk, hipEventDisableTiming));
         }
         if ((rc = eta_p.alloc(N)) || (rc = u_p.alloc(N)) || (rc = v_p.alloc(N))) return rc;
         if ((rc = work.alloc(18 * H))) return rc;
@@ -376,6 +391,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         if (!flat) {
             k2b_y1<T, NX, NY><<<gy, by, 0, stream>>>(args);
             k2b_x1<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+            if (brk_now) { int rc = breaking_detect(); if (rc) return rc; }
             k2b_y2<T, NX, NY><<<gy, by, 0, stream>>>(args);
             k2b_x3<T, NX, NY><<<gx, bx, 0, stream>>>(args);
             k2b_y4<T, NX, NY><<<gy, by, 0, stream>>>(args);
@@ -386,6 +402,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         } else {
             k2f_y1<T, NX, NY><<<gy, by, 0, stream>>>(args);
             k2f_x2<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+            if (brk_now) { int rc = breaking_detect(); if (rc) return rc; }
             k2f_y3<T, NX, NY><<<gy, by, 0, stream>>>(args);
             k2f_x4<T, NX, NY><<<gx, bx, 0, stream>>>(args);
             k2f_y5<T, NX, NY><<<gy, by, 0, stream>>>(args);
@@ -401,8 +418,11 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
                 k_eta_project<T><<<(NY + 255) / 256, 256, 0, stream>>>(args.k_eta, col0.p, aal.p, NX, NY);
                 HW_LAUNCH_CHECK();
             }
-            k_add_sb<T><<<grid1(H), 256, 0, stream>>>(args.k_u, args.k_v, sbh.p, sbh.p + H, aal.p, H);
-            HW_LAUNCH_CHECK();
+            last_gated = !brk_now;
+            if (!sb_zero) {
+                k_add_sb<T><<<grid1(H), 256, 0, stream>>>(args.k_u, args.k_v, sbh.p, sbh.p + H, aal.p, H);
+                HW_LAUNCH_CHECK();
+            }
         }
         HW_TRY(hipEventRecord(ev_b[nev], stream));
         nev++;
@@ -413,35 +433,43 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
     // ------------------------------------------------------- breaking --
     // breaking_process (2d/breaking.c:505-733) + fun_sbreaking
     // (2d/fun_odesolver.c:985-1019) for one RHS call at time t.
+    // Peaks, Hilbert transform and crest test (breaking_criterion) need only
+    // eta, u, v of this call, so they are launched right after the first
+    // x-pass; the counts and the first crest candidates come back with an
+    // asynchronous copy while the rest of the pipeline runs.
+    int breaking_detect() {
+        const dim3 gx(NY / XG::RR), bx(XG::NT);
+        const dim3 gy((NXH + YG::CC - 1) / YG::CC), by(YG::NT);
+        k_fill<int><<<1, 64, 0, stream>>>(pkcount.p, 0, 2);
+        k_brk_peaks<T><<<dim3((NX + 255) / 256, NY), 256, 0, stream>>>(eta_p.p, pbc.p, MaxEtaInit, NX, NY, peaks.p, pkcount.p, maxpk);
+        k_brk_hilb_y<T, NX, NY><<<gy, by, 0, stream>>>(args, hilx.p, pkcount.p);
+        k_t_xinv<T, NX, NY><<<gx, bx, 0, stream>>>(hilx.p, hilp.p, nullptr, twx.p, pkcount.p);
+        k_brk_crit<T><<<64, 256, 0, stream>>>(peaks.p, pkcount.p, maxpk, eta_p.p, hilp.p, u_p.p, v_p.p, depthv.p, Bf.p,
+                                             dxg, dyg, Ucur, KBC, NX, NY, cand.p);
+        HW_LAUNCH_CHECK();
+        HW_TRY(hipMemcpyAsync(h_cnt, pkcount.p, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
+        HW_TRY(hipMemcpyAsync(h_cand, cand.p, CAPH * sizeof(BrkCand), hipMemcpyDeviceToHost, stream));
+        HW_TRY(hipEventRecord(ev_brk, stream));
+        return HW_OK;
+    }
+
     int breaking(double t) {
         brk_calls++;
         const dim3 gx(NY / XG::RR), bx(XG::NT);
         const dim3 gy((NXH + YG::CC - 1) / YG::CC), by(YG::NT);
-        int npk = 0;
-        k_fill<int><<<1, 64, 0, stream>>>(pkcount.p, 0, 1);
-        k_brk_peaks<T><<<dim3((NX + 255) / 256, NY), 256, 0, stream>>>(eta_p.p, pbc.p, MaxEtaInit, NX, NY, peaks.p, pkcount.p, maxpk);
-        HW_LAUNCH_CHECK();
-        HW_TRY(hipMemcpyAsync(&npk, pkcount.p, sizeof(int), hipMemcpyDeviceToHost, stream));
-        HW_TRY(hipStreamSynchronize(stream));
+        HW_TRY(hipEventSynchronize(ev_brk));
+        const int npk = h_cnt[0], ncand = h_cnt[1];
         if (npk > maxpk) { set_error("breaking: %d peaks exceed the buffer of %d", npk, maxpk); return HW_ERR_ARG; }
         // ---- criterion: candidate crests in the reference's peak order
         std::vector<CrestRec> cb;
         std::vector<int> cb_prevnode;
-        if (npk > 0) {
-            C* Xh = args.W[10];
-            T* Hp = reinterpret_cast<T*>(args.W[11]);
-            k_brk_hilb_y<T, NX, NY><<<gy, by, 0, stream>>>(args, Xh);
-            k_t_xinv<T, NX, NY><<<gx, bx, 0, stream>>>(Xh, Hp, nullptr, twx.p);
-            k_brk_crit<T><<<(npk + 127) / 128, 128, 0, stream>>>(peaks.p, npk, eta_p.p, Hp, u_p.p, v_p.p, depthv.p, Bf.p, dxg,
-                                                                  dyg, Ucur, KBC, NX, NY, crit.p);
-            HW_LAUNCH_CHECK();
-            std::vector<BrkPeak> hp(npk);
-            std::vector<BrkCrit> hc(npk);
-            HW_TRY(hipMemcpyAsync(hp.data(), peaks.p, sizeof(BrkPeak) * npk, hipMemcpyDeviceToHost, stream));
-            HW_TRY(hipMemcpyAsync(hc.data(), crit.p, sizeof(BrkCrit) * npk, hipMemcpyDeviceToHost, stream));
-            HW_TRY(hipStreamSynchronize(stream));
+        if (ncand > 0) {
+            std::vector<BrkCand> hc(ncand);
+            std::copy(h_cand, h_cand + std::min(ncand, CAPH), hc.begin());
+            if (ncand > CAPH)
+                HW_TRY(hipMemcpy(hc.data() + CAPH, cand.p + CAPH, sizeof(BrkCand) * (ncand - CAPH), hipMemcpyDeviceToHost));
             // fun_peaks order: interior x-major, then x=0, x=nx-1, y=0, y=ny-1 edges
-            auto key = [](const BrkPeak& p) -> long long {
+            auto key = [](const BrkCand& p) -> long long {
                 int cls;
                 long long a, b;
                 if (p.x >= 1 && p.x <= NX - 2 && p.y >= 1 && p.y <= NY - 2) { cls = 0; a = p.x; b = p.y; }
@@ -451,14 +479,13 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
                 else { cls = 4; a = p.x; b = 0; }
                 return ((long long)cls << 40) | (a << 20) | b;
             };
-            std::vector<int> ord(npk);
-            for (int i = 0; i < npk; ++i) ord[i] = i;
-            std::sort(ord.begin(), ord.end(), [&](int a, int b) { return key(hp[a]) < key(hp[b]); });
+            std::vector<int> ord(ncand);
+            for (int i = 0; i < ncand; ++i) ord[i] = i;
+            std::sort(ord.begin(), ord.end(), [&](int a, int b) { return key(hc[a]) < key(hc[b]); });
             for (int i : ord) {
-                if (!hc[i].pass) continue;
                 CrestRec r;
-                r.xindex = r.xNow = hp[i].x;
-                r.yindex = r.yNow = hp[i].y;
+                r.xindex = r.xNow = hc[i].x;
+                r.yindex = r.yNow = hc[i].y;
                 r.Ucrest = hc[i].U;
                 r.time = t;
                 r.theta = hc[i].theta;
@@ -540,11 +567,13 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         }
         if (!have_b) {
             // B == 0 everywhere: zero flux, zero source
-            k_fill<T><<<grid1(N), 256, 0, stream>>>(Bf.p, (T)0, N);
-            k_fill<C><<<grid1(2 * H), 256, 0, stream>>>(sbh.p, cmk_h((T)0, (T)0), 2 * H);
+            if (!b_zero) k_fill<T><<<grid1(N), 256, 0, stream>>>(Bf.p, (T)0, N);
+            if (!sb_zero) k_fill<C><<<grid1(2 * H), 256, 0, stream>>>(sbh.p, cmk_h((T)0, (T)0), 2 * H);
             HW_LAUNCH_CHECK();
+            b_zero = sb_zero = true;
             return HW_OK;
         }
+        b_zero = sb_zero = false;
         // ---- fun_sbreaking: delphi, flux, clipped gradient, transforms
         C* Xg = args.W[12];
         C* Xd = args.W[13];
@@ -562,7 +591,9 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
     int breaking_state(double* Bout, int* fl) override {
         if (fl) *fl = flagbr;
         if (Bout) {
-            if (!brk_on) {
+            // the reference clears B on every call and fills it only when
+            // breaking runs (2d/fun_odesolver.c:4581-4585)
+            if (!brk_on || last_gated) {
                 for (size_t i = 0; i < N; ++i) Bout[i] = 0.0;
             } else {
                 std::vector<T> hb(N);
