@@ -303,7 +303,7 @@ __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k_brk_g_x(Rhs2DArgs<T> a, c
     ph_load<T, NX, NY>(t, a.chi, y0, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) {
-        int pos = tid + q * G::NT;
+        int pos = G::ppos(tid, q);
         int r = pos / G::L;
         T s = s_eta[y0 + (r < G::RR ? r : 0)];
         f[q] = cmk<C>(t[q].x * s, -t[q].y * s);

@@ -193,9 +193,149 @@ __device__ __forceinline__ void fft_lds(C* lds, int LS, const C* __restrict__ tw
     fft_lds_pass<C, L, B, NT, DIR, 1>(lds, LS, tw, TS, tid);
 }
 
+// ------------------------------------------- register-resident Stockham ---
+// The 2-D passes keep the E elements a thread owns in registers for the whole
+// transform: each radix-R pass applies twiddles and the radix-R DFT in
+// registers, and only the hand-over between passes goes through LDS (one
+// write, one barrier, one read).  Two exchange buffers alternate, so a pass
+// needs one barrier; when the block's LDS budget cannot hold two, one buffer
+// is used with a barrier before every write.
+//
+// Schedule: radix E (4 or 8) everywhere, with the remainder factor as the
+// SECOND pass so that the first and last radix are equal.  Then the register
+// layout of the input ("slot p*R0 + r of thread t holds element j + r*(L/R0)
+// of sequence b, (b, j) = divmod(t + p*NT, L/R0)") is also the layout of the
+// output, so an inverse transform leaves data exactly where the next forward
+// transform expects it.
+#ifndef HW_LDS_PAD
+#define HW_LDS_PAD 5
+#endif
+// exchange-buffer index: one pad element every 2^HW_LDS_PAD (bank spread).
+// pidx(a + b) == pidx(a) + pidx(b) whenever a is a multiple of 2^HW_LDS_PAD.
+__host__ __device__ constexpr int pidx(int i) { return i + (i >> HW_LDS_PAD); }
+constexpr int PADQ = 1 << HW_LDS_PAD;
+
+constexpr int rs_log(int L, int E) { int a = 0; while (L > 1 && L % E == 0) { L /= E; ++a; } return a; }
+constexpr int rs_rem(int L, int E) { int a = rs_log(L, E); for (int i = 0; i < a; ++i) L /= E; return L; }
+constexpr int rs_np(int L, int E) {
+    const int a = rs_log(L, E), s = rs_rem(L, E);
+    if (s == 1) return a;
+    if (a >= 2) return a + 1;
+    if (a == 1) return (E == 8 && L == 16) ? 2 : 3;   // 16 = 4*4; 32 = 2*8*2; 8 = 2*2*2 (E = 4)
+    return 1;
+}
+constexpr int rs_radix(int L, int E, int i) {
+    const int a = rs_log(L, E), s = rs_rem(L, E);
+    if (s == 1) return E;
+    if (a >= 2) return i == 1 ? s : E;
+    if (a == 1) {
+        if (E == 8) return L == 16 ? 4 : (i == 1 ? 8 : 2);
+        return 2;
+    }
+    return L;
+}
+constexpr int rs_ns(int L, int E, int i) { int ns = 1; for (int k = 0; k < i; ++k) ns *= rs_radix(L, E, k); return ns; }
+
+// v[r] *= w^r, w = tw[ex*TS] (conjugated for DIR = +1).  Radix 8/16 build w^r
+// from w, w^4 (and w^8) read from the table: at most two roundings more than
+// a table entry.
+template <typename C, int R, int DIR>
+__device__ __forceinline__ void tw_apply(C* v, const C* __restrict__ tw, int ex, int TS) {
+    if constexpr (R <= 4) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+            C w = tw[(r * ex) * TS];
+            if (DIR > 0) w = cconj(w);
+            v[r] = cmul(v[r], w);
+        }
+    } else {
+        C wp[R];
+        C w1 = tw[ex * TS], w4 = tw[4 * ex * TS];
+        if (DIR > 0) { w1 = cconj(w1); w4 = cconj(w4); }
+        wp[1] = w1;
+        wp[2] = cmul(w1, w1);
+        wp[3] = cmul(wp[2], w1);
+        wp[4] = w4;
+        wp[5] = cmul(w4, w1);
+        wp[6] = cmul(w4, wp[2]);
+        wp[7] = cmul(w4, wp[3]);
+        if constexpr (R == 16) {
+            C w8 = tw[8 * ex * TS];
+            if (DIR > 0) w8 = cconj(w8);
+            wp[8] = w8;
+#pragma unroll
+            for (int r = 1; r < 8; ++r) wp[8 + r] = cmul(w8, wp[r]);
+        }
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[r] = cmul(v[r], wp[r]);
+    }
+}
+
+// Pass I of a length-L transform of B sequences by NT threads holding E
+// elements each.  Exchange buffers: B rows of stride LSb at bufA / bufB
+// (bufB == bufA when TWO is false).
+template <typename C, int L, int B, int NT, int E, int DIR, bool TWO, int I>
+__device__ __forceinline__ void fr_pass(C (&v)[E], C* bufA, C* bufB, int LSb, const C* __restrict__ tw, int TS, int tid) {
+    constexpr int NP = rs_np(L, E);
+    static_assert(B * L == NT * E, "register layout must cover the block exactly");
+    if constexpr (I < NP) {
+        constexpr int R = rs_radix(L, E, I), NS = rs_ns(L, E, I), M = L / R;
+#pragma unroll
+        for (int p = 0; p < E / R; ++p) {
+            const int j = (tid + p * NT) % M;
+            if constexpr (NS > 1) tw_apply<C, R, DIR>(&v[p * R], tw, (j % NS) * (L / (NS * R)), TS);
+            dftR<R, DIR>(&v[p * R]);
+        }
+        if constexpr (I + 1 < NP) {
+            C* buf = (TWO && (I % 2)) ? bufB : bufA;
+            if constexpr (!TWO) __syncthreads();
+#pragma unroll
+            for (int p = 0; p < E / R; ++p) {
+                const int bf = tid + p * NT, b = bf / M, j = bf % M, k = j % NS;
+                // element r goes to idxD + r*NS; that sum never carries across
+                // a pad boundary that idxD itself does not (powers of two)
+                C* dst = buf + b * LSb + pidx((j / NS) * NS * R + k);
+#pragma unroll
+                for (int r = 0; r < R; ++r) dst[pidx(r * NS)] = v[p * R + r];
+            }
+            __syncthreads();
+            constexpr int R2 = rs_radix(L, E, I + 1), M2 = L / R2;
+#pragma unroll
+            for (int p = 0; p < E / R2; ++p) {
+                const int bf = tid + p * NT, b = bf / M2, j = bf % M2;
+                const C* src = buf + b * LSb;
+                if constexpr (M2 % PADQ == 0) {
+                    src += pidx(j);
+#pragma unroll
+                    for (int r = 0; r < R2; ++r) v[p * R2 + r] = src[pidx(r * M2)];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < R2; ++r) v[p * R2 + r] = src[pidx(j + r * M2)];
+                }
+            }
+            fr_pass<C, L, B, NT, E, DIR, TWO, I + 1>(v, bufA, bufB, LSb, tw, TS, tid);
+        }
+    }
+}
+
+// Unnormalised transform of B sequences held in registers (layout above).
+// The caller guarantees that no thread still reads bufA/bufB from earlier work.
+template <typename C, int L, int B, int NT, int E, int DIR, bool TWO>
+__device__ __forceinline__ void fft_reg(C (&v)[E], C* bufA, C* bufB, int LSb, const C* __restrict__ tw, int TS, int tid) {
+    fr_pass<C, L, B, NT, E, DIR, TWO, 0>(v, bufA, bufB, LSb, tw, TS, tid);
+}
+// whether the last exchange of fft_reg used bufA
+template <int L, int E> constexpr bool fr_last_a() { return rs_np(L, E) >= 2 && (rs_np(L, E) - 2) % 2 == 0; }
+
+// slot s of thread tid -> (sequence b, index n)
+template <int L, int NT, int E> __device__ __forceinline__ void fr_slot(int tid, int s, int& b, int& n) {
+    constexpr int R0 = rs_radix(L, E, 0), M0 = L / R0;
+    const int bf = tid + (s / R0) * NT;
+    b = bf / M0;
+    n = bf % M0 + (s % R0) * M0;
+}
+
 // ------------------------------------------------------- pass geometry ----
-// Elements each thread holds between transforms: fewer elements per thread
-// means fewer live registers and more waves in flight per CU.
 #ifndef HW_X_RUN_BYTES
 #define HW_X_RUN_BYTES 16
 #endif
@@ -203,36 +343,61 @@ __device__ __forceinline__ void fft_lds(C* lds, int LS, const C* __restrict__ tw
 #define HW_X_PER_THREAD 8
 #endif
 #ifndef HW_Y_PER_THREAD
-#define HW_Y_PER_THREAD 4
+#define HW_Y_PER_THREAD 8
 #endif
-// x-pass: RR rows per block so that an RR-element run of one spectral column
-// (the transposed write/read) is 32 bytes long; blocks are remapped so that
-// the row groups sharing a 128-byte line run on the same XCD (same L2).
+constexpr int HW_LDS_BUDGET = 65536;   // bytes of static LDS per block
+
+// x-pass: RR rows per block, so that the transposed spectral runs
+// X[kx][y0 .. y0+RR) are HW_X_RUN_BYTES long (at least one wave per block).
+// A row is a real FFT of length NX done as a complex FFT of L = NX/2 points
+// z[n] = x[2n] + i x[2n+1]; each thread holds E such pairs.
 template <typename T, int NX> struct XGeo {
     using C = typename ctype<T>::type;
-    static constexpr int RR = HW_X_RUN_BYTES / (int)sizeof(C);   // rows per block: 32-byte runs
-    static_assert(RR >= 1, "run shorter than one element");
-    static constexpr int L = NX / 2;                    // half-length complex FFT
-    static constexpr int LS = L + 2;                    // LDS row stride (complex)
-    static constexpr int NPAIR = RR * L;                // physical (x,x+1) pairs per block
-    static constexpr int NT0 = NPAIR / HW_X_PER_THREAD;
-    static constexpr int NT = NT0 > 1024 ? 1024 : (NT0 < 64 ? 64 : NT0);
-    static constexpr int PP = (NPAIR + NT - 1) / NT;    // pairs per thread
+    static constexpr int L = NX / 2;
+    static constexpr int E = HW_X_PER_THREAD;
+    static constexpr int RR0 = HW_X_RUN_BYTES / (int)sizeof(C) > 0 ? HW_X_RUN_BYTES / (int)sizeof(C) : 1;
+    static constexpr int RRW = (64 * E + L - 1) / L;
+    static constexpr int RR = RR0 > RRW ? RR0 : RRW;
+    static constexpr int NT = RR * L / E;
+    static constexpr int PP = E;                        // pairs per thread
+    static constexpr int NPAIR = RR * L;
+    static constexpr int LSb = pidx(L) + 2;             // buffer row: X[0..L] while staging
     static constexpr int NSPEC = RR * (L + 1);          // spectral values per block
     static constexpr int PS = (NSPEC + NT - 1) / NT;
-    static constexpr int LDS_ELEMS = RR * LS;
+    static constexpr int SK = NT / RR;                  // kx step between a thread's spectral slots
+    // two exchange buffers when they fit beside the twiddle table, else one;
+    // the twiddle table is staged in LDS when it fits
+    static constexpr bool TWO = (2 * RR * LSb + NX) * (int)sizeof(C) + 64 <= HW_LDS_BUDGET;
+    static constexpr int LDS_ELEMS = (TWO ? 2 : 1) * RR * LSb;
+    static constexpr bool TWL = (LDS_ELEMS + NX) * (int)sizeof(C) + 64 <= HW_LDS_BUDGET;
+    // physical pair q of thread tid -> row * L + n
+    __device__ static __forceinline__ int ppos(int tid, int q) {
+        int b, n;
+        fr_slot<L, NT, E>(tid, q, b, n);
+        return b * L + n;
+    }
+    static_assert(NT % RR == 0 && NT <= 1024, "x-pass block shape");
 };
 
-// y-pass: CC whole columns per block.
+// y-pass: CC whole columns per block (about 256 threads), E elements each.
 template <typename T, int NY> struct YGeo {
     using C = typename ctype<T>::type;
-    static constexpr int CC = NY >= 1024 ? 1 : 1024 / NY;
-    static constexpr int LS = NY;
+    static constexpr int E = HW_Y_PER_THREAD;
+    static constexpr int CC = NY >= 256 * E ? 1 : (256 * E) / NY;
+    static constexpr int LS = pidx(NY - 1) + 1 + ((pidx(NY - 1) + 1) & 1);
     static constexpr int NE = CC * NY;
-    static constexpr int NT0 = NE / HW_Y_PER_THREAD;
-    static constexpr int NT = NT0 > 1024 ? 1024 : (NT0 < 64 ? 64 : NT0);
-    static constexpr int PE = (NE + NT - 1) / NT;       // elements per thread
-    static constexpr int LDS_ELEMS = CC * LS;
+    static constexpr int NT = NE / E;
+    static constexpr int PE = E;                        // elements per thread
+    static constexpr bool TWO = (2 * CC * LS + NY) * (int)sizeof(C) <= HW_LDS_BUDGET;
+    static constexpr int LDS_ELEMS = (TWO ? 2 : 1) * CC * LS;
+    static constexpr bool TWL = (LDS_ELEMS + NY) * (int)sizeof(C) <= HW_LDS_BUDGET;
+    // element e of thread tid -> column * NY + ky within the block
+    __device__ static __forceinline__ int eidx(int tid, int e) {
+        int b, n;
+        fr_slot<NY, NT, E>(tid, e, b, n);
+        return b * NY + n;
+    }
+    static_assert(NT <= 1024 && NT >= 64, "y-pass block shape");
 };
 
 // MI355X dispatches consecutive workgroups round-robin over its 8 XCDs.  This
@@ -250,16 +415,8 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 template <typename C> struct Tw { const C* x; const C* y; };
 
 // ----------------------------------------------------------- x-pass ops ---
-// Inverse x-pass: spectral X[kx][y0..y0+RR) (kx = 0..NX/2) -> physical rows,
-// delivered as (x[2n], x[2n+1]) pairs in registers, scaled by `scale`.
-// Pair q of this thread is pos = tid + q*NT: row = pos / L, n = pos % L.
-// Also returns, per row of the block, Im of the Nyquist bin times `scale`
-// in nyq_im[row] (LDS scalar array of RR entries), which the adjust operator
-// needs for the reference's complex inverse.
-// The inverse x-pass in two halves so a kernel can issue the (transposed,
-// 32-byte-run) global loads of the next field before running the FFT of the
-// current one: x_spec_load only issues loads into registers; x_inverse_b
-// consumes them.
+// Spectral values of a block: slot q of thread tid is idx = tid + q*NT,
+// kx = idx / RR, row idx % RR (runs of RR consecutive y per kx in global).
 template <typename T, int NX> using XSpec = carr<T, XGeo<T, NX>::PS>;
 
 template <typename T, int NX, int NY>
@@ -277,58 +434,73 @@ __device__ __forceinline__ void x_spec_load(const typename ctype<T>::type* __res
 }
 
 template <typename T, int NX, int NY>
+__device__ __forceinline__ void x_store_spec(typename ctype<T>::type* __restrict__ X, int y0, const XSpec<T, NX>& b,
+                                             int tid) {
+    using G = XGeo<T, NX>;
+#pragma unroll
+    for (int q = 0; q < G::PS; ++q) {
+        int idx = tid + q * G::NT;
+        if (G::NSPEC % G::NT == 0 || idx < G::NSPEC) {
+            int kx = idx / G::RR, r = idx % G::RR;
+            X[(size_t)kx * NY + y0 + r] = b[q];
+        }
+    }
+}
+
+// Inverse x-pass: spectral values (x_spec_load layout) -> physical pairs
+// (x[2n], x[2n+1]) of this thread's slots (XGeo::ppos), scaled by `scale`.
+// Also returns, per row of the block, Im of the Nyquist bin times `scale` in
+// nyq_im[row] (LDS array of RR entries), which the adjust operator needs for
+// the reference's complex inverse.  The imaginary parts of the DC and Nyquist
+// bins are ignored (real-field inverse).
+template <typename T, int NX, int NY>
 __device__ __forceinline__ void x_inverse_b(const XSpec<T, NX>& b, int y0,
                                             typename ctype<T>::type* lds, const typename ctype<T>::type* __restrict__ twx,
                                             carr<T, XGeo<T, NX>::PP>& out, T scale, T* nyq_im, int tid) {
     using G = XGeo<T, NX>;
     using C = typename ctype<T>::type;
-    constexpr int L = G::L, RR = G::RR, LS = G::LS, NT = G::NT;
+    constexpr int L = G::L, RR = G::RR, LSb = G::LSb, NT = G::NT, E = G::E;
     (void)y0;
+    C* S = lds;                                  // staging, then the second exchange buffer
+    C* A = G::TWO ? lds + RR * LSb : lds;
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < G::PS; ++q) {
         int idx = tid + q * NT;
         if (G::NSPEC % NT == 0 || idx < G::NSPEC) {
             int kx = idx / RR, r = idx % RR;
-            lds[r * LS + kx] = b[q];
+            S[r * LSb + pidx(kx)] = b[q];
         }
     }
     __syncthreads();
-    if (nyq_im && tid < RR) nyq_im[tid] = lds[tid * LS + L].y * scale;
-    // pre-process pairs (k, L-k): Z[k] = (X[k] + conj X[L-k]) + i (X[k] - conj X[L-k]) W^-k
-    constexpr int NPR = RR * (L / 2 + 1);
-    for (int idx = tid; idx < NPR; idx += NT) {
-        int r = idx / (L / 2 + 1), k = idx % (L / 2 + 1);
-        C* row = lds + r * LS;
-        if (k == 0) {
-            T a = row[0].x, b = row[L].x;      // imaginary parts of DC / Nyquist ignored
-            T e = a + b, o = a - b;
-            row[0] = cmk<C>(e, o);
-        } else {
-            C xa = row[k], xb = row[L - k];
-            C wa = cconj(twx[k]);              // W^-k = exp(+2 pi i k / NX)
-            C ea = cadd(xa, cconj(xb));
-            C oa = cmul(csub(xa, cconj(xb)), wa);
-            C za = cadd(ea, cmul_i<1>(oa));
-            if (k != L - k) {
-                C wb = cconj(twx[L - k]);
-                C eb = cadd(xb, cconj(xa));
-                C ob = cmul(csub(xb, cconj(xa)), wb);
-                row[L - k] = cadd(eb, cmul_i<1>(ob));
-            }
-            row[k] = za;
-        }
-    }
-    __syncthreads();
-    fft_lds<C, L, RR, NT, +1>(lds, LS, twx, 2, tid);
+    if (nyq_im && tid < RR) nyq_im[tid] = S[tid * LSb + pidx(L)].y * scale;
+    // Z[k] = (X[k] + conj X[L-k]) + i (X[k] - conj X[L-k]) W^-k, W = exp(-2 pi i / NX)
+    constexpr int R0 = rs_radix(L, E, 0), M0 = L / R0;
 #pragma unroll
-    for (int q = 0; q < G::PP; ++q) {
-        int pos = tid + q * NT;
-        if (G::NPAIR % NT == 0 || pos < G::NPAIR) {
-            int r = pos / L, n = pos % L;
-            out[q] = cscale(lds[r * LS + n], scale);
+    for (int p = 0; p < E / R0; ++p) {
+        const int bf = tid + p * NT, rb = bf / M0, j = bf % M0;
+        const C* row = S + rb * LSb;
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+            const int k = j + r * M0;
+            C xa, xb;
+            if constexpr (M0 % PADQ == 0) {
+                xa = row[pidx(j) + pidx(r * M0)];
+                xb = row[pidx(M0 - j) + pidx(L - (r + 1) * M0)];
+            } else {
+                xa = row[pidx(k)];
+                xb = row[pidx(L - k)];
+            }
+            if (r == 0 && j == 0) { xa.y = (T)0; xb.y = (T)0; }
+            const C wa = cconj(twx[k]);
+            const C ea = cadd(xa, cconj(xb));
+            const C oa = cmul(csub(xa, cconj(xb)), wa);
+            out[p * R0 + r] = cadd(ea, cmul_i<1>(oa));
         }
     }
+    fft_reg<C, L, RR, NT, E, +1, G::TWO>(out, A, S, LSb, twx, 2, tid);
+#pragma unroll
+    for (int s = 0; s < E; ++s) out[s] = cscale(out[s], scale);
 }
 
 template <typename T, int NX, int NY>
@@ -340,134 +512,121 @@ __device__ __forceinline__ void x_inverse(const typename ctype<T>::type* __restr
     x_inverse_b<T, NX, NY>(b, y0, lds, twx, out, scale, nyq_im, tid);
 }
 
-// Forward x-pass of a real field given as (x[2n], x[2n+1]) pairs in registers.
-// Leaves X[k], k = 0..L, of every row in LDS (row r at lds[r*LS + k]).
+// Forward x-pass of a real field given as pairs in registers (XGeo::ppos
+// layout) -> this thread's spectral values X[k], k = 0..L (x_spec_load layout).
 template <typename T, int NX>
-__device__ __forceinline__ void x_forward_lds(const carr<T, XGeo<T, NX>::PP>& in,
-                                              typename ctype<T>::type* lds, const typename ctype<T>::type* __restrict__ twx,
-                                              int tid) {
+__device__ __forceinline__ void x_forward_regs(const carr<T, XGeo<T, NX>::PP>& f, XSpec<T, NX>& bs,
+                                               typename ctype<T>::type* lds,
+                                               const typename ctype<T>::type* __restrict__ twx, int tid) {
     using G = XGeo<T, NX>;
     using C = typename ctype<T>::type;
-    constexpr int L = G::L, RR = G::RR, LS = G::LS, NT = G::NT;
-    __syncthreads();
+    constexpr int L = G::L, RR = G::RR, LSb = G::LSb, NT = G::NT, E = G::E, SK = G::SK;
+    C* S = lds;
+    C* A = G::TWO ? lds + RR * LSb : lds;
+    C v[E];
 #pragma unroll
-    for (int q = 0; q < G::PP; ++q) {
-        int pos = tid + q * NT;
-        if (G::NPAIR % NT == 0 || pos < G::NPAIR) {
-            int r = pos / L, n = pos % L;
-            lds[r * LS + n] = in[q];
-        }
-    }
+    for (int s = 0; s < E; ++s) v[s] = f[s];
     __syncthreads();
-    fft_lds<C, L, RR, NT, -1>(lds, LS, twx, 2, tid);
-    // post-process pairs: X[k] = (Z[k] + conj Z[L-k])/2 - i/2 W^k (Z[k] - conj Z[L-k])
-    constexpr int NPR = RR * (L / 2 + 1);
-    for (int idx = tid; idx < NPR; idx += NT) {
-        int r = idx / (L / 2 + 1), k = idx % (L / 2 + 1);
-        C* row = lds + r * LS;
-        if (k == 0) {
-            C z = row[0];
-            row[0] = cmk<C>(z.x + z.y, (T)0);
-            row[L] = cmk<C>(z.x - z.y, (T)0);
+    fft_reg<C, L, RR, NT, E, -1, G::TWO>(v, A, S, LSb, twx, 2, tid);
+    // Z to LDS in natural order (the buffer the last exchange did not use)
+    C* O = G::TWO ? (fr_last_a<L, E>() ? S : A) : S;
+    if constexpr (!G::TWO) __syncthreads();
+    constexpr int R0 = rs_radix(L, E, 0), M0 = L / R0;
+#pragma unroll
+    for (int p = 0; p < E / R0; ++p) {
+        const int bf = tid + p * NT, rb = bf / M0, j = bf % M0;
+        C* row = O + rb * LSb;
+        if constexpr (M0 % PADQ == 0) {
+            row += pidx(j);
+#pragma unroll
+            for (int r = 0; r < R0; ++r) row[pidx(r * M0)] = v[p * R0 + r];
         } else {
-            C za = row[k], zb = row[L - k];
-            C ea = cscale(cadd(za, cconj(zb)), (T)0.5);
-            C da = cscale(csub(za, cconj(zb)), (T)0.5);
-            C xa = cadd(ea, cmul_i<-1>(cmul(twx[k], da)));
-            if (k != L - k) {
-                C eb = cscale(cadd(zb, cconj(za)), (T)0.5);
-                C db = cscale(csub(zb, cconj(za)), (T)0.5);
-                row[L - k] = cadd(eb, cmul_i<-1>(cmul(twx[L - k], db)));
-            }
-            row[k] = xa;
+#pragma unroll
+            for (int r = 0; r < R0; ++r) row[pidx(j + r * M0)] = v[p * R0 + r];
         }
     }
     __syncthreads();
-}
-
-// Transposed store of X[k] (k = 0..L) of the RR rows in LDS to X[kx][y0+r].
-template <typename T, int NX, int NY>
-__device__ __forceinline__ void x_store_spec(typename ctype<T>::type* __restrict__ X, int y0,
-                                             const typename ctype<T>::type* lds, int tid) {
-    using G = XGeo<T, NX>;
-    constexpr int RR = G::RR, LS = G::LS, NT = G::NT;
+    // X[k] = (Z[k] + conj Z[L-k])/2 - i/2 W^k (Z[k] - conj Z[L-k])
+    const int kx0 = tid / RR, rr = tid % RR;
+    const C* row = O + rr * LSb;
 #pragma unroll
     for (int q = 0; q < G::PS; ++q) {
-        int idx = tid + q * NT;
-        if (G::NSPEC % NT == 0 || idx < G::NSPEC) {
-            int kx = idx / RR, r = idx % RR;
-            X[(size_t)kx * NY + y0 + r] = lds[r * LS + kx];
-        }
-    }
-}
-
-// Read the spectral values of the block (same ownership as x_store_spec) from LDS.
-template <typename T, int NX>
-__device__ __forceinline__ void x_spec_to_regs(carr<T, XGeo<T, NX>::PS>& v,
-                                               const typename ctype<T>::type* lds, int tid) {
-    using G = XGeo<T, NX>;
-    constexpr int RR = G::RR, LS = G::LS, NT = G::NT;
-#pragma unroll
-    for (int q = 0; q < G::PS; ++q) {
-        int idx = tid + q * NT;
-        if (G::NSPEC % NT == 0 || idx < G::NSPEC) {
-            int kx = idx / RR, r = idx % RR;
-            v[q] = lds[r * LS + kx];
+        const int kx = kx0 + q * SK;
+        if (G::NSPEC % NT == 0 || kx <= L) {
+            if (kx == 0 || kx == L) {
+                const C z = row[0];
+                bs[q] = kx == 0 ? cmk<C>(z.x + z.y, (T)0) : cmk<C>(z.x - z.y, (T)0);
+            } else {
+                C za, zb;
+                if constexpr (SK % PADQ == 0) {
+                    za = row[pidx(kx0) + pidx(q * SK)];
+                    zb = row[pidx(SK - kx0) + pidx(L - (q + 1) * SK)];
+                } else {
+                    za = row[pidx(kx)];
+                    zb = row[pidx(L - kx)];
+                }
+                const C ea = cscale(cadd(za, cconj(zb)), (T)0.5);
+                const C da = cscale(csub(za, cconj(zb)), (T)0.5);
+                bs[q] = cadd(ea, cmul_i<-1>(cmul(twx[kx], da)));
+            }
         }
     }
 }
 
 // ----------------------------------------------------------- y-pass ops ---
-// Load CC columns (col0 ...) of a spectral array, multiply by a real
-// per-entry multiplier (optional, same [kx][ky] layout), into LDS.
-// Element e of this thread: idx = tid + e*NT -> c = idx / NY, ky = idx % NY.
-template <typename T, int NY>
-__device__ __forceinline__ void y_regs_to_lds(const carr<T, YGeo<T, NY>::PE>& v,
-                                              typename ctype<T>::type* lds, int tid) {
-    using G = YGeo<T, NY>;
-#pragma unroll
-    for (int e = 0; e < G::PE; ++e) {
-        int idx = tid + e * G::NT;
-        if (G::NE % G::NT == 0 || idx < G::NE) lds[idx] = v[e];
-    }
-}
-
-template <typename T, int NY>
-__device__ __forceinline__ void y_lds_to_regs(carr<T, YGeo<T, NY>::PE>& v,
-                                              const typename ctype<T>::type* lds, int tid) {
-    using G = YGeo<T, NY>;
-#pragma unroll
-    for (int e = 0; e < G::PE; ++e) {
-        int idx = tid + e * G::NT;
-        if (G::NE % G::NT == 0 || idx < G::NE) v[e] = lds[idx];
-    }
-}
-
 // Hermitian projection of the reference (2d/functions1.c:492-521) restricted
-// to the stored half: column kx=0 gets its rows ky>NY/2 from conj(rows NY-ky)
-// and Im zeroed at ky=0, NY/2; column kx=NX/2 gets Im zeroed at ky=0, NY/2.
-template <typename T, int NY>
-__device__ __forceinline__ void y_project(typename ctype<T>::type* lds, int col0, int ncolx, int tid) {
+// to the stored half, applied to a block's columns in registers: column kx=0
+// gets its rows ky>NY/2 from conj(rows NY-ky) and Im zeroed at ky=0, NY/2;
+// column kx=NX/2 gets Im zeroed at ky=0, NY/2.  `buf` is an LDS row of NY
+// (padded) no thread reads until the next barrier.
+template <typename T, int NX, int NY>
+__device__ __forceinline__ void y_project_regs(carr<T, YGeo<T, NY>::PE>& v, typename ctype<T>::type* buf, int col0,
+                                               int tid) {
     using G = YGeo<T, NY>;
     using C = typename ctype<T>::type;
-    for (int c = 0; c < G::CC; ++c) {
-        int kx = col0 + c;
-        if (kx == 0) {
-            C* col = lds + c * G::LS;
-            for (int ky = NY / 2 + 1 + tid; ky < NY; ky += G::NT) col[ky] = cconj(col[NY - ky]);
-            if (tid == 0) { col[0].y = 0; col[NY / 2].y = 0; }
-        } else if (kx == ncolx - 1) {
-            C* col = lds + c * G::LS;
-            if (tid == 0) { col[0].y = 0; col[NY / 2].y = 0; }
+    if (col0 == 0) {   // block-uniform
+#pragma unroll
+        for (int e = 0; e < G::PE; ++e) {
+            int b, n;
+            fr_slot<NY, G::NT, G::E>(tid, e, b, n);
+            if (b == 0) buf[pidx(n)] = v[e];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < G::PE; ++e) {
+            int b, n;
+            fr_slot<NY, G::NT, G::E>(tid, e, b, n);
+            if (b == 0 && n > NY / 2) v[e] = cconj(buf[pidx(NY - n)]);
         }
     }
+#pragma unroll
+    for (int e = 0; e < G::PE; ++e) {
+        int b, n;
+        fr_slot<NY, G::NT, G::E>(tid, e, b, n);
+        const int kx = col0 + b;
+        if ((kx == 0 || kx == NX / 2) && (n == 0 || n == NY / 2)) v[e].y = (T)0;
+    }
 }
 
-template <typename T, int NY, int DIR>
-__device__ __forceinline__ void y_fft(typename ctype<T>::type* lds, const typename ctype<T>::type* __restrict__ twy, int tid) {
+template <typename T, int NX, int NY>
+__device__ __forceinline__ void y_fwd_regs(carr<T, YGeo<T, NY>::PE>& v, typename ctype<T>::type* lds,
+                                           const typename ctype<T>::type* twy, int tid) {
     using G = YGeo<T, NY>;
     using C = typename ctype<T>::type;
-    fft_lds<C, NY, G::CC, G::NT, DIR>(lds, G::LS, twy, 1, tid);
+    __syncthreads();
+    fft_reg<C, NY, G::CC, G::NT, G::E, -1, G::TWO>(v, lds, G::TWO ? lds + G::CC * G::LS : lds, G::LS, twy, 1, tid);
+}
+
+template <typename T, int NX, int NY>
+__device__ __forceinline__ void y_inv_regs(carr<T, YGeo<T, NY>::PE>& v, typename ctype<T>::type* lds,
+                                           const typename ctype<T>::type* twy, int col0, int tid) {
+    using G = YGeo<T, NY>;
+    using C = typename ctype<T>::type;
+    __syncthreads();
+    // the projection stages column 0 in the buffer the first exchange does not write
+    C* pb = G::TWO ? lds + G::CC * G::LS : lds;
+    y_project_regs<T, NX, NY>(v, pb, col0, tid);
+    fft_reg<C, NY, G::CC, G::NT, G::E, +1, G::TWO>(v, lds, G::TWO ? lds + G::CC * G::LS : lds, G::LS, twy, 1, tid);
 }
 
 }  // namespace hw

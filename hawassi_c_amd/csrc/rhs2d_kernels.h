@@ -70,7 +70,7 @@ __device__ __forceinline__ void ph_load(carr<T, XGeo<T, NX>::PP>& v, const T* __
     const C* a2 = reinterpret_cast<const C*>(a) + (size_t)y0 * G::L;
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) {
-        int pos = tid + q * G::NT;
+        int pos = G::ppos(tid, q);
         if (G::NPAIR % G::NT == 0 || pos < G::NPAIR) v[q] = a2[pos];
     }
 }
@@ -82,7 +82,7 @@ __device__ __forceinline__ void ph_store(T* __restrict__ a, const carr<T, XGeo<T
     C* a2 = reinterpret_cast<C*>(a) + (size_t)y0 * G::L;
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) {
-        int pos = tid + q * G::NT;
+        int pos = G::ppos(tid, q);
         if (G::NPAIR % G::NT == 0 || pos < G::NPAIR) a2[pos] = v[q];
     }
 }
@@ -95,8 +95,9 @@ template <typename C> __device__ __forceinline__ C pfma(C a, C b, C c) { return 
 template <typename T, int NX, int NY>
 __device__ __forceinline__ void x_fwd_store(const carr<T, XGeo<T, NX>::PP>& f, typename ctype<T>::type* X,
                                             int y0, typename ctype<T>::type* lds, const typename ctype<T>::type* twx, int tid) {
-    x_forward_lds<T, NX>(f, lds, twx, tid);
-    x_store_spec<T, NX, NY>(X, y0, lds, tid);
+    XSpec<T, NX> s;
+    x_forward_regs<T, NX>(f, s, lds, twx, tid);
+    x_store_spec<T, NX, NY>(X, y0, s, tid);
 }
 
 // forward x of f1 + i f2 (both real): X = F(f1) + i F(f2)
@@ -105,20 +106,12 @@ __device__ __forceinline__ void x_fwd2_store(const carr<T, XGeo<T, NX>::PP>& f1,
                                              const carr<T, XGeo<T, NX>::PP>& f2, typename ctype<T>::type* X,
                                              int y0, typename ctype<T>::type* lds, const typename ctype<T>::type* twx, int tid) {
     using G = XGeo<T, NX>;
-    using C = typename ctype<T>::type;
-    C s1[G::PS];
-    x_forward_lds<T, NX>(f1, lds, twx, tid);
-    x_spec_to_regs<T, NX>(s1, lds, tid);
-    x_forward_lds<T, NX>(f2, lds, twx, tid);
+    XSpec<T, NX> s1, s2;
+    x_forward_regs<T, NX>(f1, s1, lds, twx, tid);
+    x_forward_regs<T, NX>(f2, s2, lds, twx, tid);
 #pragma unroll
-    for (int q = 0; q < G::PS; ++q) {
-        int idx = tid + q * G::NT;
-        if (G::NSPEC % G::NT == 0 || idx < G::NSPEC) {
-            int kx = idx / G::RR, r = idx % G::RR;
-            C b = lds[r * G::LS + kx];
-            X[(size_t)kx * NY + y0 + r] = cadd(s1[q], cmul_i<1>(b));
-        }
-    }
+    for (int q = 0; q < G::PS; ++q) s1[q] = cadd(s1[q], cmul_i<1>(s2[q]));
+    x_store_spec<T, NX, NY>(X, y0, s1, tid);
 }
 
 // y-pass column element load with optional real multiplier
@@ -130,7 +123,7 @@ __device__ __forceinline__ void y_load(carr<T, YGeo<T, NY>::PE>& v, const typena
     constexpr int NXH = NX / 2 + 1;
 #pragma unroll
     for (int e = 0; e < G::PE; ++e) {
-        int idx = tid + e * G::NT;
+        int idx = G::eidx(tid, e);
         int c = idx / NY;
         bool ok = (G::NE % G::NT == 0 || idx < G::NE) && (col0 + c < NXH);
         if (ok) {
@@ -151,41 +144,18 @@ __device__ __forceinline__ void y_store(typename ctype<T>::type* __restrict__ ds
     constexpr int NXH = NX / 2 + 1;
 #pragma unroll
     for (int e = 0; e < G::PE; ++e) {
-        int idx = tid + e * G::NT;
+        int idx = G::eidx(tid, e);
         int c = idx / NY;
         bool ok = (G::NE % G::NT == 0 || idx < G::NE) && (col0 + c < NXH);
         if (ok) dst[(size_t)col0 * NY + idx] = v[e];
     }
 }
 
-// in-register column transforms (LDS round trip)
-template <typename T, int NX, int NY>
-__device__ __forceinline__ void y_fwd_regs(carr<T, YGeo<T, NY>::PE>& v, typename ctype<T>::type* lds,
-                                           const typename ctype<T>::type* twy, int tid) {
-    __syncthreads();
-    y_regs_to_lds<T, NY>(v, lds, tid);
-    __syncthreads();
-    y_fft<T, NY, -1>(lds, twy, tid);
-    y_lds_to_regs<T, NY>(v, lds, tid);
-}
-
-template <typename T, int NX, int NY>
-__device__ __forceinline__ void y_inv_regs(carr<T, YGeo<T, NY>::PE>& v, typename ctype<T>::type* lds,
-                                           const typename ctype<T>::type* twy, int col0, int tid) {
-    __syncthreads();
-    y_regs_to_lds<T, NY>(v, lds, tid);
-    __syncthreads();
-    y_project<T, NY>(lds, col0, NX / 2 + 1, tid);
-    __syncthreads();
-    y_fft<T, NY, +1>(lds, twy, tid);
-    y_lds_to_regs<T, NY>(v, lds, tid);
-}
-
 // per-element spectral coordinates of this thread's y-pass element e
 template <typename T, int NX, int NY> struct YIdx {
     __device__ __forceinline__ static void get(int col0, int tid, int e, int& kx, int& ky, size_t& gi) {
         using G = YGeo<T, NY>;
-        int idx = tid + e * G::NT;
+        int idx = G::eidx(tid, e);
         kx = col0 + idx / NY;
         ky = idx % NY;
         gi = (size_t)col0 * NY + idx;
@@ -199,11 +169,13 @@ template <typename T, int NX, int NY> struct YIdx {
     using G = XGeo<T, NX>;                                               \
     __shared__ C lds[G::LDS_ELEMS];                                      \
     __shared__ T nyq[G::RR];                                             \
-    __shared__ C twxs[NX];                                               \
+    __shared__ C twxs_l[G::TWL ? NX : 1];                                \
+    const C* twxs = G::TWL ? twxs_l : (TWSRC);                           \
     const int tid = threadIdx.x;                                         \
     const int y0 = xcd_remap(blockIdx.x, gridDim.x) * G::RR;             \
     const T sc = (T)1 / ((T)NX * (T)NY);                                 \
-    for (int i_ = tid; i_ < NX; i_ += G::NT) twxs[i_] = (TWSRC)[i_];     \
+    if (G::TWL)                                                          \
+        for (int i_ = tid; i_ < NX; i_ += G::NT) twxs_l[i_] = (TWSRC)[i_]; \
     __syncthreads();                                                     \
     (void)nyq; (void)sc;
 
@@ -212,10 +184,12 @@ template <typename T, int NX, int NY> struct YIdx {
     using C = typename ctype<T>::type;                                   \
     using G = YGeo<T, NY>;                                               \
     __shared__ C lds[G::LDS_ELEMS];                                      \
-    __shared__ C twys[NY];                                               \
+    __shared__ C twys_l[G::TWL ? NY : 1];                                \
+    const C* twys = G::TWL ? twys_l : (TWSRC);                           \
     const int tid = threadIdx.x;                                         \
     const int col0 = blockIdx.x * G::CC;                                 \
-    for (int i_ = tid; i_ < NY; i_ += G::NT) twys[i_] = (TWSRC)[i_];     \
+    if (G::TWL)                                                          \
+        for (int i_ = tid; i_ < NY; i_ += G::NT) twys_l[i_] = (TWSRC)[i_]; \
     __syncthreads();
 
 // W slots used by the 2DB pipeline
@@ -491,7 +465,7 @@ __device__ __forceinline__ void x_vel_out(const Rhs2DArgs<T>& a, const XSpec<T, 
     x_inverse_b<T, NX, NY>(b, y0, lds, twl, f1, sc, nyq, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) {
-        int pos = tid + q * G::NT;
+        int pos = G::ppos(tid, q);
         int r = pos / G::L;
         T s = nyq[r < G::RR ? r : 0];
         f1[q] = csub(cmk<C>(-tc[q].x * f1[q].x, -tc[q].y * f1[q].y), pmul(tf[q], v[q]));
@@ -620,7 +594,7 @@ __device__ __forceinline__ void x_eta_out(const Rhs2DArgs<T>& a, const XSpec<T, 
     if (a.s_eta_out && tid < G::RR) a.s_eta_out[y0 + tid] = nyq[tid];
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) {
-        int pos = tid + q * G::NT;
+        int pos = G::ppos(tid, q);
         int r = pos / G::L;
         T s = nyq[r < G::RR ? r : 0];
         f1[q] = csub(pmul(tc[q], f1[q]), pmul(tf[q], te[q]));
@@ -629,7 +603,7 @@ __device__ __forceinline__ void x_eta_out(const Rhs2DArgs<T>& a, const XSpec<T, 
     if (a.src_on) {
 #pragma unroll
         for (int q = 0; q < G::PP; ++q) {
-            int pos = tid + q * G::NT;
+            int pos = G::ppos(tid, q);
             T s0, s1;
             src_pair<T, NX>(a, y0, pos, tn[q], ts[q], s0, s1);
             f1[q] = cadd(f1[q], cmk<C>(s0, s1));
