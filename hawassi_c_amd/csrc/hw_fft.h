@@ -337,7 +337,7 @@ template <int L, int NT, int E> __device__ __forceinline__ void fr_slot(int tid,
 
 // ------------------------------------------------------- pass geometry ----
 #ifndef HW_X_RUN_BYTES
-#define HW_X_RUN_BYTES 16
+#define HW_X_RUN_BYTES 64
 #endif
 #ifndef HW_X_PER_THREAD
 #define HW_X_PER_THREAD 8
@@ -345,10 +345,21 @@ template <int L, int NT, int E> __device__ __forceinline__ void fr_slot(int tid,
 #ifndef HW_Y_PER_THREAD
 #define HW_Y_PER_THREAD 8
 #endif
-constexpr int HW_LDS_BUDGET = 65536;   // bytes of static LDS per block
+// LDS budgets per block (gfx950 has 160 KB per CU).  An x-pass block is the
+// only block on its CU (1024 threads), a y-pass block shares the CU.
+#ifndef HW_X_LDS_BUDGET
+#define HW_X_LDS_BUDGET 163840
+#endif
+#ifndef HW_Y_LDS_BUDGET
+#define HW_Y_LDS_BUDGET 65536
+#endif
 
 // x-pass: RR rows per block, so that the transposed spectral runs
 // X[kx][y0 .. y0+RR) are HW_X_RUN_BYTES long (at least one wave per block).
+// Short runs cost far more than their bytes: the lines they touch are shared
+// by several blocks, so each line crosses the L2 -> L1 path once per block and
+// partial-line stores are merged in L2; 64-byte runs with streaming stores
+// keep the transposed traffic close to its byte count.
 // A row is a real FFT of length NX done as a complex FFT of L = NX/2 points
 // z[n] = x[2n] + i x[2n+1]; each thread holds E such pairs.
 template <typename T, int NX> struct XGeo {
@@ -367,9 +378,9 @@ template <typename T, int NX> struct XGeo {
     static constexpr int SK = NT / RR;                  // kx step between a thread's spectral slots
     // two exchange buffers when they fit beside the twiddle table, else one;
     // the twiddle table is staged in LDS when it fits
-    static constexpr bool TWO = (2 * RR * LSb + NX) * (int)sizeof(C) + 64 <= HW_LDS_BUDGET;
+    static constexpr bool TWO = (2 * RR * LSb + NX) * (int)sizeof(C) + 64 <= HW_X_LDS_BUDGET;
     static constexpr int LDS_ELEMS = (TWO ? 2 : 1) * RR * LSb;
-    static constexpr bool TWL = (LDS_ELEMS + NX) * (int)sizeof(C) + 64 <= HW_LDS_BUDGET;
+    static constexpr bool TWL = (LDS_ELEMS + NX) * (int)sizeof(C) + 64 <= HW_X_LDS_BUDGET;
     // physical pair q of thread tid -> row * L + n
     __device__ static __forceinline__ int ppos(int tid, int q) {
         int b, n;
@@ -388,9 +399,9 @@ template <typename T, int NY> struct YGeo {
     static constexpr int NE = CC * NY;
     static constexpr int NT = NE / E;
     static constexpr int PE = E;                        // elements per thread
-    static constexpr bool TWO = (2 * CC * LS + NY) * (int)sizeof(C) <= HW_LDS_BUDGET;
+    static constexpr bool TWO = (2 * CC * LS + NY) * (int)sizeof(C) <= HW_Y_LDS_BUDGET;
     static constexpr int LDS_ELEMS = (TWO ? 2 : 1) * CC * LS;
-    static constexpr bool TWL = (LDS_ELEMS + NY) * (int)sizeof(C) <= HW_LDS_BUDGET;
+    static constexpr bool TWL = (LDS_ELEMS + NY) * (int)sizeof(C) <= HW_Y_LDS_BUDGET;
     // element e of thread tid -> column * NY + ky within the block
     __device__ static __forceinline__ int eidx(int tid, int e) {
         int b, n;
@@ -442,7 +453,9 @@ __device__ __forceinline__ void x_store_spec(typename ctype<T>::type* __restrict
         int idx = tid + q * G::NT;
         if (G::NSPEC % G::NT == 0 || idx < G::NSPEC) {
             int kx = idx / G::RR, r = idx % G::RR;
-            X[(size_t)kx * NY + y0 + r] = b[q];
+            typename ctype<T>::type* p = X + (size_t)kx * NY + y0 + r;
+            __builtin_nontemporal_store(b[q].x, &p->x);   // streaming: written once, read by the next kernel
+            __builtin_nontemporal_store(b[q].y, &p->y);
         }
     }
 }
@@ -453,25 +466,32 @@ __device__ __forceinline__ void x_store_spec(typename ctype<T>::type* __restrict
 // nyq_im[row] (LDS array of RR entries), which the adjust operator needs for
 // the reference's complex inverse.  The imaginary parts of the DC and Nyquist
 // bins are ignored (real-field inverse).
-template <typename T, int NX, int NY>
-__device__ __forceinline__ void x_inverse_b(const XSpec<T, NX>& b, int y0,
-                                            typename ctype<T>::type* lds, const typename ctype<T>::type* __restrict__ twx,
-                                            carr<T, XGeo<T, NX>::PP>& out, T scale, T* nyq_im, int tid) {
+// In two halves: x_inv_stage hands the spectral registers to LDS (after which
+// the caller may reuse them, e.g. to issue the next field's loads so that they
+// are in flight during the FFT), x_inv_finish runs the transform.
+template <typename T, int NX>
+__device__ __forceinline__ void x_inv_stage(const XSpec<T, NX>& b, typename ctype<T>::type* lds, int tid) {
     using G = XGeo<T, NX>;
-    using C = typename ctype<T>::type;
-    constexpr int L = G::L, RR = G::RR, LSb = G::LSb, NT = G::NT, E = G::E;
-    (void)y0;
-    C* S = lds;                                  // staging, then the second exchange buffer
-    C* A = G::TWO ? lds + RR * LSb : lds;
+    constexpr int RR = G::RR, LSb = G::LSb, NT = G::NT;
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < G::PS; ++q) {
         int idx = tid + q * NT;
         if (G::NSPEC % NT == 0 || idx < G::NSPEC) {
             int kx = idx / RR, r = idx % RR;
-            S[r * LSb + pidx(kx)] = b[q];
+            lds[r * LSb + pidx(kx)] = b[q];
         }
     }
+}
+
+template <typename T, int NX>
+__device__ __forceinline__ void x_inv_finish(typename ctype<T>::type* lds, const typename ctype<T>::type* __restrict__ twx,
+                                             carr<T, XGeo<T, NX>::PP>& out, T scale, T* nyq_im, int tid) {
+    using G = XGeo<T, NX>;
+    using C = typename ctype<T>::type;
+    constexpr int L = G::L, RR = G::RR, LSb = G::LSb, NT = G::NT, E = G::E;
+    C* S = lds;                                  // staging, then the second exchange buffer
+    C* A = G::TWO ? lds + RR * LSb : lds;
     __syncthreads();
     if (nyq_im && tid < RR) nyq_im[tid] = S[tid * LSb + pidx(L)].y * scale;
     // Z[k] = (X[k] + conj X[L-k]) + i (X[k] - conj X[L-k]) W^-k, W = exp(-2 pi i / NX)
@@ -501,6 +521,15 @@ __device__ __forceinline__ void x_inverse_b(const XSpec<T, NX>& b, int y0,
     fft_reg<C, L, RR, NT, E, +1, G::TWO>(out, A, S, LSb, twx, 2, tid);
 #pragma unroll
     for (int s = 0; s < E; ++s) out[s] = cscale(out[s], scale);
+}
+
+template <typename T, int NX, int NY>
+__device__ __forceinline__ void x_inverse_b(const XSpec<T, NX>& b, int y0,
+                                            typename ctype<T>::type* lds, const typename ctype<T>::type* __restrict__ twx,
+                                            carr<T, XGeo<T, NX>::PP>& out, T scale, T* nyq_im, int tid) {
+    (void)y0;
+    x_inv_stage<T, NX>(b, lds, tid);
+    x_inv_finish<T, NX>(lds, twx, out, scale, nyq_im, tid);
 }
 
 template <typename T, int NX, int NY>

@@ -114,6 +114,42 @@ __device__ __forceinline__ void x_fwd2_store(const carr<T, XGeo<T, NX>::PP>& f1,
     x_store_spec<T, NX, NY>(X, y0, s1, tid);
 }
 
+// forward x of f1 + i f2 where f2 = c (-1)^x s(y) (c real per point, s per
+// row): the adjust operator's imaginary part.  f2 is formed after F(f1) so
+// that only c stays live during the first transform.
+template <typename T, int NX, int NY>
+__device__ __forceinline__ void x_fwd_adj_store(const carr<T, XGeo<T, NX>::PP>& f1, const carr<T, XGeo<T, NX>::PP>& c,
+                                                T sgn, const T* nyq, typename ctype<T>::type* X, int y0,
+                                                typename ctype<T>::type* lds, const typename ctype<T>::type* twx, int tid) {
+    using G = XGeo<T, NX>;
+    using C = typename ctype<T>::type;
+    XSpec<T, NX> s1, s2;
+    x_forward_regs<T, NX>(f1, s1, lds, twx, tid);
+    C f2[G::PP];
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) {
+        const int r = G::ppos(tid, q) / G::L;
+        const T sv = sgn * nyq[r < G::RR ? r : 0];
+        f2[q] = cmk<C>(c[q].x * sv, -c[q].y * sv);
+    }
+    x_forward_regs<T, NX>(f2, s2, lds, twx, tid);
+#pragma unroll
+    for (int q = 0; q < G::PS; ++q) s1[q] = cadd(s1[q], cmul_i<1>(s2[q]));
+    x_store_spec<T, NX, NY>(X, y0, s1, tid);
+}
+
+// Inverse x of the field held in b; b is refilled with `next` (when set)
+// while the transform runs, so one spectral buffer keeps the loads of the
+// following field in flight.
+template <typename T, int NX, int NY>
+__device__ __forceinline__ void x_inv_roll(XSpec<T, NX>& b, const typename ctype<T>::type* next, int y0,
+                                           typename ctype<T>::type* lds, const typename ctype<T>::type* twl,
+                                           carr<T, XGeo<T, NX>::PP>& out, T scale, T* nyq_im, int tid) {
+    x_inv_stage<T, NX>(b, lds, tid);
+    if (next) x_spec_load<T, NX, NY>(next, y0, b, tid);
+    x_inv_finish<T, NX>(lds, twl, out, scale, nyq_im, tid);
+}
+
 // y-pass column element load with optional real multiplier
 template <typename T, int NX, int NY>
 __device__ __forceinline__ void y_load(carr<T, YGeo<T, NY>::PE>& v, const typename ctype<T>::type* __restrict__ src,
@@ -238,53 +274,36 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y1(Rhs2DArgs<T> a) {
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x1(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
-    // eta, u, v and the HSS inverses of u^ and v^; a rolling pair of load
-    // buffers keeps the next field's global loads in flight during each FFT
-    C f[G::PP], acc[G::PP], gm[G::PP], gp[G::PP], u[G::PP];
-    XSpec<T, NX> b0, b1;
-    x_spec_load<T, NX, NY>(a.W[S_ETA], y0, b0, tid);
-    x_spec_load<T, NX, NY>(a.W[S_U], y0, b1, tid);
+    // eta, u, v and the HSS inverses of u^ and v^.  One spectral buffer rolls
+    // through the seven inputs (each inverse refills it with the next one).
+    C f[G::PP], u[G::PP], gm[G::PP], gp[G::PP];
+    XSpec<T, NX> b;
+    x_spec_load<T, NX, NY>(a.W[S_ETA], y0, b, tid);
     ph_load<T, NX, NY>(gm, a.gam_m, y0, tid);
     ph_load<T, NX, NY>(gp, a.gam_p, y0, tid);
-    x_inverse_b<T, NX, NY>(b0, y0, lds, twxs, f, sc, nullptr, tid);
-    x_spec_load<T, NX, NY>(a.W[S_AMU], y0, b0, tid);
+    x_inv_roll<T, NX, NY>(b, a.W[S_U], y0, lds, twxs, f, sc, nullptr, tid);
     ph_store<T, NX, NY>(a.eta_p, f, y0, tid);
-    // --- u: b1 = U, b0 = AMU
-    x_inverse_b<T, NX, NY>(b1, y0, lds, twxs, u, sc, nullptr, tid);
-    x_spec_load<T, NX, NY>(a.W[S_APU], y0, b1, tid);
-    ph_store<T, NX, NY>(a.u_p, u, y0, tid);
+    #pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int s_am = c == 0 ? S_AMU : S_AMV, s_ap = c == 0 ? S_APU : S_APV;
+        // velocity: store it, F(g- vel), F(g+ vel)
+        x_inv_roll<T, NX, NY>(b, a.W[s_am], y0, lds, twxs, u, sc, nullptr, tid);
+        ph_store<T, NX, NY>(c == 0 ? a.u_p : a.v_p, u, y0, tid);
 #pragma unroll
-    for (int q = 0; q < G::PP; ++q) f[q] = pmul(gm[q], u[q]);
-    x_fwd_store<T, NX, NY>(f, a.W[S_FGMU], y0, lds, twxs, tid);
+        for (int q = 0; q < G::PP; ++q) f[q] = pmul(gm[q], u[q]);
+        x_fwd_store<T, NX, NY>(f, a.W[c == 0 ? S_FGMU : S_FGMV], y0, lds, twxs, tid);
 #pragma unroll
-    for (int q = 0; q < G::PP; ++q) f[q] = pmul(gp[q], u[q]);
-    x_fwd_store<T, NX, NY>(f, a.W[S_FGPU], y0, lds, twxs, tid);
-    x_inverse_b<T, NX, NY>(b0, y0, lds, twxs, f, sc, nullptr, tid);
-    x_spec_load<T, NX, NY>(a.W[S_V], y0, b0, tid);
+        for (int q = 0; q < G::PP; ++q) f[q] = pmul(gp[q], u[q]);
+        x_fwd_store<T, NX, NY>(f, a.W[c == 0 ? S_FGPU : S_FGPV], y0, lds, twxs, tid);
+        // F(g- F^-1(C2- vel^) + g+ F^-1(C2+ vel^)), accumulated in u
+        x_inv_roll<T, NX, NY>(b, a.W[s_ap], y0, lds, twxs, f, sc, nullptr, tid);
 #pragma unroll
-    for (int q = 0; q < G::PP; ++q) acc[q] = pmul(gm[q], f[q]);
-    x_inverse_b<T, NX, NY>(b1, y0, lds, twxs, f, sc, nullptr, tid);
-    x_spec_load<T, NX, NY>(a.W[S_AMV], y0, b1, tid);
+        for (int q = 0; q < G::PP; ++q) u[q] = pmul(gm[q], f[q]);
+        x_inv_roll<T, NX, NY>(b, c == 0 ? a.W[S_V] : nullptr, y0, lds, twxs, f, sc, nullptr, tid);
 #pragma unroll
-    for (int q = 0; q < G::PP; ++q) acc[q] = pfma(gp[q], f[q], acc[q]);
-    x_fwd_store<T, NX, NY>(acc, a.W[S_FGAU], y0, lds, twxs, tid);
-    // --- v: b0 = V, b1 = AMV
-    x_inverse_b<T, NX, NY>(b0, y0, lds, twxs, u, sc, nullptr, tid);
-    x_spec_load<T, NX, NY>(a.W[S_APV], y0, b0, tid);
-    ph_store<T, NX, NY>(a.v_p, u, y0, tid);
-#pragma unroll
-    for (int q = 0; q < G::PP; ++q) f[q] = pmul(gm[q], u[q]);
-    x_fwd_store<T, NX, NY>(f, a.W[S_FGMV], y0, lds, twxs, tid);
-#pragma unroll
-    for (int q = 0; q < G::PP; ++q) f[q] = pmul(gp[q], u[q]);
-    x_fwd_store<T, NX, NY>(f, a.W[S_FGPV], y0, lds, twxs, tid);
-    x_inverse_b<T, NX, NY>(b1, y0, lds, twxs, f, sc, nullptr, tid);
-#pragma unroll
-    for (int q = 0; q < G::PP; ++q) acc[q] = pmul(gm[q], f[q]);
-    x_inverse_b<T, NX, NY>(b0, y0, lds, twxs, f, sc, nullptr, tid);
-#pragma unroll
-    for (int q = 0; q < G::PP; ++q) acc[q] = pfma(gp[q], f[q], acc[q]);
-    x_fwd_store<T, NX, NY>(acc, a.W[S_FGAV], y0, lds, twxs, tid);
+        for (int q = 0; q < G::PP; ++q) u[q] = pfma(gp[q], f[q], u[q]);
+        x_fwd_store<T, NX, NY>(u, a.W[c == 0 ? S_FGAU : S_FGAV], y0, lds, twxs, tid);
+    }
 }
 
 // W slots after Y2
@@ -339,45 +358,52 @@ enum : int { S_P1 = 2, S_Q1 = 3, S_G1M = 4, S_G1P = 5, S_P2 = 6, S_Q2 = 7, S_G2M
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x3(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
-    C m0u[G::PP], m0v[G::PP], w[G::PP], uu[G::PP], vv[G::PP], f[G::PP];
-    XSpec<T, NX> b0, b1;
-    x_spec_load<T, NX, NY>(a.W[S_M0U], y0, b0, tid);
-    x_spec_load<T, NX, NY>(a.W[S_M0V], y0, b1, tid);
-    ph_load<T, NX, NY>(w, a.wallchar, y0, tid);
-    ph_load<T, NX, NY>(uu, a.u_p, y0, tid);
-    ph_load<T, NX, NY>(vv, a.v_p, y0, tid);
-    x_inverse_b<T, NX, NY>(b0, y0, lds, twxs, m0u, sc, nullptr, tid);
-    x_inverse_b<T, NX, NY>(b1, y0, lds, twxs, m0v, sc, nullptr, tid);
+    // u and v are read twice (for E and for the products) so that at most
+    // four pair arrays are live during a transform
+    C m0u[G::PP], m0v[G::PP], w[G::PP], x[G::PP];
+    XSpec<T, NX> b;
+    x_spec_load<T, NX, NY>(a.W[S_M0U], y0, b, tid);
+    x_inv_roll<T, NX, NY>(b, a.W[S_M0V], y0, lds, twxs, m0u, sc, nullptr, tid);
+    ph_load<T, NX, NY>(w, a.u_p, y0, tid);
+    x_inv_roll<T, NX, NY>(b, nullptr, y0, lds, twxs, m0v, sc, nullptr, tid);
     // E = 0.5 (u^2 + v^2 - Lphi^2) w, Lphi = M0u + M0v  (2d/fun_odesolver.c:3426)
+    ph_load<T, NX, NY>(x, a.v_p, y0, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) {
         C lp = cadd(m0u[q], m0v[q]);
-        f[q] = cmk<C>((T)0.5 * ((uu[q].x * uu[q].x + vv[q].x * vv[q].x) - lp.x * lp.x) * w[q].x,
-                      (T)0.5 * ((uu[q].y * uu[q].y + vv[q].y * vv[q].y) - lp.y * lp.y) * w[q].y);
+        x[q] = cmk<C>((w[q].x * w[q].x + x[q].x * x[q].x) - lp.x * lp.x, (w[q].y * w[q].y + x[q].y * x[q].y) - lp.y * lp.y);
     }
-    x_fwd_store<T, NX, NY>(f, a.W[S_E], y0, lds, twxs, tid);
-    ph_load<T, NX, NY>(f, a.eta_p, y0, tid);   // f holds eta from here on
+    ph_load<T, NX, NY>(w, a.wallchar, y0, tid);
 #pragma unroll
-    for (int q = 0; q < G::PP; ++q) { C ew = pmul(f[q], w[q]); uu[q] = pmul(ew, uu[q]); vv[q] = pmul(ew, vv[q]);
-                                      m0u[q] = pmul(ew, m0u[q]); m0v[q] = pmul(ew, m0v[q]); }
-    x_fwd_store<T, NX, NY>(uu, a.W[S_Q1], y0, lds, twxs, tid);
-    x_fwd_store<T, NX, NY>(vv, a.W[S_Q2], y0, lds, twxs, tid);
+    for (int q = 0; q < G::PP; ++q) x[q] = cmk<C>((T)0.5 * x[q].x * w[q].x, (T)0.5 * x[q].y * w[q].y);
+    x_fwd_store<T, NX, NY>(x, a.W[S_E], y0, lds, twxs, tid);
+    // w <- eta w; Q = F(eta u w), F(eta v w)
+    ph_load<T, NX, NY>(x, a.eta_p, y0, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) w[q] = pmul(x[q], w[q]);
+    ph_load<T, NX, NY>(x, a.u_p, y0, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) x[q] = pmul(w[q], x[q]);
+    x_fwd_store<T, NX, NY>(x, a.W[S_Q1], y0, lds, twxs, tid);
+    ph_load<T, NX, NY>(x, a.v_p, y0, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) x[q] = pmul(w[q], x[q]);
+    x_fwd_store<T, NX, NY>(x, a.W[S_Q2], y0, lds, twxs, tid);
+    // P = F(eta M0 w), G = F(g-+ eta M0 w)
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) { m0u[q] = pmul(w[q], m0u[q]); m0v[q] = pmul(w[q], m0v[q]); }
     x_fwd_store<T, NX, NY>(m0u, a.W[S_P1], y0, lds, twxs, tid);
     x_fwd_store<T, NX, NY>(m0v, a.W[S_P2], y0, lds, twxs, tid);
-    ph_load<T, NX, NY>(w, a.gam_m, y0, tid);
+    #pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        ph_load<T, NX, NY>(w, s == 0 ? a.gam_m : a.gam_p, y0, tid);
 #pragma unroll
-    for (int q = 0; q < G::PP; ++q) f[q] = pmul(w[q], m0u[q]);
-    x_fwd_store<T, NX, NY>(f, a.W[S_G1M], y0, lds, twxs, tid);
+        for (int q = 0; q < G::PP; ++q) x[q] = pmul(w[q], m0u[q]);
+        x_fwd_store<T, NX, NY>(x, a.W[s == 0 ? S_G1M : S_G1P], y0, lds, twxs, tid);
 #pragma unroll
-    for (int q = 0; q < G::PP; ++q) f[q] = pmul(w[q], m0v[q]);
-    x_fwd_store<T, NX, NY>(f, a.W[S_G2M], y0, lds, twxs, tid);
-    ph_load<T, NX, NY>(w, a.gam_p, y0, tid);
-#pragma unroll
-    for (int q = 0; q < G::PP; ++q) f[q] = pmul(w[q], m0u[q]);
-    x_fwd_store<T, NX, NY>(f, a.W[S_G1P], y0, lds, twxs, tid);
-#pragma unroll
-    for (int q = 0; q < G::PP; ++q) f[q] = pmul(w[q], m0v[q]);
-    x_fwd_store<T, NX, NY>(f, a.W[S_G2P], y0, lds, twxs, tid);
+        for (int q = 0; q < G::PP; ++q) x[q] = pmul(w[q], m0v[q]);
+        x_fwd_store<T, NX, NY>(x, a.W[s == 0 ? S_G2M : S_G2P], y0, lds, twxs, tid);
+    }
 }
 
 // W slots after Y4 (inputs S_P1.. are consumed)
@@ -449,63 +475,61 @@ enum : int { S_ACC1 = 2, S_ACC2 = 3, S_ZU = 4, S_ZV = 5 };
 
 
 // output transform of a velocity equation: f1 = Sb - chi A - fbdy7 vel,
-// f2 = -chi (-1)^x s(y)   (the imaginary part of fun_adjust's inverse)
+// f2 = -chi (-1)^x s(y)   (the imaginary part of fun_adjust's inverse).
+// b holds the adjust input A^; it is refilled with `next`.
 template <typename T, int NX, int NY>
-__device__ __forceinline__ void x_vel_out(const Rhs2DArgs<T>& a, const XSpec<T, NX>& b, const T* vel, const T* sb, int so,
-                                          int y0, typename ctype<T>::type* lds, const typename ctype<T>::type* twl, T* nyq,
-                                          int tid) {
+__device__ __forceinline__ void x_vel_out(const Rhs2DArgs<T>& a, XSpec<T, NX>& b, const typename ctype<T>::type* next,
+                                          const T* vel, const T* sb, int so, int y0, typename ctype<T>::type* lds,
+                                          const typename ctype<T>::type* twl, T* nyq, int tid) {
     using C = typename ctype<T>::type;
     using G = XGeo<T, NX>;
     const T sc = (T)1 / ((T)NX * (T)NY);
-    C f1[G::PP], f2[G::PP], tc[G::PP], tf[G::PP], v[G::PP], tb[G::PP];
+    C f1[G::PP], tc[G::PP], t[G::PP];
+    x_inv_roll<T, NX, NY>(b, next, y0, lds, twl, f1, sc, nyq, tid);
     ph_load<T, NX, NY>(tc, a.chi, y0, tid);
-    ph_load<T, NX, NY>(tf, a.fbdy7, y0, tid);
-    ph_load<T, NX, NY>(v, vel, y0, tid);
-    if (sb) ph_load<T, NX, NY>(tb, sb, y0, tid);
-    x_inverse_b<T, NX, NY>(b, y0, lds, twl, f1, sc, nyq, tid);
 #pragma unroll
-    for (int q = 0; q < G::PP; ++q) {
-        int pos = G::ppos(tid, q);
-        int r = pos / G::L;
-        T s = nyq[r < G::RR ? r : 0];
-        f1[q] = csub(cmk<C>(-tc[q].x * f1[q].x, -tc[q].y * f1[q].y), pmul(tf[q], v[q]));
-        if (sb) f1[q] = cadd(f1[q], tb[q]);
-        f2[q] = cmk<C>(-tc[q].x * s, tc[q].y * s);
+    for (int q = 0; q < G::PP; ++q) f1[q] = cmk<C>(-tc[q].x * f1[q].x, -tc[q].y * f1[q].y);
+    if (sb) {
+        ph_load<T, NX, NY>(t, sb, y0, tid);
+#pragma unroll
+        for (int q = 0; q < G::PP; ++q) f1[q] = cadd(f1[q], t[q]);
     }
-    x_fwd2_store<T, NX, NY>(f1, f2, a.W[so], y0, lds, twl, tid);
+    {
+        C v[G::PP];
+        ph_load<T, NX, NY>(t, a.fbdy7, y0, tid);
+        ph_load<T, NX, NY>(v, vel, y0, tid);
+#pragma unroll
+        for (int q = 0; q < G::PP; ++q) f1[q] = csub(f1[q], pmul(t[q], v[q]));
+    }
+    x_fwd_adj_store<T, NX, NY>(f1, tc, (T)-1, nyq, a.W[so], y0, lds, twl, tid);
 }
 
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x5(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
     // F(g- b- + g+ b+) of both HSS tails, then the two velocity output
-    // transforms; the next field's loads are issued before each transform
+    // transforms; one spectral buffer rolls through the six inputs
     C f[G::PP], acc[G::PP], gm[G::PP], gp[G::PP];
-    XSpec<T, NX> b0, b1;
-    x_spec_load<T, NX, NY>(a.W[S_B1M], y0, b0, tid);
-    x_spec_load<T, NX, NY>(a.W[S_B1P], y0, b1, tid);
+    XSpec<T, NX> b;
+    x_spec_load<T, NX, NY>(a.W[S_B1M], y0, b, tid);
     ph_load<T, NX, NY>(gm, a.gam_m, y0, tid);
     ph_load<T, NX, NY>(gp, a.gam_p, y0, tid);
-    x_inverse_b<T, NX, NY>(b0, y0, lds, twxs, f, sc, nullptr, tid);
-    x_spec_load<T, NX, NY>(a.W[S_B2M], y0, b0, tid);
+    x_inv_roll<T, NX, NY>(b, a.W[S_B1P], y0, lds, twxs, f, sc, nullptr, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) acc[q] = pmul(gm[q], f[q]);
-    x_inverse_b<T, NX, NY>(b1, y0, lds, twxs, f, sc, nullptr, tid);
-    x_spec_load<T, NX, NY>(a.W[S_B2P], y0, b1, tid);
+    x_inv_roll<T, NX, NY>(b, a.W[S_B2M], y0, lds, twxs, f, sc, nullptr, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) acc[q] = pfma(gp[q], f[q], acc[q]);
     x_fwd_store<T, NX, NY>(acc, a.W[S_ACC1], y0, lds, twxs, tid);
-    x_inverse_b<T, NX, NY>(b0, y0, lds, twxs, f, sc, nullptr, tid);
-    x_spec_load<T, NX, NY>(a.W[S_AX], y0, b0, tid);
+    x_inv_roll<T, NX, NY>(b, a.W[S_B2P], y0, lds, twxs, f, sc, nullptr, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) acc[q] = pmul(gm[q], f[q]);
-    x_inverse_b<T, NX, NY>(b1, y0, lds, twxs, f, sc, nullptr, tid);
-    x_spec_load<T, NX, NY>(a.W[S_AY], y0, b1, tid);
+    x_inv_roll<T, NX, NY>(b, a.W[S_AX], y0, lds, twxs, f, sc, nullptr, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) acc[q] = pfma(gp[q], f[q], acc[q]);
     x_fwd_store<T, NX, NY>(acc, a.W[S_ACC2], y0, lds, twxs, tid);
-    x_vel_out<T, NX, NY>(a, b0, a.u_p, a.sbx, S_ZU, y0, lds, twxs, nyq, tid);
-    x_vel_out<T, NX, NY>(a, b1, a.v_p, a.sby, S_ZV, y0, lds, twxs, nyq, tid);
+    x_vel_out<T, NX, NY>(a, b, a.W[S_AY], a.u_p, a.sbx, S_ZU, y0, lds, twxs, nyq, tid);
+    x_vel_out<T, NX, NY>(a, b, nullptr, a.v_p, a.sby, S_ZV, y0, lds, twxs, nyq, tid);
 }
 
 // W slots after Y6
@@ -575,41 +599,33 @@ __device__ __forceinline__ void src_pair(const Rhs2DArgs<T>& a, int y0, int pos,
 }
 
 // output transform of the eta equation: f1 = S(t) - fbdy7 eta + chi A,
-// f2 = chi (-1)^x s(y)
+// f2 = chi (-1)^x s(y).  b holds A^; it is refilled with `next`.
 template <typename T, int NX, int NY>
-__device__ __forceinline__ void x_eta_out(const Rhs2DArgs<T>& a, const XSpec<T, NX>& b, int so, int y0,
-                                          typename ctype<T>::type* lds, const typename ctype<T>::type* twl, T* nyq, int tid) {
+__device__ __forceinline__ void x_eta_out(const Rhs2DArgs<T>& a, XSpec<T, NX>& b, const typename ctype<T>::type* next,
+                                          int so, int y0, typename ctype<T>::type* lds,
+                                          const typename ctype<T>::type* twl, T* nyq, int tid) {
     using C = typename ctype<T>::type;
     using G = XGeo<T, NX>;
     const T sc = (T)1 / ((T)NX * (T)NY);
-    C f1[G::PP], f2[G::PP], tc[G::PP], tf[G::PP], te[G::PP], tn[G::PP], ts[G::PP];
+    C f1[G::PP], tc[G::PP], t[G::PP], e[G::PP];
+    x_inv_roll<T, NX, NY>(b, next, y0, lds, twl, f1, sc, nyq, tid);
     ph_load<T, NX, NY>(tc, a.chi, y0, tid);
-    x_inverse_b<T, NX, NY>(b, y0, lds, twl, f1, sc, nyq, tid);
-    ph_load<T, NX, NY>(tf, a.fbdy7, y0, tid);
-    ph_load<T, NX, NY>(te, a.eta_p, y0, tid);
-    if (a.src_on) {
-        ph_load<T, NX, NY>(tn, a.new_gam, y0, tid);
-        ph_load<T, NX, NY>(ts, a.skew_gam, y0, tid);
-    }
+    ph_load<T, NX, NY>(t, a.fbdy7, y0, tid);
+    ph_load<T, NX, NY>(e, a.eta_p, y0, tid);
     if (a.s_eta_out && tid < G::RR) a.s_eta_out[y0 + tid] = nyq[tid];
 #pragma unroll
-    for (int q = 0; q < G::PP; ++q) {
-        int pos = G::ppos(tid, q);
-        int r = pos / G::L;
-        T s = nyq[r < G::RR ? r : 0];
-        f1[q] = csub(pmul(tc[q], f1[q]), pmul(tf[q], te[q]));
-        f2[q] = cmk<C>(tc[q].x * s, -tc[q].y * s);
-    }
+    for (int q = 0; q < G::PP; ++q) f1[q] = csub(pmul(tc[q], f1[q]), pmul(t[q], e[q]));
     if (a.src_on) {
+        ph_load<T, NX, NY>(t, a.new_gam, y0, tid);
+        ph_load<T, NX, NY>(e, a.skew_gam, y0, tid);
 #pragma unroll
         for (int q = 0; q < G::PP; ++q) {
-            int pos = G::ppos(tid, q);
             T s0, s1;
-            src_pair<T, NX>(a, y0, pos, tn[q], ts[q], s0, s1);
+            src_pair<T, NX>(a, y0, G::ppos(tid, q), t[q], e[q], s0, s1);
             f1[q] = cadd(f1[q], cmk<C>(s0, s1));
         }
     }
-    x_fwd2_store<T, NX, NY>(f1, f2, a.W[so], y0, lds, twl, tid);
+    x_fwd_adj_store<T, NX, NY>(f1, tc, (T)1, nyq, a.W[so], y0, lds, twl, tid);
 }
 
 template <typename T, int NX, int NY>
@@ -617,7 +633,7 @@ __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x7(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
     XSpec<T, NX> b;
     x_spec_load<T, NX, NY>(a.W[S_AETA], y0, b, tid);
-    x_eta_out<T, NX, NY>(a, b, S_ZU, y0, lds, twxs, nyq, tid);
+    x_eta_out<T, NX, NY>(a, b, nullptr, S_ZU, y0, lds, twxs, nyq, tid);
 }
 
 template <typename T, int NX, int NY>
@@ -675,35 +691,37 @@ enum : int { F_P1 = 5, F_Q1 = 6, F_P2 = 7, F_Q2 = 8, F_E = 9 };
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2f_x2(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
-    C e[G::PP], w[G::PP], u[G::PP], lp[G::PP], m[G::PP], f[G::PP];
-    x_inverse<T, NX, NY>(a.W[F_ETA], y0, lds, twxs, e, sc, nullptr, tid);
-    ph_store<T, NX, NY>(a.eta_p, e, y0, tid);
-    ph_load<T, NX, NY>(w, a.wallchar, y0, tid);
-    x_inverse<T, NX, NY>(a.W[F_U], y0, lds, twxs, u, sc, nullptr, tid);
-    ph_store<T, NX, NY>(a.u_p, u, y0, tid);
+    C ew[G::PP], m[G::PP], q2[G::PP], lp[G::PP];
+    XSpec<T, NX> b;
+    x_spec_load<T, NX, NY>(a.W[F_ETA], y0, b, tid);
+    x_inv_roll<T, NX, NY>(b, a.W[F_U], y0, lds, twxs, ew, sc, nullptr, tid);
+    ph_store<T, NX, NY>(a.eta_p, ew, y0, tid);
+    ph_load<T, NX, NY>(m, a.wallchar, y0, tid);
 #pragma unroll
-    for (int q = 0; q < G::PP; ++q) f[q] = pmul(pmul(e[q], u[q]), w[q]);
-    x_fwd_store<T, NX, NY>(f, a.W[F_Q1], y0, lds, twxs, tid);
-    x_inverse<T, NX, NY>(a.W[F_V], y0, lds, twxs, m, sc, nullptr, tid);
+    for (int q = 0; q < G::PP; ++q) ew[q] = pmul(ew[q], m[q]);   // eta w
+    x_inv_roll<T, NX, NY>(b, a.W[F_V], y0, lds, twxs, m, sc, nullptr, tid);
+    ph_store<T, NX, NY>(a.u_p, m, y0, tid);
+#pragma unroll
+    for (int q = 0; q < G::PP; ++q) { q2[q] = pmul(m[q], m[q]); m[q] = pmul(ew[q], m[q]); }
+    x_fwd_store<T, NX, NY>(m, a.W[F_Q1], y0, lds, twxs, tid);
+    x_inv_roll<T, NX, NY>(b, a.W[F_M0U], y0, lds, twxs, m, sc, nullptr, tid);
     ph_store<T, NX, NY>(a.v_p, m, y0, tid);
 #pragma unroll
-    for (int q = 0; q < G::PP; ++q) {
-        f[q] = pmul(pmul(e[q], m[q]), w[q]);
-        u[q] = cmk<C>(u[q].x * u[q].x + m[q].x * m[q].x, u[q].y * u[q].y + m[q].y * m[q].y);   // u^2 + v^2
-    }
-    x_fwd_store<T, NX, NY>(f, a.W[F_Q2], y0, lds, twxs, tid);
-    x_inverse<T, NX, NY>(a.W[F_M0U], y0, lds, twxs, m, sc, nullptr, tid);
+    for (int q = 0; q < G::PP; ++q) { q2[q] = pfma(m[q], m[q], q2[q]); m[q] = pmul(ew[q], m[q]); }   // u^2 + v^2
+    x_fwd_store<T, NX, NY>(m, a.W[F_Q2], y0, lds, twxs, tid);
+    x_inv_roll<T, NX, NY>(b, a.W[F_M0V], y0, lds, twxs, lp, sc, nullptr, tid);
 #pragma unroll
-    for (int q = 0; q < G::PP; ++q) { f[q] = pmul(pmul(e[q], m[q]), w[q]); lp[q] = m[q]; }
-    x_fwd_store<T, NX, NY>(f, a.W[F_P1], y0, lds, twxs, tid);
-    x_inverse<T, NX, NY>(a.W[F_M0V], y0, lds, twxs, m, sc, nullptr, tid);
+    for (int q = 0; q < G::PP; ++q) m[q] = pmul(ew[q], lp[q]);
+    x_fwd_store<T, NX, NY>(m, a.W[F_P1], y0, lds, twxs, tid);
+    x_inv_roll<T, NX, NY>(b, nullptr, y0, lds, twxs, m, sc, nullptr, tid);
 #pragma unroll
-    for (int q = 0; q < G::PP; ++q) { f[q] = pmul(pmul(e[q], m[q]), w[q]); lp[q] = cadd(lp[q], m[q]); }
-    x_fwd_store<T, NX, NY>(f, a.W[F_P2], y0, lds, twxs, tid);
+    for (int q = 0; q < G::PP; ++q) { lp[q] = cadd(lp[q], m[q]); m[q] = pmul(ew[q], m[q]); }
+    x_fwd_store<T, NX, NY>(m, a.W[F_P2], y0, lds, twxs, tid);
+    ph_load<T, NX, NY>(m, a.wallchar, y0, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q)
-        f[q] = cmk<C>((T)0.5 * (u[q].x - lp[q].x * lp[q].x) * w[q].x, (T)0.5 * (u[q].y - lp[q].y * lp[q].y) * w[q].y);
-    x_fwd_store<T, NX, NY>(f, a.W[F_E], y0, lds, twxs, tid);
+        m[q] = cmk<C>((T)0.5 * (q2[q].x - lp[q].x * lp[q].x) * m[q].x, (T)0.5 * (q2[q].y - lp[q].y * lp[q].y) * m[q].y);
+    x_fwd_store<T, NX, NY>(m, a.W[F_E], y0, lds, twxs, tid);
 }
 
 enum : int { F_AETA = 0, F_AX = 1, F_AY = 2 };
@@ -756,13 +774,11 @@ enum : int { F_ZETA = 3, F_ZU = 4, F_ZV = 5 };
 template <typename T, int NX, int NY>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2f_x4(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
-    XSpec<T, NX> b0, b1, b2;
-    x_spec_load<T, NX, NY>(a.W[F_AETA], y0, b0, tid);
-    x_spec_load<T, NX, NY>(a.W[F_AX], y0, b1, tid);
-    x_spec_load<T, NX, NY>(a.W[F_AY], y0, b2, tid);
-    x_eta_out<T, NX, NY>(a, b0, F_ZETA, y0, lds, twxs, nyq, tid);
-    x_vel_out<T, NX, NY>(a, b1, a.u_p, a.sbx, F_ZU, y0, lds, twxs, nyq, tid);
-    x_vel_out<T, NX, NY>(a, b2, a.v_p, a.sby, F_ZV, y0, lds, twxs, nyq, tid);
+    XSpec<T, NX> b;
+    x_spec_load<T, NX, NY>(a.W[F_AETA], y0, b, tid);
+    x_eta_out<T, NX, NY>(a, b, a.W[F_AX], F_ZETA, y0, lds, twxs, nyq, tid);
+    x_vel_out<T, NX, NY>(a, b, a.W[F_AY], a.u_p, a.sbx, F_ZU, y0, lds, twxs, nyq, tid);
+    x_vel_out<T, NX, NY>(a, b, nullptr, a.v_p, a.sby, F_ZV, y0, lds, twxs, nyq, tid);
 }
 
 template <typename T, int NX, int NY>
