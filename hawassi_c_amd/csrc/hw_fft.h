@@ -445,18 +445,17 @@ __device__ __forceinline__ void x_spec_load(const typename ctype<T>::type* __res
 }
 
 template <typename T, int NX, int NY>
+__device__ __forceinline__ void x_store_slot(typename ctype<T>::type* __restrict__ X, int y0, int q,
+                                             typename ctype<T>::type v, int tid);
+
+template <typename T, int NX, int NY>
 __device__ __forceinline__ void x_store_spec(typename ctype<T>::type* __restrict__ X, int y0, const XSpec<T, NX>& b,
                                              int tid) {
     using G = XGeo<T, NX>;
 #pragma unroll
     for (int q = 0; q < G::PS; ++q) {
         int idx = tid + q * G::NT;
-        if (G::NSPEC % G::NT == 0 || idx < G::NSPEC) {
-            int kx = idx / G::RR, r = idx % G::RR;
-            typename ctype<T>::type* p = X + (size_t)kx * NY + y0 + r;
-            __builtin_nontemporal_store(b[q].x, &p->x);   // streaming: written once, read by the next kernel
-            __builtin_nontemporal_store(b[q].y, &p->y);
-        }
+        if (G::NSPEC % G::NT == 0 || idx < G::NSPEC) x_store_slot<T, NX, NY>(X, y0, q, b[q], tid);   // streaming
     }
 }
 
@@ -542,14 +541,18 @@ __device__ __forceinline__ void x_inverse(const typename ctype<T>::type* __restr
 }
 
 // Forward x-pass of a real field given as pairs in registers (XGeo::ppos
-// layout) -> this thread's spectral values X[k], k = 0..L (x_spec_load layout).
+// layout).  x_fwd_to_lds runs the transform and leaves Z (the half-length
+// complex spectrum of every row) in LDS; x_post forms the real-field spectrum
+// X[k] of one spectral slot from it, so callers can store slot by slot
+// without holding the whole spectral array in registers.
 template <typename T, int NX>
-__device__ __forceinline__ void x_forward_regs(const carr<T, XGeo<T, NX>::PP>& f, XSpec<T, NX>& bs,
-                                               typename ctype<T>::type* lds,
-                                               const typename ctype<T>::type* __restrict__ twx, int tid) {
+__device__ __forceinline__ const typename ctype<T>::type* x_fwd_to_lds(const carr<T, XGeo<T, NX>::PP>& f,
+                                                                      typename ctype<T>::type* lds,
+                                                                      const typename ctype<T>::type* __restrict__ twx,
+                                                                      int tid) {
     using G = XGeo<T, NX>;
     using C = typename ctype<T>::type;
-    constexpr int L = G::L, RR = G::RR, LSb = G::LSb, NT = G::NT, E = G::E, SK = G::SK;
+    constexpr int L = G::L, RR = G::RR, LSb = G::LSb, NT = G::NT, E = G::E;
     C* S = lds;
     C* A = G::TWO ? lds + RR * LSb : lds;
     C v[E];
@@ -575,31 +578,63 @@ __device__ __forceinline__ void x_forward_regs(const carr<T, XGeo<T, NX>::PP>& f
         }
     }
     __syncthreads();
-    // X[k] = (Z[k] + conj Z[L-k])/2 - i/2 W^k (Z[k] - conj Z[L-k])
-    const int kx0 = tid / RR, rr = tid % RR;
-    const C* row = O + rr * LSb;
-#pragma unroll
-    for (int q = 0; q < G::PS; ++q) {
-        const int kx = kx0 + q * SK;
-        if (G::NSPEC % NT == 0 || kx <= L) {
-            if (kx == 0 || kx == L) {
-                const C z = row[0];
-                bs[q] = kx == 0 ? cmk<C>(z.x + z.y, (T)0) : cmk<C>(z.x - z.y, (T)0);
-            } else {
-                C za, zb;
-                if constexpr (SK % PADQ == 0) {
-                    za = row[pidx(kx0) + pidx(q * SK)];
-                    zb = row[pidx(SK - kx0) + pidx(L - (q + 1) * SK)];
-                } else {
-                    za = row[pidx(kx)];
-                    zb = row[pidx(L - kx)];
-                }
-                const C ea = cscale(cadd(za, cconj(zb)), (T)0.5);
-                const C da = cscale(csub(za, cconj(zb)), (T)0.5);
-                bs[q] = cadd(ea, cmul_i<-1>(cmul(twx[kx], da)));
-            }
-        }
+    return O;
+}
+
+// X[k] = (Z[k] + conj Z[L-k])/2 - i/2 W^k (Z[k] - conj Z[L-k]) for spectral
+// slot q of this thread (kx = tid / RR + q * NT / RR, row tid % RR); the
+// caller checks kx <= L.
+template <typename T, int NX>
+__device__ __forceinline__ typename ctype<T>::type x_post(const typename ctype<T>::type* O,
+                                                          const typename ctype<T>::type* __restrict__ twx, int q, int tid) {
+    using G = XGeo<T, NX>;
+    using C = typename ctype<T>::type;
+    constexpr int L = G::L, RR = G::RR, LSb = G::LSb, SK = G::SK;
+    const int kx0 = tid / RR, kx = kx0 + q * SK;
+    const C* row = O + (tid % RR) * LSb;
+    if (kx == 0 || kx == L) {
+        const C z = row[0];
+        return kx == 0 ? cmk<C>(z.x + z.y, (T)0) : cmk<C>(z.x - z.y, (T)0);
     }
+    C za, zb;
+    if constexpr (SK % PADQ == 0) {
+        za = row[pidx(kx0) + pidx(q * SK)];
+        zb = row[pidx(SK - kx0) + pidx(L - (q + 1) * SK)];
+    } else {
+        za = row[pidx(kx)];
+        zb = row[pidx(L - kx)];
+    }
+    const C ea = cscale(cadd(za, cconj(zb)), (T)0.5);
+    const C da = cscale(csub(za, cconj(zb)), (T)0.5);
+    return cadd(ea, cmul_i<-1>(cmul(twx[kx], da)));
+}
+
+// spectral slot q of this thread holds a value (kx <= L)
+template <typename T, int NX> __device__ __forceinline__ bool x_slot_ok(int q, int tid) {
+    using G = XGeo<T, NX>;
+    return G::NSPEC % G::NT == 0 || tid + q * G::NT < G::NSPEC;
+}
+
+template <typename T, int NX>
+__device__ __forceinline__ void x_forward_regs(const carr<T, XGeo<T, NX>::PP>& f, XSpec<T, NX>& bs,
+                                               typename ctype<T>::type* lds,
+                                               const typename ctype<T>::type* __restrict__ twx, int tid) {
+    using G = XGeo<T, NX>;
+    const auto* O = x_fwd_to_lds<T, NX>(f, lds, twx, tid);
+#pragma unroll
+    for (int q = 0; q < G::PS; ++q)
+        if (x_slot_ok<T, NX>(q, tid)) bs[q] = x_post<T, NX>(O, twx, q, tid);
+}
+
+// streaming store of one spectral value of slot q to X[kx][y0 + row]
+template <typename T, int NX, int NY>
+__device__ __forceinline__ void x_store_slot(typename ctype<T>::type* __restrict__ X, int y0, int q,
+                                             typename ctype<T>::type v, int tid) {
+    using G = XGeo<T, NX>;
+    const int idx = tid + q * G::NT;
+    typename ctype<T>::type* p = X + (size_t)(idx / G::RR) * NY + y0 + idx % G::RR;
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
 }
 
 // ----------------------------------------------------------- y-pass ops ---

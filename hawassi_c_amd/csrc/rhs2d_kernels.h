@@ -95,9 +95,11 @@ template <typename C> __device__ __forceinline__ C pfma(C a, C b, C c) { return 
 template <typename T, int NX, int NY>
 __device__ __forceinline__ void x_fwd_store(const carr<T, XGeo<T, NX>::PP>& f, typename ctype<T>::type* X,
                                             int y0, typename ctype<T>::type* lds, const typename ctype<T>::type* twx, int tid) {
-    XSpec<T, NX> s;
-    x_forward_regs<T, NX>(f, s, lds, twx, tid);
-    x_store_spec<T, NX, NY>(X, y0, s, tid);
+    using G = XGeo<T, NX>;
+    const auto* O = x_fwd_to_lds<T, NX>(f, lds, twx, tid);
+#pragma unroll
+    for (int q = 0; q < G::PS; ++q)
+        if (x_slot_ok<T, NX>(q, tid)) x_store_slot<T, NX, NY>(X, y0, q, x_post<T, NX>(O, twx, q, tid), tid);
 }
 
 // forward x of f1 + i f2 (both real): X = F(f1) + i F(f2)
@@ -108,10 +110,11 @@ __device__ __forceinline__ void x_fwd2_store(const carr<T, XGeo<T, NX>::PP>& f1,
     using G = XGeo<T, NX>;
     XSpec<T, NX> s1, s2;
     x_forward_regs<T, NX>(f1, s1, lds, twx, tid);
-    x_forward_regs<T, NX>(f2, s2, lds, twx, tid);
+    const auto* O = x_fwd_to_lds<T, NX>(f2, lds, twx, tid);
 #pragma unroll
-    for (int q = 0; q < G::PS; ++q) s1[q] = cadd(s1[q], cmul_i<1>(s2[q]));
-    x_store_spec<T, NX, NY>(X, y0, s1, tid);
+    for (int q = 0; q < G::PS; ++q)
+        if (x_slot_ok<T, NX>(q, tid))
+            x_store_slot<T, NX, NY>(X, y0, q, cadd(s1[q], cmul_i<1>(x_post<T, NX>(O, twx, q, tid))), tid);
 }
 
 // forward x of f1 + i f2 where f2 = c (-1)^x s(y) (c real per point, s per
@@ -123,7 +126,7 @@ __device__ __forceinline__ void x_fwd_adj_store(const carr<T, XGeo<T, NX>::PP>& 
                                                 typename ctype<T>::type* lds, const typename ctype<T>::type* twx, int tid) {
     using G = XGeo<T, NX>;
     using C = typename ctype<T>::type;
-    XSpec<T, NX> s1, s2;
+    XSpec<T, NX> s1;
     x_forward_regs<T, NX>(f1, s1, lds, twx, tid);
     C f2[G::PP];
 #pragma unroll
@@ -132,10 +135,11 @@ __device__ __forceinline__ void x_fwd_adj_store(const carr<T, XGeo<T, NX>::PP>& 
         const T sv = sgn * nyq[r < G::RR ? r : 0];
         f2[q] = cmk<C>(c[q].x * sv, -c[q].y * sv);
     }
-    x_forward_regs<T, NX>(f2, s2, lds, twx, tid);
+    const auto* O = x_fwd_to_lds<T, NX>(f2, lds, twx, tid);
 #pragma unroll
-    for (int q = 0; q < G::PS; ++q) s1[q] = cadd(s1[q], cmul_i<1>(s2[q]));
-    x_store_spec<T, NX, NY>(X, y0, s1, tid);
+    for (int q = 0; q < G::PS; ++q)
+        if (x_slot_ok<T, NX>(q, tid))
+            x_store_slot<T, NX, NY>(X, y0, q, cadd(s1[q], cmul_i<1>(x_post<T, NX>(O, twx, q, tid))), tid);
 }
 
 // Inverse x of the field held in b; b is refilled with `next` (when set)
@@ -287,13 +291,16 @@ __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x1(Rhs2DArgs<T> a) {
     for (int c = 0; c < 2; ++c) {
         const int s_am = c == 0 ? S_AMU : S_AMV, s_ap = c == 0 ? S_APU : S_APV;
         // velocity: store it, F(g- vel), F(g+ vel)
-        x_inv_roll<T, NX, NY>(b, a.W[s_am], y0, lds, twxs, u, sc, nullptr, tid);
+        // (the next input is loaded after the two forward transforms: holding
+        // it through them would exceed the register budget)
+        x_inv_roll<T, NX, NY>(b, nullptr, y0, lds, twxs, u, sc, nullptr, tid);
         ph_store<T, NX, NY>(c == 0 ? a.u_p : a.v_p, u, y0, tid);
 #pragma unroll
         for (int q = 0; q < G::PP; ++q) f[q] = pmul(gm[q], u[q]);
         x_fwd_store<T, NX, NY>(f, a.W[c == 0 ? S_FGMU : S_FGMV], y0, lds, twxs, tid);
 #pragma unroll
         for (int q = 0; q < G::PP; ++q) f[q] = pmul(gp[q], u[q]);
+        x_spec_load<T, NX, NY>(a.W[s_am], y0, b, tid);
         x_fwd_store<T, NX, NY>(f, a.W[c == 0 ? S_FGPU : S_FGPV], y0, lds, twxs, tid);
         // F(g- F^-1(C2- vel^) + g+ F^-1(C2+ vel^)), accumulated in u
         x_inv_roll<T, NX, NY>(b, a.W[s_ap], y0, lds, twxs, f, sc, nullptr, tid);
