@@ -402,6 +402,9 @@ template <typename T, int NY> struct YGeo {
     static constexpr bool TWO = (2 * CC * LS + NY) * (int)sizeof(C) <= HW_Y_LDS_BUDGET;
     static constexpr int LDS_ELEMS = (TWO ? 2 : 1) * CC * LS;
     static constexpr bool TWL = (LDS_ELEMS + NY) * (int)sizeof(C) <= HW_Y_LDS_BUDGET;
+    // waves per SIMD the y kernels are compiled for: FP32 blocks fit three
+    // to a CU by LDS, so registers should not be the tighter limit
+    static constexpr int WPE = sizeof(T) == 4 ? 3 : 1;
     // element e of thread tid -> column * NY + ky within the block
     __device__ static __forceinline__ int eidx(int tid, int e) {
         int b, n;

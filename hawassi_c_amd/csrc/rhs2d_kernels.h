@@ -239,7 +239,7 @@ enum : int {
 };
 
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y1(Rhs2DArgs<T> a) {
+__global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves_per_eu(YGeo<T, NY>::WPE))) k2b_y1(Rhs2DArgs<T> a) {
     HW_YK_PROLOGUE
     // all inputs and multipliers are loaded up front; the transforms follow
     C e0[G::PE], uu[G::PE], vv[G::PE], w[G::PE];
@@ -417,52 +417,71 @@ __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x3(Rhs2DArgs<T> a) {
 enum : int { S_B1M = 11, S_B1P = 12, S_B2M = 13, S_B2P = 14, S_AX = 0, S_AY = 1 };
 
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y4(Rhs2DArgs<T> a) {
+__global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves_per_eu(YGeo<T, NY>::WPE))) k2b_y4(Rhs2DArgs<T> a) {
     HW_YK_PROLOGUE
-    C P[G::PE], Q[G::PE], Gm[G::PE], Gp[G::PE], part[G::PE], w[G::PE];
+    // Every field's loads are issued one transform ahead of its use (the
+    // arrays rotate roles).
+    C A[G::PE], B[G::PE], D[G::PE], F[G::PE], part[G::PE];
     const T ig = (T)1 / a.g;
-    const int sl[2][4] = {{S_P1, S_Q1, S_G1M, S_G1P}, {S_P2, S_Q2, S_G2M, S_G2P}};
-    const int so[2][2] = {{S_B1M, S_B1P}, {S_B2M, S_B2P}};
-    #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-        y_load<T, NX, NY>(P, a.W[sl[c][0]], nullptr, col0, tid);
-        y_fwd_regs<T, NX, NY>(P, lds, twys, tid);
-        y_load<T, NX, NY>(Q, a.W[sl[c][1]], nullptr, col0, tid);
-        y_fwd_regs<T, NX, NY>(Q, lds, twys, tid);
-        y_load<T, NX, NY>(Gm, a.W[sl[c][2]], nullptr, col0, tid);
-        y_fwd_regs<T, NX, NY>(Gm, lds, twys, tid);
-        y_load<T, NX, NY>(Gp, a.W[sl[c][3]], nullptr, col0, tid);
-        y_fwd_regs<T, NX, NY>(Gp, lds, twys, tid);
+    // Om2-+ at element e (read where used: holding them costs more registers than the cached re-reads)
+    auto om = [&](int s, int e) -> T {
+        int kx, ky; size_t gi;
+        YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
+        return kx < NX / 2 + 1 ? (s == 0 ? a.om2_m[gi] : a.om2_p[gi]) : (T)0;
+    };
+    // -i k (eta u)^ - (Om2- F(g- p) + Om2+ F(g+ p)) / (2 g), k = kx (c = 0) or ky (c = 1)
+    auto combine = [&](int c, const C* Qv, const C* Gm, const C* Gp) {
 #pragma unroll
         for (int e = 0; e < G::PE; ++e) {
             int kx, ky; size_t gi;
             YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
             if (kx < NX / 2 + 1) {
                 T kk = c == 0 ? a.kxh[kx] : a.ky[ky];
-                // -i k (eta u)^ - (Om2- F(g- p) + Om2+ F(g+ p)) / (2 g)
-                C t1 = cmul_i<-1>(cscale(Q[e], kk));
-                C t2 = cscale(cadd(cscale(Gm[e], a.om2_m[gi]), cscale(Gp[e], a.om2_p[gi])), (T)0.5 * ig);
+                C t1 = cmul_i<-1>(cscale(Qv[e], kk));
+                C t2 = cscale(cadd(cscale(Gm[e], om(0, e)), cscale(Gp[e], om(1, e))), (T)0.5 * ig);
                 C d = csub(t1, t2);
                 part[e] = c == 0 ? d : cadd(part[e], d);
+            } else if (c == 0) {
+                part[e] = cmk<C>((T)0, (T)0);
             }
         }
-        #pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const T* mul = s == 0 ? a.om2_m : a.om2_p;
+    };
+    // inverse y of Om2-+ F(eta M0 w): w uses the scratch array
+    auto om_inverses = [&](const C* Pv, C* w, int s_m, int s_p) {
 #pragma unroll
-            for (int e = 0; e < G::PE; ++e) {
-                int kx, ky; size_t gi;
-                YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
-                w[e] = (kx < NX / 2 + 1) ? cscale(P[e], mul[gi]) : cmk<C>((T)0, (T)0);
-            }
-            y_inv_regs<T, NX, NY>(w, lds, twys, col0, tid);
-            y_store<T, NX, NY>(a.W[so[c][s]], w, col0, tid);
+        for (int s = 0; s < 2; ++s) {
+#pragma unroll
+            for (int e = 0; e < G::PE; ++e) w[e] = cscale(Pv[e], om(s, e));
+            y_inv_regs<T, NX, NY>(*reinterpret_cast<C(*)[G::PE]>(w), lds, twys, col0, tid);
+            y_store<T, NX, NY>(a.W[s == 0 ? s_m : s_p], *reinterpret_cast<C(*)[G::PE]>(w), col0, tid);
         }
-    }
+    };
+    // x momentum terms: A = P1, B = Q1, D = G1M, F = G1P
+    y_load<T, NX, NY>(A, a.W[S_P1], nullptr, col0, tid);
+    y_load<T, NX, NY>(B, a.W[S_Q1], nullptr, col0, tid);
+    y_fwd_regs<T, NX, NY>(A, lds, twys, tid);
+    y_load<T, NX, NY>(D, a.W[S_G1M], nullptr, col0, tid);
+    y_fwd_regs<T, NX, NY>(B, lds, twys, tid);
+    y_load<T, NX, NY>(F, a.W[S_G1P], nullptr, col0, tid);
+    y_fwd_regs<T, NX, NY>(D, lds, twys, tid);
+    y_fwd_regs<T, NX, NY>(F, lds, twys, tid);
+    combine(0, B, D, F);
+    y_load<T, NX, NY>(B, a.W[S_Q2], nullptr, col0, tid);
+    y_load<T, NX, NY>(D, a.W[S_P2], nullptr, col0, tid);
+    om_inverses(A, F, S_B1M, S_B1P);
+    // y momentum terms: D = P2, B = Q2, A = G2M, F = G2P
+    y_fwd_regs<T, NX, NY>(D, lds, twys, tid);
+    y_load<T, NX, NY>(A, a.W[S_G2M], nullptr, col0, tid);
+    y_fwd_regs<T, NX, NY>(B, lds, twys, tid);
+    y_load<T, NX, NY>(F, a.W[S_G2P], nullptr, col0, tid);
+    y_fwd_regs<T, NX, NY>(A, lds, twys, tid);
+    y_fwd_regs<T, NX, NY>(F, lds, twys, tid);
+    combine(1, B, A, F);
+    y_load<T, NX, NY>(B, a.W[S_E], nullptr, col0, tid);
+    om_inverses(D, F, S_B2M, S_B2P);
     y_store<T, NX, NY>(a.dH2p, part, col0, tid);
     // adjust inputs i kx E^, i ky E^
-    y_load<T, NX, NY>(P, a.W[S_E], nullptr, col0, tid);
-    y_fwd_regs<T, NX, NY>(P, lds, twys, tid);
+    y_fwd_regs<T, NX, NY>(B, lds, twys, tid);
     #pragma unroll
     for (int s = 0; s < 2; ++s) {
 #pragma unroll
@@ -470,10 +489,10 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y4(Rhs2DArgs<T> a) {
             int kx, ky; size_t gi;
             YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
             T kk = (kx < NX / 2 + 1) ? (s == 0 ? a.kxh[kx] : a.ky[ky]) : (T)0;
-            w[e] = cmul_i<1>(cscale(P[e], kk));
+            F[e] = cmul_i<1>(cscale(B[e], kk));
         }
-        y_inv_regs<T, NX, NY>(w, lds, twys, col0, tid);
-        y_store<T, NX, NY>(a.W[s == 0 ? S_AX : S_AY], w, col0, tid);
+        y_inv_regs<T, NX, NY>(F, lds, twys, col0, tid);
+        y_store<T, NX, NY>(a.W[s == 0 ? S_AX : S_AY], F, col0, tid);
     }
 }
 
@@ -543,7 +562,7 @@ __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x5(Rhs2DArgs<T> a) {
 enum : int { S_AETA = 6 };
 
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y6(Rhs2DArgs<T> a) {
+__global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves_per_eu(YGeo<T, NY>::WPE))) k2b_y6(Rhs2DArgs<T> a) {
     HW_YK_PROLOGUE
     C F1[G::PE], F2[G::PE], Z[G::PE], eh[G::PE], Zu[G::PE], Zv[G::PE];
     T al[G::PE], kk0[G::PE], kk1[G::PE];
@@ -670,7 +689,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y8(Rhs2DArgs<T> a) {
 enum : int { F_ETA = 0, F_U = 1, F_V = 2, F_M0U = 3, F_M0V = 4 };
 
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2f_y1(Rhs2DArgs<T> a) {
+__global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves_per_eu(YGeo<T, NY>::WPE))) k2f_y1(Rhs2DArgs<T> a) {
     HW_YK_PROLOGUE
     C v[G::PE], w[G::PE];
     y_load<T, NX, NY>(v, a.eta_hat, nullptr, col0, tid);
@@ -734,7 +753,7 @@ __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2f_x2(Rhs2DArgs<T> a) {
 enum : int { F_AETA = 0, F_AX = 1, F_AY = 2 };
 
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2f_y3(Rhs2DArgs<T> a) {
+__global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves_per_eu(YGeo<T, NY>::WPE))) k2f_y3(Rhs2DArgs<T> a) {
     HW_YK_PROLOGUE
     C P[G::PE], Q[G::PE], acc[G::PE];
     const int sl[2][2] = {{F_P1, F_Q1}, {F_P2, F_Q2}};
@@ -789,7 +808,7 @@ __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2f_x4(Rhs2DArgs<T> a) {
 }
 
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2f_y5(Rhs2DArgs<T> a) {
+__global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves_per_eu(YGeo<T, NY>::WPE))) k2f_y5(Rhs2DArgs<T> a) {
     HW_YK_PROLOGUE
     C Z[G::PE], eh[G::PE], uh[G::PE], vh[G::PE];
     y_load<T, NX, NY>(eh, a.eta_hat, nullptr, col0, tid);
