@@ -57,7 +57,7 @@ class HwStats(ct.Structure):
 
 EXPORTS = [
     "hw_ctx_create_2d", "hw_ctx_destroy", "hw_rhs", "hw_rhs_device", "hw_set_state", "hw_get_state",
-    "hw_evolve_apply", "hw_integrate", "hw_get_fields", "hw_get_stats", "hw_reset_stats", "hw_sync",
+    "hw_evolve_apply", "hw_integrate", "hw_integrate_batch", "hw_get_fields", "hw_get_stats", "hw_reset_stats", "hw_sync",
     "hw_fft2_forward", "hw_fft2_inverse", "hw_supported", "hw_last_error", "hw_version",
     "hw_set_tprev", "hw_get_breaking",
     "hw1d_create", "hw1d_destroy", "hw1d_rhs", "hw1d_rhs_at", "hw1d_set_state", "hw1d_get_state",
@@ -90,6 +90,7 @@ def load(path: Path | None = None) -> ct.CDLL:
     lib.hw_get_state.argtypes = [vp, _dp]
     lib.hw_evolve_apply.argtypes = [vp, _dp, ct.c_double, _dp]
     lib.hw_integrate.argtypes = [vp, _dp, ct.c_double, _dp]
+    lib.hw_integrate_batch.argtypes = [ct.POINTER(vp), ct.c_int, _dp, ct.c_double, _dp]
     lib.hw_get_fields.argtypes = [vp, _dp, _dp, _dp]
     lib.hw_get_stats.argtypes = [vp, ct.POINTER(HwStats)]
     lib.hw_reset_stats.argtypes = [vp]

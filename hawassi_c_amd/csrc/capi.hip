@@ -1,5 +1,6 @@
 // capi.hip -- the extern "C" entry points declared in include/hawassi_hip.h.
 #include <hip/hip_runtime.h>
+#include <vector>
 
 #include <cstdarg>
 #include <cstdio>
@@ -134,6 +135,65 @@ int hw_integrate(void* ctx, double* t, double t1, double* h) {
         if (rc) return rc;
     }
     return HW_OK;
+}
+
+int hw_integrate_batch(void* const* ctxs, int n, double* t, double t1, double* h) {
+    if (n < 0 || (n > 0 && (!ctxs || !t || !h))) { set_error("null argument"); return HW_ERR_ARG; }
+    struct Lane { double t0, h0; bool fin, need_k0, active; };
+    std::vector<Lane> ln((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        if (!ctxs[i]) { set_error("null context %d", i); return HW_ERR_ARG; }
+        const double dt = t1 - t[i];
+        if ((dt < 0.0 && h[i] > 0.0) || (dt > 0.0 && h[i] < 0.0)) {
+            set_error("context %d: step direction must match interval direction", i);
+            return HW_ERR_ARG;
+        }
+        ln[i] = Lane{t[i], h[i], false, true, t[i] < t1};
+    }
+    for (;;) {
+        bool any = false;
+        // enqueue one attempt per unfinished context (each on its own stream) ...
+        for (int i = 0; i < n; ++i) {
+            Lane& L = ln[i];
+            if (!L.active) continue;
+            any = true;
+            const double dt = t1 - L.t0;
+            L.fin = false;
+            if ((dt >= 0.0 && L.h0 > dt) || (dt < 0.0 && L.h0 < dt)) { L.h0 = dt; L.fin = true; }
+            int rc = static_cast<Solver2D*>(ctxs[i])->attempt_issue(L.t0, L.h0, L.need_k0);
+            if (rc) return rc;
+            L.need_k0 = false;
+        }
+        if (!any) return HW_OK;
+        // ... then apply each one's step control (gsl_odeiv2_evolve_apply + control_y)
+        for (int i = 0; i < n; ++i) {
+            Lane& L = ln[i];
+            if (!L.active) continue;
+            Solver2D* s = static_cast<Solver2D*>(ctxs[i]);
+            double rm;
+            int rc = s->attempt_result(&rm);
+            if (rc) return rc;
+            const double tn = L.fin ? t1 : L.t0 + L.h0;
+            const double h_old = L.h0;
+            double hn = L.h0;
+            if (hadjust_rule(rm, &hn) < 0) {
+                if (std::fabs(hn) < std::fabs(h_old) && tn + hn != tn) {
+                    s->st.steps_rejected++;
+                    L.h0 = hn;
+                    continue;
+                }
+                t[i] = L.t0;
+                h[i] = hn;
+                set_error("context %d: step size underflow at t=%g", i, L.t0);
+                return HW_ERR_STEP;
+            }
+            if ((rc = s->accept_step(tn))) return rc;
+            t[i] = L.t0 = tn;
+            h[i] = L.h0 = hn;
+            L.need_k0 = true;
+            L.active = tn < t1;
+        }
+    }
 }
 
 int hw_get_fields(void* ctx, double* eta, double* u, double* v) {

@@ -85,6 +85,10 @@ struct Solver2D {
     virtual int set_state_dev(const double* d_full) = 0;
     virtual int get_state_dev(double* d_full) = 0;
     virtual int evolve_apply(double* t, double t1, double* h) = 0;
+    // evolve_apply in pieces, for advancing several contexts in lockstep
+    virtual int attempt_issue(double t0, double h0, bool need_k0) = 0;
+    virtual int attempt_result(double* rmax) = 0;
+    virtual int accept_step(double t) = 0;
     virtual int fields_dev(double* eta, double* u, double* v) = 0;
     virtual int breaking_state(double* B, int* flagbr) = 0;
     double tprev = 0.0;    // the reference's global tprev
@@ -655,9 +659,60 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
 
     C* K(int i) { return kbuf.p + (size_t)i * 3 * H; }
 
+    // One attempted rkf45 step from (t0, y) with step h0, enqueued on the
+    // stream: K0 = f(t0, y) first when need_k0 (gsl's dydt_in, computed once
+    // per evolve_apply call and reused by retried attempts), the five further
+    // stages, the error norm and, with breaking, rkf45's dydt_out evaluation.
+    // The norm's copy to the host is asynchronous; attempt_result waits for it.
+    int attempt_issue(double t0, double h0, bool need_k0) override {
+        int rc;
+        if (need_k0) {
+            if ((rc = stage(0.0, {}, {})) || (rc = rhs(t0, ytmp.p, K(0)))) return rc;
+        }
+        if ((rc = stage(h0, {0}, {RK_AH[0]})) || (rc = rhs(t0 + RK_AH[0] * h0, ytmp.p, K(1)))) return rc;
+        if ((rc = stage(h0, {0, 1}, {RK_B3[0], RK_B3[1]})) || (rc = rhs(t0 + RK_AH[1] * h0, ytmp.p, K(2)))) return rc;
+        if ((rc = stage(h0, {0, 1, 2}, {RK_B4[0], RK_B4[1], RK_B4[2]})) || (rc = rhs(t0 + RK_AH[2] * h0, ytmp.p, K(3)))) return rc;
+        if ((rc = stage(h0, {0, 1, 2, 3}, {RK_B5[0], RK_B5[1], RK_B5[2], RK_B5[3]})) ||
+            (rc = rhs(t0 + RK_AH[3] * h0, ytmp.p, K(4))))
+            return rc;
+        if ((rc = stage(h0, {0, 1, 2, 3, 4}, {RK_B6[0], RK_B6[1], RK_B6[2], RK_B6[3], RK_B6[4]})) ||
+            (rc = rhs(t0 + RK_AH[4] * h0, ytmp.p, K(5))))
+            return rc;
+        k_zero_u64<<<1, 64, 0, stream>>>(rmax.p, 1);
+        k_rk_final<T><<<dim3(2048), 256, 0, stream>>>(y.p, K(0), K(2), K(3), K(4), K(5), h0, eps_abs, eps_rel, ynew.p,
+                                                        rmax.p, 3 * H);
+        HW_LAUNCH_CHECK();
+        if (brk_on) {
+            // rkf45's dydt_out = f(t + h, y_new): its breaking side effects
+            // are part of the reference's sequence (K(1) is free here)
+            k_convert<double2, C><<<grid1(3 * H), 256, 0, stream>>>(ynew.p, ytmp.p, 3 * H);
+            HW_LAUNCH_CHECK();
+            if ((rc = rhs(t0 + h0, ytmp.p, K(1)))) return rc;
+        }
+        HW_TRY(hipMemcpyAsync(h_rmax, rmax.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+        return HW_OK;
+    }
+
+    // max_i E_i / D_i of the last issued attempt (GSL starts rmax at DBL_MIN)
+    int attempt_result(double* rm) override {
+        HW_TRY(hipStreamSynchronize(stream));
+        std::memcpy(rm, h_rmax, sizeof(double));
+        if (*rm < 2.2250738585072014e-308) *rm = 2.2250738585072014e-308;
+        return HW_OK;
+    }
+
+    // the last attempt is accepted: y <- y_new at time t
+    int accept_step(double t) override {
+        std::swap(y.p, ynew.p);
+        st.steps_accepted++;
+        tprev = t;   // ode_solver_uv: tprev = t after each evolve_apply
+        return collect_timing();
+    }
+
     // gsl_odeiv2_evolve_apply with rkf45 + control_y, restated (see header).
-    // The rkf45 stepper's dydt_out evaluation is not performed: control_y
-    // does not read it and the RHS has no side effects on this path.
+    // The rkf45 stepper's dydt_out evaluation is not performed without
+    // breaking: control_y does not read it and the RHS has no other side
+    // effects on this path.
     int evolve_apply(double* t, double t1, double* h) override {
         const double t0 = *t;
         double h0 = *h;
@@ -666,37 +721,14 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
             set_error("step direction must match interval direction");
             return HW_ERR_ARG;
         }
-        int rc;
-        if ((rc = stage(0.0, {}, {}))) return rc;
-        if ((rc = rhs(t0, ytmp.p, K(0)))) return rc;
+        bool need_k0 = true;
         for (;;) {
             bool final_step = false;
             if ((dt >= 0.0 && h0 > dt) || (dt < 0.0 && h0 < dt)) { h0 = dt; final_step = true; }
-            if ((rc = stage(h0, {0}, {RK_AH[0]})) || (rc = rhs(t0 + RK_AH[0] * h0, ytmp.p, K(1)))) return rc;
-            if ((rc = stage(h0, {0, 1}, {RK_B3[0], RK_B3[1]})) || (rc = rhs(t0 + RK_AH[1] * h0, ytmp.p, K(2)))) return rc;
-            if ((rc = stage(h0, {0, 1, 2}, {RK_B4[0], RK_B4[1], RK_B4[2]})) || (rc = rhs(t0 + RK_AH[2] * h0, ytmp.p, K(3)))) return rc;
-            if ((rc = stage(h0, {0, 1, 2, 3}, {RK_B5[0], RK_B5[1], RK_B5[2], RK_B5[3]})) ||
-                (rc = rhs(t0 + RK_AH[3] * h0, ytmp.p, K(4))))
-                return rc;
-            if ((rc = stage(h0, {0, 1, 2, 3, 4}, {RK_B6[0], RK_B6[1], RK_B6[2], RK_B6[3], RK_B6[4]})) ||
-                (rc = rhs(t0 + RK_AH[4] * h0, ytmp.p, K(5))))
-                return rc;
-            k_zero_u64<<<1, 64, 0, stream>>>(rmax.p, 1);
-            k_rk_final<T><<<dim3(2048), 256, 0, stream>>>(y.p, K(0), K(2), K(3), K(4), K(5), h0, eps_abs, eps_rel, ynew.p,
-                                                            rmax.p, 3 * H);
-            HW_LAUNCH_CHECK();
-            if (brk_on) {
-                // rkf45's dydt_out = f(t + h, y_new): its breaking side effects
-                // are part of the reference's sequence (K(1) is free here)
-                k_convert<double2, C><<<grid1(3 * H), 256, 0, stream>>>(ynew.p, ytmp.p, 3 * H);
-                HW_LAUNCH_CHECK();
-                if ((rc = rhs(t0 + h0, ytmp.p, K(1)))) return rc;
-            }
-            HW_TRY(hipMemcpyAsync(h_rmax, rmax.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
-            HW_TRY(hipStreamSynchronize(stream));
+            int rc;
             double rm;
-            std::memcpy(&rm, h_rmax, sizeof(double));
-            if (rm < 2.2250738585072014e-308) rm = 2.2250738585072014e-308;   // GSL starts rmax at DBL_MIN
+            if ((rc = attempt_issue(t0, h0, need_k0)) || (rc = attempt_result(&rm))) return rc;
+            need_k0 = false;
             *t = final_step ? t1 : t0 + h0;
             double h_old = h0;
             int adj = hadjust_rule(rm, &h0);
@@ -711,11 +743,8 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
                 set_error("step size underflow at t=%g", t0);
                 return HW_ERR_STEP;
             }
-            std::swap(y.p, ynew.p);
-            st.steps_accepted++;
             *h = h0;
-            tprev = *t;   // ode_solver_uv: tprev = t after each evolve_apply
-            return collect_timing();
+            return accept_step(*t);
         }
     }
 

@@ -157,6 +157,23 @@ class Hawassi2D:
         _lib.check(self.lib.hw_sync(self.ctx), "hw_sync")
 
 
+def integrate_batch(solvers: "list[Hawassi2D]", t: "list[float]", t1: float, h: "list[float]"):
+    """Advance several contexts to t1 in lockstep (hw_integrate_batch).
+
+    Each context keeps its own step control; the result equals calling
+    integrate on each alone.  Returns the new (t, h) lists.
+    """
+    n = len(solvers)
+    if n == 0:
+        return [], []
+    lib = solvers[0].lib
+    arr = (ct.c_void_p * n)(*[s.ctx for s in solvers])
+    tt = np.ascontiguousarray(t, dtype=np.float64).copy()
+    hh = np.ascontiguousarray(h, dtype=np.float64).copy()
+    _lib.check(lib.hw_integrate_batch(arr, n, _lib.ptr(tt), float(t1), _lib.ptr(hh)), "hw_integrate_batch")
+    return tt.tolist(), hh.tolist()
+
+
 def fft2_forward(phys: np.ndarray, precision: int = _lib.HW_PREC_FP64):
     """Device forward transform of a real [ny, nx] field (full-grid result)."""
     lib = _lib.load()

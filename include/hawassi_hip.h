@@ -117,6 +117,15 @@ int hw_evolve_apply(void *ctx, double *t, double t1, double *h);
  * (2d/fun_odesolver.c:2641-2647). */
 int hw_integrate(void *ctx, double *t, double t1, double *h);
 
+/* Batched ode_solver_uv loop: advances n independent contexts (e.g. the
+ * realizations of one sea state) to t1, each with its own t[i], h[i] and step
+ * control exactly as hw_integrate does for it alone (results are identical);
+ * one attempted step of every unfinished context is enqueued (each context
+ * on its own stream) before any step-control decision waits, so the
+ * contexts' kernels overlap on the GPU.  The reference runs one
+ * realization per process (2d/fun_odesolver.c:2641-2647). */
+int hw_integrate_batch(void *const *ctxs, int n, double *t, double t1, double *h);
+
 /* Physical eta, u, v [ny*nx] of the current state, as the output IFFTs of
  * ode_solver_uv (2d/fun_odesolver.c:2663-2666); any pointer may be NULL. */
 int hw_get_fields(void *ctx, double *eta, double *u, double *v);
