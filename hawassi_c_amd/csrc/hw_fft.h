@@ -18,6 +18,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <cmath>
+#include <vector>
 
 namespace hw {
 
@@ -235,47 +237,33 @@ constexpr int rs_radix(int L, int E, int i) {
     return L;
 }
 constexpr int rs_ns(int L, int E, int i) { int ns = 1; for (int k = 0; k < i; ++k) ns *= rs_radix(L, E, k); return ns; }
+// Per-pass twiddle tables: pass I >= 1 (radix R, NS = product of the earlier
+// radices) needs w^(r k L/(NS R)), w = exp(-2 pi i / L), for r = 1..R-1 and
+// k < NS; it is stored at [(r-1) NS + k] from rs_twoff(L, E, I), so a wave
+// reads consecutive entries (no LDS bank conflicts) and no twiddle is formed
+// by multiplication.
+constexpr int rs_twoff(int L, int E, int I) {
+    int o = 0;
+    for (int i = 1; i < I; ++i) o += (rs_radix(L, E, i) - 1) * rs_ns(L, E, i);
+    return o;
+}
+constexpr int rs_twsize(int L, int E) { return rs_twoff(L, E, rs_np(L, E)); }
 
-// v[r] *= w^r, w = tw[ex*TS] (conjugated for DIR = +1).  Radix 8/16 build w^r
-// from w, w^4 (and w^8) read from the table: at most two roundings more than
-// a table entry.
-template <typename C, int R, int DIR>
-__device__ __forceinline__ void tw_apply(C* v, const C* __restrict__ tw, int ex, int TS) {
-    if constexpr (R <= 4) {
+// v[r] *= t[(r-1) NS] (conjugated for DIR = +1), t = the pass's table at k
+template <typename C, int R, int NS, int DIR>
+__device__ __forceinline__ void tw_apply(C* v, const C* __restrict__ t) {
 #pragma unroll
-        for (int r = 1; r < R; ++r) {
-            C w = tw[(r * ex) * TS];
-            if (DIR > 0) w = cconj(w);
-            v[r] = cmul(v[r], w);
-        }
-    } else {
-        C wp[R];
-        C w1 = tw[ex * TS], w4 = tw[4 * ex * TS];
-        if (DIR > 0) { w1 = cconj(w1); w4 = cconj(w4); }
-        wp[1] = w1;
-        wp[2] = cmul(w1, w1);
-        wp[3] = cmul(wp[2], w1);
-        wp[4] = w4;
-        wp[5] = cmul(w4, w1);
-        wp[6] = cmul(w4, wp[2]);
-        wp[7] = cmul(w4, wp[3]);
-        if constexpr (R == 16) {
-            C w8 = tw[8 * ex * TS];
-            if (DIR > 0) w8 = cconj(w8);
-            wp[8] = w8;
-#pragma unroll
-            for (int r = 1; r < 8; ++r) wp[8 + r] = cmul(w8, wp[r]);
-        }
-#pragma unroll
-        for (int r = 1; r < R; ++r) v[r] = cmul(v[r], wp[r]);
+    for (int r = 1; r < R; ++r) {
+        const C w = t[(r - 1) * NS];
+        v[r] = DIR > 0 ? cmk<C>(v[r].x * w.x + v[r].y * w.y, v[r].y * w.x - v[r].x * w.y) : cmul(v[r], w);
     }
 }
 
 // Pass I of a length-L transform of B sequences by NT threads holding E
 // elements each.  Exchange buffers: B rows of stride LSb at bufA / bufB
-// (bufB == bufA when TWO is false).
+// (bufB == bufA when TWO is false); tw = the per-pass twiddle tables.
 template <typename C, int L, int B, int NT, int E, int DIR, bool TWO, int I>
-__device__ __forceinline__ void fr_pass(C (&v)[E], C* bufA, C* bufB, int LSb, const C* __restrict__ tw, int TS, int tid) {
+__device__ __forceinline__ void fr_pass(C (&v)[E], C* bufA, C* bufB, int LSb, const C* __restrict__ tw, int tid) {
     constexpr int NP = rs_np(L, E);
     static_assert(B * L == NT * E, "register layout must cover the block exactly");
     if constexpr (I < NP) {
@@ -283,7 +271,7 @@ __device__ __forceinline__ void fr_pass(C (&v)[E], C* bufA, C* bufB, int LSb, co
 #pragma unroll
         for (int p = 0; p < E / R; ++p) {
             const int j = (tid + p * NT) % M;
-            if constexpr (NS > 1) tw_apply<C, R, DIR>(&v[p * R], tw, (j % NS) * (L / (NS * R)), TS);
+            if constexpr (NS > 1) tw_apply<C, R, NS, DIR>(&v[p * R], tw + rs_twoff(L, E, I) + j % NS);
             dftR<R, DIR>(&v[p * R]);
         }
         if constexpr (I + 1 < NP) {
@@ -313,7 +301,7 @@ __device__ __forceinline__ void fr_pass(C (&v)[E], C* bufA, C* bufB, int LSb, co
                     for (int r = 0; r < R2; ++r) v[p * R2 + r] = src[pidx(j + r * M2)];
                 }
             }
-            fr_pass<C, L, B, NT, E, DIR, TWO, I + 1>(v, bufA, bufB, LSb, tw, TS, tid);
+            fr_pass<C, L, B, NT, E, DIR, TWO, I + 1>(v, bufA, bufB, LSb, tw, tid);
         }
     }
 }
@@ -321,8 +309,8 @@ __device__ __forceinline__ void fr_pass(C (&v)[E], C* bufA, C* bufB, int LSb, co
 // Unnormalised transform of B sequences held in registers (layout above).
 // The caller guarantees that no thread still reads bufA/bufB from earlier work.
 template <typename C, int L, int B, int NT, int E, int DIR, bool TWO>
-__device__ __forceinline__ void fft_reg(C (&v)[E], C* bufA, C* bufB, int LSb, const C* __restrict__ tw, int TS, int tid) {
-    fr_pass<C, L, B, NT, E, DIR, TWO, 0>(v, bufA, bufB, LSb, tw, TS, tid);
+__device__ __forceinline__ void fft_reg(C (&v)[E], C* bufA, C* bufB, int LSb, const C* __restrict__ tw, int tid) {
+    fr_pass<C, L, B, NT, E, DIR, TWO, 0>(v, bufA, bufB, LSb, tw, tid);
 }
 // whether the last exchange of fft_reg used bufA
 template <int L, int E> constexpr bool fr_last_a() { return rs_np(L, E) >= 2 && (rs_np(L, E) - 2) % 2 == 0; }
@@ -378,9 +366,13 @@ template <typename T, int NX> struct XGeo {
     static constexpr int SK = NT / RR;                  // kx step between a thread's spectral slots
     // two exchange buffers when they fit beside the twiddle table, else one;
     // the twiddle table is staged in LDS when it fits
-    static constexpr bool TWO = (2 * RR * LSb + NX) * (int)sizeof(C) + 64 <= HW_X_LDS_BUDGET;
+    // twiddles: the FFT's per-pass tables, then exp(-2 pi i k / NX), k = 0..L,
+    // for the real/complex pre- and post-processing
+    static constexpr int TWPP = rs_twsize(L, E);
+    static constexpr int TWN = TWPP + L + 1;
+    static constexpr bool TWO = (2 * RR * LSb + TWN) * (int)sizeof(C) + 64 <= HW_X_LDS_BUDGET;
     static constexpr int LDS_ELEMS = (TWO ? 2 : 1) * RR * LSb;
-    static constexpr bool TWL = (LDS_ELEMS + NX) * (int)sizeof(C) + 64 <= HW_X_LDS_BUDGET;
+    static constexpr bool TWL = (LDS_ELEMS + TWN) * (int)sizeof(C) + 64 <= HW_X_LDS_BUDGET;
     // physical pair q of thread tid -> row * L + n
     __device__ static __forceinline__ int ppos(int tid, int q) {
         int b, n;
@@ -399,9 +391,10 @@ template <typename T, int NY> struct YGeo {
     static constexpr int NE = CC * NY;
     static constexpr int NT = NE / E;
     static constexpr int PE = E;                        // elements per thread
-    static constexpr bool TWO = (2 * CC * LS + NY) * (int)sizeof(C) <= HW_Y_LDS_BUDGET;
+    static constexpr int TWN = rs_twsize(NY, E);         // per-pass twiddle tables
+    static constexpr bool TWO = (2 * CC * LS + TWN) * (int)sizeof(C) <= HW_Y_LDS_BUDGET;
     static constexpr int LDS_ELEMS = (TWO ? 2 : 1) * CC * LS;
-    static constexpr bool TWL = (LDS_ELEMS + NY) * (int)sizeof(C) <= HW_Y_LDS_BUDGET;
+    static constexpr bool TWL = (LDS_ELEMS + TWN) * (int)sizeof(C) <= HW_Y_LDS_BUDGET;
     // waves per SIMD the y kernels are compiled for: FP32 blocks fit three
     // to a CU by LDS, so registers should not be the tighter limit
     static constexpr int WPE = sizeof(T) == 4 ? 3 : 1;
@@ -423,10 +416,11 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
     return (b % 8) * per + b / 8;
 }
 
-// Twiddle tables handed to kernels (device pointers):
-//   twx[m] = exp(-2 pi i m / NX), m < NX    (x-pass: the L=NX/2 FFT uses stride 2)
-//   twy[m] = exp(-2 pi i m / NY), m < NY
-template <typename C> struct Tw { const C* x; const C* y; };
+// Twiddle tables handed to kernels (device pointers, built on the host in
+// FP64 and rounded once):
+//   twx: per-pass tables of the length-NX/2 FFT (rs_twoff), then
+//        exp(-2 pi i k / NX) for k = 0..NX/2 from XGeo::TWPP
+//   twy: per-pass tables of the length-NY FFT
 
 // ----------------------------------------------------------- x-pass ops ---
 // Spectral values of a block: slot q of thread tid is idx = tid + q*NT,
@@ -514,13 +508,13 @@ __device__ __forceinline__ void x_inv_finish(typename ctype<T>::type* lds, const
                 xb = row[pidx(L - k)];
             }
             if (r == 0 && j == 0) { xa.y = (T)0; xb.y = (T)0; }
-            const C wa = cconj(twx[k]);
+            const C wa = cconj(twx[G::TWPP + k]);
             const C ea = cadd(xa, cconj(xb));
             const C oa = cmul(csub(xa, cconj(xb)), wa);
             out[p * R0 + r] = cadd(ea, cmul_i<1>(oa));
         }
     }
-    fft_reg<C, L, RR, NT, E, +1, G::TWO>(out, A, S, LSb, twx, 2, tid);
+    fft_reg<C, L, RR, NT, E, +1, G::TWO>(out, A, S, LSb, twx, tid);
 #pragma unroll
     for (int s = 0; s < E; ++s) out[s] = cscale(out[s], scale);
 }
@@ -562,7 +556,7 @@ __device__ __forceinline__ const typename ctype<T>::type* x_fwd_to_lds(const car
 #pragma unroll
     for (int s = 0; s < E; ++s) v[s] = f[s];
     __syncthreads();
-    fft_reg<C, L, RR, NT, E, -1, G::TWO>(v, A, S, LSb, twx, 2, tid);
+    fft_reg<C, L, RR, NT, E, -1, G::TWO>(v, A, S, LSb, twx, tid);
     // Z to LDS in natural order (the buffer the last exchange did not use)
     C* O = G::TWO ? (fr_last_a<L, E>() ? S : A) : S;
     if constexpr (!G::TWO) __syncthreads();
@@ -609,7 +603,7 @@ __device__ __forceinline__ typename ctype<T>::type x_post(const typename ctype<T
     }
     const C ea = cscale(cadd(za, cconj(zb)), (T)0.5);
     const C da = cscale(csub(za, cconj(zb)), (T)0.5);
-    return cadd(ea, cmul_i<-1>(cmul(twx[kx], da)));
+    return cadd(ea, cmul_i<-1>(cmul(twx[G::TWPP + kx], da)));
 }
 
 // spectral slot q of this thread holds a value (kx <= L)
@@ -681,7 +675,7 @@ __device__ __forceinline__ void y_fwd_regs(carr<T, YGeo<T, NY>::PE>& v, typename
     using G = YGeo<T, NY>;
     using C = typename ctype<T>::type;
     __syncthreads();
-    fft_reg<C, NY, G::CC, G::NT, G::E, -1, G::TWO>(v, lds, G::TWO ? lds + G::CC * G::LS : lds, G::LS, twy, 1, tid);
+    fft_reg<C, NY, G::CC, G::NT, G::E, -1, G::TWO>(v, lds, G::TWO ? lds + G::CC * G::LS : lds, G::LS, twy, tid);
 }
 
 template <typename T, int NX, int NY>
@@ -693,7 +687,37 @@ __device__ __forceinline__ void y_inv_regs(carr<T, YGeo<T, NY>::PE>& v, typename
     // the projection stages column 0 in the buffer the first exchange does not write
     C* pb = G::TWO ? lds + G::CC * G::LS : lds;
     y_project_regs<T, NX, NY>(v, pb, col0, tid);
-    fft_reg<C, NY, G::CC, G::NT, G::E, +1, G::TWO>(v, lds, G::TWO ? lds + G::CC * G::LS : lds, G::LS, twy, 1, tid);
+    fft_reg<C, NY, G::CC, G::NT, G::E, +1, G::TWO>(v, lds, G::TWO ? lds + G::CC * G::LS : lds, G::LS, twy, tid);
+}
+
+// Host: twiddle tables of the 2-D passes (FP64 values rounded once to T).
+template <typename T> std::vector<typename ctype<T>::type> fr_pass_tables(int L, int E) {
+    std::vector<typename ctype<T>::type> t((size_t)rs_twsize(L, E));
+    for (int I = 1; I < rs_np(L, E); ++I) {
+        const int R = rs_radix(L, E, I), NS = rs_ns(L, E, I), o = rs_twoff(L, E, I);
+        for (int r = 1; r < R; ++r)
+            for (int k = 0; k < NS; ++k) {
+                const long m = (long)r * k * (L / (NS * R)) % L;
+                const double a = -2.0 * M_PI * (double)m / (double)L;
+                t[(size_t)o + (size_t)(r - 1) * NS + k].x = (T)std::cos(a);
+                t[(size_t)o + (size_t)(r - 1) * NS + k].y = (T)std::sin(a);
+            }
+    }
+    return t;
+}
+template <typename T, int NX> std::vector<typename ctype<T>::type> x_twiddles() {
+    using G = XGeo<T, NX>;
+    auto t = fr_pass_tables<T>(G::L, G::E);
+    t.resize((size_t)G::TWN);
+    for (int k = 0; k <= G::L; ++k) {
+        const double a = -2.0 * M_PI * (double)k / (double)NX;
+        t[(size_t)G::TWPP + k].x = (T)std::cos(a);
+        t[(size_t)G::TWPP + k].y = (T)std::sin(a);
+    }
+    return t;
+}
+template <typename T, int NY> std::vector<typename ctype<T>::type> y_twiddles() {
+    return fr_pass_tables<T>(NY, YGeo<T, NY>::E);
 }
 
 }  // namespace hw

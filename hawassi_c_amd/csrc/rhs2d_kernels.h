@@ -209,13 +209,13 @@ template <typename T, int NX, int NY> struct YIdx {
     using G = XGeo<T, NX>;                                               \
     __shared__ C lds[G::LDS_ELEMS];                                      \
     __shared__ T nyq[G::RR];                                             \
-    __shared__ C twxs_l[G::TWL ? NX : 1];                                \
+    __shared__ C twxs_l[G::TWL ? G::TWN : 1];                            \
     const C* twxs = G::TWL ? twxs_l : (TWSRC);                           \
     const int tid = threadIdx.x;                                         \
     const int y0 = xcd_remap(blockIdx.x, gridDim.x) * G::RR;             \
     const T sc = (T)1 / ((T)NX * (T)NY);                                 \
     if (G::TWL)                                                          \
-        for (int i_ = tid; i_ < NX; i_ += G::NT) twxs_l[i_] = (TWSRC)[i_]; \
+        for (int i_ = tid; i_ < G::TWN; i_ += G::NT) twxs_l[i_] = (TWSRC)[i_]; \
     __syncthreads();                                                     \
     (void)nyq; (void)sc;
 
@@ -224,12 +224,12 @@ template <typename T, int NX, int NY> struct YIdx {
     using C = typename ctype<T>::type;                                   \
     using G = YGeo<T, NY>;                                               \
     __shared__ C lds[G::LDS_ELEMS];                                      \
-    __shared__ C twys_l[G::TWL ? NY : 1];                                \
+    __shared__ C twys_l[G::TWL ? G::TWN : 1];                            \
     const C* twys = G::TWL ? twys_l : (TWSRC);                           \
     const int tid = threadIdx.x;                                         \
     const int col0 = blockIdx.x * G::CC;                                 \
     if (G::TWL)                                                          \
-        for (int i_ = tid; i_ < NY; i_ += G::NT) twys_l[i_] = (TWSRC)[i_]; \
+        for (int i_ = tid; i_ < G::TWN; i_ += G::NT) twys_l[i_] = (TWSRC)[i_]; \
     __syncthreads();
 
 // W slots used by the 2DB pipeline

@@ -206,16 +206,6 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         for (size_t i = 0; i < N; ++i) out[i] = (T)(a[i] * scale);
         return out;
     }
-    static std::vector<C> twiddles(int n) {
-        std::vector<C> tw(n);
-        for (int m = 0; m < n; ++m) {
-            double a = -2.0 * M_PI * (double)m / (double)n;
-            tw[m].x = (T)std::cos(a);
-            tw[m].y = (T)std::sin(a);
-        }
-        return tw;
-    }
-
     int init(const hw_config_2d* c) override {
         nx = NX; ny = NY; prec = c->precision;
         flat = c->flat != 0;
@@ -253,7 +243,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
                 (rc = om2_m.upload(to_half(c->om2_min), stream)) || (rc = om2_p.upload(to_half(c->om2_plus), stream)))
                 return rc;
         }
-        if ((rc = twx.upload(twiddles(NX), stream)) || (rc = twy.upload(twiddles(NY), stream))) return rc;
+        if ((rc = twx.upload(x_twiddles<T, NX>(), stream)) || (rc = twy.upload(y_twiddles<T, NY>(), stream))) return rc;
         src_on = c->source_on;
         if (src_on) {
             nline = c->nline; nt = c->nt;
@@ -779,7 +769,7 @@ int fft2_forward(const double* phys, double* re, double* im) {
     DevBuf<C> X, S, tx, ty;
     int rc;
     if ((rc = dp.upload(hp, 0)) || (rc = X.alloc(H)) || (rc = S.alloc(H)) ||
-        (rc = tx.upload(Solver2DT<T, NX, NY>::twiddles(NX), 0)) || (rc = ty.upload(Solver2DT<T, NX, NY>::twiddles(NY), 0)))
+        (rc = tx.upload(x_twiddles<T, NX>(), 0)) || (rc = ty.upload(y_twiddles<T, NY>(), 0)))
         return rc;
     k_t_xfwd<T, NX, NY><<<NY / XGeo<T, NX>::RR, XGeo<T, NX>::NT>>>(dp.p, X.p, tx.p);
     k_t_yfwd<T, NX, NY><<<(NXH + YGeo<T, NY>::CC - 1) / YGeo<T, NY>::CC, YGeo<T, NY>::NT>>>(X.p, S.p, ty.p);
@@ -812,7 +802,7 @@ int fft2_inverse(const double* re, const double* im, double* phys, double* nyq_i
     DevBuf<T> dp, dn;
     int rc;
     if ((rc = S.upload(hs, 0)) || (rc = X.alloc(H)) || (rc = dp.alloc(N)) || (rc = dn.alloc(NY)) ||
-        (rc = tx.upload(Solver2DT<T, NX, NY>::twiddles(NX), 0)) || (rc = ty.upload(Solver2DT<T, NX, NY>::twiddles(NY), 0)))
+        (rc = tx.upload(x_twiddles<T, NX>(), 0)) || (rc = ty.upload(y_twiddles<T, NY>(), 0)))
         return rc;
     k_t_yinv<T, NX, NY><<<(NXH + YGeo<T, NY>::CC - 1) / YGeo<T, NY>::CC, YGeo<T, NY>::NT>>>(S.p, X.p, ty.p);
     k_t_xinv<T, NX, NY><<<NY / XGeo<T, NX>::RR, XGeo<T, NX>::NT>>>(X.p, dp.p, dn.p, tx.p);
