@@ -53,8 +53,8 @@ struct Brk1D {
     int pad[2];
 };
 
-enum : int { R1_ETA = 0, R1_U, R1_M0U, R1_H, R1_P2, R1_FLUX, R1_NR };
-enum : int { C1_EH = 0, C1_M0H, C1_EUH, C1_U2H, C1_HP, C1_NC };
+enum : int { R1_ETA = 0, R1_U, R1_M0U, R1_H, R1_P2, R1_FLUX, R1_AU, R1_AE, R1_NR };
+enum : int { C1_EH = 0, C1_M0H, C1_EUH, C1_U2H, C1_HP, C1_UH, C1_NC };
 
 template <typename T> struct Batch1DArgs {
     using C = typename ctype<T>::type;
@@ -212,59 +212,50 @@ template <typename T, int N, int NT> struct P1 {
         for (int e = 0; e < EH; ++e) { p[kl(tid, e)] = l[e]; p[kh(tid, e)] = h[e]; }
     }
 
-    // (a, b) = (Re F^-1 X, Re F^-1 Y), F^-1 = backward DFT / N
-    __device__ static void inv_pair(const C (&Xl)[EH], const C (&Xh)[EH], const C (&Yl)[EH], const C (&Yh)[EH],
-                                    T (&al)[EH], T (&ah)[EH], T (&bl)[EH], T (&bh)[EH], C* lds,
-                                    const C* __restrict__ tw, int tid) {
+    // Per-slot pieces of the paired transforms.  Slot e of a thread is the
+    // index pair (kl, kh); outside the FFT itself a thread only touches its
+    // own two LDS entries, so reading a transform's output and writing the
+    // next transform's input at a slot needs no barrier, and no field has to
+    // be held in registers across a transform.
+    //   inverse: (a, b) = (Re F^-1 X, Re F^-1 Y) from one complex FFT of
+    //            Herm(X) + i Herm(Y), Herm(X)[k] = (X[k] + conj X[-k]) / 2
+    //   forward: (A, B) = (F a, F b) for real a, b from one FFT of a + i b
+    __device__ static __forceinline__ void inv_in(C* lds, int tid, int e, C xl, C xh, C yl, C yh) {
         const T hf = (T)0.5;
-        __syncthreads();
-#pragma unroll
-        for (int e = 0; e < EH; ++e) {
-            const bool z = kl(tid, e) == 0;
-            C xml = z ? Xl[e] : Xh[e], xmh = z ? Xh[e] : Xl[e];
-            C yml = z ? Yl[e] : Yh[e], ymh = z ? Yh[e] : Yl[e];
-            // Herm(X)[k] + i Herm(Y)[k], Herm(X)[k] = (X[k] + conj X[-k]) / 2
-            T hxr = hf * (Xl[e].x + xml.x), hxi = hf * (Xl[e].y - xml.y);
-            T hyr = hf * (Yl[e].x + yml.x), hyi = hf * (Yl[e].y - yml.y);
-            lds[kl(tid, e)] = cmk<C>(hxr - hyi, hxi + hyr);
-            hxr = hf * (Xh[e].x + xmh.x); hxi = hf * (Xh[e].y - xmh.y);
-            hyr = hf * (Yh[e].x + ymh.x); hyi = hf * (Yh[e].y - ymh.y);
-            lds[kh(tid, e)] = cmk<C>(hxr - hyi, hxi + hyr);
-        }
-        __syncthreads();
-        fft_lds<C, N, 1, NT, 1>(lds, N, tw, 1, tid);
-        const T s = (T)1 / (T)N;
-#pragma unroll
-        for (int e = 0; e < EH; ++e) {
-            C zl = lds[kl(tid, e)], zh = lds[kh(tid, e)];
-            al[e] = zl.x * s; bl[e] = zl.y * s;
-            ah[e] = zh.x * s; bh[e] = zh.y * s;
-        }
+        const bool z = kl(tid, e) == 0;
+        C xml = z ? xl : xh, xmh = z ? xh : xl;
+        C yml = z ? yl : yh, ymh = z ? yh : yl;
+        T hxr = hf * (xl.x + xml.x), hxi = hf * (xl.y - xml.y);
+        T hyr = hf * (yl.x + yml.x), hyi = hf * (yl.y - yml.y);
+        lds[kl(tid, e)] = cmk<C>(hxr - hyi, hxi + hyr);
+        hxr = hf * (xh.x + xmh.x); hxi = hf * (xh.y - xmh.y);
+        hyr = hf * (yh.x + ymh.x); hyi = hf * (yh.y - ymh.y);
+        lds[kh(tid, e)] = cmk<C>(hxr - hyi, hxi + hyr);
     }
-
-    // (A, B) = (F a, F b) for real a, b; F = forward DFT (unnormalised)
-    __device__ static void fwd_pair(const T (&al)[EH], const T (&ah)[EH], const T (&bl)[EH], const T (&bh)[EH],
-                                    C (&Al)[EH], C (&Ah)[EH], C (&Bl)[EH], C (&Bh)[EH], C* lds,
-                                    const C* __restrict__ tw, int tid) {
+    __device__ static __forceinline__ void inv_out(const C* lds, int tid, int e, T& al, T& ah, T& bl, T& bh) {
+        const T s = (T)1 / (T)N;
+        const C zl = lds[kl(tid, e)], zh = lds[kh(tid, e)];
+        al = zl.x * s; bl = zl.y * s;
+        ah = zh.x * s; bh = zh.y * s;
+    }
+    __device__ static __forceinline__ void fwd_in(C* lds, int tid, int e, T al, T ah, T bl, T bh) {
+        lds[kl(tid, e)] = cmk<C>(al, bl);
+        lds[kh(tid, e)] = cmk<C>(ah, bh);
+    }
+    __device__ static __forceinline__ void fwd_out(const C* lds, int tid, int e, C& Al, C& Ah, C& Bl, C& Bh) {
         const T hf = (T)0.5;
+        const bool z = kl(tid, e) == 0;
+        const C zl = lds[kl(tid, e)], zh = lds[kh(tid, e)];
+        const C ml = z ? zl : zh, mh = z ? zh : zl;
+        Al = cmk<C>(hf * (zl.x + ml.x), hf * (zl.y - ml.y));
+        Bl = cmk<C>(hf * (zl.y + ml.y), -hf * (zl.x - ml.x));
+        Ah = cmk<C>(hf * (zh.x + mh.x), hf * (zh.y - mh.y));
+        Bh = cmk<C>(hf * (zh.y + mh.y), -hf * (zh.x - mh.x));
+    }
+    // the transform itself: inputs of every slot written -> outputs readable
+    template <int DIR> __device__ static __forceinline__ void run(C* lds, const C* __restrict__ tw, int tid) {
         __syncthreads();
-#pragma unroll
-        for (int e = 0; e < EH; ++e) {
-            lds[kl(tid, e)] = cmk<C>(al[e], bl[e]);
-            lds[kh(tid, e)] = cmk<C>(ah[e], bh[e]);
-        }
-        __syncthreads();
-        fft_lds<C, N, 1, NT, -1>(lds, N, tw, 1, tid);
-#pragma unroll
-        for (int e = 0; e < EH; ++e) {
-            const bool z = kl(tid, e) == 0;
-            C zl = lds[kl(tid, e)], zh = lds[kh(tid, e)];
-            C ml = z ? zl : zh, mh = z ? zh : zl;
-            Al[e] = cmk<C>(hf * (zl.x + ml.x), hf * (zl.y - ml.y));
-            Bl[e] = cmk<C>(hf * (zl.y + ml.y), -hf * (zl.x - ml.x));
-            Ah[e] = cmk<C>(hf * (zh.x + mh.x), hf * (zh.y - mh.y));
-            Bh[e] = cmk<C>(hf * (zh.y + mh.y), -hf * (zh.x - mh.x));
-        }
+        fft_lds<C, N, 1, NT, DIR>(lds, N, tw, 1, tid);
     }
 };
 
@@ -276,7 +267,7 @@ template <typename T, int N, int NT> struct P1 {
 // between barriers; scans over nodes are block-parallel reductions that keep
 // the reference's first-index semantics.
 template <typename T, int N, int NT>
-__device__ void brk1d_process(const Batch1DArgs<T>& a, const Real1D& R, double t, const T* __restrict__ eta,
+__device__ __attribute__((noinline)) void brk1d_process(const Batch1DArgs<T>& a, const Real1D& R, double t, const T* __restrict__ eta,
                               const T* __restrict__ u, const T* __restrict__ H, T* __restrict__ B, int* sm,
                               double* smd) {
     const int tid = threadIdx.x;
@@ -530,14 +521,34 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
         smd[31] = pvs;
         S->rhs += 1;
     }
-    C ehl[EH], ehh[EH], uhl[EH], uhh[EH];
+    // Every transform is a chain link in LDS: the per-slot epilogue of one
+    // transform writes the next one's input in place, and fields needed
+    // again later are kept in this realization's scratch (sr, sc) rather
+    // than in registers.
+    const T* opa = a.opa + off;
+    const T* opb = a.opb + off;
+    const T* gmv = a.gm + off;
+    const T* gpv = a.gp + off;
+    T* s_eta = sr + R1_ETA * N;
+    T* s_u = sr + R1_U * N;
+    T* s_m0 = sr + R1_M0U * N;
+    T* s_p = sr + R1_P2 * N;
+    T* s_au = sr + R1_AU * N;
+    T* s_ae = sr + R1_AE * N;
+    C* c_eh = sc + C1_EH * N;
+    C* c_uh = sc + C1_UH * N;
+    C* c_m0h = sc + C1_M0H * N;
+    C* c_euh = sc + C1_EUH * N;
+    C* c_u2h = sc + C1_U2H * N;
+    C* c_hp = sc + C1_HP * N;
+    // ---- stage input (the RK combination is fused into the load); eta, u
     {
         const double2* base = (stage == 0 ? a.ext : a.y) + 2 * off;
         const int nk = stage >= 2 ? 6 : 0;
         double co[6];
 #pragma unroll
         for (int j = 0; j < 6; ++j) co[j] = c_rk1d[stage][j];
-#pragma unroll
+#pragma unroll 1
         for (int e = 0; e < EH; ++e) {
             int idx[4] = {P::kl(tid, e), P::kh(tid, e), N + P::kl(tid, e), N + P::kh(tid, e)};
             C v[4];
@@ -554,232 +565,221 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
                 }
                 v[q] = cmk<C>((T)(b.x + h0 * sx), (T)(b.y + h0 * sy));
             }
-            ehl[e] = v[0]; ehh[e] = v[1]; uhl[e] = v[2]; uhh[e] = v[3];
+            c_eh[idx[0]] = v[0]; c_eh[idx[1]] = v[1];
+            c_uh[idx[0]] = v[2]; c_uh[idx[1]] = v[3];
+            P::inv_in(lds, tid, e, v[0], v[1], v[2], v[3]);
         }
     }
-    P::stc(sc + C1_EH * N, ehl, ehh, tid);
-    // ---- eta, u
-    T etl[EH], eth[EH], ul[EH], uh_[EH];
-    P::inv_pair(ehl, ehh, uhl, uhh, etl, eth, ul, uh_, lds, tw, tid);
-    P::str(sr + R1_ETA * N, etl, eth, tid);
-    P::str(sr + R1_U * N, ul, uh_, tid);
-    // ---- M0u^ = deluH1 = -i k HSS(u^)/g (two depths) or -i k C^2 u^/g (flat)
-    C ml[EH], mh[EH];
-    if (!R.flat) {
-        C xl[EH], xh[EH], yl[EH], yh[EH];
-#pragma unroll
-        for (int e = 0; e < EH; ++e) {
-            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
-            T cm0 = a.opa[off + i0], cm1 = a.opa[off + i1], cp0 = a.opb[off + i0], cp1 = a.opb[off + i1];
-            xl[e] = cscale(uhl[e], cm0); xh[e] = cscale(uhh[e], cm1);
-            yl[e] = cscale(uhl[e], cp0); yh[e] = cscale(uhh[e], cp1);
+    P::template run<1>(lds, tw, tid);
+    // ---- M0u^ = deluH1 = -i k HSS(u^)/g (two depths) or -i k C^2 u^/g (flat);
+    // its inverse shares a transform with the Hilbert transform of eta
+    auto hilbert_in = [&](int e, C m0l, C m0h) {
+        const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+        C xl = cmk<C>((T)0, (T)0), xh = xl;
+        if (R.brk) {
+            T s0 = kk[i0] == (T)0 ? (T)0 : (kk[i0] > (T)0 ? (T)1 : (T)-1);
+            T s1 = kk[i1] == (T)0 ? (T)0 : (kk[i1] > (T)0 ? (T)1 : (T)-1);
+            xl = cmul_i<-1>(cscale(c_eh[i0], s0));
+            xh = cmul_i<-1>(cscale(c_eh[i1], s1));
         }
-        T aml[EH], amh[EH], apl[EH], aph[EH];
-        P::inv_pair(xl, xh, yl, yh, aml, amh, apl, aph, lds, tw, tid);
-        T f1l[EH], f1h[EH], f2l[EH], f2h[EH];
-#pragma unroll
-        for (int e = 0; e < EH; ++e) {
-            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
-            T g0 = a.gm[off + i0], g1 = a.gm[off + i1], q0 = a.gp[off + i0], q1 = a.gp[off + i1];
-            f1l[e] = g0 * aml[e] + q0 * apl[e]; f1h[e] = g1 * amh[e] + q1 * aph[e];
-            f2l[e] = g0 * ul[e]; f2h[e] = g1 * uh_[e];
-        }
-        P::fwd_pair(f1l, f1h, f2l, f2h, xl, xh, yl, yh, lds, tw, tid);
-        // HP = (F(g- a- + g+ a+) + C- F(g- u)) / 2
-#pragma unroll
-        for (int e = 0; e < EH; ++e) {
-            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
-            T cm0 = a.opa[off + i0], cm1 = a.opa[off + i1];
-            ml[e] = cscale(cadd(xl[e], cscale(yl[e], cm0)), (T)0.5);
-            mh[e] = cscale(cadd(xh[e], cscale(yh[e], cm1)), (T)0.5);
-        }
-#pragma unroll
-        for (int e = 0; e < EH; ++e) {
-            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
-            f1l[e] = a.gp[off + i0] * ul[e]; f1h[e] = a.gp[off + i1] * uh_[e];
-            f2l[e] = etl[e] * ul[e]; f2h[e] = eth[e] * uh_[e];
-        }
-        P::fwd_pair(f1l, f1h, f2l, f2h, xl, xh, yl, yh, lds, tw, tid);
-        P::stc(sc + C1_EUH * N, yl, yh, tid);   // F(eta u)
-#pragma unroll
-        for (int e = 0; e < EH; ++e) {
-            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
-            T cp0 = a.opb[off + i0], cp1 = a.opb[off + i1];
-            C H0 = cadd(ml[e], cscale(xl[e], (T)0.5 * cp0)), H1 = cadd(mh[e], cscale(xh[e], (T)0.5 * cp1));
-            ml[e] = cmul_i<-1>(cscale(H0, kk[i0] / g));
-            mh[e] = cmul_i<-1>(cscale(H1, kk[i1] / g));
-        }
-    } else {
-#pragma unroll
-        for (int e = 0; e < EH; ++e) {
-            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
-            ml[e] = cmul_i<-1>(cscale(uhl[e], kk[i0] * a.opa[off + i0] / g));
-            mh[e] = cmul_i<-1>(cscale(uhh[e], kk[i1] * a.opa[off + i1] / g));
-        }
-    }
-    P::stc(sc + C1_M0H * N, ml, mh, tid);
-    // ---- M0u and the Hilbert transform of eta (breaking)
-    T m0l[EH], m0h[EH], hl[EH], hh[EH];
-    {
-        C xl[EH], xh[EH];
-#pragma unroll
-        for (int e = 0; e < EH; ++e) {
-            if (R.brk) {
-                int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
-                T s0 = kk[i0] == (T)0 ? (T)0 : (kk[i0] > (T)0 ? (T)1 : (T)-1);
-                T s1 = kk[i1] == (T)0 ? (T)0 : (kk[i1] > (T)0 ? (T)1 : (T)-1);
-                xl[e] = cmul_i<-1>(cscale(ehl[e], s0));
-                xh[e] = cmul_i<-1>(cscale(ehh[e], s1));
-            } else {
-                xl[e] = cmk<C>((T)0, (T)0);
-                xh[e] = xl[e];
-            }
-        }
-        P::inv_pair(ml, mh, xl, xh, m0l, m0h, hl, hh, lds, tw, tid);
-    }
-    P::str(sr + R1_M0U * N, m0l, m0h, tid);
-    if (R.brk) P::str(sr + R1_H * N, hl, hh, tid);
-    // ---- products: F(eta M0u), F(u^2 - M0u^2), F(eta u) for flat
-    C eml[EH], emh[EH], u2l[EH], u2h[EH];
-    {
-        T pl[EH], ph[EH], ql[EH], qh[EH];
-#pragma unroll
-        for (int e = 0; e < EH; ++e) {
-            pl[e] = etl[e] * m0l[e]; ph[e] = eth[e] * m0h[e];
-            ql[e] = ul[e] * ul[e] - m0l[e] * m0l[e]; qh[e] = uh_[e] * uh_[e] - m0h[e] * m0h[e];
-        }
-        P::str(sr + R1_P2 * N, pl, ph, tid);
+        P::inv_in(lds, tid, e, m0l, m0h, xl, xh);
+    };
+#pragma unroll 1
+    for (int e = 0; e < EH; ++e) {
+        const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+        T etl, eth, ul, uh;
+        P::inv_out(lds, tid, e, etl, eth, ul, uh);
+        s_eta[i0] = etl; s_eta[i1] = eth;
+        s_u[i0] = ul; s_u[i1] = uh;
+        const C u0 = c_uh[i0], u1 = c_uh[i1];
         if (!R.flat) {
-            P::fwd_pair(pl, ph, ql, qh, eml, emh, u2l, u2h, lds, tw, tid);
+            P::inv_in(lds, tid, e, cscale(u0, opa[i0]), cscale(u1, opa[i1]), cscale(u0, opb[i0]), cscale(u1, opb[i1]));
         } else {
-            T el[EH], eh2[EH];
-#pragma unroll
-            for (int e = 0; e < EH; ++e) { el[e] = etl[e] * ul[e]; eh2[e] = eth[e] * uh_[e]; }
-            C al_[EH], ah_[EH];
-            P::fwd_pair(el, eh2, pl, ph, al_, ah_, eml, emh, lds, tw, tid);
-            P::stc(sc + C1_EUH * N, al_, ah_, tid);
-#pragma unroll
-            for (int e = 0; e < EH; ++e) { el[e] = (T)0; eh2[e] = (T)0; }
-            P::fwd_pair(ql, qh, el, eh2, u2l, u2h, al_, ah_, lds, tw, tid);
+            const C m0l = cmul_i<-1>(cscale(u0, kk[i0] * opa[i0] / g));
+            const C m0h = cmul_i<-1>(cscale(u1, kk[i1] * opa[i1] / g));
+            c_m0h[i0] = m0l; c_m0h[i1] = m0h;
+            hilbert_in(e, m0l, m0h);
         }
     }
-    // ---- L(eta M0u): k^2 HSS(F(eta M0u))/g (two depths) or Lop F(eta M0u) (flat)
-    C lel[EH], leh[EH];
     if (!R.flat) {
-        C xl[EH], xh[EH], yl[EH], yh[EH];
-#pragma unroll
+        P::template run<1>(lds, tw, tid);
+        // (a-, a+) = F^-1(C2-+ u^); F(g- a- + g+ a+), F(g- u)
+#pragma unroll 1
         for (int e = 0; e < EH; ++e) {
-            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
-            xl[e] = cscale(eml[e], a.opa[off + i0]); xh[e] = cscale(emh[e], a.opa[off + i1]);
-            yl[e] = cscale(eml[e], a.opb[off + i0]); yh[e] = cscale(emh[e], a.opb[off + i1]);
+            const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            T aml, amh, apl, aph;
+            P::inv_out(lds, tid, e, aml, amh, apl, aph);
+            const T g0 = gmv[i0], g1 = gmv[i1], q0 = gpv[i0], q1 = gpv[i1];
+            P::fwd_in(lds, tid, e, g0 * aml + q0 * apl, g1 * amh + q1 * aph, g0 * s_u[i0], g1 * s_u[i1]);
         }
-        T bml[EH], bmh[EH], bpl[EH], bph[EH];
-        P::inv_pair(xl, xh, yl, yh, bml, bmh, bpl, bph, lds, tw, tid);
-        T pl[EH], ph[EH];
-        P::ldr(sr + R1_P2 * N, pl, ph, tid);
-        T f1l[EH], f1h[EH], f2l[EH], f2h[EH];
-#pragma unroll
+        P::template run<-1>(lds, tw, tid);
+        // HP = (F(g- a- + g+ a+) + C- F(g- u)) / 2; then F(g+ u), F(eta u)
+#pragma unroll 1
         for (int e = 0; e < EH; ++e) {
-            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
-            T g0 = a.gm[off + i0], g1 = a.gm[off + i1], q0 = a.gp[off + i0], q1 = a.gp[off + i1];
-            f1l[e] = g0 * bml[e] + q0 * bpl[e]; f1h[e] = g1 * bmh[e] + q1 * bph[e];
-            f2l[e] = g0 * pl[e]; f2h[e] = g1 * ph[e];
+            const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            C xl, xh, yl, yh;
+            P::fwd_out(lds, tid, e, xl, xh, yl, yh);
+            c_hp[i0] = cscale(cadd(xl, cscale(yl, opa[i0])), (T)0.5);
+            c_hp[i1] = cscale(cadd(xh, cscale(yh, opa[i1])), (T)0.5);
+            const T u0 = s_u[i0], u1 = s_u[i1];
+            P::fwd_in(lds, tid, e, gpv[i0] * u0, gpv[i1] * u1, s_eta[i0] * u0, s_eta[i1] * u1);
         }
-        P::fwd_pair(f1l, f1h, f2l, f2h, xl, xh, yl, yh, lds, tw, tid);
-#pragma unroll
+        P::template run<-1>(lds, tw, tid);
+#pragma unroll 1
         for (int e = 0; e < EH; ++e) {
-            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
-            lel[e] = cscale(cadd(xl[e], cscale(yl[e], a.opa[off + i0])), (T)0.5);
-            leh[e] = cscale(cadd(xh[e], cscale(yh[e], a.opa[off + i1])), (T)0.5);
-            f1l[e] = a.gp[off + i0] * pl[e]; f1h[e] = a.gp[off + i1] * ph[e];
-            f2l[e] = (T)0; f2h[e] = (T)0;
+            const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            C xl, xh, yl, yh;
+            P::fwd_out(lds, tid, e, xl, xh, yl, yh);
+            c_euh[i0] = yl; c_euh[i1] = yh;   // F(eta u)
+            const C H0 = cadd(c_hp[i0], cscale(xl, (T)0.5 * opb[i0]));
+            const C H1 = cadd(c_hp[i1], cscale(xh, (T)0.5 * opb[i1]));
+            const C m0l = cmul_i<-1>(cscale(H0, kk[i0] / g));
+            const C m0h = cmul_i<-1>(cscale(H1, kk[i1] / g));
+            c_m0h[i0] = m0l; c_m0h[i1] = m0h;
+            hilbert_in(e, m0l, m0h);
         }
-        P::fwd_pair(f1l, f1h, f2l, f2h, xl, xh, yl, yh, lds, tw, tid);
-#pragma unroll
+    }
+    P::template run<1>(lds, tw, tid);
+    // ---- M0u and H(eta); products F(eta M0u), F(u^2 - M0u^2), F(eta u) (flat)
+#pragma unroll 1
+    for (int e = 0; e < EH; ++e) {
+        const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+        T m0l, m0h, hl, hh;
+        P::inv_out(lds, tid, e, m0l, m0h, hl, hh);
+        s_m0[i0] = m0l; s_m0[i1] = m0h;
+        if (R.brk) { sr[R1_H * N + i0] = hl; sr[R1_H * N + i1] = hh; }
+        const T pl = s_eta[i0] * m0l, ph = s_eta[i1] * m0h;
+        s_p[i0] = pl; s_p[i1] = ph;
+        if (!R.flat) {
+            const T u0 = s_u[i0], u1 = s_u[i1];
+            P::fwd_in(lds, tid, e, pl, ph, u0 * u0 - m0l * m0l, u1 * u1 - m0h * m0h);
+        } else {
+            P::fwd_in(lds, tid, e, s_eta[i0] * s_u[i0], s_eta[i1] * s_u[i1], pl, ph);
+        }
+    }
+    P::template run<-1>(lds, tw, tid);
+    // ---- L(eta M0u): k^2 HSS(F(eta M0u))/g (two depths) or Lop F(eta M0u) (flat),
+    // left in c_hp; u2 = F(u^2 - M0u^2) in c_u2h
+    if (!R.flat) {
+#pragma unroll 1
         for (int e = 0; e < EH; ++e) {
-            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
-            T k0 = kk[i0], k1 = kk[i1];
-            lel[e] = cscale(cadd(lel[e], cscale(xl[e], (T)0.5 * a.opb[off + i0])), k0 * k0 / g);
-            leh[e] = cscale(cadd(leh[e], cscale(xh[e], (T)0.5 * a.opb[off + i1])), k1 * k1 / g);
+            const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            C eml, emh, u2l, u2h;
+            P::fwd_out(lds, tid, e, eml, emh, u2l, u2h);
+            c_u2h[i0] = u2l; c_u2h[i1] = u2h;
+            P::inv_in(lds, tid, e, cscale(eml, opa[i0]), cscale(emh, opa[i1]), cscale(eml, opb[i0]), cscale(emh, opb[i1]));
         }
+        P::template run<1>(lds, tw, tid);
+#pragma unroll 1
+        for (int e = 0; e < EH; ++e) {
+            const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            T bml, bmh, bpl, bph;
+            P::inv_out(lds, tid, e, bml, bmh, bpl, bph);
+            const T g0 = gmv[i0], g1 = gmv[i1], q0 = gpv[i0], q1 = gpv[i1];
+            P::fwd_in(lds, tid, e, g0 * bml + q0 * bpl, g1 * bmh + q1 * bph, g0 * s_p[i0], g1 * s_p[i1]);
+        }
+        P::template run<-1>(lds, tw, tid);
+#pragma unroll 1
+        for (int e = 0; e < EH; ++e) {
+            const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            C xl, xh, yl, yh;
+            P::fwd_out(lds, tid, e, xl, xh, yl, yh);
+            c_hp[i0] = cscale(cadd(xl, cscale(yl, opa[i0])), (T)0.5);
+            c_hp[i1] = cscale(cadd(xh, cscale(yh, opa[i1])), (T)0.5);
+            P::fwd_in(lds, tid, e, gpv[i0] * s_p[i0], gpv[i1] * s_p[i1], (T)0, (T)0);
+        }
+        P::template run<-1>(lds, tw, tid);
     } else {
-#pragma unroll
+#pragma unroll 1
         for (int e = 0; e < EH; ++e) {
-            lel[e] = cscale(eml[e], a.opb[off + P::kl(tid, e)]);
-            leh[e] = cscale(emh[e], a.opb[off + P::kh(tid, e)]);
+            const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            C eul, euh, eml, emh;
+            P::fwd_out(lds, tid, e, eul, euh, eml, emh);
+            c_euh[i0] = eul; c_euh[i1] = euh;
+            c_hp[i0] = cscale(eml, opb[i0]);
+            c_hp[i1] = cscale(emh, opb[i1]);
+            const T u0 = s_u[i0], u1 = s_u[i1], m0 = s_m0[i0], m1 = s_m0[i1];
+            P::fwd_in(lds, tid, e, u0 * u0 - m0 * m0, u1 * u1 - m1 * m1, (T)0, (T)0);
         }
+        P::template run<-1>(lds, tw, tid);
     }
     // ---- delH2 terms (1d/rhs.c:141-147) and their adjustment (adjust_delH)
-    T aul[EH], auh[EH], ael[EH], aeh[EH];
-    {
-        C eul[EH], euh[EH];
-        P::ldc(sc + C1_EUH * N, eul, euh, tid);
-        C dul[EH], duh[EH], del_[EH], deh[EH];
-#pragma unroll
-        for (int e = 0; e < EH; ++e) {
-            T k0 = kk[P::kl(tid, e)], k1 = kk[P::kh(tid, e)];
-            dul[e] = csub(cmk<C>(-lel[e].x, -lel[e].y), cmul_i<1>(cscale(eul[e], k0)));
-            duh[e] = csub(cmk<C>(-leh[e].x, -leh[e].y), cmul_i<1>(cscale(euh[e], k1)));
-            del_[e] = cscale(cmul_i<1>(cscale(u2l[e], k0)), (T)-0.5);
-            deh[e] = cscale(cmul_i<1>(cscale(u2h[e], k1)), (T)-0.5);
+#pragma unroll 1
+    for (int e = 0; e < EH; ++e) {
+        const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+        const T k0 = kk[i0], k1 = kk[i1];
+        C lel, leh, u2l, u2h;
+        if (!R.flat) {
+            C xl, xh, yl, yh;
+            P::fwd_out(lds, tid, e, xl, xh, yl, yh);
+            lel = cscale(cadd(c_hp[i0], cscale(xl, (T)0.5 * opb[i0])), k0 * k0 / g);
+            leh = cscale(cadd(c_hp[i1], cscale(xh, (T)0.5 * opb[i1])), k1 * k1 / g);
+            u2l = c_u2h[i0]; u2h = c_u2h[i1];
+        } else {
+            C yl, yh;
+            P::fwd_out(lds, tid, e, u2l, u2h, yl, yh);
+            lel = c_hp[i0]; leh = c_hp[i1];
         }
-        P::inv_pair(dul, duh, del_, deh, aul, auh, ael, aeh, lds, tw, tid);
-#pragma unroll
-        for (int e = 0; e < EH; ++e) {
-            T c0 = a.chi[off + P::kl(tid, e)], c1 = a.chi[off + P::kh(tid, e)];
-            aul[e] *= c0; auh[e] *= c1; ael[e] *= c0; aeh[e] *= c1;
-        }
+        const C eul = c_euh[i0], euh = c_euh[i1];
+        const C dul = csub(cmk<C>(-lel.x, -lel.y), cmul_i<1>(cscale(eul, k0)));
+        const C duh = csub(cmk<C>(-leh.x, -leh.y), cmul_i<1>(cscale(euh, k1)));
+        const C del_ = cscale(cmul_i<1>(cscale(u2l, k0)), (T)-0.5);
+        const C deh = cscale(cmul_i<1>(cscale(u2h, k1)), (T)-0.5);
+        P::inv_in(lds, tid, e, dul, duh, del_, deh);
+    }
+    P::template run<1>(lds, tw, tid);
+#pragma unroll 1
+    for (int e = 0; e < EH; ++e) {
+        const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+        T aul, auh, ael, aeh;
+        P::inv_out(lds, tid, e, aul, auh, ael, aeh);
+        const T c0 = a.chi[off + i0], c1 = a.chi[off + i1];
+        s_au[i0] = aul * c0; s_au[i1] = auh * c1;
+        s_ae[i0] = ael * c0; s_ae[i1] = aeh * c1;
     }
     // ---- breaking source Rb (rhs_breaking, 1d/rhs.c:617-657)
-    T rbl[EH], rbh[EH];
-#pragma unroll
-    for (int e = 0; e < EH; ++e) { rbl[e] = (T)0; rbh[e] = (T)0; }
+    T* Bp = a.B + off;
+    T* flux = sr + R1_FLUX * N;
     if (R.brk) {
-        T* Bp = a.B + off;
-        const T* etap = sr + R1_ETA * N;
-        brk1d_process<T, N, NT>(a, R, t, etap, sr + R1_U * N, sr + R1_H * N, Bp, sm, smd);
-        T* flux = sr + R1_FLUX * N;
+        brk1d_process<T, N, NT>(a, R, t, s_eta, s_u, sr + R1_H * N, Bp, sm, smd);
         const T db2 = (T)(R.delb * R.delb);
-#pragma unroll
+#pragma unroll 1
         for (int e = 0; e < EH; ++e) {
-            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
-            T d0 = m0l[e] + aul[e], d1 = m0h[e] + auh[e];   // Re F^-1(deluH12)
-            flux[i0] = -Bp[i0] * db2 * (a.depth[off + i0] + etl[e]) * d0 * d0;
-            flux[i1] = -Bp[i1] * db2 * (a.depth[off + i1] + eth[e]) * d1 * d1;
+            const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            const T d0 = s_m0[i0] + s_au[i0], d1 = s_m0[i1] + s_au[i1];   // Re F^-1(deluH12)
+            flux[i0] = -Bp[i0] * db2 * (a.depth[off + i0] + s_eta[i0]) * d0 * d0;
+            flux[i1] = -Bp[i1] * db2 * (a.depth[off + i1] + s_eta[i1]) * d1 * d1;
         }
         __syncthreads();
-        const T idx1 = (T)(1.0 / R.dx), idx2 = (T)(0.5 / R.dx);
-        auto gradf = [&](int i) -> T {
-            if (i == 0) return (flux[1] - flux[0]) * idx1;
-            if (i == N - 1) return (flux[N - 1] - flux[N - 2]) * idx1;
-            return (flux[i + 1] - flux[i - 1]) * idx2;
-        };
-#pragma unroll
-        for (int e = 0; e < EH; ++e) {
-            int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
-            rbl[e] = gradf(i0) / (a.depth[off + i0] + etl[e]);
-            rbh[e] = gradf(i1) / (a.depth[off + i1] + eth[e]);
-        }
     }
+    const T idx1 = (T)(1.0 / R.dx), idx2 = (T)(0.5 / R.dx);
+    auto gradf = [&](int i) -> T {
+        if (i == 0) return (flux[1] - flux[0]) * idx1;
+        if (i == N - 1) return (flux[N - 1] - flux[N - 2]) * idx1;
+        return (flux[i + 1] - flux[i - 1]) * idx2;
+    };
     // ---- output transforms with damping folded in, forcing, mask
-    {
-        C oel[EH], oeh[EH], oul[EH], ouh[EH];
-#pragma unroll
-        for (int e = 0; e < EH; ++e) {
-            T d0 = a.damp[off + P::kl(tid, e)], d1 = a.damp[off + P::kh(tid, e)];
-            aul[e] = aul[e] - d0 * etl[e];
-            auh[e] = auh[e] - d1 * eth[e];
-            ael[e] = ael[e] - d0 * ul[e] + rbl[e];
-            aeh[e] = aeh[e] - d1 * uh_[e] + rbh[e];
+#pragma unroll 1
+    for (int e = 0; e < EH; ++e) {
+        const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+        const T d0 = a.damp[off + i0], d1 = a.damp[off + i1];
+        T rbl = (T)0, rbh = (T)0;
+        if (R.brk) {
+            rbl = gradf(i0) / (a.depth[off + i0] + s_eta[i0]);
+            rbh = gradf(i1) / (a.depth[off + i1] + s_eta[i1]);
         }
-        P::fwd_pair(aul, auh, ael, aeh, oel, oeh, oul, ouh, lds, tw, tid);
-        C m0hl[EH], m0hh[EH];
-        P::ldc(sc + C1_M0H * N, m0hl, m0hh, tid);
+        P::fwd_in(lds, tid, e, s_au[i0] - d0 * s_eta[i0], s_au[i1] - d1 * s_eta[i1],
+                  s_ae[i0] - d0 * s_u[i0] + rbl, s_ae[i1] - d1 * s_u[i1] + rbh);
+    }
+    P::template run<-1>(lds, tw, tid);
+    {
         const T pv = (T)smd[30], pvs = (T)smd[31];
         C* Ko = (stage == 0 || stage == 7) ? a.K[6] : a.K[stage - 1];
         Ko += 2 * off;
-#pragma unroll
+#pragma unroll 1
         for (int e = 0; e < EH; ++e) {
+            C oel, oeh, oul, ouh;
+            P::fwd_out(lds, tid, e, oel, oeh, oul, ouh);
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
                 const int i = q == 0 ? P::kl(tid, e) : P::kh(tid, e);
@@ -791,12 +791,10 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
                     else if (R.prop == 2) fo = cmk<C>((T)2 * pv * gx.x, (T)2 * pv * gx.y);
                 }
                 const T al = a.aal[off + i];
-                C m0 = q == 0 ? m0hl[e] : m0hh[e];
-                C oe = q == 0 ? oel[e] : oeh[e];
-                C ou = q == 0 ? oul[e] : ouh[e];
-                C ehv = q == 0 ? ehl[e] : ehh[e];
-                C ke = cscale(cadd(cadd(m0, fo), oe), al);
-                C ku = cscale(cadd(cmul_i<-1>(cscale(ehv, g * kk[i])), ou), al);
+                const C oe = q == 0 ? oel : oeh;
+                const C ou = q == 0 ? oul : ouh;
+                C ke = cscale(cadd(cadd(c_m0h[i], fo), oe), al);
+                C ku = cscale(cadd(cmul_i<-1>(cscale(c_eh[i], g * kk[i])), ou), al);
                 Ko[i] = ke;
                 Ko[N + i] = ku;
             }
