@@ -31,6 +31,7 @@ class HwConfig2D(ct.Structure):
         ("max_eta_init", ct.c_double), ("tstar", ct.c_double), ("rsearch", ct.c_double), ("current", ct.c_double),
         ("pbchar", _dp), ("depth", _dp), ("x", _dp), ("y", _dp),
         ("t0", ct.c_double),
+        ("model", ct.c_int),
     ]
 
 

@@ -843,4 +843,74 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves
     }
 }
 
+// ========================================================== HS1 kernels ==
+// rhs_AB1_2DB_uv / rhs_AB1_2DF_uv (2d/fun_odesolver.c:4350-4456), the linear
+// model: d eta^/dt = aal (S^ + delphiH1 - F(7 Cp fbdy eta)),
+// d u^/dt = aal (-i g kx eta^ - F(7 Cp fbdy u)), v likewise; no breaking.
+// Two-depth: Y1, X1 and Y2 of the HS2 pipeline give delphiH1 (dH1) and the
+// physical eta, u, v; flat: the Y1 of the flat pipeline gives the x-
+// intermediates of eta^, u^, v^ and delphiH1 is pointwise.
+enum : int { H1_ZE = 13, H1_ZU = 14, H1_ZV = 15 };
+
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k1_x(Rhs2DArgs<T> a, int from_phys) {
+    HW_XK_PROLOGUE
+    // forward x of source - fbdy7 eta, -fbdy7 u, -fbdy7 v
+    C f[G::PP], d[G::PP];
+    ph_load<T, NX, NY>(d, a.fbdy7, y0, tid);
+    #pragma unroll 1
+    for (int c = 0; c < 3; ++c) {
+        if (from_phys) ph_load<T, NX, NY>(f, c == 0 ? a.eta_p : (c == 1 ? a.u_p : a.v_p), y0, tid);
+        else x_inverse<T, NX, NY>(a.W[c == 0 ? F_ETA : (c == 1 ? F_U : F_V)], y0, lds, twxs, f, sc, nullptr, tid);
+#pragma unroll
+        for (int q = 0; q < G::PP; ++q) f[q] = cmk<C>(-d[q].x * f[q].x, -d[q].y * f[q].y);
+        if (c == 0 && a.src_on) {
+            C ng[G::PP], sg[G::PP];
+            ph_load<T, NX, NY>(ng, a.new_gam, y0, tid);
+            ph_load<T, NX, NY>(sg, a.skew_gam, y0, tid);
+#pragma unroll
+            for (int q = 0; q < G::PP; ++q) {
+                T s0, s1;
+                src_pair<T, NX>(a, y0, G::ppos(tid, q), ng[q], sg[q], s0, s1);
+                f[q] = cadd(f[q], cmk<C>(s0, s1));
+            }
+        }
+        x_fwd_store<T, NX, NY>(f, a.W[H1_ZE + c], y0, lds, twxs, tid);
+    }
+}
+
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__((YGeo<T, NY>::NT)) k1_y(Rhs2DArgs<T> a, int flat) {
+    HW_YK_PROLOGUE
+    C Z[G::PE], eh[G::PE];
+    y_load<T, NX, NY>(eh, a.eta_hat, nullptr, col0, tid);
+    C* outs[3] = {a.k_eta, a.k_u, a.k_v};
+    #pragma unroll 1
+    for (int c = 0; c < 3; ++c) {
+        y_load<T, NX, NY>(Z, a.W[H1_ZE + c], nullptr, col0, tid);
+        y_fwd_regs<T, NX, NY>(Z, lds, twys, tid);
+#pragma unroll
+        for (int e = 0; e < G::PE; ++e) {
+            int kx, ky; size_t gi;
+            YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
+            if (kx < NX / 2 + 1) {
+                C lin;
+                if (c == 0) {
+                    if (flat) {
+                        // delphiH1 = -i C2d^2/g (kx u^ + ky v^)  (2d/fun_odesolver.c:2223-2236)
+                        C s = cadd(cscale(a.u_hat[gi], a.kxh[kx]), cscale(a.v_hat[gi], a.ky[ky]));
+                        lin = cmul_i<-1>(cscale(s, a.c2d2g[gi]));
+                    } else {
+                        lin = a.dH1[gi];
+                    }
+                } else {
+                    lin = cmul_i<-1>(cscale(eh[e], a.g * (c == 1 ? a.kxh[kx] : a.ky[ky])));
+                }
+                Z[e] = cscale(cadd(lin, Z[e]), a.aal[gi]);
+            }
+        }
+        y_store<T, NX, NY>(outs[c], Z, col0, tid);
+    }
+}
+
 }  // namespace hw

@@ -173,6 +173,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
     std::vector<CrestRec> cbprev;          // CBPrev entries (count = size)
     long brk_calls = 0;
     bool flat = false;
+    bool hs1 = false;   // HS1 dynamics (linear AB1 right-hand side)
     int src_on = 0;
     double g = 9.80665, eps_abs = 1e-3, eps_rel = 1e-2;
     // HIP events bracketing every RHS evaluation on the context's stream
@@ -209,6 +210,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
     int init(const hw_config_2d* c) override {
         nx = NX; ny = NY; prec = c->precision;
         flat = c->flat != 0;
+        hs1 = c->model == 1;
         g = c->grav; eps_abs = c->eps_abs; eps_rel = c->eps_rel;
         HW_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         for (int i = 0; i < NEV; ++i) {
@@ -274,7 +276,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
             if ((rc = newgam.upload(to_phys(c->new_gam), stream)) || (rc = skewgam.upload(to_phys(c->skew_gam), stream))) return rc;
         }
         tprev = c->t0;
-        brk_on = c->breaking_on;
+        brk_on = c->breaking_on && !hs1;   // rhs_AB1_* never calls breaking_process
         if ((rc = s_eta.alloc(NY))) return rc;
         if (brk_on) {
             if (!c->pbchar || !c->depth || !c->x || !c->y || c->rsearch <= 0) {
@@ -382,7 +384,18 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         // 2d/fun_odesolver.c:4580-4587: breaking runs only when tprev < t
         const bool brk_now = brk_on && tprev < t;
         args.eta_col0 = brk_now ? col0.p : nullptr;
-        if (!flat) {
+        if (hs1) {
+            // rhs_AB1_2DB_uv / rhs_AB1_2DF_uv (2d/fun_odesolver.c:4350-4456)
+            if (!flat) {
+                k2b_y1<T, NX, NY><<<gy, by, 0, stream>>>(args);
+                k2b_x1<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+                k2b_y2<T, NX, NY><<<gy, by, 0, stream>>>(args);
+            } else {
+                k2f_y1<T, NX, NY><<<gy, by, 0, stream>>>(args);
+            }
+            k1_x<T, NX, NY><<<gx, bx, 0, stream>>>(args, flat ? 0 : 1);
+            k1_y<T, NX, NY><<<gy, by, 0, stream>>>(args, flat ? 1 : 0);
+        } else if (!flat) {
             k2b_y1<T, NX, NY><<<gy, by, 0, stream>>>(args);
             k2b_x1<T, NX, NY><<<gx, bx, 0, stream>>>(args);
             if (brk_now) { int rc = breaking_detect(); if (rc) return rc; }

@@ -84,6 +84,7 @@ class Hawassi2D:
             c.depth = arr("depth", model.depth)
             c.x, c.y = arr("x", model.x), arr("y", model.y)
         c.t0 = float(model.meta.get("t0", 0.0))
+        c.model = int(model.order)
         self._cfg, self._keep = c, keep
         h = ct.c_void_p()
         _lib.check(self.lib.hw_ctx_create_2d(ct.byref(c), ct.byref(h)), "hw_ctx_create_2d")

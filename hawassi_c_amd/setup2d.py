@@ -498,6 +498,7 @@ class Model2D:
     eps_abs: float = 1e-3
     eps_rel: float = 1e-2
     breaking: dict | None = None        # see breaking_params()
+    order: int = 2                      # 1: HS1 (rhs_AB1_2D*_uv, linear), 2: HS2 (rhs_AB2_2D*_uv)
     meta: dict = field(default_factory=dict)
 
     @property

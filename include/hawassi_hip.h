@@ -79,6 +79,9 @@ typedef struct hw_config_2d {
     const double *depth;              /* dominfo.bathy.pos [ny*nx] (positive depth) */
     const double *x, *y;              /* dominfo.x [nx], dominfo.y [ny] */
     double t0;                        /* initial tprev (dominfo.t[0], 2d/main.c:120) */
+    int model;                        /* dynamics: 1 "HS1" (rhs_AB1_2DF_uv / rhs_AB1_2DB_uv,
+                                         2d/fun_odesolver.c:4350-4456: linear, no breaking call);
+                                         2 or 0 "HS2" (rhs_AB2_2D*_uv); 2d/main.c:209-224 */
 } hw_config_2d;
 
 typedef struct hw_stats {

@@ -105,6 +105,7 @@ typedef struct {
     const double *pbchar;                 /* pb.Char [Ny][Nx] (0/1) */
     const double *depth;                  /* dominfo.bathy.pos [Ny][Nx] (positive depth) */
     const double *x, *y;                  /* dominfo.x [Nx], dominfo.y [Ny] */
+    int order;                            /* 1: HS1 (rhs_AB1_2D*_uv), else HS2 (rhs_AB2_2D*_uv) */
 } orc_params;
 
 typedef struct { int x, y; } orc_v2i;
@@ -747,6 +748,48 @@ int orc_rhs(void *vc, double t, const double *Z, double *dZ)
             Lphihat[2 * q] = duH1[2 * q] + dvH1[2 * q];
             Lphihat[2 * q + 1] = duH1[2 * q + 1] + dvH1[2 * q + 1];
         }
+    }
+
+    if (p->order == 1) {
+        /* ---- HS1: rhs_AB1_2DB_uv (2d/fun_odesolver.c:4350-4402) / rhs_AB1_2DF_uv
+           (:4404-4456): the linear terms, damping and source only; no
+           breaking call, so Sb stays zero.  The flat variant's damping comes
+           from fun_termshat_AB1_2DF_uv (:2962-3027). ---- */
+        if (p->flat) {
+            memcpy(eta, etahat, sizeof(double) * 2 * N); memcpy(u, uhat, sizeof(double) * 2 * N);
+            memcpy(v, vhat, sizeof(double) * 2 * N);
+            orc_hermitian_project(eta, Nx, Ny); orc_hermitian_project(u, Nx, Ny); orc_hermitian_project(v, Nx, Ny);
+            fft2(&c->P, eta, +1); fft2(&c->P, u, +1); fft2(&c->P, v, +1);
+            #pragma omp parallel for schedule(static)
+            for (size_t q = 0; q < N; q++) {
+                dampeta[2 * q] = 7 * p->Cpeak * p->fbdy[q] * eta[2 * q] / (double)N; dampeta[2 * q + 1] = 0;
+                dampu[2 * q] = 7 * p->Cpeak * p->fbdy[q] * u[2 * q] / (double)N; dampu[2 * q + 1] = 0;
+                dampv[2 * q] = 7 * p->Cpeak * p->fbdy[q] * v[2 * q] / (double)N; dampv[2 * q + 1] = 0;
+            }
+            fft2(&c->P, dampeta, -1); fft2(&c->P, dampu, -1); fft2(&c->P, dampv, -1);
+        }
+        double *Shat1 = cal(N);
+        source_hat(c, t, Shat1);
+        #pragma omp parallel for schedule(static)
+        for (int i = 0; i < Ny; i++)
+            for (int j = 0; j < Nx; j++) {
+                size_t q = (size_t)i * Nx + j;
+                double a = p->aal[q];
+                double dr = p->flat ? delphiH1[2 * q] : duH1[2 * q] + dvH1[2 * q];
+                double di = p->flat ? delphiH1[2 * q + 1] : duH1[2 * q + 1] + dvH1[2 * q + 1];
+                dZ[q] = (Shat1[2 * q] + dr - dampeta[2 * q]) * a;
+                dZ[N + q] = (Shat1[2 * q + 1] + di - dampeta[2 * q + 1]) * a;
+                dZ[2 * N + q] = (g * kx[j] * Z[N + q] - dampu[2 * q]) * a;
+                dZ[3 * N + q] = (-g * kx[j] * Z[q] - dampu[2 * q + 1]) * a;
+                dZ[4 * N + q] = (g * ky[i] * Z[N + q] - dampv[2 * q]) * a;
+                dZ[5 * N + q] = (-g * ky[i] * Z[q] - dampv[2 * q + 1]) * a;
+            }
+        free(Shat1);
+        free(eta); free(u); free(v); free(dampeta); free(dampu); free(dampv);
+        free(ikxE_adj); free(ikyE_adj); free(duH2_adj); free(dvH2_adj); free(delphiH1);
+        free(duH1); free(dvH1); free(etahat); free(uhat); free(vhat);
+        free(Lphihat); free(M0uhat); free(M0vhat);
+        return 0;
     }
 
     /* ---- shared AB2 body (2DF:3182-3303, 2DB:3372-3487) ---- */

@@ -35,6 +35,7 @@ class OrcParams(ct.Structure):
         ("KBC", ct.c_double), ("TC", ct.c_double), ("Tchar", ct.c_double), ("delb", ct.c_double),
         ("MaxEtaInit", ct.c_double), ("Tstar", ct.c_double), ("Rsearch", ct.c_double), ("U", ct.c_double),
         ("pbchar", _dp), ("depth", _dp), ("x", _dp), ("y", _dp),
+        ("order", ct.c_int),
     ]
 
 
@@ -128,6 +129,7 @@ class Oracle2D:
 
         p = OrcParams()
         p.Nx, p.Ny, p.flat, p.wall, p.refl_one = model.nx, model.ny, int(model.flat), int(model.wall), int(model.refl_one)
+        p.order = int(getattr(model, "order", 2))
         p.Cpeak = model.cpeak
         p.kx, p.ky = arr("kx", model.kx), arr("ky", model.ky)
         p.aal, p.fbdy = arr("aal", model.aal), arr("fbdy", model.fbdy)

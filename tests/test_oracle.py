@@ -211,3 +211,68 @@ def test_evolve_apply_step_control():
     # final-step clipping: request past t1
     rc, t2, h2, st2 = o.evolve_apply(t, t + 1e-4, 1.0, y, 1e-3, 1e-2)
     assert rc == 0 and t2 == t + 1e-4
+
+
+# ----------------------------------------------------------------- HS1 ---
+def test_hs1_single_mode_any_amplitude():
+    """HS1 (rhs_AB1_2DF_uv) is the linear model: a single Fourier mode
+    oscillates at omega(k) of the exact dispersion relation at any amplitude
+    (HS2 does so only in the small-amplitude limit)."""
+    m = linear_flat_model()
+    m.order = 1
+    n = m.nx
+    p, q = 2, 3
+    eta0 = single_mode_state(m, p, q, 0.5)
+    from hawassi_c_amd.model2d import initial_state
+    y = initial_state(eta0)
+    kk = math.hypot(m.kx[q], m.ky[p])
+    w = math.sqrt(S.GRAV * kk * math.tanh(kk * 5.0))
+    T = 2.7
+    rc, t, h, st = O.Oracle2D(m).integrate(0.0, T, 0.01, y, 1e-14, 1e-10)
+    assert rc == 0 and abs(t - T) < 1e-12
+    e0 = np.fft.fft2(eta0)[p, q]
+    eT = (y[: n * n] + 1j * y[n * n: 2 * n * n]).reshape(n, n)[p, q]
+    assert abs(eT / e0 - math.cos(w * T)) < 1e-7
+    # the same amplitude under HS2 leaves the linear solution
+    m2 = linear_flat_model()
+    y2 = initial_state(eta0)
+    O.Oracle2D(m2).integrate(0.0, T, 0.01, y2, 1e-14, 1e-10)
+    e2 = (y2[: n * n] + 1j * y2[n * n: 2 * n * n]).reshape(n, n)[p, q]
+    assert abs(e2 / e0 - math.cos(w * T)) > 1e-4
+
+
+@pytest.mark.parametrize("kind", ["short_crested", "harbour"])
+def test_hs1_rhs_is_affine(kind):
+    """HS1 = linear operator + source: R(a y1 + b y2) - R(0) equals
+    a (R(y1) - R(0)) + b (R(y2) - R(0)), with the influx source on, flat
+    and two-depth; HS2 is not."""
+    m = getattr(W, kind)(32, t_end=3.0, **({"breaking": False} if kind == "harbour" else {}))
+    n = m.nx
+    y1, y2 = W.random_state(m, 3, kfrac=0.5), W.random_state(m, 4, kfrac=0.5)
+    z = np.zeros(6 * n * n)
+    for order, linear in ((1, True), (2, False)):
+        m.order = order
+        o = O.Oracle2D(m)
+        r0, r1, r2 = o.rhs(1.3, z), o.rhs(1.3, y1), o.rhs(1.3, y2)
+        r12 = o.rhs(1.3, 0.7 * y1 - 1.9 * y2)
+        lhs, rhs = r12 - r0, 0.7 * (r1 - r0) - 1.9 * (r2 - r0)
+        e = relerr(HS.oracle_half(lhs, n, n), HS.oracle_half(rhs, n, n))
+        assert (e < 1e-10) if linear else (e > 1e-6)
+
+
+def test_hs1_two_depth_with_unit_weights_equals_flat():
+    """As for HS2: with gamma_- = 1, gamma_+ = 0 and csq_- = C^2, the two-depth
+    HS1 right-hand side is the flat one at every spectral point the
+    projection does not rebuild."""
+    mf = W.short_crested(32, t_end=3.0)
+    mf.order = 1
+    n = mf.nx
+    one, zero = np.ones((n, n)), np.zeros((n, n))
+    ops = dict(gam_min=one, gam_plus=zero, csq_min=mf.c2d ** 2, csq_plus=mf.c2d ** 2 * 0.5,
+               om2_min=S.GRAV * mf.l2d, om2_plus=S.GRAV * mf.l2d * 0.5)
+    mb = S.Model2D(**{**mf.__dict__, "flat": False, "ops": ops})
+    y = W.random_state(mf, 11, kfrac=0.5)
+    df = HS.oracle_half(O.Oracle2D(mf).rhs(0.7, y), n, n)
+    db = HS.oracle_half(O.Oracle2D(mb).rhs(0.7, y), n, n)
+    assert relerr(db[:, :, 1:], df[:, :, 1:]) < 1e-10
+    assert relerr(db[:, : n // 2 + 1, 0], df[:, : n // 2 + 1, 0]) < 1e-10
