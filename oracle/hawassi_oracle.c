@@ -105,7 +105,7 @@ typedef struct {
     const double *pbchar;                 /* pb.Char [Ny][Nx] (0/1) */
     const double *depth;                  /* dominfo.bathy.pos [Ny][Nx] (positive depth) */
     const double *x, *y;                  /* dominfo.x [Nx], dominfo.y [Ny] */
-    int order;                            /* 1: HS1 (rhs_AB1_2D*_uv), else HS2 (rhs_AB2_2D*_uv) */
+    int order;                            /* 1: HS1 (rhs_AB1_2D*_uv), 3: HS3 (rhs_AB3_2D*_uv), else HS2 */
 } orc_params;
 
 typedef struct { int x, y; } orc_v2i;
@@ -861,6 +861,89 @@ int orc_rhs(void *vc, double t, const double *Z, double *dZ)
                 dvH2[2 * q + 1] = -ky[i] * evh[2 * q] - LeM0v[2 * q + 1] / g;
             }
         }
+    /* ---- HS3: the third-order terms of fun_termshat_AB3_2DF_uv
+       (2d/fun_odesolver.c:3512-3897) / fun_termshat_AB3_2DB_uv (:3899-4349):
+       deluH3, delvH3 (to the eta equation) and i k deletaH3 (to the velocity
+       equations), each adjusted; adjust is linear, so they are added to the
+       HS2 adjust inputs here. ---- */
+    if (p->order == 3) {
+        double *LM0u = cal(N), *LM0v = cal(N), *LLphi = cal(N), *dgp = cal(N), *dxu = cal(N), *dyv = cal(N);
+        #pragma omp parallel for schedule(static)
+        for (int i = 0; i < Ny; i++)
+            for (int j = 0; j < Nx; j++) {
+                size_t q = (size_t)i * Nx + j;
+                const double sc = p->flat ? 1.0 : 1.0 / g;   /* LetaM0*_hat (2DB: HSS / grav) */
+                LM0u[2 * q] = LeM0u[2 * q] * sc; LM0u[2 * q + 1] = LeM0u[2 * q + 1] * sc;
+                LM0v[2 * q] = LeM0v[2 * q] * sc; LM0v[2 * q + 1] = LeM0v[2 * q + 1] * sc;
+                LLphi[2 * q] = LM0u[2 * q] + LM0v[2 * q]; LLphi[2 * q + 1] = LM0u[2 * q + 1] + LM0v[2 * q + 1];
+                dgp[2 * q] = -kx[j] * Z[3 * N + q] - ky[i] * Z[5 * N + q];
+                dgp[2 * q + 1] = kx[j] * Z[2 * N + q] + ky[i] * Z[4 * N + q];
+                dxu[2 * q] = -kx[j] * Z[3 * N + q]; dxu[2 * q + 1] = kx[j] * Z[2 * N + q];
+                dyv[2 * q] = -ky[i] * Z[5 * N + q]; dyv[2 * q + 1] = ky[i] * Z[4 * N + q];
+            }
+        double *ph[6] = {LM0u, LM0v, LLphi, dgp, dxu, dyv};
+        for (int f = 0; f < 6; f++) { orc_hermitian_project(ph[f], Nx, Ny); fft2(&c->P, ph[f], +1); }
+        const double N3 = N2 * (double)N;
+        double *e2M0u = cal(N), *e2M0v = cal(N), *eLu = cal(N), *eLv = cal(N), *e2dxu = cal(N), *e2dyv = cal(N),
+               *dE3 = cal(N);
+        #pragma omp parallel for schedule(static)
+        for (size_t q = 0; q < N; q++) {
+            double w = p->wallchar[q], e = eta[2 * q], Nd = (double)N;
+            e2M0u[2 * q] = e * e * M0u[2 * q] / N3 * w;
+            e2M0v[2 * q] = e * e * M0v[2 * q] / N3 * w;
+            eLu[2 * q] = e / Nd * LM0u[2 * q] / Nd;
+            eLv[2 * q] = e / Nd * LM0v[2 * q] / Nd;
+            e2dxu[2 * q] = e * e * dxu[2 * q] / N3 * w;
+            e2dyv[2 * q] = e * e * dyv[2 * q] / N3 * w;
+            dE3[2 * q] = Lphi[2 * q] / Nd * e / Nd * dgp[2 * q] / Nd + Lphi[2 * q] / Nd * LLphi[2 * q] / Nd;
+        }
+        double *phys[7] = {e2M0u, e2M0v, eLu, eLv, e2dxu, e2dyv, dE3};
+        double *hat[7];
+        for (int f = 0; f < 7; f++) {
+            hat[f] = cal(N);
+            memcpy(hat[f], phys[f], sizeof(double) * 2 * N);
+            fft2(&c->P, hat[f], -1);
+        }
+        double *LLu = cal(N), *LLv = cal(N), *L2x = cal(N), *L2y = cal(N);
+        if (p->flat) {
+            #pragma omp parallel for schedule(static)
+            for (size_t q = 0; q < N; q++) {
+                double l = p->L2d[q];
+                LLu[2 * q] = l * hat[2][2 * q]; LLu[2 * q + 1] = l * hat[2][2 * q + 1];
+                LLv[2 * q] = l * hat[3][2 * q]; LLv[2 * q + 1] = l * hat[3][2 * q + 1];
+                L2x[2 * q] = l * hat[4][2 * q]; L2x[2 * q + 1] = l * hat[4][2 * q + 1];
+                L2y[2 * q] = l * hat[5][2 * q]; L2y[2 * q + 1] = l * hat[5][2 * q + 1];
+            }
+        } else {
+            #pragma omp parallel for schedule(static)
+            for (size_t q = 0; q < N; q++) {
+                eLu[2 * q] *= (double)N; eLv[2 * q] *= (double)N;
+                e2dxu[2 * q] *= (double)N; e2dyv[2 * q] *= (double)N;
+            }
+            hss_2depth(c, eLu, hat[2], p->om2_m, p->om2_p, LLu, (double)N);
+            hss_2depth(c, eLv, hat[3], p->om2_m, p->om2_p, LLv, (double)N);
+            hss_2depth(c, e2dxu, hat[4], p->om2_m, p->om2_p, L2x, (double)N);
+            hss_2depth(c, e2dyv, hat[5], p->om2_m, p->om2_p, L2y, (double)N);
+            #pragma omp parallel for schedule(static)
+            for (size_t q = 0; q < 2 * N; q++) { LLu[q] /= g; LLv[q] /= g; L2x[q] /= g; L2y[q] /= g; }
+        }
+        #pragma omp parallel for schedule(static)
+        for (int i = 0; i < Ny; i++)
+            for (int j = 0; j < Nx; j++) {
+                size_t q = (size_t)i * Nx + j;
+                for (int r = 0; r < 2; r++) {
+                    double du3 = LLu[2 * q + r] + 0.5 * L2x[2 * q + r] - 0.5 * (kx[j] * kx[j] * hat[0][2 * q + r]);
+                    double dv3 = LLv[2 * q + r] + 0.5 * L2y[2 * q + r] - 0.5 * (ky[i] * ky[i] * hat[1][2 * q + r]);
+                    duH2[2 * q + r] += du3;   /* adjust(duH2) + adjust(duH3) = adjust(duH2 + duH3) */
+                    dvH2[2 * q + r] += dv3;
+                }
+                ikxE[2 * q] += -kx[j] * hat[6][2 * q + 1]; ikxE[2 * q + 1] += kx[j] * hat[6][2 * q];
+                ikyE[2 * q] += -ky[i] * hat[6][2 * q + 1]; ikyE[2 * q + 1] += ky[i] * hat[6][2 * q];
+            }
+        for (int f = 0; f < 6; f++) free(ph[f]);
+        for (int f = 0; f < 7; f++) { free(phys[f]); free(hat[f]); }
+        free(LLu); free(LLv); free(L2x); free(L2y);
+    }
     adjust(c, duH2_adj, duH2);
     adjust(c, dvH2_adj, dvH2);
     adjust(c, ikxE_adj, ikxE);

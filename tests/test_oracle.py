@@ -276,3 +276,67 @@ def test_hs1_two_depth_with_unit_weights_equals_flat():
     db = HS.oracle_half(O.Oracle2D(mb).rhs(0.7, y), n, n)
     assert relerr(db[:, :, 1:], df[:, :, 1:]) < 1e-10
     assert relerr(db[:, : n // 2 + 1, 0], df[:, : n // 2 + 1, 0]) < 1e-10
+
+
+# ----------------------------------------------------------------- HS3 ---
+@pytest.mark.parametrize("kind", ["short_crested", "harbour"])
+def test_hs3_extra_terms_are_cubic(kind):
+    """The HS3 terms (deluH3, delvH3, i k deletaH3, all adjusted) are
+    homogeneous of degree three in the state: R3(s y) - R2(s y) scales as s^3
+    with the source off, flat and two-depth; they vanish when u = v = 0."""
+    m = getattr(W, kind)(32, t_end=3.0, **({"breaking": False} if kind == "harbour" else {}))
+    m.source_on = False
+    n = m.nx
+    y = W.random_state(m, 5, kfrac=0.5, amp=0.3)
+    d = []
+    for s in (1.0, 2.0):
+        m.order = 3
+        r3 = O.Oracle2D(m).rhs(0.4, s * y)
+        m.order = 2
+        r2 = O.Oracle2D(m).rhs(0.4, s * y)
+        d.append(HS.oracle_half(r3 - r2, n, n))
+    assert np.abs(d[0]).max() > 1e-8
+    assert relerr(d[1], 8.0 * d[0]) < 1e-8
+    ye = y.copy()
+    ye[2 * n * n:] = 0.0          # u = v = 0
+    m.order = 3
+    r3 = O.Oracle2D(m).rhs(0.4, ye)
+    m.order = 2
+    r2 = O.Oracle2D(m).rhs(0.4, ye)
+    assert np.abs(r3 - r2).max() == 0.0
+
+
+def test_hs3_single_mode_small_amplitude():
+    """In the small-amplitude limit HS3 is the exact linear dispersion too."""
+    m = linear_flat_model()
+    m.order = 3
+    n = m.nx
+    p, q = 2, 3
+    eta0 = single_mode_state(m, p, q, 1e-7)
+    from hawassi_c_amd.model2d import initial_state
+    y = initial_state(eta0)
+    kk = math.hypot(m.kx[q], m.ky[p])
+    w = math.sqrt(S.GRAV * kk * math.tanh(kk * 5.0))
+    T = 2.7
+    rc, t, h, st = O.Oracle2D(m).integrate(0.0, T, 0.01, y, 1e-14, 1e-10)
+    assert rc == 0
+    e0 = np.fft.fft2(eta0)[p, q]
+    eT = (y[: n * n] + 1j * y[n * n: 2 * n * n]).reshape(n, n)[p, q]
+    assert abs(eT / e0 - math.cos(w * T)) < 1e-7
+
+
+def test_hs3_two_depth_with_unit_weights_equals_flat():
+    """As for HS2: gamma_- = 1, gamma_+ = 0 with om2_- = g L2d makes every
+    HSS of the HS3 terms the flat L2d multiplier."""
+    mf = W.short_crested(32, t_end=3.0)
+    mf.order = 3
+    n = mf.nx
+    one, zero = np.ones((n, n)), np.zeros((n, n))
+    ops = dict(gam_min=one, gam_plus=zero, csq_min=mf.c2d ** 2, csq_plus=mf.c2d ** 2 * 0.5,
+               om2_min=S.GRAV * mf.l2d, om2_plus=S.GRAV * mf.l2d * 0.5)
+    mb = S.Model2D(**{**mf.__dict__, "flat": False, "ops": ops})
+    y = W.random_state(mf, 11, kfrac=0.5, amp=0.3)
+    df = HS.oracle_half(O.Oracle2D(mf).rhs(0.7, y), n, n)
+    db = HS.oracle_half(O.Oracle2D(mb).rhs(0.7, y), n, n)
+    assert relerr(db[:, :, 1:], df[:, :, 1:]) < 1e-8
+    assert relerr(db[:, : n // 2 + 1, 0], df[:, : n // 2 + 1, 0]) < 1e-8
