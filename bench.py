@@ -11,7 +11,10 @@ sequence), timed with HIP events on the library's stream around every
 evaluation of the timed region.  Its algorithmic HBM bytes per evaluation are
 B_rhs = T_alg (4N + 8H) + C_phys 4N + C_spec 4H + 2 S (input state read and
 derivative write), T_alg = 36, C_phys = 10, C_spec = 5, S = 3*16*H,
-N = nx*ny, H = (nx/2+1)*ny.  The whole-step figure (adding the RKF45 vector
+N = nx*ny, H = (nx/2+1)*ny.  With --order 3 (the AB3 "HS3" right-hand side,
+14 kernels) T_alg = 47 (LLphi and dgp inverted; A, g-A, g+A, eta^2 M0u w,
+eta^2 M0v w and dE3 forward; the HSS tail of A: two inverses, one forward)
+and C_phys = 14 (physical M0u, M0v written once and read once).  The whole-step figure (adding the RKF45 vector
 traffic, 5.5 S per evaluation) is reported as roofline_job.
 """
 from __future__ import annotations
@@ -73,6 +76,8 @@ def main():
                     help="harbour2048: the 2-D headline; slope1d: the 1-D batch of uneven realizations; "
                          "opensea1024x8: eight flat-bottom 1024^2 realizations advanced in lockstep")
     ap.add_argument("--nreal", type=int, default=256, help="slope1d: realizations in the batch")
+    ap.add_argument("--order", type=int, default=2, choices=[2, 3],
+                    help="harbour2048: dynamics, 2 = HS2 (rhs_AB2_2DB_uv), 3 = HS3 (rhs_AB3_2DB_uv)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="launch one elementwise copy kernel (torch.neg) over this many MiB of fp32 first, so a "
                          "counter pass over this command holds a dispatch of known HBM traffic")
@@ -103,6 +108,7 @@ def main():
     t_end = args.spinup + 60.0
     log(f"setup harbour {n}x{n}")
     model = W.harbour(n, t_end=t_end)
+    model.order = args.order
     dev = Hawassi2D(model, HW_PREC_FP32)
     dim = 6 * n * n
     dev.set_state(np.zeros(dim))
@@ -127,7 +133,7 @@ def main():
     log(f"timed {args.steps} steps: wall {wall:.3f} s, sim {sim:.3f} s, {st}")
     rhs_per_s = st["rhs_evals"] / wall
     avg_rhs_ms = st["rhs_ms_total"] / max(st["rhs_evals"], 1)
-    b_rhs, b_job = algorithmic_bytes(n, n)
+    b_rhs, b_job = algorithmic_bytes(n, n) if args.order == 2 else algorithmic_bytes(n, n, t_alg=47, c_phys=14)
     achieved = b_rhs / (avg_rhs_ms * 1e-3) / 1e9
     job = rhs_per_s * b_job / 1e9
     out = {
@@ -143,8 +149,8 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": f"harbour {n}x{n}: 1.5 km square, depth 20->4 m + seeded bumps, two-depth AB2 (HS2) "
-                               "operator, breakwater R=0.5 with 100 m gap, directional JONSWAP line influx; "
+        "config": {"workload": f"harbour {n}x{n}: 1.5 km square, depth 20->4 m + seeded bumps, two-depth "
+                               + ("AB2 (HS2)" if args.order == 2 else "AB3 (HS3)") + " operator, breakwater R=0.5 with 100 m gap, directional JONSWAP line influx; "
                                "RKF45 eps_rel 1e-2 eps_abs 1e-3 (reference tolerances)",
                    "nx": n, "ny": n, "state": "fp64 half spectrum", "sim_window_s": [t_start, t]},
         "rhs_evals_per_s": rhs_per_s,
@@ -153,8 +159,9 @@ def main():
         "avg_rhs_ms": avg_rhs_ms,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": RHS_TRAFFIC_BYTES_2048 if n == 2048 else None,
-                     "unit_of_work": "one RHS evaluation (9 kernels), %.1f MB algorithmic" % (b_rhs / 1e6)},
+                     "traffic": RHS_TRAFFIC_BYTES_2048 if n == 2048 and args.order == 2 else None,
+                     "unit_of_work": "one RHS evaluation (%d kernels), %.1f MB algorithmic"
+                                     % (9 if args.order == 2 else 14, b_rhs / 1e6)},
         "roofline_job": {"achieved": job, "frac": job / HBM_PEAK_GBS, "unit": "GB/s",
                          "bytes_per_rhs": b_job},
     }

@@ -52,6 +52,7 @@ template <typename T> struct Rhs2DArgs {
     T g;
     T* s_eta_out;   // per-row imaginary Nyquist part of the eta adjust (read by breaking)
     C* eta_col0;    // when set: the pre-mask eta derivative of column kx = 0, [ky]
+    T *m0u_p, *m0v_p;   // physical M0u, M0v (HS3 only)
 };
 
 template <typename T, int NX, int NY> struct Geo2 {
@@ -362,7 +363,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y2(Rhs2DArgs<T> a) {
 // W slots after X3
 enum : int { S_P1 = 2, S_Q1 = 3, S_G1M = 4, S_G1P = 5, S_P2 = 6, S_Q2 = 7, S_G2M = 8, S_G2P = 9, S_E = 10 };
 
-template <typename T, int NX, int NY>
+template <typename T, int NX, int NY, bool H3 = false>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x3(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
     // u and v are read twice (for E and for the products) so that at most
@@ -371,8 +372,10 @@ __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x3(Rhs2DArgs<T> a) {
     XSpec<T, NX> b;
     x_spec_load<T, NX, NY>(a.W[S_M0U], y0, b, tid);
     x_inv_roll<T, NX, NY>(b, a.W[S_M0V], y0, lds, twxs, m0u, sc, nullptr, tid);
+    if constexpr (H3) ph_store<T, NX, NY>(a.m0u_p, m0u, y0, tid);
     ph_load<T, NX, NY>(w, a.u_p, y0, tid);
     x_inv_roll<T, NX, NY>(b, nullptr, y0, lds, twxs, m0v, sc, nullptr, tid);
+    if constexpr (H3) ph_store<T, NX, NY>(a.m0v_p, m0v, y0, tid);
     // E = 0.5 (u^2 + v^2 - Lphi^2) w, Lphi = M0u + M0v  (2d/fun_odesolver.c:3426)
     ph_load<T, NX, NY>(x, a.v_p, y0, tid);
 #pragma unroll
@@ -561,14 +564,18 @@ __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x5(Rhs2DArgs<T> a) {
 // W slots after Y6
 enum : int { S_AETA = 6 };
 
-template <typename T, int NX, int NY>
+// HS3 slots (two-depth): the HS2 tail of the y momentum moves to slot 8 so
+// that F(eta u w), F(eta v w) stay readable, and the HS3 tail is slot 14.
+enum : int { S3_ACC2 = 8, S3_ACC3 = 14 };
+
+template <typename T, int NX, int NY, bool H3 = false>
 __global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves_per_eu(YGeo<T, NY>::WPE))) k2b_y6(Rhs2DArgs<T> a) {
     HW_YK_PROLOGUE
     C F1[G::PE], F2[G::PE], Z[G::PE], eh[G::PE], Zu[G::PE], Zv[G::PE];
     T al[G::PE], kk0[G::PE], kk1[G::PE];
     const T ig = (T)1 / a.g;
     y_load<T, NX, NY>(F1, a.W[S_ACC1], nullptr, col0, tid);
-    y_load<T, NX, NY>(F2, a.W[S_ACC2], nullptr, col0, tid);
+    y_load<T, NX, NY>(F2, a.W[H3 ? (int)S3_ACC2 : (int)S_ACC2], nullptr, col0, tid);
     y_load<T, NX, NY>(Z, a.dH2p, nullptr, col0, tid);
     y_load<T, NX, NY>(eh, a.eta_hat, nullptr, col0, tid);
     y_load<T, NX, NY>(Zu, a.W[S_ZU], nullptr, col0, tid);
@@ -584,6 +591,15 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves
     }
     y_fwd_regs<T, NX, NY>(F1, lds, twys, tid);
     y_fwd_regs<T, NX, NY>(F2, lds, twys, tid);
+    if constexpr (H3) {
+        // the HS3 tail F(g- b3- + g+ b3+) enters with the opposite sign
+#pragma unroll
+        for (int e = 0; e < G::PE; ++e) F1[e] = cadd(F1[e], F2[e]);
+        y_load<T, NX, NY>(F2, a.W[S3_ACC3], nullptr, col0, tid);
+        y_fwd_regs<T, NX, NY>(F2, lds, twys, tid);
+#pragma unroll
+        for (int e = 0; e < G::PE; ++e) F2[e] = cmk<C>(-F2[e].x, -F2[e].y);
+    }
 #pragma unroll
     for (int e = 0; e < G::PE; ++e) Z[e] = csub(Z[e], cscale(cadd(F1[e], F2[e]), (T)0.5 * ig));
     y_inv_regs<T, NX, NY>(Z, lds, twys, col0, tid);
@@ -714,7 +730,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves
 
 enum : int { F_P1 = 5, F_Q1 = 6, F_P2 = 7, F_Q2 = 8, F_E = 9 };
 
-template <typename T, int NX, int NY>
+template <typename T, int NX, int NY, bool H3 = false>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2f_x2(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
     C ew[G::PP], m[G::PP], q2[G::PP], lp[G::PP];
@@ -736,10 +752,12 @@ __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2f_x2(Rhs2DArgs<T> a) {
     for (int q = 0; q < G::PP; ++q) { q2[q] = pfma(m[q], m[q], q2[q]); m[q] = pmul(ew[q], m[q]); }   // u^2 + v^2
     x_fwd_store<T, NX, NY>(m, a.W[F_Q2], y0, lds, twxs, tid);
     x_inv_roll<T, NX, NY>(b, a.W[F_M0V], y0, lds, twxs, lp, sc, nullptr, tid);
+    if constexpr (H3) ph_store<T, NX, NY>(a.m0u_p, lp, y0, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) m[q] = pmul(ew[q], lp[q]);
     x_fwd_store<T, NX, NY>(m, a.W[F_P1], y0, lds, twxs, tid);
     x_inv_roll<T, NX, NY>(b, nullptr, y0, lds, twxs, m, sc, nullptr, tid);
+    if constexpr (H3) ph_store<T, NX, NY>(a.m0v_p, m, y0, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) { lp[q] = cadd(lp[q], m[q]); m[q] = pmul(ew[q], m[q]); }
     x_fwd_store<T, NX, NY>(m, a.W[F_P2], y0, lds, twxs, tid);

@@ -13,6 +13,7 @@
 #include "../../include/hawassi_hip.h"
 #include "aux_kernels.h"
 #include "breaking2d.h"
+#include "rhs2d_hs3.h"
 #include <algorithm>
 #include <climits>
 
@@ -137,6 +138,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
     std::vector<double> tsig;
     int nline = 0, nt = 0;
     DevBuf<T> eta_p, u_p, v_p;
+    DevBuf<T> m0u_p, m0v_p;   // HS3 only
     DevBuf<C> work;     // 16 W slots + dH1 + dH2p
     DevBuf<double2> y, ynew;
     DevBuf<C> kbuf;     // 6 stage derivatives
@@ -174,6 +176,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
     long brk_calls = 0;
     bool flat = false;
     bool hs1 = false;   // HS1 dynamics (linear AB1 right-hand side)
+    bool hs3 = false;   // HS3 dynamics (AB3: the AB2 terms plus the third-order ones)
     int src_on = 0;
     double g = 9.80665, eps_abs = 1e-3, eps_rel = 1e-2;
     // HIP events bracketing every RHS evaluation on the context's stream
@@ -210,7 +213,12 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
     int init(const hw_config_2d* c) override {
         nx = NX; ny = NY; prec = c->precision;
         flat = c->flat != 0;
+        if (c->model < 0 || c->model > 3) {
+            set_error("hw_config_2d.model must be 0 or 2 (HS2), 1 (HS1) or 3 (HS3), got %d", c->model);
+            return HW_ERR_ARG;
+        }
         hs1 = c->model == 1;
+        hs3 = c->model == 3;
         g = c->grav; eps_abs = c->eps_abs; eps_rel = c->eps_rel;
         HW_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         for (int i = 0; i < NEV; ++i) {
@@ -305,6 +313,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
             HW_TRY(hipEventCreateWithFlags(&ev_brk, hipEventDisableTiming));
         }
         if ((rc = eta_p.alloc(N)) || (rc = u_p.alloc(N)) || (rc = v_p.alloc(N))) return rc;
+        if (hs3 && ((rc = m0u_p.alloc(N)) || (rc = m0v_p.alloc(N)))) return rc;
         if ((rc = work.alloc(18 * H))) return rc;
         if ((rc = y.alloc(3 * H)) || (rc = ynew.alloc(3 * H)) || (rc = kbuf.alloc(6 * 3 * H)) || (rc = ytmp.alloc(3 * H))) return rc;
         if ((rc = rmax.alloc(1))) return rc;
@@ -319,6 +328,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         args.src_on = src_on; args.vertical = c->vertical; args.propagation = c->propagation;
         args.sbx = nullptr; args.sby = nullptr;
         args.eta_p = eta_p.p; args.u_p = u_p.p; args.v_p = v_p.p;
+        args.m0u_p = m0u_p.p; args.m0v_p = m0v_p.p;
         for (int i = 0; i < 16; ++i) args.W[i] = work.p + (size_t)i * H;
         args.dH1 = work.p + 16 * H;
         args.dH2p = work.p + 17 * H;
@@ -395,6 +405,33 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
             }
             k1_x<T, NX, NY><<<gx, bx, 0, stream>>>(args, flat ? 0 : 1);
             k1_y<T, NX, NY><<<gy, by, 0, stream>>>(args, flat ? 1 : 0);
+        } else if (hs3 && !flat) {
+            // rhs_AB3_2DB_uv (2d/fun_odesolver.c:4735-4843)
+            k2b_y1<T, NX, NY><<<gy, by, 0, stream>>>(args);
+            k2b_x1<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+            if (brk_now) { int rc = breaking_detect(); if (rc) return rc; }
+            k2b_y2<T, NX, NY><<<gy, by, 0, stream>>>(args);
+            k2b_x3<T, NX, NY, true><<<gx, bx, 0, stream>>>(args);
+            k2b_y4<T, NX, NY><<<gy, by, 0, stream>>>(args);
+            k3b_x5a<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+            k3_ya<T, NX, NY, false><<<gy, by, 0, stream>>>(args);
+            k3_xb<T, NX, NY, false><<<gx, bx, 0, stream>>>(args);
+            k3_yc<T, NX, NY, false><<<gy, by, 0, stream>>>(args);
+            k3b_x5b<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+            k2b_y6<T, NX, NY, true><<<gy, by, 0, stream>>>(args);
+            k2b_x7<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+            k2b_y8<T, NX, NY><<<gy, by, 0, stream>>>(args);
+        } else if (hs3) {
+            // rhs_AB3_2DF_uv (2d/fun_odesolver.c:4628-4733)
+            k2f_y1<T, NX, NY><<<gy, by, 0, stream>>>(args);
+            k2f_x2<T, NX, NY, true><<<gx, bx, 0, stream>>>(args);
+            if (brk_now) { int rc = breaking_detect(); if (rc) return rc; }
+            k2f_y3<T, NX, NY><<<gy, by, 0, stream>>>(args);
+            k3_ya<T, NX, NY, true><<<gy, by, 0, stream>>>(args);
+            k3_xb<T, NX, NY, true><<<gx, bx, 0, stream>>>(args);
+            k3_yc<T, NX, NY, true><<<gy, by, 0, stream>>>(args);
+            k2f_x4<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+            k2f_y5<T, NX, NY><<<gy, by, 0, stream>>>(args);
         } else if (!flat) {
             k2b_y1<T, NX, NY><<<gy, by, 0, stream>>>(args);
             k2b_x1<T, NX, NY><<<gx, bx, 0, stream>>>(args);

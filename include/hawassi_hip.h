@@ -81,7 +81,9 @@ typedef struct hw_config_2d {
     double t0;                        /* initial tprev (dominfo.t[0], 2d/main.c:120) */
     int model;                        /* dynamics: 1 "HS1" (rhs_AB1_2DF_uv / rhs_AB1_2DB_uv,
                                          2d/fun_odesolver.c:4350-4456: linear, no breaking call);
-                                         2 or 0 "HS2" (rhs_AB2_2D*_uv); 2d/main.c:209-224 */
+                                         2 or 0 "HS2" (rhs_AB2_2D*_uv); 3 "HS3" (rhs_AB3_2DF_uv /
+                                         rhs_AB3_2DB_uv, 2d/fun_odesolver.c:4628-4843);
+                                         2d/main.c:209-232 */
 } hw_config_2d;
 
 typedef struct hw_stats {
