@@ -12,7 +12,7 @@ evaluation of the timed region.  Its algorithmic HBM bytes per evaluation are
 B_rhs = T_alg (4N + 8H) + C_phys 4N + C_spec 4H + 2 S (input state read and
 derivative write), T_alg = 36, C_phys = 10, C_spec = 5, S = 3*16*H,
 N = nx*ny, H = (nx/2+1)*ny.  With --order 3 (the AB3 "HS3" right-hand side,
-14 kernels) T_alg = 47 (LLphi and dgp inverted; A, g-A, g+A, eta^2 M0u w,
+13 kernels) T_alg = 47 (LLphi and dgp inverted; A, g-A, g+A, eta^2 M0u w,
 eta^2 M0v w and dE3 forward; the HSS tail of A: two inverses, one forward)
 and C_phys = 14 (physical M0u, M0v written once and read once).  The whole-step figure (adding the RKF45 vector
 traffic, 5.5 S per evaluation) is reported as roofline_job.
@@ -33,7 +33,9 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE counter passes over this script (FETCH_SIZE scaled
 # by 2.0, WRITE_SIZE by 1.0, the factors that make a known 512 MiB copy read
 # right), summed over the evaluation's kernels; profiles/rhs2d_harbour2048_fp32.md.
-RHS_TRAFFIC_BYTES_2048 = 2090.8e6
+RHS_TRAFFIC_BYTES_2048 = 2080.1e6
+# the same for the AB3 (HS3) right-hand side, profiles/rhs2d_harbour2048_hs3_fp32.md
+RHS_TRAFFIC_BYTES_2048_HS3 = 2952.3e6
 METRIC = "sim-sec/wall-sec + RHS evals/s, 2D 2048^2 FP32 vs FP64 oracle at relL2<=1e-4"
 
 
@@ -159,9 +161,10 @@ def main():
         "avg_rhs_ms": avg_rhs_ms,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": RHS_TRAFFIC_BYTES_2048 if n == 2048 and args.order == 2 else None,
+                     "traffic": (RHS_TRAFFIC_BYTES_2048 if args.order == 2 else RHS_TRAFFIC_BYTES_2048_HS3)
+                                if n == 2048 else None,
                      "unit_of_work": "one RHS evaluation (%d kernels), %.1f MB algorithmic"
-                                     % (9 if args.order == 2 else 14, b_rhs / 1e6)},
+                                     % (9 if args.order == 2 else 13, b_rhs / 1e6)},
         "roofline_job": {"achieved": job, "frac": job / HBM_PEAK_GBS, "unit": "GB/s",
                          "bytes_per_rhs": b_job},
     }
