@@ -28,7 +28,7 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 SIZES = (
     [(0, n, n) for n in (32, 64, 128, 256, 512, 1024, 2048)]
     + [(1, n, n) for n in (32, 64, 128, 256, 512, 1024, 2048)]
-    + [(p, a, b) for p in (0, 1) for (a, b) in ((64, 32), (32, 64), (128, 64))]
+    + [(p, a, b) for p in (0, 1) for (a, b) in ((64, 32), (32, 64), (128, 64), (256, 64), (512, 256))]
 )
 
 

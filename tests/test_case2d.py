@@ -1,0 +1,127 @@
+"""Setup of a reference case directory (hawassi_c_amd.case2d) on the CPU:
+the ramps, the symmetrisations, the area-source signal in closed form, the
+influx.dat reader and whole models built from the reference's input.txt."""
+import math
+import shutil
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from hawassi_c_amd import case2d as C
+from hawassi_c_amd import setup2d as S
+
+GOLD = Path(__file__).parent / "golden" / "input2d"
+
+
+def test_tapered_ramp_closed_form():
+    t = np.arange(0, 100.0, 0.5)
+    y = np.linspace(0, 400, 65)
+    r = C.tapered_ramp(t, y, y[0], y[-1], 24.0, 0.0, y[1] - y[0], 0.5, 2.0, 0.0)
+    assert r.shape == (len(t), len(y)) and np.all(r == r[:, :1])
+    i = np.searchsorted(t, 12.0)
+    # half-cosine rise over LrampT = 24 s at both ends of the record
+    assert r[0, 0] == 0.0 and abs(r[i, 0] - 0.5) < 1e-12 and np.all(r[(t >= 24) & (t <= t[-1] - 24), 0] == 1.0)
+    assert abs(r[-1, 0]) < 1e-12
+    assert np.all(C.tapered_ramp(t, y, 0, 400, 24.0, 0.0, 6.25, 0.5, 0.0, 0.0) == 1.0)
+    # a taper along the line: zero at the ends, one in the middle
+    r = C.tapered_ramp(t, y, y[0], y[-1], 24.0, 100.0, y[1] - y[0], 0.5, 0.0, 1.0)
+    assert r[5, 0] == 0.0 and r[5, 32] == 1.0 and abs(r[5, 8] - 0.5) < 1e-12
+
+
+def test_symmetrisations():
+    g = np.random.default_rng(1)
+    a = g.standard_normal((16, 8))
+    # the reference's projection leaves the transform of a real array alone
+    h = np.fft.fft2(a)
+    assert np.allclose(C.complex_2d_sym(h), h, atol=1e-12)
+    # and leaves an inverse whose imaginary part comes from the Nyquist
+    # column alone (it is not mirrored): (-1)^j s(i)
+    z = C.complex_2d_sym(g.standard_normal((16, 8)) + 1j * g.standard_normal((16, 8)))
+    im = np.fft.ifft2(z).imag
+    alt = (-1.0) ** np.arange(8)
+    assert np.abs(im - im[:, :1] * alt[None, :]).max() < 1e-14
+    zn = z.copy()
+    zn[:, 4] = 0
+    assert np.abs(np.fft.ifft2(C.complex_2d_sym(zn)).imag).max() < 1e-14
+    s = C.real_2d_sym(g.standard_normal((16, 8)))
+    assert np.array_equal(C.real_2d_sym(s), s)
+    im = np.fft.ifft2(s).imag
+    assert np.abs(im - im[:, :1] * alt[None, :]).max() < 1e-14
+
+
+def test_area_signal_closed_form():
+    """A line record uniform along the line and harmonic on a DFT bin: the
+    area signal is dy/(2 pi)^2 times the record, and the skew signal the
+    quadrature record divided by k(omega) (up to the O(1/N) asymmetry of the
+    reference's frequency grid)."""
+    nt, ny, depth, dy = 256, 32, 20.0, 6.25
+    t = np.arange(nt) * 0.5
+    ww = S.freqspace(t[-1] - t[0], nt)
+    kw, _ = C.influx_wavenumbers(ww, depth, True)
+    m = 21
+    ph = 2 * math.pi * m * np.arange(nt) / nt
+    eta = np.repeat(np.cos(ph)[:, None], ny, axis=1)
+    ky = S.freqspace(ny * dy - dy, ny)
+    sig, skew = C.area_signals(eta, ky, kw, depth, dy, np.ones((nt, ny)))
+    c = dy / (2 * math.pi) ** 2
+    assert sig.shape == (ny, nt)
+    assert np.abs(sig - c * np.cos(ph)[None, :]).max() < 1e-3 * c
+    assert np.abs(skew - c / kw[m] * np.sin(ph)[None, :]).max() < 1e-2 * c / kw[m]
+
+
+def test_influx_wavenumbers_invert_the_dispersion():
+    ww = S.freqspace(199.0, 400)
+    k_f, _ = C.influx_wavenumbers(ww, 15.0, True)
+    k_s, _ = C.influx_wavenumbers(ww, 15.0, False)
+    w_back = S.exact_disp1d(k_f, 15.0)
+    assert np.abs(w_back - ww).max() < 1e-9
+    assert np.abs(k_s - k_f).max() < 1e-6 * np.abs(k_f).max()
+
+
+def test_read_influx(tmp_path):
+    f = tmp_path / "influx.dat"
+    rows = ["0 " + " ".join("%f" % v for v in (1.0, 1.0, 1.0)), "0 " + " ".join("%f" % v for v in (0.0, 5.0, 10.0))]
+    for j in range(4):
+        rows.append("%f " % (0.5 * j) + " ".join("%f" % (j + 0.1 * i) for i in range(3)))
+    f.write_text("\n".join(rows))
+    x, y, t, eta = C.read_influx(f, 64, 4)
+    assert x.tolist() == [1.0, 1.0, 1.0] and y.tolist() == [0.0, 5.0, 10.0]
+    assert t.tolist() == [0.0, 0.5, 1.0, 1.5] and eta[2].tolist() == [2.0, 2.1, 2.2]
+
+
+def test_model_from_input_flat_harmonic(tmp_path):
+    d = tmp_path / "2dF001harm"
+    d.mkdir()
+    shutil.copy(GOLD / "2dF001harm.txt", d / "input.txt")
+    m = C.model_from_input(str(d))
+    assert (m.nx, m.ny, m.flat, m.order, m.propagation, m.vertical) == (256, 64, True, 3, 0, True)
+    assert m.sig.shape == (64, len(m.t_sig)) and np.isfinite(m.sig).all() and np.isfinite(m.skew_sig).all()
+    # the peak frequency comes from the record's spectrum (nearest bin to 2 pi / 12)
+    assert abs(m.meta["tp"] - 12.0) < 0.5 and m.meta["hs"] == 2.0
+    assert abs(m.meta["kp"] - S.invers_omega(2 * math.pi / m.meta["tp"], 20.0)) < 1e-12
+    # the source kernel sits on the influx line x = 0
+    ix = int(np.argmax(np.abs(m.new_gam[32])))
+    assert abs(m.x[ix]) <= m.x[1] - m.x[0]
+
+
+def test_model_from_input_two_depth_partial_line(tmp_path):
+    """2dB001HarmRefr's input with a synthetic bath.dat of its 512 x 256 grid:
+    two-depth operators and the adjustment zone of a line shorter than the
+    domain side."""
+    d = tmp_path / "case"
+    d.mkdir()
+    shutil.copy(GOLD / "2dB001HarmRefr.txt", d / "input.txt")
+    nx, ny = 512, 256
+    xx = np.linspace(0, 3000, nx)
+    depth = np.tile(30.0 - 20.0 * xx / 3000.0, (ny, 1))
+    (d / "bath.dat").write_text("\n".join(" ".join("%.7e" % v for v in row) for row in depth))
+    m = C.model_from_input(str(d))
+    assert (m.nx, m.ny, m.flat, m.order) == (512, 256, False, 3)
+    assert abs(m.meta["meandepth"] - depth[0, C.io2d.c_round((400 - 0) / (xx[1] - xx[0]))]) < 0.1
+    g = m.ops["gam_min"] + m.ops["gam_plus"]
+    assert np.isfinite(g).all()
+    # chi: 1 far from the line, 0 on it, 1 beyond the line's end in y
+    iy_mid = int(np.argmin(np.abs(m.y - 600)))
+    ix_l = int(np.argmin(np.abs(m.x - 400)))
+    assert m.chi[iy_mid, ix_l] < 0.01 and m.chi[iy_mid, -1] == 1.0 and m.chi[-1, ix_l] == 1.0
