@@ -240,6 +240,66 @@ def disp_interpolation_2p(depth: np.ndarray, kx: np.ndarray, ky: np.ndarray, nu:
                 table_depth=DD, table_gam_min=gm, table_gam_plus=gp)
 
 
+def disp_interpolation_3p(depth: np.ndarray, kx: np.ndarray, ky: np.ndarray, nu: float, mid: str = "default",
+                          ux: float = 0.0, uy: float = 0.0, mdir: float = 0.0) -> dict:
+    """funOprt_DispInterpolation_3p (2d/functions.c:1627-1762): three
+    reference depths D-, Dmid (the midpoint, or the depth_ref value), D+.
+    On a 100-point depth table the weights solve
+        sum_j gamma_j C0(D_j) = C0(D),  sum_j gamma_j w^2(D_j) = w^2(D),
+        sum_j gamma_j Cp(D_j) = Cp(D)
+    at the wavenumber whose frequency is nu (C0 = sqrt(g D)), by Cramer's
+    rule, and are splined in D.  Returns the 2p keys plus the *_mid ones."""
+    U = math.sqrt(ux * ux + uy * uy)
+    KK = np.sqrt(kx[None, :] ** 2 + ky[:, None] ** 2)
+    Dmin, Dmax = float(depth.min()), float(depth.max())
+    kkmax = float(KK.max())
+    if kkmax < nu:
+        kkmax = 2 * nu
+    Dmid = (Dmax + Dmin) / 2 if mid == "default" else float(mid)
+    N = 100
+    dD = (Dmax - Dmin) / (N - 1)
+    dkk = kkmax / (N - 1)
+    DD = Dmin + np.arange(N) * dD
+    kk = np.arange(N) * dkk
+    DD[N - 1] = DD[N - 1] + 0.1 * dD
+    G = np.empty((3, N))
+    c0 = [math.sqrt(GRAV * Dmin), math.sqrt(GRAV * Dmid), math.sqrt(GRAV * Dmax)]
+    for i in range(N):
+        Om = exact_disp1d(kk, DD[i], U)
+        kap = inv_om_spline(Om, kk, nu)
+        cp = [float(exact_cp1d(kap, d, U)) for d in (Dmin, Dmid, Dmax)]
+        o2 = [float(exact_disp1d(kap, d, U)) ** 2 for d in (Dmin, Dmid, Dmax)]
+        cpD = float(exact_cp1d(kap, DD[i], U))
+        o2D = float(exact_disp1d(kap, DD[i], U)) ** 2
+        c0D = math.sqrt(GRAV * DD[i])
+        A = o2[1] * cp[2] - o2[2] * cp[1]
+        B = -(o2[0] * cp[2]) + o2[2] * cp[0]
+        Cc = o2[0] * cp[1] - o2[1] * cp[0]
+        D_ = -(c0[1] * cp[2]) + c0[2] * cp[1]
+        E = c0[0] * cp[2] - c0[2] * cp[0]
+        F = -(c0[0] * cp[1]) + c0[1] * cp[0]
+        G_ = c0[1] * o2[2] - c0[2] * o2[1]
+        H = -(c0[0] * o2[2]) + c0[2] * o2[0]
+        I_ = c0[0] * o2[1] - c0[1] * o2[0]
+        det = c0[0] * A + c0[1] * B + c0[2] * Cc
+        G[0, i] = (A * c0D + D_ * o2D + G_ * cpD) / det
+        G[1, i] = (B * c0D + E * o2D + H * cpD) / det
+        G[2, i] = (Cc * c0D + F * o2D + I_ * cpD) / det
+    flat_d = depth.ravel()
+    gam = [_spline_eval_vec(DD, G[j], spline_coeffs(DD, G[j]), flat_d).reshape(depth.shape) for j in range(3)]
+    KX, KY = kx[None, :], ky[:, None]
+    out = dict(d_min=Dmin, d_mid=Dmid, d_max=Dmax, table_depth=DD, table_gam=G)
+    for j, (name, d) in enumerate((("min", Dmin), ("mid", Dmid), ("plus", Dmax))):
+        om = exact_disp2d(KX, KY, d, mdir, ux, uy)
+        out["gam_" + name] = gam[j]
+        out["om_" + name] = om
+        out["om2_" + name] = om ** 2
+        out["l2d_" + name] = om ** 2 / GRAV
+        out["c2d_" + name] = exact_cp2d(KX, KY, d, ux, uy)
+        out["csq_" + name] = exact_cp1d(KK, d, U) ** 2
+    return out
+
+
 def _spline_eval_vec(x, y, c, t):
     """Vectorised gsl cspline eval at many points (same interval rule)."""
     t = np.asarray(t, float)

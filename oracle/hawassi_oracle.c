@@ -9,7 +9,7 @@
  *
  * What it follows (reference file:line):
  *   fftwcomplex_2d_sym           2d/functions1.c:492-521   -> orc_hermitian_project
- *   HSSgen_2depth                2d/fun_odesolver.c:500-585 -> hss_2depth
+ *   HSSgen_2depth / _3depth      2d/fun_odesolver.c:500-697 -> hss_2depth
  *   fun_adjust                   2d/fun_odesolver.c:810-833 -> adjust
  *   fun_source                   2d/fun_odesolver.c:886-983 -> source_hat
  *   fun_delphiH1hat              2d/fun_odesolver.c:2100-2239 -> delphiH1_flat
@@ -106,6 +106,8 @@ typedef struct {
     const double *depth;                  /* dominfo.bathy.pos [Ny][Nx] (positive depth) */
     const double *x, *y;                  /* dominfo.x [Nx], dominfo.y [Ny] */
     int order;                            /* 1: HS1 (rhs_AB1_2D*_uv), 3: HS3 (rhs_AB3_2D*_uv), else HS2 */
+    /* third reference depth (ninterp = 3, HSSgen_3depth); NULL for two depths */
+    const double *gam_c, *csq_c, *om2_c;
 } orc_params;
 
 typedef struct { int x, y; } orc_v2i;
@@ -211,37 +213,42 @@ static double *cal(size_t n) { return (double *)calloc(2 * n, sizeof(double)); }
 
 /* HSSgen_2depth (2d/fun_odesolver.c:500-585).  phi: complex array whose real
  * part is the physical field times factIFFT; phihat: its spectrum. */
-static void hss_2depth(orc_ctx *c, const double *phi, const double *phihat, const double *Om, const double *Op,
-                       double *result, double factIFFT)
+/* HSSgen_2depth (2d/fun_odesolver.c:500-585) and, when the context holds a
+ * third depth, HSSgen_3depth (:588-697): for each reference depth j,
+ * (F(gam_j F^-1(P Om_j phihat)/N) + Om_j F(gam_j phi/factIFFT)) / 2, summed.
+ * Oc is the middle depth's multiplier (ignored for two depths). */
+static void hss_term(orc_ctx *c, const double *phi, const double *phihat, const double *O, const double *G,
+                     double *acc, double factIFFT)
 {
     int Nx = c->p.Nx, Ny = c->p.Ny;
     size_t N = (size_t)Nx * Ny;
-    double *Lm = cal(N), *Lp = cal(N);
+    double *L = cal(N), *g = cal(N), *gL = cal(N);
+    #pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < N; q++) { L[2 * q] = O[q] * phihat[2 * q]; L[2 * q + 1] = O[q] * phihat[2 * q + 1]; }
+    orc_hermitian_project(L, Nx, Ny);
+    fft2(&c->P, L, +1);
     #pragma omp parallel for schedule(static)
     for (size_t q = 0; q < N; q++) {
-        Lm[2 * q] = Om[q] * phihat[2 * q]; Lm[2 * q + 1] = Om[q] * phihat[2 * q + 1];
-        Lp[2 * q] = Op[q] * phihat[2 * q]; Lp[2 * q + 1] = Op[q] * phihat[2 * q + 1];
+        g[2 * q] = G[q] * phi[2 * q] / factIFFT;
+        gL[2 * q] = G[q] * L[2 * q] / (double)N;
     }
-    orc_hermitian_project(Lm, Nx, Ny);
-    orc_hermitian_project(Lp, Nx, Ny);
-    fft2(&c->P, Lm, +1);
-    fft2(&c->P, Lp, +1);
-    double *gm = cal(N), *gp = cal(N), *gmL = cal(N), *gpL = cal(N);
-    const double *Gm = c->p.gam_m, *Gp = c->p.gam_p;
+    fft2(&c->P, g, -1); fft2(&c->P, gL, -1);
     #pragma omp parallel for schedule(static)
     for (size_t q = 0; q < N; q++) {
-        gm[2 * q] = Gm[q] * phi[2 * q] / factIFFT;
-        gp[2 * q] = Gp[q] * phi[2 * q] / factIFFT;
-        gmL[2 * q] = Gm[q] * Lm[2 * q] / (double)N;
-        gpL[2 * q] = Gp[q] * Lp[2 * q] / (double)N;
+        acc[2 * q] += (gL[2 * q] + O[q] * g[2 * q]) / 2;
+        acc[2 * q + 1] += (gL[2 * q + 1] + O[q] * g[2 * q + 1]) / 2;
     }
-    fft2(&c->P, gm, -1); fft2(&c->P, gp, -1); fft2(&c->P, gmL, -1); fft2(&c->P, gpL, -1);
-    #pragma omp parallel for schedule(static)
-    for (size_t q = 0; q < N; q++) {
-        result[2 * q] = (gmL[2 * q] + Om[q] * gm[2 * q] + gpL[2 * q] + Op[q] * gp[2 * q]) / 2;
-        result[2 * q + 1] = (gmL[2 * q + 1] + Om[q] * gm[2 * q + 1] + gpL[2 * q + 1] + Op[q] * gp[2 * q + 1]) / 2;
-    }
-    free(Lm); free(Lp); free(gm); free(gp); free(gmL); free(gpL);
+    free(L); free(g); free(gL);
+}
+
+static void hss_2depth(orc_ctx *c, const double *phi, const double *phihat, const double *Om, const double *Op,
+                       const double *Oc, double *result, double factIFFT)
+{
+    size_t N = (size_t)c->p.Nx * c->p.Ny;
+    memset(result, 0, sizeof(double) * 2 * N);
+    hss_term(c, phi, phihat, Om, c->p.gam_m, result, factIFFT);
+    if (c->p.gam_c) hss_term(c, phi, phihat, Oc, c->p.gam_c, result, factIFFT);
+    hss_term(c, phi, phihat, Op, c->p.gam_p, result, factIFFT);
 }
 
 /* fun_adjust (2d/fun_odesolver.c:810-833): out = F(chi * F^-1(P Ahat)/N),
@@ -718,8 +725,8 @@ int orc_rhs(void *vc, double t, const double *Z, double *dZ)
         memcpy(v, vhat, sizeof(double) * 2 * N);
         fft2(&c->P, eta, +1); fft2(&c->P, u, +1); fft2(&c->P, v, +1);
         double *Cu = cal(N), *Cv = cal(N);
-        hss_2depth(c, u, uhat, p->csq_m, p->csq_p, Cu, (double)N);
-        hss_2depth(c, v, vhat, p->csq_m, p->csq_p, Cv, (double)N);
+        hss_2depth(c, u, uhat, p->csq_m, p->csq_p, p->csq_c, Cu, (double)N);
+        hss_2depth(c, v, vhat, p->csq_m, p->csq_p, p->csq_c, Cv, (double)N);
         #pragma omp parallel for schedule(static)
         for (size_t q = 0; q < N; q++) {
             dampeta[2 * q] = 7 * p->Cpeak * p->fbdy[q] * eta[2 * q] / (double)N;
@@ -839,8 +846,8 @@ int orc_rhs(void *vc, double t, const double *Z, double *dZ)
     } else {
         #pragma omp parallel for schedule(static)
         for (size_t q = 0; q < N; q++) { eM0u[2 * q] *= (double)N; eM0v[2 * q] *= (double)N; }
-        hss_2depth(c, eM0u, eM0uh, p->om2_m, p->om2_p, LeM0u, (double)N);
-        hss_2depth(c, eM0v, eM0vh, p->om2_m, p->om2_p, LeM0v, (double)N);
+        hss_2depth(c, eM0u, eM0uh, p->om2_m, p->om2_p, p->om2_c, LeM0u, (double)N);
+        hss_2depth(c, eM0v, eM0vh, p->om2_m, p->om2_p, p->om2_c, LeM0v, (double)N);
     }
     double *ikxE = cal(N), *ikyE = cal(N), *duH2 = cal(N), *dvH2 = cal(N);
     #pragma omp parallel for schedule(static)
@@ -920,10 +927,10 @@ int orc_rhs(void *vc, double t, const double *Z, double *dZ)
                 eLu[2 * q] *= (double)N; eLv[2 * q] *= (double)N;
                 e2dxu[2 * q] *= (double)N; e2dyv[2 * q] *= (double)N;
             }
-            hss_2depth(c, eLu, hat[2], p->om2_m, p->om2_p, LLu, (double)N);
-            hss_2depth(c, eLv, hat[3], p->om2_m, p->om2_p, LLv, (double)N);
-            hss_2depth(c, e2dxu, hat[4], p->om2_m, p->om2_p, L2x, (double)N);
-            hss_2depth(c, e2dyv, hat[5], p->om2_m, p->om2_p, L2y, (double)N);
+            hss_2depth(c, eLu, hat[2], p->om2_m, p->om2_p, p->om2_c, LLu, (double)N);
+            hss_2depth(c, eLv, hat[3], p->om2_m, p->om2_p, p->om2_c, LLv, (double)N);
+            hss_2depth(c, e2dxu, hat[4], p->om2_m, p->om2_p, p->om2_c, L2x, (double)N);
+            hss_2depth(c, e2dyv, hat[5], p->om2_m, p->om2_p, p->om2_c, L2y, (double)N);
             #pragma omp parallel for schedule(static)
             for (size_t q = 0; q < 2 * N; q++) { LLu[q] /= g; LLv[q] /= g; L2x[q] /= g; L2y[q] /= g; }
         }

@@ -36,6 +36,7 @@ class OrcParams(ct.Structure):
         ("MaxEtaInit", ct.c_double), ("Tstar", ct.c_double), ("Rsearch", ct.c_double), ("U", ct.c_double),
         ("pbchar", _dp), ("depth", _dp), ("x", _dp), ("y", _dp),
         ("order", ct.c_int),
+        ("gam_c", _dp), ("csq_c", _dp), ("om2_c", _dp),
     ]
 
 
@@ -142,6 +143,8 @@ class Oracle2D:
             p.gam_m, p.gam_p = arr("gm", o["gam_min"]), arr("gp", o["gam_plus"])
             p.csq_m, p.csq_p = arr("cm", o["csq_min"]), arr("cp", o["csq_plus"])
             p.om2_m, p.om2_p = arr("om", o["om2_min"]), arr("op", o["om2_plus"])
+            if o.get("gam_mid") is not None:
+                p.gam_c, p.csq_c, p.om2_c = arr("gc", o["gam_mid"]), arr("cc", o["csq_mid"]), arr("oc", o["om2_mid"])
         p.source_on = int(model.source_on)
         if model.source_on:
             p.vertical, p.propagation = int(model.vertical), int(model.propagation)

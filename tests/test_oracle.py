@@ -340,3 +340,68 @@ def test_hs3_two_depth_with_unit_weights_equals_flat():
     db = HS.oracle_half(O.Oracle2D(mb).rhs(0.7, y), n, n)
     assert relerr(db[:, :, 1:], df[:, :, 1:]) < 1e-8
     assert relerr(db[:, : n // 2 + 1, 0], df[:, : n // 2 + 1, 0]) < 1e-8
+
+
+# ------------------------------------------------------- three depths ---
+def three_depth_model(kind="harbour", n=32, order=2):
+    m = W.harbour(n, t_end=3.0, breaking=False)
+    m.order = order
+    o = S.disp_interpolation_3p(m.depth, m.kx, m.ky, 2 * math.pi / 8.0)
+    m.ops = {**m.ops, **{k: o[k] for k in ("gam_min", "gam_mid", "gam_plus", "csq_min", "csq_mid", "csq_plus",
+                                           "om2_min", "om2_mid", "om2_plus")}}
+    return m
+
+
+def test_three_depth_weights_solve_the_interpolation_system():
+    """funOprt_DispInterpolation_3p: at every table depth the weights satisfy
+    sum_j g_j X(D_j) = X(D) for X = sqrt(g D), w^2, Cp at the peak
+    wavenumber; at the shallowest depth they are (1, 0, 0)."""
+    m = W.harbour(32, t_end=3.0)
+    nu = 2 * math.pi / 8.0
+    o = S.disp_interpolation_3p(m.depth, m.kx, m.ky, nu)
+    Ds = (o["d_min"], o["d_mid"], o["d_max"])
+    KK = np.sqrt(m.kx[None, :] ** 2 + m.ky[:, None] ** 2)
+    kmax = KK.max() if KK.max() >= nu else 2 * nu      # the reference's table extent rule
+    kk = np.arange(100) * kmax / 99
+    for i in (0, 17, 50, 99):
+        D = o["table_depth"][i]
+        kap = S.inv_om_spline(S.exact_disp1d(kk, D), kk, nu)
+        M = np.array([[math.sqrt(S.GRAV * d) for d in Ds], [float(S.exact_disp1d(kap, d)) ** 2 for d in Ds],
+                      [float(S.exact_cp1d(kap, d)) for d in Ds]])
+        rhs = np.array([math.sqrt(S.GRAV * D), float(S.exact_disp1d(kap, D)) ** 2, float(S.exact_cp1d(kap, D))])
+        assert np.allclose(o["table_gam"][:, i], np.linalg.solve(M, rhs), rtol=1e-9, atol=1e-9)
+    assert np.allclose(o["table_gam"][:, 0], [1, 0, 0], atol=1e-12)
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_three_depth_with_zero_middle_weight_is_two_depth(order):
+    m = three_depth_model(order=order)
+    n = m.nx
+    y = W.random_state(m, 9, kfrac=0.5, amp=0.3)
+    r3 = O.Oracle2D(m).rhs(0.4, y)
+    m.ops["gam_mid"] = np.zeros_like(m.ops["gam_mid"])
+    r3z = O.Oracle2D(m).rhs(0.4, y)
+    for k in ("gam_mid", "csq_mid", "om2_mid"):
+        del m.ops[k]
+    r2 = O.Oracle2D(m).rhs(0.4, y)
+    assert np.array_equal(r3z, r2)
+    assert relerr(r3, r2) > 1e-6      # the middle depth matters on this bathymetry
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_three_depth_with_unit_middle_weight_equals_flat(order):
+    """gamma = (0, 1, 0) with the middle multipliers of the flat operators
+    (C2d^2, g L2d) reduces every HSS to the flat multiplier."""
+    mf = W.short_crested(32, t_end=3.0)
+    mf.order = order
+    n = mf.nx
+    one, zero = np.ones((n, n)), np.zeros((n, n))
+    ops = dict(gam_min=zero, gam_plus=zero, gam_mid=one, csq_min=mf.c2d ** 2 * 0.3, csq_plus=mf.c2d ** 2 * 0.7,
+               csq_mid=mf.c2d ** 2, om2_min=S.GRAV * mf.l2d * 0.2, om2_plus=S.GRAV * mf.l2d * 3.0,
+               om2_mid=S.GRAV * mf.l2d)
+    mb = S.Model2D(**{**mf.__dict__, "flat": False, "ops": ops})
+    y = W.random_state(mf, 11, kfrac=0.5, amp=0.3)
+    df = HS.oracle_half(O.Oracle2D(mf).rhs(0.7, y), n, n)
+    db = HS.oracle_half(O.Oracle2D(mb).rhs(0.7, y), n, n)
+    assert relerr(db[:, :, 1:], df[:, :, 1:]) < 1e-8
+    assert relerr(db[:, : n // 2 + 1, 0], df[:, : n // 2 + 1, 0]) < 1e-8
