@@ -32,6 +32,7 @@ class HwConfig2D(ct.Structure):
         ("pbchar", _dp), ("depth", _dp), ("x", _dp), ("y", _dp),
         ("t0", ct.c_double),
         ("model", ct.c_int),
+        ("gam_mid", _dp), ("csq_mid", _dp), ("om2_mid", _dp),
     ]
 
 

@@ -14,7 +14,7 @@ before its time loop (2d/main.c:103-150), for what the device path supports
   newGam / SkewGam and the signals Signal / SkewSignal that fun_source
   splines in time (the library builds the same natural-spline tables);
 * the operators: fun_operator_setup (:1462-1533) with
-  funOprt_DispInterpolation_2p for a varying bottom.
+  funOprt_DispInterpolation_2p or _3p (depth_ref 2 or 3) for a varying bottom.
 """
 from __future__ import annotations
 
@@ -351,9 +351,12 @@ def model_from_input(case_dir: str, eps_abs: float = 1e-3, eps_rel: float = 1e-2
         ops = {}
     else:
         cpeak = float(S.exact_cp1d(kp, meandepth, p.U))
-        if p.ninterp != 2:
-            raise NotImplementedError("depth_ref = 3 (HSSgen_3depth) is not supported on the device")
-        ops = S.disp_interpolation_2p(-bathy, kx, ky, wp, p.current[0], p.current[1], p.mdir)
+        if p.ninterp == 2:
+            ops = S.disp_interpolation_2p(-bathy, kx, ky, wp, p.current[0], p.current[1], p.mdir)
+        elif p.ninterp == 3:
+            ops = S.disp_interpolation_3p(-bathy, kx, ky, wp, p.mid, p.current[0], p.current[1], p.mdir)
+        else:
+            raise ValueError(f"depth_ref = {p.ninterp}: the reference accepts 2 or 3")
     brk = None
     if p.breaking == "yes":
         brk = S.breaking_params(Hs, Tp, kp, cfsa, chi, KBC=p.KBC, TC=p.TC, Tchar=p.Tchar, delb=p.delb)

@@ -43,11 +43,11 @@ namespace hw {
 template <bool FLAT> struct H3Slots;
 template <> struct H3Slots<false> {
     enum : int { LL = 4, DGP = 5, A = 3, GMA = 6, GPA = 7, B = 9, CC = 10, D = 11, TM = 12, TP = 13,
-                 AX = S_AX, AY = S_AY };
+                 AX = S_AX, AY = S_AY, GCA = 15, TC = 9 };
 };
 template <> struct H3Slots<true> {
     enum : int { LL = 3, DGP = 4, A = 6, GMA = -1, GPA = -1, B = 8, CC = 9, D = 10, TM = -1, TP = -1,
-                 AX = F_AX, AY = F_AY };
+                 AX = F_AX, AY = F_AY, GCA = -1, TC = -1 };
 };
 
 // X5a: F(g- b- + g+ b+) of the two AB2 HSS tails (the first half of k2b_x5,
@@ -137,8 +137,8 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves
     y_store<T, NX, NY>(a.W[SL::DGP], X, col0, tid);
 }
 
-// XB: the physical products, forward in x
-template <typename T, int NX, int NY, bool FLAT>
+// XB: the physical products, forward in x (ND3: also g_mid A)
+template <typename T, int NX, int NY, bool FLAT, bool ND3 = false>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k3_xb(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
     using SL = H3Slots<FLAT>;
@@ -170,6 +170,12 @@ __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k3_xb(Rhs2DArgs<T> a) {
 #pragma unroll
         for (int q = 0; q < G::PP; ++q) dg[q] = pmul(ll[q], x[q]);
         x_fwd_store<T, NX, NY>(dg, a.W[SL::GPA], y0, lds, twxs, tid);
+        if constexpr (ND3) {
+            ph_load<T, NX, NY>(ll, a.gam_c, y0, tid);
+#pragma unroll
+            for (int q = 0; q < G::PP; ++q) dg[q] = pmul(ll[q], x[q]);
+            x_fwd_store<T, NX, NY>(dg, a.W[SL::GCA], y0, lds, twxs, tid);
+        }
     }
     x_fwd_store<T, NX, NY>(x, a.W[SL::A], y0, lds, twxs, tid);
     // B = eta^2 M0u w, Cc = eta^2 M0v w
@@ -188,7 +194,7 @@ __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k3_xb(Rhs2DArgs<T> a) {
 // YC: the eta term (two-depth: into dH2p, with the HSS tails inverted in y;
 // flat: inverted in y and added to the eta adjust input) and the velocity
 // terms added to the velocity adjust inputs
-template <typename T, int NX, int NY, bool FLAT>
+template <typename T, int NX, int NY, bool FLAT, bool ND3 = false>
 __global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves_per_eu(YGeo<T, NY>::WPE))) k3_yc(Rhs2DArgs<T> a) {
     HW_YK_PROLOGUE
     using SL = H3Slots<FLAT>;
@@ -214,6 +220,16 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves
             YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
             P[e] = kx < NX / 2 + 1 ? cscale(cadd(cscale(P[e], a.om2_m[gi]), cscale(Q[e], a.om2_p[gi])), (T)0.5 * ig)
                                    : cmk<C>((T)0, (T)0);
+        }
+        if constexpr (ND3) {
+            y_load<T, NX, NY>(Q, a.W[SL::GCA], nullptr, col0, tid);
+            y_fwd_regs<T, NX, NY>(Q, lds, twys, tid);
+#pragma unroll
+            for (int e = 0; e < G::PE; ++e) {
+                int kx, ky; size_t gi;
+                YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
+                if (kx < NX / 2 + 1) P[e] = cadd(P[e], cscale(Q[e], (T)0.5 * ig * a.om2_c[gi]));
+            }
         }
     }
     // - (kx^2 B^ + ky^2 Cc^) / 2
@@ -246,15 +262,16 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves
         y_load<T, NX, NY>(P, a.W[SL::A], nullptr, col0, tid);
         y_fwd_regs<T, NX, NY>(P, lds, twys, tid);
         #pragma unroll
-        for (int s = 0; s < 2; ++s) {
+        for (int s = 0; s < (ND3 ? 3 : 2); ++s) {
 #pragma unroll
             for (int e = 0; e < G::PE; ++e) {
                 int kx, ky; size_t gi;
                 YIdx<T, NX, NY>::get(col0, tid, e, kx, ky, gi);
-                Q[e] = kx < NX / 2 + 1 ? cscale(P[e], s == 0 ? a.om2_m[gi] : a.om2_p[gi]) : cmk<C>((T)0, (T)0);
+                const T* om = s == 0 ? a.om2_m : s == 1 ? a.om2_p : a.om2_c;
+                Q[e] = kx < NX / 2 + 1 ? cscale(P[e], om[gi]) : cmk<C>((T)0, (T)0);
             }
             y_inv_regs<T, NX, NY>(Q, lds, twys, col0, tid);
-            y_store<T, NX, NY>(a.W[s == 0 ? SL::TM : SL::TP], Q, col0, tid);
+            y_store<T, NX, NY>(a.W[s == 0 ? SL::TM : s == 1 ? SL::TP : SL::TC], Q, col0, tid);
         }
     }
     // velocity adjust inputs += i kx D^, i ky D^
@@ -277,8 +294,9 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves
     }
 }
 
-// X5b: F(g- b3- + g+ b3+) of the HS3 tail; the two velocity output transforms
-template <typename T, int NX, int NY>
+// X5b: F(g- b3- + g+ b3+) of the HS3 tail (ND3: + g_mid b3_mid); the two
+// velocity output transforms
+template <typename T, int NX, int NY, bool ND3 = false>
 __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k3b_x5b(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
     using SL = H3Slots<false>;
@@ -289,10 +307,16 @@ __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k3b_x5b(Rhs2DArgs<T> a) {
     ph_load<T, NX, NY>(gg, a.gam_m, y0, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) acc[q] = pmul(gg[q], f[q]);
-    x_inv_roll<T, NX, NY>(b, a.W[S_AX], y0, lds, twxs, f, sc, nullptr, tid);
+    x_inv_roll<T, NX, NY>(b, ND3 ? a.W[SL::TC] : a.W[S_AX], y0, lds, twxs, f, sc, nullptr, tid);
     ph_load<T, NX, NY>(gg, a.gam_p, y0, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) acc[q] = pfma(gg[q], f[q], acc[q]);
+    if constexpr (ND3) {
+        x_inv_roll<T, NX, NY>(b, a.W[S_AX], y0, lds, twxs, f, sc, nullptr, tid);
+        ph_load<T, NX, NY>(gg, a.gam_c, y0, tid);
+#pragma unroll
+        for (int q = 0; q < G::PP; ++q) acc[q] = pfma(gg[q], f[q], acc[q]);
+    }
     x_fwd_store<T, NX, NY>(acc, a.W[S3_ACC3], y0, lds, twxs, tid);
     x_vel_out<T, NX, NY>(a, b, a.W[S_AY], a.u_p, a.sbx, S_ZU, y0, lds, twxs, nyq, tid);
     x_vel_out<T, NX, NY>(a, b, nullptr, a.v_p, a.sby, S_ZV, y0, lds, twxs, nyq, tid);

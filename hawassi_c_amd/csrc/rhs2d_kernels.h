@@ -52,7 +52,10 @@ template <typename T> struct Rhs2DArgs {
     T g;
     T* s_eta_out;   // per-row imaginary Nyquist part of the eta adjust (read by breaking)
     C* eta_col0;    // when set: the pre-mask eta derivative of column kx = 0, [ky]
-    T *m0u_p, *m0v_p;   // physical M0u, M0v (HS3 only)
+    T *m0u_p, *m0v_p;   // physical M0u, M0v (HS3 and three reference depths)
+    // third reference depth (HSSgen_3depth): gamma_mid [y][x], C^2_mid and
+    // Om^2_mid [kx][ky]; null for two depths
+    const T *gam_c, *csq_c, *om2_c;
 };
 
 template <typename T, int NX, int NY> struct Geo2 {

@@ -13,7 +13,7 @@
 #include "../../include/hawassi_hip.h"
 #include "aux_kernels.h"
 #include "breaking2d.h"
-#include "rhs2d_hs3.h"
+#include "rhs2d_3depth.h"
 #include <algorithm>
 #include <climits>
 
@@ -132,6 +132,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
 
     DevBuf<T> csq_m, csq_p, om2_m, om2_p, aal, c2d2g, l2d, kxh, kyv;
     DevBuf<T> gam_m, gam_p, wall, fbdy7, chi, newgam, skewgam;
+    DevBuf<T> gam_c, csq_c, om2_c;   // third reference depth
     DevBuf<C> twx, twy;
     DevBuf<T> src_sig, src_skew;
     DevBuf<double> sig_y, sig_c, skew_y, skew_c;
@@ -177,6 +178,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
     bool flat = false;
     bool hs1 = false;   // HS1 dynamics (linear AB1 right-hand side)
     bool hs3 = false;   // HS3 dynamics (AB3: the AB2 terms plus the third-order ones)
+    bool three = false; // three reference depths (HSSgen_3depth)
     int src_on = 0;
     double g = 9.80665, eps_abs = 1e-3, eps_rel = 1e-2;
     // HIP events bracketing every RHS evaluation on the context's stream
@@ -252,6 +254,16 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
                 (rc = csq_m.upload(to_half(c->csq_min), stream)) || (rc = csq_p.upload(to_half(c->csq_plus), stream)) ||
                 (rc = om2_m.upload(to_half(c->om2_min), stream)) || (rc = om2_p.upload(to_half(c->om2_plus), stream)))
                 return rc;
+            three = c->gam_mid != nullptr;
+            if (three) {
+                if (!c->csq_mid || !c->om2_mid) {
+                    set_error("three reference depths need gam_mid, csq_mid and om2_mid");
+                    return HW_ERR_ARG;
+                }
+                if ((rc = gam_c.upload(to_phys(c->gam_mid), stream)) || (rc = csq_c.upload(to_half(c->csq_mid), stream)) ||
+                    (rc = om2_c.upload(to_half(c->om2_mid), stream)))
+                    return rc;
+            }
         }
         if ((rc = twx.upload(x_twiddles<T, NX>(), stream)) || (rc = twy.upload(y_twiddles<T, NY>(), stream))) return rc;
         src_on = c->source_on;
@@ -313,7 +325,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
             HW_TRY(hipEventCreateWithFlags(&ev_brk, hipEventDisableTiming));
         }
         if ((rc = eta_p.alloc(N)) || (rc = u_p.alloc(N)) || (rc = v_p.alloc(N))) return rc;
-        if (hs3 && ((rc = m0u_p.alloc(N)) || (rc = m0v_p.alloc(N)))) return rc;
+        if ((hs3 || three) && ((rc = m0u_p.alloc(N)) || (rc = m0v_p.alloc(N)))) return rc;
         if ((rc = work.alloc(18 * H))) return rc;
         if ((rc = y.alloc(3 * H)) || (rc = ynew.alloc(3 * H)) || (rc = kbuf.alloc(6 * 3 * H)) || (rc = ytmp.alloc(3 * H))) return rc;
         if ((rc = rmax.alloc(1))) return rc;
@@ -329,6 +341,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         args.sbx = nullptr; args.sby = nullptr;
         args.eta_p = eta_p.p; args.u_p = u_p.p; args.v_p = v_p.p;
         args.m0u_p = m0u_p.p; args.m0v_p = m0v_p.p;
+        args.gam_c = gam_c.p; args.csq_c = csq_c.p; args.om2_c = om2_c.p;
         for (int i = 0; i < 16; ++i) args.W[i] = work.p + (size_t)i * H;
         args.dH1 = work.p + 16 * H;
         args.dH2p = work.p + 17 * H;
@@ -394,12 +407,27 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         // 2d/fun_odesolver.c:4580-4587: breaking runs only when tprev < t
         const bool brk_now = brk_on && tprev < t;
         args.eta_col0 = brk_now ? col0.p : nullptr;
+        // the middle reference depth's share of each HSS (HSSgen_3depth)
+        auto mid_m0 = [&]() {
+            k3d_y1<T, NX, NY><<<gy, by, 0, stream>>>(args);
+            k3d_x2<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+            k3d_y3<T, NX, NY><<<gy, by, 0, stream>>>(args);
+        };
+        auto mid_lem0_a = [&]() {
+            k3d_x4<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+            k3d_y5<T, NX, NY><<<gy, by, 0, stream>>>(args);
+        };
+        auto mid_lem0_b = [&]() {
+            k3d_x6<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+            k_add_x<C><<<grid1(H), 256, 0, stream>>>(args.W[S_ACC1], args.W[E3_ACC], H);
+        };
         if (hs1) {
             // rhs_AB1_2DB_uv / rhs_AB1_2DF_uv (2d/fun_odesolver.c:4350-4456)
             if (!flat) {
                 k2b_y1<T, NX, NY><<<gy, by, 0, stream>>>(args);
                 k2b_x1<T, NX, NY><<<gx, bx, 0, stream>>>(args);
                 k2b_y2<T, NX, NY><<<gy, by, 0, stream>>>(args);
+                if (three) mid_m0();
             } else {
                 k2f_y1<T, NX, NY><<<gy, by, 0, stream>>>(args);
             }
@@ -411,13 +439,22 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
             k2b_x1<T, NX, NY><<<gx, bx, 0, stream>>>(args);
             if (brk_now) { int rc = breaking_detect(); if (rc) return rc; }
             k2b_y2<T, NX, NY><<<gy, by, 0, stream>>>(args);
+            if (three) mid_m0();
             k2b_x3<T, NX, NY, true><<<gx, bx, 0, stream>>>(args);
             k2b_y4<T, NX, NY><<<gy, by, 0, stream>>>(args);
+            if (three) mid_lem0_a();
             k3b_x5a<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+            if (three) mid_lem0_b();
             k3_ya<T, NX, NY, false><<<gy, by, 0, stream>>>(args);
-            k3_xb<T, NX, NY, false><<<gx, bx, 0, stream>>>(args);
-            k3_yc<T, NX, NY, false><<<gy, by, 0, stream>>>(args);
-            k3b_x5b<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+            if (three) {
+                k3_xb<T, NX, NY, false, true><<<gx, bx, 0, stream>>>(args);
+                k3_yc<T, NX, NY, false, true><<<gy, by, 0, stream>>>(args);
+                k3b_x5b<T, NX, NY, true><<<gx, bx, 0, stream>>>(args);
+            } else {
+                k3_xb<T, NX, NY, false><<<gx, bx, 0, stream>>>(args);
+                k3_yc<T, NX, NY, false><<<gy, by, 0, stream>>>(args);
+                k3b_x5b<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+            }
             k2b_y6<T, NX, NY, true><<<gy, by, 0, stream>>>(args);
             k2b_x7<T, NX, NY><<<gx, bx, 0, stream>>>(args);
             k2b_y8<T, NX, NY><<<gy, by, 0, stream>>>(args);
@@ -437,9 +474,18 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
             k2b_x1<T, NX, NY><<<gx, bx, 0, stream>>>(args);
             if (brk_now) { int rc = breaking_detect(); if (rc) return rc; }
             k2b_y2<T, NX, NY><<<gy, by, 0, stream>>>(args);
-            k2b_x3<T, NX, NY><<<gx, bx, 0, stream>>>(args);
-            k2b_y4<T, NX, NY><<<gy, by, 0, stream>>>(args);
-            k2b_x5<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+            if (three) {
+                mid_m0();
+                k2b_x3<T, NX, NY, true><<<gx, bx, 0, stream>>>(args);
+                k2b_y4<T, NX, NY><<<gy, by, 0, stream>>>(args);
+                mid_lem0_a();
+                k2b_x5<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+                mid_lem0_b();
+            } else {
+                k2b_x3<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+                k2b_y4<T, NX, NY><<<gy, by, 0, stream>>>(args);
+                k2b_x5<T, NX, NY><<<gx, bx, 0, stream>>>(args);
+            }
             k2b_y6<T, NX, NY><<<gy, by, 0, stream>>>(args);
             k2b_x7<T, NX, NY><<<gx, bx, 0, stream>>>(args);
             k2b_y8<T, NX, NY><<<gy, by, 0, stream>>>(args);

@@ -68,6 +68,8 @@ class Hawassi2D:
             c.gam_min, c.gam_plus = arr("gm", o["gam_min"]), arr("gp", o["gam_plus"])
             c.csq_min, c.csq_plus = arr("cm", o["csq_min"]), arr("cp", o["csq_plus"])
             c.om2_min, c.om2_plus = arr("om", o["om2_min"]), arr("op", o["om2_plus"])
+            if o.get("gam_mid") is not None:
+                c.gam_mid, c.csq_mid, c.om2_mid = arr("gc", o["gam_mid"]), arr("cc", o["csq_mid"]), arr("oc", o["om2_mid"])
         c.source_on = int(model.source_on)
         if model.source_on:
             c.vertical, c.propagation = int(model.vertical), int(model.propagation)

@@ -84,6 +84,12 @@ typedef struct hw_config_2d {
                                          2 or 0 "HS2" (rhs_AB2_2D*_uv); 3 "HS3" (rhs_AB3_2DF_uv /
                                          rhs_AB3_2DB_uv, 2d/fun_odesolver.c:4628-4843);
                                          2d/main.c:209-232 */
+    /* third reference depth (depth_ref = 3: funOprt_DispInterpolation_3p,
+       2d/functions.c:1627-1762; HSSgen_3depth, 2d/fun_odesolver.c:588-697),
+       two-depth bottom only; NULL for two reference depths */
+    const double *gam_mid;            /* Oprt.InterpD.Gam_mid [ny*nx] */
+    const double *csq_mid;            /* Oprt.InterpD.Csqr2d_mid [ny*nx] */
+    const double *om2_mid;            /* Oprt.InterpD.Om2dSq_mid [ny*nx] */
 } hw_config_2d;
 
 typedef struct hw_stats {
