@@ -31,7 +31,7 @@ struct Real1D {                 // read-only model description of one realizatio
     long long cb_off;           // offset into the crest arrays (sum of earlier caps)
     double dx, KBC, TC, Tstar, MaxEtaInit, delb, eps_abs, eps_rel;
     int xbstart, xbend, xbendL, xbstartR;
-    int lenIF, ILL, IFF, pad;
+    int lenIF, ILL, IFF, mid;   // mid: a third reference depth (Up3IP)
 };
 
 struct Rk1D {                   // driver state of one realization (FP64)
@@ -53,7 +53,7 @@ struct Brk1D {
     int pad[2];
 };
 
-enum : int { R1_ETA = 0, R1_U, R1_M0U, R1_H, R1_P2, R1_FLUX, R1_AU, R1_AE, R1_NR };
+enum : int { R1_ETA = 0, R1_U, R1_M0U, R1_H, R1_P2, R1_FLUX, R1_AU, R1_AE, R1_AC, R1_NR };
 enum : int { C1_EH = 0, C1_M0H, C1_EUH, C1_U2H, C1_HP, C1_UH, C1_NC };
 
 template <typename T> struct Batch1DArgs {
@@ -63,6 +63,7 @@ template <typename T> struct Batch1DArgs {
     const int* list;            // realization ids of this launch (blockIdx.x -> id)
     // per-element model arrays, offset `off`
     const T *k, *aal, *chi, *damp, *opa, *opb, *gm, *gp, *depth;
+    const T *opc, *gc;          // third reference depth: Csq_mid, gam_mid (zero when unused)
     const C *gamx, *gamxs;
     // influx signals (natural spline: values and half second derivatives)
     const double *tsig, *sig, *sigc, *skw, *skwc;
@@ -529,6 +530,11 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
     const T* opb = a.opb + off;
     const T* gmv = a.gm + off;
     const T* gpv = a.gp + off;
+    // third-depth arrays are addressed where used (a.opc[off + i], ...), so
+    // the two-depth path keeps no extra pointers live
+#define HW1_OPC(i) a.opc[off + (i)]
+#define HW1_GC(i) a.gc[off + (i)]
+#define HW1_AC(i) sr[R1_AC * N + (i)]
     T* s_eta = sr + R1_ETA * N;
     T* s_u = sr + R1_U * N;
     T* s_m0 = sr + R1_M0U * N;
@@ -592,7 +598,10 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
         s_eta[i0] = etl; s_eta[i1] = eth;
         s_u[i0] = ul; s_u[i1] = uh;
         const C u0 = c_uh[i0], u1 = c_uh[i1];
-        if (!R.flat) {
+        if (!R.flat && R.mid) {
+            // the middle depth first: F^-1(C2_mid u^) is kept in s_ac
+            P::inv_in(lds, tid, e, cscale(u0, HW1_OPC(i0)), cscale(u1, HW1_OPC(i1)), cmk<C>((T)0, (T)0), cmk<C>((T)0, (T)0));
+        } else if (!R.flat) {
             P::inv_in(lds, tid, e, cscale(u0, opa[i0]), cscale(u1, opa[i1]), cscale(u0, opb[i0]), cscale(u1, opb[i1]));
         } else {
             const C m0l = cmul_i<-1>(cscale(u0, kk[i0] * opa[i0] / g));
@@ -602,15 +611,29 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
         }
     }
     if (!R.flat) {
+        if (R.mid) {
+            P::template run<1>(lds, tw, tid);
+#pragma unroll 1
+            for (int e = 0; e < EH; ++e) {
+                const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+                T acl, ach, z0, z1;
+                P::inv_out(lds, tid, e, acl, ach, z0, z1);
+                HW1_AC(i0) = acl; HW1_AC(i1) = ach;
+                const C u0 = c_uh[i0], u1 = c_uh[i1];
+                P::inv_in(lds, tid, e, cscale(u0, opa[i0]), cscale(u1, opa[i1]), cscale(u0, opb[i0]), cscale(u1, opb[i1]));
+            }
+        }
         P::template run<1>(lds, tw, tid);
-        // (a-, a+) = F^-1(C2-+ u^); F(g- a- + g+ a+), F(g- u)
+        // (a-, a+) = F^-1(C2-+ u^); F(g- a- + g+ a+ [+ g_mid a_mid]), F(g- u)
 #pragma unroll 1
         for (int e = 0; e < EH; ++e) {
             const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
             T aml, amh, apl, aph;
             P::inv_out(lds, tid, e, aml, amh, apl, aph);
             const T g0 = gmv[i0], g1 = gmv[i1], q0 = gpv[i0], q1 = gpv[i1];
-            P::fwd_in(lds, tid, e, g0 * aml + q0 * apl, g1 * amh + q1 * aph, g0 * s_u[i0], g1 * s_u[i1]);
+            T f0 = g0 * aml + q0 * apl, f1 = g1 * amh + q1 * aph;
+            if (R.mid) { f0 += HW1_GC(i0) * HW1_AC(i0); f1 += HW1_GC(i1) * HW1_AC(i1); }
+            P::fwd_in(lds, tid, e, f0, f1, g0 * s_u[i0], g1 * s_u[i1]);
         }
         P::template run<-1>(lds, tw, tid);
         // HP = (F(g- a- + g+ a+) + C- F(g- u)) / 2; then F(g+ u), F(eta u)
@@ -625,14 +648,34 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
             P::fwd_in(lds, tid, e, gpv[i0] * u0, gpv[i1] * u1, s_eta[i0] * u0, s_eta[i1] * u1);
         }
         P::template run<-1>(lds, tw, tid);
+        if (R.mid) {
+            // + C2_mid F(g_mid u) / 2 needs one more forward transform
+#pragma unroll 1
+            for (int e = 0; e < EH; ++e) {
+                const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+                C xl, xh, yl, yh;
+                P::fwd_out(lds, tid, e, xl, xh, yl, yh);
+                c_euh[i0] = yl; c_euh[i1] = yh;   // F(eta u)
+                c_hp[i0] = cadd(c_hp[i0], cscale(xl, (T)0.5 * opb[i0]));
+                c_hp[i1] = cadd(c_hp[i1], cscale(xh, (T)0.5 * opb[i1]));
+                P::fwd_in(lds, tid, e, HW1_GC(i0) * s_u[i0], HW1_GC(i1) * s_u[i1], (T)0, (T)0);
+            }
+            P::template run<-1>(lds, tw, tid);
+        }
 #pragma unroll 1
         for (int e = 0; e < EH; ++e) {
             const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
             C xl, xh, yl, yh;
             P::fwd_out(lds, tid, e, xl, xh, yl, yh);
-            c_euh[i0] = yl; c_euh[i1] = yh;   // F(eta u)
-            const C H0 = cadd(c_hp[i0], cscale(xl, (T)0.5 * opb[i0]));
-            const C H1 = cadd(c_hp[i1], cscale(xh, (T)0.5 * opb[i1]));
+            C H0, H1;
+            if (R.mid) {
+                H0 = cadd(c_hp[i0], cscale(xl, (T)0.5 * HW1_OPC(i0)));
+                H1 = cadd(c_hp[i1], cscale(xh, (T)0.5 * HW1_OPC(i1)));
+            } else {
+                c_euh[i0] = yl; c_euh[i1] = yh;   // F(eta u)
+                H0 = cadd(c_hp[i0], cscale(xl, (T)0.5 * opb[i0]));
+                H1 = cadd(c_hp[i1], cscale(xh, (T)0.5 * opb[i1]));
+            }
             const C m0l = cmul_i<-1>(cscale(H0, kk[i0] / g));
             const C m0h = cmul_i<-1>(cscale(H1, kk[i1] / g));
             c_m0h[i0] = m0l; c_m0h[i1] = m0h;
@@ -667,7 +710,25 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
             C eml, emh, u2l, u2h;
             P::fwd_out(lds, tid, e, eml, emh, u2l, u2h);
             c_u2h[i0] = u2l; c_u2h[i1] = u2h;
-            P::inv_in(lds, tid, e, cscale(eml, opa[i0]), cscale(emh, opa[i1]), cscale(eml, opb[i0]), cscale(emh, opb[i1]));
+            if (R.mid) {
+                // the middle depth first; F(eta M0u) waits in c_hp
+                c_hp[i0] = eml; c_hp[i1] = emh;
+                P::inv_in(lds, tid, e, cscale(eml, HW1_OPC(i0)), cscale(emh, HW1_OPC(i1)), cmk<C>((T)0, (T)0), cmk<C>((T)0, (T)0));
+            } else {
+                P::inv_in(lds, tid, e, cscale(eml, opa[i0]), cscale(emh, opa[i1]), cscale(eml, opb[i0]), cscale(emh, opb[i1]));
+            }
+        }
+        if (R.mid) {
+            P::template run<1>(lds, tw, tid);
+#pragma unroll 1
+            for (int e = 0; e < EH; ++e) {
+                const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+                T bcl, bch, z0, z1;
+                P::inv_out(lds, tid, e, bcl, bch, z0, z1);
+                HW1_AC(i0) = bcl; HW1_AC(i1) = bch;
+                const C eml = c_hp[i0], emh = c_hp[i1];
+                P::inv_in(lds, tid, e, cscale(eml, opa[i0]), cscale(emh, opa[i1]), cscale(eml, opb[i0]), cscale(emh, opb[i1]));
+            }
         }
         P::template run<1>(lds, tw, tid);
 #pragma unroll 1
@@ -676,7 +737,9 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
             T bml, bmh, bpl, bph;
             P::inv_out(lds, tid, e, bml, bmh, bpl, bph);
             const T g0 = gmv[i0], g1 = gmv[i1], q0 = gpv[i0], q1 = gpv[i1];
-            P::fwd_in(lds, tid, e, g0 * bml + q0 * bpl, g1 * bmh + q1 * bph, g0 * s_p[i0], g1 * s_p[i1]);
+            T f0 = g0 * bml + q0 * bpl, f1 = g1 * bmh + q1 * bph;
+            if (R.mid) { f0 += HW1_GC(i0) * HW1_AC(i0); f1 += HW1_GC(i1) * HW1_AC(i1); }
+            P::fwd_in(lds, tid, e, f0, f1, g0 * s_p[i0], g1 * s_p[i1]);
         }
         P::template run<-1>(lds, tw, tid);
 #pragma unroll 1
@@ -686,7 +749,8 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
             P::fwd_out(lds, tid, e, xl, xh, yl, yh);
             c_hp[i0] = cscale(cadd(xl, cscale(yl, opa[i0])), (T)0.5);
             c_hp[i1] = cscale(cadd(xh, cscale(yh, opa[i1])), (T)0.5);
-            P::fwd_in(lds, tid, e, gpv[i0] * s_p[i0], gpv[i1] * s_p[i1], (T)0, (T)0);
+            // F(g+ p) and (middle depth) F(g_mid p)
+            P::fwd_in(lds, tid, e, gpv[i0] * s_p[i0], gpv[i1] * s_p[i1], HW1_GC(i0) * s_p[i0], HW1_GC(i1) * s_p[i1]);
         }
         P::template run<-1>(lds, tw, tid);
     } else {
@@ -712,8 +776,11 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
         if (!R.flat) {
             C xl, xh, yl, yh;
             P::fwd_out(lds, tid, e, xl, xh, yl, yh);
-            lel = cscale(cadd(c_hp[i0], cscale(xl, (T)0.5 * opb[i0])), k0 * k0 / g);
-            leh = cscale(cadd(c_hp[i1], cscale(xh, (T)0.5 * opb[i1])), k1 * k1 / g);
+            C hl = cadd(c_hp[i0], cscale(xl, (T)0.5 * opb[i0]));
+            C hh = cadd(c_hp[i1], cscale(xh, (T)0.5 * opb[i1]));
+            if (R.mid) { hl = cadd(hl, cscale(yl, (T)0.5 * HW1_OPC(i0))); hh = cadd(hh, cscale(yh, (T)0.5 * HW1_OPC(i1))); }
+            lel = cscale(hl, k0 * k0 / g);
+            leh = cscale(hh, k1 * k1 / g);
             u2l = c_u2h[i0]; u2h = c_u2h[i1];
         } else {
             C yl, yh;
@@ -801,6 +868,9 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
         }
     }
 }
+#undef HW1_OPC
+#undef HW1_GC
+#undef HW1_AC
 
 template <typename T, int NT>
 __global__ void __launch_bounds__(NT) k1d_rhs(Batch1DArgs<T> a, int stage) {

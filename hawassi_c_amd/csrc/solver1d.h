@@ -65,6 +65,7 @@ template <typename T> struct Solver1DT : Solver1D {
     DevBuf<Rk1D> d_rk;
     DevBuf<int> d_list, d_count;
     DevBuf<T> k, aal, chi, damp, opa, opb, gm, gp, depth, sr, B;
+    DevBuf<T> opc, gc;   // third reference depth (zero where unused)
     DevBuf<C> gamx, gamxs, Kb[7], sc;
     DevBuf<double> tsig, sig, sigc, skw, skwc;
     DevBuf<double2> y, ext;
@@ -118,6 +119,8 @@ template <typename T> struct Solver1DT : Solver1D {
             R.delb = c.delb; R.eps_abs = c.eps_abs; R.eps_rel = c.eps_rel;
             R.xbstart = c.xbstart; R.xbend = c.xbend; R.xbendL = c.xbendL; R.xbstartR = c.xbstartR;
             R.lenIF = c.lenIF; R.ILL = c.ILL; R.IFF = c.IFF;
+            R.mid = !c.flat && c.gam_mid != nullptr;
+            if (R.mid && !c.csq_mid) { set_error("realization %d: gam_mid without csq_mid", r); return HW_ERR_ARG; }
             if (c.breaking_on) {
                 int hi = R.xbstart + (R.xbendL == 0 && R.xbstartR == 0 ? R.lenIF : R.ILL);
                 int hi2 = R.xbstartR + R.IFF;
@@ -141,7 +144,7 @@ template <typename T> struct Solver1DT : Solver1D {
         HW_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         // ---- per-element arrays
         std::vector<T> hk(ntot), haal(ntot), hchi(ntot), hdamp(ntot), hopa(ntot), hopb(ntot), hgm(ntot), hgp(ntot),
-            hdep(ntot);
+            hdep(ntot), hopc(ntot, (T)0), hgc(ntot, (T)0);
         std::vector<C> hgx(ntot), hgs(ntot);
         std::vector<double> hts(sigtot), hsig(sigtot), hsigc(sigtot), hskw(sigtot), hskwc(sigtot);
         for (int r = 0; r < n; ++r) {
@@ -155,6 +158,7 @@ template <typename T> struct Solver1DT : Solver1D {
                 else {
                     hopa[q] = (T)c.csq_min[i]; hopb[q] = (T)c.csq_plus[i];
                     hgm[q] = (T)c.gam_min[i]; hgp[q] = (T)c.gam_plus[i];
+                    if (R.mid) { hopc[q] = (T)c.csq_mid[i]; hgc[q] = (T)c.gam_mid[i]; }
                 }
                 if (c.source_on) {
                     hgx[q].x = (T)c.gamx_re[i]; hgx[q].y = (T)c.gamx_im[i];
@@ -175,7 +179,8 @@ template <typename T> struct Solver1DT : Solver1D {
         if ((rc = k.upload(hk, stream)) || (rc = aal.upload(haal, stream)) || (rc = chi.upload(hchi, stream)) ||
             (rc = damp.upload(hdamp, stream)) || (rc = opa.upload(hopa, stream)) || (rc = opb.upload(hopb, stream)) ||
             (rc = gm.upload(hgm, stream)) || (rc = gp.upload(hgp, stream)) || (rc = depth.upload(hdep, stream)) ||
-            (rc = gamx.upload(hgx, stream)) || (rc = gamxs.upload(hgs, stream)))
+            (rc = gamx.upload(hgx, stream)) || (rc = gamxs.upload(hgs, stream)) || (rc = opc.upload(hopc, stream)) ||
+            (rc = gc.upload(hgc, stream)))
             return rc;
         if (sigtot) {
             if ((rc = tsig.upload(hts, stream)) || (rc = sig.upload(hsig, stream)) || (rc = sigc.upload(hsigc, stream)) ||
@@ -227,6 +232,7 @@ template <typename T> struct Solver1DT : Solver1D {
         a.real = d_real.p; a.rk = d_rk.p; a.list = d_list.p;
         a.k = k.p; a.aal = aal.p; a.chi = chi.p; a.damp = damp.p; a.opa = opa.p; a.opb = opb.p;
         a.gm = gm.p; a.gp = gp.p; a.depth = depth.p; a.gamx = gamx.p; a.gamxs = gamxs.p;
+        a.opc = opc.p; a.gc = gc.p;
         a.tsig = tsig.p; a.sig = sig.p; a.sigc = sigc.p; a.skw = skw.p; a.skwc = skwc.p;
         a.y = y.p;
         for (int j = 0; j < 7; ++j) a.K[j] = Kb[j].p;

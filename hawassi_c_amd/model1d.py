@@ -50,6 +50,8 @@ class Hawassi1D:
                 o = m.ops
                 c.csq_min, c.csq_plus = arr(o["csq_min"]), arr(o["csq_plus"])
                 c.gam_min, c.gam_plus = arr(o["gam_min"]), arr(o["gam_plus"])
+                if o.get("gam_mid") is not None:
+                    c.csq_mid, c.gam_mid = arr(o["csq_mid"]), arr(o["gam_mid"])
             if m.source_on:
                 c.gamx_re, c.gamx_im = arr(m.gamx.real), arr(m.gamx.imag)
                 c.gamx_skew_re, c.gamx_skew_im = arr(m.gamx_skew.real), arr(m.gamx_skew.imag)

@@ -195,6 +195,41 @@ def up2ip(depth: np.ndarray, k: np.ndarray, nu: float, n: int = 101):
                 csq_plus=phase_velocity_v(k, dplus) ** 2, dmin=dmin, dplus=dplus)
 
 
+def up3ip(depth: np.ndarray, k: np.ndarray, nu: float, mid: str = "default", n: int = 101):
+    """Up3IP (1d/functions.c:2247-2340): three reference depths D-, Dmid (the
+    midpoint, or the depth_ref value), D+.  On a 101-point depth table the
+    weights solve sum_j g_j X(D_j) = X(D) for X = sqrt(g D), omega and Cp^2
+    at kappa(nu) by Cramer's rule, splined in D; Csq_* = Cp(k, D_*)^2."""
+    dmin, dplus = float(depth.min()), float(depth.max())
+    dmid = (dplus + dmin) / 2 if mid == "default" else float(mid)
+    dd = (dplus - dmin) / (n - 1)
+    D = dmin + np.arange(n) * dd
+    G = np.empty((3, n))
+    c0 = [math.sqrt(GRAV * dmin), math.sqrt(GRAV * dmid), math.sqrt(GRAV * dplus)]
+    for i in range(n):
+        kap = invers_omega(nu, D[i])
+        cp = [phase_velocity_i(kap, d) ** 2 for d in (dmin, dmid, dplus)]
+        om = [omega_i(kap, d) for d in (dmin, dmid, dplus)]
+        c0d, omd, cpd = math.sqrt(GRAV * D[i]), omega_i(kap, D[i]), phase_velocity_i(kap, D[i]) ** 2
+        A = om[1] * cp[2] - om[2] * cp[1]
+        B = -(om[0] * cp[2]) + om[2] * cp[0]
+        C = om[0] * cp[1] - om[1] * cp[0]
+        Dd = -(c0[1] * cp[2]) + c0[2] * cp[1]
+        E = c0[0] * cp[2] - c0[2] * cp[0]
+        F = -(c0[0] * cp[1]) + c0[1] * cp[0]
+        Gg = c0[1] * om[2] - c0[2] * om[1]
+        H = -(c0[0] * om[2]) + c0[2] * om[0]
+        J = c0[0] * om[1] - c0[1] * om[0]
+        det = c0[0] * A + c0[1] * B + c0[2] * C
+        G[0, i] = (A * c0d + Dd * omd + Gg * cpd) / det
+        G[1, i] = (B * c0d + E * omd + H * cpd) / det
+        G[2, i] = (C * c0d + F * omd + J * cpd) / det
+    gam = [_spline_eval_vec(D, G[j], spline_coeffs(D, G[j]), depth) for j in range(3)]
+    return dict(gam_min=gam[0], gam_mid=gam[1], gam_plus=gam[2], csq_min=phase_velocity_v(k, dmin) ** 2,
+                csq_mid=phase_velocity_v(k, dmid) ** 2, csq_plus=phase_velocity_v(k, dplus) ** 2,
+                dmin=dmin, dmid=dmid, dplus=dplus, table_depth=D, table_gam=G)
+
+
 def flat_operators(k: np.ndarray, depthinf: float):
     """fun_csqr / operator_L (1d/functions.c:1430-1436, 1796-1800)."""
     csqr = phase_velocity_v(k, depthinf) ** 2

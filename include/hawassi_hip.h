@@ -191,6 +191,9 @@ typedef struct hw_config_1d {
     const double *depth;      /* positive depth [n] */
     double eps_abs, eps_rel;  /* control_y_new (1d/functions.c:529-534: 1e-5, 1e-4) */
     double t0, h0;            /* initial time and step (1d/functions.c:540-541: h = 1e-4) */
+    /* third reference depth (mid[0] == 3: Up3IP, 1d/functions.c:2247-2340, and
+       the middle term of HSS, :2929-2975); NULL for two reference depths */
+    const double *csq_mid, *gam_mid;
 } hw_config_1d;
 
 /* Build a batch of nreal realizations; precision applies to all. */

@@ -9,7 +9,7 @@
  *   rhs_HsF2                      1d/rhs.c:96-241        -> orc1d_rhs
  *   define_delH1                  1d/functions.c:3414-3438
  *   ifft_fft_Hsf2 / _v            1d/functions.c:1458-1526, 3334-3412
- *   HSS (two reference depths)    1d/functions.c:2861-2927 (the mid[0] != 3 part)
+ *   HSS (two or three depths)     1d/functions.c:2861-2977
  *   Op_M0 / Op_M1                 1d/functions.c:2979-3016
  *   adjust_delH                   1d/rhs.c:695-715
  *   define_forcing                1d/rhs.c:659-693
@@ -55,6 +55,8 @@ typedef struct {
     double KBC, TC, Tstar, MaxEtaInit, delb, dx;
     int xbstart, xbend, xbendL, xbstartR, lenIF, ILL, IFF;
     const double *depth, *x;
+    /* third reference depth (mid[0] == 3: Up3IP, 1d/functions.c:2247-2340); NULL for two */
+    const double *csq_mid, *gam_mid;
 } orc1d_params;
 
 typedef struct {
@@ -102,7 +104,8 @@ static void fft_re(const orc1d_ctx *c, const double *in, double *Xh)
     fft1(c->plan, Xh, -1);
 }
 
-/* HSS with two reference depths (1d/functions.c:2861-2927 without the mid part) */
+/* HSS (1d/functions.c:2861-2977): two reference depths, plus the middle
+ * depth's (F(g_mid F^-1(C2_mid u^)) + C2_mid F(g_mid u)) / 2 when set */
 static void hss(const orc1d_ctx *c, const double *uh, double *res)
 {
     const orc1d_params *p = &c->p;
@@ -128,6 +131,17 @@ static void hss(const orc1d_ctx *c, const double *uh, double *res)
     for (int i = 0; i < N; i++)
         for (int r = 0; r < 2; r++)
             res[2 * i + r] = (h1m[2 * i + r] + h1p[2 * i + r] + p->csq_min[i] * fgm[2 * i + r] + p->csq_plus[i] * fgp[2 * i + r]) / 2;
+    if (p->gam_mid) {
+        for (int i = 0; i < N; i++) { cm[2 * i] = p->csq_mid[i] * uh[2 * i]; cm[2 * i + 1] = p->csq_mid[i] * uh[2 * i + 1]; }
+        ifft_re(c, cm, am);
+        for (int i = 0; i < N; i++) f[i] = p->gam_mid[i] * am[i];
+        fft_re(c, f, h1m);
+        for (int i = 0; i < N; i++) f[i] = p->gam_mid[i] * u[i];
+        fft_re(c, f, fgm);
+        for (int i = 0; i < N; i++)
+            for (int r = 0; r < 2; r++)
+                res[2 * i + r] = res[2 * i + r] + (h1m[2 * i + r] + p->csq_mid[i] * fgm[2 * i + r]) / 2;
+    }
     free(cm); free(cp); free(u); free(am); free(ap); free(f); free(h1m); free(h1p); free(fgm); free(fgp);
 }
 

@@ -53,6 +53,7 @@ class Orc1DParams(ct.Structure):
         ("xbstart", ct.c_int), ("xbend", ct.c_int), ("xbendL", ct.c_int), ("xbstartR", ct.c_int),
         ("lenIF", ct.c_int), ("ILL", ct.c_int), ("IFF", ct.c_int),
         ("depth", _dp), ("x", _dp),
+        ("csq_mid", _dp), ("gam_mid", _dp),
     ]
 
 
@@ -242,6 +243,8 @@ class Oracle1D:
             o = m.ops
             p.csq_min, p.csq_plus = arr("cm", o["csq_min"]), arr("cp", o["csq_plus"])
             p.gam_min, p.gam_plus = arr("gm", o["gam_min"]), arr("gp", o["gam_plus"])
+            if o.get("gam_mid") is not None:
+                p.csq_mid, p.gam_mid = arr("cc", o["csq_mid"]), arr("gc", o["gam_mid"])
         if m.source_on:
             p.gamx, p.gamx_skew = arr("gx", _cplx(m.gamx)), arr("gs", _cplx(m.gamx_skew))
             p.nt = len(m.t_sig)

@@ -172,3 +172,48 @@ def test_integrate_reaches_target_and_counts():
     assert rc == 0 and t == 0.5 and st[0] > 0
     # 5 stage evaluations + dydt_out per attempt, dydt_in once per evolve_apply
     assert o.rhs_count() == 6 * (st[0] + st[1]) + st[0]
+
+
+# ------------------------------------------------------- three depths ---
+def test_three_depth_middle_weight_reduces_to_flat_and_to_two_depths():
+    """Up3IP / HSS with a middle depth: gamma = (0, 1, 0) with C2_mid the
+    flat multiplier is the flat operator; a zero middle weight is the
+    two-depth operator bit for bit."""
+    mf = _bare(W1.channel(256, t_end=2.0))
+    n = mf.n
+    sym = _sym_multiplier(n, 200.0, 1.0)
+    mf.csqr, mf.lop = sym, mf.k ** 2 * sym / S.GRAV
+    one, zero = np.ones(n), np.zeros(n)
+    ops = dict(gam_min=zero, gam_plus=zero, gam_mid=one, csq_min=0.3 * sym, csq_plus=2.0 * sym, csq_mid=sym)
+    mb = S.Model1D(**{**mf.__dict__, "flat": False, "ops": ops})
+    y = W1.random_state(mf, 5, amp=0.05, kfrac=0.3)
+    df = O.Oracle1D(mf).rhs(0.3, y)
+    db = O.Oracle1D(mb).rhs(0.3, y)
+    assert relerr(db, df) < 1e-12
+    m3 = W1.slope(512, t_end=2.0, breaking=False)
+    o3 = S.up3ip(m3.depth, m3.k, 2 * math.pi / 2.0)
+    m3.ops = dict(o3)
+    y = W1.random_state(m3, 6, amp=0.05, kfrac=0.3)
+    r3 = O.Oracle1D(m3).rhs(0.3, y)
+    m3.ops["gam_mid"] = np.zeros(m3.n)
+    r3z = O.Oracle1D(m3).rhs(0.3, y)
+    m3.ops = {k: v for k, v in o3.items() if k not in ("gam_mid", "csq_mid")}
+    r2 = O.Oracle1D(m3).rhs(0.3, y)
+    assert np.array_equal(r3z, r2)
+    assert relerr(r3, r2) > 1e-6
+
+
+def test_up3ip_solves_the_interpolation_system():
+    m = W1.slope(512, t_end=2.0, breaking=False)
+    nu = 2 * math.pi / 2.0
+    o = S.up3ip(m.depth, m.k, nu)
+    Ds = (o["dmin"], o["dmid"], o["dplus"])
+    for i in (0, 33, 100):
+        D = o["table_depth"][i]
+        kap = S.invers_omega(nu, D)
+        M = np.array([[math.sqrt(S.GRAV * d) for d in Ds], [S.omega_i(kap, d) for d in Ds],
+                      [S.phase_velocity_i(kap, d) ** 2 for d in Ds]])
+        rhs = np.array([math.sqrt(S.GRAV * D), S.omega_i(kap, D), S.phase_velocity_i(kap, D) ** 2])
+        assert np.allclose(o["table_gam"][:, i], np.linalg.solve(M, rhs), rtol=1e-9, atol=1e-9)
+    assert np.allclose(o["table_gam"][:, 0], [1, 0, 0], atol=1e-12)
+    assert np.allclose(o["table_gam"][:, 100], [0, 0, 1], atol=1e-12)
