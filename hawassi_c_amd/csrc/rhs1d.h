@@ -479,7 +479,9 @@ __device__ __attribute__((noinline)) void brk1d_process(const Batch1DArgs<T>& a,
 }
 
 // ------------------------------------------------------------ the RHS ----
-template <typename T, int N, int NT>
+// MID: the realization has a third reference depth (its code is a separate
+// instance so that the two-depth path keeps its register allocation)
+template <typename T, int N, int NT, bool MID>
 __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename ctype<T>::type* lds, int* sm,
                            double* smd) {
     using C = typename ctype<T>::type;
@@ -598,7 +600,7 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
         s_eta[i0] = etl; s_eta[i1] = eth;
         s_u[i0] = ul; s_u[i1] = uh;
         const C u0 = c_uh[i0], u1 = c_uh[i1];
-        if (!R.flat && R.mid) {
+        if (!R.flat && MID) {
             // the middle depth first: F^-1(C2_mid u^) is kept in s_ac
             P::inv_in(lds, tid, e, cscale(u0, HW1_OPC(i0)), cscale(u1, HW1_OPC(i1)), cmk<C>((T)0, (T)0), cmk<C>((T)0, (T)0));
         } else if (!R.flat) {
@@ -611,7 +613,7 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
         }
     }
     if (!R.flat) {
-        if (R.mid) {
+        if (MID) {
             P::template run<1>(lds, tw, tid);
 #pragma unroll 1
             for (int e = 0; e < EH; ++e) {
@@ -632,7 +634,7 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
             P::inv_out(lds, tid, e, aml, amh, apl, aph);
             const T g0 = gmv[i0], g1 = gmv[i1], q0 = gpv[i0], q1 = gpv[i1];
             T f0 = g0 * aml + q0 * apl, f1 = g1 * amh + q1 * aph;
-            if (R.mid) { f0 += HW1_GC(i0) * HW1_AC(i0); f1 += HW1_GC(i1) * HW1_AC(i1); }
+            if (MID) { f0 += HW1_GC(i0) * HW1_AC(i0); f1 += HW1_GC(i1) * HW1_AC(i1); }
             P::fwd_in(lds, tid, e, f0, f1, g0 * s_u[i0], g1 * s_u[i1]);
         }
         P::template run<-1>(lds, tw, tid);
@@ -648,7 +650,7 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
             P::fwd_in(lds, tid, e, gpv[i0] * u0, gpv[i1] * u1, s_eta[i0] * u0, s_eta[i1] * u1);
         }
         P::template run<-1>(lds, tw, tid);
-        if (R.mid) {
+        if (MID) {
             // + C2_mid F(g_mid u) / 2 needs one more forward transform
 #pragma unroll 1
             for (int e = 0; e < EH; ++e) {
@@ -668,7 +670,7 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
             C xl, xh, yl, yh;
             P::fwd_out(lds, tid, e, xl, xh, yl, yh);
             C H0, H1;
-            if (R.mid) {
+            if (MID) {
                 H0 = cadd(c_hp[i0], cscale(xl, (T)0.5 * HW1_OPC(i0)));
                 H1 = cadd(c_hp[i1], cscale(xh, (T)0.5 * HW1_OPC(i1)));
             } else {
@@ -710,7 +712,7 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
             C eml, emh, u2l, u2h;
             P::fwd_out(lds, tid, e, eml, emh, u2l, u2h);
             c_u2h[i0] = u2l; c_u2h[i1] = u2h;
-            if (R.mid) {
+            if (MID) {
                 // the middle depth first; F(eta M0u) waits in c_hp
                 c_hp[i0] = eml; c_hp[i1] = emh;
                 P::inv_in(lds, tid, e, cscale(eml, HW1_OPC(i0)), cscale(emh, HW1_OPC(i1)), cmk<C>((T)0, (T)0), cmk<C>((T)0, (T)0));
@@ -718,7 +720,7 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
                 P::inv_in(lds, tid, e, cscale(eml, opa[i0]), cscale(emh, opa[i1]), cscale(eml, opb[i0]), cscale(emh, opb[i1]));
             }
         }
-        if (R.mid) {
+        if (MID) {
             P::template run<1>(lds, tw, tid);
 #pragma unroll 1
             for (int e = 0; e < EH; ++e) {
@@ -738,7 +740,7 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
             P::inv_out(lds, tid, e, bml, bmh, bpl, bph);
             const T g0 = gmv[i0], g1 = gmv[i1], q0 = gpv[i0], q1 = gpv[i1];
             T f0 = g0 * bml + q0 * bpl, f1 = g1 * bmh + q1 * bph;
-            if (R.mid) { f0 += HW1_GC(i0) * HW1_AC(i0); f1 += HW1_GC(i1) * HW1_AC(i1); }
+            if (MID) { f0 += HW1_GC(i0) * HW1_AC(i0); f1 += HW1_GC(i1) * HW1_AC(i1); }
             P::fwd_in(lds, tid, e, f0, f1, g0 * s_p[i0], g1 * s_p[i1]);
         }
         P::template run<-1>(lds, tw, tid);
@@ -750,7 +752,8 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
             c_hp[i0] = cscale(cadd(xl, cscale(yl, opa[i0])), (T)0.5);
             c_hp[i1] = cscale(cadd(xh, cscale(yh, opa[i1])), (T)0.5);
             // F(g+ p) and (middle depth) F(g_mid p)
-            P::fwd_in(lds, tid, e, gpv[i0] * s_p[i0], gpv[i1] * s_p[i1], HW1_GC(i0) * s_p[i0], HW1_GC(i1) * s_p[i1]);
+            if (MID) P::fwd_in(lds, tid, e, gpv[i0] * s_p[i0], gpv[i1] * s_p[i1], HW1_GC(i0) * s_p[i0], HW1_GC(i1) * s_p[i1]);
+            else P::fwd_in(lds, tid, e, gpv[i0] * s_p[i0], gpv[i1] * s_p[i1], (T)0, (T)0);
         }
         P::template run<-1>(lds, tw, tid);
     } else {
@@ -778,7 +781,7 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
             P::fwd_out(lds, tid, e, xl, xh, yl, yh);
             C hl = cadd(c_hp[i0], cscale(xl, (T)0.5 * opb[i0]));
             C hh = cadd(c_hp[i1], cscale(xh, (T)0.5 * opb[i1]));
-            if (R.mid) { hl = cadd(hl, cscale(yl, (T)0.5 * HW1_OPC(i0))); hh = cadd(hh, cscale(yh, (T)0.5 * HW1_OPC(i1))); }
+            if (MID) { hl = cadd(hl, cscale(yl, (T)0.5 * HW1_OPC(i0))); hh = cadd(hh, cscale(yh, (T)0.5 * HW1_OPC(i1))); }
             lel = cscale(hl, k0 * k0 / g);
             leh = cscale(hh, k1 * k1 / g);
             u2l = c_u2h[i0]; u2h = c_u2h[i1];
@@ -889,7 +892,8 @@ __global__ void __launch_bounds__(NT) k1d_rhs(Batch1DArgs<T> a, int stage) {
 #define HW_1D_CASE(NN)                                                         \
     case NN:                                                                   \
         if constexpr (NN >= 2 * NT && NN <= 16 * NT && NN * sizeof(C) <= 131072) \
-            rhs1d_body<T, NN, NT>(a, r, stage, lds, sm, smd);                  \
+            { if (a.real[r].mid) rhs1d_body<T, NN, NT, true>(a, r, stage, lds, sm, smd);          \
+              else rhs1d_body<T, NN, NT, false>(a, r, stage, lds, sm, smd); }                   \
         break;
         HW_1D_CASE(128) HW_1D_CASE(256) HW_1D_CASE(512) HW_1D_CASE(1024) HW_1D_CASE(2048)
         HW_1D_CASE(4096) HW_1D_CASE(8192) HW_1D_CASE(16384)
