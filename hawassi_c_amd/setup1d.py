@@ -315,4 +315,5 @@ class Model1D:
     breaking: dict | None = None
     eps_abs: float = 1e-5
     eps_rel: float = 1e-4
+    order: int = 2                      # 1: rhs_HsF1, 2: rhs_HsF2, 3: rhs_HsF3
     meta: dict = field(default_factory=dict)

@@ -57,6 +57,7 @@ typedef struct {
     const double *depth, *x;
     /* third reference depth (mid[0] == 3: Up3IP, 1d/functions.c:2247-2340); NULL for two */
     const double *csq_mid, *gam_mid;
+    int order;           /* 1: rhs_HsF1, 3: rhs_HsF3, else rhs_HsF2 */
 } orc1d_params;
 
 typedef struct {
@@ -464,13 +465,59 @@ int orc1d_rhs(void *vc, double t, const double *z, double *dz)
         }
         free(H);
     }
-    /* nonlinear terms (1d/rhs.c:141-147) */
+    /* nonlinear terms (1d/rhs.c:141-147); rhs_HsF1 (1d/rhs.c:1-94) has none */
     double *duH2 = cz(N), *deH2 = cz(N);
-    for (int i = 0; i < N; i++) {
-        duH2[2 * i] = -LeM[2 * i] - (-k[i] * euh[2 * i + 1]);
-        duH2[2 * i + 1] = -LeM[2 * i + 1] - (k[i] * euh[2 * i]);
-        deH2[2 * i] = -(-k[i] * u2h[2 * i + 1]) / 2;
-        deH2[2 * i + 1] = -(k[i] * u2h[2 * i]) / 2;
+    if (p->order != 1)
+        for (int i = 0; i < N; i++) {
+            duH2[2 * i] = -LeM[2 * i] - (-k[i] * euh[2 * i + 1]);
+            duH2[2 * i + 1] = -LeM[2 * i + 1] - (k[i] * euh[2 * i]);
+            deH2[2 * i] = -(-k[i] * u2h[2 * i + 1]) / 2;
+            deH2[2 * i + 1] = -(k[i] * u2h[2 * i]) / 2;
+        }
+    if (p->order == 3) {
+        /* rhs_HsF3 (1d/rhs.c:243-399) with ifft_fft_Hsf3 (1d/functions.c:1568-1640)
+           and ifft_fft_Hsf3_v (:3440-3555): deluH3, deletaH3 added to the H2
+           terms before adjust_delH */
+        double *dxuh = cz(N), *dxu = malloc(sizeof(double) * N), *LM = malloc(sizeof(double) * N);
+        for (int i = 0; i < N; i++) { dxuh[2 * i] = -k[i] * uh[2 * i + 1]; dxuh[2 * i + 1] = k[i] * uh[2 * i]; }
+        ifft_re(c, dxuh, dxu);
+        ifft_re(c, LeM, LM);
+        double *Xh = cz(N), *A3h = cz(N), *Dh = cz(N), *tmp = cz(N);
+        for (int i = 0; i < N; i++) f[i] = eta[i] * LM[i];
+        fft_re(c, f, Xh);
+        for (int i = 0; i < N; i++) f[i] = eta[i] * eta[i] * dxu[i];
+        fft_re(c, f, tmp);
+        for (size_t q = 0; q < 2 * (size_t)N; q++) Xh[q] += 0.5 * tmp[q];
+        for (int i = 0; i < N; i++) f[i] = eta[i] * eta[i] * M0u[i];
+        fft_re(c, f, A3h);
+        for (int i = 0; i < N; i++) f[i] = M0u[i] * (eta[i] * dxu[i] + LM[i]);
+        fft_re(c, f, Dh);
+        for (int i = 0; i < N; i++) {
+            double lr, li;
+            if (p->flat) {
+                lr = p->Lop[i] * Xh[2 * i]; li = p->Lop[i] * Xh[2 * i + 1];
+            } else {
+                lr = 0; li = 0;   /* filled below from the HSS */
+            }
+            tmp[2 * i] = lr; tmp[2 * i + 1] = li;
+        }
+        if (!p->flat) {
+            double *H = cz(N);
+            hss(c, Xh, H);
+            for (int i = 0; i < N; i++) {
+                double m1r = -1.0 / g * (-k[i] * H[2 * i + 1]), m1i = -1.0 / g * (k[i] * H[2 * i]);  /* Op_M0 */
+                tmp[2 * i] = -k[i] * m1i;                  /* prod_1ik(k, .) */
+                tmp[2 * i + 1] = k[i] * m1r;
+            }
+            free(H);
+        }
+        for (int i = 0; i < N; i++) {
+            duH2[2 * i] += tmp[2 * i] - 0.5 * k[i] * k[i] * A3h[2 * i];
+            duH2[2 * i + 1] += tmp[2 * i + 1] - 0.5 * k[i] * k[i] * A3h[2 * i + 1];
+            deH2[2 * i] += k[i] * Dh[2 * i + 1];              /* -prod_1ik(k, F(deletaH3)) */
+            deH2[2 * i + 1] += -k[i] * Dh[2 * i];
+        }
+        free(dxuh); free(dxu); free(LM); free(Xh); free(A3h); free(Dh); free(tmp);
     }
     /* adjust_delH */
     double *a1 = malloc(sizeof(double) * N), *duA = cz(N), *deA = cz(N);
@@ -482,8 +529,9 @@ int orc1d_rhs(void *vc, double t, const double *z, double *dz)
     fft_re(c, a1, deA);
     double *du12 = cz(N), *de12 = cz(N);
     for (size_t q = 0; q < 2 * (size_t)N; q++) { du12[q] = duH1[q] + duA[q]; de12[q] = deH1[q] + deA[q]; }
-    /* rhs_breaking (1d/rhs.c:617-657) */
-    if (p->breaking_on) {
+    /* rhs_breaking (1d/rhs.c:617-657); rhs_HsF1 never calls it */
+    const int brk_on = p->breaking_on && p->order != 1;
+    if (brk_on) {
         double *e2 = malloc(sizeof(double) * N), *u2 = malloc(sizeof(double) * N);
         ifft_re(c, eh, e2);
         ifft_re(c, uh, u2);
@@ -515,8 +563,8 @@ int orc1d_rhs(void *vc, double t, const double *z, double *dz)
         const double *rb = c->Rb_hat + 2 * i;
         dz[i] = (du12[2 * i] + 2 * fr - Deh[2 * i]) * a;
         dz[i + N] = (du12[2 * i + 1] + 2 * fi - Deh[2 * i + 1]) * a;
-        dz[i + 2 * N] = (de12[2 * i] + (p->breaking_on ? rb[0] : 0.0) - Duh[2 * i]) * a;
-        dz[i + 3 * N] = (de12[2 * i + 1] + (p->breaking_on ? rb[1] : 0.0) - Duh[2 * i + 1]) * a;
+        dz[i + 2 * N] = (de12[2 * i] + (brk_on ? rb[0] : 0.0) - Duh[2 * i]) * a;
+        dz[i + 3 * N] = (de12[2 * i + 1] + (brk_on ? rb[1] : 0.0) - Duh[2 * i + 1]) * a;
     }
     free(eh); free(uh); free(duH1); free(deH1); free(Cv); free(M0uh); free(eta); free(u); free(M0u); free(f);
     free(euh); free(eMh); free(u2h); free(Deh); free(Duh); free(LeM); free(duH2); free(deH2); free(a1); free(duA);

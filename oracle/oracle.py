@@ -54,6 +54,7 @@ class Orc1DParams(ct.Structure):
         ("lenIF", ct.c_int), ("ILL", ct.c_int), ("IFF", ct.c_int),
         ("depth", _dp), ("x", _dp),
         ("csq_mid", _dp), ("gam_mid", _dp),
+        ("order", ct.c_int),
     ]
 
 
@@ -235,6 +236,7 @@ class Oracle1D:
 
         p = Orc1DParams()
         p.N, p.flat, p.source_on = m.n, int(m.flat), int(m.source_on)
+        p.order = int(getattr(m, "order", 2))
         p.propagation = {"Uni+": 0, "Uni-": 1, "Bi": 2, "None": 3}[m.propagation]
         p.k, p.aal, p.chi, p.damp = arr("k", m.k), arr("aal", m.aal), arr("chi", m.chi), arr("damp", m.damp)
         if m.flat:

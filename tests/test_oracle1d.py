@@ -217,3 +217,55 @@ def test_up3ip_solves_the_interpolation_system():
         assert np.allclose(o["table_gam"][:, i], np.linalg.solve(M, rhs), rtol=1e-9, atol=1e-9)
     assert np.allclose(o["table_gam"][:, 0], [1, 0, 0], atol=1e-12)
     assert np.allclose(o["table_gam"][:, 100], [0, 0, 1], atol=1e-12)
+
+
+# ------------------------------------------------------- HS1 and HS3 ---
+def test_hs1_single_mode_any_amplitude():
+    """rhs_HsF1 is linear: a single mode of any amplitude keeps the exact
+    dispersion frequency."""
+    m = _bare(W1.channel(256, t_end=2.0))
+    m.order = 1
+    n = m.n
+    j = 9
+    eta_hat = np.zeros(n, complex)
+    eta_hat[j] = eta_hat[n - j] = 0.3 * n / 2
+    y = np.concatenate([eta_hat.real, eta_hat.imag, np.zeros(n), np.zeros(n)])
+    rc, t, h, _ = O.Oracle1D(m).integrate(0.0, 1.7, 1e-3, y, 1e-14, 1e-10)
+    assert rc == 0
+    for b in (j, n - j):
+        w = math.sqrt(S.GRAV * abs(m.k[b]) * math.tanh(abs(m.k[b]) * 1.0))
+        assert abs(y[b] + 1j * y[n + b] - eta_hat[b] * math.cos(w * 1.7)) < 1e-7 * abs(eta_hat[b])
+
+
+@pytest.mark.parametrize("kind", ["channel", "slope"])
+def test_hs1_is_affine_and_hs3_increment_is_cubic(kind):
+    m = getattr(W1, kind)(512, t_end=2.0, **({"breaking": False} if kind == "slope" else {}))
+    m.source_on = False
+    y = W1.random_state(m, 4, amp=0.05, kfrac=0.3)
+    m.order = 1
+    r1 = [O.Oracle1D(m).rhs(0.3, s * y) for s in (1.0, 2.0)]
+    assert relerr(r1[1], 2.0 * r1[0]) < 1e-12          # linear, source off
+    d = []
+    for s in (1.0, 2.0):
+        m.order = 3
+        r3 = O.Oracle1D(m).rhs(0.3, s * y)
+        m.order = 2
+        r2 = O.Oracle1D(m).rhs(0.3, s * y)
+        d.append(r3 - r2)
+    assert np.abs(d[0]).max() > 1e-10
+    assert relerr(d[1], 8.0 * d[0]) < 1e-8
+
+
+def test_hs3_two_depth_with_unit_weights_equals_flat():
+    mf = _bare(W1.channel(256, t_end=2.0))
+    mf.order = 3
+    n = mf.n
+    sym = _sym_multiplier(n, 200.0, 1.0)
+    mf.csqr, mf.lop = sym, mf.k ** 2 * sym / S.GRAV
+    one, zero = np.ones(n), np.zeros(n)
+    ops = dict(gam_min=one, gam_plus=zero, csq_min=sym, csq_plus=0.5 * sym)
+    mb = S.Model1D(**{**mf.__dict__, "flat": False, "ops": ops})
+    y = W1.random_state(mf, 5, amp=0.2, kfrac=0.3)
+    assert relerr(O.Oracle1D(mb).rhs(0.3, y), O.Oracle1D(mf).rhs(0.3, y)) < 1e-12
+    mf.order = 2
+    assert relerr(O.Oracle1D(mb).rhs(0.3, y), O.Oracle1D(mf).rhs(0.3, y)) > 1e-8
