@@ -50,6 +50,7 @@ class HwConfig1D(ct.Structure):
         ("lenIF", ct.c_int), ("ILL", ct.c_int), ("IFF", ct.c_int),
         ("depth", _dp), ("eps_abs", ct.c_double), ("eps_rel", ct.c_double), ("t0", ct.c_double), ("h0", ct.c_double),
         ("csq_mid", _dp), ("gam_mid", _dp),
+        ("order", ct.c_int),
     ]
 
 

@@ -32,6 +32,7 @@ struct Real1D {                 // read-only model description of one realizatio
     double dx, KBC, TC, Tstar, MaxEtaInit, delb, eps_abs, eps_rel;
     int xbstart, xbend, xbendL, xbstartR;
     int lenIF, ILL, IFF, mid;   // mid: a third reference depth (Up3IP)
+    int order, pad2;            // order: 1 rhs_HsF1, 2 rhs_HsF2, 3 rhs_HsF3
 };
 
 struct Rk1D {                   // driver state of one realization (FP64)
@@ -770,6 +771,138 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
         }
         P::template run<-1>(lds, tw, tid);
     }
+    // ---- rhs_HsF3's third-order terms (1d/rhs.c:243-399; ifft_fft_Hsf3,
+    // 1d/functions.c:1568-1640, and ifft_fft_Hsf3_v, :3440-3555):
+    //   deluH3   = L'(F(eta LM0u + eta^2 dxu / 2)) - k^2 F(eta^2 M0u) / 2
+    //   deletaH3 = -i k F(M0u (eta dxu + LM0u))
+    // with L' the operator of LetaM0u (flat: L; two depths: k^2 HSS / g).
+    // They are added to the H2 terms before adjust_delH; on return the
+    // adjust inputs are in the exchange buffer.
+    auto hs3_terms = [&]() {
+        T* s_x = sr + R1_P2 * N;     // eta LM0u + eta^2 dxu / 2 (physical)
+        T* s_d = sr + R1_FLUX * N;   // M0u (eta dxu + LM0u)
+        C* c_x = c_euh;              // its transform (two depths)
+        P::template run<1>(lds, tw, tid);
+#pragma unroll 1
+        for (int e = 0; e < EH; ++e) {
+            const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            T l0, l1, d0, d1;
+            P::inv_out(lds, tid, e, l0, l1, d0, d1);
+            const T e0 = s_eta[i0], e1 = s_eta[i1], m0 = s_m0[i0], m1 = s_m0[i1];
+            const T x0 = e0 * l0 + (T)0.5 * e0 * e0 * d0, x1 = e1 * l1 + (T)0.5 * e1 * e1 * d1;
+            s_d[i0] = m0 * (e0 * d0 + l0); s_d[i1] = m1 * (e1 * d1 + l1);
+            s_x[i0] = x0; s_x[i1] = x1;
+            P::fwd_in(lds, tid, e, x0, x1, e0 * e0 * m0, e1 * e1 * m1);
+        }
+        P::template run<-1>(lds, tw, tid);
+        if (R.flat) {
+#pragma unroll 1
+            for (int e = 0; e < EH; ++e) {
+                const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+                const T k0 = kk[i0], k1 = kk[i1];
+                C xl, xh, al, ah;
+                P::fwd_out(lds, tid, e, xl, xh, al, ah);
+                c_hp[i0] = cadd(c_hp[i0], csub(cscale(xl, opb[i0]), cscale(al, (T)0.5 * k0 * k0)));
+                c_hp[i1] = cadd(c_hp[i1], csub(cscale(xh, opb[i1]), cscale(ah, (T)0.5 * k1 * k1)));
+                P::fwd_in(lds, tid, e, s_d[i0], s_d[i1], (T)0, (T)0);
+            }
+            P::template run<-1>(lds, tw, tid);
+#pragma unroll 1
+            for (int e = 0; e < EH; ++e) {
+                const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+                C dl, dh, z0, z1;
+                P::fwd_out(lds, tid, e, dl, dh, z0, z1);
+                const C del_ = csub(c_u2h[i0], cmul_i<1>(cscale(dl, kk[i0])));
+                const C deh = csub(c_u2h[i1], cmul_i<1>(cscale(dh, kk[i1])));
+                P::inv_in(lds, tid, e, c_hp[i0], c_hp[i1], del_, deh);
+            }
+            return;
+        }
+        // two depths: HSS of X^ as for LetaM0u (middle depth first when present)
+#pragma unroll 1
+        for (int e = 0; e < EH; ++e) {
+            const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            const T k0 = kk[i0], k1 = kk[i1];
+            C xl, xh, al, ah;
+            P::fwd_out(lds, tid, e, xl, xh, al, ah);
+            c_hp[i0] = csub(c_hp[i0], cscale(al, (T)0.5 * k0 * k0));
+            c_hp[i1] = csub(c_hp[i1], cscale(ah, (T)0.5 * k1 * k1));
+            c_x[i0] = xl; c_x[i1] = xh;
+            if (MID)
+                P::inv_in(lds, tid, e, cscale(xl, HW1_OPC(i0)), cscale(xh, HW1_OPC(i1)), cmk<C>((T)0, (T)0), cmk<C>((T)0, (T)0));
+            else
+                P::inv_in(lds, tid, e, cscale(xl, opa[i0]), cscale(xh, opa[i1]), cscale(xl, opb[i0]), cscale(xh, opb[i1]));
+        }
+        if (MID) {
+            P::template run<1>(lds, tw, tid);
+#pragma unroll 1
+            for (int e = 0; e < EH; ++e) {
+                const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+                T cl, ch, z0, z1;
+                P::inv_out(lds, tid, e, cl, ch, z0, z1);
+                HW1_AC(i0) = cl; HW1_AC(i1) = ch;
+                const C xl = c_x[i0], xh = c_x[i1];
+                P::inv_in(lds, tid, e, cscale(xl, opa[i0]), cscale(xh, opa[i1]), cscale(xl, opb[i0]), cscale(xh, opb[i1]));
+            }
+        }
+        P::template run<1>(lds, tw, tid);
+#pragma unroll 1
+        for (int e = 0; e < EH; ++e) {
+            const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            T bml, bmh, bpl, bph;
+            P::inv_out(lds, tid, e, bml, bmh, bpl, bph);
+            const T g0 = gmv[i0], g1 = gmv[i1], q0 = gpv[i0], q1 = gpv[i1];
+            T f0 = g0 * bml + q0 * bpl, f1 = g1 * bmh + q1 * bph;
+            if (MID) { f0 += HW1_GC(i0) * HW1_AC(i0); f1 += HW1_GC(i1) * HW1_AC(i1); }
+            P::fwd_in(lds, tid, e, f0, f1, g0 * s_x[i0], g1 * s_x[i1]);
+        }
+        P::template run<-1>(lds, tw, tid);
+#pragma unroll 1
+        for (int e = 0; e < EH; ++e) {
+            const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            C xl, xh, yl, yh;
+            P::fwd_out(lds, tid, e, xl, xh, yl, yh);
+            c_x[i0] = cscale(cadd(xl, cscale(yl, opa[i0])), (T)0.5);   // HSS partial
+            c_x[i1] = cscale(cadd(xh, cscale(yh, opa[i1])), (T)0.5);
+            P::fwd_in(lds, tid, e, gpv[i0] * s_x[i0], gpv[i1] * s_x[i1], s_d[i0], s_d[i1]);
+        }
+        P::template run<-1>(lds, tw, tid);
+        if (MID) {
+#pragma unroll 1
+            for (int e = 0; e < EH; ++e) {
+                const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+                C xl, xh, dl, dh;
+                P::fwd_out(lds, tid, e, xl, xh, dl, dh);
+                c_x[i0] = cadd(c_x[i0], cscale(xl, (T)0.5 * opb[i0]));
+                c_x[i1] = cadd(c_x[i1], cscale(xh, (T)0.5 * opb[i1]));
+                c_u2h[i0] = csub(c_u2h[i0], cmul_i<1>(cscale(dl, kk[i0])));
+                c_u2h[i1] = csub(c_u2h[i1], cmul_i<1>(cscale(dh, kk[i1])));
+                P::fwd_in(lds, tid, e, HW1_GC(i0) * s_x[i0], HW1_GC(i1) * s_x[i1], (T)0, (T)0);
+            }
+            P::template run<-1>(lds, tw, tid);
+        }
+#pragma unroll 1
+        for (int e = 0; e < EH; ++e) {
+            const int i0 = P::kl(tid, e), i1 = P::kh(tid, e);
+            const T k0 = kk[i0], k1 = kk[i1];
+            C xl, xh, dl, dh;
+            P::fwd_out(lds, tid, e, xl, xh, dl, dh);
+            C h0, h1, del_, deh;
+            if (MID) {
+                h0 = cadd(c_x[i0], cscale(xl, (T)0.5 * HW1_OPC(i0)));
+                h1 = cadd(c_x[i1], cscale(xh, (T)0.5 * HW1_OPC(i1)));
+                del_ = c_u2h[i0]; deh = c_u2h[i1];
+            } else {
+                h0 = cadd(c_x[i0], cscale(xl, (T)0.5 * opb[i0]));
+                h1 = cadd(c_x[i1], cscale(xh, (T)0.5 * opb[i1]));
+                del_ = csub(c_u2h[i0], cmul_i<1>(cscale(dl, k0)));
+                deh = csub(c_u2h[i1], cmul_i<1>(cscale(dh, k1)));
+            }
+            const C dul = cadd(c_hp[i0], cscale(h0, k0 * k0 / g));
+            const C duh = cadd(c_hp[i1], cscale(h1, k1 * k1 / g));
+            P::inv_in(lds, tid, e, dul, duh, del_, deh);
+        }
+    };
     // ---- delH2 terms (1d/rhs.c:141-147) and their adjustment (adjust_delH)
 #pragma unroll 1
     for (int e = 0; e < EH; ++e) {
@@ -791,12 +924,23 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
             lel = c_hp[i0]; leh = c_hp[i1];
         }
         const C eul = c_euh[i0], euh = c_euh[i1];
-        const C dul = csub(cmk<C>(-lel.x, -lel.y), cmul_i<1>(cscale(eul, k0)));
-        const C duh = csub(cmk<C>(-leh.x, -leh.y), cmul_i<1>(cscale(euh, k1)));
-        const C del_ = cscale(cmul_i<1>(cscale(u2l, k0)), (T)-0.5);
-        const C deh = cscale(cmul_i<1>(cscale(u2h, k1)), (T)-0.5);
-        P::inv_in(lds, tid, e, dul, duh, del_, deh);
+        C dul = csub(cmk<C>(-lel.x, -lel.y), cmul_i<1>(cscale(eul, k0)));
+        C duh = csub(cmk<C>(-leh.x, -leh.y), cmul_i<1>(cscale(euh, k1)));
+        C del_ = cscale(cmul_i<1>(cscale(u2l, k0)), (T)-0.5);
+        C deh = cscale(cmul_i<1>(cscale(u2h, k1)), (T)-0.5);
+        if (R.order == 1) {   // rhs_HsF1: no nonlinear terms
+            dul = duh = del_ = deh = cmk<C>((T)0, (T)0);
+        }
+        if (R.order == 3) {
+            // rhs_HsF3: the H2 terms wait in c_hp / c_u2h; LM0u^ (= lel) and
+            // dxu^ = i k u^ are inverted together
+            c_hp[i0] = dul; c_hp[i1] = duh; c_u2h[i0] = del_; c_u2h[i1] = deh;
+            P::inv_in(lds, tid, e, lel, leh, cmul_i<1>(cscale(c_uh[i0], k0)), cmul_i<1>(cscale(c_uh[i1], k1)));
+        } else {
+            P::inv_in(lds, tid, e, dul, duh, del_, deh);
+        }
     }
+    if (R.order == 3) hs3_terms();
     P::template run<1>(lds, tw, tid);
 #pragma unroll 1
     for (int e = 0; e < EH; ++e) {

@@ -120,8 +120,11 @@ template <typename T> struct Solver1DT : Solver1D {
             R.xbstart = c.xbstart; R.xbend = c.xbend; R.xbendL = c.xbendL; R.xbstartR = c.xbstartR;
             R.lenIF = c.lenIF; R.ILL = c.ILL; R.IFF = c.IFF;
             R.mid = !c.flat && c.gam_mid != nullptr;
+            R.order = c.order == 0 ? 2 : c.order;
+            if (R.order < 1 || R.order > 3) { set_error("realization %d: order must be 1, 2 or 3", r); return HW_ERR_ARG; }
+            if (R.order == 1) R.brk = 0;   // rhs_HsF1 never calls rhs_breaking
             if (R.mid && !c.csq_mid) { set_error("realization %d: gam_mid without csq_mid", r); return HW_ERR_ARG; }
-            if (c.breaking_on) {
+            if (R.brk) {
                 int hi = R.xbstart + (R.xbendL == 0 && R.xbstartR == 0 ? R.lenIF : R.ILL);
                 int hi2 = R.xbstartR + R.IFF;
                 if (R.xbstart < 0 || R.xbend >= nn || hi > nn || hi2 > nn || R.xbstartR < 0 || c.dx <= 0) {
@@ -134,7 +137,7 @@ template <typename T> struct Solver1DT : Solver1D {
             cbtot += R.cap;
             if (nn < nmin) nmin = nn;
             if (nn > maxn) maxn = nn;
-            any_brk |= c.breaking_on;
+            any_brk |= R.brk;
         }
         NT = nmin / 2 < 1024 ? nmin / 2 : 1024;
         if (maxn > 16 * NT) {

@@ -41,6 +41,7 @@ class Hawassi1D:
         cfgs = (_lib.HwConfig1D * len(models))()
         for c, m in zip(cfgs, models):
             c.n, c.flat, c.source_on = m.n, int(m.flat), int(m.source_on)
+            c.order = int(getattr(m, "order", 2))
             c.propagation = PROPAGATION[m.propagation]
             c.dx = m.dx
             c.k, c.aal, c.chi, c.damp, c.depth = arr(m.k), arr(m.aal), arr(m.chi), arr(m.damp), arr(m.depth)

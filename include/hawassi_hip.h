@@ -194,6 +194,8 @@ typedef struct hw_config_1d {
     /* third reference depth (mid[0] == 3: Up3IP, 1d/functions.c:2247-2340, and
        the middle term of HSS, :2929-2975); NULL for two reference depths */
     const double *csq_mid, *gam_mid;
+    int order;                /* dynamics: 1 rhs_HsF1, 2 or 0 rhs_HsF2, 3 rhs_HsF3 (1d/rhs.c:1-400;
+                                 1d/functions.c:68-86) */
 } hw_config_1d;
 
 /* Build a batch of nreal realizations; precision applies to all. */
