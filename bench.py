@@ -275,8 +275,8 @@ def bench_opensea(args):
 
 
 def bench_slope1d(args):
-    """1-D batch: nreal realizations of the sloping-beach HS2 case (two-depth
-    operator, breaking) with n drawn from {4096, 8192, 16384}, FP32
+    """1-D batch: nreal realizations of the sloping-beach HS2 case (three
+    reference depths, breaking) with n drawn from {4096, 8192, 16384}, FP32
     transforms / FP64 state, advanced together by the device driver through
     the reference's output times (every 0.05 s).  A step is one output
     interval of the whole batch; value = realizations x simulated seconds per
@@ -320,10 +320,11 @@ def bench_slope1d(args):
     nrhs = st1["rhs_evals"] - st0["rhs_evals"]
     log(f"timed {args.steps} output intervals: wall {wall:.3f} s, sim {sim:.3f} s, rhs {nrhs}")
     # algorithmic bytes per realization-RHS (transform field traffic model,
-    # T_alg = 20, plus coefficients and the FP64 state in/out)
+    # T_alg = 23 with three reference depths (20 with two), plus coefficients
+    # and the FP64 state in/out)
     def b_rhs(n):
         h = n // 2 + 1
-        return 20 * (4 * n + 8 * h) + 8 * 4 * n + 2 * (2 * 16 * h)
+        return 23 * (4 * n + 8 * h) + 8 * 4 * n + 2 * (2 * 16 * h)
     mean_b = float(np.mean([b_rhs(n) for n in ns]))
     out = {
         "metric": "sim-sec/wall-sec (1-D batch, realizations x simulated seconds)",
@@ -335,7 +336,8 @@ def bench_slope1d(args):
         "dtype": "fp32", "data": "synthetic",
         "config": {"workload": f"slope1d batch of {args.nreal}: n in {{4096, 8192, 16384}} "
                                f"(counts {ns.count(4096)}/{ns.count(8192)}/{ns.count(16384)}), 400 m, depth 1.0 -> 0.2 m "
-                               "two-depth operator, JONSWAP Hs 0.10 Tp 2.0, breaking KBC 0.85",
+                               "three reference depths (Up3IP, the reference's 1-D default), "
+                               "JONSWAP Hs 0.10 Tp 2.0, breaking KBC 0.85",
                    "sim_window_s": [t_start, t]},
         "rhs_evals_per_s": nrhs / wall,
         "realization_rhs_bytes_mean": mean_b,

@@ -43,7 +43,7 @@ def jonswap_record(t: np.ndarray, hs: float, tp: float, gamma: float, seed: int,
 
 def _build(x, dx, depth, flat, hs, tp, xinflux, dampl, dampr, t_end, dt_sig, seed, breaking,
            propagation="Uni+", adjcoef=1.0, kbc=0.85, tc=0.2, tchar=0.5, eps_abs=1e-5, eps_rel=1e-4,
-           name="1d"):
+           name="1d", ndepth=3):
     n = len(x)
     xstart, xend = float(x[0]), float(x[-1])
     length = xend - xstart
@@ -72,6 +72,11 @@ def _build(x, dx, depth, flat, hs, tp, xinflux, dampl, dampr, t_end, dt_sig, see
                             depthinf=depthinf, seed=seed))
     if flat:
         m.csqr, m.lop = S.flat_operators(k, depthinf)
+    elif ndepth == 3:
+        # the reference's 1-D program always takes three reference depths for
+        # a varying bottom (mid[0] = 3, 1d/main.cc:35-36; set_operator_bathy,
+        # 1d/functions.c:2186-2193)
+        m.ops = S.up3ip(depth, k, nu)
     else:
         m.ops = S.up2ip(depth, k, nu)
     if breaking:
@@ -92,10 +97,10 @@ def slope_depth(x: np.ndarray) -> np.ndarray:
 
 
 def slope(n: int = 4096, seed: int = 2, t_end: float = 120.0, dt_sig: float = 0.05, breaking: bool = True,
-          kbc: float = 0.85) -> S.Model1D:
+          kbc: float = 0.85, ndepth: int = 3) -> S.Model1D:
     x, dx = S.domain_n(0.0, 400.0, n)
     return _build(x, dx, slope_depth(x), False, 0.10, 2.0, 50.0, 20.0, 20.0, t_end, dt_sig, seed,
-                  breaking=breaking, kbc=kbc, name=f"slope{n}")
+                  breaking=breaking, kbc=kbc, name=f"slope{n}", ndepth=ndepth)
 
 
 def slope_batch(nreal: int = 256, seed: int = 7, t_end: float = 120.0, sizes=(4096, 8192, 16384),

@@ -23,10 +23,8 @@ def say(*a):
 
 
 def model(kind, n, order, mid=False):
-    m = getattr(W1, kind)(n, t_end=4.0)
+    m = getattr(W1, kind)(n, t_end=4.0, **({} if kind == "channel" else {"ndepth": 3 if mid else 2}))
     m.order = order
-    if mid:
-        m.ops = dict(S.up3ip(m.depth, m.k, 2 * math.pi / 2.0))
     return m
 
 

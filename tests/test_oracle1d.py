@@ -100,7 +100,7 @@ def test_two_depth_reduces_to_flat(which):
 
 
 def test_up2ip_closed_forms():
-    m = W1.slope(512, t_end=2.0, breaking=False)
+    m = W1.slope(512, t_end=2.0, breaking=False, ndepth=2)
     o = m.ops
     d = m.depth
     nu = 2 * math.pi / 2.0
