@@ -197,9 +197,9 @@ def up2ip(depth: np.ndarray, k: np.ndarray, nu: float, n: int = 101):
 
 def up3ip(depth: np.ndarray, k: np.ndarray, nu: float, mid: str = "default", n: int = 101):
     """Up3IP (1d/functions.c:2247-2340): three reference depths D-, Dmid (the
-    midpoint, or the value given; the reference's program passes its
-    hard-coded mid[1] = 0.5 whenever mid_depth_ref is not "default",
-    1d/main.cc:35-36), D+.  On a 101-point depth table the
+    midpoint, or the value of mid_depth_ref when it is not "default":
+    read_kinematics sets mid[1] = atof(mid_depth_ref) before define_operator,
+    1d/functions.c:3560-3562, 1d/main.cc:203-204), D+.  On a 101-point depth table the
     weights solve sum_j g_j X(D_j) = X(D) for X = sqrt(g D), omega and Cp^2
     at kappa(nu) by Cramer's rule, splined in D; Csq_* = Cp(k, D_*)^2."""
     dmin, dplus = float(depth.min()), float(depth.max())
