@@ -51,6 +51,8 @@ class HwConfig1D(ct.Structure):
         ("depth", _dp), ("eps_abs", ct.c_double), ("eps_rel", ct.c_double), ("t0", ct.c_double), ("h0", ct.c_double),
         ("csq_mid", _dp), ("gam_mid", _dp),
         ("order", ct.c_int),
+        ("wall_on", ct.c_int), ("idx_wall", ct.c_int), ("refl_coef", ct.c_double), ("t_wall0", ct.c_double),
+        ("gamw_skew_re", _dp), ("gamw_skew_im", _dp),
     ]
 
 

@@ -35,8 +35,10 @@ a ``Model1D``, and its run with the reference's outputs.
 
 ``run_case(case, out_dir)`` is ode_solver_Ham's loop (1d/functions.c:528-782):
 integrate to every sample time of the record and write the gauge file and
-the processed influx.  Not supported (they raise): walls, runup (a bathymetry
-above the still water level), coupling, friction, kinematics output.
+the processed influx.  A wall (define_wall / wall_setup) adds its skew source
+kernel and ChiAdj factor; the wall node's output is scaled by 1 + refl_coef.
+Not supported (they raise): runup (a bathymetry above the still water level),
+coupling, friction, kinematics output.
 """
 from __future__ import annotations
 
@@ -112,12 +114,16 @@ def influx_ramp(t: np.ndarray, ntramp: float, tp_peak: float) -> np.ndarray:
     return ramp * ramp[::-1]
 
 
-def area_gamx(k, x, xinflux, xstart, dx, depthinf, lam, short: bool):
-    """The Area / AreaShort kernel of x_space (1d/x_space.c:55-82)."""
+def area_gamx(k, x, xinflux, xstart, dx, depthinf, lam, short: bool, shift=None):
+    """The Area / AreaShort kernel of x_space (1d/x_space.c:55-82): the real
+    part of IFFT(Ug) rolled by int((Xinflux - Xstart)/dx) nodes (or `shift`),
+    transformed and normalised to gamX[0] = sqrt(g D_inf)/dx (scaling_influx,
+    1d/functions.c:1747-1759).  define_wall builds its kernel the same way,
+    rolled by the wall's node index (1d/functions.c:19-30)."""
     n = len(k)
     ug = np.array([S.group_velocity_i(float(kk), depthinf) for kk in k])
     sp = np.real(np.fft.ifft(ug))
-    sp = np.roll(sp, int((xinflux - xstart) / dx))
+    sp = np.roll(sp, int((xinflux - xstart) / dx) if shift is None else int(shift))
     if short:
         lb = 2 * (0.4 * lam) ** 2
         i1, i2 = S.closest(x, xinflux - 2 * lam), S.closest(x, xinflux + 2 * lam)
@@ -137,7 +143,7 @@ def _spline(xk, yk, t):
 
 def case_from_dir(case_dir: str) -> Case1D:
     p = io1d.read_input(os.path.join(case_dir, "input.txt"))
-    for name, bad in (("wall", "yes"), ("coupling", "yes"), ("friction", "yes")):
+    for name, bad in (("coupling", "yes"), ("friction", "yes")):
         if getattr(p, name) == bad:
             raise NotImplementedError(f"1-D case option {name} = {bad} is not supported")
     if p.order == 0:
@@ -270,7 +276,26 @@ def case_from_dir(case_dir: str) -> Case1D:
         tstar = 5 * math.sqrt(depthinf * GRAV) if p.initial != "zero" else p.parTchar * d["Tp_peak"]
         w = S.breaking_window(x, xinflux, p.adjcoef, lam, p.propagation)
         m.breaking = dict(KBC=p.parKBC, TC=p.parTC, Tstar=tstar, MaxEtaInit=d["MaxEtaInit"], delb=1.2, **w)
+    if p.wall == "yes":
+        m.wall = wall_setup(p, m, x, k, dx, depthinf, xinflux, lam, float(ttrace[0]))
     return Case1D(params=p, model=m, x=x, t_out=ttrace, gauges=gauges, y0=y0, insig=insig, derived=d)
+
+
+def wall_setup(p, m, x, k, dx, depthinf, xinflux, lam, t0) -> dict:
+    """define_wall (1d/functions.c:1-47) and wall_setup (:49-66): the wall
+    node, the skew source kernel i omega(k) gamW (negated when the wall lies
+    on the influx's left, wallInfdir = 1), and ChiAdj multiplied by
+    1 - cf(x, Xwall - adjcoef lambda_p) + cf(x, Xwall) (m.chi in place)."""
+    if lam == 0:
+        raise NotImplementedError("a wall without an influx record (lambda_p = 0) is not supported")
+    idx = S.closest(x, p.Xwall)
+    gw = area_gamx(k, x, xinflux, x[0], dx, depthinf, lam, short=False, shift=idx)
+    gws = 1j * S.omega_v(k, depthinf) * gw
+    if not p.Xwall > xinflux:
+        gws = -gws
+    dl = p.adjcoef * lam
+    m.chi = m.chi * (1 - S.cf(x, p.Xwall - dl, dl) + S.cf(x, p.Xwall, dl))
+    return dict(idx=idx, refl=p.refl_coef, t0=t0, gamw_skew=gws)
 
 
 def run_case(case: Case1D, out_dir: str, precision=None, progress=None) -> dict:
@@ -296,6 +321,10 @@ def run_case(case: Case1D, out_dir: str, precision=None, progress=None) -> dict:
                 hw.integrate(float(t))
             y = hw.get_state()
             eta = np.real(np.fft.ifft(y[:n] + 1j * y[n:2 * n]))
+            if case.model.wall is not None:
+                # the wall node's output (1d/functions.c:598-601)
+                w = case.model.wall
+                eta[w["idx"]] *= 1 + w["refl"]
             if gw is not None:
                 vals[i] = gw.write(float(t), eta)
             if progress is not None and i % 500 == 0:

@@ -341,6 +341,10 @@ template <int L, int NT, int E> __device__ __forceinline__ void fr_slot(int tid,
 #ifndef HW_Y_LDS_BUDGET
 #define HW_Y_LDS_BUDGET 65536
 #endif
+// waves per SIMD the FP32 y kernels are compiled for
+#ifndef HW_Y_WPE32
+#define HW_Y_WPE32 3
+#endif
 
 // x-pass: RR rows per block, so that the transposed spectral runs
 // X[kx][y0 .. y0+RR) are HW_X_RUN_BYTES long (at least one wave per block).
@@ -397,7 +401,7 @@ template <typename T, int NY> struct YGeo {
     static constexpr bool TWL = (LDS_ELEMS + TWN) * (int)sizeof(C) <= HW_Y_LDS_BUDGET;
     // waves per SIMD the y kernels are compiled for: FP32 blocks fit three
     // to a CU by LDS, so registers should not be the tighter limit
-    static constexpr int WPE = sizeof(T) == 4 ? 3 : 1;
+    static constexpr int WPE = sizeof(T) == 4 ? HW_Y_WPE32 : 1;
     // element e of thread tid -> column * NY + ky within the block
     __device__ static __forceinline__ int eidx(int tid, int e) {
         int b, n;

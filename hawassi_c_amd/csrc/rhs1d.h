@@ -32,7 +32,9 @@ struct Real1D {                 // read-only model description of one realizatio
     double dx, KBC, TC, Tstar, MaxEtaInit, delb, eps_abs, eps_rel;
     int xbstart, xbend, xbendL, xbstartR;
     int lenIF, ILL, IFF, mid;   // mid: a third reference depth (Up3IP)
-    int order, pad2;            // order: 1 rhs_HsF1, 2 rhs_HsF2, 3 rhs_HsF3
+    int order, wall;            // order: 1 rhs_HsF1, 2 rhs_HsF2, 3 rhs_HsF3; wall: wall source on
+    int idx_wall, pad3;
+    double refl;                // reflection coefficient of the wall
 };
 
 struct Rk1D {                   // driver state of one realization (FP64)
@@ -66,6 +68,8 @@ template <typename T> struct Batch1DArgs {
     const T *k, *aal, *chi, *damp, *opa, *opb, *gm, *gp, *depth;
     const T *opc, *gc;          // third reference depth: Csq_mid, gam_mid (zero when unused)
     const C *gamx, *gamxs;
+    const C* gamws;             // wall skew kernel gamW_skew (zero without a wall)
+    double* wst;                // wall source state per realization: t_prev, S, previous refl*eta, calls
     // influx signals (natural spline: values and half second derivatives)
     const double *tsig, *sig, *sigc, *skw, *skwc;
     // state (FP64) and stage derivatives, offset 2*off: [eta^ (n)][u^ (n)]
@@ -966,6 +970,20 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
         }
         __syncthreads();
     }
+    // ---- wall source (fun_wallinfluxsource, 1d/rhs.c:1216-1239): the
+    // trapezoid integral of refl * eta(x_wall) over the times of successive
+    // calls, advanced only by a call at a later time than the last one
+    if (R.wall && tid == 0) {
+        double* W = a.wst + 4 * r;
+        if (t > W[0]) {
+            const double se = R.refl * (double)s_eta[R.idx_wall];
+            W[1] = W[3] < 1.0 ? 0.0 : W[1] + (se + W[2]) * (t - W[0]) / 2;
+            W[3] += 1.0;
+            W[0] = t;
+            W[2] = se;
+        }
+        smd[32] = 2.0 * W[1];
+    }
     const T idx1 = (T)(1.0 / R.dx), idx2 = (T)(0.5 / R.dx);
     auto gradf = [&](int i) -> T {
         if (i == 0) return (flux[1] - flux[0]) * idx1;
@@ -988,6 +1006,7 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
     P::template run<-1>(lds, tw, tid);
     {
         const T pv = (T)smd[30], pvs = (T)smd[31];
+        const T sw = R.wall ? (T)smd[32] : (T)0;
         C* Ko = (stage == 0 || stage == 7) ? a.K[6] : a.K[stage - 1];
         Ko += 2 * off;
 #pragma unroll 1
@@ -1007,6 +1026,7 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
                 const T al = a.aal[off + i];
                 const C oe = q == 0 ? oel : oeh;
                 const C ou = q == 0 ? oul : ouh;
+                if (R.wall) fo = cadd(fo, cscale(a.gamws[off + i], sw));
                 C ke = cscale(cadd(cadd(c_m0h[i], fo), oe), al);
                 C ku = cscale(cadd(cmul_i<-1>(cscale(c_eh[i], g * kk[i])), ou), al);
                 Ko[i] = ke;

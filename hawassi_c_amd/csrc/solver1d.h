@@ -66,7 +66,8 @@ template <typename T> struct Solver1DT : Solver1D {
     DevBuf<int> d_list, d_count;
     DevBuf<T> k, aal, chi, damp, opa, opb, gm, gp, depth, sr, B;
     DevBuf<T> opc, gc;   // third reference depth (zero where unused)
-    DevBuf<C> gamx, gamxs, Kb[7], sc;
+    DevBuf<C> gamx, gamxs, gamws, Kb[7], sc;
+    DevBuf<double> wst;
     DevBuf<double> tsig, sig, sigc, skw, skwc;
     DevBuf<double2> y, ext;
     DevBuf<int> peaks, ide;
@@ -124,6 +125,16 @@ template <typename T> struct Solver1DT : Solver1D {
             if (R.order < 1 || R.order > 3) { set_error("realization %d: order must be 1, 2 or 3", r); return HW_ERR_ARG; }
             if (R.order == 1) R.brk = 0;   // rhs_HsF1 never calls rhs_breaking
             if (R.mid && !c.csq_mid) { set_error("realization %d: gam_mid without csq_mid", r); return HW_ERR_ARG; }
+            // rhs_HsF3 neither updates nor adds the wall source (1d/rhs.c:243-399)
+            R.wall = c.wall_on && R.order != 3;
+            if (R.wall) {
+                if (!c.gamw_skew_re || !c.gamw_skew_im || c.idx_wall < 0 || c.idx_wall >= nn) {
+                    set_error("realization %d: wall node outside the grid or kernel missing", r);
+                    return HW_ERR_ARG;
+                }
+                R.idx_wall = c.idx_wall;
+                R.refl = c.refl_coef;
+            }
             if (R.brk) {
                 int hi = R.xbstart + (R.xbendL == 0 && R.xbstartR == 0 ? R.lenIF : R.ILL);
                 int hi2 = R.xbstartR + R.IFF;
@@ -137,7 +148,9 @@ template <typename T> struct Solver1DT : Solver1D {
             cbtot += R.cap;
             if (nn < nmin) nmin = nn;
             if (nn > maxn) maxn = nn;
-            any_brk |= R.brk;
+            // the stepper's dydt_out evaluation has side effects only through
+            // breaking and the wall source
+            any_brk |= R.brk | R.wall;
         }
         NT = nmin / 2 < 1024 ? nmin / 2 : 1024;
         if (maxn > 16 * NT) {
@@ -148,7 +161,7 @@ template <typename T> struct Solver1DT : Solver1D {
         // ---- per-element arrays
         std::vector<T> hk(ntot), haal(ntot), hchi(ntot), hdamp(ntot), hopa(ntot), hopb(ntot), hgm(ntot), hgp(ntot),
             hdep(ntot), hopc(ntot, (T)0), hgc(ntot, (T)0);
-        std::vector<C> hgx(ntot), hgs(ntot);
+        std::vector<C> hgx(ntot), hgs(ntot), hgw(ntot);
         std::vector<double> hts(sigtot), hsig(sigtot), hsigc(sigtot), hskw(sigtot), hskwc(sigtot);
         for (int r = 0; r < n; ++r) {
             const hw_config_1d& c = cfg[r];
@@ -169,6 +182,8 @@ template <typename T> struct Solver1DT : Solver1D {
                 } else {
                     hgx[q].x = hgx[q].y = hgs[q].x = hgs[q].y = (T)0;
                 }
+                hgw[q].x = R.wall ? (T)c.gamw_skew_re[i] : (T)0;
+                hgw[q].y = R.wall ? (T)c.gamw_skew_im[i] : (T)0;
             }
             if (R.nt) {
                 std::memcpy(&hts[R.sig_off], c.t_sig, sizeof(double) * R.nt);
@@ -182,7 +197,8 @@ template <typename T> struct Solver1DT : Solver1D {
         if ((rc = k.upload(hk, stream)) || (rc = aal.upload(haal, stream)) || (rc = chi.upload(hchi, stream)) ||
             (rc = damp.upload(hdamp, stream)) || (rc = opa.upload(hopa, stream)) || (rc = opb.upload(hopb, stream)) ||
             (rc = gm.upload(hgm, stream)) || (rc = gp.upload(hgp, stream)) || (rc = depth.upload(hdep, stream)) ||
-            (rc = gamx.upload(hgx, stream)) || (rc = gamxs.upload(hgs, stream)) || (rc = opc.upload(hopc, stream)) ||
+            (rc = gamx.upload(hgx, stream)) || (rc = gamxs.upload(hgs, stream)) || (rc = gamws.upload(hgw, stream)) ||
+            (rc = opc.upload(hopc, stream)) ||
             (rc = gc.upload(hgc, stream)))
             return rc;
         if (sigtot) {
@@ -214,11 +230,15 @@ template <typename T> struct Solver1DT : Solver1D {
         for (int r = 0; r < n; ++r) hl[r] = r;
         std::vector<Brk1D> hb(n);
         std::memset(hb.data(), 0, sizeof(Brk1D) * n);
+        // wall source state: t_prev = the first record time (define_wall,
+        // 1d/functions.c:9), S = 0, previous refl*eta = 0, calls = 0
+        std::vector<double> hw(4 * (size_t)n, 0.0);
+        for (int r = 0; r < n; ++r) hw[4 * r] = cfg[r].t_wall0;
         if ((rc = d_real.upload(hreal, stream)) || (rc = d_rk.upload(hrk, stream)) || (rc = d_list.upload(hl, stream)) ||
             (rc = d_count.alloc(1)) || (rc = y.alloc(2 * ntot)) || (rc = ext.alloc(2 * (size_t)maxn)) ||
             (rc = sr.alloc(R1_NR * ntot)) || (rc = sc.alloc(C1_NC * ntot)) || (rc = B.alloc(ntot)) ||
             (rc = peaks.alloc(ntot)) || (rc = ide.alloc(cbtot)) || (rc = cb.alloc(cbtot)) || (rc = cbn.alloc(cbtot)) ||
-            (rc = cbp.alloc(cbtot)) || (rc = bs.upload(hb, stream)))
+            (rc = cbp.alloc(cbtot)) || (rc = bs.upload(hb, stream)) || (rc = wst.upload(hw, stream)))
             return rc;
         for (int j = 0; j < 7; ++j)
             if ((rc = Kb[j].alloc(2 * ntot))) return rc;
@@ -235,6 +255,7 @@ template <typename T> struct Solver1DT : Solver1D {
         a.real = d_real.p; a.rk = d_rk.p; a.list = d_list.p;
         a.k = k.p; a.aal = aal.p; a.chi = chi.p; a.damp = damp.p; a.opa = opa.p; a.opb = opb.p;
         a.gm = gm.p; a.gp = gp.p; a.depth = depth.p; a.gamx = gamx.p; a.gamxs = gamxs.p;
+        a.gamws = gamws.p; a.wst = wst.p;
         a.opc = opc.p; a.gc = gc.p;
         a.tsig = tsig.p; a.sig = sig.p; a.sigc = sigc.p; a.skw = skw.p; a.skwc = skwc.p;
         a.y = y.p;

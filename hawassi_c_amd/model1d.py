@@ -64,6 +64,10 @@ class Hawassi1D:
                 c.KBC, c.TC, c.Tstar, c.max_eta_init, c.delb = br["KBC"], br["TC"], br["Tstar"], br["MaxEtaInit"], br["delb"]
                 for f in ("xbstart", "xbend", "xbendL", "xbstartR", "lenIF", "ILL", "IFF"):
                     setattr(c, f, int(br[f]))
+            w = getattr(m, "wall", None)
+            if w is not None:
+                c.wall_on, c.idx_wall, c.refl_coef, c.t_wall0 = 1, int(w["idx"]), float(w["refl"]), float(w["t0"])
+                c.gamw_skew_re, c.gamw_skew_im = arr(w["gamw_skew"].real), arr(w["gamw_skew"].imag)
             c.eps_abs, c.eps_rel = m.eps_abs, m.eps_rel
             c.t0, c.h0 = t0, h0
         h = ct.c_void_p()

@@ -146,7 +146,8 @@ def cf(x: np.ndarray, xi: float, dl: float) -> np.ndarray:
     """cf (1d/x_space.c:1-16): 0 left of xi, cosine ramp to 1 over dl."""
     out = np.zeros_like(x)
     with np.errstate(invalid="ignore", divide="ignore"):
-        r = np.maximum((x - (xi + dl)) / np.abs(x - (xi + dl)), (1 - np.cos((x - xi) * math.pi / dl)) / 2)
+        # fmax, as C: a NaN argument (0/0 at x = xi + dl) yields the other
+        r = np.fmax((x - (xi + dl)) / np.abs(x - (xi + dl)), (1 - np.cos((x - xi) * math.pi / dl)) / 2)
     out[x > xi] = r[x > xi]
     out[x == xi] = 0.5 * r[x == xi]
     return out
@@ -318,4 +319,5 @@ class Model1D:
     eps_abs: float = 1e-5
     eps_rel: float = 1e-4
     order: int = 2                      # 1: rhs_HsF1, 2: rhs_HsF2, 3: rhs_HsF3
+    wall: dict | None = None            # idx, refl, t0, gamw_skew (define_wall)
     meta: dict = field(default_factory=dict)

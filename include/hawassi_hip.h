@@ -196,6 +196,17 @@ typedef struct hw_config_1d {
     const double *csq_mid, *gam_mid;
     int order;                /* dynamics: 1 rhs_HsF1, 2 or 0 rhs_HsF2, 3 rhs_HsF3 (1d/rhs.c:1-400;
                                  1d/functions.c:68-86) */
+    /* reflecting wall (define_wall 1d/functions.c:1-47, fun_wallinfluxsource
+       1d/rhs.c:1216-1239; rhs_HsF1 and rhs_HsF2 add it, rhs_HsF3 does not):
+       wall_on 0 for none.  Every right-hand side evaluation at a time t past
+       the previous one accumulates S += (refl_coef eta(x_wall, t) + the
+       previous value) (t - t_prev) / 2 (S = 0 on the first such call; t_prev
+       starts at t_wall0, the first record time) and adds 2 S gamW_skew to
+       the eta equation.  gamw_skew: gamW_skew [n] with the reference's sign
+       for the wall's side of the influx. */
+    int wall_on, idx_wall;
+    double refl_coef, t_wall0;
+    const double *gamw_skew_re, *gamw_skew_im;
 } hw_config_1d;
 
 /* Build a batch of nreal realizations; precision applies to all. */

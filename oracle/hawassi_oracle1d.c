@@ -58,6 +58,12 @@ typedef struct {
     /* third reference depth (mid[0] == 3: Up3IP, 1d/functions.c:2247-2340); NULL for two */
     const double *csq_mid, *gam_mid;
     int order;           /* 1: rhs_HsF1, 3: rhs_HsF3, else rhs_HsF2 */
+    /* wall (define_wall 1d/functions.c:1-47, fun_wallinfluxsource
+       1d/rhs.c:1216-1239): idx_wall, refl_coef, the first record time
+       (tprevW's initial value) and gamWall_skew_hat, interleaved complex [N] */
+    int wall_on, idx_wall;
+    double refl_coef, t_wall0;
+    const double *gamw_skew;
 } orc1d_params;
 
 typedef struct {
@@ -78,6 +84,10 @@ typedef struct {
     int *ID_Node_end;
     double *B;
     double *Rb_hat;    /* interleaved complex [N], kept between calls */
+    /* wall source state (1d/header.h:128-130): ITERdtW, tprevW, Swt_skew,
+       Swall_tS_eta, Swall_tS_eta_prev */
+    int iterW;
+    double tprevW, swt_skew, sw_eta, sw_eta_prev;
     long nrhs;
     /* rk work */
     size_t dim;
@@ -370,6 +380,7 @@ void *orc1d_create(const orc1d_params *prm)
     c->ID_Node_end = (int *)calloc(c->ncap, sizeof(int));
     c->B = (double *)calloc(N, sizeof(double));
     c->Rb_hat = cz(N);
+    c->tprevW = prm->t_wall0;
     c->dim = 4 * (size_t)N;
     return c;
 }
@@ -544,6 +555,19 @@ int orc1d_rhs(void *vc, double t, const double *z, double *dz)
         fft_re(c, f, c->Rb_hat);
         free(e2); free(u2); free(dte); free(Fl); free(dF);
     }
+    /* fun_wallinfluxsource (1d/rhs.c:1216-1239), called by rhs_HsF1 and
+       rhs_HsF2 (1d/rhs.c:52-55, 194-197) with eta = ifft_complex_real_nx */
+    const int wall = p->wall_on && p->order != 3;
+    if (wall) {
+        if (t > c->tprevW) {
+            c->sw_eta = p->refl_coef * eta[p->idx_wall];
+            if (c->iterW < 1) c->swt_skew = 0;
+            else c->swt_skew = c->swt_skew + (c->sw_eta + c->sw_eta_prev) * (t - c->tprevW) / 2;
+            c->iterW = c->iterW + 1;
+            c->tprevW = t;
+            c->sw_eta_prev = c->sw_eta;
+        }
+    }
     /* define_forcing (1d/rhs.c:659-693) */
     double pv = 0, pvs = 0;
     if (p->source_on && p->nt > 0) {
@@ -561,8 +585,10 @@ int orc1d_rhs(void *vc, double t, const double *z, double *dz)
         }
         double a = p->aal[i];
         const double *rb = c->Rb_hat + 2 * i;
-        dz[i] = (du12[2 * i] + 2 * fr - Deh[2 * i]) * a;
-        dz[i + N] = (du12[2 * i + 1] + 2 * fi - Deh[2 * i + 1]) * a;
+        double swr = 0, swi = 0;     /* Swall_hat = 2 Swt_skew gamWall_skew_hat */
+        if (wall) { swr = 2 * c->swt_skew * p->gamw_skew[2 * i]; swi = 2 * c->swt_skew * p->gamw_skew[2 * i + 1]; }
+        dz[i] = (du12[2 * i] + swr + 2 * fr - Deh[2 * i]) * a;
+        dz[i + N] = (du12[2 * i + 1] + swi + 2 * fi - Deh[2 * i + 1]) * a;
         dz[i + 2 * N] = (de12[2 * i] + (brk_on ? rb[0] : 0.0) - Duh[2 * i]) * a;
         dz[i + 3 * N] = (de12[2 * i + 1] + (brk_on ? rb[1] : 0.0) - Duh[2 * i + 1]) * a;
     }

@@ -55,6 +55,8 @@ class Orc1DParams(ct.Structure):
         ("depth", _dp), ("x", _dp),
         ("csq_mid", _dp), ("gam_mid", _dp),
         ("order", ct.c_int),
+        ("wall_on", ct.c_int), ("idx_wall", ct.c_int), ("refl_coef", ct.c_double), ("t_wall0", ct.c_double),
+        ("gamw_skew", _dp),
     ]
 
 
@@ -221,7 +223,8 @@ def _cplx(a):
 
 
 class Oracle1D:
-    """FP64 CPU restatement of rhs_HsF2 (flat or two-depth) + breaking + rkf45."""
+    """FP64 CPU restatement of rhs_HsF1 / rhs_HsF2 / rhs_HsF3 (flat, two or three
+    reference depths) + breaking + the wall source + rkf45."""
 
     def __init__(self, m):
         L = lib()
@@ -259,6 +262,10 @@ class Oracle1D:
             p.KBC, p.TC, p.Tstar, p.MaxEtaInit, p.delb = br["KBC"], br["TC"], br["Tstar"], br["MaxEtaInit"], br["delb"]
             for f in ("xbstart", "xbend", "xbendL", "xbstartR", "lenIF", "ILL", "IFF"):
                 setattr(p, f, int(br[f]))
+        w = getattr(m, "wall", None)
+        if w is not None:
+            p.wall_on, p.idx_wall, p.refl_coef, p.t_wall0 = 1, int(w["idx"]), float(w["refl"]), float(w["t0"])
+            p.gamw_skew = arr("gw", _cplx(w["gamw_skew"]))
         self._keep, self._p = keep, p
         self.ctx = L.orc1d_create(ct.byref(p))
         self.model = m

@@ -150,10 +150,13 @@ def test_ramp_edges():
 
 
 def test_unsupported_options_raise(tmp_path):
-    for case in ("1dF004Wall", "1dF005WallFreq"):
+    # friction (1dRBr001Harm) and runup (1dR001Harm: a bathymetry above the
+    # still water level) are refused
+    for case in ("1dRBr001Harm", "1dR001Harm"):
         d = tmp_path / case
         d.mkdir()
         shutil.copyfile(GOLD / "input1d" / f"{case}.txt", d / "input.txt")
+        (d / "bath.dat").write_text("".join("%g\n" % v for v in np.linspace(5.0, -5.0, 5000)))
         with pytest.raises(NotImplementedError):
             case1d.case_from_dir(str(d))
 
@@ -189,3 +192,85 @@ def test_oracle_reproduces_reference_record(tmp_path, name, nsamples):
     got = _oracle_gauge_rows(d, nsamples)
     assert max(abs(float(v)) for r in ref[1:nsamples + 1] for v in r.split()[1:]) > 0.01
     assert got == [r for r in ref[1:nsamples + 1]]
+
+
+def _wall_case(tmp_path, name="1dF005WallFreq", refl=None):
+    """A wall case from the reference's input file with a synthetic record
+    (the reference ships none for these cases): a seeded ramped sum of
+    sines sampled every 0.1 s."""
+    d = tmp_path / name
+    d.mkdir(exist_ok=True)
+    txt = (GOLD / "input1d" / f"{name}.txt").read_text()
+    if refl is not None:
+        txt = txt.replace("walltype 	= 2 2500 1", f"walltype 	= 2 2500 {refl}")
+    (d / "input.txt").write_text(txt)
+    t = np.arange(0, 400.0, 0.1)
+    rs = np.random.default_rng(5)
+    e = sum(0.02 * rs.uniform(0.5, 1) * np.sin(2 * math.pi * t / tp + rs.uniform(0, 6.28)) for tp in (10, 12, 15))
+    (d / "influx.dat").write_text("".join("%.6f %.8f\n" % (a, b) for a, b in zip(t, e)))
+    (d / "gauges.dat").write_text("100\n")
+    return case1d.case_from_dir(str(d))
+
+
+def test_wall_kernel_and_chi():
+    """define_wall's kernel is the Area kernel rolled to the wall node,
+    times i omega(k), negated for a wall left of the influx; wall_setup
+    multiplies ChiAdj by 1 - cf(Xwall - d) + cf(Xwall)."""
+    n, L, D = 256, 40.0, 0.5
+    x, dx = S.domain_n(-10.0, -10.0 + L, n)
+    k = S.freqspace1d(L, n)
+    for xwall, sign in ((20.0, 1.0), (-5.0, -1.0)):
+        p = io1d.Input1D(Xwall=xwall, refl_coef=0.7, adjcoef=0.5)
+        m = S.Model1D(n=n, x=x, dx=dx, k=k, aal=np.ones(n), chi=np.ones(n), damp=np.zeros(n), depth=np.full(n, D),
+                      flat=True)
+        w = case1d.wall_setup(p, m, x, k, dx, D, 1.7, 2.0, 0.0)
+        idx = S.closest(x, xwall)
+        ug = np.array([S.group_velocity_i(float(v), D) for v in k])
+        mm = np.arange(n)
+        g = np.exp(-2j * np.pi * mm * idx / n) * (ug + np.conj(ug[(-mm) % n])) / 2
+        g *= math.sqrt(S.GRAV * D) / dx / g[0].real
+        want = sign * 1j * S.omega_v(k, D) * g
+        assert w["idx"] == idx and w["refl"] == 0.7
+        assert np.allclose(w["gamw_skew"], want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+        ramp = 1 - S.cf(x, xwall - 1.0, 1.0) + S.cf(x, xwall, 1.0)
+        assert np.array_equal(m.chi, ramp)
+        assert m.chi[x < xwall - 1.0].min() == 1.0 and m.chi[x > xwall + 1.0].min() == 1.0
+        assert m.chi[idx] < 1.0
+
+
+def test_oracle_wall_source_history(tmp_path):
+    """fun_wallinfluxsource in the oracle over a scripted call sequence: the
+    eta equation gains aal * 2 S gamW_skew with S the trapezoid integral of
+    refl * eta(x_wall) over the times of the calls that advanced time (the
+    first such call sets S = 0; calls at an earlier or equal time reuse S);
+    the u equation is unchanged; refl = 0 gives no source."""
+    from oracle import oracle as O
+    c = _wall_case(tmp_path)
+    m = c.model
+    w = m.wall
+    no = S.Model1D(**{**m.__dict__, "wall": None})
+    ow, on = O.Oracle1D(m), O.Oracle1D(no)
+    n = m.n
+    rs = np.random.default_rng(11)
+    calls = [0.5, 0.9, 0.9, 0.7, 1.4, 2.0]
+    S_, tprev, prev, it = 0.0, float(c.t_out[0]), 0.0, 0
+    for t in calls:
+        eta = 0.01 * rs.standard_normal(n)
+        eh = np.fft.fft(eta)
+        y = np.concatenate([eh.real, eh.imag, 0.1 * eh.real, 0.1 * eh.imag])
+        if t > tprev:
+            se = w["refl"] * np.real(np.fft.ifft(eh))[w["idx"]]
+            S_ = 0.0 if it < 1 else S_ + (se + prev) * (t - tprev) / 2
+            it, tprev, prev = it + 1, t, se
+        d = ow.rhs(t, y) - on.rhs(t, y)
+        want = m.aal * 2 * S_ * w["gamw_skew"]
+        scale = np.abs(ow.rhs(t, y)[:2 * n]).max()
+        assert np.abs(d[:n] - want.real).max() <= 1e-12 * scale
+        assert np.abs(d[n:2 * n] - want.imag).max() <= 1e-12 * scale
+        assert np.abs(d[2 * n:]).max() == 0.0
+    assert S_ != 0.0
+    c0 = _wall_case(tmp_path / "r0", refl=0) if (tmp_path / "r0").mkdir() is None else None
+    o0 = O.Oracle1D(c0.model)
+    on0 = O.Oracle1D(S.Model1D(**{**c0.model.__dict__, "wall": None}))
+    for t in calls:
+        assert np.array_equal(o0.rhs(t, y), on0.rhs(t, y))

@@ -22,7 +22,7 @@ import pytest
 from conftest import relerr
 from test_case1d import unpack_case
 
-from hawassi_c_amd import HW_PREC_FP64, case1d, io1d
+from hawassi_c_amd import HW_PREC_FP32, HW_PREC_FP64, case1d, io1d
 from hawassi_c_amd.model1d import Hawassi1D
 from oracle import oracle as O
 
@@ -99,3 +99,68 @@ def test_case_against_reference_record(tmp_path, name):
         a = np.abs(v[:, j] - ref_v[:, j]).max()
         say(f"  gauge x={g:g}: rel RMS {r:.3e} max abs {a:.3e} m, record max {np.abs(ref_v[:, j]).max():.4f} m")
         assert r < rel_b and a < abs_b
+
+
+@pytest.mark.parametrize("prec", [HW_PREC_FP64, HW_PREC_FP32])
+def test_wall_source_call_sequence(tmp_path, prec):
+    """The wall source's history on the device follows the oracle's over a
+    scripted sequence of stand-alone RHS calls (times forward, repeated and
+    backward; 1dF005WallFreq's input with a synthetic record)."""
+    from test_case1d import _wall_case
+    say("wall call sequence", prec)
+    c = _wall_case(tmp_path)
+    m = c.model
+    d = Hawassi1D([m], prec, t0=float(c.t_out[0]), h0=1e-4)
+    o = O.Oracle1D(m)
+    rs = np.random.default_rng(3)
+    n = m.n
+    tol = 1e-12 if prec == HW_PREC_FP64 else 2e-5
+    for t in (0.5, 0.9, 0.9, 0.7, 1.4, 2.0, 3.5):
+        eta = 0.01 * rs.standard_normal(n)
+        eh = np.fft.fft(eta)
+        y = np.concatenate([eh.real, eh.imag, 0.1 * eh.real, 0.1 * eh.imag])
+        e = relerr(d.rhs(t, y), o.rhs(t, y))
+        say("  t", t, "relL2", e)
+        assert e < tol
+
+
+@pytest.mark.parametrize("name,t1", [("1dF005WallFreq", 260.0), ("1dF004Wall", 110.0)])
+def test_wall_case_trajectory(tmp_path, name, t1):
+    """Both reference wall cases (1dF005WallFreq, HS2, a synthetic record;
+    1dF004Wall, HS1, its harmonic record) integrated by the device and the
+    oracle, FP64, until after the waves reach the wall: states within 1e-9
+    and identical RHS counts (the stepper's dydt_out evaluation advances the
+    wall history, so it is not skipped)."""
+    from test_case1d import _wall_case, GOLD
+    say("wall trajectory", name, t1)
+    if name == "1dF005WallFreq":
+        c = _wall_case(tmp_path)
+    else:
+        dd = tmp_path / name
+        dd.mkdir()
+        (dd / "input.txt").write_bytes((GOLD / "input1d" / f"{name}.txt").read_bytes())
+        (dd / "gauges.dat").write_text("150\n")
+        c = case1d.case_from_dir(str(dd))
+    m = c.model
+    assert m.wall is not None
+    d = Hawassi1D([m], HW_PREC_FP64, t0=float(c.t_out[0]), h0=1e-4)
+    o = O.Oracle1D(m)
+    y = c.y0.copy()
+    t, h = float(c.t_out[0]), 1e-4
+    for ti in c.t_out[1:]:
+        if ti > t1 + 1e-12:
+            break
+        d.integrate(float(ti))
+        rc, t, h, _ = o.integrate(t, float(ti), h, y)
+        assert rc == 0
+    got = d.get_state(0)
+    e = relerr(got, y)
+    st = d.stats(0)
+    n = m.n
+    eta = np.real(np.fft.ifft(y[:n] + 1j * y[n:2 * n]))
+    near = np.abs(c.x - c.x[m.wall["idx"]]) < c.derived["lambda_p"]
+    say("  relL2", e, "device rhs", st["rhs_evals"], "oracle rhs", o.rhs_count(), "max eta", np.abs(eta).max(),
+        "max eta within a wave length of the wall", np.abs(eta[near]).max())
+    # the waves have reached the wall, so its source is active
+    assert np.abs(eta[near]).max() > 1e-3
+    assert e < 1e-9 and st["rhs_evals"] == o.rhs_count()
