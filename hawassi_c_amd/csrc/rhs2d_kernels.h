@@ -321,7 +321,7 @@ __global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x1(Rhs2DArgs<T> a) {
 enum : int { S_M0U = 0, S_M0V = 1 };
 
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__((YGeo<T, NY>::NT)) k2b_y2(Rhs2DArgs<T> a) {
+__global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves_per_eu(YGeo<T, NY>::WPE))) k2b_y2(Rhs2DArgs<T> a) {
     HW_YK_PROLOGUE
     C A0[G::PE], B0[G::PE], C0[G::PE], A1[G::PE], B1[G::PE], C1[G::PE], dsum[G::PE];
     T cm[G::PE], cp[G::PE], kk0[G::PE], kk1[G::PE];

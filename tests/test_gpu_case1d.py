@@ -101,15 +101,18 @@ def test_case_against_reference_record(tmp_path, name):
         assert r < rel_b and a < abs_b
 
 
+@pytest.mark.parametrize("order", [1, 2])
 @pytest.mark.parametrize("prec", [HW_PREC_FP64, HW_PREC_FP32])
-def test_wall_source_call_sequence(tmp_path, prec):
+def test_wall_source_call_sequence(tmp_path, prec, order):
     """The wall source's history on the device follows the oracle's over a
     scripted sequence of stand-alone RHS calls (times forward, repeated and
-    backward; 1dF005WallFreq's input with a synthetic record)."""
+    backward; 1dF005WallFreq's input with a synthetic record), for rhs_HsF2
+    and rhs_HsF1."""
     from test_case1d import _wall_case
-    say("wall call sequence", prec)
+    say("wall call sequence", prec, "order", order)
     c = _wall_case(tmp_path)
     m = c.model
+    m.order = order
     d = Hawassi1D([m], prec, t0=float(c.t_out[0]), h0=1e-4)
     o = O.Oracle1D(m)
     rs = np.random.default_rng(3)
@@ -124,8 +127,14 @@ def test_wall_source_call_sequence(tmp_path, prec):
         assert e < tol
 
 
-@pytest.mark.parametrize("name,t1", [("1dF005WallFreq", 260.0), ("1dF004Wall", 110.0)])
-def test_wall_case_trajectory(tmp_path, name, t1):
+# 1dF004Wall (HS1) is integrated only until shortly after its waves reach the
+# wall: from about 80 s on the wall history makes its step sequence sensitive
+# to rounding (the oracle against itself with one state value perturbed by
+# 1e-13 takes 1589 = 1589 RHS evaluations to 10 s, 9995 = 9995 to 70 s, then
+# 11947 against 11856 to 80 s, and differs by 7e-3 at 110 s), so identical
+# step counts are not a property of the algorithm past that point.
+@pytest.mark.parametrize("name,t1,near_min", [("1dF005WallFreq", 260.0, 1e-3), ("1dF004Wall", 74.0, 3e-4)])
+def test_wall_case_trajectory(tmp_path, name, t1, near_min):
     """Both reference wall cases (1dF005WallFreq, HS2, a synthetic record;
     1dF004Wall, HS1, its harmonic record) integrated by the device and the
     oracle, FP64, until after the waves reach the wall: states within 1e-9
@@ -162,5 +171,5 @@ def test_wall_case_trajectory(tmp_path, name, t1):
     say("  relL2", e, "device rhs", st["rhs_evals"], "oracle rhs", o.rhs_count(), "max eta", np.abs(eta).max(),
         "max eta within a wave length of the wall", np.abs(eta[near]).max())
     # the waves have reached the wall, so its source is active
-    assert np.abs(eta[near]).max() > 1e-3
+    assert np.abs(eta[near]).max() > near_min
     assert e < 1e-9 and st["rhs_evals"] == o.rhs_count()
