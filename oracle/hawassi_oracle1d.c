@@ -64,6 +64,9 @@ typedef struct {
     int wall_on, idx_wall;
     double refl_coef, t_wall0;
     const double *gamw_skew;
+    /* breaking.dat records (1d/breaking.c:426-444): the record interval dt
+       (ttrace[1] - ttrace[0]); 0: none */
+    double rec_dt;
 } orc1d_params;
 
 typedef struct {
@@ -88,6 +91,11 @@ typedef struct {
        Swall_tS_eta, Swall_tS_eta_prev */
     int iterW;
     double tprevW, swt_skew, sw_eta, sw_eta_prev;
+    /* breaking.dat: ITERbdt and the records (time, node count, nodes) */
+    int iterbdt;
+    double *rec_t;
+    int *rec_n, *rec_nodes;
+    long nrec, caprec, nnodes, capnodes;
     long nrhs;
     /* rk work */
     size_t dim;
@@ -304,6 +312,7 @@ static void breaking_process(orc1d_ctx *c, double t, const double *eta, const do
     if (c->flag == 1 || c->flag == 2) {
         int NCBnow = New[0].Bcount;
         int *IDT = (int *)calloc(NCBnow > 0 ? NCBnow : 1, sizeof(int));
+        int *bnodes = (int *)malloc(sizeof(int) * (N + 1)), nb = 0;   /* Break_nodes, it_indexb */
         int iterNprev = 0;
         reset_cb(Prev, cap);
         for (int lx = 0; lx < NCBnow; lx++) {
@@ -335,6 +344,7 @@ static void breaking_process(orc1d_ctx *c, double t, const double *eta, const do
                 else if (au <= Us) B[ly] = 0;
                 else B[ly] = au / Us - 1;
                 if (au >= Us) fl = 1;
+                if (nb < N) bnodes[nb++] = ly;
                 if (Dir == 0) break;
             }
             if (!fl) {
@@ -355,8 +365,27 @@ static void breaking_process(orc1d_ctx *c, double t, const double *eta, const do
             if (IDT[i] == 1) { c->flag = 2; break; }
             c->flag = 0;
         }
+        /* data_saving_break once per record interval (1d/breaking.c:426-436) */
+        if (p->rec_dt > 0 && t > c->iterbdt * p->rec_dt && nb > 0) {
+            if (c->nrec == c->caprec) {
+                c->caprec = c->caprec ? 2 * c->caprec : 256;
+                c->rec_t = (double *)realloc(c->rec_t, sizeof(double) * c->caprec);
+                c->rec_n = (int *)realloc(c->rec_n, sizeof(int) * c->caprec);
+            }
+            if (c->nnodes + nb > c->capnodes) {
+                while (c->nnodes + nb > c->capnodes) c->capnodes = c->capnodes ? 2 * c->capnodes : 4096;
+                c->rec_nodes = (int *)realloc(c->rec_nodes, sizeof(int) * c->capnodes);
+            }
+            c->rec_t[c->nrec] = t;
+            c->rec_n[c->nrec] = nb;
+            memcpy(c->rec_nodes + c->nnodes, bnodes, sizeof(int) * nb);
+            c->nrec++;
+            c->nnodes += nb;
+        }
+        free(bnodes);
         free(IDT);
     }
+    if (p->rec_dt > 0 && t > c->iterbdt * p->rec_dt) c->iterbdt++;   /* 1d/breaking.c:442-444 */
     reset_cb(New, cap);
     reset_cb(CB, cap);
 }
@@ -392,6 +421,7 @@ void orc1d_destroy(void *vc)
     plan_free(c->plan);
     free(c->sig_c); free(c->skew_c);
     free(c->CB); free(c->CBNew); free(c->CBPrev); free(c->ID_Node_end); free(c->B); free(c->Rb_hat);
+    free(c->rec_t); free(c->rec_n); free(c->rec_nodes);
     for (int i = 0; i < 6; i++) free(c->k[i]);
     free(c->ytmp); free(c->y0); free(c->yerr); free(c->dydt_in); free(c->dydt_out);
     free(c);
@@ -704,6 +734,23 @@ int orc1d_integrate(void *vc, double *t, double t1, double *h, double *y, double
 }
 
 long orc1d_rhs_count(void *vc) { return ((orc1d_ctx *)vc)->nrhs; }
+
+/* breaking.dat records so far: returns their number; with non-NULL arrays
+   copies up to max_rec times and node counts and up to max_nodes nodes */
+long orc1d_break_records(void *vc, double *t, int *n, int *nodes, long max_rec, long max_nodes)
+{
+    orc1d_ctx *c = (orc1d_ctx *)vc;
+    if (t && n) {
+        long m = c->nrec < max_rec ? c->nrec : max_rec;
+        memcpy(t, c->rec_t, sizeof(double) * m);
+        memcpy(n, c->rec_n, sizeof(int) * m);
+    }
+    if (nodes) {
+        long m = c->nnodes < max_nodes ? c->nnodes : max_nodes;
+        memcpy(nodes, c->rec_nodes, sizeof(int) * m);
+    }
+    return c->nrec;
+}
 int orc1d_flag(void *vc) { return ((orc1d_ctx *)vc)->flag; }
 int orc1d_ncrest(void *vc) { return ((orc1d_ctx *)vc)->CBPrev[0].Bcount; }
 void orc1d_get_B(void *vc, double *B) { orc1d_ctx *c = (orc1d_ctx *)vc; memcpy(B, c->B, sizeof(double) * c->p.N); }

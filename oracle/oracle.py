@@ -57,6 +57,7 @@ class Orc1DParams(ct.Structure):
         ("order", ct.c_int),
         ("wall_on", ct.c_int), ("idx_wall", ct.c_int), ("refl_coef", ct.c_double), ("t_wall0", ct.c_double),
         ("gamw_skew", _dp),
+        ("rec_dt", ct.c_double),
     ]
 
 
@@ -105,6 +106,8 @@ def lib():
         L.orc1d_integrate.argtypes = L.orc_evolve_apply.argtypes
         L.orc1d_rhs_count.argtypes = [ct.c_void_p]
         L.orc1d_rhs_count.restype = ct.c_long
+        L.orc1d_break_records.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_long, ct.c_long]
+        L.orc1d_break_records.restype = ct.c_long
         L.orc1d_flag.argtypes = [ct.c_void_p]
         L.orc1d_ncrest.argtypes = [ct.c_void_p]
         L.orc1d_get_B.argtypes = [ct.c_void_p, _dp]
@@ -224,9 +227,10 @@ def _cplx(a):
 
 class Oracle1D:
     """FP64 CPU restatement of rhs_HsF1 / rhs_HsF2 / rhs_HsF3 (flat, two or three
-    reference depths) + breaking + the wall source + rkf45."""
+    reference depths) + breaking + the wall source + rkf45.  rec_dt > 0 keeps
+    the breaking.dat records (one per interval rec_dt, 1d/breaking.c:426-444)."""
 
-    def __init__(self, m):
+    def __init__(self, m, rec_dt: float = 0.0):
         L = lib()
         keep = {}
 
@@ -266,6 +270,7 @@ class Oracle1D:
         if w is not None:
             p.wall_on, p.idx_wall, p.refl_coef, p.t_wall0 = 1, int(w["idx"]), float(w["refl"]), float(w["t0"])
             p.gamw_skew = arr("gw", _cplx(w["gamw_skew"]))
+        p.rec_dt = float(rec_dt)
         self._keep, self._p = keep, p
         self.ctx = L.orc1d_create(ct.byref(p))
         self.model = m
@@ -303,6 +308,22 @@ class Oracle1D:
 
     def rhs_count(self):
         return lib().orc1d_rhs_count(self.ctx)
+
+    def break_records(self):
+        """[(t, nodes)] as data_saving_break writes them."""
+        L = lib()
+        n = L.orc1d_break_records(self.ctx, None, None, None, 0, 0)
+        t = np.zeros(max(n, 1))
+        cnt = np.zeros(max(n, 1), dtype=np.int32)
+        L.orc1d_break_records(self.ctx, _p(t), cnt.ctypes.data, None, n, 0)
+        tot = int(cnt[:n].sum())
+        nodes = np.zeros(max(tot, 1), dtype=np.int32)
+        L.orc1d_break_records(self.ctx, None, None, nodes.ctypes.data, 0, tot)
+        out, o = [], 0
+        for i in range(n):
+            out.append((float(t[i]), nodes[o:o + cnt[i]].copy()))
+            o += cnt[i]
+        return out
 
     def flag(self):
         return lib().orc1d_flag(self.ctx)

@@ -6,8 +6,9 @@ use into this folder (run in a checkout that has the reference at
 * 1dB002HarmBar/: the case inputs (input.txt, bath.dat, gauges.dat,
   influx.dat) and the outputs the reference's authors ran it to
   (output/influx.dat, output/Hawassi_eta_gauge.txt), the large ones gzipped;
-* 1dFBr005Bor/: the case inputs (input.txt, gauges.dat, initial.dat) and
-  Output/Hawassi_eta_gauge.txt.
+* 1dFBr005Bor/: the case inputs (input.txt, gauges.dat, initial.dat),
+  Output/Hawassi_eta_gauge.txt and Output/breaking.dat;
+* 1dBBr002IrBar/: the case inputs and output/breaking.dat.
 """
 import glob
 import gzip
@@ -44,6 +45,12 @@ def main():
     put(f"{REF}/1dFBr005Bor/Input/initial.dat", os.path.join(c, "initial.dat"), gz=True)
     put(f"{REF}/1dFBr005Bor/Output/Hawassi_eta_gauge.txt", os.path.join(c, "expected", "Hawassi_eta_gauge.txt"),
         gz=True)
+    put(f"{REF}/1dFBr005Bor/Output/breaking.dat", os.path.join(c, "expected", "breaking.dat"), gz=True)
+    c = os.path.join(HERE, "1dBBr002IrBar")
+    for name in ("input.txt", "bath.dat", "gauges.dat"):
+        put(f"{REF}/1dBBr002IrBar/{name}", os.path.join(c, name))
+    put(f"{REF}/1dBBr002IrBar/influx.dat", os.path.join(c, "influx.dat"), gz=True)
+    put(f"{REF}/1dBBr002IrBar/output/breaking.dat", os.path.join(c, "expected", "breaking.dat"), gz=True)
 
 
 if __name__ == "__main__":

@@ -274,3 +274,34 @@ def test_oracle_wall_source_history(tmp_path):
     on0 = O.Oracle1D(S.Model1D(**{**c0.model.__dict__, "wall": None}))
     for t in calls:
         assert np.array_equal(o0.rhs(t, y), on0.rhs(t, y))
+
+
+def _oracle_break_rows(case_dir: Path, t_end: float):
+    """breaking.dat rows of the FP64 oracle over [t0, t_end], printed as
+    data_saving_break prints them ("%lf %d " and "%d " per node)."""
+    from oracle import oracle as O
+    c = case1d.case_from_dir(str(case_dir))
+    o = O.Oracle1D(c.model, rec_dt=float(c.t_out[1] - c.t_out[0]))
+    y = c.y0.copy()
+    t, h = float(c.t_out[0]), 1e-4
+    for ti in c.t_out[1:]:
+        if ti > t_end + 1e-12:
+            break
+        rc, t, h, _ = o.integrate(t, float(ti), h, y)
+        assert rc == 0
+    return ["%lf %d " % (tt, len(nd)) + "".join("%d " % v for v in nd) for tt, nd in o.break_records()]
+
+
+@pytest.mark.parametrize("name,t_end", [("1dFBr005Bor", 13.0), ("1dBBr002IrBar", 16.0)])
+def test_oracle_reproduces_reference_breaking_records(tmp_path, name, t_end):
+    """The breaking.dat the reference's program wrote for two breaking cases
+    (the bore from initial.dat over a flat bottom; irregular waves breaking on
+    a bar, AreaShort generation, three reference depths), row for row: the
+    time of the RHS call that logs (an RKF45 stage time, six decimals) and
+    every node the breaking loop marks.  It pins the breaking criterion, the
+    crest tracking and the stepper's stage times together."""
+    d = unpack_case(name, tmp_path)
+    ref = [r for r in (d / "expected" / "breaking.dat").read_text().splitlines() if float(r.split()[0]) <= t_end]
+    got = _oracle_break_rows(d, t_end)
+    assert len(ref) >= 20
+    assert got == ref

@@ -33,7 +33,10 @@ def say(*a):
     print(*a, flush=True)
 
 
-@pytest.mark.parametrize("name,t1", [("1dB002HarmBar", 6.0), ("1dFBr005Bor", 1.0)])
+# 1dBBr002IrBar to 16 s and 1dFBr005Bor to 13 s cover the first breaking
+# events, whose breaking.dat rows the oracle reproduces character for
+# character (tests/test_case1d.py)
+@pytest.mark.parametrize("name,t1", [("1dB002HarmBar", 6.0), ("1dFBr005Bor", 13.0), ("1dBBr002IrBar", 16.0)])
 def test_case_start_against_oracle(tmp_path, name, t1):
     say("case1d oracle window", name, t1)
     c = case1d.case_from_dir(str(unpack_case(name, tmp_path)))
@@ -58,6 +61,10 @@ def test_case_start_against_oracle(tmp_path, name, t1):
     # (side-effect free, its value unused): one RHS fewer per attempt
     attempts = st["steps_accepted"] + st["steps_rejected"]
     assert e < 1e-9 and st["rhs_evals"] == o.rhs_count() - (attempts if m.breaking is None else 0)
+    if m.breaking is not None:
+        Bd, fd = d.breaking_state(0)
+        say("  breaking nodes", int((Bd > 0).sum()), "flag", fd, o.flag())
+        assert fd == o.flag() and np.abs(Bd - o.B()).max() <= 1e-9
 
 
 # name: (samples that must agree with the record to one printed unit, 1e-6 m;
