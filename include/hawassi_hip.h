@@ -221,6 +221,10 @@ int hw1d_rhs(double t, const double *y, double *dydt, void *ctx);
 /* Same for realization r. */
 int hw1d_rhs_at(void *ctx, int r, double t, const double *y, double *dydt);
 
+/* State and time of realization r.  Like the reference's globals, the
+   breaking and wall histories (crest lists, B, the wall integral and its
+   last call time) carry on across these calls; a fresh history needs a
+   fresh context. */
 int hw1d_set_state(void *ctx, int r, const double *y);
 int hw1d_get_state(void *ctx, int r, double *y);
 int hw1d_set_time(void *ctx, int r, double t, double h);
