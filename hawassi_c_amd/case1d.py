@@ -298,10 +298,11 @@ def wall_setup(p, m, x, k, dx, depthinf, xinflux, lam, t0) -> dict:
     return dict(idx=idx, refl=p.refl_coef, t0=t0, gamw_skew=gws)
 
 
-def run_case(case: Case1D, out_dir: str, precision=None, progress=None) -> dict:
-    """ode_solver_Ham's loop (1d/functions.c:537-552, 700-744): from
+def run_case(case: Case1D, out_dir: str, precision=None, progress=None, fields: bool = True) -> dict:
+    """ode_solver_Ham's loop (1d/functions.c:537-552, 604-666, 700-744): from
     t = ttrace[0] with h = 1e-4, integrate to each sample time and write
-    Hawassi_eta_gauge.txt; influx.dat gets the processed record
+    Hawassi_eta_gauge.txt and (fields=True) the partitioned Constants_XX.dat
+    and Hawassi_eta/u/phi_XX.dat; influx.dat gets the processed record
     (read_influx_file).  Returns the gauge series and the solver counters."""
     from . import _lib
     from .model1d import Hawassi1D
@@ -313,7 +314,13 @@ def run_case(case: Case1D, out_dir: str, precision=None, progress=None) -> dict:
     hw.set_state(case.y0)
     gw = io1d.GaugeWriter1D(os.path.join(out_dir, "Hawassi_eta_gauge.txt"), case.gauges, case.x) \
         if len(case.gauges) else None
-    n = case.model.n
+    m = case.model
+    fw = io1d.FieldWriter1D(out_dir, case.x, case.t_out, m.depth, m.k, case.derived["nu_peak"],
+                            case.params.npartition) if fields else None
+    n = m.n
+    # phi^ = (-i / k) u^, 0 where k = 0 (1d/functions.c:576-590)
+    with np.errstate(divide="ignore"):
+        a = np.where(m.k == 0, 0.0, -1.0 / np.where(m.k == 0, 1.0, m.k))
     vals = np.zeros((len(case.t_out), len(case.gauges)))
     try:
         for i, t in enumerate(case.t_out):
@@ -321,10 +328,14 @@ def run_case(case: Case1D, out_dir: str, precision=None, progress=None) -> dict:
                 hw.integrate(float(t))
             y = hw.get_state()
             eta = np.real(np.fft.ifft(y[:n] + 1j * y[n:2 * n]))
-            if case.model.wall is not None:
+            if m.wall is not None:
                 # the wall node's output (1d/functions.c:598-601)
-                w = case.model.wall
-                eta[w["idx"]] *= 1 + w["refl"]
+                eta[m.wall["idx"]] *= 1 + m.wall["refl"]
+            if fw is not None:
+                uh = y[2 * n:3 * n] + 1j * y[3 * n:]
+                u = np.real(np.fft.ifft(uh))
+                phi = np.real(np.fft.ifft(-(a * uh.imag) + 1j * (a * uh.real)))
+                fw.write(eta, u, phi)
             if gw is not None:
                 vals[i] = gw.write(float(t), eta)
             if progress is not None and i % 500 == 0:
@@ -333,5 +344,7 @@ def run_case(case: Case1D, out_dir: str, precision=None, progress=None) -> dict:
     finally:
         if gw is not None:
             gw.close()
+        if fw is not None:
+            fw.close()
         hw.close()
     return dict(t=case.t_out, gauges=vals, stats=stats)

@@ -15,6 +15,9 @@ run unmodified around this library.
 * ``read_initial``: initial.dat, Nx rows of eta and u (1d/read_input.c:62-75).
 * ``write_influx``: the processed influx ``influx.dat`` in the output folder
   (read_influx_file, 1d/read_input.c:81-89), "%g %g" per sample.
+* ``write_constant`` / ``FieldWriter1D``: ``Constants_XX.dat`` and the
+  full-domain ``Hawassi_eta/u/phi_XX.dat`` frames, per output partition
+  (print_xt, 1d/functions.c:604-666, 784-802).
 * ``GaugeWriter1D``: ``Hawassi_eta_gauge.txt`` (ode_solver_Ham's gauge block
   and print_data_gauge, 1d/functions.c:500-526, 700-744): a header row
   "0 " plus each gauge position, then per output time t and the natural
@@ -253,6 +256,64 @@ class GaugeWriter1D:
     def close(self):
         if self.f:
             self.f.close()
+            self.f = None
+
+
+def write_constant(path, x: np.ndarray, t_part: np.ndarray, depth: np.ndarray, k: np.ndarray, nu_peak: float):
+    """Constants_XX.dat (print_xt, 1d/functions.c:784-802): doubles Nx, x[Nx],
+    Nt, t[Nt] (the partition's output times), depth[Nx], k[Nx], nu_peak."""
+    with open(path, "wb") as f:
+        for a in ([float(len(x))], x, [float(len(t_part))], t_part, depth, k, [nu_peak]):
+            np.asarray(a, dtype="<f8").tofile(f)
+
+
+class FieldWriter1D:
+    """The full-domain outputs of ode_solver_Ham (1d/functions.c:604-666):
+    the record's n output times are cut into npartition parts of
+    NtoutIP = n / npartition times and a remainder part of Ntrest
+    (1d/functions.c:3585-3598); each part has Constants_XX.dat and the frames
+    Hawassi_eta_XX.dat, Hawassi_u_XX.dat, Hawassi_phi_XX.dat (Nx doubles per
+    output time, no header), XX = 01, 02, ..."""
+
+    def __init__(self, outdir, x, t_out, depth, k, nu_peak: float, npartition: int):
+        import os
+        self.dir = outdir
+        self.x, self.t, self.depth, self.k, self.nu = x, np.asarray(t_out), depth, k, nu_peak
+        n = len(self.t)
+        npart = max(int(npartition), 1)
+        self.ntout = n // npart
+        self.ntrest = n - npart * self.ntout
+        self.npart = npart
+        self.ip = 0
+        self.i = 0
+        self.f = None
+        os.makedirs(outdir, exist_ok=True)
+
+    def _open(self):
+        import os
+        self.close()
+        self.ip += 1
+        tag = "%02d.dat" % self.ip
+        n1 = self.ntout if self.ip <= self.npart else self.ntrest
+        # print_xt writes trest whenever N1 == Ntrest (1d/functions.c:792-797)
+        if n1 == self.ntrest:
+            tp = self.t[self.npart * self.ntout:]
+        else:
+            tp = self.t[(self.ip - 1) * self.ntout:self.ip * self.ntout]
+        write_constant(os.path.join(self.dir, "Constants_" + tag), self.x, tp, self.depth, self.k, self.nu)
+        self.f = [open(os.path.join(self.dir, f"Hawassi_{v}_" + tag), "wb") for v in ("eta", "u", "phi")]
+
+    def write(self, eta, u, phi):
+        if self.ntout > 0 and self.i % self.ntout == 0:
+            self._open()
+        for f, a in zip(self.f, (eta, u, phi)):
+            np.asarray(a, dtype="<f8").tofile(f)
+        self.i += 1
+
+    def close(self):
+        if self.f:
+            for f in self.f:
+                f.close()
             self.f = None
 
 

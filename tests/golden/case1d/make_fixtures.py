@@ -5,9 +5,10 @@ use into this folder (run in a checkout that has the reference at
 * ../input1d/<case>.txt: every 1-D case's input.txt (the parser fixtures);
 * 1dB002HarmBar/: the case inputs (input.txt, bath.dat, gauges.dat,
   influx.dat) and the outputs the reference's authors ran it to
-  (output/influx.dat, output/Hawassi_eta_gauge.txt), the large ones gzipped;
+  (output/influx.dat, output/Hawassi_eta_gauge.txt, output/Constants_01.dat), the
+  large ones gzipped;
 * 1dFBr005Bor/: the case inputs (input.txt, gauges.dat, initial.dat),
-  Output/Hawassi_eta_gauge.txt and Output/breaking.dat;
+  Output/Hawassi_eta_gauge.txt, Output/breaking.dat and Output/Constants_01.dat;
 * 1dBBr002IrBar/: the case inputs and output/breaking.dat.
 """
 import glob
@@ -39,6 +40,7 @@ def main():
     put(f"{REF}/1dB002HarmBar/output/influx.dat", os.path.join(c, "expected", "influx.dat"), gz=True)
     put(f"{REF}/1dB002HarmBar/output/Hawassi_eta_gauge.txt", os.path.join(c, "expected", "Hawassi_eta_gauge.txt"),
         gz=True)
+    put(f"{REF}/1dB002HarmBar/output/Constants_01.dat", os.path.join(c, "expected", "Constants_01.dat"), gz=True)
     c = os.path.join(HERE, "1dFBr005Bor")
     for name in ("input.txt", "gauges.dat"):
         put(f"{REF}/1dFBr005Bor/Input/{name}", os.path.join(c, name))
@@ -46,6 +48,7 @@ def main():
     put(f"{REF}/1dFBr005Bor/Output/Hawassi_eta_gauge.txt", os.path.join(c, "expected", "Hawassi_eta_gauge.txt"),
         gz=True)
     put(f"{REF}/1dFBr005Bor/Output/breaking.dat", os.path.join(c, "expected", "breaking.dat"), gz=True)
+    put(f"{REF}/1dFBr005Bor/Output/Constants_01.dat", os.path.join(c, "expected", "Constants_01.dat"), gz=True)
     c = os.path.join(HERE, "1dBBr002IrBar")
     for name in ("input.txt", "bath.dat", "gauges.dat"):
         put(f"{REF}/1dBBr002IrBar/{name}", os.path.join(c, name))

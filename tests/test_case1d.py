@@ -305,3 +305,40 @@ def test_oracle_reproduces_reference_breaking_records(tmp_path, name, t_end):
     got = _oracle_break_rows(d, t_end)
     assert len(ref) >= 20
     assert got == ref
+
+
+@pytest.mark.parametrize("name", ["1dB002HarmBar", "1dFBr005Bor"])
+def test_constants_file_is_the_references(tmp_path, name):
+    """Constants_01.dat (print_xt: Nx, x, Nt, the output times, depth, k,
+    nu_peak as raw doubles) byte for byte as the reference's program wrote it
+    for these cases: the grid, the record times, bath.dat, freqspace's k and
+    the peak frequency (from the influx record's spectrum; for the bore, from
+    the initial elevation's)."""
+    d = unpack_case(name, tmp_path)
+    c = case1d.case_from_dir(str(d))
+    m = c.model
+    out = tmp_path / "out"
+    fw = io1d.FieldWriter1D(str(out), c.x, c.t_out, m.depth, m.k, c.derived["nu_peak"], c.params.npartition)
+    n = m.n
+    fw.write(np.zeros(n), np.zeros(n), np.zeros(n))
+    fw.close()
+    assert (out / "Constants_01.dat").read_bytes() == (d / "expected" / "Constants_01.dat").read_bytes()
+    assert (out / "Hawassi_eta_01.dat").stat().st_size == 8 * n
+
+
+def test_field_writer_partitions(tmp_path):
+    """n = 11 output times in npartition = 3 parts: 3 + 3 + 3 and a remainder
+    part of 2 (Constants_04 holds the remainder times)."""
+    x = np.linspace(0, 1, 8)
+    t = np.arange(11) * 0.5
+    fw = io1d.FieldWriter1D(str(tmp_path), x, t, np.ones(8), np.arange(8.0), 2.5, 3)
+    for i in range(11):
+        fw.write(np.full(8, i), np.full(8, -i), np.full(8, 2 * i))
+    fw.close()
+    for ip, tp in ((1, t[0:3]), (2, t[3:6]), (3, t[6:9]), (4, t[9:])):
+        c = np.fromfile(tmp_path / f"Constants_{ip:02d}.dat", "<f8")
+        assert c[0] == 8 and np.array_equal(c[1:9], x) and c[9] == len(tp)
+        assert np.array_equal(c[10:10 + len(tp)], tp) and c[-1] == 2.5
+        e = np.fromfile(tmp_path / f"Hawassi_eta_{ip:02d}.dat", "<f8").reshape(-1, 8)
+        assert np.array_equal(e[:, 0], tp / 0.5)
+    assert not (tmp_path / "Constants_05.dat").exists()

@@ -89,7 +89,7 @@ def test_case_against_reference_record(tmp_path, name):
     c = case1d.case_from_dir(str(d))
     out = tmp_path / "out"
     t0 = time.time()
-    res = case1d.run_case(c, str(out), progress=lambda i, t: say("   sample", i, "t", round(t, 3)))
+    res = case1d.run_case(c, str(out), progress=lambda i, t: say("   sample", i, "t", round(t, 3)), fields=False)
     wall = time.time() - t0
     ref_head, ref_t, ref_v = io1d.read_gauge_file(d / "expected" / "Hawassi_eta_gauge.txt")
     head, t, v = io1d.read_gauge_file(out / "Hawassi_eta_gauge.txt")
@@ -184,3 +184,30 @@ def test_wall_case_trajectory(tmp_path, name, t1, near_min):
     # the waves have reached the wall, so its source is active
     assert np.abs(eta[near]).max() > near_min
     assert e < 1e-9 and st["rhs_evals"] == o.rhs_count()
+
+
+def test_case_field_outputs(tmp_path):
+    """run_case's full-domain outputs over the first 300 samples of
+    1dB002HarmBar: one partition, Constants_01.dat with those times, and
+    eta / u / phi frames whose eta reproduces the gauge file through the
+    gauge spline."""
+    say("case1d field outputs")
+    d = unpack_case("1dB002HarmBar", tmp_path)
+    c = case1d.case_from_dir(str(d))
+    c.t_out = c.t_out[:300]
+    out = tmp_path / "out"
+    case1d.run_case(c, str(out))
+    n = c.model.n
+    cst = np.fromfile(out / "Constants_01.dat", "<f8")
+    assert cst[0] == n and cst[n + 1] == 300 and np.array_equal(cst[n + 2:n + 302], c.t_out)
+    eta = np.fromfile(out / "Hawassi_eta_01.dat", "<f8").reshape(300, n)
+    u = np.fromfile(out / "Hawassi_u_01.dat", "<f8").reshape(300, n)
+    phi = np.fromfile(out / "Hawassi_phi_01.dat", "<f8").reshape(300, n)
+    head, t, v = io1d.read_gauge_file(out / "Hawassi_eta_gauge.txt")
+    w = io1d.gauge_weights(c.x, head)
+    assert np.abs(eta @ w.T - v).max() <= 5e-7 + 1e-12
+    # phi is Re F^-1((-i/k) u^) of the full complex state spectrum, whose
+    # upper half is not exactly the mirror of the lower, so it is not a
+    # function of the u frame (Re F^-1 u^) alone; it is checked for size
+    assert np.isfinite(phi).all() and np.isfinite(u).all() and np.abs(phi[-1]).max() > 0
+    assert np.abs(eta[-1]).max() > 1e-3
