@@ -76,13 +76,23 @@ def test_read_input_fields():
     assert r.mid_temp == "6.666666666666667" and r.cutfracwn == 8
 
 
-def test_b002_processed_influx_is_the_references(tmp_path):
-    d = unpack_case("1dB002HarmBar", tmp_path)
+@pytest.mark.parametrize("name", ["1dB002HarmBar", "1dB001IrSlope"])
+def test_processed_influx_is_the_references(tmp_path, name):
+    """The processed influx.dat (ramped record, ramp length from the detected
+    peak period) byte for byte as the reference's program wrote it: a
+    harmonic record (5000 samples, ramp 8 periods) and an irregular one
+    (91969 samples, ramp 4 periods)."""
+    d = unpack_case(name, tmp_path)
     c = case1d.case_from_dir(str(d))
-    assert c.model.n == 2048                           # logfile: Number of Nodes
+    assert c.model.n == 2048                           # logfiles: Number of Nodes
     out = tmp_path / "influx.dat"
     io1d.write_influx(out, c.t_out, c.insig)
     assert out.read_bytes() == (d / "expected" / "influx.dat").read_bytes()
+
+
+def test_b002_derived_values(tmp_path):
+    d = unpack_case("1dB002HarmBar", tmp_path)
+    c = case1d.case_from_dir(str(d))
     # logfile: peak frequency 3.10 rad/s, kp 1.68, wave length 3.75 m,
     # steepness kp Hs/2 0.02
     dv = c.derived
@@ -307,7 +317,7 @@ def test_oracle_reproduces_reference_breaking_records(tmp_path, name, t_end):
     assert got == ref
 
 
-@pytest.mark.parametrize("name", ["1dB002HarmBar", "1dFBr005Bor"])
+@pytest.mark.parametrize("name", ["1dB002HarmBar", "1dFBr005Bor", "1dB001IrSlope"])
 def test_constants_file_is_the_references(tmp_path, name):
     """Constants_01.dat (print_xt: Nx, x, Nt, the output times, depth, k,
     nu_peak as raw doubles) byte for byte as the reference's program wrote it
