@@ -9,7 +9,7 @@ use into this folder (run in a checkout that has the reference at
   large ones gzipped;
 * 1dFBr005Bor/: the case inputs (input.txt, gauges.dat, initial.dat),
   Output/Hawassi_eta_gauge.txt, Output/breaking.dat and Output/Constants_01.dat;
-* 1dBBr002IrBar/: the case inputs and output/breaking.dat;
+* 1dBBr002IrBar/: the case inputs, output/breaking.dat and Constants_01.dat;
 * 1dB001IrSlope/: the case inputs, output/Output/influx.dat and Constants_01.dat.
 """
 import glob
@@ -55,6 +55,7 @@ def main():
         put(f"{REF}/1dBBr002IrBar/{name}", os.path.join(c, name))
     put(f"{REF}/1dBBr002IrBar/influx.dat", os.path.join(c, "influx.dat"), gz=True)
     put(f"{REF}/1dBBr002IrBar/output/breaking.dat", os.path.join(c, "expected", "breaking.dat"), gz=True)
+    put(f"{REF}/1dBBr002IrBar/output/Constants_01.dat", os.path.join(c, "expected", "Constants_01.dat"), gz=True)
     c = os.path.join(HERE, "1dB001IrSlope")
     for name in ("input.txt", "bath.dat", "gauges.dat"):
         put(f"{REF}/1dB001IrSlope/Input/{name}", os.path.join(c, name))

@@ -317,7 +317,7 @@ def test_oracle_reproduces_reference_breaking_records(tmp_path, name, t_end):
     assert got == ref
 
 
-@pytest.mark.parametrize("name", ["1dB002HarmBar", "1dFBr005Bor", "1dB001IrSlope"])
+@pytest.mark.parametrize("name", ["1dB002HarmBar", "1dFBr005Bor", "1dB001IrSlope", "1dBBr002IrBar"])
 def test_constants_file_is_the_references(tmp_path, name):
     """Constants_01.dat (print_xt: Nx, x, Nt, the output times, depth, k,
     nu_peak as raw doubles) byte for byte as the reference's program wrote it
