@@ -1,5 +1,6 @@
 /*
- * hawassi_oracle1d.c -- FP64 CPU restatement of the reference's 1-D HS2 path.
+ * hawassi_oracle1d.c -- FP64 CPU restatement of the reference's 1-D path
+ * (HS1 / HS2 / HS3, flat or two / three reference depths, breaking, wall).
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, the smoke check and bench.py's CPU
  * baseline load it, as the checker; the product never links or calls it.
@@ -20,6 +21,17 @@
  *   gsl_odeiv_evolve_apply + rkf45 + control_y (GSL's documented algorithm,
  *   with the dydt_out evaluation of rkf45) driven as ode_solver_Ham
  *   (1d/functions.c:528-552)
+ *   rhs_HsF1 / rhs_HsF3           1d/rhs.c:1-94, 243-399 (order 1 / 3)
+ *   fun_wallinfluxsource          1d/rhs.c:1216-1239
+ *   data_saving_break records     1d/breaking.c:426-444
+ *
+ * How it is pinned: besides closed forms and invariants (single Fourier
+ * modes at omega(k), linearity of HS1, the cubic HS3 increment, unit-weight
+ * depth reductions), by the reference's own outputs: run from the reference's
+ * case files it writes 1dB002HarmBar's Hawassi_eta_gauge.txt rows character
+ * for character (800 samples in the CPU tests, all 5000 through the device)
+ * and the breaking.dat rows of 1dFBr005Bor and 1dBBr002IrBar through their
+ * first breaking events (tests/test_case1d.py).
  *
  * The state is the reference's: 4 planes of N doubles, [Re eta^][Im eta^]
  * [Re u^][Im u^] of the full complex spectrum (1d/functions.c:537-538).
