@@ -58,7 +58,7 @@ class HwConfig1D(ct.Structure):
 
 class HwStats(ct.Structure):
     _fields_ = [("rhs_evals", ct.c_long), ("steps_accepted", ct.c_long), ("steps_rejected", ct.c_long),
-                ("rhs_ms_total", ct.c_double)]
+                ("rhs_ms_total", ct.c_double), ("rhs_timed", ct.c_long)]
 
 
 EXPORTS = [

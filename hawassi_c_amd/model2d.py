@@ -142,7 +142,7 @@ class Hawassi2D:
         s = _lib.HwStats()
         _lib.check(self.lib.hw_get_stats(self.ctx, ct.byref(s)), "hw_get_stats")
         return dict(rhs_evals=s.rhs_evals, steps_accepted=s.steps_accepted, steps_rejected=s.steps_rejected,
-                    rhs_ms_total=s.rhs_ms_total)
+                    rhs_ms_total=s.rhs_ms_total, rhs_timed=s.rhs_timed)
 
     def set_tprev(self, t: float):
         _lib.check(self.lib.hw_set_tprev(self.ctx, float(t)), "hw_set_tprev")

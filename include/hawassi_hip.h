@@ -97,6 +97,9 @@ typedef struct hw_stats {
     long steps_accepted;
     long steps_rejected;
     double rhs_ms_total;      /* device time in RHS evaluations (HIP events), ms */
+    long rhs_timed;           /* evaluations that rhs_ms_total covers: every fourth
+                                 2-D evaluation is bracketed by events (an event
+                                 record idles the GPU for a few microseconds) */
 } hw_stats;
 
 /* Build a context (device buffers, multipliers in device layout) from the
