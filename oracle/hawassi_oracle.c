@@ -38,6 +38,18 @@
 
 #include "orc_common.h"
 
+/* Planted mistakes, for the tests that show a check is sensitive: each code
+ * makes one plausible misreading of the reference (0 = none, the default).
+ *   1  E = (u^2 + v^2 - Lphi^2) w without the 1/2
+ *   2  E = (u^2 + v^2 + Lphi^2) w / 2 (sign of the L-term)
+ *   3  HSS keeps only its first half F(gam F^-1(L phi^))
+ *   4  Hilbert multiplier +i sign(kx + ky) instead of -i sign(kx + ky)
+ *   5  eddy-viscosity ramp B = U / U* (the "- 1" dropped)
+ *   6  gravity term of the u equation with ky in place of kx
+ *   7  3x3 peak test without its 0.001 tolerance (E(n) - E(c) <= 0) */
+static int orc_plant = 0;
+void orc_set_plant(int code) { orc_plant = code; }
+
 typedef struct { int Nx, Ny; fftplan *px, *py; } fft2plan;
 
 /* full complex 2-D transform of an [Ny][Nx] interleaved array (x fastest) */
@@ -235,8 +247,9 @@ static void hss_term(orc_ctx *c, const double *phi, const double *phihat, const 
     fft2(&c->P, g, -1); fft2(&c->P, gL, -1);
     #pragma omp parallel for schedule(static)
     for (size_t q = 0; q < N; q++) {
-        acc[2 * q] += (gL[2 * q] + O[q] * g[2 * q]) / 2;
-        acc[2 * q + 1] += (gL[2 * q + 1] + O[q] * g[2 * q + 1]) / 2;
+        double keep = orc_plant == 3 ? 0.0 : 1.0;
+        acc[2 * q] += (gL[2 * q] + keep * O[q] * g[2 * q]) / 2;
+        acc[2 * q + 1] += (gL[2 * q + 1] + keep * O[q] * g[2 * q + 1]) / 2;
     }
     free(L); free(g); free(gL);
 }
@@ -324,37 +337,38 @@ static double cp1d(double k, double D, double U)
 /* fun_peaks (2d/breaking.c:55-133) on EC [Ny][Nx] */
 static int find_peaks(const double *EC, int Nx, int Ny, double thr, orc_v2i *pk)
 {
+    const double tol = orc_plant == 7 ? 0.0 : 0.001;
 #define E(yy, xx) EC[(size_t)(yy) * Nx + (xx)]
     int n = 0;
     for (int row = 1; row < Nx - 1; ++row)
         for (int col = 1; col < Ny - 1; ++col) {
             int cnt = 0;
             for (int i = -1; i < 2; ++i)
-                for (int j = -1; j < 2; ++j) cnt += (E(col + i, row + j) - E(col, row) <= 0.001) ? 1 : 0;
+                for (int j = -1; j < 2; ++j) cnt += (E(col + i, row + j) - E(col, row) <= tol) ? 1 : 0;
             if (cnt == 9 && E(col, row) > thr) { pk[n].x = row; pk[n].y = col; n++; }
         }
     for (int col = 1; col < Ny - 1; ++col) {           /* x = 0 */
         int row = 0, cnt = 0;
         for (int i = -1; i < 2; ++i)
-            for (int j = 0; j < 2; ++j) cnt += (E(col + i, row + j) - E(col, row) <= 0.001) ? 1 : 0;
+            for (int j = 0; j < 2; ++j) cnt += (E(col + i, row + j) - E(col, row) <= tol) ? 1 : 0;
         if (cnt == 6 && E(col, row) > thr) { pk[n].x = row; pk[n].y = col; n++; }
     }
     for (int col = 1; col < Ny - 1; ++col) {           /* x = Nx-1 */
         int row = Nx - 1, cnt = 0;
         for (int i = -1; i < 2; ++i)
-            for (int j = -1; j < 1; ++j) cnt += (E(col + i, row + j) - E(col, row) <= 0.001) ? 1 : 0;
+            for (int j = -1; j < 1; ++j) cnt += (E(col + i, row + j) - E(col, row) <= tol) ? 1 : 0;
         if (cnt == 6 && E(col, row) > thr) { pk[n].x = row; pk[n].y = col; n++; }
     }
     for (int row = 1; row < Nx - 1; ++row) {           /* y = 0 */
         int col = 0, cnt = 0;
         for (int i = 0; i < 2; ++i)
-            for (int j = -1; j < 2; ++j) cnt += (E(col + i, row + j) - E(col, row) <= 0.001) ? 1 : 0;
+            for (int j = -1; j < 2; ++j) cnt += (E(col + i, row + j) - E(col, row) <= tol) ? 1 : 0;
         if (cnt == 6 && E(col, row) > thr) { pk[n].x = row; pk[n].y = col; n++; }
     }
     for (int row = 1; row < Nx - 1; ++row) {           /* y = Ny-1 */
         int col = Ny - 1, cnt = 0;
         for (int i = -1; i < 1; ++i)
-            for (int j = -1; j < 2; ++j) cnt += (E(col + i, row + j) - E(col, row) <= 0.001) ? 1 : 0;
+            for (int j = -1; j < 2; ++j) cnt += (E(col + i, row + j) - E(col, row) <= tol) ? 1 : 0;
         if (cnt == 6 && E(col, row) > thr) { pk[n].x = row; pk[n].y = col; n++; }
     }
 #undef E
@@ -423,6 +437,7 @@ static void breaking_process(orc_ctx *c, double t, const double *Z, const double
                 for (int j = 0; j < Nx; j++) {
                     size_t q = (size_t)i * Nx + j;
                     double sg = (p->kx[j] + p->ky[i]) < 0 ? -1.0 : 1.0;
+                    if (orc_plant == 4) sg = -sg;
                     H[2 * q] = Z[N + q] * sg;
                     H[2 * q + 1] = -Z[q] * sg;
                 }
@@ -529,7 +544,7 @@ static void breaking_process(orc_ctx *c, double t, const double *Z, const double
                         if (c->B[q] == 0) {
                             if (U[q] >= 2 * Us) c->B[q] = 1;
                             else if (U[q] <= Us) c->B[q] = 0;
-                            else c->B[q] = U[q] / Us - 1;
+                            else c->B[q] = U[q] / Us - (orc_plant == 5 ? 0.0 : 1.0);
                             if (c->B[q] > 0) { c->bn[nb].x = j; c->bn[nb].y = i; nb++; }
                         }
                     }
@@ -827,7 +842,8 @@ int orc_rhs(void *vc, double t, const double *Z, double *dZ)
         eu[2 * q] = eta[2 * q] * u[2 * q] / N2 * w;
         eM0v[2 * q] = eta[2 * q] * M0v[2 * q] / N2 * w;
         ev[2 * q] = eta[2 * q] * v[2 * q] / N2 * w;
-        dE[2 * q] = 0.5 * ((u[2 * q] * u[2 * q] + v[2 * q] * v[2 * q]) - Lphi[2 * q] * Lphi[2 * q]) / N2 * w;
+        double half = orc_plant == 1 ? 1.0 : 0.5, lsg = orc_plant == 2 ? -1.0 : 1.0;
+        dE[2 * q] = half * ((u[2 * q] * u[2 * q] + v[2 * q] * v[2 * q]) - lsg * Lphi[2 * q] * Lphi[2 * q]) / N2 * w;
     }
     double *eM0uh = cal(N), *euh = cal(N), *eM0vh = cal(N), *evh = cal(N), *dEh = cal(N);
     memcpy(eM0uh, eM0u, sizeof(double) * 2 * N); memcpy(euh, eu, sizeof(double) * 2 * N);
@@ -989,8 +1005,9 @@ int orc_rhs(void *vc, double t, const double *Z, double *dZ)
             double a = p->aal[q];
             dZ[q] = (Shat[2 * q] + dph[2 * q] - dampeta[2 * q]) * a;
             dZ[N + q] = (Shat[2 * q + 1] + dph[2 * q + 1] - dampeta[2 * q + 1]) * a;
-            dZ[2 * N + q] = (g * kx[j] * Z[N + q] + Sbx[2 * q] - ikxE_adj[2 * q] - dampu[2 * q]) * a;
-            dZ[3 * N + q] = (-g * kx[j] * Z[q] + Sbx[2 * q + 1] - ikxE_adj[2 * q + 1] - dampu[2 * q + 1]) * a;
+            double kg = orc_plant == 6 ? ky[i] : kx[j];
+            dZ[2 * N + q] = (g * kg * Z[N + q] + Sbx[2 * q] - ikxE_adj[2 * q] - dampu[2 * q]) * a;
+            dZ[3 * N + q] = (-g * kg * Z[q] + Sbx[2 * q + 1] - ikxE_adj[2 * q + 1] - dampu[2 * q + 1]) * a;
             dZ[4 * N + q] = (g * ky[i] * Z[N + q] + Sby[2 * q] - ikyE_adj[2 * q] - dampv[2 * q]) * a;
             dZ[5 * N + q] = (-g * ky[i] * Z[q] + Sby[2 * q + 1] - ikyE_adj[2 * q + 1] - dampv[2 * q + 1]) * a;
         }
@@ -1155,3 +1172,74 @@ int orc_integrate(void *vc, double *t, double t1, double *h, double *y, double e
 
 int orc_num_threads(void) { return omp_get_max_threads(); }
 void orc_set_threads(int n) { omp_set_num_threads(n); }
+
+/* ------------------------------------------------ pieces for the tests -- */
+/* HSSgen_2depth (or _3depth) of a real field phi [Ny][Nx] with the context's
+ * weights: which = 0 uses the C^2 multipliers (Csqr2d_*), 1 the omega^2 ones
+ * (Om2dSq_*).  out: full-grid complex spectrum, interleaved [Ny][Nx][2]. */
+void orc_hss(void *vc, const double *phi, int which, double *out)
+{
+    orc_ctx *c = (orc_ctx *)vc;
+    const orc_params *p = &c->p;
+    size_t N = (size_t)p->Nx * p->Ny;
+    double *ph = cal(N), *hat = cal(N);
+    for (size_t q = 0; q < N; q++) { ph[2 * q] = phi[q]; hat[2 * q] = phi[q]; }
+    fft2(&c->P, hat, -1);
+    if (which == 0) hss_2depth(c, ph, hat, p->csq_m, p->csq_p, p->csq_c, out, 1.0);
+    else hss_2depth(c, ph, hat, p->om2_m, p->om2_p, p->om2_c, out, 1.0);
+    free(ph); free(hat);
+}
+
+/* breaking_criterion's local quantities over the whole grid
+ * (2d/breaking.c:248-281): the Hilbert transform of eta, Hr =
+ * F^-1(P(-i sign(kx+ky) eta^)), and the local wavenumber
+ * |(eta grad Hr - Hr grad eta) / (eta^2 + Hr^2)| with gradient2D's centred /
+ * one-sided differences.  Z: the 6-plane state; outputs [Ny][Nx]. */
+void orc_break_diag(void *vc, const double *Z, double *eta_out, double *Hr_out, double *K_out)
+{
+    orc_ctx *c = (orc_ctx *)vc;
+    const orc_params *p = &c->p;
+    int Nx = p->Nx, Ny = p->Ny;
+    size_t N = (size_t)Nx * Ny;
+    double *e = cal(N), *H = cal(N);
+    for (int i = 0; i < Ny; i++)
+        for (int j = 0; j < Nx; j++) {
+            size_t q = (size_t)i * Nx + j;
+            double sg = (p->kx[j] + p->ky[i]) < 0 ? -1.0 : 1.0;
+            if (orc_plant == 4) sg = -sg;
+            e[2 * q] = Z[q]; e[2 * q + 1] = Z[N + q];
+            H[2 * q] = Z[N + q] * sg;
+            H[2 * q + 1] = -Z[q] * sg;
+        }
+    orc_hermitian_project(e, Nx, Ny);
+    orc_hermitian_project(H, Nx, Ny);
+    fft2(&c->P, e, +1);
+    fft2(&c->P, H, +1);
+    for (size_t q = 0; q < N; q++) { eta_out[q] = e[2 * q] / (double)N; Hr_out[q] = H[2 * q] / (double)N; }
+    double dx = p->x[1] - p->x[0], dy = p->y[1] - p->y[0];
+    const double *A = eta_out, *B = Hr_out;
+    for (int iy = 0; iy < Ny; iy++)
+        for (int ix = 0; ix < Nx; ix++) {
+#define GX(F) (ix == 0 ? (F[(size_t)iy * Nx + 1] - F[(size_t)iy * Nx]) / dx : ix == Nx - 1 ? (F[(size_t)iy * Nx + Nx - 1] - F[(size_t)iy * Nx + Nx - 2]) / dx : (F[(size_t)iy * Nx + ix + 1] - F[(size_t)iy * Nx + ix - 1]) / (2 * dx))
+#define GY(F) (iy == 0 ? (F[(size_t)Nx + ix] - F[ix]) / dy : iy == Ny - 1 ? (F[(size_t)(Ny - 1) * Nx + ix] - F[(size_t)(Ny - 2) * Nx + ix]) / dy : (F[(size_t)(iy + 1) * Nx + ix] - F[(size_t)(iy - 1) * Nx + ix]) / (2 * dy))
+            size_t q = (size_t)iy * Nx + ix;
+            double ev = A[q], hv = B[q];
+            double Kx = (ev * GX(B) - hv * GX(A)) / (ev * ev + hv * hv);
+            double Ky = (ev * GY(B) - hv * GY(A)) / (ev * ev + hv * hv);
+#undef GX
+#undef GY
+            K_out[q] = sqrt(Kx * Kx + Ky * Ky);
+        }
+    free(e); free(H);
+}
+
+/* fun_peaks (2d/breaking.c:55-133) on EC [Ny][Nx]: returns the count and the
+ * peaks' (x, y) indices in the reference's order */
+int orc_find_peaks(const double *EC, int Nx, int Ny, double thr, int *xs, int *ys)
+{
+    orc_v2i *pk = (orc_v2i *)malloc(sizeof(orc_v2i) * (size_t)Nx * Ny);
+    int n = find_peaks(EC, Nx, Ny, thr, pk);
+    for (int i = 0; i < n; i++) { xs[i] = pk[i].x; ys[i] = pk[i].y; }
+    free(pk);
+    return n;
+}

@@ -113,6 +113,11 @@ def lib():
         L.orc1d_get_B.argtypes = [ct.c_void_p, _dp]
         L.orc1d_fft.argtypes = [ct.c_int, _dp, ct.c_int]
         L.orc_set_threads.argtypes = [ct.c_int]
+        L.orc_set_plant.argtypes = [ct.c_int]
+        L.orc_hss.argtypes = [ct.c_void_p, _dp, ct.c_int, _dp]
+        L.orc_break_diag.argtypes = [ct.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_find_peaks.argtypes = [_dp, ct.c_int, ct.c_int, ct.c_double, ct.c_void_p, ct.c_void_p]
+        L.orc_find_peaks.restype = ct.c_int
         _lib = L
     return _lib
 
@@ -215,6 +220,23 @@ class Oracle2D:
         out = np.empty(self.model.nx * self.model.ny)
         lib().orc_get_B(self.ctx, _p(out))
         return out.reshape(self.model.ny, self.model.nx)
+
+    def hss(self, phi, which="csq"):
+        """HSSgen_2depth (or _3depth) of the real field phi [ny, nx] with the
+        C^2 ("csq") or omega^2 ("om2") multipliers: full-grid spectrum."""
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        out = np.empty(2 * phi.size)
+        lib().orc_hss(self.ctx, _p(phi), 0 if which == "csq" else 1, _p(out))
+        return (out[0::2] + 1j * out[1::2]).reshape(phi.shape)
+
+    def break_diag(self, y):
+        """(eta, Hilbert eta, local wavenumber K) over the grid, as
+        breaking_criterion forms them from the state y."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        sh = (self.model.ny, self.model.nx)
+        e, h, k = np.empty(sh), np.empty(sh), np.empty(sh)
+        lib().orc_break_diag(self.ctx, _p(y), _p(e), _p(h), _p(k))
+        return e, h, k
 
 
 def _cplx(a):
@@ -369,6 +391,23 @@ def ifft_c2r(spec: np.ndarray) -> np.ndarray:
     out = np.empty((ny, nx))
     lib().orc_ifft_c2r(nx, ny, _p(re), _p(im), _p(out))
     return out
+
+
+def set_plant(code: int):
+    """Switch on one planted mistake in the 2-D oracle (0 = none); see the
+    list at the top of hawassi_oracle.c.  For tests that show a check fails
+    on a plausible misreading."""
+    lib().orc_set_plant(int(code))
+
+
+def find_peaks(ec: np.ndarray, thr: float):
+    """fun_peaks on the [ny, nx] field ec: list of (x, y) in the reference's order."""
+    ec = np.ascontiguousarray(ec, dtype=np.float64)
+    ny, nx = ec.shape
+    xs = np.zeros(nx * ny, dtype=np.int32)
+    ys = np.zeros(nx * ny, dtype=np.int32)
+    n = lib().orc_find_peaks(_p(ec), nx, ny, float(thr), xs.ctypes.data, ys.ctypes.data)
+    return list(zip(xs[:n].tolist(), ys[:n].tolist()))
 
 
 def set_threads(n: int):
