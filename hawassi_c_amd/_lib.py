@@ -64,7 +64,7 @@ class HwStats(ct.Structure):
 EXPORTS = [
     "hw_ctx_create_2d", "hw_ctx_destroy", "hw_rhs", "hw_rhs_device", "hw_set_state", "hw_get_state",
     "hw_evolve_apply", "hw_integrate", "hw_integrate_batch", "hw_get_fields", "hw_get_stats", "hw_reset_stats", "hw_sync",
-    "hw_fft2_forward", "hw_fft2_inverse", "hw_supported", "hw_last_error", "hw_version",
+    "hw_fft2_forward", "hw_fft2_inverse", "hw_fft2_forward_noalloc", "hw_supported", "hw_last_error", "hw_version",
     "hw_set_tprev", "hw_get_breaking",
     "hw1d_create", "hw1d_destroy", "hw1d_rhs", "hw1d_rhs_at", "hw1d_set_state", "hw1d_get_state",
     "hw1d_set_time", "hw1d_get_time", "hw1d_integrate", "hw1d_get_breaking", "hw1d_get_stats",
@@ -105,6 +105,7 @@ def load(path: Path | None = None) -> ct.CDLL:
     lib.hw_get_breaking.argtypes = [vp, _dp, ct.POINTER(ct.c_int)]
     lib.hw_fft2_forward.argtypes = [ct.c_int, ct.c_int, ct.c_int, _dp, _dp, _dp]
     lib.hw_fft2_inverse.argtypes = [ct.c_int, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp]
+    lib.hw_fft2_forward_noalloc.argtypes = [ct.c_int, ct.c_int, ct.c_int, vp, vp, vp, vp]
     lib.hw_supported.argtypes = [ct.c_int, ct.c_int, ct.c_int]
     lib.hw1d_create.argtypes = [ct.POINTER(HwConfig1D), ct.c_int, ct.c_int, ct.POINTER(vp)]
     lib.hw1d_destroy.argtypes = [vp]

@@ -41,6 +41,34 @@ static int check_device() {
     return HW_OK;
 }
 
+// ---- scratch of the zero-allocation entry point (hw_fft2_forward_noalloc) --
+// One array per GPU, held until the process ends.  Host code never uses its
+// name as an address: each device's copy lives at its own address, which
+// hipGetSymbolAddress gives once per device (na_scratch_ptr), and that
+// pointer is what the kernels receive.
+static __device__ __align__(256) unsigned char scratch[HW_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_na_mutex;
+static void* g_na_ptr[64] = {};
+
+static int na_scratch_ptr(unsigned char** out) {
+    int dev = 0;
+    HW_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) { set_error("device ordinal %d out of range", dev); return HW_ERR_ARG; }
+    std::lock_guard<std::mutex> lk(g_na_mutex);
+    if (!g_na_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+            return HW_ERR_HIP;
+        }
+        g_na_ptr[dev] = p;
+    }
+    *out = static_cast<unsigned char*>(g_na_ptr[dev]);
+    return HW_OK;
+}
+
 }  // namespace hw
 
 using namespace hw;
@@ -261,6 +289,17 @@ int hw_fft2_inverse(int nx, int ny, int precision, const double* re, const doubl
     int rc = check_device();
     if (rc) return rc;
     return e->inv(re, im, phys, nyq_im);
+}
+
+int hw_fft2_forward_noalloc(int nx, int ny, int precision, const double* d_phys, double* d_re, double* d_im,
+                            void* stream) {
+    const Entry* e = find_entry(precision, nx, ny);
+    if (!e) { set_error("unsupported grid %dx%d precision %d", nx, ny, precision); return HW_ERR_ARG; }
+    if (!d_phys || !d_re || !d_im) { set_error("null argument"); return HW_ERR_ARG; }
+    unsigned char* ws = nullptr;
+    int rc = na_scratch_ptr(&ws);
+    if (rc) return rc;
+    return e->fwd_na(d_phys, d_re, d_im, static_cast<hipStream_t>(stream), ws, (size_t)HW_NOALLOC_SCRATCH_BYTES);
 }
 
 }  // extern "C"

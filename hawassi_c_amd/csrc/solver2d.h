@@ -920,12 +920,114 @@ int fft2_inverse(const double* re, const double* im, double* phys, double* nyq_i
     return HW_OK;
 }
 
+// ------------------------------------- zero-allocation forward transform --
+// The forward transform of hw_fft2_forward on device arrays, with every
+// intermediate (twiddle tables, the T-precision field, the x-pass and y-pass
+// spectra) carved out of a caller-supplied region of device memory `ws`
+// (the library's static scratch array): no allocation, no synchronisation,
+// everything enqueued on `stream`.
+
+// Twiddle tables on the device, the same formulas as x_twiddles / y_twiddles
+// (angles in FP64, rounded once to T).
+template <typename T, int NX, int NY>
+__global__ void k_na_twiddles(typename ctype<T>::type* twx, typename ctype<T>::type* twy) {
+    using XG = XGeo<T, NX>;
+    using YG = YGeo<T, NY>;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    auto pass_entry = [](int L, int E, int q, double& a) -> bool {
+        for (int I = 1; I < rs_np(L, E); ++I) {
+            const int R = rs_radix(L, E, I), NS = rs_ns(L, E, I), o = rs_twoff(L, E, I);
+            if (q >= o && q < o + (R - 1) * NS) {
+                const int r = (q - o) / NS + 1, k = (q - o) % NS;
+                const long m = (long)r * k * (L / (NS * R)) % L;
+                a = -2.0 * M_PI * (double)m / (double)L;
+                return true;
+            }
+        }
+        return false;
+    };
+    if (i < XG::TWN) {
+        double a = 0.0;
+        bool set = i < XG::TWPP ? pass_entry(XG::L, XG::E, i, a) : true;
+        if (i >= XG::TWPP) a = -2.0 * M_PI * (double)(i - XG::TWPP) / (double)NX;
+        twx[i].x = set ? (T)cos(a) : (T)0;
+        twx[i].y = set ? (T)sin(a) : (T)0;
+    }
+    if (i < YG::TWN) {
+        double a = 0.0;
+        bool set = pass_entry(NY, YG::E, i, a);
+        twy[i].x = set ? (T)cos(a) : (T)0;
+        twy[i].y = set ? (T)sin(a) : (T)0;
+    }
+}
+
+// half spectrum [kx][ky] (kx = 0..NX/2) -> full-grid planes [ky][kx], the
+// columns kx > NX/2 as the conjugate mirror (what hw_fft2_forward returns)
+template <typename T, int NX, int NY>
+__global__ void k_na_expand(const typename ctype<T>::type* __restrict__ S, double* __restrict__ re,
+                            double* __restrict__ im) {
+    constexpr int NXH = NX / 2 + 1;
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)NX * NY) return;
+    const int ky = (int)(q / NX), kx = (int)(q % NX);
+    double a, b;
+    if (kx < NXH) {
+        const auto v = S[(size_t)kx * NY + ky];
+        a = (double)v.x; b = (double)v.y;
+    } else {
+        const auto v = S[(size_t)(NX - kx) * NY + (NY - ky) % NY];
+        a = (double)v.x; b = -(double)v.y;
+    }
+    re[q] = a;
+    im[q] = b;
+}
+
+constexpr size_t na_align(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// bytes of scratch the zero-allocation forward transform needs at (T, NX, NY)
+template <typename T, int NX, int NY> constexpr size_t fft2_na_bytes() {
+    using C = typename ctype<T>::type;
+    constexpr size_t H = (size_t)(NX / 2 + 1) * NY, N = (size_t)NX * NY;
+    return na_align(sizeof(C) * XGeo<T, NX>::TWN) + na_align(sizeof(C) * YGeo<T, NY>::TWN) +
+           na_align(sizeof(T) * N) + 2 * na_align(sizeof(C) * H);
+}
+
+template <typename T, int NX, int NY>
+int fft2_forward_noalloc(const double* d_phys, double* d_re, double* d_im, hipStream_t stream, unsigned char* ws,
+                         size_t ws_bytes) {
+    using C = typename ctype<T>::type;
+    constexpr int NXH = NX / 2 + 1;
+    constexpr size_t H = (size_t)NXH * NY, N = (size_t)NX * NY;
+    constexpr size_t need = fft2_na_bytes<T, NX, NY>();
+    if (need > ws_bytes) {
+        set_error("hw_fft2_forward_noalloc: %dx%d (%s) needs %zu bytes of scratch, the static array holds %zu "
+                  "(grids up to 512x512 fit)", NX, NY, sizeof(T) == 8 ? "FP64" : "FP32", need, ws_bytes);
+        return HW_ERR_ARG;
+    }
+    unsigned char* p = ws;
+    C* twx = (C*)p; p += na_align(sizeof(C) * XGeo<T, NX>::TWN);
+    C* twy = (C*)p; p += na_align(sizeof(C) * YGeo<T, NY>::TWN);
+    T* ph = (T*)p; p += na_align(sizeof(T) * N);
+    C* X = (C*)p; p += na_align(sizeof(C) * H);
+    C* S = (C*)p;
+    constexpr int TWMAX = XGeo<T, NX>::TWN > YGeo<T, NY>::TWN ? XGeo<T, NX>::TWN : YGeo<T, NY>::TWN;
+    k_na_twiddles<T, NX, NY><<<(TWMAX + 255) / 256, 256, 0, stream>>>(twx, twy);
+    k_convert_scalar<double, T><<<(unsigned)((N + 255) / 256 < 4096 ? (N + 255) / 256 : 4096), 256, 0, stream>>>(
+        d_phys, ph, N);
+    k_t_xfwd<T, NX, NY><<<NY / XGeo<T, NX>::RR, XGeo<T, NX>::NT, 0, stream>>>(ph, X, twx);
+    k_t_yfwd<T, NX, NY><<<(NXH + YGeo<T, NY>::CC - 1) / YGeo<T, NY>::CC, YGeo<T, NY>::NT, 0, stream>>>(X, S, twy);
+    k_na_expand<T, NX, NY><<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(S, d_re, d_im);
+    HW_LAUNCH_CHECK();
+    return HW_OK;
+}
+
 // ------------------------------------------------------------- registry --
 struct Entry {
     int prec, nx, ny;
     Solver2D* (*make)();
     int (*fwd)(const double*, double*, double*);
     int (*inv)(const double*, const double*, double*, double*);
+    int (*fwd_na)(const double*, double*, double*, hipStream_t, unsigned char*, size_t);
 };
 std::vector<Entry>& registry();
 struct Registrar {
@@ -944,5 +1046,6 @@ struct Registrar {
         return ::hw::fft2_inverse<T, NX, NY>(r, i, p, n);                                   \
     }                                                                                       \
     ::hw::Registrar reg_##PREC##_##NX##_##NY({PREC, NX, NY, make_##PREC##_##NX##_##NY,       \
-                                              fwd_##PREC##_##NX##_##NY, inv_##PREC##_##NX##_##NY}); \
+                                              fwd_##PREC##_##NX##_##NY, inv_##PREC##_##NX##_##NY, \
+                                              ::hw::fft2_forward_noalloc<T, NX, NY>});       \
     }

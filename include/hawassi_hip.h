@@ -168,6 +168,23 @@ int hw_sync(void *ctx);
 int hw_fft2_forward(int nx, int ny, int precision, const double *phys, double *spec_re, double *spec_im);
 int hw_fft2_inverse(int nx, int ny, int precision, const double *spec_re, const double *spec_im, double *phys, double *nyq_im);
 
+/* hw_fft2_forward on DEVICE arrays without allocating: this entry point has no
+ * counterpart in the reference; it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  d_phys [ny*nx] and d_re, d_im [ny*nx] are FP64 device
+ * arrays in the layouts of hw_fft2_forward; stream is a hipStream_t (NULL:
+ * the default stream).  The work is enqueued on stream and not waited for; it
+ * calls no hipMalloc, hipFree or allocating routine.  Its intermediates live
+ * in one static device array of HW_NOALLOC_SCRATCH_BYTES per GPU (twiddle
+ * tables, the field in the working precision, the x- and y-pass spectra):
+ * every compiled grid up to 512 x 512 fits in either precision, larger grids
+ * return HW_ERR_ARG.  All calls on one GPU share that array, so calls that
+ * overlap on one GPU (other streams, other threads) race; nothing in it needs
+ * to start at zero. */
+#define HW_NOALLOC_SCRATCH_BYTES (8u << 20)
+int hw_fft2_forward_noalloc(int nx, int ny, int precision, const double *d_phys, double *d_re, double *d_im,
+                            void *stream);
+
 /* ------------------------------------------------------------------ 1-D --
  * The 1-D HS2 path (rhs_HsF2, 1d/rhs.c:96-241) for a batch of independent
  * realizations of possibly different n.  State layout per realization: 4
