@@ -71,7 +71,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--spinup", type=float, default=10.0, help="simulated seconds integrated before warmup")
+    ap.add_argument("--spinup", type=float, default=150.0, help="simulated seconds integrated from rest before warmup")
+    ap.add_argument("--acc-window", type=float, default=60.0,
+                    help="harbour2048: FP32 vs FP64 over this many simulated seconds ending at the spin-up time")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -79,6 +81,9 @@ def main():
                     help="harbour2048: the 2-D headline; slope1d: the 1-D batch of uneven realizations; "
                          "opensea1024x8: eight flat-bottom 1024^2 realizations advanced in lockstep")
     ap.add_argument("--nreal", type=int, default=256, help="slope1d: realizations in the batch")
+    ap.add_argument("--depths", type=int, default=2, choices=[2, 3], help="slope1d: reference depths")
+    ap.add_argument("--round-limit", type=int, default=0,
+                    help="slope1d: rounds per output interval before the run stops with an error (0: none)")
     ap.add_argument("--order", type=int, default=2, choices=[2, 3],
                     help="harbour2048: dynamics, 2 = HS2 (rhs_AB2_2DB_uv), 3 = HS3 (rhs_AB3_2DB_uv)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -108,7 +113,9 @@ def main():
     from hawassi_c_amd.model2d import Hawassi2D
 
     n = args.n
-    t_end = args.spinup + 60.0
+    spin = args.spinup
+    t_acc0 = max(0.0, spin - args.acc_window)
+    t_end = spin + 40.0
     log(f"setup harbour {n}x{n}")
     model = W.harbour(n, t_end=t_end)
     model.order = args.order
@@ -116,15 +123,28 @@ def main():
     dim = 6 * n * n
     dev.set_state(np.zeros(dim))
     t, h = 0.0, 0.05
-    # spin-up: let the influx build a wave field so step sizes are representative
-    t, h = dev.integrate(t, args.spinup, h)
-    log(f"spin-up done t={t:.3f} h={h:.4f} stats={dev.stats()}")
+    # spin-up from rest: the influx fills the domain, the waves pass the
+    # breakwater gap (x = 1000 m, reached at ~80 s) and run onto the shallows;
+    # the state at t_acc0 starts the accuracy comparison
+    w0 = time.perf_counter()
+    state_acc0, h_acc0 = None, None
+    for tc in np.arange(10.0, spin + 1e-9, 10.0):
+        if state_acc0 is None and tc > t_acc0 + 1e-9:
+            t, h = dev.integrate(t, t_acc0, h)
+            state_acc0, h_acc0 = dev.get_state(), h
+        t, h = dev.integrate(t, float(tc), h)
+        st = dev.stats()
+        log(f"spin-up t={t:.1f} h={h:.4f} rhs={st['rhs_evals']} rejected={st['steps_rejected']} "
+            f"wall {time.perf_counter() - w0:.1f} s")
+    if state_acc0 is None:
+        state_acc0, h_acc0 = dev.get_state(), h
     for _ in range(args.warmup):
         t, h = dev.evolve_apply(t, t_end, h)
     dev.sync()
     dev.reset_stats()
     t_start, h_start = t, h
-    state_start = dev.get_state() if not (args.no_cpu_baseline and args.no_accuracy) else None
+    state_start = dev.get_state()
+    _, flag_start = dev.breaking_state()
     dev.sync()
     w0 = time.perf_counter()
     for _ in range(args.steps):
@@ -154,8 +174,10 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"harbour {n}x{n}: 1.5 km square, depth 20->4 m + seeded bumps, two-depth "
                                + ("AB2 (HS2)" if args.order == 2 else "AB3 (HS3)") + " operator, breakwater R=0.5 with 100 m gap, directional JONSWAP line influx; "
-                               "RKF45 eps_rel 1e-2 eps_abs 1e-3 (reference tolerances)",
-                   "nx": n, "ny": n, "state": "fp64 half spectrum", "sim_window_s": [t_start, t]},
+                               "RKF45 eps_rel 1e-2 eps_abs 1e-3 (reference tolerances); timed after a spin-up from rest "
+                               "with the sea developed past the breakwater",
+                   "nx": n, "ny": n, "state": "fp64 half spectrum", "sim_window_s": [t_start, t],
+                   "breaking_flag_at_start": flag_start},
         "rhs_evals_per_s": rhs_per_s,
         "rhs_evals": st["rhs_evals"],
         "steps_rejected": st["steps_rejected"],
@@ -170,48 +192,105 @@ def main():
                          "bytes_per_rhs": b_job},
     }
     dev.close()
+    rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
     if not args.no_accuracy:
-        # FP32 path vs the FP64 device path (itself parity-checked against the
-        # oracle at <=1e-11) over the timed window, same start state and end time
-        log("accuracy check vs FP64 device path")
-        d64 = Hawassi2D(model, HW_PREC_FP64)
-        d64.set_state(state_start)
-        d64.set_tprev(t_start)
-        t64, _ = d64.integrate(t_start, t, h_start)
-        e64, u64, v64 = d64.fields()
-        d64.close()
-        d32 = Hawassi2D(model, HW_PREC_FP32)
-        d32.set_state(state_start)
-        d32.set_tprev(t_start)
-        d32.integrate(t_start, t, h_start)
-        e32, u32, v32 = d32.fields()
-        d32.close()
-        rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
-        out["accuracy"] = {"relL2_eta": rel(e32, e64), "relL2_u": rel(u32, u64), "relL2_v": rel(v32, v64),
-                           "reference": "FP64 device path (oracle-checked in tests)", "window_s": [t_start, t]}
+        # FP32 vs FP64 on the device over the developed-sea window t_acc0..spin,
+        # both from the same FP32-run state (fresh breaking histories)
+        acc = {"window_s": [t_acc0, spin], "reference": "FP64 device path (oracle-checked in tests)"}
+        res = {}
+        for name, prec in (("fp64", HW_PREC_FP64), ("fp32", HW_PREC_FP32)):
+            d = Hawassi2D(model, prec)
+            d.set_state(state_acc0)
+            d.set_tprev(t_acc0)
+            w1 = time.perf_counter()
+            ta, ha = t_acc0, h_acc0
+            for tc in np.arange(t_acc0 + 10.0, spin + 1e-9, 10.0):
+                ta, ha = d.integrate(ta, float(tc), ha)
+            log(f"accuracy run {name}: t={ta:.2f} wall {time.perf_counter() - w1:.1f} s {d.stats()}")
+            res[name] = (physical_fields(model, d.get_state()), d.breaking_state())
+            d.close()
+        (f64, (B64, fl64)), (f32, (B32, fl32)) = res["fp64"], res["fp32"]
+        for k in ("eta", "u", "v", "phi"):
+            acc["relL2_" + k] = rel(f32[k], f64[k])
+        acc["breaking_flag"] = [int(fl32), int(fl64)]
+        acc["relL2_B"] = rel(B32, B64) if np.any(B64) else ("B = 0 in both" if not np.any(B32) else "B64 = 0, B32 != 0")
+        acc["pass_1e-4"] = bool(max(acc["relL2_" + k] for k in ("eta", "u", "v", "phi")) <= 1e-4)
+        out["accuracy"] = acc
 
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
         from oracle import oracle as O
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         O.set_threads(threads)
-        log(f"cpu baseline: oracle, {threads} threads")
+        log(f"cpu baseline: oracle, {threads} threads, 3 RKF45 steps from t={t_start:.3f}")
         orc = O.Oracle2D(model)
         orc.set_tprev(t_start)   # the driver's tprev after the previous accepted step
         y = state_start.copy()
         c0 = orc.rhs_count()
         w0 = time.perf_counter()
-        rc, t1, h1, stc = orc.evolve_apply(t_start, t_end, h_start, y, model.eps_abs, model.eps_rel)
+        tt, hh = t_start, h_start
+        y_one = None
+        for k in range(3):
+            rc, tt, hh, stc = orc.evolve_apply(tt, t_end, hh, y, model.eps_abs, model.eps_rel)
+            if k == 0:
+                y_one, t_one = y.copy(), tt
+            log(f"  oracle step {k + 1}: t={tt:.4f} wall {time.perf_counter() - w0:.1f} s")
         cw = time.perf_counter() - w0
         nr = orc.rhs_count() - c0
-        out["cpu_baseline"] = {"value": (t1 - t_start) / cw, "unit": "sim-s/wall-s", "cores": threads,
+        # scaling of the oracle's right-hand side with host threads (one evaluation each)
+        scale = {}
+        for th in (1, 2, 4, 8):
+            O.set_threads(th)
+            o2 = O.Oracle2D(model)
+            o2.set_tprev(t_start)
+            w1 = time.perf_counter()
+            o2.rhs(t_start, state_start)
+            scale[str(th)] = 1.0 / (time.perf_counter() - w1)
+            log(f"  oracle RHS with {th} threads: {1.0 / scale[str(th)]:.2f} s")
+        O.set_threads(threads)
+        out["cpu_baseline"] = {"value": (tt - t_start) / cw, "unit": "sim-s/wall-s", "cores": threads,
                                "kind": "port",
-                               "sample": f"1 RKF45 step ({nr} RHS incl. GSL dydt_out, h={t1 - t_start:.4g} s) of the "
-                                         f"same {n}x{n} harbour from the same state at t={t_start:.3f} s; FP64 "
-                                         f"oracle (restated CPU path, radix-2 FFT, OpenMP), {cpu_model()}",
-                               "rhs_evals_per_s": nr / cw, "wall_s": cw}
+                               "sample": f"3 RKF45 steps ({nr} RHS incl. GSL dydt_out, {tt - t_start:.4g} s simulated) of "
+                                         f"the same {n}x{n} harbour from the same state at t={t_start:.3f} s; FP64 "
+                                         f"oracle (restated CPU path, the project's radix-2 FFT -- no FFTW -- with "
+                                         f"OpenMP), {cpu_model()}",
+                               "rhs_evals_per_s": nr / cw, "wall_s": cw,
+                               "rhs_per_s_by_threads": scale}
+        # the device from the same state, one RKF45 step, against the oracle's step
+        ver = {"t": [t_start, t_one]}
+        fo = physical_fields(model, y_one)
+        for name, prec in (("fp32", HW_PREC_FP32), ("fp64", HW_PREC_FP64)):
+            d = Hawassi2D(model, prec)
+            d.set_state(state_start)
+            d.set_tprev(t_start)
+            td, _ = d.evolve_apply(t_start, t_end, h_start)
+            fd = physical_fields(model, d.get_state())
+            ver[name] = {k: rel(fd[k], fo[k]) for k in ("eta", "u", "v", "phi")}
+            ver[name]["t_end"] = td
+            d.close()
+        out["accuracy_vs_oracle_one_step"] = ver
     print(json.dumps(out))
+
+
+def physical_fields(model, y):
+    """eta, u, v and the curl-free potential phi (phi^ = -i (kx u^ + ky v^)/|k|^2)
+    of a full-grid state, on the host (the reference's projected inverse)."""
+    nx, ny = model.nx, model.ny
+    N = nx * ny
+    pl = [(y[2 * f * N:(2 * f + 1) * N] + 1j * y[(2 * f + 1) * N:(2 * f + 2) * N]).reshape(ny, nx)[:, : nx // 2 + 1]
+          for f in range(3)]
+    KX, KY = model.kx[None, : nx // 2 + 1], model.ky[:, None]
+    k2 = KX * KX + KY * KY
+    ph = -1j * (KX * pl[1] + KY * pl[2]) / np.where(k2 > 1e-12, k2, 1.0)
+    ph[0, 0] = 0.0
+    out = {}
+    for name, s in zip(("eta", "u", "v", "phi"), (pl[0], pl[1], pl[2], ph)):
+        s = s.copy()
+        s[ny // 2 + 1:, 0] = np.conj(s[1: ny // 2][::-1, 0])   # the projection of column 0
+        s[0, 0] = s[0, 0].real
+        out[name] = np.fft.irfft2(s, s=(ny, nx))
+    return out
 
 
 def bench_opensea(args):
@@ -226,7 +305,7 @@ def bench_opensea(args):
     from hawassi_c_amd.model2d import Hawassi2D, integrate_batch
 
     nb, n, dt_out = 8, 1024, 0.5
-    spin = args.spinup if args.spinup != 10.0 else 10.0
+    spin = args.spinup if args.spinup != 150.0 else 10.0
     t_end = spin + (args.warmup + args.steps + 2) * dt_out
     log(f"setup {nb} open-sea realizations {n}x{n}")
     ds = [Hawassi2D(W.short_crested(n, seed=3 + b, t_end=t_end), HW_PREC_FP32) for b in range(nb)]
@@ -276,57 +355,70 @@ def bench_opensea(args):
 
 
 def bench_slope1d(args):
-    """1-D batch: nreal realizations of the sloping-beach HS2 case (three
-    reference depths, breaking) with n drawn from {4096, 8192, 16384}, FP32
-    transforms / FP64 state, advanced together by the device driver through
-    the reference's output times (every 0.05 s).  A step is one output
-    interval of the whole batch; value = realizations x simulated seconds per
-    wall second."""
-    from hawassi_c_amd import HW_PREC_FP32, HW_PREC_FP64
+    """1-D batch: nreal realizations of the sloping-beach HS2 case (two
+    reference depths, Up2IP, as the configuration names; breaking) with n drawn
+    from {4096, 8192, 16384}, FP32 transforms / FP64 state, advanced together
+    by the device driver through the reference's output times (every 0.05 s).
+    A step is one output interval of the whole batch; value = realizations x
+    simulated seconds per wall second.  Progress is printed every output
+    interval; --round-limit bounds one interval's rounds (the reference's
+    step control rejects steps in cascades in
+    breaking realizations (the FP64 oracle needs up to ~1400 right-hand sides
+    per 0.05 s for one of them, with steps down to 1e-6 s), and the batch
+    advances at the pace of its slowest realization."""
+    from hawassi_c_amd import HW_PREC_FP32
     from hawassi_c_amd import workloads1d as W1
     from hawassi_c_amd.model1d import Hawassi1D
 
     dt_out = 0.05
-    spin = args.spinup if args.spinup != 10.0 else 2.0
-    # the window this workload is quoted on (t = 2.05 .. 2.55 s); later the
-    # breaking at the shallow end makes some realizations take much smaller
-    # steps, which a default run should not wait for
+    spin = args.spinup if args.spinup != 150.0 else 2.0
     if args.steps == 20 and args.warmup == 10:
         args.steps, args.warmup = 10, 1
     t_end = spin + (args.warmup + args.steps + 4) * dt_out
-    log(f"setup {args.nreal} realizations")
-    models = W1.slope_batch(args.nreal, seed=7, t_end=t_end)
+    log(f"setup {args.nreal} realizations, {args.depths} reference depths")
+    models = W1.slope_batch(args.nreal, seed=7, t_end=t_end, ndepth=args.depths)
     ns = [m.n for m in models]
     dev = Hawassi1D(models, HW_PREC_FP32, t0=0.0, h0=1e-4)
+    dev.set_round_limit(args.round_limit)
     t = 0.0
     w0 = time.perf_counter()
+
+    def advance(t_to):
+        dev.integrate(t_to)
+        return t_to
+
     while t < spin - 1e-12:
-        t = min(spin, t + dt_out)
-        dev.integrate(t)
+        t = advance(min(spin, round(t + dt_out, 10)))
         log(f"  spin-up t={t:.2f} wall {time.perf_counter() - w0:.1f} s")
     log(f"spin-up done t={t:.2f} stats={dev.stats()}")
     for _ in range(args.warmup):
-        t += dt_out
-        dev.integrate(t)
+        t = advance(round(t + dt_out, 10))
     st0 = dev.stats()
     t_start = t
     w0 = time.perf_counter()
     for i in range(args.steps):
-        t += dt_out
-        dev.integrate(t)
-        log(f"  timed {i + 1}/{args.steps} wall {time.perf_counter() - w0:.1f} s")
+        t = advance(round(t + dt_out, 10))
+        st = dev.stats()
+        log(f"  timed {i + 1}/{args.steps} t={t:.2f} wall {time.perf_counter() - w0:.1f} s "
+            f"rhs {st['rhs_evals'] - st0['rhs_evals']} rejected {st['steps_rejected'] - st0['steps_rejected']}")
     wall = time.perf_counter() - w0
     st1 = dev.stats()
     sim = t - t_start
     nrhs = st1["rhs_evals"] - st0["rhs_evals"]
     log(f"timed {args.steps} output intervals: wall {wall:.3f} s, sim {sim:.3f} s, rhs {nrhs}")
     # algorithmic bytes per realization-RHS (transform field traffic model,
-    # T_alg = 23 with three reference depths (20 with two), plus coefficients
-    # and the FP64 state in/out)
+    # T_alg = 20 with two reference depths (23 with three), plus the
+    # coefficient fields and the FP64 state in/out)
+    t_alg = 20 if args.depths == 2 else 23
+
     def b_rhs(n):
         h = n // 2 + 1
-        return 23 * (4 * n + 8 * h) + 8 * 4 * n + 2 * (2 * 16 * h)
-    mean_b = float(np.mean([b_rhs(n) for n in ns]))
+        return t_alg * (4 * n + 8 * h) + 8 * 4 * n + 2 * (2 * 16 * h)
+    batch_b = float(sum(b_rhs(n) for n in ns))
+    mean_b = batch_b / len(ns)
+    dt_launch = (st1["rhs_ms_total"] - st0["rhs_ms_total"]) / max(st1["rhs_timed"] - st0["rhs_timed"], 1)
+    achieved = batch_b / (dt_launch * 1e-3) / 1e9
+    job = nrhs * mean_b / wall / 1e9
     out = {
         "metric": "sim-sec/wall-sec (1-D batch, realizations x simulated seconds)",
         "value": args.nreal * sim / wall,
@@ -336,12 +428,18 @@ def bench_slope1d(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32", "data": "synthetic",
         "config": {"workload": f"slope1d batch of {args.nreal}: n in {{4096, 8192, 16384}} "
-                               f"(counts {ns.count(4096)}/{ns.count(8192)}/{ns.count(16384)}), 400 m, depth 1.0 -> 0.2 m "
-                               "three reference depths (Up3IP, the reference's 1-D default), "
-                               "JONSWAP Hs 0.10 Tp 2.0, breaking KBC 0.85",
+                               f"(counts {ns.count(4096)}/{ns.count(8192)}/{ns.count(16384)}), 400 m, depth 1.0 -> 0.2 m, "
+                               + ("two reference depths (Up2IP)" if args.depths == 2 else
+                                  "three reference depths (Up3IP, the reference's 1-D default)")
+                               + ", JONSWAP Hs 0.10 Tp 2.0, breaking KBC 0.85",
                    "sim_window_s": [t_start, t]},
         "rhs_evals_per_s": nrhs / wall,
         "realization_rhs_bytes_mean": mean_b,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit_of_work": "one launch of the batch right-hand side (k1d_rhs, every realization active), "
+                                     "%.1f MB algorithmic, %.3f ms event-timed" % (batch_b / 1e6, dt_launch)},
+        "roofline_job": {"achieved": job, "frac": job / HBM_PEAK_GBS, "unit": "GB/s"},
     }
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))

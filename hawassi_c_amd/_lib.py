@@ -67,7 +67,8 @@ EXPORTS = [
     "hw_fft2_forward", "hw_fft2_inverse", "hw_fft2_forward_noalloc", "hw_supported", "hw_last_error", "hw_version",
     "hw_set_tprev", "hw_get_breaking",
     "hw1d_create", "hw1d_destroy", "hw1d_rhs", "hw1d_rhs_at", "hw1d_set_state", "hw1d_get_state",
-    "hw1d_set_time", "hw1d_get_time", "hw1d_integrate", "hw1d_get_breaking", "hw1d_get_stats",
+    "hw1d_set_time", "hw1d_get_time", "hw1d_integrate", "hw1d_set_round_limit", "hw1d_get_breaking",
+    "hw1d_get_stats",
 ]
 
 _lib = None
@@ -117,6 +118,7 @@ def load(path: Path | None = None) -> ct.CDLL:
     lib.hw1d_set_time.argtypes = [vp, ct.c_int, ct.c_double, ct.c_double]
     lib.hw1d_get_time.argtypes = [vp, ct.c_int, _dp, _dp]
     lib.hw1d_integrate.argtypes = [vp, ct.c_double]
+    lib.hw1d_set_round_limit.argtypes = [vp, ct.c_long]
     lib.hw1d_get_breaking.argtypes = [vp, ct.c_int, _dp, ct.POINTER(ct.c_int)]
     lib.hw1d_get_stats.argtypes = [vp, ct.c_int, ct.POINTER(HwStats)]
     lib.hw_last_error.restype = ct.c_char_p

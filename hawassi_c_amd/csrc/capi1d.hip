@@ -104,7 +104,14 @@ int hw1d_get_time(void* ctx, int r, double* t, double* h) {
 int hw1d_integrate(void* ctx, double t1) {
     Solver1D* s = static_cast<Solver1D*>(ctx);
     if (!s) { set_error("null argument"); return HW_ERR_ARG; }
-    return s->integrate(t1, 10000000L);
+    return s->integrate(t1, 0L);
+}
+
+int hw1d_set_round_limit(void* ctx, long max_rounds) {
+    Solver1D* s = static_cast<Solver1D*>(ctx);
+    if (!s || max_rounds < 0) { set_error("null context or negative limit"); return HW_ERR_ARG; }
+    s->round_limit = max_rounds;
+    return HW_OK;
 }
 
 int hw1d_get_breaking(void* ctx, int r, double* B, int* flag) {

@@ -1172,15 +1172,19 @@ static __global__ void k1d_begin(Rk1D* rk, int nreal, double t1) {
     else { S->h0 = h; S->final_step = 0; }
 }
 
-// active count (for the host's loop condition)
-static __global__ void k1d_count(const Rk1D* rk, int nreal, int* out) {
-    __shared__ int c;
-    if (threadIdx.x == 0) c = 0;
+// active count and crest-list overflows (for the host's loop condition):
+// out[0] = realizations still short of t1, out[1] = realizations whose
+// breaking crest list overflowed
+static __global__ void k1d_count(const Rk1D* rk, const Brk1D* bs, int nreal, int* out) {
+    __shared__ int c, o;
+    if (threadIdx.x == 0) { c = 0; o = 0; }
     __syncthreads();
-    for (int r = threadIdx.x; r < nreal; r += blockDim.x)
+    for (int r = threadIdx.x; r < nreal; r += blockDim.x) {
         if (rk[r].active) atomicAdd(&c, 1);
+        if (bs[r].overflow) atomicAdd(&o, 1);
+    }
     __syncthreads();
-    if (threadIdx.x == 0) *out = c;
+    if (threadIdx.x == 0) { out[0] = c; out[1] = o; }
 }
 
 }  // namespace hw

@@ -35,6 +35,7 @@ inline void natural_spline_c(const double* x, const double* yv, int n, double* c
 
 struct Solver1D {
     int prec = 0, nreal = 0, NT = 0, maxn = 0, any_brk = 0;
+    long round_limit = 0;   // rounds per integrate call before HW_ERR_STEP (0: no limit)
     hipStream_t stream = nullptr;
     std::vector<Real1D> hreal;
     long long ntot = 0;
@@ -76,9 +77,17 @@ template <typename T> struct Solver1DT : Solver1D {
     DevBuf<C> tw[16];
     Batch1DArgs<T> args{};
     int* h_count = nullptr;
+    // timing of one right-hand side launch of the whole batch: an event pair
+    // around the third stage of every round in which all realizations are
+    // active
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double launch_ms = 0.0;
+    long launches_timed = 0, rounds_total = 0;
 
     ~Solver1DT() override {
         if (h_count) (void)hipHostFree(h_count);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
     }
 
     static int ilog2(int n) { int l = 0; while ((1 << l) < n) ++l; return l; }
@@ -154,7 +163,9 @@ template <typename T> struct Solver1DT : Solver1D {
         }
         NT = nmin / 2 < 1024 ? nmin / 2 : 1024;
         if (maxn > 16 * NT) {
-            set_error("sizes %d..%d cannot share one launch (largest / smallest must be <= 32)", nmin, maxn);
+            set_error("sizes %d..%d cannot share one launch (the largest may be at most 16 x %d = %d points: "
+                      "16 elements per thread of the %d-thread blocks the smallest size sets)",
+                      nmin, maxn, NT, 16 * NT, NT);
             return HW_ERR_ARG;
         }
         HW_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -235,7 +246,7 @@ template <typename T> struct Solver1DT : Solver1D {
         std::vector<double> hw(4 * (size_t)n, 0.0);
         for (int r = 0; r < n; ++r) hw[4 * r] = cfg[r].t_wall0;
         if ((rc = d_real.upload(hreal, stream)) || (rc = d_rk.upload(hrk, stream)) || (rc = d_list.upload(hl, stream)) ||
-            (rc = d_count.alloc(1)) || (rc = y.alloc(2 * ntot)) || (rc = ext.alloc(2 * (size_t)maxn)) ||
+            (rc = d_count.alloc(2)) || (rc = y.alloc(2 * ntot)) || (rc = ext.alloc(2 * (size_t)maxn)) ||
             (rc = sr.alloc(R1_NR * ntot)) || (rc = sc.alloc(C1_NC * ntot)) || (rc = B.alloc(ntot)) ||
             (rc = peaks.alloc(ntot)) || (rc = ide.alloc(cbtot)) || (rc = cb.alloc(cbtot)) || (rc = cbn.alloc(cbtot)) ||
             (rc = cbp.alloc(cbtot)) || (rc = bs.upload(hb, stream)) || (rc = wst.upload(hw, stream)))
@@ -248,7 +259,9 @@ template <typename T> struct Solver1DT : Solver1D {
         HW_TRY(hipMemsetAsync(cbn.p, 0, sizeof(Cb1) * cbtot, stream));
         HW_TRY(hipMemsetAsync(cbp.p, 0, sizeof(Cb1) * cbtot, stream));
         HW_TRY(hipMemsetAsync(ide.p, 0, sizeof(int) * cbtot, stream));
-        HW_TRY(hipHostMalloc((void**)&h_count, sizeof(int), hipHostMallocDefault));
+        HW_TRY(hipHostMalloc((void**)&h_count, 2 * sizeof(int), hipHostMallocDefault));
+        HW_TRY(hipEventCreate(&ev0));
+        HW_TRY(hipEventCreate(&ev1));
         HW_TRY(K1D<T>::set_lds(NT, sizeof(C) * maxn));
         // ---- kernel arguments
         Batch1DArgs<T>& a = args;
@@ -363,15 +376,34 @@ template <typename T> struct Solver1DT : Solver1D {
         k1d_begin<<<(nreal + 255) / 256, 256, 0, stream>>>(d_rk.p, nreal, t1);
         HW_LAUNCH_CHECK();
         const size_t lds = sizeof(C) * maxn;
+        int active_prev = nreal;
+        if (round_limit > 0 && (max_rounds <= 0 || round_limit < max_rounds)) max_rounds = round_limit;
         for (long round = 0; max_rounds <= 0 || round < max_rounds; ++round) {
-            for (int s = 1; s <= 6; ++s) HW_TRY(K1D<T>::launch(NT, dim3(nreal), lds, stream, args, s));
+            // time the launches of rounds in which every realization is active
+            // (the first rounds of an interval), so one timed launch is the
+            // whole batch's right-hand side
+            const bool timed = active_prev == nreal;
+            rounds_total++;
+            for (int s = 1; s <= 6; ++s) {
+                if (timed && s == 3) HW_TRY(hipEventRecord(ev0, stream));
+                HW_TRY(K1D<T>::launch(NT, dim3(nreal), lds, stream, args, s));
+                if (timed && s == 3) HW_TRY(hipEventRecord(ev1, stream));
+            }
             if (any_brk) HW_TRY(K1D<T>::launch(NT, dim3(nreal), lds, stream, args, 7));
             HW_TRY(K1D<T>::control(NT, dim3(nreal), stream, args));
-            k1d_count<<<1, 256, 0, stream>>>(d_rk.p, nreal, d_count.p);
+            k1d_count<<<1, 256, 0, stream>>>(d_rk.p, bs.p, nreal, d_count.p);
             HW_LAUNCH_CHECK();
-            HW_TRY(hipMemcpyAsync(h_count, d_count.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+            HW_TRY(hipMemcpyAsync(h_count, d_count.p, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
             HW_TRY(hipStreamSynchronize(stream));
-            if (*h_count == 0) return check_overflow();
+            if (timed) {
+                float ms = 0.f;
+                HW_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+                launch_ms += ms;
+                launches_timed += 1;
+            }
+            if (h_count[1] > 0) return check_overflow();
+            if (h_count[0] == 0) return check_overflow();
+            active_prev = h_count[0];
         }
         set_error("integrate: %ld rounds without reaching t1 = %g", max_rounds, t1);
         return HW_ERR_STEP;
@@ -418,6 +450,10 @@ template <typename T> struct Solver1DT : Solver1D {
             st->rhs_evals += h[q].rhs;
             st->steps_accepted += h[q].accepted;
             st->steps_rejected += h[q].rejected;
+        }
+        if (r < 0) {          // per launch, all realizations of the batch together
+            st->rhs_ms_total = launch_ms;
+            st->rhs_timed = launches_timed;
         }
         return HW_OK;
     }

@@ -109,6 +109,11 @@ class Hawassi1D:
         _lib.check(self.lib.hw1d_get_time(self.ctx, r, ct.byref(t), ct.byref(h)), "hw1d_get_time")
         return t.value, h.value
 
+    def set_round_limit(self, max_rounds: int):
+        """Bound the rounds one integrate call may take (0: none); past it the
+        call raises HwError (HW_ERR_STEP) with the realizations where they got."""
+        _lib.check(self.lib.hw1d_set_round_limit(self.ctx, int(max_rounds)), "hw1d_set_round_limit")
+
     def integrate(self, t1: float):
         _lib.check(self.lib.hw1d_integrate(self.ctx, float(t1)), "hw1d_integrate")
 
@@ -121,4 +126,5 @@ class Hawassi1D:
     def stats(self, r: int = -1) -> dict:
         s = _lib.HwStats()
         _lib.check(self.lib.hw1d_get_stats(self.ctx, r, ct.byref(s)), "hw1d_get_stats")
-        return dict(rhs_evals=s.rhs_evals, steps_accepted=s.steps_accepted, steps_rejected=s.steps_rejected)
+        return dict(rhs_evals=s.rhs_evals, steps_accepted=s.steps_accepted, steps_rejected=s.steps_rejected,
+                    rhs_ms_total=s.rhs_ms_total, rhs_timed=s.rhs_timed)

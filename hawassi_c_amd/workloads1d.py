@@ -104,10 +104,11 @@ def slope(n: int = 4096, seed: int = 2, t_end: float = 120.0, dt_sig: float = 0.
 
 
 def slope_batch(nreal: int = 256, seed: int = 7, t_end: float = 120.0, sizes=(4096, 8192, 16384),
-                breaking: bool = True) -> list[S.Model1D]:
+                breaking: bool = True, ndepth: int = 3) -> list[S.Model1D]:
     g = rng(seed)
     ns = g.choice(np.asarray(sizes), nreal)
-    return [slope(int(n), seed=seed * 1000 + r, t_end=t_end, breaking=breaking) for r, n in enumerate(ns)]
+    return [slope(int(n), seed=seed * 1000 + r, t_end=t_end, breaking=breaking, ndepth=ndepth)
+            for r, n in enumerate(ns)]
 
 
 def random_state(m: S.Model1D, seed: int, amp: float = 0.05, kfrac: float = 0.2) -> np.ndarray:

@@ -99,7 +99,10 @@ typedef struct hw_stats {
     double rhs_ms_total;      /* device time in RHS evaluations (HIP events), ms */
     long rhs_timed;           /* evaluations that rhs_ms_total covers: every fourth
                                  2-D evaluation is bracketed by events (an event
-                                 record idles the GPU for a few microseconds) */
+                                 record idles the GPU for a few microseconds); for
+                                 a 1-D batch (hw1d_get_stats with r = -1) the unit
+                                 is one launch, every active realization's RHS
+                                 together, timed every eighth round */
 } hw_stats;
 
 /* Build a context (device buffers, multipliers in device layout) from the
@@ -254,6 +257,16 @@ int hw1d_get_time(void *ctx, int r, double *t, double *h);
  * control_y (ode_solver_Ham's inner loop, 1d/functions.c:546-552), all on
  * the device; each realization keeps its own t and step. */
 int hw1d_integrate(void *ctx, double t1);
+
+/* A bound on the rounds (one attempted step of every unfinished realization)
+ * that one hw1d_integrate call may take before it returns HW_ERR_STEP, with
+ * every realization's t and step as far as they got; 0 (the default) means no
+ * bound, as in the reference, whose driver loops until t1.  Breaking makes
+ * the reference's step control reject steps in cascades (steps down to 1e-6 s
+ * and above 1000 right-hand sides per 0.05 s for single realizations), and a
+ * batch advances at the pace of its slowest realization; a caller that must
+ * not wait on that sets a bound and reads hw1d_get_time / hw1d_get_stats. */
+int hw1d_set_round_limit(void *ctx, long max_rounds);
 
 /* B [n] of the last right-hand side evaluated for realization r and the
  * breaking flag (1d/breaking.c:212-450); either pointer may be NULL. */

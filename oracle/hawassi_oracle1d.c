@@ -116,13 +116,40 @@ typedef struct {
 
 static double *cz(size_t n) { return (double *)calloc(2 * n, sizeof(double)); }
 
+/* FP32 emulation of the transforms (a diagnostic for the tests and studies
+ * of the FP32 device path, off by default): every transform's input is
+ * rounded to float and the radix-2 butterflies run in float arithmetic with
+ * float twiddles, the output is widened back to double. */
+static int orc1d_fp32 = 0;
+void orc1d_set_fp32_emulation(int on) { orc1d_fp32 = on; }
+
+static void fft1_f32(const fftplan *p, double *x, int sign)
+{
+    int n = p->n;
+    float *a = (float *)malloc(sizeof(float) * 2 * n);
+    for (int i = 0; i < n; i++) { int r = p->rev[i]; a[2 * r] = (float)x[2 * i]; a[2 * r + 1] = (float)x[2 * i + 1]; }
+    for (int len = 2; len <= n; len <<= 1) {
+        int half = len >> 1, step = n / len;
+        for (int i = 0; i < n; i += len)
+            for (int j = 0; j < half; j++) {
+                float wr = (float)p->tw[2 * j * step], wi = (float)p->tw[2 * j * step + 1] * (sign > 0 ? -1.0f : 1.0f);
+                float *u = a + 2 * (i + j), *v = a + 2 * (i + j + half);
+                float tr = v[0] * wr - v[1] * wi, ti = v[0] * wi + v[1] * wr;
+                v[0] = u[0] - tr; v[1] = u[1] - ti;
+                u[0] = u[0] + tr; u[1] = u[1] + ti;
+            }
+    }
+    for (int i = 0; i < 2 * n; i++) x[i] = (double)a[i];
+    free(a);
+}
+
 /* ifft_complex_real_nx: Re(backward DFT)/N */
 static void ifft_re(const orc1d_ctx *c, const double *Xh, double *out)
 {
     int N = c->p.N;
     double *a = (double *)malloc(sizeof(double) * 2 * N);
     memcpy(a, Xh, sizeof(double) * 2 * N);
-    fft1(c->plan, a, +1);
+    if (orc1d_fp32) fft1_f32(c->plan, a, +1); else fft1(c->plan, a, +1);
     for (int i = 0; i < N; i++) out[i] = a[2 * i] / (double)N;
     free(a);
 }
@@ -132,7 +159,7 @@ static void fft_re(const orc1d_ctx *c, const double *in, double *Xh)
 {
     int N = c->p.N;
     for (int i = 0; i < N; i++) { Xh[2 * i] = in[i]; Xh[2 * i + 1] = 0.0; }
-    fft1(c->plan, Xh, -1);
+    if (orc1d_fp32) fft1_f32(c->plan, Xh, -1); else fft1(c->plan, Xh, -1);
 }
 
 /* HSS (1d/functions.c:2861-2977): two reference depths, plus the middle

@@ -114,6 +114,7 @@ def lib():
         L.orc1d_fft.argtypes = [ct.c_int, _dp, ct.c_int]
         L.orc_set_threads.argtypes = [ct.c_int]
         L.orc_set_plant.argtypes = [ct.c_int]
+        L.orc1d_set_fp32_emulation.argtypes = [ct.c_int]
         L.orc_hss.argtypes = [ct.c_void_p, _dp, ct.c_int, _dp]
         L.orc_break_diag.argtypes = [ct.c_void_p, _dp, _dp, _dp, _dp]
         L.orc_find_peaks.argtypes = [_dp, ct.c_int, ct.c_int, ct.c_double, ct.c_void_p, ct.c_void_p]
@@ -391,6 +392,12 @@ def ifft_c2r(spec: np.ndarray) -> np.ndarray:
     out = np.empty((ny, nx))
     lib().orc_ifft_c2r(nx, ny, _p(re), _p(im), _p(out))
     return out
+
+
+def set_fp32_emulation_1d(on: bool):
+    """Run the 1-D oracle's transforms in float arithmetic (a diagnostic of the
+    FP32 device path's rounding; off by default)."""
+    lib().orc1d_set_fp32_emulation(int(bool(on)))
 
 
 def set_plant(code: int):
