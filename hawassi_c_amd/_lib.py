@@ -61,6 +61,12 @@ class HwStats(ct.Structure):
                 ("rhs_ms_total", ct.c_double), ("rhs_timed", ct.c_long)]
 
 
+class HwFnoConfig(ct.Structure):
+    _fields_ = [("n", ct.c_int), ("width", ct.c_int), ("modes", ct.c_int), ("layers", ct.c_int),
+                ("hidden", ct.c_int), ("in_ch", ct.c_int), ("out_ch", ct.c_int), ("max_batch", ct.c_int),
+                ("seed", ct.c_ulonglong)]
+
+
 EXPORTS = [
     "hw_ctx_create_2d", "hw_ctx_destroy", "hw_rhs", "hw_rhs_device", "hw_set_state", "hw_get_state",
     "hw_evolve_apply", "hw_integrate", "hw_integrate_batch", "hw_get_fields", "hw_get_stats", "hw_reset_stats", "hw_sync",
@@ -69,6 +75,8 @@ EXPORTS = [
     "hw1d_create", "hw1d_destroy", "hw1d_rhs", "hw1d_rhs_at", "hw1d_set_state", "hw1d_get_state",
     "hw1d_set_time", "hw1d_get_time", "hw1d_integrate", "hw1d_set_round_limit", "hw1d_get_breaking",
     "hw1d_get_stats",
+    "hw_fno_create", "hw_fno_destroy", "hw_fno_param_count", "hw_fno_get_params", "hw_fno_set_params",
+    "hw_fno_forward", "hw_fno_loss_grad", "hw_fno_set_dataset", "hw_fno_train_step", "hw_fno_gemm_check",
 ]
 
 _lib = None
@@ -121,6 +129,19 @@ def load(path: Path | None = None) -> ct.CDLL:
     lib.hw1d_set_round_limit.argtypes = [vp, ct.c_long]
     lib.hw1d_get_breaking.argtypes = [vp, ct.c_int, _dp, ct.POINTER(ct.c_int)]
     lib.hw1d_get_stats.argtypes = [vp, ct.c_int, ct.POINTER(HwStats)]
+    _fp = ct.POINTER(ct.c_float)
+    lib.hw_fno_create.argtypes = [ct.POINTER(HwFnoConfig), ct.POINTER(vp)]
+    lib.hw_fno_destroy.argtypes = [vp]
+    lib.hw_fno_destroy.restype = None
+    lib.hw_fno_param_count.argtypes = [vp]
+    lib.hw_fno_param_count.restype = ct.c_longlong
+    lib.hw_fno_get_params.argtypes = [vp, _fp]
+    lib.hw_fno_set_params.argtypes = [vp, _fp]
+    lib.hw_fno_forward.argtypes = [vp, ct.c_int, _fp, _fp]
+    lib.hw_fno_loss_grad.argtypes = [vp, ct.c_int, _fp, _fp, _dp, _fp]
+    lib.hw_fno_set_dataset.argtypes = [vp, ct.c_int, _fp, _fp]
+    lib.hw_fno_train_step.argtypes = [vp, ct.c_int, ct.POINTER(ct.c_int), ct.c_double, _dp]
+    lib.hw_fno_gemm_check.argtypes = [ct.c_int, ct.c_int, ct.c_int, _fp, _fp, _fp]
     lib.hw_last_error.restype = ct.c_char_p
     lib.hw_version.restype = ct.c_char_p
     if path is None:

@@ -39,7 +39,7 @@ NT1D = (64, 128, 256, 512, 1024)
 def _units():
     gen = BUILD / "gen"
     gen.mkdir(parents=True, exist_ok=True)
-    units = [CSRC / "capi.hip", CSRC / "capi1d.hip"]
+    units = [CSRC / "capi.hip", CSRC / "capi1d.hip", CSRC / "fno.hip"]
     for prec in (0, 1):
         t = "double" if prec == 0 else "float"
         for nt in NT1D:

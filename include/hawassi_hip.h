@@ -275,6 +275,47 @@ int hw1d_get_breaking(void *ctx, int r, double *B, int *flag);
 /* Counters of realization r, or summed over the batch for r = -1. */
 int hw1d_get_stats(void *ctx, int r, hw_stats *st);
 
+/* --------------------------------------------- learned coarse propagator --
+ * A spectral neural operator (Fourier neural operator) for the coarse
+ * propagator of the parareal driver.  No counterpart in the reference: the
+ * reference integrates serially with the fine solver only (ode_solver_uv,
+ * 2d/fun_odesolver.c:2565-2872).  Inputs are [in_ch][n][n] FP32 fields per
+ * sample (normalised [eta, phi, depth]), outputs [out_ch][n][n] (the
+ * normalised increment of [eta, phi] over one coarse step).  Every matrix
+ * product of the forward and backward passes runs on the matrix cores (BF16
+ * operands, FP32 accumulation); parameters, gradients and Adam moments are
+ * FP32 on the device. */
+typedef struct hw_fno_config {
+    int n;                    /* grid points per side (square grid) */
+    int width;                /* channels of the Fourier layers */
+    int modes;                /* retained modes: kx < modes, ky < modes or ky >= n - modes */
+    int layers;               /* Fourier layers */
+    int hidden;               /* hidden width of the projection */
+    int in_ch, out_ch;        /* input / output fields per sample */
+    int max_batch;            /* largest batch of one call */
+    unsigned long long seed;  /* parameter initialisation */
+} hw_fno_config;
+
+int hw_fno_create(const hw_fno_config *cfg, void **ctx);
+void hw_fno_destroy(void *ctx);
+/* number of FP32 parameters, in the order: lift W [width][in_ch], b [width];
+ * per layer W [width][width], b [width], spectral weights
+ * [2 (re, im)][width (in)][width (out)][2 modes][modes]; projection
+ * W1 [hidden][width], b1 [hidden], W2 [out_ch][hidden], b2 [out_ch] */
+long long hw_fno_param_count(void *ctx);
+int hw_fno_get_params(void *ctx, float *out);
+int hw_fno_set_params(void *ctx, const float *in);       /* also resets the Adam state */
+/* y [batch][out_ch][n][n] = net(x [batch][in_ch][n][n]), host arrays */
+int hw_fno_forward(void *ctx, int batch, const float *x, float *y);
+/* mean squared error against target and its gradient (grad may be NULL) */
+int hw_fno_loss_grad(void *ctx, int batch, const float *x, const float *target, double *loss, float *grad);
+/* the training set, kept on the device: x [count][in_ch][n][n], y [count][out_ch][n][n] */
+int hw_fno_set_dataset(void *ctx, int count, const float *x, const float *y);
+/* one Adam step (beta 0.9 / 0.999) on the samples idx[0..batch) of the training set */
+int hw_fno_train_step(void *ctx, int batch, const int *idx, double lr, double *loss);
+/* C [M][N] = A [M][K] B [K][N] (row-major host arrays) through the BF16-MFMA GEMM */
+int hw_fno_gemm_check(int M, int N, int K, const float *A, const float *B, float *C);
+
 /* Sizes this build supports (1 if (nx, ny, precision) has compiled kernels). */
 int hw_supported(int nx, int ny, int precision);
 
