@@ -71,11 +71,17 @@ constexpr int LDK = KT + 8;     // LDS row length in bf16 (padding against bank 
 // one 32 x 32 output tile.  MFMA lane layout (v_mfma_f32_32x32x16_bf16):
 // lane l holds A[l % 32][8 (l/32) + j] and B[8 (l/32) + j][l % 32], j < 8, and
 // result r < 16 is D[8 (r/4) + 4 (l/32) + r % 4][l % 32].
-template <int BM, int BN>
+//
+// X3 (split precision): every operand is split as v = hi + lo with hi =
+// bf16(v), lo = bf16(v - hi), and the tile product is hi*hi + hi*lo + lo*hi,
+// three MFMA per K step; operands then carry 16 mantissa bits and the product
+// is good to about 1e-5 relative, where plain BF16 is good to about 4e-3.
+template <int BM, int BN, bool X3>
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
     static_assert((BM / 32) * (BN / 32) == 4, "four waves");
-    __shared__ __attribute__((aligned(16))) __bf16 As[BM * LDK];
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[BN * LDK];
+    constexpr int NP = X3 ? 2 : 1;   // hi (and lo) planes
+    __shared__ __attribute__((aligned(16))) __bf16 As[NP][BM * LDK];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[NP][BN * LDK];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave / (BN / 32), wn = wave % (BN / 32);
     const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
@@ -95,7 +101,9 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
             else { mm = idx % BM; kk = idx / BM; }
             const int m = m0 + mm, k = k0 + kk;
             float v = (m < g.M && k < k_end) ? A[(long long)m * g.a_rs + (long long)k * g.a_cs] : 0.f;
-            As[mm * LDK + kk] = (__bf16)v;
+            const __bf16 hi = (__bf16)v;
+            As[0][mm * LDK + kk] = hi;
+            if constexpr (X3) As[NP - 1][mm * LDK + kk] = (__bf16)(v - (float)hi);
         }
         // B tile KT x BN -> Bs[n][k]
         for (int idx = tid; idx < BN * KT; idx += 256) {
@@ -104,13 +112,23 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
             else { nn = idx % BN; kk = idx / BN; }
             const int n = n0 + nn, k = k0 + kk;
             float v = (n < g.N && k < k_end) ? B[(long long)k * g.b_rs + (long long)n * g.b_cs] : 0.f;
-            Bs[nn * LDK + kk] = (__bf16)v;
+            const __bf16 hi = (__bf16)v;
+            Bs[0][nn * LDK + kk] = hi;
+            if constexpr (X3) Bs[NP - 1][nn * LDK + kk] = (__bf16)(v - (float)hi);
         }
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < KT; kk += 16) {
-            const bf16x8 av = *reinterpret_cast<const bf16x8*>(&As[(wm * 32 + (lane & 31)) * LDK + kk + 8 * (lane >> 5)]);
-            const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&Bs[(wn * 32 + (lane & 31)) * LDK + kk + 8 * (lane >> 5)]);
+            const int ao = (wm * 32 + (lane & 31)) * LDK + kk + 8 * (lane >> 5);
+            const int bo = (wn * 32 + (lane & 31)) * LDK + kk + 8 * (lane >> 5);
+            const bf16x8 av = *reinterpret_cast<const bf16x8*>(&As[0][ao]);
+            const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&Bs[0][bo]);
+            if constexpr (X3) {
+                const bf16x8 al = *reinterpret_cast<const bf16x8*>(&As[NP - 1][ao]);
+                const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&Bs[NP - 1][bo]);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bv, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bl, acc, 0, 0, 0);
+            }
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
         }
         __syncthreads();
@@ -307,6 +325,7 @@ struct Net {
     std::vector<long long> o_w, o_b, o_r;
     float *prm = nullptr, *grd = nullptr, *am = nullptr, *av = nullptr;
     long long adam_t = 0;
+    bool split_precision = false;   // products in split BF16 (HW_FNO_SPLIT) rather than BF16
     // DFT matrices
     float *Fx = nullptr, *A1 = nullptr, *A2 = nullptr, *A3 = nullptr, *A4 = nullptr, *Hx = nullptr;
     // activations (max batch)
@@ -333,6 +352,17 @@ struct Net {
     }
 
     // ---------------------------------------------------------------- gemm
+    template <bool X3>
+    void launch_gemm(const GemmArgs& g, int M, int N, unsigned zs) {
+        if (M <= 32) {
+            hipLaunchKernelGGL((k_gemm<32, 128, X3>), dim3((N + 127) / 128, (M + 31) / 32, zs), dim3(256), 0, stream, g);
+        } else if (N <= 32) {
+            hipLaunchKernelGGL((k_gemm<128, 32, X3>), dim3((N + 31) / 32, (M + 127) / 128, zs), dim3(256), 0, stream, g);
+        } else {
+            hipLaunchKernelGGL((k_gemm<64, 64, X3>), dim3((N + 63) / 64, (M + 63) / 64, zs), dim3(256), 0, stream, g);
+        }
+    }
+
     int gemm(int M, int N, int K, int batch, const float* A, long long a_rs, long long a_cs, long long a_bs,
              const float* B, long long b_rs, long long b_cs, long long b_bs, float* C, long long c_rs, long long c_cs,
              long long c_bs, float alpha, float beta, int splits = 1, bool sum_batch = false) {
@@ -355,13 +385,8 @@ struct Net {
             g.W = ws;
         }
         const unsigned zs = (unsigned)(batch * splits);
-        if (M <= 32) {
-            hipLaunchKernelGGL((k_gemm<32, 128>), dim3((N + 127) / 128, (M + 31) / 32, zs), dim3(256), 0, stream, g);
-        } else if (N <= 32) {
-            hipLaunchKernelGGL((k_gemm<128, 32>), dim3((N + 31) / 32, (M + 127) / 128, zs), dim3(256), 0, stream, g);
-        } else {
-            hipLaunchKernelGGL((k_gemm<64, 64>), dim3((N + 63) / 64, (M + 63) / 64, zs), dim3(256), 0, stream, g);
-        }
+        if (split_precision) launch_gemm<true>(g, M, N, zs);
+        else launch_gemm<false>(g, M, N, zs);
         FNO_CHECK();
         if (g.to_ws) {
             const long long outs = (long long)M * N * (sum_batch ? 1 : batch);
@@ -752,11 +777,24 @@ int hw_fno_train_step(void* ctx, int batch, const int* idx, double lr, double* l
     return HW_OK;
 }
 
-int hw_fno_gemm_check(int M, int N, int K, const float* A, const float* B, float* C) {
-    // one plain product C[M][N] = A[M][K] B[K][N] through the BF16-MFMA GEMM
+int hw_fno_set_precision(void* ctx, int precision) {
+    if (!ctx || (precision != HW_FNO_BF16 && precision != HW_FNO_SPLIT)) {
+        hw::set_error("hw_fno_set_precision: bad argument");
+        return HW_ERR_ARG;
+    }
+    static_cast<Net*>(ctx)->split_precision = precision == HW_FNO_SPLIT;
+    return HW_OK;
+}
+
+int hw_fno_gemm_check(int M, int N, int K, const float* A, const float* B, float* C, int precision) {
+    // one plain product C[M][N] = A[M][K] B[K][N] through the MFMA GEMM
     // (row-major host arrays), for the kernel's own tests
-    if (!A || !B || !C || M < 1 || N < 1 || K < 1) { hw::set_error("bad argument"); return HW_ERR_ARG; }
+    if (!A || !B || !C || M < 1 || N < 1 || K < 1 || (precision != HW_FNO_BF16 && precision != HW_FNO_SPLIT)) {
+        hw::set_error("bad argument");
+        return HW_ERR_ARG;
+    }
     Net net;
+    net.split_precision = precision == HW_FNO_SPLIT;
     FNO_TRY(hipStreamCreateWithFlags(&net.stream, hipStreamNonBlocking));
     float *dA, *dB, *dC;
     int rc;

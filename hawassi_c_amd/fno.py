@@ -45,6 +45,12 @@ class FNO:
         except Exception:
             pass
 
+    def set_precision(self, split: bool):
+        """Products of later calls in split BF16 (three MFMA, ~1e-5 relative)
+        when `split`, else in BF16 (~4e-3 relative, the training default)."""
+        _lib.check(self.lib.hw_fno_set_precision(self.ctx, _lib.HW_FNO_SPLIT if split else _lib.HW_FNO_BF16),
+                   "hw_fno_set_precision")
+
     def params(self) -> np.ndarray:
         p = np.empty(self.nparam, dtype=np.float32)
         _lib.check(self.lib.hw_fno_get_params(self.ctx, _f(p)), "hw_fno_get_params")
@@ -89,13 +95,14 @@ class FNO:
         return loss.value
 
 
-def gemm_check(a: np.ndarray, b: np.ndarray) -> np.ndarray:
-    """C = A B through the device's BF16-MFMA GEMM (for its tests)."""
+def gemm_check(a: np.ndarray, b: np.ndarray, split: bool = False) -> np.ndarray:
+    """C = A B through the device's MFMA GEMM, BF16 or split BF16 (for its tests)."""
     lib = _lib.load()
     a = np.ascontiguousarray(a, dtype=np.float32)
     b = np.ascontiguousarray(b, dtype=np.float32)
     M, K = a.shape
     N = b.shape[1]
     c = np.empty((M, N), dtype=np.float32)
-    _lib.check(lib.hw_fno_gemm_check(M, N, K, _f(a), _f(b), _f(c)), "hw_fno_gemm_check")
+    _lib.check(lib.hw_fno_gemm_check(M, N, K, _f(a), _f(b), _f(c), _lib.HW_FNO_SPLIT if split else _lib.HW_FNO_BF16),
+               "hw_fno_gemm_check")
     return c

@@ -8,6 +8,13 @@
   coarser grid for tests.
 * ``short_crested(n)`` -- flat 30 m, 2 km x 2 km, absorbing strips of 200 m,
   directional JONSWAP line influx (Hs 3 m, Tp 10 s, s 10) at x = 250 m.
+* ``shoal(n)`` + ``shoal_sea(model, seed)`` -- the learned-parareal case: a
+  periodic 100 m x 100 m basin, depth 1.0 m over a Gaussian shoal rising to
+  0.4 m, two-reference-depth operators, no influx, absorbing strips or
+  adjustment zone, breaking off (so the dynamics is autonomous and keeps
+  u = grad phi); the initial condition is a seeded directional JONSWAP sea
+  (Hs 0.08 m, Tp 2 s, cos^2s spreading s = 10) in linear wave theory.
+  ``shoal(n, shoal_depth=depth0)`` is the flat basin (flat-bottom operators).
 
 Influx: the line signal eta_line(y_j, t) is a seeded random-phase sum of
 directional JONSWAP components, used as the source signal with the
@@ -170,6 +177,67 @@ def short_crested(n: int = 1024, seed: int = 3, t_end: float = 60.0, dt_sig: flo
                      source_on=True, vertical=True, propagation=2, new_gam=ng, skew_gam=sg, t_sig=t_sig, sig=sig,
                      skew_sig=np.zeros_like(sig), eps_abs=eps_abs, eps_rel=eps_rel,
                      meta=dict(name=f"short_crested{n}", tp=tp, hs=hs, kp=kp, lam=lam, line_x=line_x))
+
+
+def shoal(n: int = 512, length: float = 100.0, depth0: float = 1.0, shoal_depth: float = 0.4,
+          shoal_width: float = 15.0, tp: float = 2.0, eps_rel: float = 1e-2, eps_abs: float = 1e-3) -> S.Model2D:
+    """Periodic basin with a Gaussian shoal; see the module docstring."""
+    x = S.interval_to_vector(0.0, length, n)
+    y = S.interval_to_vector(0.0, length, n)
+    kx = S.freqspace(x[-1] - x[0], n)
+    ky = S.freqspace(y[-1] - y[0], n)
+    X, Y = np.meshgrid(x, y)
+    r2 = (X - 0.5 * length) ** 2 + (Y - 0.5 * length) ** 2
+    depth = depth0 - (depth0 - shoal_depth) * np.exp(-r2 / (2 * shoal_width ** 2))
+    one = np.ones((n, n))
+    wp = 2 * math.pi / tp
+    kp = S.invers_omega(wp, depth0)
+    if shoal_depth == depth0:
+        # no shoal: the flat-bottom operators
+        om2d = S.exact_disp2d(kx[None, :], ky[:, None], depth0)
+        return S.Model2D(nx=n, ny=n, x=x, y=y, kx=kx, ky=ky, flat=True, cpeak=float(S.exact_cp1d(kp, depth0, 0.0)),
+                         aal=S.aal2d(4, kx, ky), fbdy=0.0 * one, wallchar=one, chi=one.copy(),
+                         depth=np.full((n, n), depth0), wall=False, c2d=S.exact_cp2d(kx[None, :], ky[:, None], depth0),
+                         l2d=om2d * om2d / S.GRAV, source_on=False, eps_abs=eps_abs, eps_rel=eps_rel, breaking=None,
+                         meta=dict(name=f"flatbasin{n}", tp=tp, kp=kp, lam=2 * math.pi / kp, length=length))
+    ops = S.disp_interpolation_2p(depth, kx, ky, wp)
+    return S.Model2D(nx=n, ny=n, x=x, y=y, kx=kx, ky=ky, flat=False, cpeak=float(S.exact_cp1d(kp, depth0, 0.0)),
+                     aal=S.aal2d(4, kx, ky), fbdy=0.0 * one, wallchar=one, chi=one.copy(), depth=depth, wall=False,
+                     ops=ops, source_on=False, eps_abs=eps_abs, eps_rel=eps_rel, breaking=None,
+                     meta=dict(name=f"shoal{n}", tp=tp, kp=kp, lam=2 * math.pi / kp, length=length))
+
+
+def shoal_sea(model: S.Model2D, seed: int, hs: float = 0.08, tp: float = 2.0, gamma: float = 3.3,
+              s_spread: float = 10.0, kmax_modes: int = 30, theta0: float = 0.0):
+    """Seeded directional JONSWAP initial sea on the basin's Fourier modes
+    (|k| below kmax_modes fundamental wavenumbers, mean direction theta0
+    from +x, cos^(2s) spreading), linear theory at the far-field depth: eta
+    and the surface potential phi = (g / omega) eta shifted by a quarter
+    period.  Returns (eta, phi) [ny, nx]."""
+    n = model.nx
+    L = model.x[-1] - model.x[0]
+    d0 = float(model.depth.max())
+    g = rng(seed)
+    k1 = 2 * math.pi / L
+    kxv = np.fft.fftfreq(n) * n * k1
+    kyv = np.fft.fftfreq(n) * n * k1
+    KX, KY = kxv[None, :], kyv[:, None]
+    K = np.hypot(KX, KY)
+    th = np.angle(np.exp(1j * (np.arctan2(KY, KX) - theta0)))
+    om = np.sqrt(S.GRAV * K * np.tanh(K * d0))
+    wp = 2 * math.pi / tp
+    with np.errstate(divide="ignore", invalid="ignore"):
+        sig = np.where(om <= wp, 0.07, 0.09)
+        spec = np.where(K > 0, om ** -5 * np.exp(-1.25 * (wp / om) ** 4)
+                        * gamma ** np.exp(-((om - wp) ** 2) / (2 * sig ** 2 * wp ** 2)), 0.0)
+    spread = np.where(np.abs(th) < math.pi / 2, np.cos(th) ** (2 * s_spread), 0.0)
+    keep = (K > 0) & (K < kmax_modes * k1)
+    amp = np.sqrt(np.where(keep, spec * spread, 0.0))
+    a = amp * np.exp(1j * g.uniform(0, 2 * math.pi, (n, n)))
+    eta = np.real(np.fft.ifft2(a))
+    phi = np.real(np.fft.ifft2(-1j * np.where(keep, S.GRAV / np.where(om > 0, om, 1.0), 0.0) * a))
+    sc = hs / 4.0 / max(float(eta.std()), 1e-300)
+    return eta * sc, phi * sc
 
 
 def random_state(model: S.Model2D, seed: int, amp: float = 0.05, kfrac: float = 0.35) -> np.ndarray:

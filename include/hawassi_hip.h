@@ -313,8 +313,17 @@ int hw_fno_loss_grad(void *ctx, int batch, const float *x, const float *target, 
 int hw_fno_set_dataset(void *ctx, int count, const float *x, const float *y);
 /* one Adam step (beta 0.9 / 0.999) on the samples idx[0..batch) of the training set */
 int hw_fno_train_step(void *ctx, int batch, const int *idx, double lr, double *loss);
-/* C [M][N] = A [M][K] B [K][N] (row-major host arrays) through the BF16-MFMA GEMM */
-int hw_fno_gemm_check(int M, int N, int K, const float *A, const float *B, float *C);
+/* Precision of the matrix products of later calls: HW_FNO_BF16 (the
+ * default; BF16 operands, about 4e-3 relative) or HW_FNO_SPLIT (each operand
+ * split into two BF16 parts, three MFMA per product, about 1e-5 relative:
+ * inference whose output differences must stay smooth, as the coarse
+ * propagator of parareal needs). */
+#define HW_FNO_BF16 0
+#define HW_FNO_SPLIT 1
+int hw_fno_set_precision(void *ctx, int precision);
+/* C [M][N] = A [M][K] B [K][N] (row-major host arrays) through the MFMA GEMM
+ * at the given precision (HW_FNO_BF16 or HW_FNO_SPLIT) */
+int hw_fno_gemm_check(int M, int N, int K, const float *A, const float *B, float *C, int precision);
 
 /* Sizes this build supports (1 if (nx, ny, precision) has compiled kernels). */
 int hw_supported(int nx, int ny, int precision);

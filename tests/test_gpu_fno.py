@@ -29,8 +29,16 @@ def test_mfma_gemm_matches_numpy(M, N, K):
     # general data: BF16 rounding of both operands, ~2^-9 relative per operand
     a = g.standard_normal((M, K)).astype(np.float32)
     b = g.standard_normal((K, N)).astype(np.float32)
+    exact = a.astype(np.float64) @ b.astype(np.float64)
     c = gemm_check(a, b)
-    assert rel(c, a.astype(np.float64) @ b.astype(np.float64)) < 8e-3
+    assert rel(c, exact) < 8e-3
+    # split BF16 (hi + lo per operand, three products): 16 mantissa bits per
+    # operand, ~2^-16 relative per product, FP32 accumulation
+    cs = gemm_check(a, b, split=True)
+    assert rel(cs, exact) < 5e-5
+    assert rel(cs, exact) < 0.05 * rel(c, exact)
+    # integer data stays exact (lo parts are zero)
+    assert np.array_equal(gemm_check(ai, bi, split=True), (ai.astype(np.int64) @ bi.astype(np.int64)).astype(np.float32))
 
 
 def _pair(n=16, width=8, modes=4, layers=3, hidden=16, batch=2, seed=5):
@@ -59,6 +67,25 @@ def test_forward_and_gradients_match_the_oracle(n, width, modes, layers, hidden)
         k = int(np.prod(shp))
         e = rel(g[off:off + k], go[off:off + k])
         assert e < 2e-2, (name, e)
+
+
+def test_split_precision_forward_is_smooth():
+    """The split-BF16 forward follows the FP64 oracle to ~1e-5 and its
+    response to a small input change follows the oracle's, where the BF16
+    forward's response is dominated by rounding."""
+    dev, orc, x, t = _pair(32, 32, 8, 4, 64)
+    p = dev.params().astype(np.float64)
+    dx = 3e-3 * np.random.default_rng(9).standard_normal(x.shape).astype(np.float32)
+    yo, _ = orc.forward(p, x.astype(np.float64))
+    yo2, _ = orc.forward(p, (x + dx).astype(np.float64))
+    dev.set_precision(True)
+    ys, ys2 = dev.forward(x), dev.forward(x + dx)
+    dev.set_precision(False)
+    yb, yb2 = dev.forward(x), dev.forward(x + dx)
+    print("split", rel(ys, yo), rel(ys2 - ys, yo2 - yo), "bf16", rel(yb, yo), rel(yb2 - yb, yo2 - yo), flush=True)
+    assert rel(ys, yo) < 1e-4
+    assert rel(ys2 - ys, yo2 - yo) < 0.1
+    assert rel(yb2 - yb, yo2 - yo) > 0.5
 
 
 def test_adam_steps_lower_the_loss():
