@@ -78,7 +78,7 @@ EXPORTS = [
     "hw1d_set_time", "hw1d_get_time", "hw1d_integrate", "hw1d_set_round_limit", "hw1d_get_breaking",
     "hw1d_get_stats",
     "hw_fno_create", "hw_fno_destroy", "hw_fno_param_count", "hw_fno_get_params", "hw_fno_set_params",
-    "hw_fno_forward", "hw_fno_loss_grad", "hw_fno_set_dataset", "hw_fno_train_step", "hw_fno_gemm_check", "hw_fno_set_precision",
+    "hw_fno_forward", "hw_fno_loss_grad", "hw_fno_set_dataset", "hw_fno_train_step", "hw_fno_gemm_check", "hw_fno_set_precision", "hw_fno_gemm_flops", "hw_fno_gemm_bytes",
 ]
 
 _lib = None
@@ -145,6 +145,10 @@ def load(path: Path | None = None) -> ct.CDLL:
     lib.hw_fno_train_step.argtypes = [vp, ct.c_int, ct.POINTER(ct.c_int), ct.c_double, _dp]
     lib.hw_fno_gemm_check.argtypes = [ct.c_int, ct.c_int, ct.c_int, _fp, _fp, _fp, ct.c_int]
     lib.hw_fno_set_precision.argtypes = [vp, ct.c_int]
+    lib.hw_fno_gemm_flops.argtypes = [vp]
+    lib.hw_fno_gemm_flops.restype = ct.c_double
+    lib.hw_fno_gemm_bytes.argtypes = [vp]
+    lib.hw_fno_gemm_bytes.restype = ct.c_double
     lib.hw_last_error.restype = ct.c_char_p
     lib.hw_version.restype = ct.c_char_p
     if path is None:

@@ -326,6 +326,8 @@ struct Net {
     float *prm = nullptr, *grd = nullptr, *am = nullptr, *av = nullptr;
     long long adam_t = 0;
     bool split_precision = false;   // products in split BF16 (HW_FNO_SPLIT) rather than BF16
+    double gemm_flops = 0.0;        // 2 M N K per product launched (one MFMA product each, split or not)
+    double gemm_bytes = 0.0;        // FP32 operands read once, result written once (read too when beta != 0)
     // DFT matrices
     float *Fx = nullptr, *A1 = nullptr, *A2 = nullptr, *A3 = nullptr, *A4 = nullptr, *Hx = nullptr;
     // activations (max batch)
@@ -367,6 +369,8 @@ struct Net {
              const float* B, long long b_rs, long long b_cs, long long b_bs, float* C, long long c_rs, long long c_cs,
              long long c_bs, float alpha, float beta, int splits = 1, bool sum_batch = false) {
         GemmArgs g{};
+        gemm_flops += 2.0 * M * N * (double)K * batch;
+        gemm_bytes += 4.0 * batch * ((double)M * K + (double)K * N + (double)M * N * (beta != 0.f ? 2 : 1));
         g.M = M; g.N = N; g.K = K; g.batch = batch;
         g.A = A; g.a_rs = a_rs; g.a_cs = a_cs; g.a_bs = a_bs;
         g.B = B; g.b_rs = b_rs; g.b_cs = b_cs; g.b_bs = b_bs;
@@ -785,6 +789,9 @@ int hw_fno_set_precision(void* ctx, int precision) {
     static_cast<Net*>(ctx)->split_precision = precision == HW_FNO_SPLIT;
     return HW_OK;
 }
+
+double hw_fno_gemm_flops(void* ctx) { return ctx ? static_cast<Net*>(ctx)->gemm_flops : -1.0; }
+double hw_fno_gemm_bytes(void* ctx) { return ctx ? static_cast<Net*>(ctx)->gemm_bytes : -1.0; }
 
 int hw_fno_gemm_check(int M, int N, int K, const float* A, const float* B, float* C, int precision) {
     // one plain product C[M][N] = A[M][K] B[K][N] through the MFMA GEMM

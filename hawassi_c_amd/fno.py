@@ -32,6 +32,7 @@ class FNO:
         _lib.check(self.lib.hw_fno_create(ct.byref(c), ct.byref(h)), "hw_fno_create")
         self.ctx = h
         self.n, self.in_ch, self.out_ch, self.max_batch = n, in_ch, out_ch, max_batch
+        self.modes = modes
         self.nparam = int(self.lib.hw_fno_param_count(self.ctx))
 
     def close(self):
@@ -50,6 +51,14 @@ class FNO:
         when `split`, else in BF16 (~4e-3 relative, the training default)."""
         _lib.check(self.lib.hw_fno_set_precision(self.ctx, _lib.HW_FNO_SPLIT if split else _lib.HW_FNO_BF16),
                    "hw_fno_set_precision")
+
+    def gemm_flops(self) -> float:
+        """Floating-point operations of the matrix products launched so far."""
+        return float(self.lib.hw_fno_gemm_flops(self.ctx))
+
+    def gemm_bytes(self) -> float:
+        """Compulsory HBM bytes of the matrix products launched so far."""
+        return float(self.lib.hw_fno_gemm_bytes(self.ctx))
 
     def params(self) -> np.ndarray:
         p = np.empty(self.nparam, dtype=np.float32)
@@ -106,3 +115,37 @@ def gemm_check(a: np.ndarray, b: np.ndarray, split: bool = False) -> np.ndarray:
     _lib.check(lib.hw_fno_gemm_check(M, N, K, _f(a), _f(b), _f(c), _lib.HW_FNO_SPLIT if split else _lib.HW_FNO_BF16),
                "hw_fno_gemm_check")
     return c
+
+
+def _main():
+    """python -m hawassi_c_amd.fno [--n 512] [--steps 20]: Adam steps on
+    seeded random fields (batch 8), for profiling the training kernels;
+    prints the matrix-product flops and the steps' wall time."""
+    import argparse
+    import time
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--modes", type=int, default=32)
+    a = ap.parse_args()
+    g = np.random.default_rng(0)
+    x = g.standard_normal((a.batch, 3, a.n, a.n)).astype(np.float32)
+    y = g.standard_normal((a.batch, 2, a.n, a.n)).astype(np.float32)
+    net = FNO(a.n, 32, a.modes, 4, 128, max_batch=a.batch, seed=1)
+    net.set_dataset(x, y)
+    idx = np.arange(a.batch)
+    net.train_step(idx, 1e-3)
+    f0, b0 = net.gemm_flops(), net.gemm_bytes()
+    w0 = time.perf_counter()
+    for i in range(a.steps):
+        net.train_step(idx, 1e-3)
+    wall = time.perf_counter() - w0
+    fl, by = net.gemm_flops() - f0, net.gemm_bytes() - b0
+    print(f"fno n={a.n} batch={a.batch}: {a.steps} Adam steps in {wall:.3f} s, {fl / a.steps / 1e9:.2f} GFLOP and "
+          f"{by / a.steps / 1e9:.2f} GB (compulsory) of matrix products per step, {fl / wall / 1e12:.2f} TFLOP/s "
+          f"over the steps", flush=True)
+
+
+if __name__ == "__main__":
+    _main()

@@ -16,8 +16,10 @@ converged when the largest relative change of a slice between iterations is
 at most `tol`.  After k iterations the first k slices equal the serial fine
 solution exactly, so iteration k advances only slices n >= k with F (all of
 them together, hw_integrate_batch).  Fallback: when the change stalls (does
-not halve over two iterations) or the iteration budget ends, the remaining
-slices are advanced serially with F from the first one not yet converged.
+not halve over two iterations) or exceeds 1, when a slice's field norm grows
+beyond grow_limit times the initial one, or when the iteration budget ends,
+the remaining slices are advanced serially with F from the first one not yet
+converged.
 """
 from __future__ import annotations
 
@@ -157,10 +159,13 @@ class LinearBlend(LinearFlat):
 
 
 class CoarseFNO:
-    """G(eta, phi) = base(eta, phi) + scale * FNO(normalised [eta, phi, depth]),
-    the network's part limited to the anti-alias band: the network learns
-    what the base propagator (LinearFlat, LinearBlend) misses over one
-    coarse step."""
+    """G(eta, phi) = base(eta, phi) + scale * FNO(normalised [eta, phi, depth]):
+    the network learns what the base propagator (LinearFlat, LinearBlend)
+    misses over one coarse step.  Its part is kept to the anti-alias band and
+    to |k| below twice its retained modes (the band of its spectral layers
+    and their quadratic harmonics): what its pointwise layers put above that
+    is noise, and noise at high wavenumbers makes the fine solver's steps
+    small and, iterated over many slices, grows."""
 
     def __init__(self, net, depth, stats, base):
         # parareal differences G(U^{k+1}) - G(U^k) over states that converge:
@@ -170,6 +175,13 @@ class CoarseFNO:
         self.depth = depth
         self.st = stats
         self.base = base
+        n = base.n
+        self.band = base.mask.copy()
+        modes = getattr(net, "modes", None)
+        if modes is not None:
+            ix = np.fft.rfftfreq(n) * n
+            iy = np.fft.fftfreq(n) * n
+            self.band = self.band * (np.hypot(ix[None, :], iy[:, None]) < 2 * modes)
 
     def inputs(self, eta, phi):
         s = self.st
@@ -180,8 +192,8 @@ class CoarseFNO:
         out = self.net.forward(self.inputs(eta, phi)[None])[0].astype(np.float64)
         b = self.base
         e0, p0 = b(eta, phi)
-        ce = _fft.irfft2(b.mask * _fft.rfft2(self.st["dsd"][0] * out[0], **_W), s=(b.n, b.n), **_W)
-        cp = _fft.irfft2(b.mask * _fft.rfft2(self.st["dsd"][1] * out[1], **_W), s=(b.n, b.n), **_W)
+        ce = _fft.irfft2(self.band * _fft.rfft2(self.st["dsd"][0] * out[0], **_W), s=(b.n, b.n), **_W)
+        cp = _fft.irfft2(self.band * _fft.rfft2(self.st["dsd"][1] * out[1], **_W), s=(b.n, b.n), **_W)
         return e0 + ce, p0 + cp
 
 
@@ -289,8 +301,13 @@ class Parareal:
     problem `model`: the fine propagator runs in `nslices` device contexts
     advanced together (hw_integrate_batch)."""
 
-    def __init__(self, model, coarse, nslices: int, dt: float, precision: int = _lib.HW_PREC_FP32, h0: float = 0.05):
+    def __init__(self, model, coarse, nslices: int, dt: float, precision: int = _lib.HW_PREC_FP32, h0: float = 0.05,
+                 grow_limit: float = 4.0):
         self.m, self.G, self.N, self.dt, self.h0 = model, coarse, nslices, dt, h0
+        # the basin conserves energy: a slice whose field norm exceeds
+        # grow_limit times the initial one has diverged, and the fine solver
+        # would crawl through its noise
+        self.grow_limit = grow_limit
         self.codec = Codec(model)
         self.ctx = [Hawassi2D(model, precision) for _ in range(nslices)]
         self.fine_rhs = 0
@@ -325,6 +342,11 @@ class Parareal:
             U[n + 1] = self.codec.from_state(c.get_state())
         return U
 
+    def grown(self, U, u0, first):
+        ref = [max(float(np.linalg.norm(u0[f])), 1e-300) for f in range(2)]
+        return any(float(np.linalg.norm(U[n][f])) > self.grow_limit * ref[f]
+                   for n in range(first, len(U)) for f in range(2))
+
     @staticmethod
     def change(a, b):
         return max(float(np.linalg.norm(a[f] - b[f]) / max(np.linalg.norm(a[f]), 1e-300)) for f in range(2))
@@ -345,7 +367,7 @@ class Parareal:
             nw = min(W, N - w0)
             self.N = nw
             ref = None if reference is None else reference[w0:w0 + nw + 1]
-            r = self.run(U[w0], tol=tol, max_iter=nw, reference=ref)
+            r = self.run(U[w0], tol=tol, max_iter=nw, reference=ref, log=log)
             for i in range(1, nw + 1):
                 U[w0 + i] = r["U"][i]
             tot["iterations"] += r["iterations"]
@@ -383,6 +405,10 @@ class Parareal:
         fallback = False
         k = 0
         converged = False
+        if self.grown(U, u0, 1):
+            max_iter = 0
+            if log:
+                log("parareal: the coarse sweep diverged")
         while k < max_iter:
             w = time.perf_counter()
             Fk = self.fine(U, first=k)
@@ -406,8 +432,10 @@ class Parareal:
                 converged = True
                 break
             # stall or divergence: hand over to the serial fine solver
-            if not np.isfinite(ch) or ch > 1.0 or (k >= 3 and hist[-1] > 0.5 * hist[-3]):
+            if not np.isfinite(ch) or ch > 1.0 or (k >= 3 and hist[-1] > 0.5 * hist[-3]) or self.grown(U, u0, k + 1):
                 break
+        if not converged and k >= N:
+            converged = True        # every slice has been advanced by F from an exact slice
         if not converged:
             # fallback: the plain fine solver from the first slice that has not converged
             fallback = True

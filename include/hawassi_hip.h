@@ -321,6 +321,12 @@ int hw_fno_train_step(void *ctx, int batch, const int *idx, double lr, double *l
 #define HW_FNO_BF16 0
 #define HW_FNO_SPLIT 1
 int hw_fno_set_precision(void *ctx, int precision);
+/* floating-point operations (2 M N K per product) and compulsory bytes (FP32
+ * operands read once, the result written once, read too when accumulated)
+ * of every matrix product launched by this network since its creation, for
+ * throughput figures */
+double hw_fno_gemm_flops(void *ctx);
+double hw_fno_gemm_bytes(void *ctx);
 /* C [M][N] = A [M][K] B [K][N] (row-major host arrays) through the MFMA GEMM
  * at the given precision (HW_FNO_BF16 or HW_FNO_SPLIT) */
 int hw_fno_gemm_check(int M, int N, int K, const float *A, const float *B, float *C, int precision);
