@@ -11,6 +11,8 @@ LIB_PATH = Path(__file__).resolve().parent / "libhawassi_hip.so"
 HW_PREC_FP64 = 0
 HW_PREC_FP32 = 1
 HW_FNO_BF16 = 0
+HW_STEP_HOST = 0
+HW_STEP_GRAPH = 1
 HW_FNO_SPLIT = 1
 
 _dp = ct.POINTER(ct.c_double)
@@ -71,7 +73,7 @@ class HwFnoConfig(ct.Structure):
 
 EXPORTS = [
     "hw_ctx_create_2d", "hw_ctx_destroy", "hw_rhs", "hw_rhs_device", "hw_set_state", "hw_get_state",
-    "hw_evolve_apply", "hw_integrate", "hw_integrate_batch", "hw_get_fields", "hw_get_stats", "hw_reset_stats", "hw_sync",
+    "hw_evolve_apply", "hw_integrate", "hw_integrate_batch", "hw_set_step_mode", "hw_ctx_create_2d_members", "hw_integrate_members", "hw_get_fields", "hw_get_stats", "hw_reset_stats", "hw_sync",
     "hw_fft2_forward", "hw_fft2_inverse", "hw_fft2_forward_noalloc", "hw_supported", "hw_last_error", "hw_version",
     "hw_set_tprev", "hw_get_breaking",
     "hw1d_create", "hw1d_destroy", "hw1d_rhs", "hw1d_rhs_at", "hw1d_set_state", "hw1d_get_state",
@@ -132,6 +134,9 @@ def load(path: Path | None = None) -> ct.CDLL:
     lib.hw1d_get_breaking.argtypes = [vp, ct.c_int, _dp, ct.POINTER(ct.c_int)]
     lib.hw1d_get_stats.argtypes = [vp, ct.c_int, ct.POINTER(HwStats)]
     _fp = ct.POINTER(ct.c_float)
+    lib.hw_set_step_mode.argtypes = [vp, ct.c_int]
+    lib.hw_ctx_create_2d_members.argtypes = [ct.POINTER(HwConfig2D), ct.c_int, ct.POINTER(vp)]
+    lib.hw_integrate_members.argtypes = [vp, _dp, ct.c_double, _dp]
     lib.hw_fno_create.argtypes = [ct.POINTER(HwFnoConfig), ct.POINTER(vp)]
     lib.hw_fno_destroy.argtypes = [vp]
     lib.hw_fno_destroy.restype = None

@@ -16,12 +16,99 @@ struct StageCoef {
 
 // out = y + h * sum_j b_j k_j, converted to the RHS precision
 template <typename T>
-__global__ void k_rk_stage(const double2* __restrict__ y, StageCoef sc, double h, typename ctype<T>::type* __restrict__ out, size_t n) {
+__device__ __forceinline__ void rk_stage_body(const double2* __restrict__ y, const StageCoef& sc, double h,
+                                              typename ctype<T>::type* __restrict__ out, size_t n) {
     using C = typename ctype<T>::type;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) {
         double2 yv = y[i];
+        double ax = 0.0, ay = 0.0;
+        for (int j = 0; j < sc.nk; ++j) {
+            C kv = reinterpret_cast<const C*>(sc.k[j])[i];
+            ax += sc.b[j] * (double)kv.x;
+            ay += sc.b[j] * (double)kv.y;
+        }
+        double rx = yv.x + h * ax, ry = yv.y + h * ay;
+        out[i] = cmk<C>((T)rx, (T)ry);
+    }
+}
+
+template <typename T>
+__global__ void k_rk_stage(const double2* __restrict__ y, StageCoef sc, double h, typename ctype<T>::type* __restrict__ out, size_t n) {
+    rk_stage_body<T>(y, sc, h, out, n);
+}
+
+// control_y hadjust (GSL standard control with a_y = 1, a_dydt = 0, order 5):
+// returns -1 decrease, 0 keep, +1 increase.  The host step loop and the
+// device step control (k_step_end) share it.
+__host__ __device__ inline int hadjust_rule(double rmax, double* h) {
+    const double S = 0.9;
+    const int ord = 5;
+    double h_old = *h;
+    if (rmax > 1.1) {
+        double r = S / pow(rmax, 1.0 / ord);
+        if (r < 0.2) r = 0.2;
+        *h = r * h_old;
+        return -1;
+    } else if (rmax < 0.5) {
+        double r = S / pow(rmax, 1.0 / (ord + 1.0));
+        if (r > 5.0) r = 5.0;
+        if (r < 1.0) r = 1.0;
+        *h = r * h_old;
+        return 1;
+    }
+    return 0;
+}
+
+// ------------------------------------------------- device step control ----
+// gsl_odeiv2_evolve_apply's state kept in device memory, so that one
+// attempted step -- K0, the five stages, the error norm and control_y -- is a
+// fixed sequence of launches that a HIP graph replays without the host:
+// k_step_begin clips the step to t1, the stage and final kernels take h and
+// the buffer holding y from here, k_step_end accepts (swapping the buffers)
+// or rejects.  Once t reaches t1 (or the step underflows) `done` is set and
+// every kernel of a further replay returns at once.
+struct DevStep {
+    double t, h, t1;            // evolve_apply's t and h, the driver's target
+    double t0, h0;              // this attempt's start and step
+    double eps_abs, eps_rel;
+    unsigned long long rmax_bits;
+    int cur;                    // buffer holding y: 0 or 1
+    int done;                   // t reached t1, or the step underflowed
+    int err;                    // step size underflow at t0
+    int final_step;
+    long long accepted, rejected, attempts;
+};
+
+static __global__ void k_step_begin(DevStep* sv, int nb) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= nb) return;
+    DevStep* s = sv + z;
+    if (s->done) return;
+    const double dt = s->t1 - s->t;
+    double h0 = s->h;
+    int fin = 0;
+    if ((dt >= 0.0 && h0 > dt) || (dt < 0.0 && h0 < dt)) { h0 = dt; fin = 1; }
+    s->t0 = s->t;
+    s->h0 = h0;
+    s->final_step = fin;
+    s->rmax_bits = 0ull;
+}
+
+// members are consecutive ranges of `per` elements, each with its own h and
+// state buffer; a finished member's range is left alone
+template <typename T>
+__global__ void k_rk_stage_g(const double2* __restrict__ ya, const double2* __restrict__ yb, const DevStep* __restrict__ s,
+                             StageCoef sc, typename ctype<T>::type* __restrict__ out, size_t per, size_t n) {
+    using C = typename ctype<T>::type;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        const DevStep& d = s[i / per];
+        if (d.done) continue;
+        const double h = d.h0;
+        const double2 yv = (d.cur ? yb : ya)[i];
         double ax = 0.0, ay = 0.0;
         for (int j = 0; j < sc.nk; ++j) {
             C kv = reinterpret_cast<const C*>(sc.k[j])[i];
@@ -40,11 +127,11 @@ __global__ void k_rk_stage(const double2* __restrict__ y, StageCoef sc, double h
 // The max is reduced with wavefront shuffles, then one atomicMax per block on
 // the bit pattern of the (non-negative) double.
 template <typename T>
-__global__ void __launch_bounds__(256) k_rk_final(const double2* __restrict__ y, const typename ctype<T>::type* __restrict__ k1,
-                                                  const typename ctype<T>::type* __restrict__ k3, const typename ctype<T>::type* __restrict__ k4,
-                                                  const typename ctype<T>::type* __restrict__ k5, const typename ctype<T>::type* __restrict__ k6,
-                                                  double h, double eps_abs, double eps_rel, double2* __restrict__ ynew,
-                                                  unsigned long long* __restrict__ rmax_bits, size_t n) {
+__device__ __forceinline__ void rk_final_body(const double2* __restrict__ y, const typename ctype<T>::type* __restrict__ k1,
+                                              const typename ctype<T>::type* __restrict__ k3, const typename ctype<T>::type* __restrict__ k4,
+                                              const typename ctype<T>::type* __restrict__ k5, const typename ctype<T>::type* __restrict__ k6,
+                                              double h, double eps_abs, double eps_rel, double2* __restrict__ ynew,
+                                              unsigned long long* __restrict__ rmax_bits, size_t n) {
     const double c1 = 902880.0 / 7618050.0, c3 = 3953664.0 / 7618050.0, c4 = 3855735.0 / 7618050.0,
                  c5 = -1371249.0 / 7618050.0, c6 = 277020.0 / 7618050.0;
     const double e1 = 1.0 / 360.0, e3 = -128.0 / 4275.0, e4 = -2197.0 / 75240.0, e5 = 1.0 / 50.0, e6 = 2.0 / 55.0;
@@ -79,6 +166,61 @@ __global__ void __launch_bounds__(256) k_rk_final(const double2* __restrict__ y,
         for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = wmax[w] > m ? wmax[w] : m;
         atomicMax(rmax_bits, (unsigned long long)__double_as_longlong(m));
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_rk_final(const double2* __restrict__ y, const typename ctype<T>::type* __restrict__ k1,
+                                                  const typename ctype<T>::type* __restrict__ k3, const typename ctype<T>::type* __restrict__ k4,
+                                                  const typename ctype<T>::type* __restrict__ k5, const typename ctype<T>::type* __restrict__ k6,
+                                                  double h, double eps_abs, double eps_rel, double2* __restrict__ ynew,
+                                                  unsigned long long* __restrict__ rmax_bits, size_t n) {
+    rk_final_body<T>(y, k1, k3, k4, k5, k6, h, eps_abs, eps_rel, ynew, rmax_bits, n);
+}
+
+// the final kernel of a graph attempt, member blockIdx.y (elements
+// [z per, (z + 1) per)): y from buffer cur, y_new into the other
+template <typename T>
+__global__ void __launch_bounds__(256) k_rk_final_g(double2* __restrict__ ya, double2* __restrict__ yb, DevStep* __restrict__ sv,
+                                                    const typename ctype<T>::type* __restrict__ k1,
+                                                    const typename ctype<T>::type* __restrict__ k3, const typename ctype<T>::type* __restrict__ k4,
+                                                    const typename ctype<T>::type* __restrict__ k5, const typename ctype<T>::type* __restrict__ k6,
+                                                    size_t per) {
+    DevStep* s = sv + blockIdx.y;
+    if (s->done) return;
+    const size_t o = (size_t)blockIdx.y * per;
+    const int cur = s->cur;
+    rk_final_body<T>((cur ? yb : ya) + o, k1 + o, k3 + o, k4 + o, k5 + o, k6 + o, s->h0, s->eps_abs, s->eps_rel,
+                     (cur ? ya : yb) + o, &s->rmax_bits, per);
+}
+
+// control_y and evolve_apply's accept / retry / underflow decision
+static __global__ void k_step_end(DevStep* sv, int nb) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= nb) return;
+    DevStep* s = sv + z;
+    if (s->done) return;
+    s->attempts++;
+    double rm = __longlong_as_double((long long)s->rmax_bits);
+    if (rm < 2.2250738585072014e-308) rm = 2.2250738585072014e-308;     // GSL starts rmax at DBL_MIN
+    const double tn = s->final_step ? s->t1 : __dadd_rn(s->t0, s->h0);
+    const double h_old = s->h0;
+    double hn = h_old;
+    if (hadjust_rule(rm, &hn) < 0) {
+        if (fabs(hn) < fabs(h_old) && __dadd_rn(tn, hn) != tn) {
+            s->rejected++;
+            s->h = hn;                  // retry from t0 with the smaller step
+            return;
+        }
+        s->h = hn;
+        s->err = 1;
+        s->done = 1;
+        return;
+    }
+    s->accepted++;
+    s->t = tn;
+    s->h = hn;
+    s->cur ^= 1;
+    if (!(tn < s->t1)) s->done = 1;     // ode_solver_uv: while (t < t1)
 }
 
 // ------------------------------------------------- boundary conversions ---
@@ -194,6 +336,34 @@ __global__ void k_src_eval(const double* __restrict__ sy, const double* __restri
         double d = (cip - ci) / (3.0 * hh);
         double v = y0 + dx * (b + dx * (ci + dx * d));
         if (s == 0) osig[j] = (T)v; else oskew[j] = (T)v;
+    }
+}
+
+// k_src_eval at the stage time t0 + frac h0 of a graph attempt: the interval
+// is found on the device by the same bisection (gsl_interp_bsearch)
+template <typename T>
+__global__ void k_src_eval_g(const DevStep* __restrict__ s, double frac, const double* __restrict__ tsig, int nt,
+                             const double* __restrict__ sy, const double* __restrict__ scf, const double* __restrict__ ky_,
+                             const double* __restrict__ kc, int nline, T* __restrict__ osig, T* __restrict__ oskew) {
+    if (s->done) return;
+    const double t = __dadd_rn(s->t0, __dmul_rn(frac, s->h0));
+    int ilo = 0, ihi = nt - 1;
+    while (ihi > ilo + 1) {
+        int i = (ihi + ilo) / 2;
+        if (tsig[i] > t) ihi = i; else ilo = i;
+    }
+    const double hh = tsig[ilo + 1] - tsig[ilo], dx = t - tsig[ilo];
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nline) return;
+    for (int q = 0; q < 2; ++q) {
+        const double* Y = q == 0 ? sy : ky_;
+        const double* Cc = q == 0 ? scf : kc;
+        double y0 = Y[(size_t)ilo * nline + j], y1 = Y[(size_t)(ilo + 1) * nline + j];
+        double ci = Cc[(size_t)ilo * nline + j], cip = Cc[(size_t)(ilo + 1) * nline + j];
+        double b = (y1 - y0) / hh - hh * (cip + 2.0 * ci) / 3.0;
+        double d = (cip - ci) / (3.0 * hh);
+        double v = y0 + dx * (b + dx * (ci + dx * d));
+        if (q == 0) osig[j] = (T)v; else oskew[j] = (T)v;
     }
 }
 

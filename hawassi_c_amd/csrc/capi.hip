@@ -1,5 +1,6 @@
 // capi.hip -- the extern "C" entry points declared in include/hawassi_hip.h.
 #include <hip/hip_runtime.h>
+#include <string>
 #include <vector>
 
 #include <cstdarg>
@@ -97,7 +98,34 @@ int hw_ctx_create_2d(const hw_config_2d* cfg, void** ctx) {
     return HW_OK;
 }
 
+int hw_ctx_create_2d_members(const hw_config_2d* cfg, int members, void** ctx) {
+    if (!cfg || !ctx || members < 1) { set_error("bad argument"); return HW_ERR_ARG; }
+    *ctx = nullptr;
+    const Entry* e = find_entry(cfg->precision, cfg->nx, cfg->ny);
+    if (!e) {
+        set_error("unsupported grid %dx%d precision %d (no compiled kernels)", cfg->nx, cfg->ny, cfg->precision);
+        return HW_ERR_ARG;
+    }
+    int rc = check_device();
+    if (rc) return rc;
+    Solver2D* s = e->make();
+    s->nb = members;
+    s->step_mode = HW_STEP_GRAPH;
+    rc = s->init(cfg);
+    if (rc) { delete s; return rc; }
+    *ctx = s;
+    return HW_OK;
+}
+
 void hw_ctx_destroy(void* ctx) { delete static_cast<Solver2D*>(ctx); }
+
+// entry points that act on one state
+static int one_member(const Solver2D* s, const char* fn) {
+    if (s->nb == 1) return HW_OK;
+    set_error("%s: the context holds %d members (hw_ctx_create_2d_members); use hw_set_state, hw_get_state and "
+              "hw_integrate_members", fn, s->nb);
+    return HW_ERR_ARG;
+}
 
 static int ensure_staging(Solver2D* s) {
     size_t bytes = sizeof(double) * 6 * (size_t)s->nx * s->ny;
@@ -109,7 +137,9 @@ static int ensure_staging(Solver2D* s) {
 int hw_rhs(double t, const double* y, double* dydt, void* ctx) {
     Solver2D* s = static_cast<Solver2D*>(ctx);
     if (!s || !y || !dydt) { set_error("null argument"); return HW_ERR_ARG; }
-    int rc = ensure_staging(s);
+    int rc = one_member(s, "hw_rhs");
+    if (rc) return rc;
+    rc = ensure_staging(s);
     if (rc) return rc;
     size_t bytes = sizeof(double) * 6 * (size_t)s->nx * s->ny;
     HW_TRY(hipMemcpyAsync(s->d_full_a, y, bytes, hipMemcpyHostToDevice, s->stream));
@@ -123,6 +153,8 @@ int hw_rhs(double t, const double* y, double* dydt, void* ctx) {
 int hw_rhs_device(void* ctx, double t, const double* d_y, double* d_dydt) {
     Solver2D* s = static_cast<Solver2D*>(ctx);
     if (!s || !d_y || !d_dydt) { set_error("null argument"); return HW_ERR_ARG; }
+    int rc = one_member(s, "hw_rhs_device");
+    if (rc) return rc;
     return s->rhs_full_dev(t, d_y, d_dydt);
 }
 
@@ -131,7 +163,15 @@ int hw_set_state(void* ctx, const double* y) {
     if (!s || !y) { set_error("null argument"); return HW_ERR_ARG; }
     int rc = ensure_staging(s);
     if (rc) return rc;
-    size_t bytes = sizeof(double) * 6 * (size_t)s->nx * s->ny;
+    size_t n6 = 6 * (size_t)s->nx * s->ny, bytes = sizeof(double) * n6;
+    if (s->nb > 1) {
+        // one full state after the other
+        for (int z = 0; z < s->nb; ++z) {
+            HW_TRY(hipMemcpyAsync(s->d_full_a, y + z * n6, bytes, hipMemcpyHostToDevice, s->stream));
+            if ((rc = s->set_member_dev(s->d_full_a, z))) return rc;
+        }
+        return HW_OK;
+    }
     HW_TRY(hipMemcpyAsync(s->d_full_a, y, bytes, hipMemcpyHostToDevice, s->stream));
     return s->set_state_dev(s->d_full_a);
 }
@@ -141,9 +181,17 @@ int hw_get_state(void* ctx, double* y) {
     if (!s || !y) { set_error("null argument"); return HW_ERR_ARG; }
     int rc = ensure_staging(s);
     if (rc) return rc;
+    size_t n6 = 6 * (size_t)s->nx * s->ny, bytes = sizeof(double) * n6;
+    if (s->nb > 1) {
+        for (int z = 0; z < s->nb; ++z) {
+            if ((rc = s->get_member_dev(s->d_full_a, z))) return rc;
+            HW_TRY(hipMemcpyAsync(y + z * n6, s->d_full_a, bytes, hipMemcpyDeviceToHost, s->stream));
+            HW_TRY(hipStreamSynchronize(s->stream));
+        }
+        return HW_OK;
+    }
     rc = s->get_state_dev(s->d_full_a);
     if (rc) return rc;
-    size_t bytes = sizeof(double) * 6 * (size_t)s->nx * s->ny;
     HW_TRY(hipMemcpyAsync(y, s->d_full_a, bytes, hipMemcpyDeviceToHost, s->stream));
     HW_TRY(hipStreamSynchronize(s->stream));
     return HW_OK;
@@ -152,12 +200,82 @@ int hw_get_state(void* ctx, double* y) {
 int hw_evolve_apply(void* ctx, double* t, double t1, double* h) {
     Solver2D* s = static_cast<Solver2D*>(ctx);
     if (!s || !t || !h) { set_error("null argument"); return HW_ERR_ARG; }
+    int rc = one_member(s, "hw_evolve_apply");
+    if (rc) return rc;
     return s->evolve_apply(t, t1, h);
+}
+
+int hw_set_step_mode(void* ctx, int mode) {
+    Solver2D* s = static_cast<Solver2D*>(ctx);
+    if (!s || (mode != HW_STEP_HOST && mode != HW_STEP_GRAPH)) { set_error("bad argument"); return HW_ERR_ARG; }
+    if (s->nb > 1 && mode != HW_STEP_GRAPH) {
+        set_error("a context with several members steps on the device only");
+        return HW_ERR_ARG;
+    }
+    s->step_mode = mode;
+    return HW_OK;
+}
+
+// t and h hold one value per member of every context, context after context
+static int graph_batch(Solver2D* const* ss, int n, double* t, double t1, double* h) {
+    std::vector<char> act((size_t)n, 0);
+    std::vector<size_t> off((size_t)n + 1, 0);
+    int rc;
+    for (int i = 0; i < n; ++i) off[i + 1] = off[i] + (size_t)ss[i]->nb;
+    for (int i = 0; i < n; ++i) {
+        bool any = false;
+        for (size_t j = off[i]; j < off[i + 1]; ++j) {
+            const double dt = t1 - t[j];
+            if ((dt < 0.0 && h[j] > 0.0) || (dt > 0.0 && h[j] < 0.0)) {
+                set_error("context %d: step direction must match interval direction", i);
+                return HW_ERR_ARG;
+            }
+            any = any || t[j] < t1;
+        }
+        if (!any) continue;
+        if ((rc = ss[i]->graph_begin(t + off[i], t1, h + off[i]))) return rc;
+        act[i] = 1;
+    }
+    for (;;) {
+        bool any = false;
+        // every unfinished context's next batch of attempts, each on its own stream ...
+        for (int i = 0; i < n; ++i)
+            if (act[i]) {
+                any = true;
+                if ((rc = ss[i]->graph_launch())) return rc;
+            }
+        if (!any) break;
+        // ... then each one's progress
+        for (int i = 0; i < n; ++i) {
+            if (!act[i]) continue;
+            bool done = false;
+            if ((rc = ss[i]->graph_poll(&done))) return rc;
+            if (done) {
+                act[i] = 0;
+                if ((rc = ss[i]->graph_finish(t + off[i], h + off[i]))) {
+                    const std::string m = hw_last_error();
+                    set_error("context %d: %s", i, m.c_str());
+                    return rc;
+                }
+            }
+        }
+    }
+    return HW_OK;
+}
+
+int hw_integrate_members(void* ctx, double* t, double t1, double* h) {
+    Solver2D* s = static_cast<Solver2D*>(ctx);
+    if (!s || !t || !h) { set_error("null argument"); return HW_ERR_ARG; }
+    if (s->step_mode != HW_STEP_GRAPH) { set_error("hw_integrate_members needs HW_STEP_GRAPH"); return HW_ERR_ARG; }
+    return graph_batch(&s, 1, t, t1, h);
 }
 
 int hw_integrate(void* ctx, double* t, double t1, double* h) {
     Solver2D* s = static_cast<Solver2D*>(ctx);
     if (!s || !t || !h) { set_error("null argument"); return HW_ERR_ARG; }
+    int rc = one_member(s, "hw_integrate");
+    if (rc) return rc;
+    if (s->step_mode == HW_STEP_GRAPH) return graph_batch(&s, 1, t, t1, h);
     while (*t < t1) {
         int rc = s->evolve_apply(t, t1, h);
         if (rc) return rc;
@@ -167,6 +285,20 @@ int hw_integrate(void* ctx, double* t, double t1, double* h) {
 
 int hw_integrate_batch(void* const* ctxs, int n, double* t, double t1, double* h) {
     if (n < 0 || (n > 0 && (!ctxs || !t || !h))) { set_error("null argument"); return HW_ERR_ARG; }
+    {
+        bool all_graph = n > 0;
+        for (int i = 0; i < n; ++i) {
+            if (!ctxs[i]) { set_error("null context %d", i); return HW_ERR_ARG; }
+            int rc = one_member(static_cast<Solver2D*>(ctxs[i]), "hw_integrate_batch");
+            if (rc) return rc;
+            all_graph = all_graph && static_cast<Solver2D*>(ctxs[i])->step_mode == HW_STEP_GRAPH;
+        }
+        if (all_graph) {
+            std::vector<Solver2D*> ss((size_t)n);
+            for (int i = 0; i < n; ++i) ss[i] = static_cast<Solver2D*>(ctxs[i]);
+            return graph_batch(ss.data(), n, t, t1, h);
+        }
+    }
     struct Lane { double t0, h0; bool fin, need_k0, active; };
     std::vector<Lane> ln((size_t)n);
     for (int i = 0; i < n; ++i) {
@@ -228,7 +360,9 @@ int hw_get_fields(void* ctx, double* eta, double* u, double* v) {
     Solver2D* s = static_cast<Solver2D*>(ctx);
     if (!s) { set_error("null argument"); return HW_ERR_ARG; }
     size_t n = (size_t)s->nx * s->ny;
-    int rc = ensure_staging(s);
+    int rc = one_member(s, "hw_get_fields");
+    if (rc) return rc;
+    rc = ensure_staging(s);
     if (rc) return rc;
     double* d[3] = {eta ? s->d_full_a : nullptr, u ? s->d_full_a + n : nullptr, v ? s->d_full_a + 2 * n : nullptr};
     rc = s->fields_dev(d[0], d[1], d[2]);

@@ -56,6 +56,22 @@ template <typename T> struct Rhs2DArgs {
     // third reference depth (HSSgen_3depth): gamma_mid [y][x], C^2_mid and
     // Om^2_mid [kx][ky]; null for two depths
     const T *gam_c, *csq_c, *om2_c;
+    // Several states of one model in one launch (grid.z = member): member z's
+    // input / output spectra, work slots, physical fields and per-row values
+    // sit these many elements after member 0's.
+    size_t zs_state, zs_work, zs_phys, zs_row;
+    __device__ __forceinline__ void shift(int z) {
+        if (z == 0) return;
+        const size_t s = (size_t)z * zs_state, w = (size_t)z * zs_work, p = (size_t)z * zs_phys;
+        eta_hat += s; u_hat += s; v_hat += s;
+        k_eta += s; k_u += s; k_v += s;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) W[i] += w;
+        dH1 += w; dH2p += w;
+        eta_p += p; u_p += p; v_p += p;
+        if (m0u_p) { m0u_p += p; m0v_p += p; }
+        if (s_eta_out) s_eta_out += (size_t)z * zs_row;
+    }
 };
 
 template <typename T, int NX, int NY> struct Geo2 {
@@ -207,7 +223,9 @@ template <typename T, int NX, int NY> struct YIdx {
 };
 
 // ========================================================== 2DB kernels ==
-#define HW_XK_PROLOGUE HW_XK_PROLOGUE_TW(a.twx)
+#define HW_XK_PROLOGUE                                                   \
+    a.shift((int)blockIdx.z);                                            \
+    HW_XK_PROLOGUE_TW(a.twx)
 #define HW_XK_PROLOGUE_TW(TWSRC)                                         \
     using C = typename ctype<T>::type;                                   \
     using G = XGeo<T, NX>;                                               \
@@ -223,7 +241,9 @@ template <typename T, int NX, int NY> struct YIdx {
     __syncthreads();                                                     \
     (void)nyq; (void)sc;
 
-#define HW_YK_PROLOGUE HW_YK_PROLOGUE_TW(a.twy)
+#define HW_YK_PROLOGUE                                                   \
+    a.shift((int)blockIdx.z);                                            \
+    HW_YK_PROLOGUE_TW(a.twy)
 #define HW_YK_PROLOGUE_TW(TWSRC)                                         \
     using C = typename ctype<T>::type;                                   \
     using G = YGeo<T, NY>;                                               \

@@ -49,27 +49,6 @@ static constexpr double RK_B5[4] = {8341.0 / 4104.0, -32832.0 / 4104.0, 29440.0 
 static constexpr double RK_B6[5] = {-6080.0 / 20520.0, 41040.0 / 20520.0, -28352.0 / 20520.0, 9295.0 / 20520.0,
                                     -5643.0 / 20520.0};
 
-// control_y hadjust (GSL standard control with a_y = 1, a_dydt = 0, order 5)
-// returns -1 decrease, 0 keep, +1 increase.
-inline int hadjust_rule(double rmax, double* h) {
-    const double S = 0.9;
-    const int ord = 5;
-    double h_old = *h;
-    if (rmax > 1.1) {
-        double r = S / std::pow(rmax, 1.0 / ord);
-        if (r < 0.2) r = 0.2;
-        *h = r * h_old;
-        return -1;
-    } else if (rmax < 0.5) {
-        double r = S / std::pow(rmax, 1.0 / (ord + 1.0));
-        if (r > 5.0) r = 5.0;
-        if (r < 1.0) r = 1.0;
-        *h = r * h_old;
-        return 1;
-    }
-    return 0;
-}
-
 inline int bsearch_interval(const std::vector<double>& x, double t) {
     int ilo = 0, ihi = (int)x.size() - 1;
     while (ihi > ilo + 1) {
@@ -92,6 +71,18 @@ struct Solver2D {
     virtual int accept_step(double t) = 0;
     virtual int fields_dev(double* eta, double* u, double* v) = 0;
     virtual int breaking_state(double* B, int* flagbr) = 0;
+    // device step control replayed from a HIP graph (HW_STEP_GRAPH): the
+    // driver loop to t1 in pieces, for one context or several in lockstep;
+    // t and h hold one value per member
+    virtual int graph_begin(const double* t, double t1, const double* h) = 0;
+    virtual int graph_launch() = 0;                  // a batch of attempts + the state copy-back
+    virtual int graph_poll(bool* done) = 0;          // waits for it
+    virtual int graph_finish(double* t, double* h) = 0;
+    // member z's state of a context holding several (nb > 1)
+    virtual int set_member_dev(const double* d_full, int z) = 0;
+    virtual int get_member_dev(double* d_full, int z) = 0;
+    int nb = 1;            // states of this model advanced together (members)
+    int step_mode = 0;     // HW_STEP_HOST or HW_STEP_GRAPH
     double tprev = 0.0;    // the reference's global tprev
     int nx = 0, ny = 0, prec = 0;
     hipStream_t stream = nullptr;
@@ -185,8 +176,25 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
     static constexpr int NEV = 64;
     hipEvent_t ev_a[NEV] = {}, ev_b[NEV] = {};
     int nev = 0;
+    // ---- device step control (graph mode)
+    DevBuf<DevStep> dstep;
+    DevStep* h_dstep = nullptr;            // pinned copy of the device state
+    DevBuf<double> tsig_d;                 // source sample times on the device
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    double2 *gA = nullptr, *gB = nullptr;  // the two state buffers the graph was captured with
+    bool capturing = false;
+    double cap_frac = 0.0;                 // stage time fraction of the RHS being captured
+    hipEvent_t gev_a = nullptr, gev_b = nullptr;
+    long long g_att = 0;                   // attempts counted into the stats so far
+    std::vector<int> mcur;                 // members (nb > 1): buffer holding each one's state
 
     ~Solver2DT() override {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (h_dstep) (void)hipHostFree(h_dstep);
+        if (gev_a) (void)hipEventDestroy(gev_a);
+        if (gev_b) (void)hipEventDestroy(gev_b);
         if (h_cnt) (void)hipHostFree(h_cnt);
         if (h_cand) (void)hipHostFree(h_cand);
         if (ev_brk) (void)hipEventDestroy(ev_brk);
@@ -297,7 +305,8 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         }
         tprev = c->t0;
         brk_on = c->breaking_on && !hs1;   // rhs_AB1_* never calls breaking_process
-        if ((rc = s_eta.alloc(NY))) return rc;
+        if (nb < 1) { set_error("members must be >= 1"); return HW_ERR_ARG; }
+        if ((rc = s_eta.alloc((size_t)NY * nb))) return rc;
         if (brk_on) {
             if (!c->pbchar || !c->depth || !c->x || !c->y || c->rsearch <= 0) {
                 set_error("breaking needs pbchar, depth, x, y and rsearch > 0");
@@ -324,13 +333,22 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
             HW_TRY(hipHostMalloc((void**)&h_cand, CAPH * sizeof(BrkCand), hipHostMallocDefault));
             HW_TRY(hipEventCreateWithFlags(&ev_brk, hipEventDisableTiming));
         }
-        if ((rc = eta_p.alloc(N)) || (rc = u_p.alloc(N)) || (rc = v_p.alloc(N))) return rc;
-        if ((hs3 || three) && ((rc = m0u_p.alloc(N)) || (rc = m0v_p.alloc(N)))) return rc;
-        if ((rc = work.alloc(18 * H))) return rc;
-        if ((rc = y.alloc(3 * H)) || (rc = ynew.alloc(3 * H)) || (rc = kbuf.alloc(6 * 3 * H)) || (rc = ytmp.alloc(3 * H))) return rc;
+        if (nb > 1 && (brk_on || src_on || three)) {
+            set_error("a context with several members needs breaking, the influx source and a third reference "
+                      "depth off");
+            return HW_ERR_ARG;
+        }
+        const size_t B = (size_t)nb;
+        if ((rc = eta_p.alloc(N * B)) || (rc = u_p.alloc(N * B)) || (rc = v_p.alloc(N * B))) return rc;
+        if ((hs3 || three) && ((rc = m0u_p.alloc(N * B)) || (rc = m0v_p.alloc(N * B)))) return rc;
+        if ((rc = work.alloc(18 * H * B))) return rc;
+        if ((rc = y.alloc(3 * H * B)) || (rc = ynew.alloc(3 * H * B)) || (rc = kbuf.alloc(6 * 3 * H * B)) ||
+            (rc = ytmp.alloc(3 * H * B)))
+            return rc;
+        mcur.assign(B, 0);
         if ((rc = rmax.alloc(1))) return rc;
         HW_TRY(hipHostMalloc((void**)&h_rmax, sizeof(unsigned long long), hipHostMallocDefault));
-        HW_TRY(hipMemsetAsync(y.p, 0, sizeof(double2) * 3 * H, stream));
+        HW_TRY(hipMemsetAsync(y.p, 0, sizeof(double2) * 3 * H * B, stream));
         // fixed kernel arguments
         args.csq_m = csq_m.p; args.csq_p = csq_p.p; args.om2_m = om2_m.p; args.om2_p = om2_p.p;
         args.aal = aal.p; args.c2d2g = c2d2g.p; args.l2d = l2d.p; args.kxh = kxh.p; args.ky = kyv.p;
@@ -348,6 +366,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         args.twx = twx.p; args.twy = twy.p;
         args.g = (T)g;
         args.s_eta_out = s_eta.p;
+        args.zs_state = 3 * H; args.zs_work = 18 * H; args.zs_phys = N; args.zs_row = NY;
         HW_TRY(hipStreamSynchronize(stream));
         return HW_OK;
     }
@@ -390,7 +409,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
     // every HW_RHS_TIME_EVERY-th evaluation is bracketed by HIP events
     static constexpr long HW_RHS_TIME_EVERY = 4;
     int rhs(double t, const C* in, C* out) {
-        const bool timed = st.rhs_evals % HW_RHS_TIME_EVERY == 0;
+        const bool timed = !capturing && st.rhs_evals % HW_RHS_TIME_EVERY == 0;
         if (timed) {
             if (nev == NEV) {
                 int rc = collect_timing();
@@ -400,15 +419,19 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         }
         args.eta_hat = in; args.u_hat = in + H; args.v_hat = in + 2 * H;
         args.k_eta = out; args.k_u = out + H; args.k_v = out + 2 * H;
-        if (src_on) {
+        if (src_on && capturing) {
+            k_src_eval_g<T><<<(nline + 255) / 256, 256, 0, stream>>>(dstep.p, cap_frac, tsig_d.p, nt, sig_y.p, sig_c.p,
+                                                                     skew_y.p, skew_c.p, nline, src_sig.p, src_skew.p);
+            HW_LAUNCH_CHECK();
+        } else if (src_on) {
             int idx = bsearch_interval(tsig, t);
             double hh = tsig[idx + 1] - tsig[idx];
             k_src_eval<T><<<(nline + 255) / 256, 256, 0, stream>>>(sig_y.p, sig_c.p, skew_y.p, skew_c.p, nline, idx,
                                                                    t - tsig[idx], hh, src_sig.p, src_skew.p);
             HW_LAUNCH_CHECK();
         }
-        const dim3 gx(NY / XG::RR), bx(XG::NT);
-        const dim3 gy((NXH + YG::CC - 1) / YG::CC), by(YG::NT);
+        const dim3 gx(NY / XG::RR, 1, nb), bx(XG::NT);
+        const dim3 gy((NXH + YG::CC - 1) / YG::CC, 1, nb), by(YG::NT);
         // 2d/fun_odesolver.c:4580-4587: breaking runs only when tprev < t
         const bool brk_now = brk_on && tprev < t;
         args.eta_col0 = brk_now ? col0.p : nullptr;
@@ -524,7 +547,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
             nev++;
             st.rhs_timed++;
         }
-        st.rhs_evals++;
+        if (!capturing) st.rhs_evals++;
         return HW_OK;
     }
 
@@ -742,7 +765,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         sc.nk = 0;
         auto bi = bs.begin();
         for (int kidx : ks) {
-            sc.k[sc.nk] = kbuf.p + (size_t)kidx * 3 * H;
+            sc.k[sc.nk] = K(kidx);
             sc.b[sc.nk] = *bi++;
             sc.nk++;
         }
@@ -751,7 +774,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         return HW_OK;
     }
 
-    C* K(int i) { return kbuf.p + (size_t)i * 3 * H; }
+    C* K(int i) { return kbuf.p + (size_t)i * 3 * H * nb; }
 
     // One attempted rkf45 step from (t0, y) with step h0, enqueued on the
     // stream: K0 = f(t0, y) first when need_k0 (gsl's dydt_in, computed once
@@ -840,6 +863,191 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
             *h = h0;
             return accept_step(*t);
         }
+    }
+
+    // ------------------------------------------- device step control --
+    // One attempt of evolve_apply as graph nodes: the stage times come from
+    // the device state (k_step_begin), K0 is evaluated in every attempt (a
+    // retried attempt recomputes the same K0 the host loop keeps), and the
+    // error norm goes to k_step_end, which accepts or rejects on the device.
+    // With several members every launch covers all of them (grid.z or the
+    // element range), each member with its own t, h and accept decision.
+    int stage_g(std::initializer_list<int> ks, std::initializer_list<double> bs) {
+        StageCoef sc{};
+        sc.nk = 0;
+        auto bi = bs.begin();
+        for (int kidx : ks) {
+            sc.k[sc.nk] = K(kidx);
+            sc.b[sc.nk] = *bi++;
+            sc.nk++;
+        }
+        k_rk_stage_g<T><<<grid1(3 * H * nb), 256, 0, stream>>>(gA, gB, dstep.p, sc, ytmp.p, 3 * H, (size_t)nb * 3 * H);
+        HW_LAUNCH_CHECK();
+        return HW_OK;
+    }
+
+    int attempt_body_g() {
+        int rc;
+        const unsigned sb = (unsigned)((nb + 63) / 64);
+        k_step_begin<<<sb, 64, 0, stream>>>(dstep.p, nb);
+        HW_LAUNCH_CHECK();
+        cap_frac = 0.0;
+        if ((rc = stage_g({}, {})) || (rc = rhs(0.0, ytmp.p, K(0)))) return rc;
+        cap_frac = RK_AH[0];
+        if ((rc = stage_g({0}, {RK_AH[0]})) || (rc = rhs(0.0, ytmp.p, K(1)))) return rc;
+        cap_frac = RK_AH[1];
+        if ((rc = stage_g({0, 1}, {RK_B3[0], RK_B3[1]})) || (rc = rhs(0.0, ytmp.p, K(2)))) return rc;
+        cap_frac = RK_AH[2];
+        if ((rc = stage_g({0, 1, 2}, {RK_B4[0], RK_B4[1], RK_B4[2]})) || (rc = rhs(0.0, ytmp.p, K(3)))) return rc;
+        cap_frac = RK_AH[3];
+        if ((rc = stage_g({0, 1, 2, 3}, {RK_B5[0], RK_B5[1], RK_B5[2], RK_B5[3]})) || (rc = rhs(0.0, ytmp.p, K(4))))
+            return rc;
+        cap_frac = RK_AH[4];
+        if ((rc = stage_g({0, 1, 2, 3, 4}, {RK_B6[0], RK_B6[1], RK_B6[2], RK_B6[3], RK_B6[4]})) ||
+            (rc = rhs(0.0, ytmp.p, K(5))))
+            return rc;
+        // one block row per member, so that each block's norm belongs to one member
+        const unsigned fb = nb == 1 ? 2048u : (unsigned)std::max<size_t>(1, std::min<size_t>(2048, (3 * H + 255) / 256));
+        k_rk_final_g<T><<<dim3(fb, nb), 256, 0, stream>>>(gA, gB, dstep.p, K(0), K(2), K(3), K(4), K(5), 3 * H);
+        HW_LAUNCH_CHECK();
+        k_step_end<<<sb, 64, 0, stream>>>(dstep.p, nb);
+        HW_LAUNCH_CHECK();
+        return HW_OK;
+    }
+
+    int capture_attempt() {
+        int rc;
+        if (!dstep.p && (rc = dstep.alloc((size_t)nb))) return rc;
+        if (!h_dstep) HW_TRY(hipHostMalloc((void**)&h_dstep, sizeof(DevStep) * nb, hipHostMallocDefault));
+        if (src_on && !tsig_d.p && (rc = tsig_d.upload(tsig, stream))) return rc;
+        if (!gev_a) {
+            HW_TRY(hipEventCreate(&gev_a));
+            HW_TRY(hipEventCreate(&gev_b));
+        }
+        if ((rc = collect_timing())) return rc;
+        HW_TRY(hipStreamSynchronize(stream));
+        gA = y.p;
+        gB = ynew.p;
+        HW_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        capturing = true;
+        rc = attempt_body_g();
+        capturing = false;
+        hipGraph_t gr = nullptr;
+        hipError_t e = hipStreamEndCapture(stream, &gr);
+        if (rc) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return rc;
+        }
+        HW_TRY(e);
+        graph = gr;
+        HW_TRY(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        return HW_OK;
+    }
+
+    int graph_begin(const double* t, double t1, const double* h) override {
+        if (brk_on) {
+            set_error("device step control (HW_STEP_GRAPH) needs breaking off");
+            return HW_ERR_ARG;
+        }
+        int rc;
+        if (!gexec && (rc = capture_attempt())) return rc;
+        if (!((y.p == gA && ynew.p == gB) || (y.p == gB && ynew.p == gA))) {
+            set_error("state buffers differ from the captured ones");
+            return HW_ERR_HIP;
+        }
+        for (int z = 0; z < nb; ++z) {
+            DevStep d{};
+            d.t = t[z]; d.h = h[z]; d.t1 = t1;
+            d.eps_abs = eps_abs; d.eps_rel = eps_rel;
+            d.cur = nb == 1 ? (y.p == gB ? 1 : 0) : mcur[z];
+            d.done = t[z] < t1 ? 0 : 1;
+            h_dstep[z] = d;
+        }
+        g_att = 0;
+        HW_TRY(hipMemcpyAsync(dstep.p, h_dstep, sizeof(DevStep) * nb, hipMemcpyHostToDevice, stream));
+        return HW_OK;
+    }
+
+    int graph_launch() override {
+        // about as many attempts as accepted steps remain at the current h
+        // of the member with the most left
+        double rem = 0.0;
+        for (int z = 0; z < nb; ++z) {
+            const DevStep& d = h_dstep[z];
+            if (!d.done) rem = std::max(rem, (d.t1 - d.t) / d.h);
+        }
+        const int nl = rem >= 64.0 ? 64 : rem >= 1.0 ? (int)rem : 1;
+        HW_TRY(hipEventRecord(gev_a, stream));
+        for (int i = 0; i < nl; ++i) HW_TRY(hipGraphLaunch(gexec, stream));
+        HW_TRY(hipEventRecord(gev_b, stream));
+        HW_TRY(hipMemcpyAsync(h_dstep, dstep.p, sizeof(DevStep) * nb, hipMemcpyDeviceToHost, stream));
+        return HW_OK;
+    }
+
+    // rhs_ms_total then covers whole attempts (stages, norm and control
+    // too) of all members; rhs_evals counts member evaluations
+    int graph_poll(bool* done) override {
+        HW_TRY(hipStreamSynchronize(stream));
+        long long att = 0;
+        bool all = true;
+        for (int z = 0; z < nb; ++z) {
+            att += h_dstep[z].attempts;
+            all = all && h_dstep[z].done != 0;
+        }
+        const long long natt = att - g_att;
+        g_att = att;
+        st.rhs_evals += 6 * natt;
+        float ms = 0.f;
+        HW_TRY(hipEventElapsedTime(&ms, gev_a, gev_b));
+        st.rhs_ms_total += ms;
+        st.rhs_timed += 6 * natt;
+        *done = all;
+        return HW_OK;
+    }
+
+    int graph_finish(double* t, double* h) override {
+        int err = -1;
+        for (int z = 0; z < nb; ++z) {
+            const DevStep& d = h_dstep[z];
+            st.steps_accepted += d.accepted;
+            st.steps_rejected += d.rejected;
+            t[z] = d.t;
+            h[z] = d.h;
+            if (d.err && err < 0) err = z;
+            if (nb > 1) mcur[z] = d.cur;
+        }
+        if (nb == 1) {
+            y.p = h_dstep[0].cur ? gB : gA;
+            ynew.p = h_dstep[0].cur ? gA : gB;
+            tprev = t[0];
+        }
+        if (err >= 0) {
+            set_error("member %d: step size underflow at t=%g", err, h_dstep[err].t);
+            return HW_ERR_STEP;
+        }
+        return HW_OK;
+    }
+
+    // members: the state of member z lives in the buffer mcur[z] names
+    double2* member_ptr(int z) {
+        double2* base = (mcur[z] ? ynew.p : y.p);
+        return base + (size_t)z * 3 * H;
+    }
+    int set_member_dev(const double* d_full, int z) override {
+        if (z < 0 || z >= nb) { set_error("member %d out of range", z); return HW_ERR_ARG; }
+        dim3 b(32, 8), gi((NXH + 31) / 32, (NY + 31) / 32, 3);
+        k_full_to_half<double2><<<gi, b, 0, stream>>>(d_full, member_ptr(z), NX, NY);
+        HW_LAUNCH_CHECK();
+        HW_TRY(hipStreamSynchronize(stream));
+        return HW_OK;
+    }
+    int get_member_dev(double* d_full, int z) override {
+        if (z < 0 || z >= nb) { set_error("member %d out of range", z); return HW_ERR_ARG; }
+        dim3 b(32, 8), go((NX + 31) / 32, (NY + 31) / 32, 3);
+        k_half_to_full<double2><<<go, b, 0, stream>>>(member_ptr(z), d_full, NX, NY);
+        HW_LAUNCH_CHECK();
+        HW_TRY(hipStreamSynchronize(stream));
+        return HW_OK;
     }
 
     int fields_dev(double* eta, double* u, double* v) override {

@@ -38,8 +38,14 @@ def initial_state(eta0: np.ndarray, u0: np.ndarray | None = None, v0: np.ndarray
 
 
 class Hawassi2D:
-    def __init__(self, model: Model2D, precision: int = _lib.HW_PREC_FP32):
+    """One device context of the 2-D path; with members > 1 it holds that many
+    states of the model, advanced together (hw_ctx_create_2d_members):
+    set_state / get_state then take / return arrays [members][6 nx ny] and
+    integrate_members advances them."""
+
+    def __init__(self, model: Model2D, precision: int = _lib.HW_PREC_FP32, members: int = 1):
         self.lib = _lib.load()
+        self.members = members
         self.model = model
         self.precision = precision
         self.nx, self.ny = model.nx, model.ny
@@ -89,7 +95,11 @@ class Hawassi2D:
         c.model = int(model.order)
         self._cfg, self._keep = c, keep
         h = ct.c_void_p()
-        _lib.check(self.lib.hw_ctx_create_2d(ct.byref(c), ct.byref(h)), "hw_ctx_create_2d")
+        if members == 1:
+            _lib.check(self.lib.hw_ctx_create_2d(ct.byref(c), ct.byref(h)), "hw_ctx_create_2d")
+        else:
+            _lib.check(self.lib.hw_ctx_create_2d_members(ct.byref(c), int(members), ct.byref(h)),
+                       "hw_ctx_create_2d_members")
         self.ctx = h
         self._keep = None   # data now lives on the device
 
@@ -114,12 +124,23 @@ class Hawassi2D:
 
     def set_state(self, y: np.ndarray):
         y = np.ascontiguousarray(y, dtype=np.float64)
+        if y.size != self.members * self.dim:
+            raise ValueError(f"state has {y.size} values, the context holds {self.members} x {self.dim}")
         _lib.check(self.lib.hw_set_state(self.ctx, _lib.ptr(y)), "hw_set_state")
 
     def get_state(self) -> np.ndarray:
-        y = np.empty(self.dim)
+        y = np.empty(self.members * self.dim)
         _lib.check(self.lib.hw_get_state(self.ctx, _lib.ptr(y)), "hw_get_state")
-        return y
+        return y if self.members == 1 else y.reshape(self.members, self.dim)
+
+    def integrate_members(self, t, t1: float, h):
+        """Advance every member to t1 (hw_integrate_members); t and h per
+        member; returns the new (t, h) lists."""
+        tt = np.ascontiguousarray(np.broadcast_to(np.asarray(t, dtype=np.float64), (self.members,)))
+        hh = np.ascontiguousarray(np.broadcast_to(np.asarray(h, dtype=np.float64), (self.members,)))
+        _lib.check(self.lib.hw_integrate_members(self.ctx, _lib.ptr(tt), float(t1), _lib.ptr(hh)),
+                   "hw_integrate_members")
+        return list(tt), list(hh)
 
     def evolve_apply(self, t: float, t1: float, h: float):
         tt, hh = ct.c_double(t), ct.c_double(h)
@@ -155,6 +176,13 @@ class Hawassi2D:
 
     def reset_stats(self):
         _lib.check(self.lib.hw_reset_stats(self.ctx), "hw_reset_stats")
+
+    def set_step_mode(self, graph: bool):
+        """integrate / integrate_batch with the step control on the device and
+        each attempted step replayed from a HIP graph (True), or on the host
+        (False, the default); see hw_set_step_mode."""
+        _lib.check(self.lib.hw_set_step_mode(self.ctx, _lib.HW_STEP_GRAPH if graph else _lib.HW_STEP_HOST),
+                   "hw_set_step_mode")
 
     def sync(self):
         _lib.check(self.lib.hw_sync(self.ctx), "hw_sync")

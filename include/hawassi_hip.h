@@ -143,6 +143,42 @@ int hw_integrate(void *ctx, double *t, double t1, double *h);
  * realization per process (2d/fun_odesolver.c:2641-2647). */
 int hw_integrate_batch(void *const *ctxs, int n, double *t, double t1, double *h);
 
+/* How hw_integrate and hw_integrate_batch take their steps.  HW_STEP_HOST
+ * (the default): evolve_apply on the host, one stream synchronisation per
+ * attempted step to read the error norm.  HW_STEP_GRAPH: the step control
+ * runs on the device (the same rkf45 + control_y arithmetic, in a kernel) and
+ * one attempted step is a HIP graph captured on first use and replayed
+ * without the host, which only checks every few dozen attempts whether t1
+ * has been reached; K0 is evaluated in every attempt, so a rejected step
+ * costs one evaluation more than on the host path.  It takes the same steps
+ * as HW_STEP_HOST up to the last bit of pow() in the step-size update.  For
+ * small grids, where a step's launches outlast its kernels, it is several
+ * times faster.  Breaking (host crest bookkeeping) is not supported in graph
+ * mode: hw_integrate then returns HW_ERR_ARG.  A batch runs in graph mode
+ * when every context in it is set to it.  No counterpart in the reference
+ * (GSL's evolve_apply runs on the host, 2d/fun_odesolver.c:2641-2647). */
+#define HW_STEP_HOST 0
+#define HW_STEP_GRAPH 1
+int hw_set_step_mode(void *ctx, int mode);
+
+/* A context holding `members` states of one model (for example the slices
+ * of a parareal iteration, or realizations of one sea state on one grid),
+ * advanced together: every kernel of a right-hand side covers all members
+ * in one launch (grid.z = member), each member keeps its own t, h and step
+ * control on the device, and one attempted step of all members is one HIP
+ * graph replay.  Small grids, whose single-state launches fill a fraction of
+ * the GPU, gain the most.  Needs breaking, the influx source and a third
+ * reference depth off.  hw_set_state / hw_get_state then take / return
+ * `members` full 6-plane states one after the other; hw_integrate_members
+ * advances every member to t1 (t[], h[] per member, as hw_integrate_batch);
+ * hw_get_stats sums over members.  The single-state entry points (hw_rhs,
+ * hw_evolve_apply, hw_integrate, hw_integrate_batch, hw_get_fields) return
+ * HW_ERR_ARG on such a context.  Each member takes the steps it takes in a
+ * context of its own in HW_STEP_GRAPH mode.  No counterpart in the reference
+ * (one realization per process, 2d/fun_odesolver.c:2641-2647). */
+int hw_ctx_create_2d_members(const hw_config_2d *cfg, int members, void **ctx);
+int hw_integrate_members(void *ctx, double *t, double t1, double *h);
+
 /* Physical eta, u, v [ny*nx] of the current state, as the output IFFTs of
  * ode_solver_uv (2d/fun_odesolver.c:2663-2666); any pointer may be NULL. */
 int hw_get_fields(void *ctx, double *eta, double *u, double *v);
