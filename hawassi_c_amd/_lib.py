@@ -37,6 +37,8 @@ class HwConfig2D(ct.Structure):
         ("t0", ct.c_double),
         ("model", ct.c_int),
         ("gam_mid", _dp), ("csq_mid", _dp), ("om2_mid", _dp),
+        ("full_grid", ct.c_int), ("wall", ct.c_int), ("refl_one", ct.c_int),
+        ("grad_wall_x", _dp), ("grad_wall_y", _dp),
     ]
 
 
@@ -74,7 +76,7 @@ class HwFnoConfig(ct.Structure):
 EXPORTS = [
     "hw_ctx_create_2d", "hw_ctx_destroy", "hw_rhs", "hw_rhs_device", "hw_set_state", "hw_get_state",
     "hw_evolve_apply", "hw_integrate", "hw_integrate_batch", "hw_set_step_mode", "hw_ctx_create_2d_members", "hw_integrate_members", "hw_get_fields", "hw_get_stats", "hw_reset_stats", "hw_sync",
-    "hw_fft2_forward", "hw_fft2_inverse", "hw_fft2_forward_noalloc", "hw_supported", "hw_last_error", "hw_version",
+    "hw_fft2_forward", "hw_fft2_inverse", "hw_fft2_forward_noalloc", "hw_supported", "hw_supported_full", "hw_last_error", "hw_version",
     "hw_set_tprev", "hw_get_breaking",
     "hw1d_create", "hw1d_destroy", "hw1d_rhs", "hw1d_rhs_at", "hw1d_set_state", "hw1d_get_state",
     "hw1d_set_time", "hw1d_get_time", "hw1d_integrate", "hw1d_set_round_limit", "hw1d_get_breaking",
@@ -120,6 +122,7 @@ def load(path: Path | None = None) -> ct.CDLL:
     lib.hw_fft2_inverse.argtypes = [ct.c_int, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp]
     lib.hw_fft2_forward_noalloc.argtypes = [ct.c_int, ct.c_int, ct.c_int, vp, vp, vp, vp]
     lib.hw_supported.argtypes = [ct.c_int, ct.c_int, ct.c_int]
+    lib.hw_supported_full.argtypes = [ct.c_int, ct.c_int, ct.c_int]
     lib.hw1d_create.argtypes = [ct.POINTER(HwConfig1D), ct.c_int, ct.c_int, ct.POINTER(vp)]
     lib.hw1d_destroy.argtypes = [vp]
     lib.hw1d_destroy.restype = None

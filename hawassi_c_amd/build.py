@@ -39,7 +39,7 @@ NT1D = (64, 128, 256, 512, 1024)
 def _units():
     gen = BUILD / "gen"
     gen.mkdir(parents=True, exist_ok=True)
-    units = [CSRC / "capi.hip", CSRC / "capi1d.hip", CSRC / "fno.hip"]
+    units = [CSRC / "capi.hip", CSRC / "capi1d.hip", CSRC / "fno.hip", CSRC / "full2d.hip"]
     for prec in (0, 1):
         t = "double" if prec == 0 else "float"
         for nt in NT1D:
@@ -70,9 +70,27 @@ def _units():
     return units
 
 
+def _includes(src: Path, seen: set) -> None:
+    """The project headers src includes, transitively (quoted includes)."""
+    for line in src.read_text().splitlines():
+        line = line.strip()
+        if not line.startswith("#include \""):
+            continue
+        name = line.split('"')[1]
+        for base in (src.parent, CSRC, PKG.parent / "include"):
+            f = (base / name).resolve()
+            if f.exists():
+                if f not in seen:
+                    seen.add(f)
+                    _includes(f, seen)
+                break
+
+
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.stem + ".o")
-    deps = [src] + sorted(CSRC.glob("*.h")) + [PKG.parent / "include" / "hawassi_hip.h"]
+    seen: set = set()
+    _includes(src, seen)
+    deps = [src] + sorted(seen)
     if obj.exists() and all(obj.stat().st_mtime >= d.stat().st_mtime for d in deps):
         return obj
     cmd = [HIPCC, *FLAGS, "-I", str(CSRC), "-I", str(PKG.parent / "include"), "-c", str(src), "-o", str(obj)]

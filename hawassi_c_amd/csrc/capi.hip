@@ -26,6 +26,12 @@ std::vector<Entry>& registry() {
     return r;
 }
 
+// the full-grid path (full2d.hip)
+Solver2D* make_solver_full(int precision);
+bool full_supported(int nx, int ny, int precision);
+int fft2_full_forward(int nx, int ny, int prec, const double* phys, double* re, double* im);
+int fft2_full_inverse(int nx, int ny, int prec, const double* re, const double* im, double* phys, double* nyq_im);
+
 static const Entry* find_entry(int prec, int nx, int ny) {
     for (const Entry& e : registry())
         if (e.prec == prec && e.nx == nx && e.ny == ny) return &e;
@@ -81,17 +87,25 @@ const char* hw_version(void) { return "hawassi_hip 0.1 (gfx950)"; }
 
 int hw_supported(int nx, int ny, int precision) { return find_entry(precision, nx, ny) ? 1 : 0; }
 
+int hw_supported_full(int nx, int ny, int precision) {
+    return (precision == HW_PREC_FP64 || precision == HW_PREC_FP32) && full_supported(nx, ny, precision) ? 1 : 0;
+}
+
 int hw_ctx_create_2d(const hw_config_2d* cfg, void** ctx) {
     if (!cfg || !ctx) { set_error("null argument"); return HW_ERR_ARG; }
     *ctx = nullptr;
     const Entry* e = find_entry(cfg->precision, cfg->nx, cfg->ny);
-    if (!e) {
-        set_error("unsupported grid %dx%d precision %d (no compiled kernels)", cfg->nx, cfg->ny, cfg->precision);
+    // the fused path has no kernels for this grid, the caller asks for the
+    // full grid, or the bottom is flat with walls: the full-grid path
+    const bool full = cfg->full_grid || !e || (cfg->flat && cfg->wall);
+    if (full && !hw_supported_full(cfg->nx, cfg->ny, cfg->precision)) {
+        set_error("unsupported grid %dx%d precision %d (the full-grid path takes sides that factor into 2, 3 and 5, "
+                  "at most 4096 in FP64 and 8192 in FP32)", cfg->nx, cfg->ny, cfg->precision);
         return HW_ERR_ARG;
     }
     int rc = check_device();
     if (rc) return rc;
-    Solver2D* s = e->make();
+    Solver2D* s = full ? make_solver_full(cfg->precision) : e->make();
     rc = s->init(cfg);
     if (rc) { delete s; return rc; }
     *ctx = s;
@@ -101,6 +115,10 @@ int hw_ctx_create_2d(const hw_config_2d* cfg, void** ctx) {
 int hw_ctx_create_2d_members(const hw_config_2d* cfg, int members, void** ctx) {
     if (!cfg || !ctx || members < 1) { set_error("bad argument"); return HW_ERR_ARG; }
     *ctx = nullptr;
+    if (cfg->full_grid || (cfg->flat && cfg->wall)) {
+        set_error("contexts with several members run on the fused path (full_grid = 0, no walls on a flat bottom)");
+        return HW_ERR_ARG;
+    }
     const Entry* e = find_entry(cfg->precision, cfg->nx, cfg->ny);
     if (!e) {
         set_error("unsupported grid %dx%d precision %d (no compiled kernels)", cfg->nx, cfg->ny, cfg->precision);
@@ -409,20 +427,26 @@ int hw_sync(void* ctx) {
 
 int hw_fft2_forward(int nx, int ny, int precision, const double* phys, double* re, double* im) {
     const Entry* e = find_entry(precision, nx, ny);
-    if (!e) { set_error("unsupported grid %dx%d precision %d", nx, ny, precision); return HW_ERR_ARG; }
+    if (!e && !hw_supported_full(nx, ny, precision)) {
+        set_error("unsupported grid %dx%d precision %d", nx, ny, precision);
+        return HW_ERR_ARG;
+    }
     if (!phys || !re || !im) { set_error("null argument"); return HW_ERR_ARG; }
     int rc = check_device();
     if (rc) return rc;
-    return e->fwd(phys, re, im);
+    return e ? e->fwd(phys, re, im) : fft2_full_forward(nx, ny, precision, phys, re, im);
 }
 
 int hw_fft2_inverse(int nx, int ny, int precision, const double* re, const double* im, double* phys, double* nyq_im) {
     const Entry* e = find_entry(precision, nx, ny);
-    if (!e) { set_error("unsupported grid %dx%d precision %d", nx, ny, precision); return HW_ERR_ARG; }
+    if (!e && !hw_supported_full(nx, ny, precision)) {
+        set_error("unsupported grid %dx%d precision %d", nx, ny, precision);
+        return HW_ERR_ARG;
+    }
     if (!phys || !re || !im) { set_error("null argument"); return HW_ERR_ARG; }
     int rc = check_device();
     if (rc) return rc;
-    return e->inv(re, im, phys, nyq_im);
+    return e ? e->inv(re, im, phys, nyq_im) : fft2_full_inverse(nx, ny, precision, re, im, phys, nyq_im);
 }
 
 int hw_fft2_forward_noalloc(int nx, int ny, int precision, const double* d_phys, double* d_re, double* d_im,

@@ -13,6 +13,7 @@
 #include "../../include/hawassi_hip.h"
 #include "aux_kernels.h"
 #include "breaking2d.h"
+#include "crest2d.h"
 #include "rhs2d_3depth.h"
 #include <algorithm>
 #include <climits>
@@ -58,6 +59,27 @@ inline int bsearch_interval(const std::vector<double>& x, double t) {
     return ilo;
 }
 
+// natural cubic spline (gsl cspline): c_i = M_i / 2 for the sample times x
+inline void spline_half_second(const double* x, const double* yv, int n, double* cc) {
+    for (int i = 0; i < n; ++i) cc[i] = 0.0;
+    if (n < 3) return;
+    int m = n - 2;
+    std::vector<double> diag(m), off(m), rhs(m);
+    for (int i = 0; i < m; ++i) {
+        double h0 = x[i + 1] - x[i], h1 = x[i + 2] - x[i + 1];
+        diag[i] = 2.0 * (h0 + h1);
+        off[i] = h1;
+        rhs[i] = 3.0 * ((yv[i + 2] - yv[i + 1]) / h1 - (yv[i + 1] - yv[i]) / h0);
+    }
+    for (int i = 1; i < m; ++i) {
+        double w = off[i - 1] / diag[i - 1];
+        diag[i] -= w * off[i - 1];
+        rhs[i] -= w * rhs[i - 1];
+    }
+    cc[m] = rhs[m - 1] / diag[m - 1];
+    for (int i = m - 2; i >= 0; --i) cc[i + 1] = (rhs[i] - off[i] * cc[i + 2]) / diag[i];
+}
+
 struct Solver2D {
     virtual ~Solver2D() {}
     virtual int init(const hw_config_2d* c) = 0;
@@ -91,28 +113,6 @@ struct Solver2D {
     double* d_full_b = nullptr;
 };
 
-template <typename T> struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    ~DevBuf() { if (p) (void)hipFree(p); }
-    int alloc(size_t count) {
-        n = count;
-        hipError_t e = hipMalloc((void**)&p, sizeof(T) * (count ? count : 1));
-        if (e != hipSuccess) {
-            set_error("hipMalloc(%zu bytes) failed: %s", sizeof(T) * count, hipGetErrorString(e));
-            p = nullptr;
-            return HW_ERR_HIP;
-        }
-        return HW_OK;
-    }
-    int upload(const std::vector<T>& h, hipStream_t s) {
-        int rc = alloc(h.size());
-        if (rc) return rc;
-        HW_TRY(hipMemcpyAsync(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
-        return HW_OK;
-    }
-};
-
 template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
     using C = typename ctype<T>::type;
     using XG = XGeo<T, NX>;
@@ -139,33 +139,17 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
     unsigned long long* h_rmax = nullptr;
     Rhs2DArgs<T> args{};
     // ---- breaking state
-    struct CrestRec {                      // CBvar without the node arrays
-        int xindex, yindex, xNow, yNow, kwadran;
-        double Ucrest, theta, time;
-    };
     int brk_on = 0;
-    double KBC = 0, TC = 0, Tchar = 0, delb = 1.5, MaxEtaInit = 0, Tstar = 0, Rsearch = 0, Ucur = 0;
+    double delb = 1.5;
     double dxg = 1, dyg = 1;
-    DevBuf<T> pbc, depthv, Bf, flux, sbx, sby, s_eta;
-    DevBuf<double> xg, yg;
-    DevBuf<int> owner, pkcount;            // pkcount[0]: peaks, pkcount[1]: crest candidates
-    DevBuf<BrkPeak> peaks;
-    DevBuf<BrkCand> cand;
+    CrestMachine<T> cm;                    // criterion, crest lists, B (crest2d.h)
+    DevBuf<T> flux, sbx, sby, s_eta;
     DevBuf<C> hilx;                        // Hilbert transform of eta: x-intermediate
     DevBuf<T> hilp;                        // ... and physical
-    static constexpr int CAPH = 256;       // candidates copied to the host with the counts
-    int* h_cnt = nullptr;
-    BrkCand* h_cand = nullptr;
-    hipEvent_t ev_brk = nullptr;
-    bool b_zero = true, sb_zero = true;    // Bf / Sb^ known to be all zero
+    bool sb_zero = true;                   // Sb^ known to be all zero
     bool last_gated = false;               // last RHS call skipped breaking (tprev >= t)
-    DevBuf<BrkCrest> crests;
     DevBuf<C> sbh;                         // Sb_x^, Sb_y^ half spectra (2H)
     DevBuf<C> col0;                        // pre-mask eta derivative, column kx = 0 (NY)
-    int maxpk = 0, maxcr = 0;
-    int flagbr = 0;
-    std::vector<CrestRec> cbprev;          // CBPrev entries (count = size)
-    long brk_calls = 0;
     bool flat = false;
     bool hs1 = false;   // HS1 dynamics (linear AB1 right-hand side)
     bool hs3 = false;   // HS3 dynamics (AB3: the AB2 terms plus the third-order ones)
@@ -195,9 +179,6 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         if (h_dstep) (void)hipHostFree(h_dstep);
         if (gev_a) (void)hipEventDestroy(gev_a);
         if (gev_b) (void)hipEventDestroy(gev_b);
-        if (h_cnt) (void)hipHostFree(h_cnt);
-        if (h_cand) (void)hipHostFree(h_cand);
-        if (ev_brk) (void)hipEventDestroy(ev_brk);
         if (h_rmax) (void)hipHostFree(h_rmax);
         for (int i = 0; i < NEV; ++i) {
             if (ev_a[i]) (void)hipEventDestroy(ev_a[i]);
@@ -308,30 +289,14 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         if (nb < 1) { set_error("members must be >= 1"); return HW_ERR_ARG; }
         if ((rc = s_eta.alloc((size_t)NY * nb))) return rc;
         if (brk_on) {
-            if (!c->pbchar || !c->depth || !c->x || !c->y || c->rsearch <= 0) {
-                set_error("breaking needs pbchar, depth, x, y and rsearch > 0");
-                return HW_ERR_ARG;
-            }
-            KBC = c->KBC; TC = c->TC; Tchar = c->Tchar; delb = c->delb; MaxEtaInit = c->max_eta_init;
-            Tstar = c->tstar; Rsearch = c->rsearch; Ucur = c->current;
-            dxg = c->x[1] - c->x[0]; dyg = c->y[1] - c->y[0];
-            std::vector<double> xv(c->x, c->x + NX), yv(c->y, c->y + NY);
-            if ((rc = pbc.upload(to_phys(c->pbchar), stream)) || (rc = depthv.upload(to_phys(c->depth), stream)) ||
-                (rc = xg.upload(xv, stream)) || (rc = yg.upload(yv, stream)))
+            if ((rc = cm.init(c, NX, NY, stream))) return rc;
+            delb = c->delb;
+            dxg = cm.dxg; dyg = cm.dyg;
+            if ((rc = flux.alloc(N)) || (rc = sbx.alloc(N)) || (rc = sby.alloc(N)) || (rc = hilx.alloc(H)) ||
+                (rc = hilp.alloc(N)) || (rc = sbh.alloc(2 * H)) || (rc = col0.alloc(NY)))
                 return rc;
-            maxpk = (int)std::max<size_t>(4096, N / 8);
-            maxcr = 65536;
-            if ((rc = Bf.alloc(N)) || (rc = flux.alloc(N)) || (rc = sbx.alloc(N)) || (rc = sby.alloc(N)) ||
-                (rc = owner.alloc(N)) || (rc = pkcount.alloc(2)) || (rc = peaks.alloc(maxpk)) || (rc = cand.alloc(maxpk)) ||
-                (rc = hilx.alloc(H)) || (rc = hilp.alloc(N)) ||
-                (rc = crests.alloc(maxcr)) || (rc = sbh.alloc(2 * H)) || (rc = col0.alloc(NY)))
-                return rc;
-            k_fill<T><<<grid1(N), 256, 0, stream>>>(Bf.p, (T)0, N);
             k_fill<C><<<grid1(2 * H), 256, 0, stream>>>(sbh.p, cmk_h((T)0, (T)0), 2 * H);
             HW_LAUNCH_CHECK();
-            HW_TRY(hipHostMalloc((void**)&h_cnt, 2 * sizeof(int), hipHostMallocDefault));
-            HW_TRY(hipHostMalloc((void**)&h_cand, CAPH * sizeof(BrkCand), hipHostMallocDefault));
-            HW_TRY(hipEventCreateWithFlags(&ev_brk, hipEventDisableTiming));
         }
         if (nb > 1 && (brk_on || src_on || three)) {
             set_error("a context with several members needs breaking, the influx source and a third reference "
@@ -372,26 +337,6 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
     }
 
     static C cmk_h(T a, T b) { C r; r.x = a; r.y = b; return r; }
-
-    static void spline_half_second(const double* x, const double* yv, int n, double* cc) {
-        for (int i = 0; i < n; ++i) cc[i] = 0.0;
-        if (n < 3) return;
-        int m = n - 2;
-        std::vector<double> diag(m), off(m), rhs(m);
-        for (int i = 0; i < m; ++i) {
-            double h0 = x[i + 1] - x[i], h1 = x[i + 2] - x[i + 1];
-            diag[i] = 2.0 * (h0 + h1);
-            off[i] = h1;
-            rhs[i] = 3.0 * ((yv[i + 2] - yv[i + 1]) / h1 - (yv[i + 1] - yv[i]) / h0);
-        }
-        for (int i = 1; i < m; ++i) {
-            double w = off[i - 1] / diag[i - 1];
-            diag[i] -= w * off[i - 1];
-            rhs[i] -= w * rhs[i - 1];
-        }
-        cc[m] = rhs[m - 1] / diag[m - 1];
-        for (int i = m - 2; i >= 0; --i) cc[i + 1] = (rhs[i] - off[i] * cc[i + 2]) / diag[i];
-    }
 
     // accumulate the recorded RHS durations into st.rhs_ms_total
     int collect_timing() {
@@ -561,148 +506,36 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
     int breaking_detect() {
         const dim3 gx(NY / XG::RR), bx(XG::NT);
         const dim3 gy((NXH + YG::CC - 1) / YG::CC), by(YG::NT);
-        k_fill<int><<<1, 64, 0, stream>>>(pkcount.p, 0, 2);
-        k_brk_peaks<T><<<dim3((NX + 255) / 256, NY), 256, 0, stream>>>(eta_p.p, pbc.p, MaxEtaInit, NX, NY, peaks.p, pkcount.p, maxpk);
-        k_brk_hilb_y<T, NX, NY><<<gy, by, 0, stream>>>(args, hilx.p, pkcount.p);
-        k_t_xinv<T, NX, NY><<<gx, bx, 0, stream>>>(hilx.p, hilp.p, nullptr, twx.p, pkcount.p);
-        k_brk_crit<T><<<64, 256, 0, stream>>>(peaks.p, pkcount.p, maxpk, eta_p.p, hilp.p, u_p.p, v_p.p, depthv.p, Bf.p,
-                                             dxg, dyg, Ucur, KBC, NX, NY, cand.p);
+        int rc;
+        if ((rc = cm.detect_begin(stream, eta_p.p))) return rc;
+        k_brk_hilb_y<T, NX, NY><<<gy, by, 0, stream>>>(args, hilx.p, cm.pkcount.p);
+        k_t_xinv<T, NX, NY><<<gx, bx, 0, stream>>>(hilx.p, hilp.p, nullptr, twx.p, cm.pkcount.p);
         HW_LAUNCH_CHECK();
-        HW_TRY(hipMemcpyAsync(h_cnt, pkcount.p, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
-        HW_TRY(hipMemcpyAsync(h_cand, cand.p, CAPH * sizeof(BrkCand), hipMemcpyDeviceToHost, stream));
-        HW_TRY(hipEventRecord(ev_brk, stream));
-        return HW_OK;
+        return cm.detect_end(stream, eta_p.p, hilp.p, u_p.p, v_p.p);
     }
 
     int breaking(double t) {
-        brk_calls++;
         const dim3 gx(NY / XG::RR), bx(XG::NT);
         const dim3 gy((NXH + YG::CC - 1) / YG::CC), by(YG::NT);
-        HW_TRY(hipEventSynchronize(ev_brk));
-        const int npk = h_cnt[0], ncand = h_cnt[1];
-        if (npk > maxpk) { set_error("breaking: %d peaks exceed the buffer of %d", npk, maxpk); return HW_ERR_ARG; }
-        // ---- criterion: candidate crests in the reference's peak order
-        std::vector<CrestRec> cb;
-        std::vector<int> cb_prevnode;
-        if (ncand > 0) {
-            std::vector<BrkCand> hc(ncand);
-            std::copy(h_cand, h_cand + std::min(ncand, CAPH), hc.begin());
-            if (ncand > CAPH)
-                HW_TRY(hipMemcpy(hc.data() + CAPH, cand.p + CAPH, sizeof(BrkCand) * (ncand - CAPH), hipMemcpyDeviceToHost));
-            // fun_peaks order: interior x-major, then x=0, x=nx-1, y=0, y=ny-1 edges
-            auto key = [](const BrkCand& p) -> long long {
-                int cls;
-                long long a, b;
-                if (p.x >= 1 && p.x <= NX - 2 && p.y >= 1 && p.y <= NY - 2) { cls = 0; a = p.x; b = p.y; }
-                else if (p.x == 0) { cls = 1; a = p.y; b = 0; }
-                else if (p.x == NX - 1) { cls = 2; a = p.y; b = 0; }
-                else if (p.y == 0) { cls = 3; a = p.x; b = 0; }
-                else { cls = 4; a = p.x; b = 0; }
-                return ((long long)cls << 40) | (a << 20) | b;
-            };
-            std::vector<int> ord(ncand);
-            for (int i = 0; i < ncand; ++i) ord[i] = i;
-            std::sort(ord.begin(), ord.end(), [&](int a, int b) { return key(hc[a]) < key(hc[b]); });
-            for (int i : ord) {
-                CrestRec r;
-                r.xindex = r.xNow = hc[i].x;
-                r.yindex = r.yNow = hc[i].y;
-                r.Ucrest = hc[i].U;
-                r.time = t;
-                r.theta = hc[i].theta;
-                r.kwadran = hc[i].kwadran;
-                cb.push_back(r);
-                cb_prevnode.push_back(hc[i].prevnode);
-            }
-        }
-        // ---- crest list for this call (2d/breaking.c:526-593)
-        const int ncb = (int)cb.size();
-        std::vector<CrestRec> cbnew;
-        if (ncb > 0 && flagbr == 0) {
-            flagbr = 1;
-            cbnew = cb;
-        } else if (ncb == 0 && flagbr == 2) {
-            cbnew = cbprev;
-        } else if (ncb > 0 && flagbr == 2) {
-            cbnew = cbprev;
-            for (int k = 0; k < ncb; ++k) {
-                bool isnew = !cb_prevnode[k];   // fun_check_crest: on a previous break node?
-                for (const CrestRec& p : cbprev)
-                    if (p.xindex == cb[k].xindex && p.yindex == cb[k].yindex) isnew = false;
-                if (isnew) cbnew.push_back(cb[k]);
-            }
-        }
         bool have_b = false;
-        if (flagbr == 1 || flagbr == 2) {
-            const int nnew = (int)cbnew.size();
-            std::vector<CrestRec> next;
-            std::vector<int> term(nnew, 0);
-            if (nnew > 0) {
-                if (nnew > maxcr) { set_error("breaking: %d crests exceed %d", nnew, maxcr); return HW_ERR_ARG; }
-                std::vector<BrkCrest> hcr(nnew);
-                for (int lx = 0; lx < nnew; ++lx) {
-                    const CrestRec& r = cbnew[lx];
-                    BrkCrest& b = hcr[lx];
-                    b.mode = flagbr == 1 ? 1 : 2;
-                    b.xNow = flagbr == 1 ? r.xindex : r.xNow;
-                    b.yNow = flagbr == 1 ? r.yindex : r.yNow;
-                    b.theta = r.theta;
-                    b.kwadran = r.kwadran;
-                    double UI = r.Ucrest, UF = TC * UI, dt = t - r.time, Us;
-                    if (dt >= Tstar) Us = UF;
-                    else if (dt >= 0 && dt < Tstar) Us = UI + (dt / Tstar) * (UF - UI);
-                    else Us = UI;
-                    b.Ustar = Us;
-                    b.UF = UF;
-                    b.found = 0;
-                    b.nb = 0;
-                }
-                k_fill<T><<<grid1(N), 256, 0, stream>>>(Bf.p, (T)0, N);
-                k_fill<int><<<grid1(N), 256, 0, stream>>>(owner.p, INT_MAX, N);
-                HW_TRY(hipMemcpyAsync(crests.p, hcr.data(), sizeof(BrkCrest) * nnew, hipMemcpyHostToDevice, stream));
-                BrkGeom geo{xg.p, yg.p, Rsearch, NX, NY};
-                k_brk_track<T><<<nnew, 256, 0, stream>>>(crests.p, nnew, geo, eta_p.p, u_p.p, v_p.p, pbc.p, owner.p);
-                k_brk_resolve<T><<<nnew, 256, 0, stream>>>(crests.p, nnew, geo, eta_p.p, u_p.p, v_p.p, pbc.p, owner.p, Bf.p);
-                HW_LAUNCH_CHECK();
-                HW_TRY(hipMemcpyAsync(hcr.data(), crests.p, sizeof(BrkCrest) * nnew, hipMemcpyDeviceToHost, stream));
-                HW_TRY(hipStreamSynchronize(stream));
-                for (int lx = 0; lx < nnew; ++lx) {
-                    const BrkCrest& b = hcr[lx];
-                    if (!b.found) { term[lx] = 1; continue; }
-                    CrestRec r = cbnew[lx];
-                    r.xNow = b.xNow; r.yNow = b.yNow; r.theta = b.theta; r.kwadran = b.kwadran;
-                    if (b.nb == 0) {
-                        term[lx] = 1;
-                    } else {
-                        term[lx] = 0;
-                        next.push_back(r);
-                        have_b = true;
-                    }
-                }
-                for (int lx = 0; lx < nnew; ++lx) {
-                    if (term[lx] == 0) { flagbr = 2; break; }
-                    flagbr = 0;
-                }
-            }
-            cbprev = next;
-        }
+        int rc = cm.run(t, stream, eta_p.p, u_p.p, v_p.p, &have_b);
+        if (rc) return rc;
         if (!have_b) {
             // B == 0 everywhere: zero flux, zero source
-            if (!b_zero) k_fill<T><<<grid1(N), 256, 0, stream>>>(Bf.p, (T)0, N);
             if (!sb_zero) k_fill<C><<<grid1(2 * H), 256, 0, stream>>>(sbh.p, cmk_h((T)0, (T)0), 2 * H);
             HW_LAUNCH_CHECK();
-            b_zero = sb_zero = true;
+            sb_zero = true;
             return HW_OK;
         }
-        b_zero = sb_zero = false;
+        sb_zero = false;
         // ---- fun_sbreaking: delphi, flux, clipped gradient, transforms
         C* Xg = args.W[12];
         C* Xd = args.W[13];
         k_brk_g_x<T, NX, NY><<<gx, bx, 0, stream>>>(args, s_eta.p, Xg);
         k_brk_del_y<T, NX, NY><<<gy, by, 0, stream>>>(args, flat ? 1 : 0, Xg, Xd);
-        k_brk_flux_x<T, NX, NY><<<gx, bx, 0, stream>>>(args, flat ? (int)F_AETA : (int)S_AETA, Xd, Bf.p, depthv.p,
+        k_brk_flux_x<T, NX, NY><<<gx, bx, 0, stream>>>(args, flat ? (int)F_AETA : (int)S_AETA, Xd, cm.Bf.p, cm.depth.p,
                                                       (T)(delb * delb), flux.p);
-        k_brk_sb<T><<<dim3((NX + 255) / 256, NY), 256, 0, stream>>>(flux.p, eta_p.p, depthv.p, dxg, dyg, NX, NY, sbx.p, sby.p);
+        k_brk_sb<T><<<dim3((NX + 255) / 256, NY), 256, 0, stream>>>(flux.p, eta_p.p, cm.depth.p, dxg, dyg, NX, NY, sbx.p, sby.p);
         k_brk_sb_x<T, NX, NY><<<gx, bx, 0, stream>>>(args, sbx.p, sby.p, args.W[10], args.W[11]);
         k_brk_sb_y<T, NX, NY><<<gy, by, 0, stream>>>(args, args.W[10], args.W[11], sbh.p, sbh.p + H);
         HW_LAUNCH_CHECK();
@@ -710,7 +543,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
     }
 
     int breaking_state(double* Bout, int* fl) override {
-        if (fl) *fl = flagbr;
+        if (fl) *fl = cm.flagbr;
         if (Bout) {
             // the reference clears B on every call and fills it only when
             // breaking runs (2d/fun_odesolver.c:4581-4585)
@@ -718,7 +551,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
                 for (size_t i = 0; i < N; ++i) Bout[i] = 0.0;
             } else {
                 std::vector<T> hb(N);
-                HW_TRY(hipMemcpyAsync(hb.data(), Bf.p, sizeof(T) * N, hipMemcpyDeviceToHost, stream));
+                HW_TRY(hipMemcpyAsync(hb.data(), cm.Bf.p, sizeof(T) * N, hipMemcpyDeviceToHost, stream));
                 HW_TRY(hipStreamSynchronize(stream));
                 for (size_t i = 0; i < N; ++i) Bout[i] = hb[i];
             }

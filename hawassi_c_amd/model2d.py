@@ -41,9 +41,11 @@ class Hawassi2D:
     """One device context of the 2-D path; with members > 1 it holds that many
     states of the model, advanced together (hw_ctx_create_2d_members):
     set_state / get_state then take / return arrays [members][6 nx ny] and
-    integrate_members advances them."""
+    integrate_members advances them.  full_grid=True (or a grid without
+    fused kernels, or walls on a flat bottom) selects the full-grid path: the
+    reference's own full complex state and transforms, any side 2^a 3^b 5^c."""
 
-    def __init__(self, model: Model2D, precision: int = _lib.HW_PREC_FP32, members: int = 1):
+    def __init__(self, model: Model2D, precision: int = _lib.HW_PREC_FP32, members: int = 1, full_grid: bool = False):
         self.lib = _lib.load()
         self.members = members
         self.model = model
@@ -66,9 +68,12 @@ class Hawassi2D:
         c.aal, c.fbdy = arr("aal", model.aal), arr("fbdy", model.fbdy)
         c.wallchar, c.chi = arr("wallchar", model.wallchar), arr("chi", model.chi)
         if model.flat:
-            if model.wall:
-                raise NotImplementedError("flat bottom with walls (fun_delphiH1hat wall branches) is not on the device path yet")
             c.c2d, c.l2d = arr("c2d", model.c2d), arr("l2d", model.l2d)
+            if model.wall:
+                # fun_delphiH1hat's wall branches: the full-grid path
+                c.wall, c.refl_one = 1, int(model.refl_one)
+                if model.refl_one:
+                    c.grad_wall_x, c.grad_wall_y = arr("gwx", model.grad_wall_x), arr("gwy", model.grad_wall_y)
         else:
             o = model.ops
             c.gam_min, c.gam_plus = arr("gm", o["gam_min"]), arr("gp", o["gam_plus"])
@@ -93,6 +98,10 @@ class Hawassi2D:
             c.x, c.y = arr("x", model.x), arr("y", model.y)
         c.t0 = float(model.meta.get("t0", 0.0))
         c.model = int(model.order)
+        # the full-grid path: asked for, a grid without fused kernels, or walls on a flat bottom
+        self.full_grid = bool(full_grid or (model.flat and model.wall) or
+                              not self.lib.hw_supported(self.nx, self.ny, precision))
+        c.full_grid = int(self.full_grid)
         self._cfg, self._keep = c, keep
         h = ct.c_void_p()
         if members == 1:

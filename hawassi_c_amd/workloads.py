@@ -100,12 +100,15 @@ def _source_shapes(x, y, kx, line_x, meandepth, Lb, Lt):
 
 def harbour(n: int = 2048, seed: int = 4, t_end: float = 60.0, dt_sig: float = 0.1, length: float = 1500.0,
             damp: float = 100.0, nadj: float = 1.0, eps_rel: float = 1e-2, eps_abs: float = 1e-3,
-            with_wall: bool = True, breaking: bool = True, kbc: float = 0.85) -> S.Model2D:
+            with_wall: bool = True, breaking: bool = True, kbc: float = 0.85, ny: int | None = None) -> S.Model2D:
+    """The harbour on an n x n grid (n x ny when ny is given, the y side
+    then ny/n of the x side's length)."""
+    m = ny or n
     x = S.interval_to_vector(0.0, length, n)
-    y = S.interval_to_vector(0.0, length, n)
+    y = S.interval_to_vector(0.0, length * m / n, m)
     dx, dy = x[1] - x[0], y[1] - y[0]
     kx = S.freqspace(x[-1] - x[0], n)
-    ky = S.freqspace(y[-1] - y[0], n)
+    ky = S.freqspace(y[-1] - y[0], m)
     g = rng(seed)
     X, Y = np.meshgrid(x, y)
     depth = 12.0 + 8.0 * np.cos(math.pi * X / length)
@@ -116,7 +119,7 @@ def harbour(n: int = 2048, seed: int = 4, t_end: float = 60.0, dt_sig: float = 0
         depth += sgn[i] * np.exp(-((X - cx[i]) ** 2 + (Y - cy[i]) ** 2) / (2 * 50.0 ** 2))
     # walls: a breakwater 12 m thick at x = 1000 m spanning y in [200, 1300]
     # with a 100 m gap centred at y = 750 m (wall.dat convention: 1 on the wall)
-    wall_ind = np.zeros((n, n))
+    wall_ind = np.zeros((m, n))
     if with_wall:
         on = (np.abs(X - 1000.0) <= 6.0) & (Y >= 200.0) & (Y <= 1300.0) & ~((Y > 700.0) & (Y < 800.0))
         wall_ind[on] = 1.0
@@ -140,7 +143,7 @@ def harbour(n: int = 2048, seed: int = 4, t_end: float = 60.0, dt_sig: float = 0
     sig = directional_line_signal(y, t_sig, hs, tp, 3.3, 15.0, meandepth, seed + 1000)
     ng, sg = _source_shapes(x, y, kx, line_x, meandepth, damp, damp)
     brk = S.breaking_params(hs, tp, kp, cfsa, chi, KBC=kbc, TC=0.2, Tchar=0.5) if breaking else None
-    return S.Model2D(nx=n, ny=n, x=x, y=y, kx=kx, ky=ky, flat=False, cpeak=cpeak, aal=aal, fbdy=fbdy,
+    return S.Model2D(nx=n, ny=m, x=x, y=y, kx=kx, ky=ky, flat=False, cpeak=cpeak, aal=aal, fbdy=fbdy,
                      wallchar=wallchar, chi=chi, depth=depth, wall=with_wall, ops=ops, source_on=True,
                      vertical=True, propagation=2, new_gam=ng, skew_gam=sg, t_sig=t_sig, sig=sig,
                      skew_sig=np.zeros_like(sig), eps_abs=eps_abs, eps_rel=eps_rel, breaking=brk,
@@ -149,15 +152,34 @@ def harbour(n: int = 2048, seed: int = 4, t_end: float = 60.0, dt_sig: float = 0
 
 def short_crested(n: int = 1024, seed: int = 3, t_end: float = 60.0, dt_sig: float = 0.1, length: float = 2000.0,
                   damp: float = 200.0, depth0: float = 30.0, nadj: float = 1.0, eps_rel: float = 1e-2,
-                  eps_abs: float = 1e-3) -> S.Model2D:
+                  eps_abs: float = 1e-3, ny: int | None = None, wall_refl: float | None = None) -> S.Model2D:
+    """The flat short-crested sea on an n x n grid (n x ny when ny is given).
+    wall_refl: a breakwater (wall.dat convention) 3/5 of the way across x,
+    spanning the middle 3/4 of y with a gap at its centre, with that
+    reflection coefficient: the flat-bottom wall branches of
+    fun_delphiH1hat (2d/fun_odesolver.c:2100-2239)."""
+    m = ny or n
     x = S.interval_to_vector(0.0, length, n)
-    y = S.interval_to_vector(0.0, length, n)
+    y = S.interval_to_vector(0.0, length * m / n, m)
+    dx, dy = x[1] - x[0], y[1] - y[0]
     kx = S.freqspace(x[-1] - x[0], n)
-    ky = S.freqspace(y[-1] - y[0], n)
+    ky = S.freqspace(y[-1] - y[0], m)
     KX, KY = kx[None, :], ky[:, None]
-    depth = np.full((n, n), depth0)
+    depth = np.full((m, n), depth0)
     cfsa = S.cfsa2d(x, y, damp, damp, damp, damp)
-    fbdy = S.fbdy2d(x, y, damp, damp, damp, damp, cfsa, wall=False)
+    wall_on = wall_refl is not None
+    wallchar = np.ones((m, n))
+    gwx = gwy = None
+    if wall_on:
+        X, Y = np.meshgrid(x, y)
+        ly = y[-1]
+        on = ((np.abs(X - 0.6 * length) <= 2.5 * dx) & (Y >= 0.125 * ly) & (Y <= 0.875 * ly) &
+              ~(np.abs(Y - 0.5 * ly) < 0.06 * ly))
+        wall_ind = np.where(on, 1.0, 0.0)
+        wallchar, cfsaw = S.wallchar_user(wall_ind, float(wall_refl))
+        cfsa = np.minimum(cfsa, cfsaw)
+        gwx, gwy = S.gradient_clipped(wallchar, dx, dy)   # fun_gradient of wallchar (2d/functions.c:741)
+    fbdy = S.fbdy2d(x, y, damp, damp, damp, damp, cfsa, wall=wall_on)
     aal = S.aal2d(4, kx, ky)
     tp, hs = 10.0, 3.0
     wp = 2 * math.pi / tp
@@ -172,8 +194,9 @@ def short_crested(n: int = 1024, seed: int = 3, t_end: float = 60.0, dt_sig: flo
     t_sig = np.arange(0.0, t_end + 2 * dt_sig, dt_sig)
     sig = directional_line_signal(y, t_sig, hs, tp, 3.3, 10.0, depth0, seed + 1000)
     ng, sg = _source_shapes(x, y, kx, line_x, depth0, damp, damp)
-    return S.Model2D(nx=n, ny=n, x=x, y=y, kx=kx, ky=ky, flat=True, cpeak=cpeak, aal=aal, fbdy=fbdy,
-                     wallchar=np.ones((n, n)), chi=chi, depth=depth, wall=False, c2d=c2d, l2d=l2d,
+    return S.Model2D(nx=n, ny=m, x=x, y=y, kx=kx, ky=ky, flat=True, cpeak=cpeak, aal=aal, fbdy=fbdy,
+                     wallchar=wallchar, chi=chi, depth=depth, wall=wall_on,
+                     refl_one=wall_on and float(wall_refl) == 1.0, grad_wall_x=gwx, grad_wall_y=gwy, c2d=c2d, l2d=l2d,
                      source_on=True, vertical=True, propagation=2, new_gam=ng, skew_gam=sg, t_sig=t_sig, sig=sig,
                      skew_sig=np.zeros_like(sig), eps_abs=eps_abs, eps_rel=eps_rel,
                      meta=dict(name=f"short_crested{n}", tp=tp, hs=hs, kp=kp, lam=lam, line_x=line_x))

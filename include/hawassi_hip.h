@@ -37,7 +37,8 @@ extern "C" {
  * process globals filled by fun_param_init / fun_operator_setup /
  * fun_calcseacharacteristic / fun_area_generation (2d/main.c:119-186). */
 typedef struct hw_config_2d {
-    int nx, ny;               /* powers of two, 32..2048 (nx, ny >= 32) */
+    int nx, ny;               /* fused path: powers of two 32..2048 (hw_supported);
+                                 full-grid path (full_grid): any 2^a 3^b 5^c */
     int precision;            /* HW_PREC_* */
     int flat;                 /* 1: rhs_AB2_2DF_uv (flat bottom), 0: rhs_AB2_2DB_uv (two-depth) */
     double grav;              /* 9.80665 in the reference */
@@ -90,6 +91,22 @@ typedef struct hw_config_2d {
     const double *gam_mid;            /* Oprt.InterpD.Gam_mid [ny*nx] */
     const double *csq_mid;            /* Oprt.InterpD.Csqr2d_mid [ny*nx] */
     const double *om2_mid;            /* Oprt.InterpD.Om2dSq_mid [ny*nx] */
+    /* The full-grid path (no counterpart to choose in the reference, which
+       has only this form): the state and every transform on the full complex
+       grid, as the reference keeps them, so hw_rhs writes the reference's
+       own values in the columns kx > nx/2 and the step control's norm runs
+       over all 6 nx ny values; sides of any length that factors into 2, 3
+       and 5, up to 4096 (FP64) or 8192 (FP32).  Taken when full_grid = 1,
+       for grids the fused path has no kernels for, and for a flat bottom
+       with walls.  One state per context; HW_STEP_GRAPH is not available. */
+    int full_grid;
+    /* walls on a flat bottom (fun_delphiH1hat, 2d/fun_odesolver.c:2100-2239):
+       wall = 1 when the case has walls, refl_one = 1 when their reflection
+       coefficient is 1 (then grad_wall_x/y, the wall characteristic's
+       gradient dominfo.wallchar gradients, are required) */
+    int wall;
+    int refl_one;
+    const double *grad_wall_x, *grad_wall_y;   /* [ny*nx] */
 } hw_config_2d;
 
 typedef struct hw_stats {
@@ -203,7 +220,9 @@ int hw_sync(void *ctx);
  * inverse = Hermitian projection (fftwcomplex_2d_sym, 2d/functions1.c:492-521)
  * then backward DFT / (nx*ny), real part (ifftw_2d_c2r, 2d/functions1.c:523-559).
  * nyq_im (optional, [ny]) receives Im of the inverse's Nyquist column after
- * the y transform, scaled by 1/(nx*ny). */
+ * the y transform, scaled by 1/(nx*ny) -- the s(y) of the projected inverse's
+ * imaginary part (-1)^x s(y).  Grids without fused kernels go through the
+ * full-grid path's transforms (sides 2^a 3^b 5^c, hw_supported_full). */
 int hw_fft2_forward(int nx, int ny, int precision, const double *phys, double *spec_re, double *spec_im);
 int hw_fft2_inverse(int nx, int ny, int precision, const double *spec_re, const double *spec_im, double *phys, double *nyq_im);
 
@@ -367,8 +386,13 @@ double hw_fno_gemm_bytes(void *ctx);
  * at the given precision (HW_FNO_BF16 or HW_FNO_SPLIT) */
 int hw_fno_gemm_check(int M, int N, int K, const float *A, const float *B, float *C, int precision);
 
-/* Sizes this build supports (1 if (nx, ny, precision) has compiled kernels). */
+/* Sizes this build supports (1 if (nx, ny, precision) has compiled kernels
+ * of the fused half-spectrum path). */
 int hw_supported(int nx, int ny, int precision);
+/* 1 if the full-grid path takes (nx, ny, precision): sides that factor into
+ * 2, 3 and 5, at most 4096 (FP64) or 8192 (FP32).  No counterpart in the
+ * reference (FFTW plans any length, 2d/fun_odesolver.c:2885-2955). */
+int hw_supported_full(int nx, int ny, int precision);
 
 const char *hw_last_error(void);
 const char *hw_version(void);

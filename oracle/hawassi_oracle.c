@@ -24,9 +24,9 @@
  *   used by ode_solver_uv    2d/fun_odesolver.c:2565-2647 -> orc_evolve_apply
  *   natural cubic spline (gsl_interp_cspline) eval used at 2d/fun_odesolver.c:901-902
  *
- * The FFT is an iterative radix-2 complex transform (FFTW sign/scale
+ * The FFT is an iterative radix-2 complex transform for powers of two and a
+ * 2/3/5 mixed-radix Stockham transform for other lengths (FFTW sign/scale
  * convention: forward exp(-i..), backward exp(+i..), both unnormalised).
- * Sizes must be powers of two.
  */
 #include <math.h>
 #include <stdlib.h>

@@ -1,31 +1,52 @@
 /* orc_common.h -- pieces shared by the 1-D and 2-D oracle restatements
- * (test infrastructure only): radix-2 complex FFT with FFTW's sign and scale
- * conventions, GSL's natural cubic spline (cspline) and the Fehlberg 4(5)
- * tableau of GSL's rkf45. */
+ * (test infrastructure only): complex FFT (radix 2, and 2/3/5 mixed radix
+ * for other lengths) with FFTW's sign and scale conventions, GSL's natural
+ * cubic spline (cspline) and the Fehlberg 4(5) tableau of GSL's rkf45. */
 #pragma once
 #include <math.h>
 #include <stdlib.h>
 
 /* ------------------------------------------------------------------ FFT -- */
-typedef struct { int n; int logn; double *tw; int *rev; } fftplan;
+/* Powers of two: iterative radix-2.  Other lengths whose prime factors are 2,
+ * 3 and 5 (the reference's non-power-of-two grids, 300x250, 600x300, 2000
+ * nodes; FFTW plans any length): a Stockham autosort over those factors with
+ * every radix-R butterfly a plain R-point DFT and twiddles from cos/sin of
+ * the exact angle.  Sizes with another prime factor are refused (n = 0). */
+typedef struct { int n; int logn; double *tw; int *rev; int nf; int fac[32]; } fftplan;
+
+static int fft_is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
 static fftplan *plan_new(int n)
 {
     fftplan *p = (fftplan *)calloc(1, sizeof(fftplan));
     p->n = n;
-    int l = 0; while ((1 << l) < n) l++;
-    p->logn = l;
-    p->tw = (double *)malloc(sizeof(double) * 2 * (n / 2 > 0 ? n / 2 : 1));
-    for (int k = 0; k < n / 2; k++) {
+    if (fft_is_pow2(n)) {
+        int l = 0; while ((1 << l) < n) l++;
+        p->logn = l;
+        p->tw = (double *)malloc(sizeof(double) * 2 * (n / 2 > 0 ? n / 2 : 1));
+        for (int k = 0; k < n / 2; k++) {
+            double a = -2.0 * M_PI * (double)k / (double)n;
+            p->tw[2 * k] = cos(a);
+            p->tw[2 * k + 1] = sin(a);
+        }
+        p->rev = (int *)malloc(sizeof(int) * n);
+        for (int i = 0; i < n; i++) {
+            int r = 0;
+            for (int b = 0; b < l; b++) if (i & (1 << b)) r |= 1 << (l - 1 - b);
+            p->rev[i] = r;
+        }
+        return p;
+    }
+    int m = n;
+    static const int primes[3] = {5, 3, 2};
+    for (int q = 0; q < 3; q++)
+        while (m % primes[q] == 0 && p->nf < 32) { p->fac[p->nf++] = primes[q]; m /= primes[q]; }
+    if (m != 1) { p->n = 0; return p; }   /* unsupported length */
+    p->tw = (double *)malloc(sizeof(double) * 2 * n);
+    for (int k = 0; k < n; k++) {
         double a = -2.0 * M_PI * (double)k / (double)n;
         p->tw[2 * k] = cos(a);
         p->tw[2 * k + 1] = sin(a);
-    }
-    p->rev = (int *)malloc(sizeof(int) * n);
-    for (int i = 0; i < n; i++) {
-        int r = 0;
-        for (int b = 0; b < l; b++) if (i & (1 << b)) r |= 1 << (l - 1 - b);
-        p->rev[i] = r;
     }
     return p;
 }
@@ -35,6 +56,48 @@ static void plan_free(fftplan *p) { if (!p) return; free(p->tw); free(p->rev); f
 static void fft1(const fftplan *p, double *x, int sign)
 {
     int n = p->n;
+    if (!fft_is_pow2(n)) {
+        /* Stockham: pass with radix R after NS = product of the earlier
+         * radices; y[(j/NS) NS R + k + q NS] = sum_r x[j + r n/R] w^(r k)
+         * e^(sign 2 pi i r q / R), k = j mod NS, w = e^(sign 2 pi i/(NS R)) */
+        double *a = x, *b = (double *)malloc(sizeof(double) * 2 * n);
+        int ns = 1;
+        double vr[5], vi[5];
+        for (int f = 0; f < p->nf; f++) {
+            int R = p->fac[f], M = n / R, step = n / (ns * R);
+            for (int j = 0; j < M; j++) {
+                int k = j % ns;
+                for (int r = 0; r < R; r++) {
+                    double xr = a[2 * (j + r * M)], xi = a[2 * (j + r * M) + 1];
+                    int e = r * k * step;    /* < n */
+                    double wr = p->tw[2 * e], wi = p->tw[2 * e + 1] * (sign < 0 ? 1.0 : -1.0);
+                    vr[r] = xr * wr - xi * wi;
+                    vi[r] = xr * wi + xi * wr;
+                }
+                int base = (j / ns) * ns * R + k;
+                for (int q = 0; q < R; q++) {
+                    double sr = 0.0, si = 0.0;
+                    for (int r = 0; r < R; r++) {
+                        int e = ((r * q) % R) * (n / R);
+                        double wr = p->tw[2 * e], wi = p->tw[2 * e + 1] * (sign < 0 ? 1.0 : -1.0);
+                        sr += vr[r] * wr - vi[r] * wi;
+                        si += vr[r] * wi + vi[r] * wr;
+                    }
+                    b[2 * (base + q * ns)] = sr;
+                    b[2 * (base + q * ns) + 1] = si;
+                }
+            }
+            double *t = a; a = b; b = t;
+            ns *= R;
+        }
+        if (a != x) {
+            for (int i = 0; i < 2 * n; i++) x[i] = a[i];
+            free(a);
+        } else {
+            free(b);
+        }
+        return;
+    }
     for (int i = 0; i < n; i++) {
         int r = p->rev[i];
         if (r > i) {

@@ -37,6 +37,11 @@ def test_supported_sizes_and_version():
     assert lib.hw_supported(1024, 1024, _lib.HW_PREC_FP32) == 1
     assert lib.hw_supported(64, 64, _lib.HW_PREC_FP64) == 1
     assert lib.hw_supported(100, 64, _lib.HW_PREC_FP64) == 0
+    # the full-grid path: sides 2^a 3^b 5^c (the reference's 300x250, 2000x2)
+    assert lib.hw_supported_full(100, 64, _lib.HW_PREC_FP64) == 1
+    assert lib.hw_supported_full(300, 250, _lib.HW_PREC_FP32) == 1
+    assert lib.hw_supported_full(2000, 2, _lib.HW_PREC_FP64) == 1
+    assert lib.hw_supported_full(70, 64, _lib.HW_PREC_FP64) == 0
     assert b"gfx950" in lib.hw_version()
 
 
@@ -44,7 +49,7 @@ def test_bad_arguments_are_errors_not_crashes():
     lib = _lib.load()
     assert lib.hw_ctx_create_2d(None, None) == 1
     c = _lib.HwConfig2D()
-    c.nx, c.ny, c.precision = 100, 64, 0
+    c.nx, c.ny, c.precision = 70, 64, 0     # 7 is a factor: no path takes it
     h = ct.c_void_p()
     assert lib.hw_ctx_create_2d(ct.byref(c), ct.byref(h)) == 1
     assert b"unsupported" in lib.hw_last_error()
