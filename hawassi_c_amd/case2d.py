@@ -235,8 +235,12 @@ def model_from_input(case_dir: str, eps_abs: float = 1e-3, eps_rel: float = 1e-2
     nx = S.cal_npoint(p.x_start, p.x_end, p.dx)
     ny = S.cal_npoint(p.y_start, p.y_end, p.dy)
     for n in (nx, ny):
-        if n < 32 or n & (n - 1):
-            raise NotImplementedError(f"grid {nx}x{ny}: the device path needs powers of two >= 32")
+        m = n
+        for f in (2, 3, 5):
+            while m % f == 0:
+                m //= f
+        if n < 2 or m != 1:
+            raise NotImplementedError(f"grid {nx}x{ny}: the device transforms take sides of 2, 3 and 5 factors")
     x = S.interval_to_vector(p.x_start, p.x_end, nx)
     y = S.interval_to_vector(p.y_start, p.y_end, ny)
     dx, dy = x[1] - x[0], y[1] - y[0]

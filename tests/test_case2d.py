@@ -125,3 +125,38 @@ def test_model_from_input_two_depth_partial_line(tmp_path):
     iy_mid = int(np.argmin(np.abs(m.y - 600)))
     ix_l = int(np.argmin(np.abs(m.x - 400)))
     assert m.chi[iy_mid, ix_l] < 0.01 and m.chi[iy_mid, -1] == 1.0 and m.chi[-1, ix_l] == 1.0
+
+
+# ---- the reference cases on grids that are not powers of two --------------
+CASE2D = Path(__file__).parent / "golden" / "case2d"
+
+
+def reference_case(name, tmp_path):
+    """The reference's case directory (input.txt, bath.dat, influx.dat) from
+    the fixtures; see tests/golden/case2d/make_fixtures.py."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("mf2d", CASE2D / "make_fixtures.py")
+    mf = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mf)
+    return mf.materialize(name, str(tmp_path / name))
+
+
+@pytest.mark.parametrize("name,shape,order,flat,brk", [
+    ("2dBBr001_HarmBr_BB93", (2000, 2), 3, False, False),
+    ("2dBBr002_Irreg", (300, 250), 3, False, True),
+    ("2dFBr001_Irreg", (600, 300), 2, True, True),
+    ("2dFBr001_TUDFoc7", (4096, 2), 3, True, True),
+])
+def test_reference_cases_on_2_3_5_grids_build(tmp_path, name, shape, order, flat, brk):
+    """The four 2-D reference cases whose grids are not powers of two build
+    from their own files, and the oracle (mixed-radix FFT) evaluates them."""
+    from hawassi_c_amd import workloads as W
+    from oracle import oracle as O
+    m = C.model_from_input(reference_case(name, tmp_path))
+    assert (m.nx, m.ny) == shape and m.order == order and m.flat == flat and (m.breaking is not None) == brk
+    if name == "2dBBr001_HarmBr_BB93":
+        assert "gam_mid" in m.ops          # depth_ref = 3 0.25
+    assert np.isfinite(m.sig).all() and np.isfinite(m.new_gam).all()
+    y = W.random_state(m, 3, amp=0.2 * m.meta["hs"], kfrac=0.3)
+    r = O.Oracle2D(m).rhs(5.0, y)
+    assert np.isfinite(r).all() and np.abs(r).max() > 0

@@ -120,3 +120,60 @@ def test_case_generated_wave_height(tmp_path):
     H = max(series) - min(series)
     say("  wave height at x = %.1f m: %.4f m (record Hs %.2f)" % (m.x[ix], H, m.meta["hs"]))
     assert abs(H - 2.0) < 0.2
+
+
+# ---- the reference cases on grids that are not powers of two: the full-grid path
+CASE2D = Path(__file__).parent / "golden" / "case2d"
+REF_CASES = ["2dBBr001_HarmBr_BB93", "2dBBr002_Irreg", "2dFBr001_Irreg", "2dFBr001_TUDFoc7"]
+
+
+def reference_case(name, tmp_path):
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("mf2d", CASE2D / "make_fixtures.py")
+    mf = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mf)
+    return C.model_from_input(mf.materialize(name, str(tmp_path / name)))
+
+
+@pytest.mark.parametrize("name", REF_CASES)
+def test_reference_case_rhs_parity_full_grid(tmp_path, name):
+    """Device (full-grid path) against the oracle on the case's own setup, on
+    every one of the 6 nx ny values; breaking runs (tprev < t)."""
+    m = reference_case(name, tmp_path)
+    say("reference case rhs", name, m.nx, m.ny, "order", m.order, "flat", m.flat)
+    y = W.random_state(m, 5, amp=0.3 * m.meta["hs"], kfrac=0.3)
+    o = O.Oracle2D(m)
+    o.set_tprev(0.0)
+    ref = o.rhs(7.3, y)
+    for prec, tol in ((HW_PREC_FP64, 1e-11), (HW_PREC_FP32, 3e-5)):
+        dev = Hawassi2D(m, prec)
+        assert dev.full_grid
+        dev.set_tprev(0.0)
+        e = relerr(dev.rhs(7.3, y), ref)
+        dev.close()
+        say("  prec", prec, "relL2", e)
+        assert e < tol
+
+
+@pytest.mark.parametrize("name", REF_CASES)
+def test_reference_case_trajectory_full_grid(tmp_path, name):
+    """From rest with the case's influx, 1.5 s (FP64): the same steps as the
+    oracle and the same state to 1e-9."""
+    m = reference_case(name, tmp_path)
+    n = m.nx * m.ny
+    y = np.zeros(6 * n)
+    o = O.Oracle2D(m)
+    dev = Hawassi2D(m, HW_PREC_FP64)
+    dev.set_state(y)
+    o.set_tprev(0.0)
+    dev.set_tprev(0.0)
+    rc, t_o, h_o, so = o.integrate(0.0, 1.5, 0.1, y, m.eps_abs, m.eps_rel)
+    t_d, h_d = dev.integrate(0.0, 1.5, 0.1)
+    st = dev.stats()
+    got = dev.get_state()
+    dev.close()
+    e = relerr(got, y)
+    say("  ", name, "steps oracle", so, "device", st["steps_accepted"], st["steps_rejected"], "relL2", e)
+    assert rc == 0 and t_o == t_d == 1.5
+    assert (st["steps_accepted"], st["steps_rejected"]) == tuple(so)
+    assert e < 1e-9

@@ -23,8 +23,10 @@ def rnd(shape, seed):
     return g.standard_normal(shape) + 1j * g.standard_normal(shape)
 
 
-@pytest.mark.parametrize("ny,nx", [(4, 8), (16, 32), (64, 64)])
+@pytest.mark.parametrize("ny,nx", [(4, 8), (16, 32), (64, 64), (2, 10), (6, 5), (3, 15), (250, 300), (2, 2000)])
 def test_fft2_matches_numpy_and_bruteforce(ny, nx):
+    """Radix 2 for powers of two, the 2/3/5 Stockham for the other lengths
+    (the reference's 300 x 250, 600 x 300 and 2000 x 2 grids)."""
     a = rnd((ny, nx), 1)
     f = O.fft2(a, -1)
     assert relerr(f, np.fft.fft2(a)) < 1e-14

@@ -29,10 +29,15 @@ def test_fft1d_matches_numpy_and_brute_force():
         a = g.standard_normal(n) + 1j * g.standard_normal(n)
         assert relerr(O.fft1d(a, -1), np.fft.fft(a)) < 1e-14
         assert relerr(O.fft1d(a, +1), np.fft.ifft(a) * n) < 1e-14
-    n = 16
-    a = g.standard_normal(n) + 1j * g.standard_normal(n)
-    W = np.exp(-2j * np.pi * np.outer(np.arange(n), np.arange(n)) / n)
-    assert relerr(O.fft1d(a, -1), W @ a) < 1e-14
+    for n in (16, 15, 30, 50):
+        a = g.standard_normal(n) + 1j * g.standard_normal(n)
+        W = np.exp(-2j * np.pi * np.outer(np.arange(n), np.arange(n)) / n)
+        assert relerr(O.fft1d(a, -1), W @ a) < 1e-14
+    # lengths of 2, 3 and 5 factors (1dBBr001HarmBar has 2000 nodes)
+    for n in (2000, 300, 4000, 3 * 5 * 5):
+        a = g.standard_normal(n) + 1j * g.standard_normal(n)
+        assert relerr(O.fft1d(a, -1), np.fft.fft(a)) < 2e-15
+        assert relerr(O.fft1d(a, +1), np.fft.ifft(a) * n) < 2e-15
 
 
 def test_freqspace1d_and_alias():
