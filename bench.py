@@ -78,10 +78,11 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--workload", default="harbour2048",
-                    choices=["harbour2048", "slope1d", "opensea1024x8", "parareal512"],
+                    choices=["harbour2048", "slope1d", "opensea1024x8", "parareal512", "ensemble"],
                     help="harbour2048: the 2-D headline; slope1d: the 1-D batch of uneven realizations; "
                          "opensea1024x8: eight flat-bottom 1024^2 realizations advanced in lockstep; "
-                         "parareal512: learned parareal on the 512^2 shoal basin (trains the network first)")
+                         "parareal512: learned parareal on the 512^2 shoal basin (trains the network first); "
+                         "ensemble: 84 sea states on 128^2/256^2/512^2 grids, members and HIP-graph steps")
     ap.add_argument("--pr-seeds", type=int, default=33, help="parareal512: training trajectories (61 snapshots each)")
     ap.add_argument("--pr-epochs", type=int, default=30, help="parareal512: training epochs")
     ap.add_argument("--pr-window", type=int, default=8, help="parareal512: slices per parareal window")
@@ -115,6 +116,8 @@ def main():
         return bench_opensea(args)
     if args.workload == "parareal512":
         return bench_parareal(args)
+    if args.workload == "ensemble":
+        return bench_ensemble(args)
 
     from hawassi_c_amd import HW_PREC_FP32, HW_PREC_FP64
     from hawassi_c_amd import workloads as W
@@ -358,6 +361,99 @@ def bench_opensea(args):
                      "unit_of_work": "one realization-RHS (5 kernels), %.1f MB algorithmic, whole-job rate" % (b_rhs / 1e6)},
     }
     for d in ds:
+        d.close()
+    print(json.dumps(out))
+
+
+def bench_ensemble(args):
+    """Batching: random-phase sea states on grids of uneven size advanced
+    together.  Realizations of the shoal basin (two reference depths, no
+    influx) at 128^2 (64 of them), 256^2 (16) and 512^2 (4): one context per
+    size holding that size's realizations as members (every kernel of a
+    right-hand side covers all members in one launch), step control on the
+    device, each attempted step one HIP-graph replay, the three contexts
+    advanced in lockstep by hw_integrate_batch.  A step is one 0.5-s output
+    interval of the whole ensemble; value = realizations x simulated seconds
+    per wall second.  Beside it: the same realizations one context at a time
+    with the host step loop (a sample of each size, scaled by its count)."""
+    from hawassi_c_amd import HW_PREC_FP32
+    from hawassi_c_amd import parareal as PR
+    from hawassi_c_amd import workloads as W
+    from hawassi_c_amd.model2d import Hawassi2D, integrate_batch
+
+    sizes = ((128, 64), (256, 16), (512, 4))
+    dt_out = 0.5
+    ctxs, states, models = [], [], {}
+    for n, nb in sizes:
+        m = W.shoal(n, shoal_depth=0.6, eps_rel=1e-4, eps_abs=1e-5)
+        codec = PR.Codec(m)
+        ys = [codec.to_state(*W.shoal_sea(m, 100 * n + i, hs=0.03 + 0.03 * (i % 5) / 4, theta0=0.37 * i))
+              for i in range(nb)]
+        d = Hawassi2D(m, HW_PREC_FP32, members=nb)
+        d.set_state(np.stack(ys))
+        ctxs.append(d)
+        states.append(ys)
+        models[n] = m
+        log(f"setup {nb} realizations {n}x{n}")
+    nmem = sum(nb for _, nb in sizes)
+    t = [0.0] * nmem
+    h = [0.02] * nmem
+    tc = 0.0
+    for _ in range(args.warmup):
+        tc += dt_out
+        t, h = integrate_batch(ctxs, t, tc, h)
+    for d in ctxs:
+        d.reset_stats()
+    t_start = tc
+    w0 = time.perf_counter()
+    for i in range(args.steps):
+        tc += dt_out
+        t, h = integrate_batch(ctxs, t, tc, h)
+        log(f"  timed {i + 1}/{args.steps} wall {time.perf_counter() - w0:.2f} s")
+    for d in ctxs:
+        d.sync()
+    wall = time.perf_counter() - w0
+    sim = tc - t_start
+    rhs = {n: d.stats()["rhs_evals"] for (n, _), d in zip(sizes, ctxs)}
+    bytes_alg = sum(rhs[n] * algorithmic_bytes(n, n)[0] for n, _ in sizes)
+    # one context at a time, host step loop: a sample per size over the same window
+    serial = {}
+    for (n, nb), ys in zip(sizes, states):
+        k = min(nb, 2)
+        timed = 0.0
+        for y0 in ys[:k]:
+            d = Hawassi2D(models[n], HW_PREC_FP32)
+            d.set_state(y0)
+            tt, hh = d.integrate(0.0, t_start, 0.02)
+            d.sync()
+            w2 = time.perf_counter()
+            d.integrate(tt, tc, hh)     # the ensemble's timed window only
+            d.sync()
+            timed += time.perf_counter() - w2
+            d.close()
+        serial[n] = timed / k * nb
+        log(f"  one at a time {n}x{n}: {serial[n]:.2f} s for {nb} realizations (sample of {k})")
+    serial_wall = sum(serial.values())
+    out = {
+        "metric": "sim-sec/wall-sec (2-D ensemble, realizations x simulated seconds)",
+        "value": nmem * sim / wall,
+        "unit": "sim-s/wall-s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "ensemble of %d shoal-basin sea states on uneven grids: %s; two reference depths, "
+                               "RKF45 eps_rel 1e-4 eps_abs 1e-5, members per size in one context, device step "
+                               "control from HIP graphs" % (nmem, ", ".join(f"{nb} x {n}^2" for n, nb in sizes)),
+                   "sim_window_s": [t_start, tc]},
+        "rhs_evals_per_s": sum(rhs.values()) / wall,
+        "one_at_a_time_host_loop_s": serial_wall,
+        "speedup_vs_one_at_a_time": serial_wall / wall,
+        "roofline": {"bound": "hbm", "achieved": bytes_alg / wall / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": bytes_alg / wall / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "unit_of_work": "one realization-RHS at its size (algorithmic bytes), whole-job rate"},
+    }
+    for d in ctxs:
         d.close()
     print(json.dumps(out))
 

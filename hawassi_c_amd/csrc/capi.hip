@@ -304,12 +304,19 @@ int hw_integrate(void* ctx, double* t, double t1, double* h) {
 int hw_integrate_batch(void* const* ctxs, int n, double* t, double t1, double* h) {
     if (n < 0 || (n > 0 && (!ctxs || !t || !h))) { set_error("null argument"); return HW_ERR_ARG; }
     {
-        bool all_graph = n > 0;
+        bool all_graph = n > 0, multi = false;
         for (int i = 0; i < n; ++i) {
             if (!ctxs[i]) { set_error("null context %d", i); return HW_ERR_ARG; }
-            int rc = one_member(static_cast<Solver2D*>(ctxs[i]), "hw_integrate_batch");
-            if (rc) return rc;
             all_graph = all_graph && static_cast<Solver2D*>(ctxs[i])->step_mode == HW_STEP_GRAPH;
+            multi = multi || static_cast<Solver2D*>(ctxs[i])->nb > 1;
+        }
+        // contexts holding several members take part only when every context
+        // steps on the device; t and h then hold one value per member
+        if (multi && !all_graph) {
+            for (int i = 0; i < n; ++i) {
+                int rc = one_member(static_cast<Solver2D*>(ctxs[i]), "hw_integrate_batch");
+                if (rc) return rc;
+            }
         }
         if (all_graph) {
             std::vector<Solver2D*> ss((size_t)n);

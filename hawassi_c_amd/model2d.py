@@ -201,11 +201,15 @@ def integrate_batch(solvers: "list[Hawassi2D]", t: "list[float]", t1: float, h: 
     """Advance several contexts to t1 in lockstep (hw_integrate_batch).
 
     Each context keeps its own step control; the result equals calling
-    integrate on each alone.  Returns the new (t, h) lists.
+    integrate on each alone.  Returns the new (t, h) lists.  When every
+    context steps on the device (set_step_mode(True)), contexts with several
+    members take part, and t and h hold one value per member.
     """
     n = len(solvers)
     if n == 0:
         return [], []
+    if len(t) != sum(s.members for s in solvers) or len(h) != len(t):
+        raise ValueError("t and h need one value per member of every context")
     lib = solvers[0].lib
     arr = (ct.c_void_p * n)(*[s.ctx for s in solvers])
     tt = np.ascontiguousarray(t, dtype=np.float64).copy()

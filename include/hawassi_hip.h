@@ -157,7 +157,11 @@ int hw_integrate(void *ctx, double *t, double t1, double *h);
  * one attempted step of every unfinished context is enqueued (each context
  * on its own stream) before any step-control decision waits, so the
  * contexts' kernels overlap on the GPU.  The reference runs one
- * realization per process (2d/fun_odesolver.c:2641-2647). */
+ * realization per process (2d/fun_odesolver.c:2641-2647).  When every
+ * context is in HW_STEP_GRAPH mode, contexts holding several members
+ * (hw_ctx_create_2d_members, e.g. realizations on grids of different sizes,
+ * one context per size) take part too: t and h then hold one value per
+ * member, context after context. */
 int hw_integrate_batch(void *const *ctxs, int n, double *t, double t1, double *h);
 
 /* How hw_integrate and hw_integrate_batch take their steps.  HW_STEP_HOST

@@ -159,7 +159,7 @@ static double *rd_arr(FILE *f, size_t count)
 
 int main(int argc, char **argv)
 {
-    int hdr[5];
+    int hdr[6];
     double par[7];
     hw_config_2d cfg;
     odeiv2_system sys;
@@ -175,7 +175,7 @@ int main(int argc, char **argv)
         return 2;
     }
     f = fopen(argv[1], "rb");
-    if (!f || rd(f, hdr, sizeof(int), 5) || rd(f, par, sizeof(double), 7)) {
+    if (!f || rd(f, hdr, sizeof(int), 6) || rd(f, par, sizeof(double), 7)) {
         fprintf(stderr, "gsl_style_driver: cannot read %s\n", argv[1]);
         return 2;
     }
@@ -185,6 +185,7 @@ int main(int argc, char **argv)
     cfg.flat = hdr[2];
     cfg.model = hdr[3];
     cfg.precision = hdr[4];
+    cfg.full_grid = hdr[5];   /* 1: the reference's own full-grid values in every column */
     cfg.grav = par[0];
     cfg.cpeak = par[1];
     cfg.eps_abs = par[2];

@@ -33,10 +33,10 @@ def build_driver(out_dir):
     return exe
 
 
-def write_model(path, m, y0, t0, t1, h0, precision):
+def write_model(path, m, y0, t0, t1, h0, precision, full_grid=0):
     n = m.nx * m.ny
     with open(path, "wb") as f:
-        f.write(struct.pack("<5i", m.nx, m.ny, int(m.flat), int(getattr(m, "order", 2)), precision))
+        f.write(struct.pack("<6i", m.nx, m.ny, int(m.flat), int(getattr(m, "order", 2)), precision, int(full_grid)))
         from hawassi_c_amd.setup2d import GRAV
         f.write(struct.pack("<7d", GRAV, m.cpeak, m.eps_abs, m.eps_rel, t0, t1, h0))
         arrs = [m.kx, m.ky, m.aal, m.fbdy, m.wallchar, m.chi]
@@ -115,3 +115,30 @@ def test_gsl_style_host_loop_matches_the_device_integrator_and_the_oracle(tmp_pa
     print("oracle steps", oacc, orej, "device", acc, rej, flush=True)
     assert abs(oacc - acc) <= 1
     assert np.abs(y[stored] - yo[stored]).max() <= 1e-8 * scale
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("flat", [False, True])
+def test_gsl_style_host_loop_on_the_full_grid_takes_the_oracles_steps(tmp_path, flat):
+    """With hw_config_2d.full_grid = 1, hw_rhs writes the reference's own
+    values in the redundant columns too, so the host GSL loop's norm sees what
+    the reference's sees: the C driver takes the oracle's step sequence (the
+    flat case included, where the mirrored half makes it take one step less)
+    and ends on the oracle's full state, every value."""
+    from hawassi_c_amd import HW_PREC_FP64
+    from oracle.oracle import Oracle2D
+    exe = build_driver(tmp_path)
+    m, y0 = _case(flat)
+    t1, h0 = 1.5, 0.05
+    model_file, out_file = str(tmp_path / "model.bin"), str(tmp_path / "out.bin")
+    write_model(model_file, m, y0, 0.0, t1, h0, HW_PREC_FP64, full_grid=1)
+    print(f"C driver on the full grid, flat={flat}", flush=True)
+    r = subprocess.run([exe, model_file, out_file], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    y, acc, rej, calls = read_out(out_file, y0.size)
+    yo = y0.copy()
+    rc, to, _, (oacc, orej) = Oracle2D(m).integrate(0.0, t1, h0, yo, m.eps_abs, m.eps_rel)
+    print("oracle steps", oacc, orej, "C driver", acc, rej, flush=True)
+    assert rc == 0 and to == t1
+    assert (acc, rej) == (oacc, orej)
+    assert np.abs(y - yo).max() <= 1e-10 * np.abs(yo).max()

@@ -73,3 +73,27 @@ def test_members_refuse_breaking():
     from hawassi_c_amd.model2d import Hawassi2D
     with pytest.raises(Exception, match="breaking"):
         Hawassi2D(W.harbour(128, t_end=2.0), 1, members=2)
+
+
+def test_member_contexts_of_uneven_sizes_in_one_batch():
+    """Realizations on grids of different sizes, one members-context per
+    size, advanced together by hw_integrate_batch in graph mode: each member
+    ends exactly where a context of its own takes it."""
+    from hawassi_c_amd.model2d import Hawassi2D, integrate_batch
+    (m1, ys1), (m2, ys2) = _case(64, [3, 4, 5]), _case(128, [6, 7])
+    d1 = Hawassi2D(m1, 1, members=3)
+    d2 = Hawassi2D(m2, 1, members=2)
+    d1.set_state(np.stack(ys1))
+    d2.set_state(np.stack(ys2))
+    t0 = [0.0, 0.2, 0.0, 0.0, 0.1]
+    h0 = [0.05, 0.05, 0.02, 0.05, 0.05]
+    print("uneven member contexts 3 x 64^2 + 2 x 128^2", flush=True)
+    t, h = integrate_batch([d1, d2], t0, 1.5, h0)
+    y1, y2 = d1.get_state(), d2.get_state()
+    d1.close()
+    d2.close()
+    for z, (m, y0, yb) in enumerate([(m1, ys1[0], y1[0]), (m1, ys1[1], y1[1]), (m1, ys1[2], y1[2]),
+                                     (m2, ys2[0], y2[0]), (m2, ys2[1], y2[1])]):
+        ys_z, _, tz, hz = _single(m, y0, 1, t0[z], 1.5, h0[z])
+        assert np.array_equal(yb, ys_z), z
+        assert t[z] == tz == 1.5 and h[z] == hz
