@@ -92,7 +92,25 @@ class HipPathMissing(RuntimeError):
     """The HIP library is not built; there is no CPU fallback."""
 
 
-def load(path: Path | None = None) -> ct.CDLL:
+class _Partial:
+    """A library that holds only some of the entry points (the autotuner's
+    candidate builds of one 2-D unit): declarations of the missing ones go
+    nowhere, calls to them fail."""
+
+    class _Sink:
+        pass
+
+    def __init__(self, lib):
+        object.__setattr__(self, "_l", lib)
+
+    def __getattr__(self, name):
+        try:
+            return getattr(self._l, name)
+        except AttributeError:
+            return _Partial._Sink()
+
+
+def load(path: Path | None = None, partial: bool = False) -> ct.CDLL:
     global _lib
     if _lib is not None and path is None:
         return _lib
@@ -101,6 +119,8 @@ def load(path: Path | None = None) -> ct.CDLL:
         raise HipPathMissing(f"{p} not found: build it with `python -m hawassi_c_amd.build` "
                              "(the product has no CPU fallback)")
     lib = ct.CDLL(str(p))
+    if partial:
+        lib = _Partial(lib)
     vp = ct.c_void_p
     lib.hw_ctx_create_2d.argtypes = [ct.POINTER(HwConfig2D), ct.POINTER(vp)]
     lib.hw_ctx_destroy.argtypes = [vp]

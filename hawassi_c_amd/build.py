@@ -7,6 +7,7 @@ linked with them into hawassi_c_amd/libhawassi_hip.so.
 from __future__ import annotations
 
 import concurrent.futures as cf
+import json
 import os
 import shutil
 import subprocess
@@ -86,14 +87,27 @@ def _includes(src: Path, seen: set) -> None:
                 break
 
 
+def _tuned_flags(src: Path) -> list[str]:
+    """-D flags the autotuner chose for this unit (hawassi_c_amd/tuned.json;
+    see autotune.py), none for units it has not tuned."""
+    tuned = PKG / "tuned.json"
+    if not tuned.exists() or not src.stem.startswith("inst_p"):
+        return []
+    t = json.loads(tuned.read_text()).get(src.stem[len("inst_"):])
+    return [f"-D{k}={v}" for k, v in sorted(t["flags"].items())] if t else []
+
+
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.stem + ".o")
     seen: set = set()
     _includes(src, seen)
     deps = [src] + sorted(seen)
+    tflags = _tuned_flags(src)
+    if tflags:
+        deps.append(PKG / "tuned.json")
     if obj.exists() and all(obj.stat().st_mtime >= d.stat().st_mtime for d in deps):
         return obj
-    cmd = [HIPCC, *FLAGS, "-I", str(CSRC), "-I", str(PKG.parent / "include"), "-c", str(src), "-o", str(obj)]
+    cmd = [HIPCC, *FLAGS, *tflags, "-I", str(CSRC), "-I", str(PKG.parent / "include"), "-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{r.stderr[-4000:]}")
