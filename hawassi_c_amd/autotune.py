@@ -25,9 +25,10 @@ Three steps, the middle one on the GPU:
       entry points and that unit only; a candidate whose geometry does not
       fit is recorded as such);
   python -m hawassi_c_amd.autotune run    [--unit ...] [--out FILE]
-      times each candidate library in a child process: the harbour model of
-      the unit's size, a developed random state, the event-timed average
-      right-hand side over a few integrated steps; writes the table as JSON;
+      times each candidate library in a child process: the model the unit
+      serves (the flat open sea at 1024^2, the two-depth harbour otherwise), a
+      random state, the event-timed average right-hand side over a few
+      integrated steps; writes the table as JSON;
   python -m hawassi_c_amd.autotune apply  FILE [FILE ...]
       writes the fastest candidate of each unit to hawassi_c_amd/tuned.json,
       which build.py reads (its -D flags go to that unit only).
@@ -168,7 +169,10 @@ def cmd_run(units: list[str], out_file: str, steps: float) -> None:
     results = []
     for unit in units:
         prec, nx, ny = unit_size(unit)
-        m = W.harbour(nx, t_end=30.0) if nx == ny else W.harbour(nx, ny=ny, t_end=30.0)
+        # the model the unit serves: the flat open sea at 1024^2 (2-D batch),
+        # the two-depth harbour otherwise
+        kind = W.short_crested if unit == "p1_1024x1024" else W.harbour
+        m = kind(nx, t_end=30.0) if nx == ny else kind(nx, ny=ny, t_end=30.0)
         y0 = W.random_state(m, 7, amp=0.5, kfrac=0.3)
         mf = tempfile.NamedTemporaryFile(suffix=".pkl", delete=False)
         pickle.dump((m, y0, prec), mf)
