@@ -10,7 +10,11 @@ block geometry at compile time, per (precision, grid) unit:
 * HW_Y_PER_THREAD  -- the same for the y-pass (and with it the columns a
                       y-pass block owns);
 * HW_Y_WPE32       -- waves per SIMD the FP32 y-pass kernels are compiled for
-                      (their register budget).
+                      (their register budget);
+* HW_X_LDS_BUDGET,
+  HW_X_WPE         -- the LDS and register budgets of an x-pass block: with
+                      smaller ones several x blocks share a CU, and one
+                      block's loads overlap another's LDS transform phases.
 
 These macros are read only by XGeo<T, NX> / YGeo<T, NY> and the kernels
 instantiated with them, so a unit whose sides occur in no other unit (the
@@ -79,11 +83,20 @@ def candidates(unit: str, center: dict | None = None, skip: list | None = None) 
         if dx and dy:
             continue
         out.append(c)
+    # several x blocks per CU: a smaller LDS budget (one exchange buffer or
+    # twiddles from global memory) and a register budget for that many waves
+    for ex, lds, wpe in ((8, 81920, 4), (8, 54613, 6), (16, 81920, 4)):
+        c = {**c0, "HW_X_RUN_BYTES": 32, "HW_X_PER_THREAD": ex, "HW_X_LDS_BUDGET": lds, "HW_X_WPE": wpe}
+        if c not in out:
+            out.append(c)
     return [c for c in out if not skip or c not in skip]
 
 
 def cand_name(c: dict) -> str:
-    return "x%d_e%d_y%d_w%d" % (c["HW_X_RUN_BYTES"], c["HW_X_PER_THREAD"], c["HW_Y_PER_THREAD"], c["HW_Y_WPE32"])
+    s = "x%d_e%d_y%d_w%d" % (c["HW_X_RUN_BYTES"], c["HW_X_PER_THREAD"], c["HW_Y_PER_THREAD"], c["HW_Y_WPE32"])
+    if "HW_X_LDS_BUDGET" in c:
+        s += "_l%d_o%d" % (c["HW_X_LDS_BUDGET"] // 1024, c["HW_X_WPE"])
+    return s
 
 
 def unit_source(unit: str) -> Path:
@@ -139,13 +152,13 @@ def cmd_build(units: list[str], jobs: int, center_file: str | None = None) -> No
 
 
 # ------------------------------------------------------------ GPU side ----
-def _child(lib_path: str, model_file: str, steps: float) -> None:
+def _child(lib_path: str, model_file: str, steps: float, partial: bool = True) -> None:
     """Times one candidate library (runs in its own process)."""
     import numpy as np
 
     from . import _lib
     from .model2d import Hawassi2D
-    _lib._lib = _lib.load(Path(lib_path), partial=True)   # every later load() returns the candidate
+    _lib._lib = _lib.load(Path(lib_path), partial=partial)   # every later load() returns the candidate
     with open(model_file, "rb") as f:
         m, y0, prec = pickle.load(f)
     d = Hawassi2D(m, prec)
@@ -178,13 +191,16 @@ def cmd_run(units: list[str], out_file: str, steps: float) -> None:
         pickle.dump((m, y0, prec), mf)
         mf.close()
         ref = None
-        for r in rows:
+        # the library as built (its tuned.json flags) first, for reference
+        main = {"unit": unit, "candidate": "built", "flags": None, "ok": True,
+                "lib": str(B.LIB.relative_to(B.PKG.parent))}
+        for r in [main] + rows:
             if r["unit"] != unit or not r["ok"]:
                 continue
             print(f"tune {unit} {r['candidate']}", flush=True)
             p = subprocess.run([sys.executable, "-m", "hawassi_c_amd.autotune", "child", str(B.PKG.parent / r["lib"]),
-                                mf.name, str(steps)], capture_output=True, text=True, timeout=300,
-                               cwd=str(B.PKG.parent))
+                                mf.name, str(steps), "full" if r["candidate"] == "built" else "partial"],
+                               capture_output=True, text=True, timeout=300, cwd=str(B.PKG.parent))
             line = [ln for ln in p.stdout.splitlines() if ln.startswith("RESULT ")]
             if p.returncode != 0 or not line:
                 results.append({**r, "rhs_us": None, "why": (p.stderr or p.stdout)[-300:]})
@@ -206,7 +222,7 @@ def cmd_apply(result_files: list[str]) -> None:
     res = [r for f in result_files for r in json.loads(Path(f).read_text())]
     tuned = json.loads(TUNED.read_text()) if TUNED.exists() else {}
     for unit in sorted({r["unit"] for r in res}):
-        ok = [r for r in res if r["unit"] == unit and r.get("rhs_us") and r.get("same_state")]
+        ok = [r for r in res if r["unit"] == unit and r.get("rhs_us") and r.get("same_state") and r.get("flags")]
         if not ok:
             continue
         best = min(ok, key=lambda r: r["rhs_us"])
@@ -236,6 +252,7 @@ def main(argv=None) -> None:
     c.add_argument("lib")
     c.add_argument("model")
     c.add_argument("steps", type=float)
+    c.add_argument("mode", choices=["full", "partial"])
     args = ap.parse_args(argv)
     units = getattr(args, "unit", None) or ["p1_2048x2048"]
     if args.cmd == "build":
@@ -245,7 +262,7 @@ def main(argv=None) -> None:
     elif args.cmd == "apply":
         cmd_apply(args.file)
     else:
-        _child(args.lib, args.model, args.steps)
+        _child(args.lib, args.model, args.steps, args.mode == "partial")
 
 
 if __name__ == "__main__":

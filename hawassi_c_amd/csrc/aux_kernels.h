@@ -370,7 +370,7 @@ __global__ void k_src_eval_g(const DevStep* __restrict__ s, double frac, const d
 // ------------------------------------------------ stand-alone transforms --
 // forward 2-D real transform: physical [ny][nx] -> half spectrum [kx][ky]
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k_t_xfwd(const T* __restrict__ phys, typename ctype<T>::type* X,
+__global__ void HW_XK_BOUNDS k_t_xfwd(const T* __restrict__ phys, typename ctype<T>::type* X,
                                                             const typename ctype<T>::type* twx) {
     HW_XK_PROLOGUE_TW(twx)
     C f[G::PP];
@@ -400,7 +400,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k_t_yinv(const typename cty
 }
 
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k_t_xinv(const typename ctype<T>::type* X, T* phys, T* nyq_out,
+__global__ void HW_XK_BOUNDS k_t_xinv(const typename ctype<T>::type* X, T* phys, T* nyq_out,
                                                             const typename ctype<T>::type* twx, const int* gate = nullptr) {
     if (gate && *gate == 0) return;
     HW_XK_PROLOGUE_TW(twx)

@@ -296,7 +296,7 @@ __global__ void k_fill(T* __restrict__ p, T val, size_t n) {
 // delphi = F^-1(P(dH1 + adjust(dH2)_eta)) = F^-1(P(dH1 + i F(g))) + chi A_eta,
 // g = chi (-1)^x s_eta(y) the imaginary part of the adjust's inverse.
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k_brk_g_x(Rhs2DArgs<T> a, const T* __restrict__ s_eta,
+__global__ void HW_XK_BOUNDS k_brk_g_x(Rhs2DArgs<T> a, const T* __restrict__ s_eta,
                                                               typename ctype<T>::type* X) {
     HW_XK_PROLOGUE
     C f[G::PP], t[G::PP];
@@ -344,7 +344,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k_brk_del_y(Rhs2DArgs<T> a,
 
 // Flux = -B delb^2 (eta + D) delphi^2 (2d/fun_odesolver.c:995-1000)
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k_brk_flux_x(Rhs2DArgs<T> a, int slot_aeta, const typename ctype<T>::type* Xd,
+__global__ void HW_XK_BOUNDS k_brk_flux_x(Rhs2DArgs<T> a, int slot_aeta, const typename ctype<T>::type* Xd,
                                                                  const T* __restrict__ B, const T* __restrict__ depth, T delb2,
                                                                  T* __restrict__ flux) {
     HW_XK_PROLOGUE
@@ -392,7 +392,7 @@ __global__ void k_brk_sb(const T* __restrict__ F, const T* __restrict__ eta, con
 
 // forward transforms of Sb_x, Sb_y into persistent half spectra
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k_brk_sb_x(Rhs2DArgs<T> a, const T* sbx, const T* sby,
+__global__ void HW_XK_BOUNDS k_brk_sb_x(Rhs2DArgs<T> a, const T* sbx, const T* sby,
                                                                typename ctype<T>::type* X1, typename ctype<T>::type* X2) {
     HW_XK_PROLOGUE
     C f[G::PP];

@@ -346,6 +346,14 @@ template <int L, int NT, int E> __device__ __forceinline__ void fr_slot(int tid,
 #define HW_Y_WPE32 3
 #endif
 
+// waves per SIMD the x kernels are compiled for (0: one block's waves per
+// SIMD, NT / 256); more than that lets several x blocks share a CU when
+// their LDS fits (HW_X_LDS_BUDGET), so that one block's loads overlap
+// another's LDS transform phases
+#ifndef HW_X_WPE
+#define HW_X_WPE 0
+#endif
+
 // x-pass: RR rows per block, so that the transposed spectral runs
 // X[kx][y0 .. y0+RR) are HW_X_RUN_BYTES long (at least one wave per block).
 // Short runs cost far more than their bytes: the lines they touch are shared
@@ -383,8 +391,11 @@ template <typename T, int NX> struct XGeo {
         fr_slot<L, NT, E>(tid, q, b, n);
         return b * L + n;
     }
+    static constexpr int WPE = HW_X_WPE > 0 ? HW_X_WPE : (NT / 256 > 0 ? NT / 256 : 1);
     static_assert(NT % RR == 0 && NT <= 1024, "x-pass block shape");
 };
+// launch bounds of every x-pass kernel (T, NX in scope)
+#define HW_XK_BOUNDS __launch_bounds__((XGeo<T, NX>::NT)) __attribute__((amdgpu_waves_per_eu(XGeo<T, NX>::WPE)))
 
 // y-pass: CC whole columns per block (about 256 threads), E elements each.
 template <typename T, int NY> struct YGeo {

@@ -41,7 +41,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves
 }
 
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k3d_x2(Rhs2DArgs<T> a) {
+__global__ void HW_XK_BOUNDS k3d_x2(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
     C f[G::PP], g[G::PP];
     XSpec<T, NX> b;
@@ -100,7 +100,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves
 }
 
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k3d_x4(Rhs2DArgs<T> a) {
+__global__ void HW_XK_BOUNDS k3d_x4(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
     C m[G::PP], t[G::PP];
     ph_load<T, NX, NY>(m, a.m0u_p, y0, tid);
@@ -149,7 +149,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves
 }
 
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k3d_x6(Rhs2DArgs<T> a) {
+__global__ void HW_XK_BOUNDS k3d_x6(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
     C f[G::PP], g[G::PP];
     XSpec<T, NX> b;

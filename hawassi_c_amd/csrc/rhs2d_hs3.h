@@ -53,7 +53,7 @@ template <> struct H3Slots<true> {
 // X5a: F(g- b- + g+ b+) of the two AB2 HSS tails (the first half of k2b_x5,
 // with the second tail in slot S3_ACC2)
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k3b_x5a(Rhs2DArgs<T> a) {
+__global__ void HW_XK_BOUNDS k3b_x5a(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
     C f[G::PP], acc[G::PP], gm[G::PP], gp[G::PP];
     XSpec<T, NX> b;
@@ -139,7 +139,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves
 
 // XB: the physical products, forward in x (ND3: also g_mid A)
 template <typename T, int NX, int NY, bool FLAT, bool ND3 = false>
-__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k3_xb(Rhs2DArgs<T> a) {
+__global__ void HW_XK_BOUNDS k3_xb(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
     using SL = H3Slots<FLAT>;
     C ll[G::PP], dg[G::PP], e[G::PP], x[G::PP], t[G::PP];
@@ -297,7 +297,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves
 // X5b: F(g- b3- + g+ b3+) of the HS3 tail (ND3: + g_mid b3_mid); the two
 // velocity output transforms
 template <typename T, int NX, int NY, bool ND3 = false>
-__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k3b_x5b(Rhs2DArgs<T> a) {
+__global__ void HW_XK_BOUNDS k3b_x5b(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
     using SL = H3Slots<false>;
     C f[G::PP], acc[G::PP], gg[G::PP];

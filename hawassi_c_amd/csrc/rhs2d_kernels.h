@@ -300,7 +300,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves
 }
 
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x1(Rhs2DArgs<T> a) {
+__global__ void HW_XK_BOUNDS k2b_x1(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
     // eta, u, v and the HSS inverses of u^ and v^.  One spectral buffer rolls
     // through the seven inputs (each inverse refills it with the next one).
@@ -387,7 +387,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves
 enum : int { S_P1 = 2, S_Q1 = 3, S_G1M = 4, S_G1P = 5, S_P2 = 6, S_Q2 = 7, S_G2M = 8, S_G2P = 9, S_E = 10 };
 
 template <typename T, int NX, int NY, bool H3 = false>
-__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x3(Rhs2DArgs<T> a) {
+__global__ void HW_XK_BOUNDS k2b_x3(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
     // u and v are read twice (for E and for the products) so that at most
     // four pair arrays are live during a transform
@@ -557,7 +557,7 @@ __device__ __forceinline__ void x_vel_out(const Rhs2DArgs<T>& a, XSpec<T, NX>& b
 }
 
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x5(Rhs2DArgs<T> a) {
+__global__ void HW_XK_BOUNDS k2b_x5(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
     // F(g- b- + g+ b+) of both HSS tails, then the two velocity output
     // transforms; one spectral buffer rolls through the six inputs
@@ -694,7 +694,7 @@ __device__ __forceinline__ void x_eta_out(const Rhs2DArgs<T>& a, XSpec<T, NX>& b
 }
 
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2b_x7(Rhs2DArgs<T> a) {
+__global__ void HW_XK_BOUNDS k2b_x7(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
     XSpec<T, NX> b;
     x_spec_load<T, NX, NY>(a.W[S_AETA], y0, b, tid);
@@ -754,7 +754,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves
 enum : int { F_P1 = 5, F_Q1 = 6, F_P2 = 7, F_Q2 = 8, F_E = 9 };
 
 template <typename T, int NX, int NY, bool H3 = false>
-__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2f_x2(Rhs2DArgs<T> a) {
+__global__ void HW_XK_BOUNDS k2f_x2(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
     C ew[G::PP], m[G::PP], q2[G::PP], lp[G::PP];
     XSpec<T, NX> b;
@@ -839,7 +839,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves
 enum : int { F_ZETA = 3, F_ZU = 4, F_ZV = 5 };
 
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k2f_x4(Rhs2DArgs<T> a) {
+__global__ void HW_XK_BOUNDS k2f_x4(Rhs2DArgs<T> a) {
     HW_XK_PROLOGUE
     XSpec<T, NX> b;
     x_spec_load<T, NX, NY>(a.W[F_AETA], y0, b, tid);
@@ -894,7 +894,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves
 enum : int { H1_ZE = 13, H1_ZU = 14, H1_ZV = 15 };
 
 template <typename T, int NX, int NY>
-__global__ void __launch_bounds__((XGeo<T, NX>::NT)) k1_x(Rhs2DArgs<T> a, int from_phys) {
+__global__ void HW_XK_BOUNDS k1_x(Rhs2DArgs<T> a, int from_phys) {
     HW_XK_PROLOGUE
     // forward x of source - fbdy7 eta, -fbdy7 u, -fbdy7 v
     C f[G::PP], d[G::PP];
