@@ -125,8 +125,9 @@ template <typename T> struct CrestMachine {
     // fun_peaks (2d/breaking.c:55-133) on eta * pb.Char; pkcount[0] holds
     // the count for the caller's Hilbert kernels, which may skip their work
     // when it is zero
-    int detect_begin(hipStream_t s, const T* eta) {
-        k_fill<int><<<1, 64, 0, s>>>(pkcount.p, 0, 2);
+    int detect_begin(hipStream_t s, const T* eta, bool zeroed = false) {
+        // zeroed: an earlier kernel of this call has zeroed the counters
+        if (!zeroed) k_fill<int><<<1, 64, 0, s>>>(pkcount.p, 0, 2);
         k_brk_peaks<T><<<dim3((nx + 255) / 256, ny), 256, 0, s>>>(eta, pbc.p, MaxEtaInit, nx, ny, peaks.p, pkcount.p,
                                                                    maxpk);
         HW_CM_TRY(hipGetLastError());
