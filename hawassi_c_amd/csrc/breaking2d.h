@@ -344,13 +344,13 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k_brk_del_y(Rhs2DArgs<T> a,
 
 // Flux = -B delb^2 (eta + D) delphi^2 (2d/fun_odesolver.c:995-1000)
 template <typename T, int NX, int NY>
-__global__ void HW_XK_BOUNDS k_brk_flux_x(Rhs2DArgs<T> a, int slot_aeta, const typename ctype<T>::type* Xd,
+__global__ void HW_XK_BOUNDS k_brk_flux_x(Rhs2DArgs<T> a, const typename ctype<T>::type* aeta, const typename ctype<T>::type* Xd,
                                                                  const T* __restrict__ B, const T* __restrict__ depth, T delb2,
                                                                  T* __restrict__ flux) {
     HW_XK_PROLOGUE
     C dp[G::PP], A[G::PP], t[G::PP];
     x_inverse<T, NX, NY>(Xd, y0, lds, twxs, dp, sc, nullptr, tid);
-    x_inverse<T, NX, NY>(a.W[slot_aeta], y0, lds, twxs, A, sc, nullptr, tid);
+    x_inverse<T, NX, NY>(aeta, y0, lds, twxs, A, sc, nullptr, tid);
     ph_load<T, NX, NY>(t, a.chi, y0, tid);
 #pragma unroll
     for (int q = 0; q < G::PP; ++q) dp[q] = pfma(t[q], A[q], dp[q]);

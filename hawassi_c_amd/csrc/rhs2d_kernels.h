@@ -254,10 +254,15 @@ __device__ __forceinline__ void y_load_in(carr<T, YGeo<T, NY>::PE>& v, const Rhs
             const size_t q = (size_t)f * H + (size_t)col0 * NY + idx;
             const double2 yv = a.st_y[q];
             double ax = 0.0, ay = 0.0;
-            for (int j = 0; j < a.st_nk; ++j) {
-                const C kv = a.st_k[j][q];
-                ax += a.st_b[j] * (double)kv.x;
-                ay += a.st_b[j] * (double)kv.y;
+            // compile-time indices: a dynamic index into the argument
+            // struct would move it to scratch memory
+#pragma unroll
+            for (int j = 0; j < 5; ++j) {
+                if (j < a.st_nk) {
+                    const C kv = a.st_k[j][q];
+                    ax += a.st_b[j] * (double)kv.x;
+                    ay += a.st_b[j] * (double)kv.y;
+                }
             }
             const double rx = yv.x + a.st_h * ax, ry = yv.y + a.st_h * ay;
             const C x = cmk<C>((T)rx, (T)ry);
@@ -1006,7 +1011,8 @@ __global__ void HW_XK_BOUNDS k1_x(Rhs2DArgs<T> a, int from_phys) {
     // forward x of source - fbdy7 eta, -fbdy7 u, -fbdy7 v
     C f[G::PP], d[G::PP];
     ph_load<T, NX, NY>(d, a.fbdy7, y0, tid);
-    #pragma unroll 1
+    // unrolled: a runtime index into a.W would move the argument struct to scratch
+    #pragma unroll
     for (int c = 0; c < 3; ++c) {
         if (from_phys) ph_load<T, NX, NY>(f, c == 0 ? a.eta_p : (c == 1 ? a.u_p : a.v_p), y0, tid);
         else x_inverse<T, NX, NY>(a.W[c == 0 ? F_ETA : (c == 1 ? F_U : F_V)], y0, lds, twxs, f, sc, nullptr, tid);
@@ -1033,7 +1039,7 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k1_y(Rhs2DArgs<T> a, int fl
     C Z[G::PE], eh[G::PE];
     y_load<T, NX, NY>(eh, a.eta_hat, nullptr, col0, tid);
     C* outs[3] = {a.k_eta, a.k_u, a.k_v};
-    #pragma unroll 1
+    #pragma unroll
     for (int c = 0; c < 3; ++c) {
         y_load<T, NX, NY>(Z, a.W[H1_ZE + c], nullptr, col0, tid);
         y_fwd_regs<T, NX, NY>(Z, lds, twys, tid);

@@ -537,7 +537,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         C* Xd = args.W[13];
         k_brk_g_x<T, NX, NY><<<gx, bx, 0, stream>>>(args, s_eta.p, Xg);
         k_brk_del_y<T, NX, NY><<<gy, by, 0, stream>>>(args, flat ? 1 : 0, Xg, Xd);
-        k_brk_flux_x<T, NX, NY><<<gx, bx, 0, stream>>>(args, flat ? (int)F_AETA : (int)S_AETA, Xd, cm.Bf.p, cm.depth.p,
+        k_brk_flux_x<T, NX, NY><<<gx, bx, 0, stream>>>(args, args.W[flat ? (int)F_AETA : (int)S_AETA], Xd, cm.Bf.p, cm.depth.p,
                                                       (T)(delb * delb), flux.p);
         k_brk_sb<T><<<dim3((NX + 255) / 256, NY), 256, 0, stream>>>(flux.p, eta_p.p, cm.depth.p, dxg, dyg, NX, NY, sbx.p, sby.p);
         k_brk_sb_x<T, NX, NY><<<gx, bx, 0, stream>>>(args, sbx.p, sby.p, args.W[10], args.W[11]);
