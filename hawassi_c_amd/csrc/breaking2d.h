@@ -417,29 +417,6 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) k_brk_sb_y(Rhs2DArgs<T> a, 
 }
 
 // k_u += aal Sb_x^, k_v += aal Sb_y^ (current or stale, 2d/fun_odesolver.c:4599-4611)
-// Stored-half effect of projecting delphiH (fftwcomplex_2d_sym,
-// 2d/functions1.c:492-521) inside fun_sbreaking: column kx = 0 rows
-// ky > NY/2 become the conjugates of rows NY - ky, and the four self-conjugate
-// bins lose their imaginary part.  The source and damping terms folded into
-// the eta derivative are transforms of real fields, hence already Hermitian
-// along column 0 and real at those bins, so projecting the pre-mask sum
-// (pre, column 0) is the same as projecting delphiH alone.
-template <typename T>
-__global__ void k_eta_project(typename ctype<T>::type* __restrict__ k_eta, const typename ctype<T>::type* __restrict__ pre,
-                              const T* __restrict__ aal, int nx, int ny) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= ny) return;
-    if (r == 0 || r == ny / 2) {
-        k_eta[r].x = pre[r].x * aal[r];
-        k_eta[r].y = (T)0;
-        k_eta[(size_t)(nx / 2) * ny + r].y = (T)0;
-    } else if (r > ny / 2) {
-        typename ctype<T>::type p = pre[ny - r];
-        k_eta[r].x = p.x * aal[r];
-        k_eta[r].y = -p.y * aal[r];
-    }
-}
-
 template <typename T>
 __global__ void k_add_sb(typename ctype<T>::type* __restrict__ ku, typename ctype<T>::type* __restrict__ kv,
                          const typename ctype<T>::type* __restrict__ sx, const typename ctype<T>::type* __restrict__ sy,

@@ -70,10 +70,16 @@ template <typename T> struct CrestMachine {
     int maxpk = 0, maxcr = 0;
     DevBuf<T> pbc, depth, Bf;
     DevBuf<double> xg, yg;
-    DevBuf<int> owner, pkcount;            // pkcount[0]: peaks, pkcount[1]: crest candidates
+    DevBuf<int> owner;
+    // one region for the counts and the candidates, so that one copy brings
+    // both back: pkcount[0] peaks, pkcount[1] crest candidates, then cand
+    static constexpr size_t CNT_BYTES = 64;
+    DevBuf<unsigned char> cc;
+    int* pkcount = nullptr;
+    BrkCand* cand = nullptr;
     DevBuf<BrkPeak> peaks;
-    DevBuf<BrkCand> cand;
     DevBuf<BrkCrest> crests;
+    unsigned char* h_cc = nullptr;         // pinned copy of the first CAPH candidates and the counts
     int* h_cnt = nullptr;
     BrkCand* h_cand = nullptr;
     hipEvent_t ev = nullptr;
@@ -83,8 +89,7 @@ template <typename T> struct CrestMachine {
     bool b_zero = true;                    // Bf known to be all zero
 
     ~CrestMachine() {
-        if (h_cnt) (void)hipHostFree(h_cnt);
-        if (h_cand) (void)hipHostFree(h_cand);
+        if (h_cc) (void)hipHostFree(h_cc);
         if (ev) (void)hipEventDestroy(ev);
     }
 
@@ -111,13 +116,16 @@ template <typename T> struct CrestMachine {
             return rc;
         maxpk = (int)std::max<size_t>(4096, N / 8);
         maxcr = 65536;
-        if ((rc = Bf.alloc(N)) || (rc = owner.alloc(N)) || (rc = pkcount.alloc(2)) || (rc = peaks.alloc(maxpk)) ||
-            (rc = cand.alloc(maxpk)) || (rc = crests.alloc(maxcr)))
+        if ((rc = Bf.alloc(N)) || (rc = owner.alloc(N)) || (rc = cc.alloc(CNT_BYTES + sizeof(BrkCand) * maxpk)) ||
+            (rc = peaks.alloc(maxpk)) || (rc = crests.alloc(maxcr)))
             return rc;
+        pkcount = reinterpret_cast<int*>(cc.p);
+        cand = reinterpret_cast<BrkCand*>(cc.p + CNT_BYTES);
         k_fill<T><<<grid1(N), 256, 0, s>>>(Bf.p, (T)0, N);
         HW_CM_TRY(hipGetLastError());
-        HW_CM_TRY(hipHostMalloc((void**)&h_cnt, 2 * sizeof(int), hipHostMallocDefault));
-        HW_CM_TRY(hipHostMalloc((void**)&h_cand, CAPH * sizeof(BrkCand), hipHostMallocDefault));
+        HW_CM_TRY(hipHostMalloc((void**)&h_cc, CNT_BYTES + CAPH * sizeof(BrkCand), hipHostMallocDefault));
+        h_cnt = reinterpret_cast<int*>(h_cc);
+        h_cand = reinterpret_cast<BrkCand*>(h_cc + CNT_BYTES);
         HW_CM_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
         return HW_OK;
     }
@@ -127,8 +135,8 @@ template <typename T> struct CrestMachine {
     // when it is zero
     int detect_begin(hipStream_t s, const T* eta, bool zeroed = false) {
         // zeroed: an earlier kernel of this call has zeroed the counters
-        if (!zeroed) k_fill<int><<<1, 64, 0, s>>>(pkcount.p, 0, 2);
-        k_brk_peaks<T><<<dim3((nx + 255) / 256, ny), 256, 0, s>>>(eta, pbc.p, MaxEtaInit, nx, ny, peaks.p, pkcount.p,
+        if (!zeroed) k_fill<int><<<1, 64, 0, s>>>(pkcount, 0, 2);
+        k_brk_peaks<T><<<dim3((nx + 255) / 256, ny), 256, 0, s>>>(eta, pbc.p, MaxEtaInit, nx, ny, peaks.p, pkcount,
                                                                    maxpk);
         HW_CM_TRY(hipGetLastError());
         return HW_OK;
@@ -138,11 +146,10 @@ template <typename T> struct CrestMachine {
     // with H the Hilbert transform of eta; counts and the first candidates
     // come back with an asynchronous copy
     int detect_end(hipStream_t s, const T* eta, const T* H, const T* u, const T* v) {
-        k_brk_crit<T><<<64, 256, 0, s>>>(peaks.p, pkcount.p, maxpk, eta, H, u, v, depth.p, Bf.p, dxg, dyg, Ucur, KBC,
-                                         nx, ny, cand.p);
+        k_brk_crit<T><<<64, 256, 0, s>>>(peaks.p, pkcount, maxpk, eta, H, u, v, depth.p, Bf.p, dxg, dyg, Ucur, KBC,
+                                         nx, ny, cand);
         HW_CM_TRY(hipGetLastError());
-        HW_CM_TRY(hipMemcpyAsync(h_cnt, pkcount.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-        HW_CM_TRY(hipMemcpyAsync(h_cand, cand.p, CAPH * sizeof(BrkCand), hipMemcpyDeviceToHost, s));
+        HW_CM_TRY(hipMemcpyAsync(h_cc, cc.p, CNT_BYTES + CAPH * sizeof(BrkCand), hipMemcpyDeviceToHost, s));
         HW_CM_TRY(hipEventRecord(ev, s));
         return HW_OK;
     }
@@ -162,7 +169,7 @@ template <typename T> struct CrestMachine {
             std::vector<BrkCand> hc(ncand);
             std::copy(h_cand, h_cand + std::min(ncand, CAPH), hc.begin());
             if (ncand > CAPH)
-                HW_CM_TRY(hipMemcpy(hc.data() + CAPH, cand.p + CAPH, sizeof(BrkCand) * (ncand - CAPH),
+                HW_CM_TRY(hipMemcpy(hc.data() + CAPH, cand + CAPH, sizeof(BrkCand) * (ncand - CAPH),
                                     hipMemcpyDeviceToHost));
             // fun_peaks order: interior x-major, then x=0, x=nx-1, y=0, y=ny-1 edges
             const int NX = nx, NY = ny;

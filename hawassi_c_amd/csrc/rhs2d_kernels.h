@@ -56,21 +56,11 @@ template <typename T> struct Rhs2DArgs {
     // third reference depth (HSSgen_3depth): gamma_mid [y][x], C^2_mid and
     // Om^2_mid [kx][ky]; null for two depths
     const T *gam_c, *csq_c, *om2_c;
-    // The host step loop hands the first y-pass three side jobs that would
+    // The host step loop hands the first y-pass two side jobs that would
     // otherwise be launches of their own:
-    // * the RKF45 stage input: when st_y is set, the state fields are formed
-    //   as y + st_h sum_j st_b[j] st_k[j] (FP64, the arithmetic of
-    //   k_rk_stage), rounded to T, used, and stored at st_out (the stage
-    //   buffer eta_hat / u_hat / v_hat point into) for the later passes;
     // * the influx line signals at t (natural cubic splines, as k_src_eval):
     //   when sp_n > 0, into src_sig_w / src_skew_w;
     // * zero_cnt: when set, the two peak-search counters are zeroed.
-    const double2* st_y;
-    const C* st_k[5];
-    double st_b[5];
-    int st_nk;
-    double st_h;
-    C* st_out;
     const double *sp_y, *sp_c, *sk_y, *sk_c;
     int sp_n, sp_idx;
     double sp_dx, sp_hh;
@@ -231,50 +221,7 @@ __device__ __forceinline__ void y_store(typename ctype<T>::type* __restrict__ ds
     }
 }
 
-// The state field f (0 eta^, 1 u^, 2 v^) of the first y-pass: loaded from
-// the RHS input, or -- with a fused RKF45 stage -- formed from the FP64
-// state and the stage derivatives and stored to the stage buffer.
-template <typename T, int NX, int NY>
-__device__ __forceinline__ void y_load_in(carr<T, YGeo<T, NY>::PE>& v, const Rhs2DArgs<T>& a, int f, int col0,
-                                          int tid) {
-    using G = YGeo<T, NY>;
-    using C = typename ctype<T>::type;
-    constexpr int NXH = NX / 2 + 1;
-    constexpr size_t H = (size_t)NXH * NY;
-    if (!a.st_y) {
-        y_load<T, NX, NY>(v, f == 0 ? a.eta_hat : (f == 1 ? a.u_hat : a.v_hat), nullptr, col0, tid);
-        return;
-    }
-#pragma unroll
-    for (int e = 0; e < G::PE; ++e) {
-        int idx = G::eidx(tid, e);
-        int c = idx / NY;
-        bool ok = (G::NE % G::NT == 0 || idx < G::NE) && (col0 + c < NXH);
-        if (ok) {
-            const size_t q = (size_t)f * H + (size_t)col0 * NY + idx;
-            const double2 yv = a.st_y[q];
-            double ax = 0.0, ay = 0.0;
-            // compile-time indices: a dynamic index into the argument
-            // struct would move it to scratch memory
-#pragma unroll
-            for (int j = 0; j < 5; ++j) {
-                if (j < a.st_nk) {
-                    const C kv = a.st_k[j][q];
-                    ax += a.st_b[j] * (double)kv.x;
-                    ay += a.st_b[j] * (double)kv.y;
-                }
-            }
-            const double rx = yv.x + a.st_h * ax, ry = yv.y + a.st_h * ay;
-            const C x = cmk<C>((T)rx, (T)ry);
-            a.st_out[q] = x;
-            v[e] = x;
-        } else {
-            v[e] = cmk<C>((T)0, (T)0);
-        }
-    }
-}
-
-// the first y-pass's other side jobs (see Rhs2DArgs): the peak-search
+// the first y-pass's side jobs (see Rhs2DArgs): the peak-search
 // counters and the influx line signals at t
 template <typename T, int NT>
 __device__ __forceinline__ void y1_side_jobs(const Rhs2DArgs<T>& a, int tid) {
@@ -313,7 +260,10 @@ template <typename T, int NX, int NY> struct YIdx {
 // the reference forms the eta derivative (2d/fun_odesolver.c:990,
 // fftwcomplex_2d_sym), which on the stored half means: column 0's rows
 // ky > ny/2 are the conjugates of rows ny - ky, and Im = 0 at ky = 0, ny/2 of
-// columns 0 and nx/2 (y_project_regs, through LDS for column 0).
+// columns 0 and nx/2 (y_project_regs, through LDS for column 0).  The
+// source and damping terms folded into the eta derivative are transforms of
+// real fields, hence already Hermitian along column 0 and real at those
+// bins, so projecting the whole pre-mask sum is projecting delphiH alone.
 template <typename T, int NX, int NY>
 __device__ __forceinline__ void y_eta_mask(carr<T, YGeo<T, NY>::PE>& Z, const Rhs2DArgs<T>& a,
                                            typename ctype<T>::type* lds, int col0, int tid) {
@@ -377,9 +327,9 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves
     C e0[G::PE], uu[G::PE], vv[G::PE], w[G::PE];
     T cm[G::PE], cp[G::PE];
     y1_side_jobs<T, G::NT>(a, tid);
-    y_load_in<T, NX, NY>(e0, a, 0, col0, tid);
-    y_load_in<T, NX, NY>(uu, a, 1, col0, tid);
-    y_load_in<T, NX, NY>(vv, a, 2, col0, tid);
+    y_load<T, NX, NY>(e0, a.eta_hat, nullptr, col0, tid);
+    y_load<T, NX, NY>(uu, a.u_hat, nullptr, col0, tid);
+    y_load<T, NX, NY>(vv, a.v_hat, nullptr, col0, tid);
 #pragma unroll
     for (int e = 0; e < G::PE; ++e) {
         int kx, ky; size_t gi;
@@ -842,12 +792,12 @@ __global__ void __launch_bounds__((YGeo<T, NY>::NT)) __attribute__((amdgpu_waves
     HW_YK_PROLOGUE
     C v[G::PE], w[G::PE];
     y1_side_jobs<T, G::NT>(a, tid);
-    y_load_in<T, NX, NY>(v, a, 0, col0, tid);
+    y_load<T, NX, NY>(v, a.eta_hat, nullptr, col0, tid);
     y_inv_regs<T, NX, NY>(v, lds, twys, col0, tid);
     y_store<T, NX, NY>(a.W[F_ETA], v, col0, tid);
     #pragma unroll
     for (int c = 0; c < 2; ++c) {
-        y_load_in<T, NX, NY>(v, a, 1 + c, col0, tid);
+        y_load<T, NX, NY>(v, c == 0 ? a.u_hat : a.v_hat, nullptr, col0, tid);
 #pragma unroll
         for (int e = 0; e < G::PE; ++e) {
             int kx, ky; size_t gi;

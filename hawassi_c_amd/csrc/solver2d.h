@@ -385,7 +385,7 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         const bool brk_now = brk_on && tprev < t;
         args.eta_col0 = brk_now ? col0.p : nullptr;
         // the peak-search counters are zeroed by the first y-pass
-        args.zero_cnt = brk_now ? cm.pkcount.p : nullptr;
+        args.zero_cnt = brk_now ? cm.pkcount : nullptr;
         // the middle reference depth's share of each HSS (HSSgen_3depth)
         auto mid_m0 = [&]() {
             k3d_y1<T, NX, NY><<<gy, by, 0, stream>>>(args);
@@ -512,8 +512,8 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         const dim3 gy((NXH + YG::CC - 1) / YG::CC), by(YG::NT);
         int rc;
         if ((rc = cm.detect_begin(stream, eta_p.p, true))) return rc;
-        k_brk_hilb_y<T, NX, NY><<<gy, by, 0, stream>>>(args, hilx.p, cm.pkcount.p);
-        k_t_xinv<T, NX, NY><<<gx, bx, 0, stream>>>(hilx.p, hilp.p, nullptr, twx.p, cm.pkcount.p);
+        k_brk_hilb_y<T, NX, NY><<<gy, by, 0, stream>>>(args, hilx.p, cm.pkcount);
+        k_t_xinv<T, NX, NY><<<gx, bx, 0, stream>>>(hilx.p, hilp.p, nullptr, twx.p, cm.pkcount);
         HW_LAUNCH_CHECK();
         return cm.detect_end(stream, eta_p.p, hilp.p, u_p.p, v_p.p);
     }
@@ -597,27 +597,21 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         return HW_OK;
     }
 
-    C* K(int i) { return kbuf.p + (size_t)i * 3 * H * nb; }
-
-    // rhs() of the stage input ybase + h sum_j b_j K(j), with the stage
-    // combination done by the first y-pass (which also stores the input to
-    // ytmp for the later passes): no separate stage kernel
-    int rhs_stage(double t, const double2* ybase, double h, std::initializer_list<int> ks,
-                  std::initializer_list<double> bs, C* out) {
-        args.st_nk = 0;
+    int stage(double h, std::initializer_list<int> ks, std::initializer_list<double> bs) {
+        StageCoef sc{};
+        sc.nk = 0;
         auto bi = bs.begin();
         for (int kidx : ks) {
-            args.st_k[args.st_nk] = K(kidx);
-            args.st_b[args.st_nk] = *bi++;
-            args.st_nk++;
+            sc.k[sc.nk] = K(kidx);
+            sc.b[sc.nk] = *bi++;
+            sc.nk++;
         }
-        args.st_y = ybase;
-        args.st_h = h;
-        args.st_out = ytmp.p;
-        int rc = rhs(t, ytmp.p, out);
-        args.st_y = nullptr;
-        return rc;
+        k_rk_stage<T><<<grid1(3 * H), 256, 0, stream>>>(y.p, sc, h, ytmp.p, 3 * H);
+        HW_LAUNCH_CHECK();
+        return HW_OK;
     }
+
+    C* K(int i) { return kbuf.p + (size_t)i * 3 * H * nb; }
 
     // One attempted rkf45 step from (t0, y) with step h0, enqueued on the
     // stream: K0 = f(t0, y) first when need_k0 (gsl's dydt_in, computed once
@@ -626,15 +620,17 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
     // The norm's copy to the host is asynchronous; attempt_result waits for it.
     int attempt_issue(double t0, double h0, bool need_k0) override {
         int rc;
-        if (need_k0 && (rc = rhs_stage(t0, y.p, 0.0, {}, {}, K(0)))) return rc;
-        if ((rc = rhs_stage(t0 + RK_AH[0] * h0, y.p, h0, {0}, {RK_AH[0]}, K(1)))) return rc;
-        if ((rc = rhs_stage(t0 + RK_AH[1] * h0, y.p, h0, {0, 1}, {RK_B3[0], RK_B3[1]}, K(2)))) return rc;
-        if ((rc = rhs_stage(t0 + RK_AH[2] * h0, y.p, h0, {0, 1, 2}, {RK_B4[0], RK_B4[1], RK_B4[2]}, K(3)))) return rc;
-        if ((rc = rhs_stage(t0 + RK_AH[3] * h0, y.p, h0, {0, 1, 2, 3}, {RK_B5[0], RK_B5[1], RK_B5[2], RK_B5[3]},
-                            K(4))))
+        if (need_k0) {
+            if ((rc = stage(0.0, {}, {})) || (rc = rhs(t0, ytmp.p, K(0)))) return rc;
+        }
+        if ((rc = stage(h0, {0}, {RK_AH[0]})) || (rc = rhs(t0 + RK_AH[0] * h0, ytmp.p, K(1)))) return rc;
+        if ((rc = stage(h0, {0, 1}, {RK_B3[0], RK_B3[1]})) || (rc = rhs(t0 + RK_AH[1] * h0, ytmp.p, K(2)))) return rc;
+        if ((rc = stage(h0, {0, 1, 2}, {RK_B4[0], RK_B4[1], RK_B4[2]})) || (rc = rhs(t0 + RK_AH[2] * h0, ytmp.p, K(3)))) return rc;
+        if ((rc = stage(h0, {0, 1, 2, 3}, {RK_B5[0], RK_B5[1], RK_B5[2], RK_B5[3]})) ||
+            (rc = rhs(t0 + RK_AH[3] * h0, ytmp.p, K(4))))
             return rc;
-        if ((rc = rhs_stage(t0 + RK_AH[4] * h0, y.p, h0, {0, 1, 2, 3, 4},
-                            {RK_B6[0], RK_B6[1], RK_B6[2], RK_B6[3], RK_B6[4]}, K(5))))
+        if ((rc = stage(h0, {0, 1, 2, 3, 4}, {RK_B6[0], RK_B6[1], RK_B6[2], RK_B6[3], RK_B6[4]})) ||
+            (rc = rhs(t0 + RK_AH[4] * h0, ytmp.p, K(5))))
             return rc;
         k_zero_u64<<<1, 64, 0, stream>>>(rmax.p, 1);
         k_rk_final<T><<<dim3(2048), 256, 0, stream>>>(y.p, K(0), K(2), K(3), K(4), K(5), h0, eps_abs, eps_rel, ynew.p,
@@ -643,7 +639,9 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         if (brk_on) {
             // rkf45's dydt_out = f(t + h, y_new): its breaking side effects
             // are part of the reference's sequence (K(1) is free here)
-            if ((rc = rhs_stage(t0 + h0, ynew.p, 0.0, {}, {}, K(1)))) return rc;
+            k_convert<double2, C><<<grid1(3 * H), 256, 0, stream>>>(ynew.p, ytmp.p, 3 * H);
+            HW_LAUNCH_CHECK();
+            if ((rc = rhs(t0 + h0, ytmp.p, K(1)))) return rc;
         }
         HW_TRY(hipMemcpyAsync(h_rmax, rmax.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
         return HW_OK;
