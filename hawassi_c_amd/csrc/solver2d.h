@@ -633,14 +633,14 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
             (rc = rhs(t0 + RK_AH[4] * h0, ytmp.p, K(5))))
             return rc;
         k_zero_u64<<<1, 64, 0, stream>>>(rmax.p, 1);
+        // with breaking, the final kernel also writes y_new in T for the
+        // dydt_out evaluation below
         k_rk_final<T><<<dim3(2048), 256, 0, stream>>>(y.p, K(0), K(2), K(3), K(4), K(5), h0, eps_abs, eps_rel, ynew.p,
-                                                        rmax.p, 3 * H);
+                                                        rmax.p, 3 * H, brk_on ? ytmp.p : nullptr);
         HW_LAUNCH_CHECK();
         if (brk_on) {
             // rkf45's dydt_out = f(t + h, y_new): its breaking side effects
             // are part of the reference's sequence (K(1) is free here)
-            k_convert<double2, C><<<grid1(3 * H), 256, 0, stream>>>(ynew.p, ytmp.p, 3 * H);
-            HW_LAUNCH_CHECK();
             if ((rc = rhs(t0 + h0, ytmp.p, K(1)))) return rc;
         }
         HW_TRY(hipMemcpyAsync(h_rmax, rmax.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));

@@ -799,12 +799,10 @@ template <typename T> struct Solver2DG : Solver2D {
             return rc;
         k_zero_u64<<<1, 64, 0, stream>>>(rmax.p, 1);
         k_rk_final<T><<<dim3(1024), 256, 0, stream>>>(y.p, K(0), K(2), K(3), K(4), K(5), h0, eps_abs, eps_rel, ynew.p,
-                                                        rmax.p, 3 * N);
+                                                        rmax.p, 3 * N, brk_on ? ytmp.p : nullptr);
         HW_LAUNCH_CHECK();
         if (brk_on) {
             // rkf45's dydt_out = f(t + h, y_new) with its breaking side effects
-            k_convert<double2, C><<<grid1(3 * N), 256, 0, stream>>>(ynew.p, ytmp.p, 3 * N);
-            HW_LAUNCH_CHECK();
             if ((rc = rhs(t0 + h0, ytmp.p, K(1)))) return rc;
         }
         HW_TRY(hipMemcpyAsync(h_rmax, rmax.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
