@@ -31,11 +31,12 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes actually moved per RHS evaluation of the 2048^2 harbour, FP32 path:
-# FETCH_SIZE and WRITE_SIZE counter passes over this script (FETCH_SIZE scaled
-# by 2.0, WRITE_SIZE by 1.0, the factors that make a known 512 MiB copy read
-# right), summed over the evaluation's kernels; profiles/rhs2d_harbour2048_fp32.md.
-RHS_TRAFFIC_BYTES_2048 = 2081.9e6
-# the same for the AB3 (HS3) right-hand side, profiles/rhs2d_harbour2048_hs3_fp32.md
+# FETCH_SIZE and WRITE_SIZE counter passes (rocprofv3 --pmc) over this script
+# (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0, the factors that make a known
+# 512 MiB copy read right), summed over the evaluation's kernels: the nine
+# transform passes (2046.7 MB) and the peak search's share (29.1 MB).
+RHS_TRAFFIC_BYTES_2048 = 2075.8e6
+# the same for the AB3 (HS3) right-hand side
 RHS_TRAFFIC_BYTES_2048_HS3 = 2963.3e6
 METRIC = "sim-sec/wall-sec + RHS evals/s, 2D 2048^2 FP32 vs FP64 oracle at relL2<=1e-4"
 
