@@ -79,11 +79,12 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--workload", default="harbour2048",
-                    choices=["harbour2048", "slope1d", "opensea1024x8", "parareal512", "ensemble"],
+                    choices=["harbour2048", "slope1d", "opensea1024x8", "parareal512", "ensemble", "refcase2d"],
                     help="harbour2048: the 2-D headline; slope1d: the 1-D batch of uneven realizations; "
                          "opensea1024x8: eight flat-bottom 1024^2 realizations advanced in lockstep; "
                          "parareal512: learned parareal on the 512^2 shoal basin (trains the network first); "
-                         "ensemble: 84 sea states on 128^2/256^2/512^2 grids, members and HIP-graph steps")
+                         "ensemble: 84 sea states on 128^2/256^2/512^2 grids, members and HIP-graph steps; "
+                         "refcase2d: the reference case 2dFBr001_Irreg (600x300, breaking) whole, full-grid path")
     ap.add_argument("--pr-seeds", type=int, default=33, help="parareal512: training trajectories (61 snapshots each)")
     ap.add_argument("--pr-epochs", type=int, default=30, help="parareal512: training epochs")
     ap.add_argument("--pr-window", type=int, default=8, help="parareal512: slices per parareal window")
@@ -119,6 +120,8 @@ def main():
         return bench_parareal(args)
     if args.workload == "ensemble":
         return bench_ensemble(args)
+    if args.workload == "refcase2d":
+        return bench_refcase2d(args)
 
     from hawassi_c_amd import HW_PREC_FP32, HW_PREC_FP64
     from hawassi_c_amd import workloads as W
@@ -456,6 +459,115 @@ def bench_ensemble(args):
     }
     for d in ctxs:
         d.close()
+    print(json.dumps(out))
+
+
+def bench_refcase2d(args):
+    """The reference's own 2-D case 2dFBr001_Irreg (600 x 300, flat bottom,
+    HS2, irregular influx from its influx.dat, breaking on), set up from its
+    files and run whole (0 -> 150 s) on the full-grid path, FP64 and FP32;
+    outputs every 5 s.  Accuracy FP32 against FP64 for eta, phi and the
+    breaking mask B at the output times where breaking is active, and one
+    second from the FP64 state at the first of them against the FP64 oracle
+    (the reference's algorithm on the CPU)."""
+    import importlib.util
+    import tempfile
+
+    from hawassi_c_amd import HW_PREC_FP32, HW_PREC_FP64
+    from hawassi_c_amd import case2d as C
+    from hawassi_c_amd.model2d import Hawassi2D
+    from oracle import oracle as O
+
+    name = "2dFBr001_Irreg"
+    here = os.path.dirname(os.path.abspath(__file__))
+    spec = importlib.util.spec_from_file_location("mf2d", os.path.join(here, "tests", "golden", "case2d", "make_fixtures.py"))
+    mf = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mf)
+    m = C.model_from_input(mf.materialize(name, os.path.join(tempfile.mkdtemp(), name)))
+    t_end = float(m.meta["t_end"])
+    outs = np.arange(5.0, t_end + 1e-9, 5.0)
+    runs = {}
+    for prec, tag in ((HW_PREC_FP64, "fp64"), (HW_PREC_FP32, "fp32")):
+        d = Hawassi2D(m, prec)
+        d.set_state(np.zeros(6 * m.nx * m.ny))
+        d.set_tprev(0.0)
+        t, h = 0.0, 0.1
+        snaps = {}
+        w0 = time.perf_counter()
+        for t1 in outs:
+            t, h = d.integrate(t, float(t1), h)
+            B, f = d.breaking_state()
+            snaps[float(t1)] = (d.get_state(), B, f, h)
+            log(f"  {tag} t={t:.0f} h={h:.4f} flagbr={f} B>0={int((B > 0).sum())}")
+        d.sync()
+        wall = time.perf_counter() - w0
+        runs[tag] = (snaps, wall, d.stats())
+        d.close()
+    def rel(a, b):
+        n = np.linalg.norm(b)
+        return float(np.linalg.norm(a - b) / n) if n > 0 else float(np.linalg.norm(a - b))
+    acc = []
+    for tt in outs:
+        y64, B64, f64, _ = runs["fp64"][0][float(tt)]
+        y32, B32, f32, _ = runs["fp32"][0][float(tt)]
+        if not ((B64 > 0).any() or (B32 > 0).any()):
+            continue
+        p64, p32 = physical_fields(m, y64), physical_fields(m, y32)
+        acc.append({"t": float(tt), "relL2_eta": rel(p32["eta"], p64["eta"]), "relL2_phi": rel(p32["phi"], p64["phi"]),
+                    "relL2_B": rel(B32, B64), "breaking_nodes": [int((B64 > 0).sum()), int((B32 > 0).sum())],
+                    "flagbr": [f64, f32]})
+    # cpu_baseline leg: the FP64 oracle (restated CPU path) over one second
+    # from the FP64 state at the first output time with breaking active,
+    # timed, and the device's FP64 run over the same second checked against it
+    vs_oracle = None
+    if acc and not args.no_cpu_baseline:
+        t0 = acc[0]["t"]
+        y0, _, _, h0 = runs["fp64"][0][t0]
+        o = O.Oracle2D(m)
+        d = Hawassi2D(m, HW_PREC_FP64)
+        d.set_state(y0)
+        o.set_tprev(t0)
+        d.set_tprev(t0)
+        yo = y0.copy()
+        w0 = time.perf_counter()
+        rc, to, _, so = o.integrate(t0, t0 + 1.0, h0, yo, m.eps_abs, m.eps_rel)
+        wo = time.perf_counter() - w0
+        d.integrate(t0, t0 + 1.0, h0)
+        yd = d.get_state()
+        st = d.stats()
+        pd, po = physical_fields(m, yd), physical_fields(m, yo)
+        Bd, fd = d.breaking_state()
+        d.close()
+        vs_oracle = {"window_s": [t0, t0 + 1.0], "steps_oracle": list(so),
+                     "steps_device": [st["steps_accepted"], st["steps_rejected"]],
+                     "relL2_eta": rel(pd["eta"], po["eta"]), "relL2_phi": rel(pd["phi"], po["phi"]),
+                     "relL2_B": rel(Bd, o.B()), "flagbr": [fd, o.flagbr()], "oracle_breaking_calls": o.break_calls(),
+                     "oracle_wall_s": wo}
+        log(f"  vs oracle: {vs_oracle}")
+    s64, w64, st64 = runs["fp64"]
+    s32, w32, st32 = runs["fp32"]
+    out = {
+        "metric": "sim-sec/wall-sec (reference case 2dFBr001_Irreg, whole run, full-grid path)",
+        "value": t_end / w32,
+        "unit": "sim-s/wall-s",
+        "n_gpus": 1, "steps": len(outs), "warmup": 0,
+        "ms_per_step": w32 / len(outs) * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "the reference's own case files (input.txt, influx.dat)",
+        "config": {"workload": "2dFBr001_Irreg 600x300, flat, HS2, breaking (KBC from its input.txt), 0-%g s, "
+                               "outputs every 5 s, full-grid path" % t_end},
+        "fp64": {"sim_s_per_wall_s": t_end / w64, "wall_s": w64, "rhs_evals": st64["rhs_evals"],
+                 "steps": [st64["steps_accepted"], st64["steps_rejected"]]},
+        "fp32": {"sim_s_per_wall_s": t_end / w32, "wall_s": w32, "rhs_evals": st32["rhs_evals"],
+                 "steps": [st32["steps_accepted"], st32["steps_rejected"]]},
+        "accuracy_fp32_vs_fp64_breaking_active": acc,
+    }
+    if vs_oracle:
+        out["cpu_baseline"] = {"value": 1.0 / vs_oracle["oracle_wall_s"], "unit": "sim-s/wall-s",
+                               "cores": O.lib().orc_num_threads(), "kind": "port",
+                               "sample": "1 s of the case from the device FP64 state at t = %g s (breaking active); "
+                                         "FP64 oracle, OpenMP, its own 2/3/5 FFT" % vs_oracle["window_s"][0]}
+        out["fp64_vs_oracle"] = vs_oracle
     print(json.dumps(out))
 
 
