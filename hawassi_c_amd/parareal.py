@@ -298,34 +298,71 @@ def train(net, X, Y, epochs: int, batch: int, lr: float, seed: int = 0, log=None
 
 class Parareal:
     """Parareal over `nslices` slices of length `dt` for the autonomous
-    problem `model`: the fine propagator runs in `nslices` device contexts
-    advanced together (hw_integrate_batch)."""
+    problem `model`.  The fine propagator of an iteration advances every
+    slice together: by default one context holding the slices as members,
+    every kernel covering all of them, step control on the device and each
+    attempted step a HIP-graph replay (hw_ctx_create_2d_members); the serial
+    fine solver is a single context stepping the same way, so batched and
+    serial slices agree bit for bit.  device_steps=False uses one context per
+    slice in lockstep with the step control on the host (hw_integrate_batch)."""
 
     def __init__(self, model, coarse, nslices: int, dt: float, precision: int = _lib.HW_PREC_FP32, h0: float = 0.05,
-                 grow_limit: float = 4.0):
+                 grow_limit: float = 4.0, device_steps: bool = True):
         self.m, self.G, self.N, self.dt, self.h0 = model, coarse, nslices, dt, h0
         # the basin conserves energy: a slice whose field norm exceeds
         # grow_limit times the initial one has diverged, and the fine solver
         # would crawl through its noise
         self.grow_limit = grow_limit
         self.codec = Codec(model)
-        self.ctx = [Hawassi2D(model, precision) for _ in range(nslices)]
+        self.precision = precision
+        self.device_steps = device_steps
+        self.single = Hawassi2D(model, precision)
+        if device_steps:
+            self.single.set_step_mode(True)
+            self.ctx = []
+            self.members = {}             # slice count -> context holding that many members
+        else:
+            self.ctx = [self.single] + [Hawassi2D(model, precision) for _ in range(nslices - 1)]
         self.fine_rhs = 0
 
     def close(self):
         for c in self.ctx:
             c.close()
+        if self.device_steps:
+            for c in self.members.values():
+                c.close()
+            self.members = {}
+            self.single.close()
         self.ctx = []
+
+    def _members(self, count):
+        if count not in self.members:
+            self.members[count] = Hawassi2D(self.m, self.precision, members=count)
+        return self.members[count]
 
     def fine(self, states, first: int = 0):
         """F(U_n) for n >= first, all together; returns the list (None below first)."""
+        out = [None] * self.N
+        if self.device_steps:
+            c = self._members(self.N)
+            ys = np.zeros((self.N, c.dim))
+            for i in range(first, self.N):
+                ys[i] = self.codec.to_state(*states[i])
+            c.set_state(ys)
+            c.reset_stats()
+            # slices below `first` start at t1: they do not move
+            c.integrate_members([self.dt if i < first else 0.0 for i in range(self.N)], self.dt, self.h0)
+            Y = c.get_state()
+            for i in range(first, self.N):
+                out[i] = self.codec.from_state(Y[i])
+            self.fine_rhs += c.stats()["rhs_evals"]
+            return out
         act = list(range(first, self.N))
         cs = [self.ctx[i] for i in act]
         for i, c in zip(act, cs):
             c.set_state(self.codec.to_state(*states[i]))
             c.reset_stats()
         integrate_batch(cs, [0.0] * len(cs), self.dt, [self.h0] * len(cs))
-        out = [None] * self.N
         for i, c in zip(act, cs):
             out[i] = self.codec.from_state(c.get_state())
             self.fine_rhs += c.stats()["rhs_evals"]
@@ -333,7 +370,7 @@ class Parareal:
 
     def fine_serial(self, u0, first: int = 0, start=None):
         """The plain fine solver, slice after slice (one context)."""
-        c = self.ctx[0]
+        c = self.single
         U = [None] * (self.N + 1)
         U[first] = u0 if start is None else start
         for n in range(first, self.N):
