@@ -1,7 +1,8 @@
 """A reference 2-D case directory (input.txt, bath.dat, influx.dat) turned
 into the Model2D the library integrates: the setup the reference's main runs
 before its time loop (2d/main.c:103-150), for what the device path supports
-(power-of-two grids of at least 32 points a side, no walls, no runup).
+(grids of 2, 3 and 5 factors, walls from wall.dat or input.txt rectangles,
+no runup).
 
 * grid, wavenumbers, bathymetry, anti-alias mask, damping: fun_param_init
   (2d/functions.c:667-874);
@@ -225,8 +226,6 @@ def read_influx(path, nxy: int, nt: int):
 def model_from_input(case_dir: str, eps_abs: float = 1e-3, eps_rel: float = 1e-2) -> S.Model2D:
     """Builds the Model2D of a reference case directory."""
     p = io2d.read_input(os.path.join(case_dir, "input.txt"))
-    if p.wall != "no" and io2d._atoi(p.wall) != 0:
-        raise NotImplementedError("walls from input.txt are not supported")
     if p.wavename not in ("harmonic", "user_defined"):
         raise NotImplementedError(f"influx '{p.wavename}' is not supported (harmonic, user_defined)")
     if p.order == 0:
@@ -257,7 +256,23 @@ def model_from_input(case_dir: str, eps_abs: float = 1e-3, eps_rel: float = 1e-2
     t = p.t_start + np.arange(nt) * p.dt
     dt = t[2] - t[1]
     cfsa = S.cfsa2d(x, y, *p.damp)
-    fbdy = S.fbdy2d(x, y, *p.damp, cfsa, wall=False, model=p.model)
+    # ---- fun_wallchar (2d/functions.c:260-390): any wall word but "no" has a
+    # wall, wall.dat for "user_defined", else atoi(word) rectangles
+    wall = p.wall != "no"
+    wallchar = np.ones((ny, nx))
+    gwx = gwy = None
+    if wall:
+        if p.wall == "user_defined":
+            wallchar, cfsaw = S.wallchar_user(S.read_wall_dat(os.path.join(case_dir, "wall.dat"), nx, ny),
+                                              p.refl_coef, p.model)
+        else:
+            wallchar, cfsaw = S.wallchar_rect(x, y, p.walls, p.refl_coef, p.model, p.wallmethod)
+        # fun_fbdy2D: cfSA = min(cfSA, cfSAwall) (2d/functions.c:1964-1970)
+        cfsa = np.minimum(cfsa, cfsaw)
+        # fun_gradient of wallchar (2d/functions.c:741), read by the flat
+        # bottom's reflecting-wall branch of fun_delphiH1hat
+        gwx, gwy = S.gradient_clipped(wallchar, dx, dy)
+    fbdy = S.fbdy2d(x, y, *p.damp, cfsa, wall=wall, model=p.model)
     vertical = p.orientation == "vertical"
     nxy = ny if vertical else nx
     # ---- fun_generation_influx
@@ -366,7 +381,8 @@ def model_from_input(case_dir: str, eps_abs: float = 1e-3, eps_rel: float = 1e-2
         brk = S.breaking_params(Hs, Tp, kp, cfsa, chi, KBC=p.KBC, TC=p.TC, Tchar=p.Tchar, delb=p.delb)
     prop = {"Uni+": 0, "Uni-": 1}.get(p.propagation, 2)
     return S.Model2D(nx=nx, ny=ny, x=x, y=y, kx=kx, ky=ky, flat=flat, cpeak=cpeak, aal=aal, fbdy=fbdy,
-                     wallchar=np.ones((ny, nx)), chi=chi, depth=-bathy, wall=False, ops=ops, source_on=True,
+                     wallchar=wallchar, chi=chi, depth=-bathy, wall=wall, refl_one=wall and p.refl_coef == 1.0,
+                     grad_wall_x=gwx, grad_wall_y=gwy, ops=ops, source_on=True,
                      vertical=vertical, propagation=prop, new_gam=ng, skew_gam=sg, t_sig=t, sig=sig,
                      skew_sig=skew, eps_abs=eps_abs, eps_rel=eps_rel, breaking=brk, order=p.order,
                      meta=dict(name=os.path.basename(os.path.normpath(case_dir)), tp=Tp, hs=Hs, kp=kp, lam=lam,

@@ -400,6 +400,76 @@ def wallchar_user(wall_ind: np.ndarray, refl_coef: float, model: str = "HS2"):
     return 1 - wall_ind * co, cfsaw
 
 
+def wallchar_rect(x, y, walls, refl_coef: float, model: str = "HS2", wallmethod: str = "energy"):
+    """fun_wallchar, rectangle branch (2d/functions.c:285-345): walls given in
+    input.txt as (X0, Y0, X1, Y1) corners (read at 2d/functions.c:184-186).
+
+    * Each wall covers the points with X0' <= x < X1' and Y0' <= y < Y1',
+      the fun_Heav2d profiles (2d/functions1.c:336-367) with bottom = X0 - x[0]
+      and top = x[-1] - X1 (a bound of exactly 0 is dropped: X0 on the first
+      grid point starts the wall there, X1 on the last point includes it);
+      X0' = x[0] + (X0 - x[0]) and X1' = x[-1] - (x[-1] - X1) as the
+      reference rounds them.
+    * wallchar = 1 - cutoff(R) on every wall point and 1 elsewhere, unless
+      wallmethod is "influxing" (wallchar = 1 everywhere, 2d/functions.c:313).
+    * cfSAwall = 0 on the index box round((X0 - x[0])/dx) .. round((X1 -
+      x[0])/dx) (inclusive, C round) of the FIRST wall only and 1 elsewhere
+      (2d/functions.c:322-337): the reference damps inside wall 0 alone.
+    The wall index lists (idxWall, dirWall, 2d/functions.c:346-389) feed only
+    fun_swall, which nothing calls; they are not built.
+    Returns (wallchar, cfSAwall), [ny, nx]."""
+    x = np.asarray(x, float)
+    y = np.asarray(y, float)
+    nx, ny = len(x), len(y)
+    heav = np.ones((ny, nx))
+    for (x0, y0, x1, y1) in walls:
+        hx = _heav2d(x, x0 - x[0], x[nx - 1] - x1)
+        hy = _heav2d(y, y0 - y[0], y[ny - 1] - y1)
+        heav = heav * (1 - hy[:, None] * hx[None, :])
+    wallchar = np.ones((ny, nx))
+    if wallmethod != "influxing":
+        wallchar = np.where(heav == 0, 1 - cutoff(refl_coef, model), 1.0)
+    cfsaw = np.ones((ny, nx))
+    if len(walls):
+        from .io2d import c_round
+        dx, dy = x[1] - x[0], y[1] - y[0]
+        x0, y0, x1, y1 = walls[0]
+        ix0, ix1 = c_round((x0 - x[0]) / dx), c_round((x1 - x[0]) / dx)
+        iy0, iy1 = c_round((y0 - y[0]) / dy), c_round((y1 - y[0]) / dy)
+        cfsaw[iy0:iy1 + 1, ix0:ix1 + 1] = 0.0
+    return wallchar, cfsaw
+
+
+def read_wall_dat(path, nx: int, ny: int) -> np.ndarray:
+    """wall.dat of a user_defined wall (2d/functions.c:268-283): Ny rows of
+    Nx values read with fscanf("%lf "), 1 on the wall and 0 elsewhere.
+    The reference's loop tests !fscanf(...), so
+    * past the end of the file (fscanf returns EOF, -1) the loop goes on and
+      every remaining point keeps wallchar's initial value 1
+      (2d/functions.c:731), i.e. is read as wall;
+    * a value that does not parse (fscanf returns 0) ends its row, and every
+      later row at its first point (the stream stays on that token), so
+      those points keep the no-wall values."""
+    from .io2d import CStream
+    out = np.zeros((ny, nx))
+    cs = CStream(open(path, "rb").read())
+    stuck = False
+    for i in range(ny):
+        for j in range(nx):
+            cs.skip_ws()
+            if cs.eof():
+                out[i, j] = 1.0
+                continue
+            if stuck:
+                break
+            v = cs.scan("%lf ")
+            if not v:
+                stuck = True
+                break
+            out[i, j] = v[0]
+    return out
+
+
 def gradient_clipped(f: np.ndarray, dx: float, dy: float):
     """fun_gradient (2d/functions.c:392-461): periodic forward difference
     clipped to <=0 plus periodic backward difference clipped to >=0."""

@@ -160,3 +160,61 @@ def test_reference_cases_on_2_3_5_grids_build(tmp_path, name, shape, order, flat
     y = W.random_state(m, 3, amp=0.2 * m.meta["hs"], kfrac=0.3)
     r = O.Oracle2D(m).rhs(5.0, y)
     assert np.isfinite(r).all() and np.abs(r).max() > 0
+
+
+def _with_walls(src, dst, wall_line, walltype_line):
+    """A reference input.txt with its `wall` / `walltype` lines replaced."""
+    lines = src.read_text().split("\n")
+    out = []
+    for ln in lines:
+        key = ln.split("=")[0].strip() if "=" in ln else ""
+        if key == "wall":
+            out.append(wall_line)
+        elif key == "walltype":
+            out.append(walltype_line)
+        else:
+            out.append(ln)
+    dst.write_text("\n".join(out))
+
+
+def test_model_from_input_rectangle_walls(tmp_path):
+    """2dF001harm (flat, HS3) with two input.txt rectangles (read at
+    2d/functions.c:177-189): wallchar = 1 - cutoff(0.5) on the walls, cfSA
+    = min(cfSA, cfSAwall) with the first wall's box fully damped
+    (2d/functions.c:1964-1970), fbdy x10 because a wall exists and the model
+    is not HS1 (2d/functions.c:2069-2075), and the clipped wall gradient
+    for the reflecting branch of fun_delphiH1hat."""
+    base = tmp_path / "nowall"
+    base.mkdir()
+    shutil.copy(GOLD / "2dF001harm.txt", base / "input.txt")
+    m0 = C.model_from_input(str(base))
+    d = tmp_path / "walls"
+    d.mkdir()
+    _with_walls(GOLD / "2dF001harm.txt", d / "input.txt", "wall \t\t= 2 600 0 660 250\n 900 300 950 400",
+                "walltype \t= 0 0.5 energy")
+    m = C.model_from_input(str(d))
+    assert m.wall and not m.refl_one and m.flat
+    x, y = m.x, m.y
+    on0 = (x[None, :] >= 600) & (x[None, :] < 660) & (y[:, None] < 250)
+    on1 = (x[None, :] >= 900) & (x[None, :] < 950) & (y[:, None] >= 300)
+    co = 0.5349009
+    assert np.allclose(m.wallchar[on0 | on1], 1 - co, atol=1e-6)
+    assert np.all(m.wallchar[~(on0 | on1)] == 1.0)
+    # outside the first wall's damping box: ten times the wall-free damping
+    ix0, ix1 = C.io2d.c_round((600 - x[0]) / (x[1] - x[0])), C.io2d.c_round((660 - x[0]) / (x[1] - x[0]))
+    iy1 = C.io2d.c_round(250 / (y[1] - y[0]))
+    box = np.zeros_like(m.fbdy, dtype=bool)
+    box[:iy1 + 1, ix0:ix1 + 1] = True
+    assert np.allclose(m.fbdy[~box], 10 * m0.fbdy[~box], rtol=1e-14, atol=0)
+    # inside it cfSA = 0: fully damped, (1 - 0) / (0.9 L) x 10
+    assert np.all(m.fbdy[box] > 0) and np.all(m.fbdy[box] >= 10 * m0.fbdy[box])
+    # the second wall is not in cfSAwall (wall 0 only)
+    assert np.all(m.fbdy[on1] == 10 * m0.fbdy[on1])
+    # R = 1: the reflecting branch and the clipped gradient of wallchar
+    d1 = tmp_path / "refl1"
+    d1.mkdir()
+    _with_walls(GOLD / "2dF001harm.txt", d1 / "input.txt", "wall \t\t= 1 600 0 660 250", "walltype \t= 0 1 energy")
+    m1 = C.model_from_input(str(d1))
+    assert m1.refl_one and np.all(m1.wallchar[on0] == 0.0)
+    gx, _ = S.gradient_clipped(m1.wallchar, x[1] - x[0], y[1] - y[0])
+    assert np.array_equal(m1.grad_wall_x, gx) and np.abs(gx).max() > 0

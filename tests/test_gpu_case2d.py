@@ -177,3 +177,69 @@ def test_reference_case_trajectory_full_grid(tmp_path, name):
     assert rc == 0 and t_o == t_d == 1.5
     assert (st["steps_accepted"], st["steps_rejected"]) == tuple(so)
     assert e < 1e-9
+
+
+def _case_with_walls(tmp_path, src, name, wall_line, walltype_line, bath=None):
+    d = tmp_path / name
+    d.mkdir()
+    out = []
+    for ln in (GOLD / src).read_text().split("\n"):
+        key = ln.split("=")[0].strip() if "=" in ln else ""
+        out.append(wall_line if key == "wall" else walltype_line if key == "walltype" else ln)
+    (d / "input.txt").write_text("\n".join(out))
+    if bath is not None:
+        (d / "bath.dat").write_text("\n".join(" ".join("%.7e" % v for v in row) for row in bath))
+    return C.model_from_input(str(d))
+
+
+@pytest.mark.parametrize("which", ["flat_R05", "flat_R1", "two_depth_R05"])
+def test_case_rectangle_walls_parity(tmp_path, which):
+    """Walls given as input.txt rectangles (fun_wallchar's rectangle branch,
+    2d/functions.c:285-345): flat bottom with R = 0.5 and R = 1 (both
+    fun_delphiH1hat wall branches, on the full-grid path) and the two-depth
+    fused path with R = 0.5; device against the oracle on the case's setup,
+    and 1.5 s of a random sea in FP64 on the full-grid path with the oracle's
+    steps."""
+    if which.startswith("flat"):
+        r = "0.5" if which == "flat_R05" else "1"
+        m = _case_with_walls(tmp_path, "2dF001harm.txt", which, "wall \t\t= 2 600 0 660 250\n 900 300 950 400",
+                             "walltype \t= 0 %s energy" % r)
+    else:
+        nx, ny = 512, 256
+        xx = np.linspace(0, 3000, nx)
+        yy = np.linspace(-250, 1500, ny)
+        depth = 30.0 - 20.0 * xx[None, :] / 3000.0 + 2.0 * np.sin(yy[:, None] / 200.0)
+        m = _case_with_walls(tmp_path, "2dB001HarmRefr.txt", which, "wall \t\t= 1 1500 -250 1560 900",
+                             "walltype \t= 0 0.5 energy", bath=depth)
+    assert m.wall
+    say("rectangle walls", which, m.nx, m.ny, "flat", m.flat, "order", m.order)
+    y = W.random_state(m, 5, amp=0.5, kfrac=0.3)
+    ref = O.Oracle2D(m).rhs(37.3, y)
+    for prec, tol in ((HW_PREC_FP64, 1e-11), (HW_PREC_FP32, 3e-5)):
+        dev = Hawassi2D(m, prec)
+        got = dev.rhs(37.3, y)
+        if not dev.full_grid:
+            got, ref_c = HS.oracle_half(got, m.nx, m.ny), HS.oracle_half(ref, m.nx, m.ny)
+        else:
+            ref_c = ref
+        dev.close()
+        e = relerr(got, ref_c)
+        say("  prec", prec, "relL2", e)
+        assert e < tol
+    # the trajectory on the full-grid path, whose error norm covers the
+    # reference's whole state (the oracle's step sequence), from a sea state
+    # that already covers the walls
+    y0 = W.random_state(m, 9, amp=0.1, kfrac=0.2)
+    o = O.Oracle2D(m)
+    dev = Hawassi2D(m, HW_PREC_FP64, full_grid=True)
+    dev.set_state(y0)
+    rc, t_o, h_o, so = o.integrate(0.0, 1.5, 0.1, y0, m.eps_abs, m.eps_rel)
+    t_d, h_d = dev.integrate(0.0, 1.5, 0.1)
+    st = dev.stats()
+    got = dev.get_state()
+    dev.close()
+    e = relerr(got, y0)
+    say("  steps oracle", so, "device", st["steps_accepted"], st["steps_rejected"], "relL2", e)
+    assert rc == 0 and t_o == t_d == 1.5
+    assert (st["steps_accepted"], st["steps_rejected"]) == tuple(so)
+    assert e < 1e-9

@@ -118,6 +118,55 @@ def test_wallchar_user_defined():
     assert cfw.sum() == 48 - 7
 
 
+# ---------------------------------------------- fun_wallchar (rectangles) --
+def test_wallchar_rectangles_by_hand():
+    """fun_wallchar rectangle branch (2d/functions.c:285-345) on x = 0..15,
+    y = 0..9 (unit spacing).  Wall 0 = (4, 2, 8, 5): the fun_Heav2d profiles
+    cover 4 <= x < 8 and 2 <= y < 5 (the upper edges are excluded), while
+    the damping box cfSAwall = 0 spans the rounded indices 4..8 x 2..5
+    inclusive -- one column and one row more than the wall.  Wall 1 =
+    (0, 7, 15, 9) starts on the first point and ends on the last one (both
+    fun_Heav2d bounds are 0, so both ends are included); it changes wallchar
+    but not cfSAwall, which the reference builds from wall 0 alone."""
+    x = np.arange(16.0)
+    y = np.arange(10.0)
+    co = 0.5349009
+    wc, cfw = S.wallchar_rect(x, y, [(4.0, 2.0, 8.0, 5.0), (0.0, 7.0, 15.0, 9.0)], 0.5)
+    expect = np.ones((10, 16))
+    expect[2:5, 4:8] = 1 - co
+    expect[7:10, :] = 1 - co
+    assert np.allclose(wc, expect, atol=1e-6, rtol=0)
+    box = np.ones((10, 16))
+    box[2:6, 4:9] = 0.0
+    assert np.array_equal(cfw, box)
+    # a wall inside [x0, x1] with both edges (a misreading of the Heaviside
+    # profiles) would also mark column 8 and row 5
+    assert wc[3, 8] == 1.0 and wc[5, 5] == 1.0 and wc[4, 7] < 1.0
+    # "influxing" keeps wallchar at 1 (no energy truncation), the damping box stays
+    wc2, cfw2 = S.wallchar_rect(x, y, [(4.0, 2.0, 8.0, 5.0)], 0.5, wallmethod="influxing")
+    assert np.array_equal(wc2, np.ones((10, 16))) and np.array_equal(cfw2, box)
+    # HS1 uses its own cutoff coefficients
+    wc3, _ = S.wallchar_rect(x, y, [(4.0, 2.0, 8.0, 5.0)], 0.5, model="HS1")
+    assert wc3[3, 5] == pytest.approx(1 - 0.5556783, abs=1e-6)
+    # R = 1: fully reflecting, wallchar 0 on the wall
+    wc4, _ = S.wallchar_rect(x, y, [(4.0, 2.0, 8.0, 5.0)], 1.0)
+    assert wc4[3, 5] == 0.0 and wc4.sum() == 160 - 12
+
+
+def test_read_wall_dat_stops_as_fscanf_does(tmp_path):
+    """wall.dat is read with `if (!fscanf(...)) break` (2d/functions.c:271-
+    274): at the end of the file fscanf returns EOF (-1), the loop goes on and
+    the remaining points keep wallchar's initial 1, i.e. are wall; a token
+    that does not parse returns 0 and ends its row and every later row."""
+    f = tmp_path / "wall.dat"
+    f.write_text("0 1 0 1\n1 1 0 0\n0 0")
+    w = S.read_wall_dat(f, 4, 3)
+    assert w.tolist() == [[0, 1, 0, 1], [1, 1, 0, 0], [0, 0, 1, 1]]
+    f.write_text("0 1 0 1\n1 x 0 0\n1 1 1 1")
+    w = S.read_wall_dat(f, 4, 3)
+    assert w.tolist() == [[0, 1, 0, 1], [1, 0, 0, 0], [0, 0, 0, 0]]
+
+
 # ------------------------------------------------------------- fun_gradient --
 def test_gradient_clipped_step():
     """fun_gradient (2d/functions.c:392-461), used on the breaking flux
