@@ -27,6 +27,7 @@ struct Real1D {                 // read-only model description of one realizatio
     int n, flat, src, prop;     // prop: 0 Uni+, 1 Uni-, 2 Bi, 3 none
     int brk, nt, cap, log2n;
     long long off;              // element offset of this realization (sum of earlier n)
+    long long tw_off;           // offset of the length-n twiddle table in Batch1DArgs::tw
     long long sig_off;          // offset into the signal arrays
     long long cb_off;           // offset into the crest arrays (sum of earlier caps)
     double dx, KBC, TC, Tstar, MaxEtaInit, delb, eps_abs, eps_rel;
@@ -86,7 +87,7 @@ template <typename T> struct Batch1DArgs {
     Cb1 *cb, *cbn, *cbp;
     int* ide;
     Brk1D* bs;
-    const C* tw[16];            // tw[log2 n][m] = exp(-2 pi i m / n)
+    const C* tw;                // twiddle tables of every size present: exp(-2 pi i m / n) at tw_off + m
     T g;
 };
 
@@ -272,8 +273,21 @@ template <typename T, int N, int NT> struct P1 {
 // `flag` are per-realization device state.  Sequential parts run on thread 0
 // between barriers; scans over nodes are block-parallel reductions that keep
 // the reference's first-index semantics.
+// What breaking_process reads of the batch, handed over by value: a
+// reference to the argument struct would make its address escape into this
+// (non-inlined) call, keep the struct in scratch memory and turn every
+// pointer the RHS reads from it into a generic (flat) one.
+template <typename T> struct Brk1DView {
+    const Real1D* real;
+    const T* depth;
+    int* peaks;
+    Cb1 *cb, *cbn, *cbp;
+    int* ide;
+    Brk1D* bs;
+};
+
 template <typename T, int N, int NT>
-__device__ __attribute__((noinline)) void brk1d_process(const Batch1DArgs<T>& a, const Real1D& R, double t, const T* __restrict__ eta,
+__device__ __attribute__((noinline)) void brk1d_process(Brk1DView<T> a, const Real1D& R, double t, const T* __restrict__ eta,
                               const T* __restrict__ u, const T* __restrict__ H, T* __restrict__ B, int* sm,
                               double* smd) {
     const int tid = threadIdx.x;
@@ -487,7 +501,8 @@ __device__ __attribute__((noinline)) void brk1d_process(const Batch1DArgs<T>& a,
 // MID: the realization has a third reference depth (its code is a separate
 // instance so that the two-depth path keeps its register allocation)
 template <typename T, int N, int NT, bool MID>
-__device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename ctype<T>::type* lds, int* sm,
+__device__ __attribute__((always_inline)) void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage,
+                                                           typename ctype<T>::type* lds, int* sm,
                            double* smd) {
     using C = typename ctype<T>::type;
     using P = P1<T, N, NT>;
@@ -496,7 +511,7 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
     const Real1D& R = a.real[r];
     Rk1D* S = a.rk + r;
     const long long off = R.off;
-    const C* tw = a.tw[R.log2n];
+    const C* tw = a.tw + R.tw_off;
     const T g = a.g;
     const T* kk = a.k + off;
     T* sr = a.sr + off * R1_NR;
@@ -569,8 +584,12 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
             for (int q = 0; q < 4; ++q) {
                 double2 b = base[idx[q]];
                 double sx = 0.0, sy = 0.0;
-                for (int j = 0; j < nk; ++j) {
-                    if (co[j] != 0.0) {
+                // fully unrolled, so every a.K[j] is a constant index (a
+                // runtime index into the argument struct would copy it to
+                // scratch and turn every pointer read from it generic)
+#pragma unroll
+                for (int j = 0; j < 6; ++j) {
+                    if (j < nk && co[j] != 0.0) {
                         C kv = a.K[j][2 * off + idx[q]];
                         sx += co[j] * (double)kv.x;
                         sy += co[j] * (double)kv.y;
@@ -959,7 +978,8 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
     T* Bp = a.B + off;
     T* flux = sr + R1_FLUX * N;
     if (R.brk) {
-        brk1d_process<T, N, NT>(a, R, t, s_eta, s_u, sr + R1_H * N, Bp, sm, smd);
+        brk1d_process<T, N, NT>(Brk1DView<T>{a.real, a.depth, a.peaks, a.cb, a.cbn, a.cbp, a.ide, a.bs}, R, t, s_eta, s_u,
+                                  sr + R1_H * N, Bp, sm, smd);
         const T db2 = (T)(R.delb * R.delb);
 #pragma unroll 1
         for (int e = 0; e < EH; ++e) {
@@ -1007,7 +1027,16 @@ __device__ void rhs1d_body(const Batch1DArgs<T>& a, int r, int stage, typename c
     {
         const T pv = (T)smd[30], pvs = (T)smd[31];
         const T sw = R.wall ? (T)smd[32] : (T)0;
-        C* Ko = (stage == 0 || stage == 7) ? a.K[6] : a.K[stage - 1];
+        C* Ko;
+        switch (stage) {   // constant indices into the argument struct (see the stage input)
+            case 1: Ko = a.K[0]; break;
+            case 2: Ko = a.K[1]; break;
+            case 3: Ko = a.K[2]; break;
+            case 4: Ko = a.K[3]; break;
+            case 5: Ko = a.K[4]; break;
+            case 6: Ko = a.K[5]; break;
+            default: Ko = a.K[6]; break;
+        }
         Ko += 2 * off;
 #pragma unroll 1
         for (int e = 0; e < EH; ++e) {

@@ -74,7 +74,7 @@ template <typename T> struct Solver1DT : Solver1D {
     DevBuf<int> peaks, ide;
     DevBuf<Cb1> cb, cbn, cbp;
     DevBuf<Brk1D> bs;
-    DevBuf<C> tw[16];
+    DevBuf<C> tw;
     Batch1DArgs<T> args{};
     int* h_count = nullptr;
     // timing of one right-hand side launch of the whole batch: an event pair
@@ -217,18 +217,28 @@ template <typename T> struct Solver1DT : Solver1D {
                 (rc = skw.upload(hskw, stream)) || (rc = skwc.upload(hskwc, stream)))
                 return rc;
         }
-        // ---- twiddles exp(-2 pi i m / n) for every size present
-        for (int r = 0; r < n; ++r) {
-            int l = hreal[r].log2n;
-            if (tw[l].p) continue;
-            int nn = 1 << l;
-            std::vector<C> h(nn);
-            for (int m = 0; m < nn; ++m) {
-                double ang = -2.0 * M_PI * (double)m / (double)nn;
-                h[m].x = (T)std::cos(ang);
-                h[m].y = (T)std::sin(ang);
+        // ---- twiddles exp(-2 pi i m / n) for every size present, one table
+        // per size in one buffer (a realization finds its table at tw_off)
+        {
+            std::vector<C> h;
+            long long at[32];
+            for (int l = 0; l < 32; ++l) at[l] = -1;
+            for (int r = 0; r < n; ++r) {
+                const int l = hreal[r].log2n;
+                if (at[l] < 0) {
+                    const int nn = 1 << l;
+                    at[l] = (long long)h.size();
+                    for (int m = 0; m < nn; ++m) {
+                        const double ang = -2.0 * M_PI * (double)m / (double)nn;
+                        C w;
+                        w.x = (T)std::cos(ang);
+                        w.y = (T)std::sin(ang);
+                        h.push_back(w);
+                    }
+                }
+                hreal[r].tw_off = at[l];
             }
-            if ((rc = tw[l].upload(h, stream))) return rc;
+            if ((rc = tw.upload(h, stream))) return rc;
         }
         // ---- state, stages, scratch, breaking
         std::vector<Rk1D> hrk(n);
@@ -276,7 +286,7 @@ template <typename T> struct Solver1DT : Solver1D {
         a.ext = ext.p;
         a.sr = sr.p; a.sc = sc.p; a.B = B.p; a.peaks = peaks.p; a.cb = cb.p; a.cbn = cbn.p; a.cbp = cbp.p;
         a.ide = ide.p; a.bs = bs.p;
-        for (int l = 0; l < 16; ++l) a.tw[l] = tw[l].p;
+        a.tw = tw.p;
         a.g = (T)9.80665;
         HW_TRY(hipStreamSynchronize(stream));
         return HW_OK;
