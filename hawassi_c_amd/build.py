@@ -33,7 +33,9 @@ SIZES = (
 )
 
 
-# 1-D batched right-hand side: one unit per (precision, workgroup size)
+# 1-D batched right-hand side: one unit per (precision, size class n), the
+# step-control kernel per (precision, workgroup size)
+N1D = (128, 256, 512, 1024, 2048, 4096, 8192, 16384)
 NT1D = (64, 128, 256, 512, 1024)
 
 
@@ -43,23 +45,38 @@ def _units():
     units = [CSRC / "capi.hip", CSRC / "capi1d.hip", CSRC / "fno.hip", CSRC / "full2d.hip"]
     for prec in (0, 1):
         t = "double" if prec == 0 else "float"
-        for nt in NT1D:
-            f = gen / f"inst1d_p{prec}_nt{nt}.hip"
+        for n in N1D:
+            if n * (16 if prec == 0 else 8) > 131072:
+                continue      # one complex field of the realization in LDS
+            f = gen / f"inst1d_p{prec}_n{n}.hip"
+            k = f"k1d_rhs_n<{t}, nt_for_n({n}), {n}"
             text = (f'// generated by hawassi_c_amd/build.py\n#include "{CSRC}/solver1d.h"\n'
                     f"namespace hw {{\n"
-                    f"hipError_t k1d_launch_{prec}_{nt}(dim3 g, size_t lds, hipStream_t s, const Batch1DArgs<{t}>& a, int st) {{\n"
-                    f"    hipLaunchKernelGGL((k1d_rhs<{t}, {nt}>), g, dim3({nt}), lds, s, a, st);\n"
+                    f"hipError_t k1d_launch_{prec}_{n}(dim3 g, size_t lds, hipStream_t s, const Batch1DArgs<{t}>& a, "
+                    f"int st, int list0, int mid) {{\n"
+                    f"    if (mid) hipLaunchKernelGGL(({k}, true>), g, dim3(nt_for_n({n})), lds, s, a, st, list0);\n"
+                    f"    else hipLaunchKernelGGL(({k}, false>), g, dim3(nt_for_n({n})), lds, s, a, st, list0);\n"
                     f"    return hipGetLastError();\n}}\n"
-                    f"hipError_t k1d_control_{prec}_{nt}(dim3 g, hipStream_t s, const Batch1DArgs<{t}>& a) {{\n"
-                    f"    hipLaunchKernelGGL((k1d_control<{t}, {nt}>), g, dim3({nt}), 0, s, a);\n"
-                    f"    return hipGetLastError();\n}}\n"
-                    f"hipError_t k1d_setlds_{prec}_{nt}(size_t b) {{\n"
-                    f"    return hipFuncSetAttribute((const void*)k1d_rhs<{t}, {nt}>, "
+                    f"hipError_t k1d_setlds_{prec}_{n}(size_t b) {{\n"
+                    f"    hipError_t e = hipFuncSetAttribute((const void*){k}, true>, "
+                    f"hipFuncAttributeMaxDynamicSharedMemorySize, (int)b);\n"
+                    f"    if (e != hipSuccess) return e;\n"
+                    f"    return hipFuncSetAttribute((const void*){k}, false>, "
                     f"hipFuncAttributeMaxDynamicSharedMemorySize, (int)b);\n}}\n"
                     f"}}  // namespace hw\n")
             if not f.exists() or f.read_text() != text:
                 f.write_text(text)
             units.append(f)
+        f = gen / f"inst1d_p{prec}_ctl.hip"
+        text = f'// generated by hawassi_c_amd/build.py\n#include "{CSRC}/solver1d.h"\nnamespace hw {{\n'
+        for nt in NT1D:
+            text += (f"hipError_t k1d_control_{prec}_{nt}(dim3 g, hipStream_t s, const Batch1DArgs<{t}>& a, int list0) {{\n"
+                     f"    hipLaunchKernelGGL((k1d_control<{t}, {nt}>), g, dim3({nt}), 0, s, a, list0);\n"
+                     f"    return hipGetLastError();\n}}\n")
+        text += "}  // namespace hw\n"
+        if not f.exists() or f.read_text() != text:
+            f.write_text(text)
+        units.append(f)
     for prec, nx, ny in SIZES:
         t = "double" if prec == 0 else "float"
         f = gen / f"inst_p{prec}_{nx}x{ny}.hip"

@@ -1,22 +1,40 @@
 // capi1d.hip -- the extern "C" 1-D entry points declared in
-// include/hawassi_hip.h and the launch dispatch over thread counts.
+// include/hawassi_hip.h and the launch dispatch over size classes.
 #include <hip/hip_runtime.h>
 
 #include "solver1d.h"
 
 namespace hw {
 
-// defined in the generated per-(precision, thread count) units
-#define HW_1D_DECL(T, P, NT)                                                                              \
-    hipError_t k1d_launch_##P##_##NT(dim3, size_t, hipStream_t, const Batch1DArgs<T>&, int);              \
-    hipError_t k1d_control_##P##_##NT(dim3, hipStream_t, const Batch1DArgs<T>&);                          \
-    hipError_t k1d_setlds_##P##_##NT(size_t);
-#define HW_1D_DECL_ALL(T, P) HW_1D_DECL(T, P, 64) HW_1D_DECL(T, P, 128) HW_1D_DECL(T, P, 256) \
-    HW_1D_DECL(T, P, 512) HW_1D_DECL(T, P, 1024)
+// defined in the generated units: per (precision, size class n) the RHS
+// launch and its LDS attribute, per (precision, thread count) the control
+#define HW_1D_DECL_N(T, P, N)                                                                             \
+    hipError_t k1d_launch_##P##_##N(dim3, size_t, hipStream_t, const Batch1DArgs<T>&, int, int, int);     \
+    hipError_t k1d_setlds_##P##_##N(size_t);
+#define HW_1D_DECL_NT(T, P, NT) hipError_t k1d_control_##P##_##NT(dim3, hipStream_t, const Batch1DArgs<T>&, int);
+#define HW_1D_DECL_ALL(T, P) HW_1D_DECL_N(T, P, 128) HW_1D_DECL_N(T, P, 256) HW_1D_DECL_N(T, P, 512) \
+    HW_1D_DECL_N(T, P, 1024) HW_1D_DECL_N(T, P, 2048) HW_1D_DECL_N(T, P, 4096) HW_1D_DECL_N(T, P, 8192) \
+    HW_1D_DECL_NT(T, P, 64) HW_1D_DECL_NT(T, P, 128) HW_1D_DECL_NT(T, P, 256) HW_1D_DECL_NT(T, P, 512) \
+    HW_1D_DECL_NT(T, P, 1024)
 HW_1D_DECL_ALL(float, 1)
 HW_1D_DECL_ALL(double, 0)
+HW_1D_DECL_N(float, 1, 16384)
 
-#define HW_1D_SWITCH(P, CALL, ...)                                          \
+#define HW_1D_SWITCH_N(P, CALL, ...)                                        \
+    switch (n) {                                                            \
+        case 128: return CALL##_##P##_128(__VA_ARGS__);                     \
+        case 256: return CALL##_##P##_256(__VA_ARGS__);                     \
+        case 512: return CALL##_##P##_512(__VA_ARGS__);                     \
+        case 1024: return CALL##_##P##_1024(__VA_ARGS__);                   \
+        case 2048: return CALL##_##P##_2048(__VA_ARGS__);                   \
+        case 4096: return CALL##_##P##_4096(__VA_ARGS__);                   \
+        case 8192: return CALL##_##P##_8192(__VA_ARGS__);                   \
+        HW_1D_CASE_16384_##P(CALL, __VA_ARGS__)                             \
+        default: return hipErrorInvalidValue;                               \
+    }
+#define HW_1D_CASE_16384_1(CALL, ...) case 16384: return CALL##_1_16384(__VA_ARGS__);
+#define HW_1D_CASE_16384_0(CALL, ...)
+#define HW_1D_SWITCH_NT(P, CALL, ...)                                       \
     switch (NT) {                                                           \
         case 64: return CALL##_##P##_64(__VA_ARGS__);                       \
         case 128: return CALL##_##P##_128(__VA_ARGS__);                     \
@@ -26,20 +44,22 @@ HW_1D_DECL_ALL(double, 0)
         default: return hipErrorInvalidValue;                               \
     }
 
-template <> hipError_t K1D<float>::launch(int NT, dim3 g, size_t lds, hipStream_t s, const Batch1DArgs<float>& a, int st) {
-    HW_1D_SWITCH(1, k1d_launch, g, lds, s, a, st)
+template <> hipError_t K1D<float>::launch(int n, dim3 g, size_t lds, hipStream_t s, const Batch1DArgs<float>& a, int st,
+                                          int list0, int mid) {
+    HW_1D_SWITCH_N(1, k1d_launch, g, lds, s, a, st, list0, mid)
 }
-template <> hipError_t K1D<float>::control(int NT, dim3 g, hipStream_t s, const Batch1DArgs<float>& a) {
-    HW_1D_SWITCH(1, k1d_control, g, s, a)
+template <> hipError_t K1D<float>::control(int NT, dim3 g, hipStream_t s, const Batch1DArgs<float>& a, int list0) {
+    HW_1D_SWITCH_NT(1, k1d_control, g, s, a, list0)
 }
-template <> hipError_t K1D<float>::set_lds(int NT, size_t b) { HW_1D_SWITCH(1, k1d_setlds, b) }
-template <> hipError_t K1D<double>::launch(int NT, dim3 g, size_t lds, hipStream_t s, const Batch1DArgs<double>& a, int st) {
-    HW_1D_SWITCH(0, k1d_launch, g, lds, s, a, st)
+template <> hipError_t K1D<float>::set_lds(int n, size_t b) { HW_1D_SWITCH_N(1, k1d_setlds, b) }
+template <> hipError_t K1D<double>::launch(int n, dim3 g, size_t lds, hipStream_t s, const Batch1DArgs<double>& a, int st,
+                                           int list0, int mid) {
+    HW_1D_SWITCH_N(0, k1d_launch, g, lds, s, a, st, list0, mid)
 }
-template <> hipError_t K1D<double>::control(int NT, dim3 g, hipStream_t s, const Batch1DArgs<double>& a) {
-    HW_1D_SWITCH(0, k1d_control, g, s, a)
+template <> hipError_t K1D<double>::control(int NT, dim3 g, hipStream_t s, const Batch1DArgs<double>& a, int list0) {
+    HW_1D_SWITCH_NT(0, k1d_control, g, s, a, list0)
 }
-template <> hipError_t K1D<double>::set_lds(int NT, size_t b) { HW_1D_SWITCH(0, k1d_setlds, b) }
+template <> hipError_t K1D<double>::set_lds(int n, size_t b) { HW_1D_SWITCH_N(0, k1d_setlds, b) }
 
 }  // namespace hw
 

@@ -129,6 +129,31 @@ __device__ __forceinline__ void fft_lds_pass(C* lds, int LS, const C* __restrict
         constexpr int M = L / R;             // butterflies per sequence
         constexpr int NB = B * M;
         constexpr int PER = (NB + NT - 1) / NT;
+        // The twiddles a butterfly needs depend only on its index: their
+        // loads are issued first, so they are in flight while the inputs come
+        // from LDS and the block meets at the barrier.  Radix 2/4: every
+        // w^r from the table; radix 8/16: w, w^4 (and w^8), the other powers
+        // formed below (at most two roundings more than a table entry).
+        constexpr int NTW = NS == 1 ? 1 : (R <= 4 ? R - 1 : (R == 16 ? 3 : 2));
+        C wt[PER][NTW];
+        if constexpr (NS > 1) {
+#pragma unroll
+            for (int p = 0; p < PER; ++p) {
+                int bf = tid + p * NT;
+                if (NB % NT == 0 || bf < NB) {
+                    const int k = (bf % M) % NS;
+                    const int ex = k * (L / (NS * R));
+                    if constexpr (R <= 4) {
+#pragma unroll
+                        for (int r = 1; r < R; ++r) wt[p][r - 1] = tw[(r * ex) * TS];
+                    } else {
+                        wt[p][0] = tw[ex * TS];
+                        wt[p][1] = tw[4 * ex * TS];
+                        if constexpr (R == 16) wt[p][2] = tw[8 * ex * TS];
+                    }
+                }
+            }
+        }
         C v[PER][R];
 #pragma unroll
         for (int p = 0; p < PER; ++p) {
@@ -147,19 +172,16 @@ __device__ __forceinline__ void fft_lds_pass(C* lds, int LS, const C* __restrict
                 int b = bf / M, j = bf % M;
                 int k = j % NS;
                 if constexpr (NS > 1) {
-                    const int ex = k * (L / (NS * R));
                     if constexpr (R <= 4) {
 #pragma unroll
                         for (int r = 1; r < R; ++r) {
-                            C w = tw[(r * ex) * TS];
+                            C w = wt[p][r - 1];
                             if (DIR > 0) w = cconj(w);
                             v[p][r] = cmul(v[p][r], w);
                         }
                     } else {
-                        // w^r from w, w^4 (and w^8) read from the table:
-                        // at most two roundings more than a table entry
                         C wp[R];
-                        C w1 = tw[ex * TS], w4 = tw[4 * ex * TS];
+                        C w1 = wt[p][0], w4 = wt[p][1];
                         if (DIR > 0) { w1 = cconj(w1); w4 = cconj(w4); }
                         wp[1] = w1;
                         wp[2] = cmul(w1, w1);
@@ -169,7 +191,7 @@ __device__ __forceinline__ void fft_lds_pass(C* lds, int LS, const C* __restrict
                         wp[6] = cmul(w4, wp[2]);
                         wp[7] = cmul(w4, wp[3]);
                         if constexpr (R == 16) {
-                            C w8 = tw[8 * ex * TS];
+                            C w8 = wt[p][2];
                             if (DIR > 0) w8 = cconj(w8);
                             wp[8] = w8;
 #pragma unroll

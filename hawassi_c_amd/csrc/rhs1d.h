@@ -12,8 +12,8 @@
 // physical) indices k = t + e*NT (k < n/2) and their partners n - k (0 is
 // paired with n/2), so the Hermitian splits are thread-local.
 //
-// Batches mix sizes: the kernel switches on the realization's n; all blocks
-// of a launch use the same thread count NT with 2 <= n/NT <= 16.
+// Batches mix sizes: the realizations are grouped by n into size classes,
+// each with its own kernel instance and thread count (nt_for_n).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -1068,32 +1068,30 @@ __device__ __attribute__((always_inline)) void rhs1d_body(const Batch1DArgs<T>& 
 #undef HW1_GC
 #undef HW1_AC
 
-template <typename T, int NT>
-__global__ void __launch_bounds__(NT) k1d_rhs(Batch1DArgs<T> a, int stage) {
+// One size class of the batch: the realizations of n = N (MID: with a
+// third reference depth) listed at a.list[list0 ..], one block each.  Each
+// class has a kernel of its own, so the body is compiled once per size with
+// its own register allocation (one kernel switching over every size spilled
+// several times more), and the classes of a batch run side by side on their
+// own streams.
+template <typename T, int NT, int N, bool MID>
+__global__ void __launch_bounds__(NT) k1d_rhs_n(Batch1DArgs<T> a, int stage, int list0) {
     using C = typename ctype<T>::type;
     extern __shared__ unsigned char hw_lds1d[];
     __shared__ int sm[64];
     __shared__ double smd[40];
-    const int r = a.list[blockIdx.x];
+    const int r = a.list[list0 + (int)blockIdx.x];
     Rk1D* S = a.rk + r;
     if (stage >= 1) {
         if (!S->active) return;
         if (stage == 1 && !S->need_in) return;
     }
-    C* lds = reinterpret_cast<C*>(hw_lds1d);
-    switch (a.real[r].n) {
-#define HW_1D_CASE(NN)                                                         \
-    case NN:                                                                   \
-        if constexpr (NN >= 2 * NT && NN <= 16 * NT && NN * sizeof(C) <= 131072) \
-            { if (a.real[r].mid) rhs1d_body<T, NN, NT, true>(a, r, stage, lds, sm, smd);          \
-              else rhs1d_body<T, NN, NT, false>(a, r, stage, lds, sm, smd); }                   \
-        break;
-        HW_1D_CASE(128) HW_1D_CASE(256) HW_1D_CASE(512) HW_1D_CASE(1024) HW_1D_CASE(2048)
-        HW_1D_CASE(4096) HW_1D_CASE(8192) HW_1D_CASE(16384)
-#undef HW_1D_CASE
-        default: break;
-    }
+    rhs1d_body<T, N, NT, MID>(a, r, stage, reinterpret_cast<C*>(hw_lds1d), sm, smd);
 }
+
+// the block size of size class n: two elements per thread up to n = 2048,
+// then 1024 threads with up to 16 elements each
+constexpr int nt_for_n(int n) { return n / 2 < 1024 ? n / 2 : 1024; }
 
 // ---------------------------------------------------- step control ----
 // Per realization: y_new = y + h0 (c1 K1 + c3 K3 + c4 K4 + c5 K5 + c6 K6),
@@ -1102,11 +1100,11 @@ __global__ void __launch_bounds__(NT) k1d_rhs(Batch1DArgs<T> a, int stage) {
 // on acceptance y <- y_new, t advances and the next evolve_apply call (which
 // evaluates dydt_in) is armed; the next attempt's step is clipped to t1.
 template <typename T, int NT>
-__global__ void __launch_bounds__(NT) k1d_control(Batch1DArgs<T> a) {
+__global__ void __launch_bounds__(NT) k1d_control(Batch1DArgs<T> a, int list0) {
     using C = typename ctype<T>::type;
     __shared__ double red[20];
     __shared__ int dec[2];
-    const int r = a.list[blockIdx.x];
+    const int r = a.list[list0 + (int)blockIdx.x];
     const Real1D& R = a.real[r];
     Rk1D* S = a.rk + r;
     if (!S->active) return;

@@ -34,7 +34,7 @@ inline void natural_spline_c(const double* x, const double* yv, int n, double* c
 }
 
 struct Solver1D {
-    int prec = 0, nreal = 0, NT = 0, maxn = 0, any_brk = 0;
+    int prec = 0, nreal = 0, maxn = 0, any_brk = 0;
     long round_limit = 0;   // rounds per integrate call before HW_ERR_STEP (0: no limit)
     hipStream_t stream = nullptr;
     std::vector<Real1D> hreal;
@@ -53,11 +53,22 @@ struct Solver1D {
     virtual int stats(int r, hw_stats* st) = 0;
 };
 
-// the RHS kernel instances available to a launch (one per NT)
+// the RHS kernel instances (one per size class n) and the step-control
+// kernels (one per thread count)
 template <typename T> struct K1D {
-    static hipError_t launch(int NT, dim3 grid, size_t lds, hipStream_t s, const Batch1DArgs<T>& a, int stage);
-    static hipError_t control(int NT, dim3 grid, hipStream_t s, const Batch1DArgs<T>& a);
-    static hipError_t set_lds(int NT, size_t bytes);
+    static hipError_t launch(int n, dim3 grid, size_t lds, hipStream_t s, const Batch1DArgs<T>& a, int stage, int list0,
+                             int mid);
+    static hipError_t control(int NT, dim3 grid, hipStream_t s, const Batch1DArgs<T>& a, int list0);
+    static hipError_t set_lds(int n, size_t bytes);
+};
+
+// A size class: the realizations of one n (and one operator kind, two or
+// three reference depths), consecutive in the launch list, advanced by
+// their own kernel on their own stream.
+struct Class1D {
+    int n = 0, mid = 0, nt = 0, list0 = 0, count = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr, pre = nullptr, t_a = nullptr, t_b = nullptr;
 };
 
 template <typename T> struct Solver1DT : Solver1D {
@@ -76,18 +87,24 @@ template <typename T> struct Solver1DT : Solver1D {
     DevBuf<Brk1D> bs;
     DevBuf<C> tw;
     Batch1DArgs<T> args{};
+    std::vector<Class1D> cls;
+    std::vector<int> pos;          // position of realization r in the launch list
     int* h_count = nullptr;
-    // timing of one right-hand side launch of the whole batch: an event pair
-    // around the third stage of every round in which all realizations are
-    // active
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // each round starts at ev0 on the main stream; the third stage of every
+    // round in which all realizations are active is timed over the classes
+    // (Class1D::t_a, t_b) as one right-hand side of the whole batch
+    hipEvent_t ev0 = nullptr;
     double launch_ms = 0.0;
     long launches_timed = 0, rounds_total = 0;
 
     ~Solver1DT() override {
         if (h_count) (void)hipHostFree(h_count);
         if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
+        for (Class1D& c : cls) {
+            for (hipEvent_t e : {c.done, c.pre, c.t_a, c.t_b})
+                if (e) (void)hipEventDestroy(e);
+            if (c.stream) (void)hipStreamDestroy(c.stream);
+        }
     }
 
     static int ilog2(int n) { int l = 0; while ((1 << l) < n) ++l; return l; }
@@ -96,7 +113,6 @@ template <typename T> struct Solver1DT : Solver1D {
         nreal = n;
         prec = sizeof(T) == 4 ? HW_PREC_FP32 : HW_PREC_FP64;
         const int nmax_prec = sizeof(T) == 4 ? 16384 : 8192;
-        int nmin = 1 << 30;
         long long cbtot = 0, sigtot = 0;
         hreal.resize(n);
         for (int r = 0; r < n; ++r) {
@@ -155,20 +171,40 @@ template <typename T> struct Solver1DT : Solver1D {
             ntot += nn;
             sigtot += R.nt;
             cbtot += R.cap;
-            if (nn < nmin) nmin = nn;
             if (nn > maxn) maxn = nn;
             // the stepper's dydt_out evaluation has side effects only through
             // breaking and the wall source
             any_brk |= R.brk | R.wall;
         }
-        NT = nmin / 2 < 1024 ? nmin / 2 : 1024;
-        if (maxn > 16 * NT) {
-            set_error("sizes %d..%d cannot share one launch (the largest may be at most 16 x %d = %d points: "
-                      "16 elements per thread of the %d-thread blocks the smallest size sets)",
-                      nmin, maxn, NT, 16 * NT, NT);
-            return HW_ERR_ARG;
-        }
         HW_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        // ---- size classes: the launch list holds each class's realizations
+        // together (in realization order within a class)
+        std::vector<int> hl;
+        pos.assign(n, 0);
+        for (int r = 0; r < n; ++r) {
+            bool seen = false;
+            for (const Class1D& c : cls) seen |= c.n == hreal[r].n && c.mid == hreal[r].mid;
+            if (seen) continue;
+            Class1D c;
+            c.n = hreal[r].n;
+            c.mid = hreal[r].mid;
+            c.nt = nt_for_n(c.n);
+            c.list0 = (int)hl.size();
+            for (int q = r; q < n; ++q)
+                if (hreal[q].n == c.n && hreal[q].mid == c.mid) {
+                    pos[q] = (int)hl.size();
+                    hl.push_back(q);
+                }
+            c.count = (int)hl.size() - c.list0;
+            cls.push_back(c);
+        }
+        for (Class1D& c : cls) {
+            HW_TRY(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+            HW_TRY(hipEventCreateWithFlags(&c.done, hipEventDisableTiming));
+            HW_TRY(hipEventCreateWithFlags(&c.pre, hipEventDisableTiming));
+            HW_TRY(hipEventCreate(&c.t_a));
+            HW_TRY(hipEventCreate(&c.t_b));
+        }
         // ---- per-element arrays
         std::vector<T> hk(ntot), haal(ntot), hchi(ntot), hdamp(ntot), hopa(ntot), hopb(ntot), hgm(ntot), hgp(ntot),
             hdep(ntot), hopc(ntot, (T)0), hgc(ntot, (T)0);
@@ -247,8 +283,6 @@ template <typename T> struct Solver1DT : Solver1D {
             hrk[r].t = cfg[r].t0;
             hrk[r].h = cfg[r].h0 > 0 ? cfg[r].h0 : 1e-4;
         }
-        std::vector<int> hl(n);
-        for (int r = 0; r < n; ++r) hl[r] = r;
         std::vector<Brk1D> hb(n);
         std::memset(hb.data(), 0, sizeof(Brk1D) * n);
         // wall source state: t_prev = the first record time (define_wall,
@@ -271,8 +305,7 @@ template <typename T> struct Solver1DT : Solver1D {
         HW_TRY(hipMemsetAsync(ide.p, 0, sizeof(int) * cbtot, stream));
         HW_TRY(hipHostMalloc((void**)&h_count, 2 * sizeof(int), hipHostMallocDefault));
         HW_TRY(hipEventCreate(&ev0));
-        HW_TRY(hipEventCreate(&ev1));
-        HW_TRY(K1D<T>::set_lds(NT, sizeof(C) * maxn));
+        for (const Class1D& c : cls) HW_TRY(K1D<T>::set_lds(c.n, sizeof(C) * c.n));
         // ---- kernel arguments
         Batch1DArgs<T>& a = args;
         a.real = d_real.p; a.rk = d_rk.p; a.list = d_list.p;
@@ -312,9 +345,7 @@ template <typename T> struct Solver1DT : Solver1D {
         Batch1DArgs<T> a = args;
         a.ext = ext.p - 2 * hreal[r].off;
         a.t_ext = t;
-        int* dl = d_list.p + r;
-        a.list = dl;
-        HW_TRY(K1D<T>::launch(NT, dim3(1), sizeof(C) * maxn, stream, a, 0));
+        HW_TRY(K1D<T>::launch(n, dim3(1), sizeof(C) * n, stream, a, 0, pos[r], hreal[r].mid));
         std::vector<C> o(2 * n);
         HW_TRY(hipMemcpyAsync(o.data(), Kb[6].p + 2 * hreal[r].off, sizeof(C) * 2 * n, hipMemcpyDeviceToHost, stream));
         HW_TRY(hipStreamSynchronize(stream));
@@ -385,30 +416,53 @@ template <typename T> struct Solver1DT : Solver1D {
     int integrate(double t1, long max_rounds) override {
         k1d_begin<<<(nreal + 255) / 256, 256, 0, stream>>>(d_rk.p, nreal, t1);
         HW_LAUNCH_CHECK();
-        const size_t lds = sizeof(C) * maxn;
         int active_prev = nreal;
         if (round_limit > 0 && (max_rounds <= 0 || round_limit < max_rounds)) max_rounds = round_limit;
         for (long round = 0; max_rounds <= 0 || round < max_rounds; ++round) {
             // time the launches of rounds in which every realization is active
-            // (the first rounds of an interval), so one timed launch is the
-            // whole batch's right-hand side
+            // (the first rounds of an interval): the classes meet before the
+            // third stage, whose launches together are one right-hand side of
+            // the whole batch
             const bool timed = active_prev == nreal;
             rounds_total++;
+            // the classes start from the round's start on the main stream
+            // (the begin kernel, or the previous round's count)
+            HW_TRY(hipEventRecord(ev0, stream));
+            for (Class1D& c : cls) HW_TRY(hipStreamWaitEvent(c.stream, ev0, 0));
             for (int s = 1; s <= 6; ++s) {
-                if (timed && s == 3) HW_TRY(hipEventRecord(ev0, stream));
-                HW_TRY(K1D<T>::launch(NT, dim3(nreal), lds, stream, args, s));
-                if (timed && s == 3) HW_TRY(hipEventRecord(ev1, stream));
+                if (timed && s == 3 && cls.size() > 1) {
+                    for (Class1D& c : cls) HW_TRY(hipEventRecord(c.pre, c.stream));
+                    for (Class1D& c : cls)
+                        for (Class1D& d : cls)
+                            if (&c != &d) HW_TRY(hipStreamWaitEvent(c.stream, d.pre, 0));
+                }
+                for (Class1D& c : cls) {
+                    if (timed && s == 3) HW_TRY(hipEventRecord(c.t_a, c.stream));
+                    HW_TRY(K1D<T>::launch(c.n, dim3(c.count), sizeof(C) * c.n, c.stream, args, s, c.list0, c.mid));
+                    if (timed && s == 3) HW_TRY(hipEventRecord(c.t_b, c.stream));
+                }
             }
-            if (any_brk) HW_TRY(K1D<T>::launch(NT, dim3(nreal), lds, stream, args, 7));
-            HW_TRY(K1D<T>::control(NT, dim3(nreal), stream, args));
+            for (Class1D& c : cls) {
+                if (any_brk)
+                    HW_TRY(K1D<T>::launch(c.n, dim3(c.count), sizeof(C) * c.n, c.stream, args, 7, c.list0, c.mid));
+                HW_TRY(K1D<T>::control(c.nt, dim3(c.count), c.stream, args, c.list0));
+                HW_TRY(hipEventRecord(c.done, c.stream));
+                HW_TRY(hipStreamWaitEvent(stream, c.done, 0));
+            }
             k1d_count<<<1, 256, 0, stream>>>(d_rk.p, bs.p, nreal, d_count.p);
             HW_LAUNCH_CHECK();
             HW_TRY(hipMemcpyAsync(h_count, d_count.p, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
             HW_TRY(hipStreamSynchronize(stream));
             if (timed) {
-                float ms = 0.f;
-                HW_TRY(hipEventElapsedTime(&ms, ev0, ev1));
-                launch_ms += ms;
+                // the classes run the third stage side by side from the same
+                // start; the batch's stage ends with its slowest class
+                float ms_max = 0.f;
+                for (const Class1D& c : cls) {
+                    float ms = 0.f;
+                    HW_TRY(hipEventElapsedTime(&ms, c.t_a, c.t_b));
+                    if (ms > ms_max) ms_max = ms;
+                }
+                launch_ms += ms_max;
                 launches_timed += 1;
             }
             if (h_count[1] > 0) return check_overflow();

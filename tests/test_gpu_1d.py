@@ -82,6 +82,38 @@ def test_batch_uneven_sizes():
         assert relerr(got, ref) < 1e-11
 
 
+def test_batch_of_any_sizes_in_size_classes():
+    """Sizes 64x apart in one batch, interleaved and with a three-depth
+    realization among them: each size class (n, depth count) has its own
+    kernel and stream.  Every realization's right-hand side equals the
+    oracle's, and the batch integrated together ends bit for bit where each
+    realization integrated alone ends (same kernels, same steps)."""
+    say("batch size classes")
+    ms = [model("slope", 256, ndepth=2), model("slope", 16384, ndepth=2), model("channel", 512),
+          model("slope", 16384, ndepth=2), model("slope", 1024, ndepth=3), model("slope", 256, ndepth=2)]
+    d = Hawassi1D(ms, HW_PREC_FP32)
+    for r, m in enumerate(ms):
+        y = W1.random_state(m, 20 + r, amp=0.05, kfrac=0.3)
+        e = relerr(d.rhs(0.6, y, r), O.Oracle1D(m).rhs(0.6, y))
+        say("  r", r, m.n, "relL2", e)
+        assert e < 2e-5
+    d = Hawassi1D(ms, HW_PREC_FP32, t0=0.0, h0=1e-3)
+    for r, m in enumerate(ms):
+        d.set_state(W1.random_state(m, 30 + r, amp=0.02, kfrac=0.2), r)
+    d.integrate(0.05)
+    for r, m in enumerate(ms):
+        a = Hawassi1D([m], HW_PREC_FP32, t0=0.0, h0=1e-3)
+        a.set_state(W1.random_state(m, 30 + r, amp=0.02, kfrac=0.2))
+        a.integrate(0.05)
+        same = np.array_equal(a.get_state(0), d.get_state(r))
+        sa, sb = a.stats(0), d.stats(r)
+        say("  r", r, "alone == batch", same, sa, sb)
+        # (rhs_evals may differ: a batch with breaking anywhere runs rkf45's
+        # dydt_out evaluation for every realization, without side effects
+        # where breaking is off)
+        assert same and (sa["steps_accepted"], sa["steps_rejected"]) == (sb["steps_accepted"], sb["steps_rejected"])
+
+
 def _traj(ms, prec, t_out):
     d = Hawassi1D(ms, prec, t0=0.0, h0=1e-4)
     os_, ys, ts, hs = [], [], [], []
