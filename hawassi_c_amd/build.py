@@ -34,9 +34,10 @@ SIZES = (
 
 
 # 1-D batched right-hand side: one unit per (precision, size class n), the
-# step-control kernel per (precision, workgroup size)
-N1D = (128, 256, 512, 1024, 2048, 4096, 8192, 16384)
-NT1D = (64, 128, 256, 512, 1024)
+# step-control kernel per (precision, workgroup size); n = 2000 is the grid
+# of the reference case 1dBBr001HarmBar (1000 threads)
+N1D = (128, 256, 512, 1024, 2000, 2048, 4096, 8192, 16384)
+NT1D = (64, 128, 256, 512, 1000, 1024)
 
 
 def _units():

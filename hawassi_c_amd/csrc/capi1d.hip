@@ -13,9 +13,9 @@ namespace hw {
     hipError_t k1d_setlds_##P##_##N(size_t);
 #define HW_1D_DECL_NT(T, P, NT) hipError_t k1d_control_##P##_##NT(dim3, hipStream_t, const Batch1DArgs<T>&, int);
 #define HW_1D_DECL_ALL(T, P) HW_1D_DECL_N(T, P, 128) HW_1D_DECL_N(T, P, 256) HW_1D_DECL_N(T, P, 512) \
-    HW_1D_DECL_N(T, P, 1024) HW_1D_DECL_N(T, P, 2048) HW_1D_DECL_N(T, P, 4096) HW_1D_DECL_N(T, P, 8192) \
-    HW_1D_DECL_NT(T, P, 64) HW_1D_DECL_NT(T, P, 128) HW_1D_DECL_NT(T, P, 256) HW_1D_DECL_NT(T, P, 512) \
-    HW_1D_DECL_NT(T, P, 1024)
+    HW_1D_DECL_N(T, P, 1024) HW_1D_DECL_N(T, P, 2000) HW_1D_DECL_N(T, P, 2048) HW_1D_DECL_N(T, P, 4096) \
+    HW_1D_DECL_N(T, P, 8192) HW_1D_DECL_NT(T, P, 64) HW_1D_DECL_NT(T, P, 128) HW_1D_DECL_NT(T, P, 256) \
+    HW_1D_DECL_NT(T, P, 512) HW_1D_DECL_NT(T, P, 1000) HW_1D_DECL_NT(T, P, 1024)
 HW_1D_DECL_ALL(float, 1)
 HW_1D_DECL_ALL(double, 0)
 HW_1D_DECL_N(float, 1, 16384)
@@ -26,6 +26,7 @@ HW_1D_DECL_N(float, 1, 16384)
         case 256: return CALL##_##P##_256(__VA_ARGS__);                     \
         case 512: return CALL##_##P##_512(__VA_ARGS__);                     \
         case 1024: return CALL##_##P##_1024(__VA_ARGS__);                   \
+        case 2000: return CALL##_##P##_2000(__VA_ARGS__);                   \
         case 2048: return CALL##_##P##_2048(__VA_ARGS__);                   \
         case 4096: return CALL##_##P##_4096(__VA_ARGS__);                   \
         case 8192: return CALL##_##P##_8192(__VA_ARGS__);                   \
@@ -40,6 +41,7 @@ HW_1D_DECL_N(float, 1, 16384)
         case 128: return CALL##_##P##_128(__VA_ARGS__);                     \
         case 256: return CALL##_##P##_256(__VA_ARGS__);                     \
         case 512: return CALL##_##P##_512(__VA_ARGS__);                     \
+        case 1000: return CALL##_##P##_1000(__VA_ARGS__);                   \
         case 1024: return CALL##_##P##_1024(__VA_ARGS__);                   \
         default: return hipErrorInvalidValue;                               \
     }

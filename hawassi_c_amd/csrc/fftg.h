@@ -59,37 +59,7 @@ template <typename C> std::vector<C> twiddles_g(int L) {
     return t;
 }
 
-// ------------------------------------------------------ small DFTs -------
-// 3-point: y_q = sum_r v_r e^(DIR 2 pi i r q / 3)
-template <int DIR, typename C> __device__ __forceinline__ void dft3g(C* v) {
-    using R = decltype(C::x);
-    const R s3 = (R)0.86602540378443864676;    // sin(2 pi / 3)
-    C a = cadd(v[1], v[2]), b = csub(v[1], v[2]);
-    C m = cmk<C>(v[0].x - (R)0.5 * a.x, v[0].y - (R)0.5 * a.y);
-    C ib = cmul_i<DIR>(cscale(b, s3));           // i DIR sin(2 pi/3) (v1 - v2)
-    v[0] = cadd(v[0], a);
-    v[1] = cadd(m, ib);
-    v[2] = csub(m, ib);
-}
-
-// 5-point with the symmetric pairs a1 = v1 + v4, b1 = v1 - v4, a2 = v2 + v3,
-// b2 = v2 - v3
-template <int DIR, typename C> __device__ __forceinline__ void dft5g(C* v) {
-    using R = decltype(C::x);
-    const R c1 = (R)0.30901699437494742410, c2 = (R)-0.80901699437494742410;   // cos(2 pi/5), cos(4 pi/5)
-    const R s1 = (R)0.95105651629515357212, s2 = (R)0.58778525229247312917;    // sin(2 pi/5), sin(4 pi/5)
-    C a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
-    C a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
-    C m1 = cmk<C>(v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y);
-    C m2 = cmk<C>(v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y);
-    C n1 = cmul_i<DIR>(cmk<C>(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
-    C n2 = cmul_i<DIR>(cmk<C>(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
-    v[0] = cadd(v[0], cadd(a1, a2));
-    v[1] = cadd(m1, n1);
-    v[4] = csub(m1, n1);
-    v[2] = cadd(m2, n2);
-    v[3] = csub(m2, n2);
-}
+// (the 3- and 5-point DFTs dft3g, dft5g are in hw_fft.h)
 
 // one radix-R butterfly of pass (R, NS) of B sequences of length L stored at
 // src[b * LS + n]; writes dst in Stockham order

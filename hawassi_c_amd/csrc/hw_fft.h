@@ -11,7 +11,7 @@
 // library divides by NX*NY at the end of the x-pass so physical fields come out
 // in physical units.
 //
-// The FFT inside a pass is a mixed radix-8/4/2 Stockham autosort working in
+// The FFT inside a pass is a mixed radix-8/4/2 (and 5/3) Stockham autosort working in
 // LDS: each radix pass reads its butterfly inputs into registers, synchronises,
 // applies twiddles (from an FP64-accurate table) and the radix-R DFT, and
 // writes the outputs in natural order for the next pass.
@@ -105,18 +105,55 @@ template <int DIR, typename C> __device__ __forceinline__ void dft16(C* v) {
     }
 }
 
+// ------------------------------------------------------ small DFTs -------
+// 3-point: y_q = sum_r v_r e^(DIR 2 pi i r q / 3)
+template <int DIR, typename C> __device__ __forceinline__ void dft3g(C* v) {
+    using R = decltype(C::x);
+    const R s3 = (R)0.86602540378443864676;    // sin(2 pi / 3)
+    C a = cadd(v[1], v[2]), b = csub(v[1], v[2]);
+    C m = cmk<C>(v[0].x - (R)0.5 * a.x, v[0].y - (R)0.5 * a.y);
+    C ib = cmul_i<DIR>(cscale(b, s3));           // i DIR sin(2 pi/3) (v1 - v2)
+    v[0] = cadd(v[0], a);
+    v[1] = cadd(m, ib);
+    v[2] = csub(m, ib);
+}
+
+// 5-point with the symmetric pairs a1 = v1 + v4, b1 = v1 - v4, a2 = v2 + v3,
+// b2 = v2 - v3
+template <int DIR, typename C> __device__ __forceinline__ void dft5g(C* v) {
+    using R = decltype(C::x);
+    const R c1 = (R)0.30901699437494742410, c2 = (R)-0.80901699437494742410;   // cos(2 pi/5), cos(4 pi/5)
+    const R s1 = (R)0.95105651629515357212, s2 = (R)0.58778525229247312917;    // sin(2 pi/5), sin(4 pi/5)
+    C a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+    C a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+    C m1 = cmk<C>(v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y);
+    C m2 = cmk<C>(v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y);
+    C n1 = cmul_i<DIR>(cmk<C>(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
+    C n2 = cmul_i<DIR>(cmk<C>(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
+    v[0] = cadd(v[0], cadd(a1, a2));
+    v[1] = cadd(m1, n1);
+    v[4] = csub(m1, n1);
+    v[2] = cadd(m2, n2);
+    v[3] = csub(m2, n2);
+}
+
 template <int R, int DIR, typename C> __device__ __forceinline__ void dftR(C* v) {
     if constexpr (R == 16) dft16<DIR>(v);
     else if constexpr (R == 8) dft8<DIR>(v);
     else if constexpr (R == 4) dft4<DIR>(v);
+    else if constexpr (R == 5) dft5g<DIR>(v);
+    else if constexpr (R == 3) dft3g<DIR>(v);
     else dft2<DIR>(v);
 }
 
 #ifndef HW_MAX_RADIX
 #define HW_MAX_RADIX 8
 #endif
+// the radix of the LDS Stockham pass that leaves rem points per sequence:
+// 16/8/4/2 while rem has a factor 2, then 5 and 3 (lengths 2^a 3^b 5^c)
 constexpr int pass_radix(int rem) {
-    return (HW_MAX_RADIX >= 16 && rem % 16 == 0) ? 16 : (rem % 8 == 0 ? 8 : (rem % 4 == 0 ? 4 : 2));
+    return (HW_MAX_RADIX >= 16 && rem % 16 == 0) ? 16
+         : rem % 8 == 0 ? 8 : rem % 4 == 0 ? 4 : rem % 2 == 0 ? 2 : rem % 5 == 0 ? 5 : 3;
 }
 
 // ------------------------------------------------------- Stockham in LDS --
@@ -134,7 +171,7 @@ __device__ __forceinline__ void fft_lds_pass(C* lds, int LS, const C* __restrict
         // from LDS and the block meets at the barrier.  Radix 2/4: every
         // w^r from the table; radix 8/16: w, w^4 (and w^8), the other powers
         // formed below (at most two roundings more than a table entry).
-        constexpr int NTW = NS == 1 ? 1 : (R <= 4 ? R - 1 : (R == 16 ? 3 : 2));
+        constexpr int NTW = NS == 1 ? 1 : (R == 16 ? 3 : R == 8 ? 2 : R - 1);
         C wt[PER][NTW];
         if constexpr (NS > 1) {
 #pragma unroll
@@ -143,7 +180,7 @@ __device__ __forceinline__ void fft_lds_pass(C* lds, int LS, const C* __restrict
                 if (NB % NT == 0 || bf < NB) {
                     const int k = (bf % M) % NS;
                     const int ex = k * (L / (NS * R));
-                    if constexpr (R <= 4) {
+                    if constexpr (R != 8 && R != 16) {
 #pragma unroll
                         for (int r = 1; r < R; ++r) wt[p][r - 1] = tw[(r * ex) * TS];
                     } else {
@@ -172,7 +209,7 @@ __device__ __forceinline__ void fft_lds_pass(C* lds, int LS, const C* __restrict
                 int b = bf / M, j = bf % M;
                 int k = j % NS;
                 if constexpr (NS > 1) {
-                    if constexpr (R <= 4) {
+                    if constexpr (R != 8 && R != 16) {
 #pragma unroll
                         for (int r = 1; r < R; ++r) {
                             C w = wt[p][r - 1];

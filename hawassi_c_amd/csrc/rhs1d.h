@@ -25,7 +25,7 @@ namespace hw {
 // ----------------------------------------------------------- descriptors --
 struct Real1D {                 // read-only model description of one realization
     int n, flat, src, prop;     // prop: 0 Uni+, 1 Uni-, 2 Bi, 3 none
-    int brk, nt, cap, log2n;
+    int brk, nt, cap, pad0;
     long long off;              // element offset of this realization (sum of earlier n)
     long long tw_off;           // offset of the length-n twiddle table in Batch1DArgs::tw
     long long sig_off;          // offset into the signal arrays
@@ -107,7 +107,22 @@ __constant__ double c_rk1d_ah[5] = {1.0 / 4.0, 3.0 / 8.0, 12.0 / 13.0, 1.0, 1.0 
 __constant__ double c_rk1d_ec[6] = {1.0 / 360.0, 0.0, -128.0 / 4275.0, -2197.0 / 75240.0, 1.0 / 50.0, 2.0 / 55.0};
 
 // ------------------------------------------------------- block helpers --
-// red: LDS scratch of at least 2*NT/64 + 8 words
+// red: LDS scratch of at least 2*NT/64 + 8 words.
+// Blocks whose size is not a multiple of 64 (n = 2000: 1000 threads) end
+// with a partial wave; its shuffles then skip partners beyond the block
+// (lane ^ o >= active lanes), so that lane 0 still reduces every thread of
+// its wave: the partners it reads, and their partners in turn, have lower
+// lane numbers than any thread whose value they carry.
+template <int NT> __device__ __forceinline__ bool shfl_ok(int o) {
+    if constexpr (NT % 64 == 0) {
+        (void)o;
+        return true;
+    } else {
+        const int lane = threadIdx.x & 63, w0 = threadIdx.x & ~63;
+        const int act = NT - w0 < 64 ? NT - w0 : 64;
+        return (lane ^ o) < act;
+    }
+}
 template <int NT> __device__ __forceinline__ int block_or(int v, int* red) {
     v = __any(v) ? 1 : 0;
     __syncthreads();
@@ -123,7 +138,7 @@ template <int NT> __device__ __forceinline__ int block_or(int v, int* red) {
 template <int NT> __device__ __forceinline__ int block_min(int v, int* red) {
     for (int o = 32; o > 0; o >>= 1) {
         int u = __shfl_xor(v, o, 64);
-        v = u < v ? u : v;
+        if (shfl_ok<NT>(o)) v = u < v ? u : v;
     }
     __syncthreads();
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
@@ -138,7 +153,7 @@ template <int NT> __device__ __forceinline__ int block_min(int v, int* red) {
 template <int NT> __device__ __forceinline__ double block_max_d(double v, double* red) {
     for (int o = 32; o > 0; o >>= 1) {
         double u = __shfl_xor(v, o, 64);
-        v = u > v ? u : v;
+        if (shfl_ok<NT>(o)) v = u > v ? u : v;
     }
     __syncthreads();
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
@@ -155,7 +170,7 @@ template <int NT> __device__ __forceinline__ void block_argmax(double& v, int& i
     for (int o = 32; o > 0; o >>= 1) {
         double u = __shfl_xor(v, o, 64);
         int j = __shfl_xor(idx, o, 64);
-        if (u > v || (u == v && j < idx)) { v = u; idx = j; }
+        if (shfl_ok<NT>(o) && (u > v || (u == v && j < idx))) { v = u; idx = j; }
     }
     __syncthreads();
     if ((threadIdx.x & 63) == 0) { redv[threadIdx.x >> 6] = v; redi[threadIdx.x >> 6] = idx; }

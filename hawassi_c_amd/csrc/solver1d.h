@@ -107,7 +107,6 @@ template <typename T> struct Solver1DT : Solver1D {
         }
     }
 
-    static int ilog2(int n) { int l = 0; while ((1 << l) < n) ++l; return l; }
 
     int init(const hw_config_1d* cfg, int n) override {
         nreal = n;
@@ -118,8 +117,10 @@ template <typename T> struct Solver1DT : Solver1D {
         for (int r = 0; r < n; ++r) {
             const hw_config_1d& c = cfg[r];
             int nn = c.n;
-            if (nn < 128 || nn > nmax_prec || (nn & (nn - 1))) {
-                set_error("realization %d: n = %d unsupported (powers of two 128..%d)", r, nn, nmax_prec);
+            // the sizes with a kernel: powers of two, and 2000 (the grid of the
+            // reference case 1dBBr001HarmBar: radix 8, 2, 5, 5, 5 passes)
+            if (!((nn >= 128 && nn <= nmax_prec && !(nn & (nn - 1))) || nn == 2000)) {
+                set_error("realization %d: n = %d unsupported (powers of two 128..%d, or 2000)", r, nn, nmax_prec);
                 return HW_ERR_ARG;
             }
             if (!c.k || !c.aal || !c.chi || !c.damp || !c.depth) { set_error("realization %d: null array", r); return HW_ERR_ARG; }
@@ -137,7 +138,6 @@ template <typename T> struct Solver1DT : Solver1D {
             R.n = nn; R.flat = c.flat; R.src = c.source_on; R.prop = c.propagation; R.brk = c.breaking_on;
             R.nt = c.source_on ? c.nt : 0;
             R.cap = nn / 4 < 1024 ? nn / 4 : 1024;
-            R.log2n = ilog2(nn);
             R.off = ntot;
             R.sig_off = sigtot;
             R.cb_off = cbtot;
@@ -257,13 +257,15 @@ template <typename T> struct Solver1DT : Solver1D {
         // per size in one buffer (a realization finds its table at tw_off)
         {
             std::vector<C> h;
-            long long at[32];
-            for (int l = 0; l < 32; ++l) at[l] = -1;
+            std::vector<std::pair<int, long long>> at;   // (n, offset of its table)
             for (int r = 0; r < n; ++r) {
-                const int l = hreal[r].log2n;
-                if (at[l] < 0) {
-                    const int nn = 1 << l;
-                    at[l] = (long long)h.size();
+                const int nn = hreal[r].n;
+                long long off = -1;
+                for (const auto& e : at)
+                    if (e.first == nn) off = e.second;
+                if (off < 0) {
+                    off = (long long)h.size();
+                    at.emplace_back(nn, off);
                     for (int m = 0; m < nn; ++m) {
                         const double ang = -2.0 * M_PI * (double)m / (double)nn;
                         C w;
@@ -272,7 +274,7 @@ template <typename T> struct Solver1DT : Solver1D {
                         h.push_back(w);
                     }
                 }
-                hreal[r].tw_off = at[l];
+                hreal[r].tw_off = off;
             }
             if ((rc = tw.upload(h, stream))) return rc;
         }

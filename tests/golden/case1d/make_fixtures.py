@@ -10,7 +10,8 @@ use into this folder (run in a checkout that has the reference at
 * 1dFBr005Bor/: the case inputs (input.txt, gauges.dat, initial.dat),
   Output/Hawassi_eta_gauge.txt, Output/breaking.dat and Output/Constants_01.dat;
 * 1dBBr002IrBar/: the case inputs, output/breaking.dat and Constants_01.dat;
-* 1dB001IrSlope/: the case inputs, output/Output/influx.dat and Constants_01.dat.
+* 1dB001IrSlope/: the case inputs, output/Output/influx.dat and Constants_01.dat;
+* 1dBBr001HarmBar/: the case inputs (a 2000-node grid; no outputs ship).
 """
 import glob
 import gzip
@@ -63,6 +64,10 @@ def main():
     put(f"{REF}/1dB001IrSlope/output/Output/influx.dat", os.path.join(c, "expected", "influx.dat"), gz=True)
     put(f"{REF}/1dB001IrSlope/output/Output/Constants_01.dat", os.path.join(c, "expected", "Constants_01.dat"),
         gz=True)
+    c = os.path.join(HERE, "1dBBr001HarmBar")
+    for name in ("input.txt", "bath.dat", "gauges.dat"):
+        put(f"{REF}/1dBBr001HarmBar/{name}", os.path.join(c, name))
+    put(f"{REF}/1dBBr001HarmBar/influx.dat", os.path.join(c, "influx.dat"), gz=True)
 
 
 if __name__ == "__main__":

@@ -36,9 +36,11 @@ def say(*a):
 # 1dBBr002IrBar to 16 s and 1dFBr005Bor to 13 s cover the first breaking
 # events, whose breaking.dat rows the oracle reproduces character for
 # character (tests/test_case1d.py); 1dB001IrSlope drives the device with a
-# 91969-sample influx record
+# 91969-sample influx record; 1dBBr001HarmBar (2000 nodes: radix 8, 2, 5, 5,
+# 5 transforms, 1000-thread blocks) to 13.7 s, through its first breaking
+# (flag 2 from 13.6 s)
 @pytest.mark.parametrize("name,t1", [("1dB002HarmBar", 6.0), ("1dFBr005Bor", 13.0), ("1dBBr002IrBar", 16.0),
-                                     ("1dB001IrSlope", 20.0)])
+                                     ("1dB001IrSlope", 20.0), ("1dBBr001HarmBar", 13.7)])
 def test_case_start_against_oracle(tmp_path, name, t1):
     say("case1d oracle window", name, t1)
     c = case1d.case_from_dir(str(unpack_case(name, tmp_path)))
