@@ -752,8 +752,10 @@ def bench_slope1d(args):
         "realization_rhs_bytes_mean": mean_b,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "unit_of_work": "one launch of the batch right-hand side (k1d_rhs, every realization active), "
-                                     "%.1f MB algorithmic, %.3f ms event-timed" % (batch_b / 1e6, dt_launch)},
+                     "unit_of_work": "one right-hand side of the whole batch (the size classes' k1d_rhs_n "
+                                     "launches of one stage side by side, every realization active), "
+                                     "%.1f MB algorithmic, %.3f ms event-timed (the slowest class)"
+                                     % (batch_b / 1e6, dt_launch)},
         "roofline_job": {"achieved": job, "frac": job / HBM_PEAK_GBS, "unit": "GB/s"},
     }
     if not args.no_cpu_baseline:

@@ -253,7 +253,8 @@ int hw_fft2_forward_noalloc(int nx, int ny, int precision, const double *d_phys,
  * planes of n doubles [Re eta^][Im eta^][Re u^][Im u^] of the full complex
  * spectrum (1d/functions.c:537-538, set_initial_ode). */
 typedef struct hw_config_1d {
-    int n;                    /* points: power of two, 128..16384 (FP32) or 128..8192 (FP64) */
+    int n;                    /* points: power of two, 128..16384 (FP32) or 128..8192 (FP64), or
+                                 2000 (the reference's 1dBBr001HarmBar grid) */
     int flat;                 /* 1: flat bottom (Csqr, L), 0: two reference depths (Up2IP + HSS) */
     int source_on;            /* influx forcing (define_forcing, 1d/rhs.c:659-693) */
     int propagation;          /* 0 "Uni+", 1 "Uni-", 2 "Bi", 3 none */
@@ -291,7 +292,9 @@ typedef struct hw_config_1d {
     const double *gamw_skew_re, *gamw_skew_im;
 } hw_config_1d;
 
-/* Build a batch of nreal realizations; precision applies to all. */
+/* Build a batch of nreal realizations; precision applies to all.  Any mix
+   of the supported n: the realizations of one n (and one depth count) form
+   a size class advanced by its own kernel on its own stream. */
 int hw1d_create(const hw_config_1d *cfgs, int nreal, int precision, void **ctx);
 void hw1d_destroy(void *ctx);
 
