@@ -38,6 +38,11 @@ HBM_PEAK_GBS = 8000.0
 RHS_TRAFFIC_BYTES_2048 = 2075.8e6
 # the same for the AB3 (HS3) right-hand side
 RHS_TRAFFIC_BYTES_2048_HS3 = 2963.3e6
+# HBM bytes per right-hand side of the default 1-D batch (256 sloping-beach
+# realizations of seed 7, two reference depths, every realization active):
+# the three size classes' largest dispatches in the FETCH_SIZE (x2.0) and
+# WRITE_SIZE (x1.0) counter passes
+RHS_TRAFFIC_BYTES_SLOPE1D = 1255.8e6
 METRIC = "sim-sec/wall-sec + RHS evals/s, 2D 2048^2 FP32 vs FP64 oracle at relL2<=1e-4"
 
 
@@ -751,7 +756,8 @@ def bench_slope1d(args):
         "rhs_evals_per_s": nrhs / wall,
         "realization_rhs_bytes_mean": mean_b,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": RHS_TRAFFIC_BYTES_SLOPE1D if (args.nreal == 256 and args.depths == 2) else None,
                      "unit_of_work": "one right-hand side of the whole batch (the size classes' k1d_rhs_n "
                                      "launches of one stage side by side, every realization active), "
                                      "%.1f MB algorithmic, %.3f ms event-timed (the slowest class)"
