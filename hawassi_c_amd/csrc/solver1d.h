@@ -90,14 +90,19 @@ template <typename T> struct Solver1DT : Solver1D {
     std::vector<Class1D> cls;
     std::vector<int> pos;          // position of realization r in the launch list
     int* h_count = nullptr;
-    // each round starts at ev0 on the main stream; the third stage of every
-    // round in which all realizations are active is timed over the classes
-    // (Class1D::t_a, t_b) as one right-hand side of the whole batch
+    // each round forks from ev0 on the main stream; the third stage of the
+    // first round of every integrate call (all realizations active) is timed
+    // over the classes (Class1D::t_a, t_b) as one right-hand side of the
+    // whole batch
     hipEvent_t ev0 = nullptr;
+    hipGraph_t round_graph = nullptr;
+    hipGraphExec_t round_exec = nullptr;
     double launch_ms = 0.0;
     long launches_timed = 0, rounds_total = 0;
 
     ~Solver1DT() override {
+        if (round_exec) (void)hipGraphExecDestroy(round_exec);
+        if (round_graph) (void)hipGraphDestroy(round_graph);
         if (h_count) (void)hipHostFree(h_count);
         if (ev0) (void)hipEventDestroy(ev0);
         for (Class1D& c : cls) {
@@ -306,7 +311,7 @@ template <typename T> struct Solver1DT : Solver1D {
         HW_TRY(hipMemsetAsync(cbp.p, 0, sizeof(Cb1) * cbtot, stream));
         HW_TRY(hipMemsetAsync(ide.p, 0, sizeof(int) * cbtot, stream));
         HW_TRY(hipHostMalloc((void**)&h_count, 2 * sizeof(int), hipHostMallocDefault));
-        HW_TRY(hipEventCreate(&ev0));
+        HW_TRY(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
         for (const Class1D& c : cls) HW_TRY(K1D<T>::set_lds(c.n, sizeof(C) * c.n));
         // ---- kernel arguments
         Batch1DArgs<T>& a = args;
@@ -410,49 +415,84 @@ template <typename T> struct Solver1DT : Solver1D {
         return HW_OK;
     }
 
+    // One round of every active realization, enqueued: the classes fork from
+    // the main stream, run their stages (dydt_in where needed, five stages,
+    // rkf45's dydt_out when breaking or a wall is on) and their step control
+    // on their own streams, and join it again for the active count.  With
+    // `timed` the classes meet before the third stage, whose launches side by
+    // side are one right-hand side of the whole batch, timed per class.
+    int enqueue_round(bool timed) {
+        HW_TRY(hipEventRecord(ev0, stream));
+        for (Class1D& c : cls) HW_TRY(hipStreamWaitEvent(c.stream, ev0, 0));
+        for (int s = 1; s <= 6; ++s) {
+            if (timed && s == 3 && cls.size() > 1) {
+                for (Class1D& c : cls) HW_TRY(hipEventRecord(c.pre, c.stream));
+                for (Class1D& c : cls)
+                    for (Class1D& d : cls)
+                        if (&c != &d) HW_TRY(hipStreamWaitEvent(c.stream, d.pre, 0));
+            }
+            for (Class1D& c : cls) {
+                if (timed && s == 3) HW_TRY(hipEventRecord(c.t_a, c.stream));
+                HW_TRY(K1D<T>::launch(c.n, dim3(c.count), sizeof(C) * c.n, c.stream, args, s, c.list0, c.mid));
+                if (timed && s == 3) HW_TRY(hipEventRecord(c.t_b, c.stream));
+            }
+        }
+        for (Class1D& c : cls) {
+            if (any_brk)
+                HW_TRY(K1D<T>::launch(c.n, dim3(c.count), sizeof(C) * c.n, c.stream, args, 7, c.list0, c.mid));
+            HW_TRY(K1D<T>::control(c.nt, dim3(c.count), c.stream, args, c.list0));
+            HW_TRY(hipEventRecord(c.done, c.stream));
+            HW_TRY(hipStreamWaitEvent(stream, c.done, 0));
+        }
+        k1d_count<<<1, 256, 0, stream>>>(d_rk.p, bs.p, nreal, d_count.p);
+        HW_LAUNCH_CHECK();
+        return HW_OK;
+    }
+
+    // the untimed round captured once into a HIP graph (fork and join
+    // included); replays of it advance every active realization by one
+    // attempted step, and a replay after every realization has reached t1
+    // only launches blocks that return at once
+    int capture_round() {
+        HW_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_round(false);
+        hipGraph_t gr = nullptr;
+        hipError_t e = hipStreamEndCapture(stream, &gr);
+        if (rc) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return rc;
+        }
+        HW_TRY(e);
+        round_graph = gr;
+        HW_TRY(hipGraphInstantiate(&round_exec, round_graph, nullptr, nullptr, 0));
+        return HW_OK;
+    }
+
     // every realization: while (t < t1) gsl_odeiv_evolve_apply (ode_solver_Ham's
     // inner loop, 1d/functions.c:546-552).  Each round is one attempt of
-    // every active realization: dydt_in where needed, five stages, the
-    // stepper's dydt_out evaluation (which matters only through breaking),
-    // then the error norm and the accept / retry decision.
+    // every active realization.  The first round of a call runs eagerly and
+    // is timed (every realization is active then); the later rounds are
+    // replays of the captured round, kGraphRounds between two reads of the
+    // active count.
+    static constexpr long kGraphRounds = 4;
     int integrate(double t1, long max_rounds) override {
         k1d_begin<<<(nreal + 255) / 256, 256, 0, stream>>>(d_rk.p, nreal, t1);
         HW_LAUNCH_CHECK();
-        int active_prev = nreal;
         if (round_limit > 0 && (max_rounds <= 0 || round_limit < max_rounds)) max_rounds = round_limit;
-        for (long round = 0; max_rounds <= 0 || round < max_rounds; ++round) {
-            // time the launches of rounds in which every realization is active
-            // (the first rounds of an interval): the classes meet before the
-            // third stage, whose launches together are one right-hand side of
-            // the whole batch
-            const bool timed = active_prev == nreal;
-            rounds_total++;
-            // the classes start from the round's start on the main stream
-            // (the begin kernel, or the previous round's count)
-            HW_TRY(hipEventRecord(ev0, stream));
-            for (Class1D& c : cls) HW_TRY(hipStreamWaitEvent(c.stream, ev0, 0));
-            for (int s = 1; s <= 6; ++s) {
-                if (timed && s == 3 && cls.size() > 1) {
-                    for (Class1D& c : cls) HW_TRY(hipEventRecord(c.pre, c.stream));
-                    for (Class1D& c : cls)
-                        for (Class1D& d : cls)
-                            if (&c != &d) HW_TRY(hipStreamWaitEvent(c.stream, d.pre, 0));
-                }
-                for (Class1D& c : cls) {
-                    if (timed && s == 3) HW_TRY(hipEventRecord(c.t_a, c.stream));
-                    HW_TRY(K1D<T>::launch(c.n, dim3(c.count), sizeof(C) * c.n, c.stream, args, s, c.list0, c.mid));
-                    if (timed && s == 3) HW_TRY(hipEventRecord(c.t_b, c.stream));
-                }
+        int rc;
+        for (long round = 0; max_rounds <= 0 || round < max_rounds;) {
+            const bool timed = round == 0;
+            long k = 1;
+            if (timed) {
+                if ((rc = enqueue_round(true))) return rc;
+            } else {
+                if (!round_exec && (rc = capture_round())) return rc;
+                k = kGraphRounds;
+                if (max_rounds > 0 && max_rounds - round < k) k = max_rounds - round;
+                for (long i = 0; i < k; ++i) HW_TRY(hipGraphLaunch(round_exec, stream));
             }
-            for (Class1D& c : cls) {
-                if (any_brk)
-                    HW_TRY(K1D<T>::launch(c.n, dim3(c.count), sizeof(C) * c.n, c.stream, args, 7, c.list0, c.mid));
-                HW_TRY(K1D<T>::control(c.nt, dim3(c.count), c.stream, args, c.list0));
-                HW_TRY(hipEventRecord(c.done, c.stream));
-                HW_TRY(hipStreamWaitEvent(stream, c.done, 0));
-            }
-            k1d_count<<<1, 256, 0, stream>>>(d_rk.p, bs.p, nreal, d_count.p);
-            HW_LAUNCH_CHECK();
+            round += k;
+            rounds_total += k;
             HW_TRY(hipMemcpyAsync(h_count, d_count.p, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
             HW_TRY(hipStreamSynchronize(stream));
             if (timed) {
@@ -469,7 +509,6 @@ template <typename T> struct Solver1DT : Solver1D {
             }
             if (h_count[1] > 0) return check_overflow();
             if (h_count[0] == 0) return check_overflow();
-            active_prev = h_count[0];
         }
         set_error("integrate: %ld rounds without reaching t1 = %g", max_rounds, t1);
         return HW_ERR_STEP;

@@ -178,3 +178,28 @@ def test_trajectory1d_fp32():
     e = relerr(eta_d, eta_o)
     say("  eta relL2", e)
     assert e < 1e-4
+
+
+def test_round_limit_returns_step_error_and_resumes():
+    """hw1d_set_round_limit bounds one integrate call's rounds (the first
+    eager, the later ones replayed from the captured round graph, four
+    between two reads of the active count): the call returns HW_ERR_STEP
+    with the realizations where they got, and a later call without the limit
+    takes them on to t1 and ends where an unlimited run ends."""
+    say("round limit")
+    ms = [model("slope", 512, ndepth=2), model("channel", 256)]
+    a = Hawassi1D(ms, HW_PREC_FP64, t0=0.0, h0=1e-4)
+    a.integrate(0.3)
+    b = Hawassi1D(ms, HW_PREC_FP64, t0=0.0, h0=1e-4)
+    b.set_round_limit(6)
+    with pytest.raises(Exception, match="rounds"):
+        b.integrate(0.3)
+    t_mid = [b.time(r)[0] for r in range(2)]
+    say("  after 6 rounds t =", t_mid)
+    assert all(0.0 < t < 0.3 for t in t_mid)
+    b.set_round_limit(0)
+    b.integrate(0.3)
+    for r in range(2):
+        e = relerr(b.get_state(r), a.get_state(r))
+        say("  r", r, "t", b.time(r)[0], "relL2 vs unlimited", e)
+        assert b.time(r)[0] == 0.3 and e < 1e-10
