@@ -436,7 +436,9 @@ def wallchar_rect(x, y, walls, refl_coef: float, model: str = "HS2", wallmethod:
         x0, y0, x1, y1 = walls[0]
         ix0, ix1 = c_round((x0 - x[0]) / dx), c_round((x1 - x[0]) / dx)
         iy0, iy1 = c_round((y0 - y[0]) / dy), c_round((y1 - y[0]) / dy)
-        cfsaw[iy0:iy1 + 1, ix0:ix1 + 1] = 0.0
+        # (a box reaching past the grid is cut at its edges; the reference
+        # would write outside its arrays there)
+        cfsaw[max(iy0, 0):max(iy1 + 1, 0), max(ix0, 0):max(ix1 + 1, 0)] = 0.0
     return wallchar, cfsaw
 
 
