@@ -7,7 +7,8 @@ no runup).
 * grid, wavenumbers, bathymetry, anti-alias mask, damping: fun_param_init
   (2d/functions.c:667-874);
 * the influx line signal: fun_generation_influx (2d/functions.c:2634-2895),
-  harmonic (harmonic_influx, :2200-2227) or user_defined (influx.dat), with
+  harmonic (harmonic_influx, :2200-2227), jonswap (fun_jonswap and
+  jonswap_influx, :2226-2446, seeded) or user_defined (influx.dat), with
   the ramp of fun_ramp2dori / fun_tapered2dori (:2522-2595);
 * the sea characteristics (Hs, peak frequency, kp, wavelength) and the
   nonlinear-adjustment zone ChiAdj: fun_calcseacharacteristic (:876-990);
@@ -201,6 +202,83 @@ def chi_adjust_partial(x, y, line_pts, dista):
     return np.where((chi != -9999) & (np.abs(out) < 0.001), 0.0, out)
 
 
+# ------------------------------------------------------------ JONSWAP --
+def jonswap_spectrum(omg, omgp: float, gamJS: float, Hs: float) -> np.ndarray:
+    """fun_jonswap (2d/functions.c:2226-2307) on the first len(omg)//2
+    frequencies: the JONSWAP spectrum alpha g^2 (2 pi)^-4 f^-5
+    exp(-5/4 (f/fp)^-4) gamma^exp(-(f/fp - 1)^2 / (2 sigma^2)), sigma 0.07 up
+    to omega_p and 0.09 above, zero at the first frequency, scaled so that
+    its trapezoid integral (functions1.c:236-246: uniform spacing omg[1] -
+    omg[0]) is Hs^2/16.  The reference reads SPcorrection.dat into a
+    corrected copy but returns the uncorrected spectrum, so no correction
+    applies."""
+    omg = np.asarray(omg, float)
+    n = len(omg) // 2
+    w = omg[:n]
+    f = w / (2 * math.pi)
+    fp = omgp / (2 * math.pi)
+    sig = np.where(w <= omgp, 0.07, 0.09)
+    js = np.zeros(n)
+    with np.errstate(divide="ignore", over="ignore", invalid="ignore"):
+        g = np.exp(-0.5 * ((f[1:] / fp - 1) / sig[1:]) ** 2)
+        js[1:] = (0.0081 * S.GRAV ** 2 * (2 * math.pi) ** -4 * f[1:] ** -5
+                  * np.exp(-(5.0 / 4.0) * (f[1:] / fp) ** -4) * gamJS ** g)
+    m0 = (js.sum() - 0.5 * (js[0] + js[-1])) * (w[1] - w[0])
+    return (Hs * Hs / 16.0) * js / m0
+
+
+def jonswap_directions(nhalf: int, s_word: str, u: np.ndarray) -> np.ndarray:
+    """The directional spreading of jonswap_influx (2d/functions.c:2318-2390):
+    one direction per frequency, 0 for long-crested waves (s = "inf"); else
+    inverse-CDF draws from cos^(2s)(theta) on theta = -pi/2 + j pi/(nhalf-1):
+    the cumulative sum normalised to 1 plus exp(-12)/pi theta (strictly
+    increasing), inverted by a natural spline (theta as a function of the
+    CDF) at the uniform draws u."""
+    if s_word == "inf":
+        return np.zeros(nhalf)
+    s = io2d._atof(s_word)
+    theta = -math.pi / 2 + np.arange(nhalf) * (math.pi / (nhalf - 1))
+    pdf = (math.gamma(s + 1.0) / (math.sqrt(math.pi) * math.gamma(s + 0.5))) * np.cos(theta) ** (2 * s)
+    cdf = np.cumsum(pdf)
+    cdf = cdf / cdf[-1] + math.exp(-12.0) / math.pi * theta
+    c = S.spline_coeffs(cdf, theta)
+    return S._spline_eval_vec(cdf, theta, c, np.asarray(u, float))
+
+
+def jonswap_line_signal(t, ww, kw, Tp, gamJS, Hs, s_word, mdir, line_xy, line0, vertical: bool,
+                        seed: int = 0) -> np.ndarray:
+    """jonswap_influx (2d/functions.c:2310-2446): the line record
+    eta(t_i, p) = sum_j a_j cos(-w_j t_i + phase_j + k_j sin(theta_j + mdir)
+    (y_p - y_0)) (vertical line; cos and x for a horizontal one), a_j =
+    sqrt(2 S_j dw), over the first Nt/2 influx frequencies ww and their
+    wavenumbers kw.  [Nt][nline].
+
+    The reference draws the directions and the phases after srand(time(0))
+    each (unreproducible); here one seeded uniform sequence u serves both, as
+    the two srand(time(0)) calls within one second give the reference one
+    sequence twice: directions from u (short-crested), phases 2 pi u."""
+    t = np.asarray(t, float)
+    nt = len(t)
+    nhalf = nt // 2
+    spec = jonswap_spectrum(ww, 2 * math.pi / Tp, gamJS, Hs)
+    dw = ww[1] - ww[0]
+    a = np.sqrt(2 * spec * dw)
+    u = np.random.default_rng(seed).random(nhalf)
+    th = jonswap_directions(nhalf, s_word, u)
+    ph = 2 * math.pi * u
+    w = np.asarray(ww[:nhalf], float)
+    k = np.asarray(kw[:nhalf], float)
+    rel = np.asarray(line_xy, float) - line0
+    proj = np.sin(th + mdir) if vertical else np.cos(th + mdir)
+    # sum_j a_j cos(psi_jp - w_j t) = Re sum_j (a_j e^{i psi_jp}) e^{-i w_j t}
+    E = a[None, :] * np.exp(1j * (ph[None, :] + (k * proj)[None, :] * rel[:, None]))
+    out = np.empty((nt, len(rel)))
+    for i0 in range(0, nt, 512):
+        tt = t[i0:i0 + 512]
+        out[i0:i0 + 512] = np.real(np.exp(-1j * np.outer(tt, w)) @ E.T)
+    return out
+
+
 def read_influx(path, nxy: int, nt: int):
     """influx.dat (2d/functions.c:2643-2683): columns = spaces on the first
     line (read_col, at most Nxy), then the line x, the line y and Nt rows of
@@ -223,11 +301,13 @@ def read_influx(path, nxy: int, nt: int):
     return np.array(xs), np.array(ys), t, eta
 
 
-def model_from_input(case_dir: str, eps_abs: float = 1e-3, eps_rel: float = 1e-2) -> S.Model2D:
-    """Builds the Model2D of a reference case directory."""
+def model_from_input(case_dir: str, eps_abs: float = 1e-3, eps_rel: float = 1e-2, jonswap_seed: int = 0) -> S.Model2D:
+    """Builds the Model2D of a reference case directory.  jonswap_seed seeds
+    the random directions and phases of a JONSWAP influx (the reference
+    seeds them from the clock)."""
     p = io2d.read_input(os.path.join(case_dir, "input.txt"))
-    if p.wavename not in ("harmonic", "user_defined"):
-        raise NotImplementedError(f"influx '{p.wavename}' is not supported (harmonic, user_defined)")
+    if p.wavename not in ("harmonic", "user_defined", "jonswap"):
+        raise NotImplementedError(f"influx '{p.wavename}' is not supported (harmonic, user_defined, jonswap)")
     if p.order == 0:
         raise ValueError(f"unknown dynamics '{p.model}'")
     # ---- fun_param_init
@@ -316,17 +396,29 @@ def model_from_input(case_dir: str, eps_abs: float = 1e-3, eps_rel: float = 1e-2
         r_ori = tapered_ramp(t, lxy, lxy[0], lxy[-1], p.ramp * Tp_in, (p.taper / 2.0 if vertical else p.taper) * Linf,
                              dy if vertical else dx, dt, p.ramp, p.taper) \
             if (p.ramp != 0.0 or p.taper != 0.0) else np.ones((nt, nl))
-        # harmonic_influx: the line coordinate is the grid's own (dominfo.y[ii])
-        W0 = 2 * math.pi / Tp_in
-        K0 = S.invers_omega(W0, meandepth)
         eta = np.zeros((nt, nxy))
         ii = np.arange(nl)
-        if ix0 == ix1:
-            eta[:, ii + iy0] = r_ori * p.Hs / 2 * np.cos(K0 * math.sin(p.mdir) * y[ii][None, :] - W0 * t[:, None])
-        else:
-            eta[:, ii + ix0] = r_ori * p.Hs / 2 * np.cos(K0 * math.cos(p.mdir) * x[ii][None, :] - W0 * t[:, None])
+        if p.wavename == "harmonic":
+            # harmonic_influx: the line coordinate is the grid's own (dominfo.y[ii])
+            W0 = 2 * math.pi / Tp_in
+            K0 = S.invers_omega(W0, meandepth)
+            if ix0 == ix1:
+                eta[:, ii + iy0] = r_ori * p.Hs / 2 * np.cos(K0 * math.sin(p.mdir) * y[ii][None, :] - W0 * t[:, None])
+            else:
+                eta[:, ii + ix0] = r_ori * p.Hs / 2 * np.cos(K0 * math.cos(p.mdir) * x[ii][None, :] - W0 * t[:, None])
     ww = S.freqspace(t[-1] - t[0], nt)
     kw, _ = influx_wavenumbers(ww, meandepth, flat, p.U)
+    if p.wavename == "jonswap":
+        # jonswap_influx on the line's points (wave_influx.x / .y), relative
+        # to the line's first grid point, ramped like the harmonic record
+        lx = np.array([q[0] for q in line_pts])
+        ly = np.array([q[1] for q in line_pts])
+        sig = jonswap_line_signal(t, ww, kw, Tp_in, p.gamJS, p.Hs, p.s, p.mdir, ly if vertical else lx,
+                                  y[iy0] if vertical else x[ix0], vertical, seed=jonswap_seed)
+        if vertical:
+            eta[:, np.arange(nl) + iy0] = sig * r_ori
+        else:
+            eta[:, np.arange(nl) + ix0] = sig * r_ori
     # ---- fun_calcseacharacteristic
     ins_hat = np.fft.fft2(eta)
     Hs = p.Hs

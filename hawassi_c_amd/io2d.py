@@ -324,6 +324,14 @@ def _atoi(s: str) -> int:
     return int(s[:j]) if j > i else 0
 
 
+def _atof(s: str) -> float:
+    """C's atof: the longest prefix strtod accepts; 0.0 when none."""
+    try:
+        return CStream(s.encode("latin-1")).real()
+    except ScanMismatch:
+        return 0.0
+
+
 def c_round(v: float) -> int:
     """C's round(): halves away from zero."""
     return int(math.copysign(math.floor(abs(v) + 0.5), v))

@@ -218,3 +218,87 @@ def test_model_from_input_rectangle_walls(tmp_path):
     assert m1.refl_one and np.all(m1.wallchar[on0] == 0.0)
     gx, _ = S.gradient_clipped(m1.wallchar, x[1] - x[0], y[1] - y[0])
     assert np.array_equal(m1.grad_wall_x, gx) and np.abs(gx).max() > 0
+
+
+# ---- JONSWAP influx (fun_jonswap, jonswap_influx) --------------------------
+def test_jonswap_spectrum_shape_and_scale():
+    """fun_jonswap (2d/functions.c:2226-2307) against its closed form: zero
+    at the first frequency; the peak enhancement over the Pierson-Moskowitz
+    shape is gamma at f_p, gamma^exp(-(0.1/0.09)^2/2) at 1.1 f_p and
+    gamma^exp(-(0.1/0.07)^2/2) at 0.9 f_p (sigma 0.09 above the peak, 0.07
+    at and below it), relative to 3 f_p where the enhancement is 1 to within
+    gamma^(e^-247); the PM shape ratio between 2 f_p and f_p is
+    2^-5 exp(-5/4 (1/16 - 1)); numpy's trapezoid integral is Hs^2/16."""
+    Tp, Hs = 8.0, 2.0
+    wp = 2 * math.pi / Tp
+    dw = wp / 40
+    ww = np.concatenate([np.arange(0, 400) * dw, -np.arange(400, 0, -1) * dw])   # FFT order, 800 points
+    ww[0] = 1e-10
+    for gam in (1.0, 3.3):
+        s = C.jonswap_spectrum(ww, wp, gam, Hs)
+        w = ww[:400]
+        assert s[0] == 0.0
+        assert abs(np.trapezoid(s, dx=w[1] - w[0]) - Hs * Hs / 16) < 1e-12
+        pm = C.jonswap_spectrum(ww, wp, 1.0, Hs)
+        enh = lambda i: (s[i] / pm[i]) / (s[120] / pm[120])      # index 40 = f_p, 120 = 3 f_p
+        assert abs(enh(40) - gam) < 1e-12 * gam
+        for i, sig in ((44, 0.09), (36, 0.07)):
+            assert abs(enh(i) - gam ** math.exp(-0.5 * (0.1 / sig) ** 2)) < 1e-12
+    pm = C.jonswap_spectrum(ww, wp, 1.0, Hs)
+    assert abs(pm[80] / pm[40] - 2.0 ** -5 * math.exp(-1.25 * (1 / 16 - 1))) < 1e-12
+
+
+def test_jonswap_directions_are_cos2s_quantiles():
+    """jonswap_influx's spreading (2d/functions.c:2340-2390): the draws are
+    quantiles of the cos^(2s) distribution on (-pi/2, pi/2) (its CDF at each
+    returned direction equals the uniform draw, to the reference's
+    discretisation on Nt/2 points); long-crested seas ("inf") keep 0."""
+    from scipy.integrate import quad
+    assert np.array_equal(C.jonswap_directions(200, "inf", np.linspace(0.1, 0.9, 200)), np.zeros(200))
+    s = 10.0
+    u = (np.arange(50) + 0.5) / 50
+    th = C.jonswap_directions(2000, "10", u)
+    norm = quad(lambda a: math.cos(a) ** (2 * s), -math.pi / 2, math.pi / 2)[0]
+    F = np.array([quad(lambda a: math.cos(a) ** (2 * s), -math.pi / 2, v)[0] / norm for v in th])
+    assert np.abs(F - u).max() < 2e-3
+    # symmetric about 0 up to the cumulative sum's one-bin offset (pi/1999)
+    assert abs(th[24] + th[25]) < 2 * math.pi / 1999
+    # a wider spreading (s = 2) spreads the same quantiles further out
+    assert abs(C.jonswap_directions(2000, "2", u)[-1]) > abs(th[-1])
+
+
+def test_jonswap_line_signal():
+    """The line record: long-crested waves along a vertical line at mean
+    direction 0 are the same at every line point, with the variance
+    sum_j a_j^2 / 2 = Hs^2/16 of the spectrum (within 5 % for one random-phase
+    realization over the record); the seeded draws repeat; a mean direction
+    of 30 degrees delays the record along the line by k sin(30 deg) y."""
+    nt, dt, Tp, Hs = 1200, 0.25, 6.0, 1.5
+    t = np.arange(nt) * dt
+    ww = S.freqspace(t[-1] - t[0], nt)
+    kw, _ = C.influx_wavenumbers(ww, 20.0, True)
+    y = np.linspace(0, 300, 41)
+    e = C.jonswap_line_signal(t, ww, kw, Tp, 3.3, Hs, "inf", 0.0, y, y[0], True, seed=4)
+    assert np.abs(e - e[:, :1]).max() < 1e-12
+    assert abs(e[:, 0].var() / (Hs * Hs / 16) - 1) < 0.05
+    assert np.array_equal(e, C.jonswap_line_signal(t, ww, kw, Tp, 3.3, Hs, "inf", 0.0, y, y[0], True, seed=4))
+    e30 = C.jonswap_line_signal(t, ww, kw, Tp, 3.3, Hs, "inf", math.radians(30), y, y[0], True, seed=4)
+    # each component is shifted in phase by k sin(30 deg) (y - y0): the
+    # record at y = 0 is unchanged, elsewhere it differs
+    assert np.abs(e30[:, 0] - e[:, 0]).max() < 1e-12 and np.abs(e30[:, -1] - e[:, -1]).max() > 0.1 * np.abs(e).max()
+
+
+def test_model_from_input_jonswap(tmp_path):
+    """2dF001Irreg's input.txt with `Influx_signal = jonswap`: the line record
+    is generated (seeded), its spectral peak near Tp = 8 s, Hs kept as
+    given (fun_calcseacharacteristic recomputes it only for user_defined)."""
+    d = tmp_path / "js"
+    d.mkdir()
+    txt = (GOLD / "2dF001Irreg.txt").read_text().replace("Influx_signal   = user_defined", "Influx_signal   = jonswap")
+    (d / "input.txt").write_text(txt)
+    m = C.model_from_input(str(d), jonswap_seed=3)
+    assert m.meta["params"].wavename == "jonswap" and m.meta["hs"] == 2.0
+    assert abs(m.meta["tp"] - 8.0) < 1.0
+    assert np.isfinite(m.sig).all() and np.abs(m.sig).max() > 0
+    m2 = C.model_from_input(str(d), jonswap_seed=3)
+    assert np.array_equal(m.sig, m2.sig)

@@ -63,9 +63,20 @@ def case_user_influx(tmp_path):
     return C.model_from_input(str(d))
 
 
-@pytest.mark.parametrize("which", ["flat_harmonic", "two_depth", "user_influx"])
+def case_jonswap(tmp_path):
+    """2dF001Irreg's input.txt with `Influx_signal = jonswap` (gamma 1, s 20):
+    the line record generated by jonswap_influx, seeded."""
+    d = tmp_path / "2dF001Irreg_jonswap"
+    d.mkdir()
+    txt = (GOLD / "2dF001Irreg.txt").read_text().replace("Influx_signal   = user_defined", "Influx_signal   = jonswap")
+    (d / "input.txt").write_text(txt)
+    return C.model_from_input(str(d), jonswap_seed=11)
+
+
+@pytest.mark.parametrize("which", ["flat_harmonic", "two_depth", "user_influx", "jonswap"])
 def test_case_rhs_parity(tmp_path, which):
-    m = {"flat_harmonic": case_flat_harmonic, "two_depth": case_two_depth, "user_influx": case_user_influx}[which](tmp_path)
+    m = {"flat_harmonic": case_flat_harmonic, "two_depth": case_two_depth, "user_influx": case_user_influx,
+         "jonswap": case_jonswap}[which](tmp_path)
     say("case rhs", which, m.nx, m.ny, "order", m.order, "flat", m.flat)
     y = W.random_state(m, 5, amp=0.5, kfrac=0.3)
     ref = HS.oracle_half(O.Oracle2D(m).rhs(37.3, y), m.nx, m.ny)
