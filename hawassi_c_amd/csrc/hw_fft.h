@@ -149,6 +149,13 @@ template <int R, int DIR, typename C> __device__ __forceinline__ void dftR(C* v)
 #ifndef HW_MAX_RADIX
 #define HW_MAX_RADIX 8
 #endif
+// Radix-16 LDS passes are switched off: built with them, the 1-D right-hand
+// side at n = 2048 (1024 threads, two elements each) took an illegal memory
+// access on gfx950, and no out-of-bounds index has been found that explains
+// it (the pass's LDS and twiddle indices stay in range in a host model).
+#if HW_MAX_RADIX > 8
+#error "radix-16 LDS passes are switched off (an unexplained device fault at n = 2048)"
+#endif
 // the radix of the LDS Stockham pass that leaves rem points per sequence:
 // 16/8/4/2 while rem has a factor 2, then 5 and 3 (lengths 2^a 3^b 5^c)
 constexpr int pass_radix(int rem) {
