@@ -160,6 +160,8 @@ template <typename T> struct CrestMachine {
         calls++;
         *have_b = false;
         HW_CM_TRY(hipEventSynchronize(ev));
+        // detect_end may have run on another stream: order s after it
+        HW_CM_TRY(hipStreamWaitEvent(s, ev, 0));
         const int npk = h_cnt[0], ncand = h_cnt[1];
         if (npk > maxpk) { set_error("breaking: %d peaks exceed the buffer of %d", npk, maxpk); return HW_ERR_ARG; }
         // ---- criterion: candidate crests in the reference's peak order

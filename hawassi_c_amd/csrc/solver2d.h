@@ -145,6 +145,13 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
     CrestMachine<T> cm;                    // criterion, crest lists, B (crest2d.h)
     DevBuf<T> flux, sbx, sby, s_eta;
     DevBuf<C> hilx;                        // Hilbert transform of eta: x-intermediate
+    // The detection kernels (peaks, Hilbert transform, crest test) read only
+    // the physical eta, u, v of the first x-pass and their results are waited
+    // for at the end of the evaluation, so they run on a stream of their own
+    // beside the rest of the pipeline (they fill the partial last round of
+    // blocks of the y-passes).
+    hipStream_t dstream = nullptr;
+    hipEvent_t ev_fork = nullptr;
     DevBuf<T> hilp;                        // ... and physical
     bool sb_zero = true;                   // Sb^ known to be all zero
     bool last_gated = false;               // last RHS call skipped breaking (tprev >= t)
@@ -186,6 +193,8 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         }
         if (d_full_a) (void)hipFree(d_full_a);
         if (d_full_b) (void)hipFree(d_full_b);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (dstream) (void)hipStreamDestroy(dstream);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -290,6 +299,8 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         if ((rc = s_eta.alloc((size_t)NY * nb))) return rc;
         if (brk_on) {
             if ((rc = cm.init(c, NX, NY, stream))) return rc;
+            HW_TRY(hipStreamCreateWithFlags(&dstream, hipStreamNonBlocking));
+            HW_TRY(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
             delb = c->delb;
             dxg = cm.dxg; dyg = cm.dyg;
             if ((rc = flux.alloc(N)) || (rc = sbx.alloc(N)) || (rc = sby.alloc(N)) || (rc = hilx.alloc(H)) ||
@@ -511,11 +522,15 @@ template <typename T, int NX, int NY> struct Solver2DT : Solver2D {
         const dim3 gx(NY / XG::RR), bx(XG::NT);
         const dim3 gy((NXH + YG::CC - 1) / YG::CC), by(YG::NT);
         int rc;
-        if ((rc = cm.detect_begin(stream, eta_p.p, true))) return rc;
-        k_brk_hilb_y<T, NX, NY><<<gy, by, 0, stream>>>(args, hilx.p, cm.pkcount);
-        k_t_xinv<T, NX, NY><<<gx, bx, 0, stream>>>(hilx.p, hilp.p, nullptr, twx.p, cm.pkcount);
+        // fork after the first x-pass; CrestMachine::run joins (host wait on
+        // its event, then the main stream waits on it too)
+        HW_TRY(hipEventRecord(ev_fork, stream));
+        HW_TRY(hipStreamWaitEvent(dstream, ev_fork, 0));
+        if ((rc = cm.detect_begin(dstream, eta_p.p, true))) return rc;
+        k_brk_hilb_y<T, NX, NY><<<gy, by, 0, dstream>>>(args, hilx.p, cm.pkcount);
+        k_t_xinv<T, NX, NY><<<gx, bx, 0, dstream>>>(hilx.p, hilp.p, nullptr, twx.p, cm.pkcount);
         HW_LAUNCH_CHECK();
-        return cm.detect_end(stream, eta_p.p, hilp.p, u_p.p, v_p.p);
+        return cm.detect_end(dstream, eta_p.p, hilp.p, u_p.p, v_p.p);
     }
 
     int breaking(double t) {
