@@ -67,6 +67,16 @@ class HwStats(ct.Structure):
                 ("rhs_ms_total", ct.c_double), ("rhs_timed", ct.c_long)]
 
 
+class HwKin1DConfig(ct.Structure):
+    _fields_ = [("n", ct.c_int), ("ix0", ct.c_int), ("nxout", ct.c_int), ("nz", ct.c_int), ("flat", ct.c_int),
+                ("ntab", ct.c_int), ("dx", ct.c_double), ("grav", ct.c_double), ("hmin", ct.c_double),
+                ("hplus", ct.c_double), ("depth", ct.POINTER(ct.c_double)), ("k", ct.POINTER(ct.c_double)),
+                ("z", ct.POINTER(ct.c_double)), ("htab", ct.POINTER(ct.c_double)),
+                ("gam_hmin", ct.POINTER(ct.c_double)), ("gam_hplus", ct.POINTER(ct.c_double)),
+                ("dztab", ct.POINTER(ct.c_double)), ("gam_dzmin", ct.POINTER(ct.c_double)),
+                ("gam_dzplus", ct.POINTER(ct.c_double))]
+
+
 class HwFnoConfig(ct.Structure):
     _fields_ = [("n", ct.c_int), ("width", ct.c_int), ("modes", ct.c_int), ("layers", ct.c_int),
                 ("hidden", ct.c_int), ("in_ch", ct.c_int), ("out_ch", ct.c_int), ("max_batch", ct.c_int),
@@ -80,7 +90,7 @@ EXPORTS = [
     "hw_set_tprev", "hw_get_breaking",
     "hw1d_create", "hw1d_destroy", "hw1d_rhs", "hw1d_rhs_at", "hw1d_set_state", "hw1d_get_state",
     "hw1d_set_time", "hw1d_get_time", "hw1d_integrate", "hw1d_set_round_limit", "hw1d_get_breaking",
-    "hw1d_get_stats",
+    "hw1d_get_stats", "hw1d_kinematics",
     "hw_fno_create", "hw_fno_destroy", "hw_fno_param_count", "hw_fno_get_params", "hw_fno_set_params",
     "hw_fno_forward", "hw_fno_loss_grad", "hw_fno_set_dataset", "hw_fno_train_step", "hw_fno_gemm_check", "hw_fno_set_precision", "hw_fno_gemm_flops", "hw_fno_gemm_bytes",
 ]
@@ -156,6 +166,8 @@ def load(path: Path | None = None, partial: bool = False) -> ct.CDLL:
     lib.hw1d_set_round_limit.argtypes = [vp, ct.c_long]
     lib.hw1d_get_breaking.argtypes = [vp, ct.c_int, _dp, ct.POINTER(ct.c_int)]
     lib.hw1d_get_stats.argtypes = [vp, ct.c_int, ct.POINTER(HwStats)]
+    lib.hw1d_kinematics.argtypes = [ct.POINTER(HwKin1DConfig), _dp, _dp, _dp, _dp, ct.c_double, ct.c_double, _dp,
+                                    _dp, _dp]
     _fp = ct.POINTER(ct.c_float)
     lib.hw_set_step_mode.argtypes = [vp, ct.c_int]
     lib.hw_ctx_create_2d_members.argtypes = [ct.POINTER(HwConfig2D), ct.c_int, ct.POINTER(vp)]

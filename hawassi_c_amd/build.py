@@ -43,7 +43,7 @@ NT1D = (64, 128, 256, 512, 1000, 1024)
 def _units():
     gen = BUILD / "gen"
     gen.mkdir(parents=True, exist_ok=True)
-    units = [CSRC / "capi.hip", CSRC / "capi1d.hip", CSRC / "fno.hip", CSRC / "full2d.hip"]
+    units = [CSRC / "capi.hip", CSRC / "capi1d.hip", CSRC / "fno.hip", CSRC / "full2d.hip", CSRC / "kin1d.hip"]
     for prec in (0, 1):
         t = "double" if prec == 0 else "float"
         for n in N1D:

@@ -337,6 +337,43 @@ int hw1d_get_breaking(void *ctx, int r, double *B, int *flag);
 /* Counters of realization r, or summed over the batch for r = -1. */
 int hw1d_get_stats(void *ctx, int r, hw_stats *st);
 
+/* ------------------------------------------------ 1-D internal kinematics --
+ * kinematic_modul (1d/kinematic_modul.c:1-247), called by ode_solver_Ham at
+ * each output time when kinematic = yes (1d/functions.c:655-698): potential
+ * flow under the surface on the output window x[ix0 .. ix0 + nxout) and the
+ * levels z[0 .. nz).  The two-depth weight tables are sinhcosh2IP's
+ * (1d/functions.c:1889-1930) on H in [hmin, hplus] (hmin = min_zsurf +
+ * min depth, hplus = max_zsurf + max depth) and on DZ in [0, hplus], ntab
+ * points each (the reference's NH + 1 = 101); their natural splines are
+ * built here.  flat = 1 when min depth == max depth.  The runup form (a
+ * bathymetry with a dry node) is not covered. */
+typedef struct hw_kin1d_config {
+    int n;                      /* nodes: a power of two 256..8192, or 2000 */
+    int ix0, nxout;             /* output window (indxIP1, NxoutIP) */
+    int nz;                     /* output levels (NzoutIP) */
+    int flat;                   /* minD == maxD */
+    int ntab;                   /* points of each weight table */
+    double dx, grav;
+    double hmin, hplus;         /* end depths of the H table (Hmin, Hplus) */
+    const double *depth, *k;    /* [n] depth (positive down) and wavenumbers in FFT order */
+    const double *z;            /* [nz] levels (zoutIP) */
+    const double *htab, *gam_hmin, *gam_hplus;     /* [ntab] H table and its weights */
+    const double *dztab, *gam_dzmin, *gam_dzplus;  /* [ntab] DZ table and its weights */
+} hw_kin1d_config;
+
+/* Host arrays: eta, deta [n] physical; u_hat, dphi_hat [n] complex
+ * (re, im interleaved, unnormalised forward DFT); velx, velz [nxout][nz] the
+ * previous velocities (NULL: zero, as the reference never updates Velx /
+ * Velz).  out [6][nxout][nz]: dtPhi, dxPhi, dzPhi, pressure, d/dt dxPhi,
+ * d/dt dzPhi, NaN above the surface and below the bottom; the accelerations
+ * divide by t - tprev as the reference does (ode_solver_Ham sets tprev = t
+ * before the call, so its own accelerations are not finite).  HW_ERR_ARG
+ * when eta + depth leaves the H table (the reference's gsl_spline_eval stops
+ * with a domain error there). */
+int hw1d_kinematics(const hw_kin1d_config *cfg, const double *eta, const double *u_hat, const double *deta,
+                    const double *dphi_hat, double t, double tprev, const double *velx, const double *velz,
+                    double *out);
+
 /* --------------------------------------------- learned coarse propagator --
  * A spectral neural operator (Fourier neural operator) for the coarse
  * propagator of the parareal driver.  No counterpart in the reference: the
