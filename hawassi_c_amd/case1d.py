@@ -322,6 +322,14 @@ def run_case(case: Case1D, out_dir: str, precision=None, progress=None, fields: 
     with np.errstate(divide="ignore"):
         a = np.where(m.k == 0, 0.0, -1.0 / np.where(m.k == 0, 1.0, m.k))
     vals = np.zeros((len(case.t_out), len(case.gauges)))
+    kin = kin_setup(case) if case.params.kinematic == "yes" else None
+    kw = hk = None
+    if kin is not None:
+        kw = io1d.KinWriter1D(out_dir, case.t_out, case.params.npartition, case.x[kin.ix0:kin.ix0 + kin.nxout],
+                              kin.z)
+        # d/dt at the output time from a context of its own (no breaking or
+        # wall state to disturb there, see kin_setup)
+        hk = Hawassi1D([case.model], precision=prec, t0=float(case.t_out[0]), h0=1e-4)
     try:
         for i, t in enumerate(case.t_out):
             if i > 0:
@@ -338,6 +346,18 @@ def run_case(case: Case1D, out_dir: str, precision=None, progress=None, fields: 
                 fw.write(eta, u, phi)
             if gw is not None:
                 vals[i] = gw.write(float(t), eta)
+            if kw is not None:
+                # kinematic_modul(t, etaf, u_hatf, dtetaf, dtphi_hatf) with
+                # dy = f(t, y) (1d/functions.c:568-595, 691) and tprev = t, as
+                # the reference's loop leaves it (1d/functions.c:552)
+                dy = hk.rhs(float(t), y)
+                uh = y[2 * n:3 * n] + 1j * y[3 * n:]
+                deta = np.real(np.fft.ifft(dy[:n] + 1j * dy[n:2 * n]))
+                dvh = dy[2 * n:3 * n] + 1j * dy[3 * n:]
+                dph = -(a * dvh.imag) + 1j * (a * dvh.real)
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    res = kin.run(eta, uh, deta, dph, float(t), float(t))
+                kw.write(eta[kin.ix0:kin.ix0 + kin.nxout], res)
             if progress is not None and i % 500 == 0:
                 progress(i, float(t))
         stats = hw.stats(0)
@@ -346,5 +366,30 @@ def run_case(case: Case1D, out_dir: str, precision=None, progress=None, fields: 
             gw.close()
         if fw is not None:
             fw.close()
+        if kw is not None:
+            kw.close()
+        if hk is not None:
+            hk.close()
         hw.close()
     return dict(t=case.t_out, gauges=vals, stats=stats)
+
+
+def kin_setup(case: Case1D):
+    """The kinematics setup of a case with `kinematic = yes`: levels over
+    [zsurf1, zsurf2] from read_kinematics (1d/functions.c:3555-3582: the
+    expert min/max_zsurf, or -/+2.5 Hs_in, -/+0.6 Hs_in with an initial
+    elevation) and the window [xinterv1, xinterv2].  The d/dt the module
+    needs comes from an extra right-hand side at the output time, which
+    breaking and the wall source would not take free of side effects, so
+    those cases are refused."""
+    from .io2d import _atof
+    from .kinematics1d import Kinematics1D
+    p, m = case.params, case.model
+    if m.breaking is not None or m.wall is not None:
+        raise NotImplementedError("kinematics output with breaking or a wall is not covered")
+    hs = case.derived["Hs_in"]
+    user = p.initial == "user_defined"
+    z1 = (-0.6 if user else -2.5) * hs if p.min_z == "default" else _atof(p.min_z)
+    z2 = (0.6 if user else 2.5) * hs if p.max_z == "default" else _atof(p.max_z)
+    return Kinematics1D(case.x, m.depth, m.k, case.derived["nu_peak"], z1, z2, p.nsurf, p.ndeep, p.xinterv1,
+                        p.xinterv2)

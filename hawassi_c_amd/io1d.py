@@ -317,6 +317,72 @@ class FieldWriter1D:
             self.f = None
 
 
+class KinWriter1D:
+    """The kinematics outputs of ode_solver_Ham (1d/functions.c:655-698,
+    save_* at 1d/functions.c:400-500, kinematic_modul's fwrites at
+    1d/kinematic_modul.c:228-246): per partition XX (the same cut as
+    FieldWriter1D) Hawassi_Elev_XX.dat (header N1*N2, N1, N2, times, xout;
+    then eta on the window per output time), Hawassi_Pressure_XX.dat,
+    Hawassi_Vel_XX.dat and Hawassi_Acc_XX.dat (header N1*N2*N3, N1, N2, N3,
+    times, xout, zout; then per output time P [nx][nz], dxPhi then dzPhi,
+    d/dt dxPhi then d/dt dzPhi), all little-endian doubles."""
+
+    def __init__(self, outdir, t_out, npartition: int, xout: np.ndarray, zout: np.ndarray):
+        self.dir = outdir
+        self.t = np.asarray(t_out)
+        n = len(self.t)
+        self.npart = max(int(npartition), 1)
+        self.ntout = n // self.npart
+        self.ntrest = n - self.npart * self.ntout
+        self.xout, self.zout = np.asarray(xout, "<f8"), np.asarray(zout, "<f8")
+        self.ip = 0
+        self.i = 0
+        self.f = None
+
+    def _open(self):
+        import os
+        self.close()
+        self.ip += 1
+        tag = "%02d.dat" % self.ip
+        n1 = self.ntout if self.ip <= self.npart else self.ntrest
+        if n1 == self.ntrest:
+            tp = self.t[self.npart * self.ntout:]
+        else:
+            tp = self.t[(self.ip - 1) * self.ntout:self.ip * self.ntout]
+        n2, n3 = float(self.xout.size), float(self.zout.size)
+        self.f = {}
+        for name in ("Elev", "Pressure", "Vel", "Acc"):
+            f = open(os.path.join(self.dir, f"Hawassi_{name}_" + tag), "wb")
+            if name == "Elev":
+                head = [n1 * n2, n1, n2]
+            else:
+                head = [n1 * n2 * n3, n1, n2, n3]
+            np.asarray(head, "<f8").tofile(f)
+            np.asarray(tp, "<f8").tofile(f)
+            self.xout.tofile(f)
+            if name != "Elev":
+                self.zout.tofile(f)
+            self.f[name] = f
+
+    def write(self, eta_cut, kin: dict):
+        if self.ntout > 0 and self.i % self.ntout == 0:
+            self._open()
+        w = lambda name, a: np.asarray(a, "<f8").tofile(self.f[name])  # noqa: E731
+        w("Elev", eta_cut)
+        w("Pressure", kin["P"])
+        w("Vel", kin["dxPhi"])
+        w("Vel", kin["dzPhi"])
+        w("Acc", kin["acc_x"])
+        w("Acc", kin["acc_z"])
+        self.i += 1
+
+    def close(self):
+        if self.f:
+            for f in self.f.values():
+                f.close()
+            self.f = None
+
+
 def read_gauge_file(path) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
     """A Hawassi_*_gauge.txt back as (gauge positions, t, values[t, gauge])."""
     rows = [ln.split() for ln in open(path).read().splitlines() if ln.strip()]

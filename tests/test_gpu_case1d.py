@@ -215,3 +215,60 @@ def test_case_field_outputs(tmp_path):
     # function of the u frame (Re F^-1 u^) alone; it is checked for size
     assert np.isfinite(phi).all() and np.isfinite(u).all() and np.abs(phi[-1]).max() > 0
     assert np.abs(eta[-1]).max() > 1e-3
+
+
+def test_case_kinematics_outputs(tmp_path):
+    """run_case with `kinematic = yes` (1dB002HarmBar, window 30..40 m, the
+    first 40 samples): the four files' headers and layout, the elevation on
+    the window equal to the eta frames, and the velocities and pressure at
+    the last sample equal to the oracle's kinematic_modul on the state the
+    device reaches there (d/dt from the right-hand side at that state);
+    the accelerations are not finite, as the reference's (tprev = t)."""
+    from oracle.kinematics1d_oracle import kinematics
+    say("case1d kinematics outputs")
+    d = unpack_case("1dB002HarmBar", tmp_path)
+    c = case1d.case_from_dir(str(d))
+    nt = 40
+    c.t_out = c.t_out[:nt]
+    c.params.kinematic = "yes"
+    c.params.xinterv1, c.params.xinterv2 = 30.0, 40.0
+    out = tmp_path / "out"
+    case1d.run_case(c, str(out))
+    kin = case1d.kin_setup(c)
+    n, nx, nz = c.model.n, kin.nxout, kin.z.size
+    el = np.fromfile(out / "Hawassi_Elev_01.dat", "<f8")
+    assert el[:3].tolist() == [nt * nx, nt, nx]
+    assert np.array_equal(el[3:3 + nt], c.t_out) and np.array_equal(el[3 + nt:3 + nt + nx], c.x[kin.ix0:kin.ix0 + nx])
+    eta = np.fromfile(out / "Hawassi_eta_01.dat", "<f8").reshape(nt, n)
+    assert np.array_equal(el[3 + nt + nx:].reshape(nt, nx), eta[:, kin.ix0:kin.ix0 + nx])
+    hdr = 4 + nt + nx + nz
+    files = {}
+    for name in ("Pressure", "Vel", "Acc"):
+        a = np.fromfile(out / f"Hawassi_{name}_01.dat", "<f8")
+        assert a[:4].tolist() == [nt * nx * nz, nt, nx, nz], name
+        assert np.array_equal(a[4 + nt + nx:hdr], kin.z), name
+        files[name] = a[hdr:]
+    P = files["Pressure"].reshape(nt, nx, nz)
+    V = files["Vel"].reshape(nt, 2, nx, nz)
+    A = files["Acc"].reshape(nt, 2, nx, nz)
+    # the state at the last sample, integrated again
+    hw = Hawassi1D([c.model], HW_PREC_FP64, t0=float(c.t_out[0]), h0=1e-4)
+    hw.set_state(c.y0)
+    for ti in c.t_out[1:]:
+        hw.integrate(float(ti))
+    y = hw.get_state()
+    t = float(c.t_out[-1])
+    dy = hw.rhs(t, y)
+    hw.close()
+    k = c.model.k
+    a = np.where(k == 0, 0.0, -1.0 / np.where(k == 0, 1.0, k))
+    uh = y[2 * n:3 * n] + 1j * y[3 * n:]
+    dvh = dy[2 * n:3 * n] + 1j * dy[3 * n:]
+    ref = kinematics(kin, eta[-1], uh, np.real(np.fft.ifft(dy[:n] + 1j * dy[n:2 * n])), -(a * dvh.imag) + 1j * (a * dvh.real),
+                     t, t + 1.0)
+    for got, want in ((V[-1, 0], ref["dxPhi"]), (V[-1, 1], ref["dzPhi"]), (P[-1], ref["P"])):
+        assert np.array_equal(np.isnan(got), np.isnan(want))
+        fin = ~np.isnan(want)
+        assert fin.any()
+        assert np.max(np.abs(got[fin] - want[fin])) <= 1e-9 * np.max(np.abs(want[fin])) + 1e-15
+    assert not np.isfinite(A[-1]).any()
