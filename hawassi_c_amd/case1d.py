@@ -37,8 +37,10 @@ a ``Model1D``, and its run with the reference's outputs.
 integrate to every sample time of the record and write the gauge file and
 the processed influx.  A wall (define_wall / wall_setup) adds its skew source
 kernel and ChiAdj factor; the wall node's output is scaled by 1 + refl_coef.
+With `kinematic = yes` it also writes the kinematics files
+(Hawassi_Elev/Pressure/Vel/Acc_XX.dat, `kin_setup`, hw1d_kinematics).
 Not supported (they raise): runup (a bathymetry above the still water level),
-coupling, friction, kinematics output.
+coupling, friction, and kinematics output together with breaking or a wall.
 """
 from __future__ import annotations
 
