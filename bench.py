@@ -84,12 +84,14 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--workload", default="harbour2048",
-                    choices=["harbour2048", "slope1d", "opensea1024x8", "parareal512", "ensemble", "refcase2d"],
+                    choices=["harbour2048", "slope1d", "opensea1024x8", "parareal512", "ensemble", "refcase2d",
+                             "kin1d"],
                     help="harbour2048: the 2-D headline; slope1d: the 1-D batch of uneven realizations; "
                          "opensea1024x8: eight flat-bottom 1024^2 realizations advanced in lockstep; "
                          "parareal512: learned parareal on the 512^2 shoal basin (trains the network first); "
                          "ensemble: 84 sea states on 128^2/256^2/512^2 grids, members and HIP-graph steps; "
-                         "refcase2d: the reference case 2dFBr001_Irreg (600x300, breaking) whole, full-grid path")
+                         "refcase2d: the reference case 2dFBr001_Irreg (600x300, breaking) whole, full-grid path; "
+                         "kin1d: the 1-D internal kinematics at one output time (4096 nodes, 90 levels)")
     ap.add_argument("--pr-seeds", type=int, default=33, help="parareal512: training trajectories (61 snapshots each)")
     ap.add_argument("--pr-epochs", type=int, default=30, help="parareal512: training epochs")
     ap.add_argument("--pr-window", type=int, default=8, help="parareal512: slices per parareal window")
@@ -127,6 +129,8 @@ def main():
         return bench_ensemble(args)
     if args.workload == "refcase2d":
         return bench_refcase2d(args)
+    if args.workload == "kin1d":
+        return bench_kin1d(args)
 
     from hawassi_c_amd import HW_PREC_FP32, HW_PREC_FP64
     from hawassi_c_amd import workloads as W
@@ -671,6 +675,57 @@ def bench_parareal(args):
                                      "(all kernels and host calls included)"},
         "runs": runs,
     }
+    print(json.dumps(out))
+
+
+def bench_kin1d(args):
+    """hw1d_kinematics (kinematic_modul, 1d/kinematic_modul.c) at one output
+    time: 4096 nodes over a sloping bottom (14 -> 6 m), the reference's
+    default 50 surface + 40 deep levels, a 2560-node window; each call takes
+    host arrays and returns the six fields (PCIe and the call's own device
+    allocation included).  CPU baseline: the numpy oracle on the same input."""
+    import numpy as np
+    from hawassi_c_amd.kinematics1d import Kinematics1D
+    n, dx = 4096, 2.0
+    rng = np.random.default_rng(7)
+    x = np.arange(n) * dx
+    L = n * dx
+    k = 2 * np.pi * np.fft.fftfreq(n, d=dx)
+    depth = 10.0 - 4.0 * np.tanh((x - 0.5 * L) / (0.1 * L))
+
+    def smooth(amp):
+        v = np.zeros(n)
+        for m in rng.integers(2, 40, size=8):
+            v += amp * rng.uniform(-1, 1) * np.cos(2 * np.pi * m * x / L + rng.uniform(0, 2 * np.pi))
+        return v
+
+    s = Kinematics1D(x, depth, k, 0.9, -0.5, 0.5, 50, 40, x[n // 8], x[n // 8 + 2559])
+    eta, uh, deta, dph = smooth(0.05), np.fft.fft(smooth(0.3)), smooth(0.1), np.fft.fft(smooth(0.5))
+    for _ in range(args.warmup):
+        s.run(eta, uh, deta, dph, 1.0, 0.5)
+    w0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.run(eta, uh, deta, dph, 1.0, 0.5)
+    wall = time.perf_counter() - w0
+    out = {
+        "metric": "1-D kinematics evaluations/s (kinematic_modul at one output time)",
+        "value": args.steps / wall, "unit": "calls/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "kin1d: 4096 nodes, sloping bottom 14 -> 6 m, %d levels, %d-node window, host arrays "
+                               "in and out" % (s.z.size, s.nxout)},
+        "roofline": None,
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        from oracle.kinematics1d_oracle import kinematics
+        c0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            kinematics(s, eta, uh, deta, dph, 1.0, 0.5)
+        cw = (time.perf_counter() - c0) / reps
+        out["cpu_baseline"] = {"value": 1.0 / cw, "unit": "calls/s", "cores": 1, "kind": "port",
+                               "sample": "3 calls of the numpy oracle on the same input"}
     print(json.dumps(out))
 
 
