@@ -155,14 +155,18 @@ template <int N, int NT> __global__ void __launch_bounds__(NT) k_kin_level(KinAr
         one_dz = dzmin == dzplus;
     }
     for (int p = 0; p < 2; ++p) {
-        for (int j = tid; j < N; j += NT) {
+        // one thread per index pair (j, N - j): Herm(X)[N - j] is the
+        // conjugate of Herm(X)[j], so each spectrum entry is evaluated once
+        for (int j = tid; j <= N / 2; j += NT) {
             const int jm = j == 0 ? 0 : N - j;
             double2 aj, bj, am, bm;
             kin_spectra(a, j, z, one_dz, dzmin, dzplus, p, aj, bj);
-            kin_spectra(a, jm, z, one_dz, dzmin, dzplus, p, am, bm);
+            if (jm != j) kin_spectra(a, jm, z, one_dz, dzmin, dzplus, p, am, bm);
+            else { am = aj; bm = bj; }
             const double hx = 0.5 * (aj.x + am.x), hy = 0.5 * (aj.y - am.y);
             const double gx = 0.5 * (bj.x + bm.x), gy = 0.5 * (bj.y - bm.y);
             kin_lds[j] = make_double2(hx - gy, hy + gx);
+            if (jm != j) kin_lds[jm] = make_double2(hx + gy, gx - hy);
         }
         __syncthreads();
         fft_lds<double2, N, 1, NT, 1>(kin_lds, N, a.tw, 1, tid);   // backward, ends with a barrier
